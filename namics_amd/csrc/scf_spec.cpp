@@ -1,0 +1,870 @@
+// `.in` reader and system builder for the SF-SCF hot path.  See scf_spec.h.
+//
+// Grammar (reference src/input.cpp:3-78): every line is `key : name : prop :
+// value`; all spaces and tabs are removed; lines starting with `//` are
+// comments; a line `start` closes a block and the values of block k are the
+// union of blocks 1..k with the last value winning (src/input.cpp:366-429).
+// Lists of names (mon, mol) keep the order of first appearance
+// (src/input.cpp:286-306).
+#include "scf_spec.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <map>
+#include <sstream>
+
+namespace nscf {
+namespace {
+
+struct Line {
+  std::string key, name, prop, value;
+};
+
+struct Parsed {
+  std::vector<Line> lines;                 // settings visible at the chosen start
+  std::vector<std::string> mons, mols;     // order of first appearance
+  std::string lat_name, sys_name = "NN", newton_name;
+};
+
+std::vector<std::string> split(const std::string &s, char c) {
+  std::vector<std::string> out;
+  std::string cur;
+  for (char ch : s) {
+    if (ch == c) {
+      out.push_back(cur);
+      cur.clear();
+    } else {
+      cur.push_back(ch);
+    }
+  }
+  out.push_back(cur);
+  return out;
+}
+
+std::vector<std::string> clean_lines(const std::string &text) {
+  std::vector<std::string> out;
+  std::istringstream is(text);
+  std::string line;
+  while (std::getline(is, line)) {
+    std::string t;
+    for (char ch : line)
+      if (ch != ' ' && ch != '\t' && ch != '\r') t.push_back(ch);
+    if (t.empty()) continue;
+    if (t.size() >= 2 && t.compare(0, 2, "//") == 0) continue;
+    out.push_back(t);
+  }
+  if (out.empty() || out.back().compare(0, 5, "start") != 0) out.push_back("start");
+  return out;
+}
+
+bool collect(const std::string &text, int start, Parsed &p, std::string &err) {
+  auto lines = clean_lines(text);
+  int n_start = 0;
+  for (auto &l : lines) {
+    if (l.compare(0, 5, "start") == 0) {
+      n_start++;
+      if (n_start >= start) break;
+      continue;
+    }
+    auto f = split(l, ':');
+    if (f.size() < 4) {
+      err = "input line '" + l + "' does not have the form key : name : prop : value";
+      return false;
+    }
+    Line ln;
+    ln.key = f[0];
+    ln.name = f[1];
+    ln.prop = f[2];
+    ln.value = f.size() >= 4 ? f[3] : "";
+    if (f.size() > 4) {  // values may contain ':' only in exotic keys; keep the rest
+      for (size_t i = 4; i < f.size(); i++) ln.value += ":" + f[i];
+    }
+    if (ln.key == "mon" && std::find(p.mons.begin(), p.mons.end(), ln.name) == p.mons.end())
+      p.mons.push_back(ln.name);
+    if (ln.key == "mol" && std::find(p.mols.begin(), p.mols.end(), ln.name) == p.mols.end())
+      p.mols.push_back(ln.name);
+    if (ln.key == "lat") p.lat_name = ln.name;
+    if (ln.key == "sys") p.sys_name = ln.name;
+    if (ln.key == "newton") p.newton_name = ln.name;
+    p.lines.push_back(ln);
+  }
+  if (n_start < start && start > 1) {
+    err = "input has fewer than the requested number of start blocks";
+    return false;
+  }
+  return true;
+}
+
+// last value wins
+bool get(const Parsed &p, const std::string &key, const std::string &name, const std::string &prop,
+         std::string &val) {
+  bool found = false;
+  for (auto &l : p.lines)
+    if (l.key == key && l.name == name && l.prop == prop) {
+      val = l.value;
+      found = true;
+    }
+  return found;
+}
+
+bool to_int(const std::string &s, int &v) {
+  if (s.empty()) return false;
+  char *end = nullptr;
+  long r = std::strtol(s.c_str(), &end, 10);
+  if (*end != 0) return false;
+  v = (int)r;
+  return true;
+}
+
+bool to_real(const std::string &s, double &v) {
+  if (s.empty()) return false;
+  char *end = nullptr;
+  double r = std::strtod(s.c_str(), &end);
+  if (*end != 0) return false;
+  v = r;
+  return true;
+}
+
+bool to_bool(const std::string &s, bool &v) {
+  if (s == "true" || s == "TRUE" || s == "True" || s == "1") {
+    v = true;
+    return true;
+  }
+  if (s == "false" || s == "FALSE" || s == "False" || s == "0") {
+    v = false;
+    return true;
+  }
+  return false;
+}
+
+int bc_of(const std::string &s, bool &ok) {
+  ok = true;
+  if (s == "mirror") return BC_MIRROR;
+  if (s == "periodic") return BC_PERIODIC;
+  if (s == "surface") return BC_SURFACE;
+  ok = false;
+  return BC_MIRROR;
+}
+
+// Boundary index tables for fjc == 1 (src/lattice.cpp:94-196).
+void boundary_indices(LatticeSpec &l) {
+  auto lower = [](int bc, int Mdim) { return bc == BC_MIRROR ? 1 : (bc == BC_PERIODIC ? Mdim : 0); };
+  auto upper = [](int bc, int Mdim) { return bc == BC_MIRROR ? Mdim : (bc == BC_PERIODIC ? 1 : Mdim + 1); };
+  l.BX1 = lower(l.bc[0], l.MX);
+  l.BXM = upper(l.bc[3], l.MX);
+  l.BY1 = lower(l.bc[1], l.MY);
+  l.BYM = upper(l.bc[4], l.MY);
+  l.BZ1 = lower(l.bc[2], l.MZ);
+  l.BZM = upper(l.bc[5], l.MZ);
+}
+
+// Site volumes and lambda weights (LG1Planar::ComputeLambdas
+// src/LG1Planar.cpp:11-26, LGrad1::ComputeLambdas src/LGrad1.cpp:16-68,
+// LGrad2::ComputeLambdas src/LGrad2.cpp:13-49).
+void compute_lambdas(LatticeSpec &l, bool markov2) {
+  const int M = l.M;
+  if (l.gradients == 3) return;
+  l.L.assign(M, 0.0);
+  l.lambda_1.assign(M, 0.0);
+  l.lambda0.assign(M, 0.0);
+  l.lambda1.assign(M, 0.0);
+  const double lam = l.lambda;
+  if (l.gradients == 1) {
+    if (l.geometry == GEOM_PLANAR) {
+      for (int i = 1; i < l.MX + 1; i++) l.L[i] = 1.0;
+    } else if (l.geometry == GEOM_CYLINDRICAL) {
+      for (int i = 1; i < l.MX + 1; i++) {
+        double r = l.offset_first_layer + i;
+        l.L[i] = kPIE * (std::pow(r, 2) - std::pow(r - 1, 2));
+        l.lambda1[i] = 2.0 * kPIE * r / l.L[i] * lam;
+        l.lambda_1[i] = 2.0 * kPIE * (r - 1) / l.L[i] * lam;
+        l.lambda0[i] = 1.0 - 2.0 * lam;
+      }
+    } else {
+      for (int i = 1; i < l.MX + 1; i++) {
+        double r = l.offset_first_layer + i;
+        l.L[i] = 4.0 / 3.0 * kPIE * (std::pow(r, 3) - std::pow(r - 1, 3));
+        l.lambda1[i] = 4.0 * kPIE * std::pow(r, 2) / l.L[i] * lam;
+        l.lambda_1[i] = 4.0 * kPIE * std::pow(r - 1, 2) / l.L[i] * lam;
+        l.lambda0[i] = 1.0 - l.lambda1[i] - l.lambda_1[i];
+      }
+    }
+    if (markov2 && l.geometry != GEOM_PLANAR) {
+      l.l1.assign(M, 0.0);
+      l.l11.assign(M, 0.0);
+      l.l_1.assign(M, 0.0);
+      l.l_11.assign(M, 0.0);
+      for (int i = 1; i < l.MX + 1; i++) {
+        l.l1[i] = l.lambda1[i] / lam;
+        l.l11[i] = 1.0 - l.l1[i];
+        l.l_1[i] = l.lambda_1[i] / lam;
+        l.l_11[i] = 1.0 - l.l_1[i];
+      }
+    }
+  } else {  // 2 gradients
+    if (l.geometry == GEOM_CYLINDRICAL) {
+      for (int x = 1; x < l.MX + 1; x++)
+        for (int y = 1; y < l.MY + 1; y++) {
+          int p = x * l.JX + y;
+          double r = l.offset_first_layer + 1.0 * x;
+          l.L[p] = kPIE * (std::pow(r, 2) - std::pow(r - 1, 2));
+          l.lambda1[p] = 2.0 * kPIE * r / l.L[p] * lam;
+          l.lambda_1[p] = 2.0 * kPIE * (r - 1) / l.L[p] * lam;
+          l.lambda0[p] = 1.0 - 2.0 * lam;
+        }
+    } else {
+      // planar 2-D: L=1 on every site (LG2Planar::ComputeLambdas src/LG2Planar.cpp:15-20)
+      for (int i = 0; i < M; i++) l.L[i] = 1.0;
+    }
+  }
+}
+
+// Parse a coordinate token of a range (firstlayer / lastlayer / lowerbound /
+// upperbound / number) for dimension d.
+bool coord(const LatticeSpec &l, const std::string &tok, int d, int &v) {
+  int n = d == 0 ? l.MX : (d == 1 ? l.MY : l.MZ);
+  if (tok == "firstlayer") {
+    v = 1;
+    return true;
+  }
+  if (tok == "lastlayer") {
+    v = n;
+    return true;
+  }
+  if (tok == "lowerbound") {
+    v = 0;
+    return l.bc[d] == BC_SURFACE;
+  }
+  if (tok == "upperbound") {
+    v = n + 1;
+    return l.bc[d + 3] == BC_SURFACE;
+  }
+  return to_int(tok, v);
+}
+
+// frozen_range / pinned_range (Segment::ParseFreedoms src/segment.cpp:230-560,
+// LGrad1/LGrad2/LGrad3::ReadRange + CreateMASK): a block of sites given by
+// two corner points, or one of the keywords lowerbound[_x|_y|_z],
+// upperbound[_x|_y|_z], firstlayer[_x...], lastlayer[_x...].
+bool parse_range(const LatticeSpec &l, std::string r, std::vector<int> &mask, std::string &err) {
+  const int g = l.gradients;
+  auto whole = [&](const std::string &which, int d) {
+    // build "a;b" where dimension d is fixed to which and others span 1..n
+    std::string lo, hi;
+    for (int k = 0; k < g; k++) {
+      int n = k == 0 ? l.MX : (k == 1 ? l.MY : l.MZ);
+      std::string a = k == d ? which : "1";
+      std::string b = k == d ? which : std::to_string(n);
+      lo += (k ? "," : "") + a;
+      hi += (k ? "," : "") + b;
+    }
+    return lo + ";" + hi;
+  };
+  static const char *axes[3] = {"_x", "_y", "_z"};
+  for (const char *kw : {"lowerbound", "upperbound", "firstlayer", "lastlayer"}) {
+    if (r == kw) r = whole(kw, 0);
+    for (int d = 0; d < 3; d++)
+      if (r == std::string(kw) + axes[d]) r = whole(kw, d);
+  }
+  auto pts = split(r, ';');
+  if (pts.size() != 2) {
+    err = "range '" + r + "' must have the form x1[,y1[,z1]];x2[,y2[,z2]]";
+    return false;
+  }
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int k = 0; k < 2; k++) {
+    auto c = split(pts[k], ',');
+    if ((int)c.size() != g) {
+      err = "range '" + r + "' needs " + std::to_string(g) + " coordinates per point";
+      return false;
+    }
+    for (int d = 0; d < g; d++) {
+      int v;
+      if (!coord(l, c[d], d, v)) {
+        err = "range coordinate '" + c[d] + "' not understood (a bound needs a surface boundary)";
+        return false;
+      }
+      (k == 0 ? lo : hi)[d] = v;
+    }
+  }
+  for (int d = 0; d < g; d++)
+    if (lo[d] > hi[d]) {
+      err = "range '" + r + "' has its first point above its second";
+      return false;
+    }
+  for (int x = lo[0]; x <= hi[0]; x++)
+    for (int y = (g > 1 ? lo[1] : 0); y <= (g > 1 ? hi[1] : 0); y++)
+      for (int z = (g > 2 ? lo[2] : 0); z <= (g > 2 ? hi[2] : 0); z++) mask[site(l, x, y, z)] = 1;
+  return true;
+}
+
+// "(A)10(B)10" -> blocks (linear chains only)
+bool parse_composition(const std::string &c, const std::vector<std::string> &mons,
+                       std::vector<BlockSpec> &blocks, std::string &err) {
+  size_t i = 0;
+  while (i < c.size()) {
+    if (c[i] != '(') {
+      err = "composition '" + c + "': only linear block sequences like (A)10(B)10 are supported";
+      return false;
+    }
+    size_t close = c.find(')', i);
+    if (close == std::string::npos) {
+      err = "composition '" + c + "': unbalanced bracket";
+      return false;
+    }
+    std::string name = c.substr(i + 1, close - i - 1);
+    if (name.find('(') != std::string::npos || name.find('[') != std::string::npos) {
+      err = "composition '" + c + "': branched architectures are not supported";
+      return false;
+    }
+    size_t j = close + 1;
+    while (j < c.size() && isdigit((unsigned char)c[j])) j++;
+    int n = 0;
+    if (!to_int(c.substr(close + 1, j - close - 1), n) || n < 1) {
+      err = "composition '" + c + "': block length must be a positive integer";
+      return false;
+    }
+    auto it = std::find(mons.begin(), mons.end(), name);
+    if (it == mons.end()) {
+      err = "composition '" + c + "': segment '" + name + "' is not defined";
+      return false;
+    }
+    blocks.push_back({(int)(it - mons.begin()), n});
+    i = j;
+  }
+  if (blocks.empty()) {
+    err = "empty composition";
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+int count_starts(const std::string &text) {
+  auto lines = clean_lines(text);
+  int n = 0;
+  for (auto &l : lines)
+    if (l.compare(0, 5, "start") == 0) n++;
+  return n;
+}
+
+bool parse_input(const std::string &text, int start, SystemSpec &s, std::string &err) {
+  Parsed p;
+  if (start < 1) start = 1;
+  if (!collect(text, start, p, err)) return false;
+  s = SystemSpec();
+  s.sys_name = p.sys_name;
+  if (!p.newton_name.empty()) s.newton_name = p.newton_name;
+  std::string v;
+  // ---------------- lattice (src/lattice.cpp:267-571) ----------------
+  LatticeSpec &l = s.lat;
+  const std::string &ln = p.lat_name;
+  if (ln.empty()) {
+    err = "no 'lat' block";
+    return false;
+  }
+  if (get(p, "lat", ln, "gradients", v) && !to_int(v, l.gradients)) l.gradients = 1;
+  if (l.gradients < 1 || l.gradients > 3) l.gradients = 1;
+  l.fjc = 1;
+  l.FJC = 3;
+  if (get(p, "lat", ln, "FJC_choices", v)) {
+    if (!to_int(v, l.FJC) || l.FJC != 3) {
+      err = "FJC_choices other than 3 (lattice refinement) is not supported";
+      return false;
+    }
+  }
+  if (get(p, "lat", ln, "b/l", v)) {
+    int b;
+    if (!to_int(v, b) || b != 1) {
+      err = "b/l other than 1 (lattice refinement) is not supported";
+      return false;
+    }
+  }
+  if (!get(p, "lat", ln, "lattice_type", v)) {
+    err = "'lattice_type' is required";
+    return false;
+  }
+  if (v == "simple_cubic") {
+    l.lattice_type = LAT_SIMPLE_CUBIC;
+    l.lambda = 1.0 / 6.0;
+  } else if (v == "hexagonal") {
+    l.lattice_type = LAT_HEXAGONAL;
+    l.lambda = 1.0 / 4.0;
+  } else {
+    err = "lattice_type '" + v + "' not recognised";
+    return false;
+  }
+  l.geometry = GEOM_PLANAR;
+  if (get(p, "lat", ln, "geometry", v)) {
+    if (v == "planar" || v == "flat")
+      l.geometry = GEOM_PLANAR;
+    else if (v == "cylindrical")
+      l.geometry = GEOM_CYLINDRICAL;
+    else if (v == "spherical")
+      l.geometry = GEOM_SPHERICAL;
+    else {
+      err = "geometry '" + v + "' not recognised";
+      return false;
+    }
+  }
+  if (l.gradients == 3 && l.geometry != GEOM_PLANAR) {
+    err = "3 gradients require planar geometry";
+    return false;
+  }
+  if (l.gradients == 2 && l.geometry == GEOM_SPHERICAL) {
+    err = "2 gradients do not support spherical geometry";
+    return false;
+  }
+  if (l.geometry != GEOM_PLANAR && get(p, "lat", ln, "offset_first_layer", v)) {
+    if (!to_real(v, l.offset_first_layer) || l.offset_first_layer < 0) l.offset_first_layer = 0;
+  }
+  auto need_int = [&](const char *k, int &dst) {
+    if (!get(p, "lat", ln, k, v) || !to_int(v, dst) || dst < 1) {
+      err = std::string("'") + k + "' is required and must be positive";
+      return false;
+    }
+    return true;
+  };
+  bool ok;
+  if (l.gradients == 1) {
+    if (!need_int("n_layers", l.MX)) return false;
+    l.bc[0] = get(p, "lat", ln, "lowerbound", v) ? bc_of(v, ok) : BC_MIRROR;
+    if (get(p, "lat", ln, "lowerbound", v) && !ok) {
+      err = "lowerbound '" + v + "' not recognised";
+      return false;
+    }
+    l.bc[3] = get(p, "lat", ln, "upperbound", v) ? bc_of(v, ok) : BC_MIRROR;
+    if (get(p, "lat", ln, "upperbound", v) && !ok) {
+      err = "upperbound '" + v + "' not recognised";
+      return false;
+    }
+  } else {
+    if (!need_int("n_layers_x", l.MX) || !need_int("n_layers_y", l.MY)) return false;
+    if (l.gradients == 3 && !need_int("n_layers_z", l.MZ)) return false;
+    static const char *lo[3] = {"lowerbound_x", "lowerbound_y", "lowerbound_z"};
+    static const char *hi[3] = {"upperbound_x", "upperbound_y", "upperbound_z"};
+    for (int d = 0; d < l.gradients; d++) {
+      for (int side = 0; side < 2; side++) {
+        const char *k = side ? hi[d] : lo[d];
+        int &b = l.bc[d + 3 * side];
+        b = BC_MIRROR;
+        if (get(p, "lat", ln, k, v)) {
+          b = bc_of(v, ok);
+          if (!ok) {
+            err = std::string(k) + " '" + v + "' not recognised";
+            return false;
+          }
+          if (l.gradients == 3 && b == BC_SURFACE) {
+            err = "surface boundaries are not available with 3 gradients";
+            return false;
+          }
+          if (l.gradients == 2 && l.geometry != GEOM_PLANAR && b == BC_PERIODIC) {
+            err = "periodic boundaries need planar geometry in 2 gradients";
+            return false;
+          }
+        }
+      }
+      if ((l.bc[d] == BC_PERIODIC) != (l.bc[d + 3] == BC_PERIODIC)) {
+        err = "a periodic boundary must be set on both sides";
+        return false;
+      }
+    }
+  }
+  if (l.gradients == 1 && ((l.bc[0] == BC_PERIODIC) != (l.bc[3] == BC_PERIODIC))) {
+    err = "a periodic boundary must be set on both sides";
+    return false;
+  }
+  l.stencil_full = true;
+  if (get(p, "lat", ln, "stencil_full", v)) {
+    bool b;
+    if (to_bool(v, b)) l.stencil_full = b;
+  }
+  // PutM (src/LGrad1.cpp:180-190, src/LGrad2.cpp:101-111, src/LGrad3.cpp:15-23)
+  if (l.gradients == 1) {
+    l.JX = 1;
+    l.JY = 0;
+    l.JZ = 0;
+    l.M = l.MX + 2;
+    if (l.geometry == GEOM_PLANAR) l.volume = l.MX;
+    if (l.geometry == GEOM_SPHERICAL)
+      l.volume = 4.0 / 3.0 * kPIE * (std::pow(l.MX + l.offset_first_layer, 3) - std::pow(l.offset_first_layer, 3));
+    if (l.geometry == GEOM_CYLINDRICAL)
+      l.volume = kPIE * (std::pow(l.MX + l.offset_first_layer, 2) - std::pow(l.offset_first_layer, 2));
+  } else if (l.gradients == 2) {
+    l.JX = l.MY + 2;
+    l.JY = 1;
+    l.JZ = 0;
+    l.M = (l.MX + 2) * (l.MY + 2);
+    if (l.geometry == GEOM_CYLINDRICAL)
+      l.volume = l.MY * kPIE * (std::pow(l.MX + l.offset_first_layer, 2) - std::pow(l.offset_first_layer, 2));
+    else
+      l.volume = (double)l.MX * l.MY;
+  } else {
+    l.JX = (l.MZ + 2) * (l.MY + 2);
+    l.JY = l.MZ + 2;
+    l.JZ = 1;
+    l.M = (l.MX + 2) * (l.MY + 2) * (l.MZ + 2);
+    l.volume = (double)l.MX * l.MY * l.MZ;
+  }
+  boundary_indices(l);
+
+  // ---------------- segments ----------------
+  const int nseg = (int)p.mons.size();
+  if (nseg == 0) {
+    err = "no 'mon' defined";
+    return false;
+  }
+  s.seg.resize(nseg);
+  for (int i = 0; i < nseg; i++) {
+    SegmentSpec &sg = s.seg[i];
+    sg.name = p.mons[i];
+    sg.mask.assign(l.M, 0);
+    if (!get(p, "mon", sg.name, "freedom", v)) {
+      err = "mon '" + sg.name + "' has no 'freedom'";
+      return false;
+    }
+    if (v == "free")
+      sg.freedom = SEG_FREE;
+    else if (v == "frozen")
+      sg.freedom = SEG_FROZEN;
+    else if (v == "pinned")
+      sg.freedom = SEG_PINNED;
+    else {
+      err = "mon '" + sg.name + "': freedom '" + v + "' is not supported (free, frozen, pinned)";
+      return false;
+    }
+    if (sg.freedom == SEG_FROZEN) {
+      if (!get(p, "mon", sg.name, "frozen_range", v)) {
+        err = "mon '" + sg.name + "': frozen segments need 'frozen_range'";
+        return false;
+      }
+      if (!parse_range(l, v, sg.mask, err)) return false;
+    }
+    if (sg.freedom == SEG_PINNED) {
+      if (!get(p, "mon", sg.name, "pinned_range", v)) {
+        err = "mon '" + sg.name + "': pinned segments need 'pinned_range'";
+        return false;
+      }
+      if (!parse_range(l, v, sg.mask, err)) return false;
+    }
+  }
+  // chi matrix (Segment::CheckInput src/segment.cpp:1105-1119,
+  // System::CheckChi_values src/system.cpp:1718-1800): unset entries take
+  // the transposed value, otherwise 0.
+  const double unset = -999.0;
+  std::vector<double> chi(nseg * nseg, unset);
+  for (int i = 0; i < nseg; i++)
+    for (int k = 0; k < nseg; k++)
+      if (get(p, "mon", s.seg[i].name, "chi_" + s.seg[k].name, v)) {
+        double c;
+        if (!to_real(v, c)) {
+          err = "chi value '" + v + "' is not a number";
+          return false;
+        }
+        chi[i * nseg + k] = (i == k) ? 0.0 : c;
+      }
+  for (int i = 0; i < nseg; i++)
+    for (int k = 0; k < nseg; k++) {
+      double &a = chi[i * nseg + k], &b = chi[k * nseg + i];
+      if (a == unset && b == unset) {
+        a = b = 0.0;
+      } else if (a != unset && b == unset) {
+        b = a;
+      }
+    }
+  for (int i = 0; i < nseg; i++)
+    for (int k = i + 1; k < nseg; k++)
+      if (chi[i * nseg + k] != chi[k * nseg + i]) {
+        err = "chi(" + s.seg[i].name + "," + s.seg[k].name + ") is not symmetric";
+        return false;
+      }
+  for (int i = 0; i < nseg; i++) s.seg[i].chi.assign(chi.begin() + i * nseg, chi.begin() + (i + 1) * nseg);
+
+  // ---------------- molecules ----------------
+  const int nmol = (int)p.mols.size();
+  if (nmol == 0) {
+    err = "no 'mol' defined";
+    return false;
+  }
+  s.mol.resize(nmol);
+  bool any_markov2 = false;
+  for (int i = 0; i < nmol; i++) {
+    MoleculeSpec &m = s.mol[i];
+    m.name = p.mols[i];
+    if (!get(p, "mol", m.name, "composition", v)) {
+      err = "mol '" + m.name + "' has no composition";
+      return false;
+    }
+    if (!parse_composition(v, p.mons, m.blocks, err)) return false;
+    m.chainlength = 0;
+    for (auto &b : m.blocks) {
+      m.chainlength += b.n;
+      if (s.seg[b.seg].freedom == SEG_FROZEN) {
+        err = "mol '" + m.name + "' contains a frozen segment";
+        return false;
+      }
+      if (s.seg[b.seg].freedom == SEG_PINNED) m.pinned = true;
+      auto it = std::find(m.molmon.begin(), m.molmon.end(), b.seg);
+      if (it == m.molmon.end()) {
+        m.block_molmon.push_back((int)m.molmon.size());
+        m.molmon.push_back(b.seg);
+      } else {
+        m.block_molmon.push_back((int)(it - m.molmon.begin()));
+      }
+    }
+    if (!get(p, "mol", m.name, "freedom", v)) {
+      err = "mol '" + m.name + "' has no freedom";
+      return false;
+    }
+    if (v == "free")
+      m.freedom = MOL_FREE;
+    else if (v == "restricted")
+      m.freedom = MOL_RESTRICTED;
+    else if (v == "solvent")
+      m.freedom = MOL_SOLVENT;
+    else {
+      err = "mol '" + m.name + "': freedom '" + v + "' is not supported (free, restricted, solvent)";
+      return false;
+    }
+    if (m.pinned && m.freedom != MOL_RESTRICTED) {
+      err = "mol '" + m.name + "' is pinned and must be restricted";
+      return false;
+    }
+    if (m.freedom == MOL_FREE) {
+      if (!get(p, "mol", m.name, "phibulk", v) || !to_real(v, m.phibulk) || m.phibulk < 0 || m.phibulk > 1) {
+        err = "mol '" + m.name + "': free molecules need phibulk in 0..1";
+        return false;
+      }
+    }
+    if (m.freedom == MOL_RESTRICTED) {
+      bool hn = get(p, "mol", m.name, "n", v);
+      std::string vt;
+      bool ht = get(p, "mol", m.name, "theta", vt);
+      if (hn == ht) {
+        err = "mol '" + m.name + "': restricted molecules need exactly one of 'n' or 'theta'";
+        return false;
+      }
+      if (hn) {
+        if (!to_real(v, m.n)) {
+          err = "mol '" + m.name + "': bad n";
+          return false;
+        }
+        m.theta = m.n * m.chainlength;
+      } else {
+        if (!to_real(vt, m.theta)) {
+          err = "mol '" + m.name + "': bad theta";
+          return false;
+        }
+        m.n = m.theta / m.chainlength;
+      }
+      if (m.theta < 0 || m.theta > l.volume) {
+        err = "mol '" + m.name + "': theta out of range 0..volume";
+        return false;
+      }
+    }
+    m.markov = 1;
+    if (get(p, "mol", m.name, "Markov", v) && (!to_int(v, m.markov) || m.markov < 1 || m.markov > 2)) {
+      err = "mol '" + m.name + "': Markov must be 1 or 2";
+      return false;
+    }
+    m.k_stiff = 0.0;
+    if (get(p, "mol", m.name, "k_stiff", v) && (!to_real(v, m.k_stiff) || m.k_stiff < 0 || m.k_stiff > 10)) {
+      err = "mol '" + m.name + "': k_stiff out of range 0..10";
+      return false;
+    }
+    m.save_memory = false;
+    if (get(p, "mol", m.name, "save_memory", v)) to_bool(v, m.save_memory);
+    if (get(p, "mol", m.name, "ring", v)) {
+      bool r = false;
+      if (to_bool(v, r) && r) {
+        err = "mol '" + m.name + "': rings are not supported";
+        return false;
+      }
+    }
+    m.size = 1;
+    if (m.markov == 2) {
+      if (l.gradients != 1 || l.lattice_type != LAT_SIMPLE_CUBIC) {
+        err = "mol '" + m.name + "': Markov 2 is supported on 1-gradient simple-cubic lattices";
+        return false;
+      }
+      any_markov2 = true;
+      m.size = l.FJC;
+      // Bending weights (Molecule::AllocateMemory src/molecule.cpp:133-153)
+      m.P.assign(l.FJC, 0.0);
+      double Q = 0;
+      for (int k = 0; k < l.FJC - 1; k++) {
+        double a = k * kPIE / (l.FJC - 1);
+        m.P[k] = std::exp(-0.5 * m.k_stiff * a * a);
+        if (k > 0)
+          Q += (l.lattice_type == LAT_HEXAGONAL ? 2 : 4) * m.P[k];
+        else
+          Q = m.P[k];
+      }
+      m.P[l.FJC - 1] = 0;
+      if (l.lattice_type == LAT_HEXAGONAL && !l.stencil_full) Q *= 2.0;
+      for (int k = 0; k < l.FJC; k++) m.P[k] /= Q;
+    }
+    if (m.freedom == MOL_SOLVENT) {
+      if (s.solvent >= 0) {
+        err = "more than one solvent molecule";
+        return false;
+      }
+      s.solvent = i;
+    }
+  }
+  if (s.solvent < 0) {
+    if (!(nmol == 1 && s.mol[0].pinned)) {
+      err = "no solvent molecule (freedom : solvent) found";
+      return false;
+    }
+  }
+  compute_lambdas(l, any_markov2);
+
+  // KSAM (System::generate_mask src/system.cpp:204-273)
+  s.ksam.assign(l.M, 1);
+  for (int i = 0; i < nseg; i++)
+    if (s.seg[i].freedom == SEG_FROZEN)
+      for (int z = 0; z < l.M; z++)
+        if (s.seg[i].mask[z]) s.ksam[z] = 0;
+
+  // Iteration list (System::MakeItsLists src/system.cpp:429-500 with
+  // IsUnique src/system.cpp: segments with identical chi rows share a field).
+  std::vector<int> sysmon;
+  for (auto &m : s.mol)
+    for (int sg : m.molmon) {
+      if (std::find(sysmon.begin(), sysmon.end(), sg) != sysmon.end()) continue;
+      sysmon.push_back(sg);
+      int copy = -1;
+      for (size_t k = 0; k < s.itmon.size() && copy < 0; k++)
+        if (s.seg[s.itmon[k]].chi == s.seg[sg].chi) copy = (int)k;
+      if (copy < 0) {
+        s.seg[sg].field_block = (int)s.itmon.size();
+        s.itmon.push_back(sg);
+      } else {
+        s.seg[sg].field_block = copy;
+      }
+    }
+
+  // ---------------- solver (src/solve_scf.cpp:114-281) ----------------
+  const std::string &nn = p.newton_name;
+  SolverSpec &so = s.solver;
+  if (!nn.empty()) {
+    if (get(p, "newton", nn, "iterationlimit", v)) {
+      if (!to_int(v, so.iterationlimit) || so.iterationlimit < 0 || so.iterationlimit > 1000000)
+        so.iterationlimit = 1000;
+    }
+    if (get(p, "newton", nn, "deltamax", v)) {
+      if (!to_real(v, so.deltamax) || so.deltamax < 0 || so.deltamax > 100) so.deltamax = 0.1;
+    }
+    if (get(p, "newton", nn, "tolerance", v)) {
+      if (!to_real(v, so.tolerance)) so.tolerance = 1e-7;
+      if (so.tolerance < 1e-16 || so.tolerance > 10) so.tolerance = 1e-5;
+    }
+    if (get(p, "newton", nn, "method", v)) {
+      if (v == "DIIS")
+        so.method = SOLVER_DIIS;
+      else if (v == "pseudohessian")
+        so.method = SOLVER_PSEUDOHESSIAN;
+      else {
+        err = "newton method '" + v + "' is not supported (DIIS, pseudohessian)";
+        return false;
+      }
+    }
+    if (get(p, "newton", nn, "m", v)) {
+      if (!to_int(v, so.m) || so.m < 0 || so.m > 100) so.m = 10;
+    }
+    if (get(p, "newton", nn, "stop_criterion", v)) so.max_g = (v == "max_of_element_of_|g|");
+  }
+  if (get(p, "sys", p.sys_name, "initial_guess", v)) s.initial_guess = v;
+  if (get(p, "sys", p.sys_name, "guess_inputfile", v)) s.guess_inputfile = v;
+  return true;
+}
+
+std::string describe_json(const SystemSpec &s) {
+  std::ostringstream o;
+  auto str = [&](const std::string &v) { o << '"' << v << '"'; };
+  o << "{\"sys_name\":";
+  str(s.sys_name);
+  o << ",\"newton_name\":";
+  str(s.newton_name);
+  const LatticeSpec &l = s.lat;
+  o << ",\"gradients\":" << l.gradients << ",\"geometry\":" << l.geometry << ",\"MX\":" << l.MX
+    << ",\"MY\":" << l.MY << ",\"MZ\":" << l.MZ << ",\"M\":" << l.M << ",\"JX\":" << l.JX
+    << ",\"JY\":" << l.JY << ",\"nvar\":" << s.nvar() << ",\"solvent\":" << s.solvent;
+  o << ",\"segments\":[";
+  for (size_t i = 0; i < s.seg.size(); i++) {
+    if (i) o << ",";
+    o << "{\"name\":";
+    str(s.seg[i].name);
+    o << ",\"freedom\":" << s.seg[i].freedom << ",\"field_block\":" << s.seg[i].field_block << "}";
+  }
+  o << "],\"molecules\":[";
+  for (size_t i = 0; i < s.mol.size(); i++) {
+    const MoleculeSpec &m = s.mol[i];
+    if (i) o << ",";
+    o << "{\"name\":";
+    str(m.name);
+    o << ",\"chainlength\":" << m.chainlength << ",\"freedom\":" << m.freedom << ",\"markov\":" << m.markov
+      << ",\"pinned\":" << (m.pinned ? "true" : "false") << ",\"save_memory\":" << (m.save_memory ? "true" : "false")
+      << ",\"size\":" << m.size << "}";
+  }
+  o << "],\"fields\":[";
+  for (size_t i = 0; i < s.itmon.size(); i++) {
+    if (i) o << ",";
+    str(s.seg[s.itmon[i]].name);
+  }
+  o << "],\"solver\":{\"method\":" << s.solver.method << ",\"tolerance\":" << s.solver.tolerance
+    << ",\"deltamax\":" << s.solver.deltamax << ",\"m\":" << s.solver.m
+    << ",\"iterationlimit\":" << s.solver.iterationlimit << ",\"max_g\":" << (s.solver.max_g ? "true" : "false")
+    << "}}";
+  return o.str();
+}
+
+bool read_guess_text(const std::string &text, const SystemSpec &s, std::vector<double> &x, std::string &err) {
+  std::istringstream is(text);
+  std::string method, charged;
+  int mx, my, mz, fjc, n1, n2;
+  if (!(is >> method >> mx >> my >> mz >> fjc >> charged >> n1)) {
+    err = "guess file header not understood";
+    return false;
+  }
+  std::vector<std::string> names(n1);
+  for (int i = 0; i < n1; i++) is >> names[i];
+  if (!(is >> n2)) {
+    err = "guess file header not understood";
+    return false;
+  }
+  for (int i = 0; i < n2; i++) {
+    std::string t;
+    is >> t;
+  }
+  const LatticeSpec &l = s.lat;
+  int m;
+  if (mz > 0)
+    m = (mx + 2) * (my + 2) * (mz + 2);
+  else if (my > 0)
+    m = (mx + 2 * fjc) * (my + 2 * fjc);
+  else
+    m = mx + 2 * fjc;
+  if (mx != l.MX || my != l.MY || mz != l.MZ || m != l.M) {
+    err = "guess file lattice size differs from the system";
+    return false;
+  }
+  int iv = (n1 + n2) * m + (charged == "true" ? m : 0);
+  std::vector<double> vals(iv);
+  for (int i = 0; i < iv; i++)
+    if (!(is >> vals[i])) {
+      err = "guess file has too few values";
+      return false;
+    }
+  x.assign(s.nvar(), 0.0);
+  for (int i = 0; i < n1; i++)
+    for (size_t j = 0; j < s.itmon.size(); j++)
+      if (names[i] == s.seg[s.itmon[j]].name)
+        std::copy(vals.begin() + i * m, vals.begin() + (i + 1) * m, x.begin() + j * l.M);
+  return true;
+}
+
+}  // namespace nscf

@@ -1,0 +1,141 @@
+// System description for the lattice self-consistent-field (SF-SCF) hot path.
+//
+// One SystemSpec is everything the residual and solver need about one SCF
+// problem: the lattice (geometry, boundary conditions, lambda weights), the
+// segment types (freedoms, masks, chi matrix), the chain molecules (block
+// sequences, normalisation mode, Markov order) and the solver settings.  It
+// is built from the reference's `.in` text format by parse_input() so the
+// same input files drive the CPU checker and the GPU path.
+//
+// Reference counterparts: Lattice::CheckInput / AllocateMemory / ComputeLambdas
+// (src/lattice.cpp:267-601, src/LGrad1.cpp:16-177, src/LGrad2.cpp:13-100,
+// src/LG1Planar.cpp:11-26), Segment::ParseFreedoms (src/segment.cpp:156-700),
+// Molecule::CheckInput (src/molecule.cpp:356-677), System::CheckChi_values /
+// MakeItsLists / IsUnique / generate_mask (src/system.cpp:204-500,1718-1790),
+// Solve_scf::CheckInput (src/solve_scf.cpp:114-281), Input parsing
+// (src/input.cpp:3-78,286-429,706-740).
+#pragma once
+#include <string>
+#include <vector>
+#include <cstdint>
+
+namespace nscf {
+
+// Constant used by the reference for pi (src/namics.cpp:49).  It enters the
+// curved-lattice weights and the Markov-2 bending weights, so the same value
+// is used here.
+constexpr double kPIE = 3.14159265;
+
+enum Geometry : int { GEOM_PLANAR = 0, GEOM_CYLINDRICAL = 1, GEOM_SPHERICAL = 2 };
+enum BoundaryKind : int { BC_MIRROR = 0, BC_PERIODIC = 1, BC_SURFACE = 2 };
+enum LatticeType : int { LAT_SIMPLE_CUBIC = 0, LAT_HEXAGONAL = 1 };
+enum SegFreedom : int { SEG_FREE = 0, SEG_FROZEN = 1, SEG_PINNED = 2 };
+enum MolFreedom : int { MOL_FREE = 0, MOL_RESTRICTED = 1, MOL_SOLVENT = 2 };
+enum SolverMethod : int { SOLVER_DIIS = 0, SOLVER_PSEUDOHESSIAN = 1 };
+
+struct LatticeSpec {
+  int gradients = 1;
+  int geometry = GEOM_PLANAR;
+  int MX = 0, MY = 0, MZ = 0;   // interior layers (times fjc)
+  int fjc = 1, FJC = 3;
+  int lattice_type = LAT_SIMPLE_CUBIC;
+  double lambda = 1.0 / 6.0;
+  // bc[0..2] lower x,y,z; bc[3..5] upper x,y,z (reference BC[6] order)
+  int bc[6] = {BC_MIRROR, BC_MIRROR, BC_MIRROR, BC_MIRROR, BC_MIRROR, BC_MIRROR};
+  double offset_first_layer = 0.0;
+  bool stencil_full = true;
+  // derived
+  int M = 0, JX = 0, JY = 0, JZ = 1;
+  int BX1 = 0, BXM = 0, BY1 = 0, BYM = 0, BZ1 = 0, BZM = 0;
+  double volume = 0.0;
+  // Site-dependent weights (1 and 2 gradients).  L is the site volume
+  // (1 inside a planar lattice, 0 on ghosts); lambda_1/lambda1 are the
+  // weights toward the lower/upper x neighbour, lambda0 the self weight.
+  std::vector<double> L, lambda_1, lambda0, lambda1;
+  // Markov-2 reflection weights for curved 1-gradient lattices.
+  std::vector<double> l1, l11, l_1, l_11;
+};
+
+struct SegmentSpec {
+  std::string name;
+  int freedom = SEG_FREE;
+  std::vector<int> mask;       // M entries, 1 on frozen / pinned sites
+  std::vector<double> chi;     // n_seg entries, symmetric chi row
+  int field_block = -1;        // which block of x holds this segment's field
+};
+
+struct BlockSpec {
+  int seg = 0;
+  int n = 0;
+};
+
+struct MoleculeSpec {
+  std::string name;
+  std::vector<BlockSpec> blocks;
+  std::vector<int> molmon;     // distinct segments in order of appearance
+  int chainlength = 0;
+  int freedom = MOL_FREE;
+  bool pinned = false;
+  bool has_n = false, has_theta = false;
+  double phibulk = 0.0, n = 0.0, theta = 0.0;
+  int markov = 1;
+  double k_stiff = 0.0;
+  bool save_memory = false;
+  int size = 1;                // direction components per site (Markov 2: FJC in 1-D)
+  std::vector<double> P;       // Markov-2 bending weights, FJC entries
+  // index of the molmon entry for every block
+  std::vector<int> block_molmon;
+};
+
+struct SolverSpec {
+  int method = SOLVER_PSEUDOHESSIAN;
+  double tolerance = 1e-7;
+  double deltamax = 0.1;
+  int m = 10;
+  int iterationlimit = 1000;
+  bool max_g = false;          // stop on max|g| instead of ||g||_2
+};
+
+struct SystemSpec {
+  std::string sys_name = "NN";
+  std::string newton_name = "NN";
+  LatticeSpec lat;
+  std::vector<SegmentSpec> seg;
+  std::vector<MoleculeSpec> mol;
+  SolverSpec solver;
+  int solvent = -1;
+  std::vector<int> itmon;      // segments owning a field block, in x order
+  std::vector<int> ksam;       // M: 1 on accessible sites, 0 on frozen sites
+  std::string initial_guess = "previous_result";
+  std::string guess_inputfile;
+  int nvar() const { return (int)itmon.size() * lat.M; }
+  int max_chain() const {
+    int n = 0;
+    for (auto &m : mol) n = m.chainlength > n ? m.chainlength : n;
+    return n;
+  }
+};
+
+// Parse the reference's `.in` text and build the system for the given start
+// block (1-based).  On failure returns false and fills err.
+bool parse_input(const std::string &text, int start, SystemSpec &out, std::string &err);
+
+// Number of `start` blocks in the text.
+int count_starts(const std::string &text);
+
+// Read a reference guess file (Lattice::ReadGuess format, src/lattice.cpp:926-968)
+// and map its field blocks onto x by segment name (Solve_scf::Guess,
+// src/solve_scf.cpp:489-524).  Returns false if the file does not fit.
+bool read_guess_text(const std::string &text, const SystemSpec &s, std::vector<double> &x,
+                     std::string &err);
+
+// JSON description of the parsed system (names, sizes, field blocks), used by
+// host tools and tests to lay out fields and results.
+std::string describe_json(const SystemSpec &s);
+
+// Lattice helpers shared by host code.
+inline int site(const LatticeSpec &l, int x, int y = 0, int z = 0) {
+  return x * l.JX + y * l.JY + z * l.JZ;
+}
+
+}  // namespace nscf

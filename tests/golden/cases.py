@@ -1,0 +1,238 @@
+"""Small SCF systems used for golden fixtures and parity tests.
+
+Each case is `.in` text in the reference's input format (no output, newton
+method or guess lines: the fixture generator and the tests add those).
+`eval` lists (seed, amplitude) pairs of uniform random fields at which the
+residual is evaluated; `converged` lists solver runs.
+"""
+
+CFG1 = """
+lat : flat : gradients : 1
+lat : flat : n_layers : 100
+lat : flat : lattice_type : simple_cubic
+lat : flat : lowerbound : surface
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -1.0
+mon : A : chi_W : 0.5
+mol : pol : composition : (A)100
+mol : pol : freedom : free
+mol : pol : phibulk : 0.01
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+newton : isaac : deltamax : 0.1
+"""
+
+CASES = {
+    # configuration 1: planar homopolymer adsorption
+    "cfg1_planar_adsorption": dict(
+        text=CFG1,
+        eval=[(1, 0.3), (2, 1.0)],
+        converged=[dict(method="pseudohessian", tol=1e-12),
+                   dict(method="DIIS", tol=1e-7)],
+    ),
+    # second-order Markov (bending energy k_stiff = 1) on the same lattice
+    "planar_markov2": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : n_layers : 80
+lat : flat : lattice_type : simple_cubic
+lat : flat : lowerbound : surface
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -1.2
+mon : A : chi_W : 0.6
+mol : pol : composition : (A)60
+mol : pol : freedom : free
+mol : pol : phibulk : 0.005
+mol : pol : Markov : 2
+mol : pol : k_stiff : 1
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(3, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-7)],
+    ),
+    # curved 1-gradient: cylindrical micelle of a restricted diblock
+    "cyl1d_micelle": dict(
+        text="""
+lat : L : gradients : 1
+lat : L : n_layers : 40
+lat : L : geometry : cylindrical
+lat : L : lattice_type : simple_cubic
+mon : A : freedom : free
+mon : B : freedom : free
+mon : W : freedom : free
+mon : B : chi_W : 1.5
+mon : A : chi_B : 1.0
+mol : surf : composition : (A)6(B)6
+mol : surf : freedom : restricted
+mol : surf : theta : 300
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(4, 0.5)],
+    ),
+    # curved 1-gradient: spherical particle with an adsorbing surface
+    "sph1d_particle": dict(
+        text="""
+lat : L : gradients : 1
+lat : L : n_layers : 50
+lat : L : geometry : spherical
+lat : L : offset_first_layer : 5
+lat : L : lattice_type : simple_cubic
+lat : L : lowerbound : surface
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -1.5
+mon : A : chi_W : 0.4
+mol : pol : composition : (A)30
+mol : pol : freedom : free
+mol : pol : phibulk : 0.02
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(5, 0.3)],
+    ),
+    # hexagonal lattice, periodic boundaries, restricted homopolymer
+    "hex_periodic_restricted": dict(
+        text="""
+lat : L : gradients : 1
+lat : L : n_layers : 60
+lat : L : lattice_type : hexagonal
+lat : L : lowerbound : periodic
+lat : L : upperbound : periodic
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.7
+mol : pol : composition : (A)20
+mol : pol : freedom : restricted
+mol : pol : theta : 8
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(6, 0.4)],
+    ),
+    # second-order Markov on a curved 1-gradient lattice
+    "cyl1d_markov2": dict(
+        text="""
+lat : L : gradients : 1
+lat : L : n_layers : 40
+lat : L : geometry : cylindrical
+lat : L : offset_first_layer : 3
+lat : L : lattice_type : simple_cubic
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.8
+mol : pol : composition : (A)25
+mol : pol : freedom : restricted
+mol : pol : theta : 40
+mol : pol : Markov : 2
+mol : pol : k_stiff : 2
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(7, 0.3)],
+    ),
+    # 2-gradient cylindrical polymer brush (small version of configuration 2)
+    "brush2d_cyl": dict(
+        text="""
+lat : cyl : gradients : 2
+lat : cyl : geometry : cylindrical
+lat : cyl : n_layers_x : 14
+lat : cyl : n_layers_y : 18
+lat : cyl : lattice_type : simple_cubic
+mon : X : freedom : pinned
+mon : X : pinned_range : firstlayer_x
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.5
+mol : brush : composition : (X)1(A)29
+mol : brush : freedom : restricted
+mol : brush : n : 2.5
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(8, 0.05), (9, 0.5)],
+        converged=[dict(method="DIIS", tol=1e-10, seed=10, amp=0.05)],
+    ),
+    # 2-gradient planar, frozen particle, periodic y
+    "planar2d_particle": dict(
+        text="""
+lat : L : gradients : 2
+lat : L : n_layers_x : 12
+lat : L : n_layers_y : 16
+lat : L : lattice_type : simple_cubic
+lat : L : lowerbound_y : periodic
+lat : L : upperbound_y : periodic
+mon : S : freedom : frozen
+mon : S : frozen_range : 5,6;7,9
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -1.0
+mon : A : chi_W : 0.5
+mol : pol : composition : (A)20
+mol : pol : freedom : free
+mol : pol : phibulk : 0.02
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(11, 0.3)],
+    ),
+    # 3-gradient periodic diblock box (small version of configuration 3)
+    "diblock3d_periodic": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 8
+lat : box : n_layers_y : 9
+lat : box : n_layers_z : 10
+lat : box : lattice_type : simple_cubic
+lat : box : lowerbound_x : periodic
+lat : box : upperbound_x : periodic
+lat : box : lowerbound_y : periodic
+lat : box : upperbound_y : periodic
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : A : freedom : free
+mon : B : freedom : free
+mon : W : freedom : free
+mon : B : chi_W : 1.6
+mon : A : chi_B : 1.6
+mol : surf : composition : (A)10(B)10
+mol : surf : freedom : restricted
+mol : surf : theta : 36
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(12, 0.1), (13, 0.6)],
+        converged=[dict(method="DIIS", tol=1e-9, seed=14, amp=0.1)],
+    ),
+    # 3-gradient mirror box with a frozen cube
+    "particle3d_mirror": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 9
+lat : box : n_layers_y : 8
+lat : box : n_layers_z : 7
+lat : box : lattice_type : simple_cubic
+mon : S : freedom : frozen
+mon : S : frozen_range : 1,1,1;3,3,3
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -0.8
+mon : A : chi_W : 0.4
+mol : pol : composition : (A)15
+mol : pol : freedom : free
+mol : pol : phibulk : 0.03
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(15, 0.3)],
+    ),
+}
