@@ -1,0 +1,156 @@
+/*
+ * namics_scf.h — C ABI of the MI355X lattice-SCF hot path.
+ *
+ * One `scf_system` is one Scheutjens-Fleer SCF problem resident on the GPU:
+ * its fields, Boltzmann weights, end-point distributions, densities and
+ * DIIS history live in one device arena owned by the library.  Callers own
+ * every host buffer they pass; no pointer ownership crosses this boundary.
+ * All functions return an int status (SCF_OK = 0, negative = error,
+ * SCF_NOT_CONVERGED = 1); no C++ exception crosses the ABI.
+ *
+ * The reference (leermakers/Namics) has no plugin API; its seams are C++
+ * virtuals.  Each entry point names the reference interface it replaces, so a
+ * maintainer can route the reference's own classes through this library
+ * (see INTEGRATION.md).
+ */
+#ifndef NAMICS_SCF_H
+#define NAMICS_SCF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct scf_system scf_system;
+
+/* status codes; the negative ones 1..4 mirror the reference's `throw int`
+ * (src/system.cpp:2398-2402, src/sfnewton.cpp:808-809, src/tools.cpp:1225-1228,
+ * src/system.cpp:2521-2525) */
+enum {
+  SCF_OK = 0,
+  SCF_NOT_CONVERGED = 1,     /* iteration limit reached (src/sfnewton.cpp:614-617) */
+  SCF_E_GN = -1,             /* partition function GN <= 0 */
+  SCF_E_NAN = -2,            /* NaN in the DIIS Gram matrix */
+  SCF_E_SVD = -3,            /* NaN in the SVD */
+  SCF_E_PHIB = -4,           /* solvent bulk volume fraction < 0 */
+  SCF_E_ARG = -10,           /* bad argument */
+  SCF_E_PARSE = -11,         /* input text rejected */
+  SCF_E_UNSUPPORTED = -12,   /* valid input outside this build's scope */
+  SCF_E_OOM = -13,           /* device allocation failed */
+  SCF_E_HIP = -14            /* HIP runtime error */
+};
+
+/* execution strategy for one system */
+enum {
+  SCF_MODE_AUTO = 0,   /* fused for 1-2 gradients, grid for 3 gradients */
+  SCF_MODE_FUSED = 1,  /* one workgroup runs the whole residual / solve */
+  SCF_MODE_GRID = 2    /* one grid-wide launch per phase and propagation step */
+};
+
+#define SCF_MAX_MOL 8
+#define SCF_MAX_SEG 8
+
+typedef struct {
+  int iterations;
+  int status;      /* SCF_OK, SCF_NOT_CONVERGED or a negative error */
+  double residual; /* ||g||_2 (or max|g| with stop_criterion max_of_element_of_|g|) */
+  double kernel_ms; /* GPU time of the launch(es) that solved this system (HIP events) */
+} scf_solve_info;
+
+typedef struct {
+  double free_energy;      /* System::GetFreeEnergy (src/system.cpp:2809-3029) */
+  double grand_potential;  /* System::GetGrandPotential (src/system.cpp:3038-3244) */
+  double residual;
+  int n_mol;
+  int n_seg;
+  double GN[SCF_MAX_MOL];       /* single-chain partition functions */
+  double n[SCF_MAX_MOL];        /* number of chains */
+  double theta[SCF_MAX_MOL];    /* weighted sum of the molecule density */
+  double phibulk[SCF_MAX_MOL];
+  double mu[SCF_MAX_MOL];       /* System::CreateMu (src/system.cpp:3246-3428) */
+  double seg_phibulk[SCF_MAX_SEG];
+} scf_results;
+
+/* Parse `.in` text (the reference's input format) at start block `start`
+ * (1-based) and allocate the system on the current HIP device.
+ * Replaces: Input + Lattice/Segment/Molecule/System/Solve_scf::CheckInput and
+ * AllocateMemory (src/namics.cpp:153-417, src/lattice.cpp:267-601). */
+int scf_system_create(const char *input_text, int start, scf_system **out, char *err, size_t errlen);
+
+void scf_system_destroy(scf_system *sys);
+
+/* JSON description: names of segments, molecules and field blocks, lattice
+ * sizes (M, MX, MY, MZ), nvar, solver settings.  Returns bytes needed. */
+int scf_system_describe(const scf_system *sys, char *buf, size_t len);
+
+/* Number of field values (n_field_blocks * M), the length of x and g. */
+int64_t scf_system_nvar(const scf_system *sys);
+
+/* Lattice sites M including ghost layers. */
+int64_t scf_system_sites(const scf_system *sys);
+
+/* Device bytes held by this system (its HBM footprint). */
+size_t scf_system_device_bytes(const scf_system *sys);
+
+/* Force an execution strategy (SCF_MODE_*). */
+int scf_system_set_mode(scf_system *sys, int mode);
+
+/* Override solver settings: tolerance, deltamax, DIIS history m, iteration
+ * limit (values <= 0 keep the input's setting). */
+int scf_system_set_solver(scf_system *sys, double tolerance, double deltamax, int m, int iterationlimit);
+
+/* One residual evaluation g = F(x) at the host fields x (nvar values).
+ * Replaces: Solve_scf::residuals -> System::Classical_residual
+ * (src/solve_scf.cpp:901-906, src/system.cpp:2087-2189), including
+ * Molecule::ComputePhi (src/mol_linear.cpp:131-149) and the thermodynamic
+ * summary of System::CheckResults (src/system.cpp:2719-2808). */
+int scf_residual(scf_system *sys, const double *x, double *g, double *residual);
+
+/* Solve the SCF equations by DIIS from the host fields x (updated in place
+ * with the final fields).  Replaces: Solve_scf::Solve with method DIIS ->
+ * SFNewton::iterate_DIIS (src/solve_scf.cpp:558-651, src/sfnewton.cpp:1037-1130). */
+int scf_solve(scf_system *sys, double *x, scf_solve_info *info);
+
+/* Solve n independent systems in one launch (one workgroup per system);
+ * x[i] are host field buffers updated in place, info[i] per system.
+ * Systems in SCF_MODE_GRID are solved one after another. */
+int scf_solve_batch(scf_system *const *systems, int n, double *const *x, scf_solve_info *info);
+
+/* Device-resident fields: copy host fields in / out of the system's arena. */
+int scf_upload_fields(scf_system *sys, const double *x);
+int scf_download_fields(const scf_system *sys, double *x);
+
+/* Start `dst` from the fields of `src` (same lattice), block by segment
+ * name, zero where `src` has no block of that name, device to device.
+ * Replaces the chaining of `start` blocks with initial_guess
+ * previous_result (Solve_scf::Guess, src/solve_scf.cpp:489-524). */
+int scf_carry_fields(scf_system *dst, const scf_system *src);
+
+/* Solve n systems from their device-resident fields, in one launch for the
+ * fused ones; no host copies of fields.  info[i] per system. */
+int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info);
+
+/* Scalars of the last residual / solve. */
+int scf_get_results(const scf_system *sys, scf_results *out);
+
+/* Densities of the last residual / solve: segment densities (n_seg x M) and
+ * molecule total densities (n_mol x M), x-major with ghost layers.
+ * Replaces: the `pro` output of mon/mol phi (src/output.cpp:385-708). */
+int scf_get_profiles(const scf_system *sys, double *seg_phi, double *mol_phi);
+
+/* Unit-test entry: N-1 Markov-1 steps from G(1) = G1 (host arrays of M),
+ * out = N x M end-point distributions.  Replaces Lattice::propagate
+ * (src/lattice.h:133; src/LG1Planar.cpp:138-159, src/LGrad2.cpp:1053-1107,
+ * src/LGrad3.cpp:673-743). */
+int scf_propagate_chain(scf_system *sys, const double *G1, int N, double *out);
+
+/* Last error message of this thread (static storage). */
+const char *scf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NAMICS_SCF_H */

@@ -1,0 +1,618 @@
+// Host side of the SCF library: builds the device description of a system
+// from its SystemSpec, owns its HBM arena, and implements the C ABI of
+// include/namics_scf.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/namics_scf.h"
+#include "scf_device.h"
+#include "scf_kernels.h"
+#include "scf_spec.h"
+
+using namespace nscf;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *where) {
+  return fail(SCF_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                 \
+  do {                                                \
+    hipError_t _e = (expr);                           \
+    if (_e != hipSuccess) return hip_fail(_e, #expr); \
+  } while (0)
+
+// Bump allocator over one device allocation.
+struct Arena {
+  size_t used = 0;
+  size_t take(size_t bytes) {
+    size_t off = used;
+    used += (bytes + 255) & ~size_t(255);
+    return off;
+  }
+};
+
+size_t g_lds_limit = 0;  // bytes of dynamic LDS one workgroup may use
+
+size_t lds_limit() {
+  if (g_lds_limit == 0) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
+      g_lds_limit = p.sharedMemPerBlock;
+      // leave room for the kernel's static shared memory
+      if (g_lds_limit > 1024) g_lds_limit -= 1024;
+    } else {
+      g_lds_limit = 64 * 1024 - 1024;
+    }
+  }
+  return g_lds_limit;
+}
+
+}  // namespace
+
+struct scf_system {
+  SystemSpec spec;
+  DevSys h;              // host copy holding device pointers
+  DevSys *d = nullptr;   // device copy
+  DevSys **d_list = nullptr;
+  char *arena = nullptr;
+  size_t arena_bytes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int mode = SCF_MODE_AUTO;
+  int nb = 0;            // workgroups of the grid path
+  int m_alloc = 0;       // DIIS history the arena was sized for
+  size_t lds_bytes = 0;  // dynamic LDS the fused path asks for (0: rows in HBM)
+  bool have_results = false;
+
+  int effective_mode() const {
+    if (mode != SCF_MODE_AUTO) return mode;
+    return spec.lat.gradients == 3 ? SCF_MODE_GRID : SCF_MODE_FUSED;
+  }
+};
+
+namespace {
+
+int validate(const SystemSpec &s, std::string &why) {
+  const LatticeSpec &l = s.lat;
+  if ((int)s.seg.size() > kMaxSeg) return why = "more than 8 segment types", SCF_E_UNSUPPORTED;
+  if ((int)s.mol.size() > kMaxMol) return why = "more than 8 molecule types", SCF_E_UNSUPPORTED;
+  int nb = 0, nmm = 0;
+  for (auto &m : s.mol) {
+    nb += (int)m.blocks.size();
+    nmm += (int)m.molmon.size();
+  }
+  if (nb > kMaxBlocks) return why = "more than 16 blocks over all molecules", SCF_E_UNSUPPORTED;
+  if (nmm > kMaxMM) return why = "more than 16 molecule-segment pairs", SCF_E_UNSUPPORTED;
+  if (s.solver.m > kMaxDIIS || s.solver.m < 1) return why = "DIIS history m must be 1..16", SCF_E_UNSUPPORTED;
+  if (l.gradients >= 2)
+    for (int d = 0; d < 6; d++)
+      if (l.bc[d] == BC_SURFACE) return why = "surface boundaries are supported with 1 gradient only", SCF_E_UNSUPPORTED;
+  if (l.gradients == 2 && l.geometry == GEOM_PLANAR && l.stencil_full && l.lattice_type != LAT_SIMPLE_CUBIC)
+    return why = "2-gradient planar hexagonal stencil is not supported", SCF_E_UNSUPPORTED;
+  if (l.gradients == 2 && l.geometry == GEOM_PLANAR && !l.stencil_full && l.lattice_type != LAT_SIMPLE_CUBIC)
+    return why = "2-gradient planar hexagonal stencil is not supported", SCF_E_UNSUPPORTED;
+  if (l.gradients == 3 && !(l.stencil_full && l.lattice_type == LAT_SIMPLE_CUBIC))
+    return why = "3 gradients: only the simple-cubic 6-point stencil is supported", SCF_E_UNSUPPORTED;
+  for (auto &m : s.mol)
+    if (m.markov == 2 && l.gradients != 1) return why = "Markov 2 needs 1 gradient", SCF_E_UNSUPPORTED;
+  return SCF_OK;
+}
+
+double fraction(const MoleculeSpec &m, int seg) {
+  int n = 0;
+  for (auto &b : m.blocks)
+    if (b.seg == seg) n += b.n;
+  return 1.0 * n / m.chainlength;
+}
+
+int build(scf_system *S) {
+  const SystemSpec &s = S->spec;
+  const LatticeSpec &l = s.lat;
+  DevSys &h = S->h;
+  std::memset(&h, 0, sizeof(DevSys));
+  const int M = l.M;
+  const int nseg = (int)s.seg.size(), nmol = (int)s.mol.size(), nit = (int)s.itmon.size();
+  const size_t nvar = (size_t)nit * M;
+  h.gradients = l.gradients;
+  h.geometry = l.geometry;
+  h.lattice_type = l.lattice_type;
+  h.stencil_full = l.stencil_full ? 1 : 0;
+  h.MX = l.MX;
+  h.MY = l.MY;
+  h.MZ = l.MZ;
+  h.M = M;
+  h.JX = l.JX;
+  h.JY = l.JY;
+  h.bx1 = l.BX1;
+  h.bxm = l.BXM;
+  h.by1 = l.BY1;
+  h.bym = l.BYM;
+  h.bz1 = l.BZ1;
+  h.bzm = l.BZM;
+  h.surface_lo = l.bc[0] == BC_SURFACE;
+  h.surface_hi = l.bc[3] == BC_SURFACE;
+  h.lambda = l.lambda;
+  h.nseg = nseg;
+  h.nit = nit;
+  for (int k = 0; k < nseg; k++) {
+    h.seg_block[k] = s.seg[k].field_block;
+    h.seg_frozen[k] = s.seg[k].freedom == SEG_FROZEN;
+    h.seg_pinned[k] = s.seg[k].freedom == SEG_PINNED;
+    for (int j = 0; j < nseg; j++) h.chi[k * kMaxSeg + j] = s.seg[k].chi[j];
+  }
+  for (int i = 0; i < nit; i++) h.itmon[i] = s.itmon[i];
+  h.nmol = nmol;
+  h.solvent = s.solvent;
+  int nb = 0, nmm = 0, maxNR = 0, maxR = 0;
+  for (int i = 0; i < nmol; i++) {
+    const MoleculeSpec &m = s.mol[i];
+    h.mol_N[i] = m.chainlength;
+    h.mol_freedom[i] = m.freedom;
+    h.mol_pinned[i] = m.pinned;
+    h.mol_markov[i] = m.markov;
+    h.mol_size[i] = m.size;
+    h.mol_block_begin[i] = nb;
+    h.mol_mm_begin[i] = nmm;
+    for (size_t b = 0; b < m.blocks.size(); b++) {
+      h.block_seg[nb] = m.blocks[b].seg;
+      h.block_n[nb] = m.blocks[b].n;
+      h.block_mm[nb] = nmm + m.block_molmon[b];
+      nb++;
+    }
+    for (int sg : m.molmon) {
+      h.mm_seg[nmm++] = sg;
+      h.sysmon[sg] = 1;
+    }
+    for (int k = 0; k < 3 && k < (int)m.P.size(); k++) h.mol_P[i][k] = m.P[k];
+    h.mol_phibulk_in[i] = m.phibulk;
+    h.mol_n_in[i] = m.n;
+    double cf = 0;
+    for (int a : m.molmon)
+      for (int b : m.molmon) cf -= fraction(m, a) * fraction(m, b) * (s.seg[a].chi[b] / 2);
+    h.mol_constF[i] = cf;
+    for (int k = 0; k < nseg; k++) h.frac[i][k] = fraction(m, k);
+    maxNR = std::max(maxNR, m.chainlength * m.size);
+    maxR = std::max(maxR, m.size);
+  }
+  h.mol_block_begin[nmol] = nb;
+  h.mol_mm_begin[nmol] = nmm;
+  h.nblocks = nb;
+  h.nmm = nmm;
+  h.m = s.solver.m;
+  h.itlim = s.solver.iterationlimit;
+  h.max_g = s.solver.max_g;
+  h.deltamax = s.solver.deltamax;
+  h.tol = s.solver.tolerance;
+
+  // ---- arena layout
+  Arena a;
+  const size_t D = sizeof(double);
+  const bool curved = l.gradients < 3;
+  const bool m2curved = !l.l1.empty();
+  size_t o_L = curved ? a.take(4 * M * D) : 0;
+  size_t o_l1 = m2curved ? a.take(4 * M * D) : 0;
+  size_t o_mask = a.take((size_t)nseg * M * D);
+  size_t o_ksam = a.take(M * D);
+  size_t o_x = a.take(nvar * D);
+  size_t o_g = a.take(nvar * D);
+  size_t o_G1 = a.take((size_t)nseg * M * D);
+  size_t o_Gf = a.take((size_t)maxNR * M * D);
+  size_t o_Gb = a.take((size_t)2 * maxR * M * D);
+  size_t o_rho = a.take((size_t)nmm * M * D);
+  size_t o_phi = a.take((size_t)nseg * M * D);
+  size_t o_side = a.take((size_t)nseg * M * D);
+  size_t o_pt = a.take(M * D);
+  size_t o_alpha = a.take(M * D);
+  size_t o_mpt = a.take((size_t)nmol * M * D);
+  S->m_alloc = h.m;
+  size_t o_xR = a.take((size_t)h.m * nvar * D);
+  size_t o_xx0 = a.take((size_t)h.m * nvar * D);
+  size_t o_x0 = a.take(nvar * D);
+  size_t o_sc = a.take(sizeof(DevScalars));
+  // grid path: partial sums (thermo needs 2+nmol columns, DIIS kMaxDIIS)
+  const int W = l.gradients == 1 ? M : l.MX * l.MY * (l.gradients == 3 ? l.MZ : 1);
+  S->nb = std::max(1, std::min((W + 255) / 256, 2048));
+  size_t o_part = a.take((size_t)S->nb * std::max(kMaxDIIS, 2 + kMaxMol) * D);
+  size_t o_dev = a.take(sizeof(DevSys));
+  size_t o_list = a.take(sizeof(DevSys *));
+  S->arena_bytes = a.used;
+  hipError_t e = hipMalloc((void **)&S->arena, a.used);
+  if (e != hipSuccess) return fail(SCF_E_OOM, "device allocation of " + std::to_string(a.used) + " bytes failed");
+  char *base = S->arena;
+  HIP_TRY(hipMemset(base, 0, a.used));
+  auto P = [&](size_t off) { return (double *)(base + off); };
+  if (curved) {
+    h.L = P(o_L);
+    h.lam_m = P(o_L) + M;
+    h.lam_0 = P(o_L) + 2 * M;
+    h.lam_p = P(o_L) + 3 * M;
+    HIP_TRY(hipMemcpy((void *)h.L, l.L.data(), M * D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((void *)h.lam_m, l.lambda_1.data(), M * D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((void *)h.lam_0, l.lambda0.data(), M * D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((void *)h.lam_p, l.lambda1.data(), M * D, hipMemcpyHostToDevice));
+  }
+  if (m2curved) {
+    h.l1 = P(o_l1);
+    h.l11 = P(o_l1) + M;
+    h.l_1 = P(o_l1) + 2 * M;
+    h.l_11 = P(o_l1) + 3 * M;
+    HIP_TRY(hipMemcpy((void *)h.l1, l.l1.data(), M * D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((void *)h.l11, l.l11.data(), M * D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((void *)h.l_1, l.l_1.data(), M * D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((void *)h.l_11, l.l_11.data(), M * D, hipMemcpyHostToDevice));
+  }
+  {
+    std::vector<double> mask((size_t)nseg * M, 0.0), ks(M, 0.0);
+    for (int k = 0; k < nseg; k++)
+      for (int i = 0; i < M; i++) mask[(size_t)k * M + i] = s.seg[k].mask[i];
+    for (int i = 0; i < M; i++) ks[i] = s.ksam[i];
+    h.seg_mask = P(o_mask);
+    h.ksam = P(o_ksam);
+    HIP_TRY(hipMemcpy((void *)h.seg_mask, mask.data(), mask.size() * D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((void *)h.ksam, ks.data(), M * D, hipMemcpyHostToDevice));
+  }
+  h.x = P(o_x);
+  h.g = P(o_g);
+  h.G1 = P(o_G1);
+  h.Gf = P(o_Gf);
+  h.Gb = P(o_Gb);
+  h.rho = P(o_rho);
+  h.phi = P(o_phi);
+  h.phi_side = P(o_side);
+  h.phitot = P(o_pt);
+  h.alpha = P(o_alpha);
+  h.mol_phitot = P(o_mpt);
+  h.xR = P(o_xR);
+  h.xx0 = P(o_xx0);
+  h.x0 = P(o_x0);
+  h.sc = (DevScalars *)(base + o_sc);
+  h.partial = P(o_part);
+  // execution choice for the fused path
+  const size_t rows = (size_t)2 * maxR * M * D;
+  h.use_lds = rows <= lds_limit() ? 1 : 0;
+  S->lds_bytes = h.use_lds ? rows : 0;
+  int work = l.gradients == 1 ? M : W;
+  h.threads = work <= 64 ? 64 : (work <= 128 ? 128 : (work <= 256 ? 256 : (work <= 512 ? 512 : 1024)));
+  S->d = (DevSys *)(base + o_dev);
+  S->d_list = (DevSys **)(base + o_list);
+  HIP_TRY(hipMemcpy(S->d, &h, sizeof(DevSys), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(S->d_list, &S->d, sizeof(DevSys *), hipMemcpyHostToDevice));
+  return SCF_OK;
+}
+
+int upload_solver(scf_system *S) {
+  HIP_TRY(hipMemcpy(S->d, &S->h, sizeof(DevSys), hipMemcpyHostToDevice));
+  return SCF_OK;
+}
+
+int read_scalars(const scf_system *S, DevScalars &sc) {
+  HIP_TRY(hipMemcpy(&sc, S->h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost));
+  return SCF_OK;
+}
+
+// host-driven DIIS over the grid path (SFNewton::iterate_DIIS src/sfnewton.cpp:1037-1130)
+int grid_solve(scf_system *S, scf_solve_info *info) {
+  const DevSys &h = S->h;
+  hipStream_t st = S->stream;
+  DevScalars sc;
+  auto residual = [&]() -> int {
+    HIP_TRY(grid_copy_x0(h, S->d, S->nb, st));
+    HIP_TRY(grid_residual(h, S->d, h.partial, S->nb, st));
+    return SCF_OK;
+  };
+  int rc = residual();
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(&sc, h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int it = 0;
+  double res = sc.residual;
+  if (sc.status == DS_OK) {
+    HIP_TRY(grid_axpy_first(h, S->d, S->nb, st));
+    int k_diis = 0;
+    while (res > h.tol && it < h.itlim) {
+      it++;
+      rc = residual();
+      if (rc) return rc;
+      const int k = it % h.m;
+      k_diis++;
+      HIP_TRY(grid_diis_step(h, S->d, h.partial, S->nb, k, k_diis, st));
+      HIP_TRY(grid_diis_combine(h, S->d, S->nb, k, k_diis, st));
+      HIP_TRY(hipMemcpyAsync(&sc, h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      if (sc.status != DS_OK) break;
+      res = sc.residual;
+    }
+  }
+  if (sc.status == DS_OK) {
+    HIP_TRY(grid_thermo(h, S->d, h.partial, S->nb, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  info->iterations = it;
+  info->residual = res;
+  info->status = sc.status != DS_OK ? sc.status : (res <= h.tol ? SCF_OK : SCF_NOT_CONVERGED);
+  return SCF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *scf_last_error(void) { return g_last_error.c_str(); }
+
+int scf_system_create(const char *input_text, int start, scf_system **out, char *err, size_t errlen) {
+  if (!input_text || !out) return fail(SCF_E_ARG, "null argument");
+  *out = nullptr;
+  auto report = [&](int code, const std::string &msg) {
+    if (err && errlen > 0) snprintf(err, errlen, "%s", msg.c_str());
+    return fail(code, msg);
+  };
+  scf_system *S = new scf_system();
+  std::string why;
+  if (!parse_input(input_text, start, S->spec, why)) {
+    delete S;
+    return report(SCF_E_PARSE, why);
+  }
+  int v = validate(S->spec, why);
+  if (v != SCF_OK) {
+    delete S;
+    return report(v, why);
+  }
+  if (hipStreamCreate(&S->stream) != hipSuccess || hipEventCreate(&S->ev0) != hipSuccess ||
+      hipEventCreate(&S->ev1) != hipSuccess) {
+    std::string msg = "hipStreamCreate / hipEventCreate failed";
+    scf_system_destroy(S);
+    return report(SCF_E_HIP, msg);
+  }
+  int rc = build(S);
+  if (rc != SCF_OK) {
+    std::string msg = g_last_error;
+    scf_system_destroy(S);
+    return report(rc, msg);
+  }
+  *out = S;
+  return SCF_OK;
+}
+
+void scf_system_destroy(scf_system *S) {
+  if (!S) return;
+  if (S->stream) hipStreamSynchronize(S->stream);
+  if (S->arena) hipFree(S->arena);
+  if (S->ev0) hipEventDestroy(S->ev0);
+  if (S->ev1) hipEventDestroy(S->ev1);
+  if (S->stream) hipStreamDestroy(S->stream);
+  delete S;
+}
+
+int scf_system_describe(const scf_system *S, char *buf, size_t len) {
+  if (!S) return fail(SCF_E_ARG, "null system");
+  std::string d = describe_json(S->spec);
+  if (buf && len > 0) snprintf(buf, len, "%s", d.c_str());
+  return (int)d.size() + 1;
+}
+
+int64_t scf_system_nvar(const scf_system *S) { return S ? (int64_t)S->spec.nvar() : -1; }
+int64_t scf_system_sites(const scf_system *S) { return S ? (int64_t)S->spec.lat.M : -1; }
+size_t scf_system_device_bytes(const scf_system *S) { return S ? S->arena_bytes : 0; }
+
+int scf_system_set_mode(scf_system *S, int mode) {
+  if (!S || mode < SCF_MODE_AUTO || mode > SCF_MODE_GRID) return fail(SCF_E_ARG, "bad mode");
+  S->mode = mode;
+  return SCF_OK;
+}
+
+int scf_system_set_solver(scf_system *S, double tolerance, double deltamax, int m, int iterationlimit) {
+  if (!S) return fail(SCF_E_ARG, "null system");
+  if (m > S->m_alloc) return fail(SCF_E_UNSUPPORTED, "DIIS history m above the input's setting");
+  if (tolerance > 0) S->h.tol = S->spec.solver.tolerance = tolerance;
+  if (deltamax > 0) S->h.deltamax = S->spec.solver.deltamax = deltamax;
+  if (m > 0) S->h.m = S->spec.solver.m = m;
+  if (iterationlimit >= 0) S->h.itlim = S->spec.solver.iterationlimit = iterationlimit;
+  return upload_solver(S);
+}
+
+int scf_residual(scf_system *S, const double *x, double *g, double *residual) {
+  if (!S || !x) return fail(SCF_E_ARG, "null argument");
+  const DevSys &h = S->h;
+  const size_t nvar = (size_t)h.nit * h.M;
+  HIP_TRY(hipMemcpyAsync(h.x, x, nvar * sizeof(double), hipMemcpyHostToDevice, S->stream));
+  if (S->effective_mode() == SCF_MODE_FUSED) {
+    HIP_TRY(launch_fused(S->d_list, 1, FUSED_RESIDUAL, h.threads, S->lds_bytes, S->stream));
+  } else {
+    HIP_TRY(grid_residual(h, S->d, h.partial, S->nb, S->stream));
+    HIP_TRY(grid_thermo(h, S->d, h.partial, S->nb, S->stream));
+  }
+  DevScalars sc;
+  HIP_TRY(hipMemcpyAsync(&sc, h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, S->stream));
+  if (g) HIP_TRY(hipMemcpyAsync(g, h.g, nvar * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  HIP_TRY(hipStreamSynchronize(S->stream));
+  if (residual) *residual = sc.residual;
+  S->have_results = sc.status == DS_OK;
+  if (sc.status != DS_OK) return fail(sc.status, "residual evaluation failed with status " + std::to_string(sc.status));
+  return SCF_OK;
+}
+
+int scf_solve(scf_system *S, double *x, scf_solve_info *info) {
+  scf_solve_info tmp;
+  if (!info) info = &tmp;
+  return scf_solve_batch(&S, 1, &x, info);
+}
+
+int scf_upload_fields(scf_system *S, const double *x) {
+  if (!S || !x) return fail(SCF_E_ARG, "null argument");
+  const size_t nvar = (size_t)S->h.nit * S->h.M;
+  HIP_TRY(hipMemcpyAsync(S->h.x, x, nvar * sizeof(double), hipMemcpyHostToDevice, S->stream));
+  HIP_TRY(hipStreamSynchronize(S->stream));
+  return SCF_OK;
+}
+
+int scf_download_fields(const scf_system *S, double *x) {
+  if (!S || !x) return fail(SCF_E_ARG, "null argument");
+  const size_t nvar = (size_t)S->h.nit * S->h.M;
+  HIP_TRY(hipMemcpyAsync(x, S->h.x, nvar * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  HIP_TRY(hipStreamSynchronize(S->stream));
+  return SCF_OK;
+}
+
+int scf_carry_fields(scf_system *dst, const scf_system *src) {
+  if (!dst || !src) return fail(SCF_E_ARG, "null argument");
+  const int M = dst->h.M;
+  if (src->h.M != M) return fail(SCF_E_ARG, "systems differ in lattice size");
+  HIP_TRY(hipStreamSynchronize(src->stream));
+  for (size_t j = 0; j < dst->spec.itmon.size(); j++) {
+    const std::string &name = dst->spec.seg[dst->spec.itmon[j]].name;
+    int found = -1;
+    for (size_t i = 0; i < src->spec.itmon.size(); i++)
+      if (src->spec.seg[src->spec.itmon[i]].name == name) found = (int)i;
+    double *d = dst->h.x + j * (size_t)M;
+    if (found >= 0)
+      HIP_TRY(hipMemcpyAsync(d, src->h.x + found * (size_t)M, M * sizeof(double), hipMemcpyDeviceToDevice,
+                             dst->stream));
+    else
+      HIP_TRY(hipMemsetAsync(d, 0, M * sizeof(double), dst->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(dst->stream));
+  return SCF_OK;
+}
+
+int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info) {
+  if (!systems || n <= 0 || !info) return fail(SCF_E_ARG, "null argument");
+  std::vector<int> fused;
+  size_t lds = 0;
+  int threads = 64;
+  for (int i = 0; i < n; i++) {
+    scf_system *S = systems[i];
+    if (!S) return fail(SCF_E_ARG, "null system");
+    HIP_TRY(hipStreamSynchronize(S->stream));
+    if (S->effective_mode() == SCF_MODE_FUSED) {
+      fused.push_back(i);
+      threads = std::max(threads, S->h.threads);
+      lds = std::max(lds, S->lds_bytes);
+    }
+  }
+  float fused_ms = 0.f;
+  // one launch for all fused systems
+  if (!fused.empty()) {
+    scf_system *S0 = systems[fused[0]];
+    hipStream_t st = S0->stream;
+    std::vector<DevSys *> list;
+    for (int i : fused) list.push_back(systems[i]->d);
+    DevSys **d_list = S0->d_list;
+    if (list.size() > 1) {
+      HIP_TRY(hipMalloc((void **)&d_list, list.size() * sizeof(DevSys *)));
+      HIP_TRY(hipMemcpy(d_list, list.data(), list.size() * sizeof(DevSys *), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipEventRecord(S0->ev0, st));
+    hipError_t le = launch_fused(d_list, (int)list.size(), FUSED_SOLVE, threads, lds, st);
+    hipError_t re = hipEventRecord(S0->ev1, st);
+    hipError_t se = hipStreamSynchronize(st);
+    if (list.size() > 1) hipFree(d_list);
+    if (le != hipSuccess) return hip_fail(le, "launch_fused");
+    if (re != hipSuccess) return hip_fail(re, "hipEventRecord");
+    if (se != hipSuccess) return hip_fail(se, "hipStreamSynchronize");
+    HIP_TRY(hipEventElapsedTime(&fused_ms, S0->ev0, S0->ev1));
+  }
+  for (int i = 0; i < n; i++) {
+    scf_system *S = systems[i];
+    if (S->effective_mode() == SCF_MODE_GRID) {
+      HIP_TRY(hipEventRecord(S->ev0, S->stream));
+      int rc = grid_solve(S, &info[i]);
+      if (rc != SCF_OK) return rc;
+      HIP_TRY(hipEventRecord(S->ev1, S->stream));
+      HIP_TRY(hipStreamSynchronize(S->stream));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, S->ev0, S->ev1));
+      info[i].kernel_ms = ms;
+    } else {
+      DevScalars sc;
+      int rc = read_scalars(S, sc);
+      if (rc != SCF_OK) return rc;
+      info[i].iterations = sc.iterations;
+      info[i].residual = sc.residual;
+      info[i].status = sc.status != DS_OK ? sc.status : (sc.converged ? SCF_OK : SCF_NOT_CONVERGED);
+      info[i].kernel_ms = fused_ms;
+    }
+    S->have_results = info[i].status >= 0;
+  }
+  return SCF_OK;
+}
+
+int scf_solve_batch(scf_system *const *systems, int n, double *const *x, scf_solve_info *info) {
+  if (!systems || n <= 0 || !x || !info) return fail(SCF_E_ARG, "null argument");
+  for (int i = 0; i < n; i++) {
+    if (!systems[i] || !x[i]) return fail(SCF_E_ARG, "null system or field buffer");
+    int rc = scf_upload_fields(systems[i], x[i]);
+    if (rc != SCF_OK) return rc;
+  }
+  int rc = scf_solve_resident(systems, n, info);
+  if (rc != SCF_OK) return rc;
+  for (int i = 0; i < n; i++) {
+    rc = scf_download_fields(systems[i], x[i]);
+    if (rc != SCF_OK) return rc;
+  }
+  return SCF_OK;
+}
+
+int scf_get_results(const scf_system *S, scf_results *out) {
+  if (!S || !out) return fail(SCF_E_ARG, "null argument");
+  DevScalars sc;
+  int rc = read_scalars(S, sc);
+  if (rc != SCF_OK) return rc;
+  std::memset(out, 0, sizeof(*out));
+  out->free_energy = sc.F;
+  out->grand_potential = sc.GP;
+  out->residual = sc.residual;
+  out->n_mol = S->h.nmol;
+  out->n_seg = S->h.nseg;
+  for (int i = 0; i < S->h.nmol; i++) {
+    out->GN[i] = sc.GN[i];
+    out->n[i] = sc.n[i];
+    out->theta[i] = sc.theta[i];
+    out->phibulk[i] = sc.phibulk[i];
+    out->mu[i] = sc.Mu[i];
+  }
+  for (int k = 0; k < S->h.nseg; k++) out->seg_phibulk[k] = sc.seg_phibulk[k];
+  return SCF_OK;
+}
+
+int scf_get_profiles(const scf_system *S, double *seg_phi, double *mol_phi) {
+  if (!S) return fail(SCF_E_ARG, "null system");
+  const size_t M = S->h.M;
+  if (seg_phi) HIP_TRY(hipMemcpy(seg_phi, S->h.phi, S->h.nseg * M * sizeof(double), hipMemcpyDeviceToHost));
+  if (mol_phi) HIP_TRY(hipMemcpy(mol_phi, S->h.mol_phitot, S->h.nmol * M * sizeof(double), hipMemcpyDeviceToHost));
+  return SCF_OK;
+}
+
+int scf_propagate_chain(scf_system *S, const double *G1, int N, double *out) {
+  if (!S || !G1 || !out || N < 1) return fail(SCF_E_ARG, "bad argument");
+  const size_t M = S->h.M;
+  double *dG1 = nullptr, *dout = nullptr;
+  HIP_TRY(hipMalloc((void **)&dG1, M * sizeof(double)));
+  HIP_TRY(hipMalloc((void **)&dout, (size_t)N * M * sizeof(double)));
+  HIP_TRY(hipMemset(dout, 0, (size_t)N * M * sizeof(double)));
+  HIP_TRY(hipMemcpy(dG1, G1, M * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(launch_propagate_chain(S->d, dG1, N, dout, S->stream));
+  HIP_TRY(hipStreamSynchronize(S->stream));
+  HIP_TRY(hipMemcpy(out, dout, (size_t)N * M * sizeof(double), hipMemcpyDeviceToHost));
+  hipFree(dG1);
+  hipFree(dout);
+  return SCF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,574 @@
+// Device-side description of one SCF system and the per-site lattice
+// operators of the hot path (propagator step, site average, densities,
+// residual, free-energy densities).
+//
+// Layout in HBM: every lattice array is x-major with one ghost layer per side
+// (M = (MX+2)(MY+2)(MZ+2) sites, last index contiguous), the reference's own
+// indexing (src/LGrad1.cpp:182, src/LGrad2.cpp:109, src/LGrad3.cpp:19), so
+// fields and profiles cross the boundary without reshuffling.  Ghost layers
+// are never filled in a separate pass: a neighbour read that falls on a ghost
+// is remapped to the site the reference's set_bounds would copy there
+// (mirror: first/last layer, periodic: opposite layer; corners map in every
+// direction).  With a `surface` boundary (1 gradient only) the ghost keeps
+// its own value and is recomputed every step with the truncated stencil, as
+// the reference does (src/LG1Planar.cpp:138-159).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "scf_linalg.h"
+
+namespace nscf {
+
+constexpr int kMaxSeg = 8;
+constexpr int kMaxMol = 8;
+constexpr int kMaxBlocks = 16;
+constexpr int kMaxMM = 16;
+
+enum DevStatus : int {
+  DS_OK = 0,
+  DS_GN = -1,        // GN <= 0 (reference throws -1, src/system.cpp:2398-2402)
+  DS_NAN = -2,       // NaN in the DIIS matrix (-2, src/sfnewton.cpp:808-809)
+  DS_SVD = -3,       // NaN in the SVD (-3, src/tools.cpp:1225-1228)
+  DS_PHIB = -4,      // negative solvent phibulk (-4, src/system.cpp:2521-2525)
+};
+
+// Scalars of one system, kept in device memory next to its fields.
+struct DevScalars {
+  double GN[kMaxMol], norm[kMaxMol], n[kMaxMol], phibulk[kMaxMol], theta[kMaxMol], Mu[kMaxMol];
+  double seg_phibulk[kMaxSeg];
+  double F, GP, residual, alpha_unused;
+  double A[kMaxDIIS * kMaxDIIS];  // DIIS Gram matrix, leading dimension m
+  double C[kMaxDIIS];
+  double red[kMaxDIIS + 4];       // reduction results handed between phases
+  int status, iterations, k_diis, converged;
+};
+
+struct DevSys {
+  // ---- lattice
+  int gradients, geometry, lattice_type, stencil_full;
+  int MX, MY, MZ, M, JX, JY;
+  int bx1, bxm, by1, bym, bz1, bzm;
+  int surface_lo, surface_hi;  // 1-gradient surface boundaries
+  double lambda;
+  const double *L, *lam_m, *lam_0, *lam_p;  // M each (1 and 2 gradients)
+  const double *l1, *l11, *l_1, *l_11;      // Markov-2 curved weights (1 gradient)
+  // ---- segments
+  int nseg, nit;
+  int seg_block[kMaxSeg];   // field block of x, -1 for frozen segments
+  int seg_frozen[kMaxSeg], seg_pinned[kMaxSeg];
+  int itmon[kMaxSeg];
+  int sysmon[kMaxSeg];      // 1 if the segment is used by a molecule
+  double chi[kMaxSeg * kMaxSeg];
+  const double *seg_mask;   // nseg * M (1.0 on frozen / pinned sites)
+  const double *ksam;       // M (0 on frozen sites)
+  // ---- molecules (linear chains: a sequence of blocks)
+  int nmol, solvent, nblocks, nmm;
+  int mol_N[kMaxMol], mol_freedom[kMaxMol], mol_pinned[kMaxMol], mol_markov[kMaxMol], mol_size[kMaxMol];
+  int mol_block_begin[kMaxMol + 1];
+  int mol_mm_begin[kMaxMol + 1];
+  int block_seg[kMaxBlocks], block_n[kMaxBlocks], block_mm[kMaxBlocks];
+  int mm_seg[kMaxMM];
+  double mol_P[kMaxMol][3];
+  double mol_phibulk_in[kMaxMol], mol_n_in[kMaxMol];
+  double frac[kMaxMol][kMaxSeg];
+  double mol_constF[kMaxMol];
+  // ---- work arrays (device)
+  double *x, *g;
+  double *G1;         // nseg * M
+  double *Gf;         // max over molecules of N * size * M
+  double *Gb;         // 2 * size * M
+  double *rho;        // nmm * M (molecule-segment densities)
+  double *phi;        // nseg * M
+  double *phi_side;   // nseg * M
+  double *phitot, *alpha;  // M
+  double *mol_phitot; // nmol * M
+  // ---- DIIS state
+  double *xR, *xx0, *x0;  // m * nvar, m * nvar, nvar
+  int m, itlim, max_g;
+  double deltamax, tol;
+  // ---- execution
+  int use_lds;            // propagator rows fit the LDS budget of the launch
+  int threads;            // preferred workgroup size for the fused kernel
+  // ---- scalars and reduction workspace
+  DevScalars *sc;
+  double *partial;        // per-workgroup partial sums (multi-workgroup path)
+};
+
+// ------------------------------------------------------------ index helpers
+__device__ __forceinline__ int map_x(const DevSys &S, int x) {
+  return x < 1 ? S.bx1 : (x > S.MX ? S.bxm : x);
+}
+__device__ __forceinline__ int map_y(const DevSys &S, int y) {
+  return y < 1 ? S.by1 : (y > S.MY ? S.bym : y);
+}
+__device__ __forceinline__ int map_z(const DevSys &S, int z) {
+  return z < 1 ? S.bz1 : (z > S.MZ ? S.bzm : z);
+}
+
+// number of sites a phase visits: all sites in 1 gradient (ghosts included,
+// for surface boundaries), interior sites otherwise
+__device__ __forceinline__ int n_work_sites(const DevSys &S) {
+  return S.gradients == 1 ? S.M : S.MX * S.MY * (S.gradients == 3 ? S.MZ : 1);
+}
+
+// work index -> lattice site (and its coordinates)
+__device__ __forceinline__ int work_site(const DevSys &S, int w, int &x, int &y, int &z) {
+  if (S.gradients == 1) {
+    x = w;
+    y = z = 0;
+    return w;
+  }
+  if (S.gradients == 2) {
+    x = w / S.MY + 1;
+    y = w - (x - 1) * S.MY + 1;
+    z = 0;
+    return x * S.JX + y;
+  }
+  int yz = S.MY * S.MZ;
+  x = w / yz + 1;
+  int r = w - (x - 1) * yz;
+  y = r / S.MZ + 1;
+  z = r - (y - 1) * S.MZ + 1;
+  return x * S.JX + y * S.JY + z;
+}
+
+__device__ __forceinline__ bool is_interior(const DevSys &S, int x, int y, int z) {
+  if (x < 1 || x > S.MX) return false;
+  if (S.gradients >= 2 && (y < 1 || y > S.MY)) return false;
+  if (S.gradients == 3 && (z < 1 || z > S.MZ)) return false;
+  return true;
+}
+
+// ----------------------------------------------------- Markov-1 step / Side
+// Lattice average of a bounded array at site i=(x,y,z) for the propagator
+// (without the Boltzmann factor).  Arithmetic order follows the reference's
+// element-wise passes (LG1Planar::propagate src/LG1Planar.cpp:146-149,
+// LGrad1::propagate src/LGrad1.cpp:441-445, LGrad2::propagate
+// src/LGrad2.cpp:1058-1073, LG2Planar::propagate src/LG2Planar.cpp:1085-1096,
+// LGrad3::propagate src/LGrad3.cpp:733-743).
+template <typename T>
+__device__ __forceinline__ T stencil_prop(const DevSys &S, const T *__restrict__ g, int i, int x, int y,
+                                          int z) {
+  if (S.gradients == 1) {
+    const int M = S.M;
+    if (S.geometry == 0) {
+      const T lam = (T)S.lambda, c0 = (T)(1.0 - 2.0 * S.lambda);
+      T v = 0;
+      if (x >= 1) v = lam * g[map_x(S, x - 1)];
+      v += c0 * g[map_x(S, x)];
+      if (x <= M - 2) v += lam * g[map_x(S, x + 1)];
+      return v;
+    }
+    T v = (T)S.lam_0[i] * g[map_x(S, x)];
+    if (x >= 1) v += (T)S.lam_m[i] * g[map_x(S, x - 1)];
+    if (x <= M - 2) v += (T)S.lam_p[i] * g[map_x(S, x + 1)];
+    return v;
+  }
+  if (S.gradients == 2) {
+    const int JX = S.JX;
+    const int xm = map_x(S, x - 1) * JX, x0 = x * JX, xp = map_x(S, x + 1) * JX;
+    const int ym = map_y(S, y - 1), yp = map_y(S, y + 1);
+    if (S.geometry == 0) {
+      if (S.stencil_full) {
+        const T C1 = (T)(16.0 / 36.0), C2 = (T)(4.0 / 36.0), C3 = (T)(1.0 / 36.0);
+        T v = C1 * g[x0 + y];
+        v += C2 * g[x0 + ym];
+        v += C2 * g[x0 + yp];
+        v += C2 * g[xm + y];
+        v += C2 * g[xp + y];
+        v += C3 * g[xm + ym];
+        v += C3 * g[xm + yp];
+        v += C3 * g[xp + ym];
+        v += C3 * g[xp + yp];
+        return v;
+      }
+      T v = g[xm + y];
+      v += g[xp + y];
+      v += g[x0 + ym];
+      v += g[x0 + yp];
+      v *= (T)0.5;
+      v += g[x0 + y];
+      return v * (T)(1.0 / 3.0);
+    }
+    const bool sc = S.lattice_type == 0;
+    const T C1 = (T)(sc ? 4.0 / 6.0 : 0.5), C2 = (T)(sc ? 1.0 / 6.0 : 0.25), C3 = (T)(sc ? 4.0 : 2.0);
+    const T lm = (T)S.lam_m[i], lp = (T)S.lam_p[i];
+    T v = C1 * g[x0 + y];
+    v += lm * g[xm + y];
+    v += lp * g[xp + y];
+    v += C2 * g[x0 + ym];
+    v += C2 * g[x0 + yp];
+    v *= C3;
+    v += lm * g[xm + ym];
+    v += lm * g[xm + yp];
+    v += lp * g[xp + ym];
+    v += lp * g[xp + yp];
+    return v * C2;
+  }
+  const int JX = S.JX, JY = S.JY;
+  const int bx = x * JX, by = y * JY;
+  T v = g[map_x(S, x - 1) * JX + by + z];
+  v += g[map_x(S, x + 1) * JX + by + z];
+  v += g[bx + map_y(S, y - 1) * JY + z];
+  v += g[bx + map_y(S, y + 1) * JY + z];
+  v += g[bx + by + map_z(S, z - 1)];
+  v += g[bx + by + map_z(S, z + 1)];
+  return v * (T)(1.0 / 6.0);
+}
+
+// Site average <X> for the Flory-Huggins term (Lattice::Side,
+// src/LG1Planar.cpp:28-55, src/LGrad1.cpp:251-281, src/LGrad2.cpp:207-240,
+// src/LG2Planar.cpp:27-100, src/LGrad3.cpp:156-166).  Identical weights to
+// the propagator; the 1-gradient planar form sums in a different order.
+__device__ __forceinline__ double stencil_side(const DevSys &S, const double *__restrict__ X, int i, int x,
+                                               int y, int z) {
+  if (S.gradients == 1 && S.geometry == 0) {
+    const int M = S.M;
+    const double lam = S.lambda;
+    double v = 0;
+    if (x >= 1) v = lam * X[map_x(S, x - 1)];
+    if (x <= M - 2) v += lam * X[map_x(S, x + 1)];
+    v += (1.0 - 2.0 * lam) * X[map_x(S, x)];
+    return v;
+  }
+  return stencil_prop<double>(S, X, i, x, y, z);
+}
+
+// ----------------------------------------------------- Markov-2 (1 gradient)
+// Bounded value of direction component c (0: +x, 1: lateral, 2: -x) of a
+// Markov-2 row at position x.  The reference bounds components 0 and 2 as a
+// pair, so a mirror reflection turns an outgoing walker into an incoming one
+// (LGrad1::set_bounds(X,Y) src/LGrad1.cpp:787-804).
+template <typename T>
+__device__ __forceinline__ T m2_val(const DevSys &S, const T *__restrict__ row, int c, int x) {
+  const int M = S.M;
+  if (x >= 1 && x <= S.MX) return row[c * M + x];
+  const bool lo = x < 1;
+  const int b = lo ? S.bx1 : S.bxm;
+  const bool surf = lo ? S.surface_lo : S.surface_hi;
+  if (c == 1) return row[M + b];
+  if (c == 0) return row[2 * M + b];
+  return surf ? row[2 * M + x] : row[b];
+}
+
+// Forward (F) or backward (B) Markov-2 step at site x: writes the three
+// direction components (without the Boltzmann factor) to out[0..2].
+// Follows LG1Planar::propagateF/B (src/LG1Planar.cpp:57-136) and
+// LGrad1::propagateF/B (src/LGrad1.cpp:285-429), simple-cubic lattice.
+template <typename T>
+__device__ __forceinline__ void m2_step(const DevSys &S, const T *__restrict__ g, const double *P, bool forward,
+                                        int x, T out[3]) {
+  const int M = S.M;
+  const T P0 = (T)P[0], P1 = (T)P[1], P1x4 = (T)(4 * P[1]), Pm = (T)(2 * P[1] + P[0]);
+  const bool hasm = x >= 1, hasp = x <= M - 2;
+  T a = 0, b = 0, c = 0;
+  if (S.geometry == 0) {
+    if (forward) {
+      if (hasm) {
+        a = P0 * m2_val(S, g, 0, x - 1);
+        a += P1x4 * m2_val(S, g, 1, x - 1);
+      }
+      b = P1 * m2_val(S, g, 0, x);
+      b += Pm * m2_val(S, g, 1, x);
+      b += P1 * m2_val(S, g, 2, x);
+      if (hasp) {
+        c = P1x4 * m2_val(S, g, 1, x + 1);
+        c += P0 * m2_val(S, g, 2, x + 1);
+      }
+    } else {
+      if (hasp) a = P0 * m2_val(S, g, 0, x + 1);
+      a += P1x4 * m2_val(S, g, 1, x);
+      if (hasp) b = P1 * m2_val(S, g, 0, x + 1);
+      b += Pm * m2_val(S, g, 1, x);
+      if (hasm) b += P1 * m2_val(S, g, 2, x - 1);
+      c = P1x4 * m2_val(S, g, 1, x);
+      if (hasm) c += P0 * m2_val(S, g, 2, x - 1);
+    }
+  } else {
+    const T l1 = (T)S.l1[x], l11 = (T)S.l11[x], l_1 = (T)S.l_1[x], l_11 = (T)S.l_11[x];
+    if (forward) {
+      if (hasm) {
+        T h = l_1 * m2_val(S, g, 0, x - 1);
+        h += l_11 * m2_val(S, g, 2, x);
+        a = P0 * h;
+        h = l_1 * m2_val(S, g, 1, x - 1);
+        h += l_11 * m2_val(S, g, 1, x);
+        a += P1x4 * h;
+      }
+      b = P1 * m2_val(S, g, 0, x);
+      b += Pm * m2_val(S, g, 1, x);
+      b += P1 * m2_val(S, g, 2, x);
+      if (hasp) {
+        T h = l1 * m2_val(S, g, 1, x + 1);
+        h += l11 * m2_val(S, g, 1, x);
+        c = P1x4 * h;
+        h = l1 * m2_val(S, g, 2, x + 1);
+        h += l11 * m2_val(S, g, 0, x);
+        c += P0 * h;
+      }
+    } else {
+      T hm = 0, hp = 0;
+      if (hasm) {
+        hm = l_1 * m2_val(S, g, 2, x - 1);
+        hm += l_11 * m2_val(S, g, 0, x);
+      }
+      if (hasp) {
+        hp = l1 * m2_val(S, g, 0, x + 1);
+        hp += l11 * m2_val(S, g, 2, x);
+      }
+      b = P1 * hm;
+      c = P0 * hm;
+      a = P1x4 * m2_val(S, g, 1, x);
+      b += Pm * m2_val(S, g, 1, x);
+      c += P1x4 * m2_val(S, g, 1, x);
+      a += P0 * hp;
+      b += P1 * hp;
+    }
+  }
+  out[0] = a;
+  out[1] = b;
+  out[2] = c;
+}
+
+// ----------------------------------------------------- per-site phases
+// Boltzmann weights of every segment at site i (Segment::PrepareForCalculations
+// src/segment.cpp:835-879): u is the segment's field block with its ghosts
+// bounded, G1 = exp(-u), times the pinning mask, times KSAM.
+__device__ __forceinline__ void site_prepare(const DevSys &S, int i, int x, int y, int z) {
+  int b;  // bounded source site for u
+  if (S.gradients == 1) {
+    b = map_x(S, x);
+  } else if (S.gradients == 2) {
+    b = map_x(S, x) * S.JX + map_y(S, y);
+  } else {
+    b = map_x(S, x) * S.JX + map_y(S, y) * S.JY + map_z(S, z);
+  }
+  const int M = S.M;
+  for (int k = 0; k < S.nseg; k++) {
+    const int blk = S.seg_block[k];
+    const double u = blk >= 0 ? S.x[blk * M + b] : 0.0;
+    double w = exp(-u);
+    if (S.seg_pinned[k]) w = w * S.seg_mask[k * M + i];
+    w = w * S.ksam[i];
+    S.G1[k * M + i] = w;
+  }
+}
+
+// Densities at site i once GN and the normalisations are known
+// (System::ComputePhis src/system.cpp:2345-2610): molecule-segment densities
+// rho/G1*norm, segment densities, total density.
+__device__ __forceinline__ void site_densities(const DevSys &S, int i) {
+  const int M = S.M;
+  const DevScalars &sc = *S.sc;
+  double phitot = 0;
+  double segphi[kMaxSeg];
+  for (int k = 0; k < S.nseg; k++) {
+    segphi[k] = S.seg_frozen[k] ? S.seg_mask[k * M + i] : 0.0;
+    if (S.seg_frozen[k]) phitot += segphi[k];
+  }
+  for (int mo = 0; mo < S.nmol; mo++) {
+    const double nrm = sc.norm[mo];
+    double mtot = 0;
+    for (int mm = S.mol_mm_begin[mo]; mm < S.mol_mm_begin[mo + 1]; mm++) {
+      const int sg = S.mm_seg[mm];
+      const double g1 = S.G1[sg * M + i];
+      double p = S.rho[mm * M + i];
+      p = g1 != 0 ? p / g1 : 0.0;
+      if (nrm > 0) p = p * nrm;
+      S.rho[mm * M + i] = p;
+      segphi[sg] += p;
+      phitot += p;
+      mtot += p;
+    }
+    S.mol_phitot[mo * M + i] = mtot;
+  }
+  for (int k = 0; k < S.nseg; k++) S.phi[k * M + i] = segphi[k];
+  S.phitot[i] = phitot;
+}
+
+// Residual at an interior site (System::Classical_residual
+// src/system.cpp:2099-2157); also stores the site averages phi_side.
+// Returns g_i^2 summed over the field blocks (for the norm).
+__device__ __forceinline__ double site_residual(const DevSys &S, int i, int x, int y, int z) {
+  const int M = S.M;
+  const DevScalars &sc = *S.sc;
+  double side[kMaxSeg];
+  for (int k = 0; k < S.nseg; k++) {
+    side[k] = stencil_side(S, S.phi + k * M, i, x, y, z);
+    S.phi_side[k * M + i] = side[k];
+  }
+  const double pt = S.phitot[i];
+  double gv[kMaxSeg];
+  double alpha = 0;
+  for (int it = 0; it < S.nit; it++) {
+    const int si = S.itmon[it];
+    double gi = S.x[it * M + i];
+    for (int k = 0; k < S.nseg; k++) {
+      const double chi = S.chi[si * kMaxSeg + k];
+      if (chi != 0 && pt > 0) gi = gi - chi * (side[k] / pt - sc.seg_phibulk[k]);
+    }
+    gv[it] = gi;
+    alpha += gi;
+  }
+  alpha = alpha * (1.0 / S.nit);
+  S.alpha[i] = alpha;
+  double g2 = 0;
+  const double ks = S.ksam[i];
+  for (int it = 0; it < S.nit; it++) {
+    double gi = pt > 0 ? gv[it] - alpha + 1 / pt - 1.0 : 0.0;
+    gi = gi * ks;
+    S.g[it * M + i] = gi;
+    g2 += gi * gi;
+  }
+  return g2;
+}
+
+// Free-energy and grand-potential densities at an interior site
+// (System::GetFreeEnergy src/system.cpp:2809-3029, GetGrandPotential
+// src/system.cpp:3038-3244).  Returns both through F and GP.
+__device__ __forceinline__ void site_thermo(const DevSys &S, int i, double logc[kMaxMol], double &F,
+                                            double &GP) {
+  const int M = S.M;
+  const DevScalars &sc = *S.sc;
+  double f = 0;
+  for (int mo = 0; mo < S.nmol; mo++) f += logc[mo] * S.mol_phitot[mo * M + i];
+  for (int j = 0; j < S.nseg; j++) {
+    if (!S.sysmon[j]) continue;
+    const double g1 = S.G1[j * M + i];
+    double t = g1 > 0 ? -log(g1) : 0.0;
+    f += -(S.phi[j * M + i] * t);
+  }
+  for (int j = 0; j < S.nseg; j++) {
+    if (S.seg_frozen[j]) continue;
+    const double pj = S.phi[j * M + i];
+    for (int k = 0; k < S.nseg; k++) {
+      double chi = S.chi[j * kMaxSeg + k];
+      if (!S.seg_frozen[k]) chi = chi / 2;
+      if (chi != 0) f += pj * S.phi_side[k * M + i] * chi;
+    }
+  }
+  for (int mo = 0; mo < S.nmol; mo++) f += S.mol_constF[mo] * S.mol_phitot[mo * M + i];
+  const double ks = S.ksam[i];
+  F = f * ks;
+  double gp = 0;
+  for (int mo = 0; mo < S.nmol; mo++)
+    gp += (S.mol_phitot[mo * M + i] - sc.phibulk[mo]) * (1.0 / S.mol_N[mo]);
+  gp += S.alpha[i];
+  for (int j = 0; j < S.nseg; j++) {
+    if (S.seg_frozen[j]) continue;
+    const double pj = S.phi[j * M + i], pbA = sc.seg_phibulk[j];
+    for (int k = 0; k < S.nseg; k++) {
+      if (S.seg_frozen[k]) continue;
+      const double chi = S.chi[j * kMaxSeg + k] / 2;
+      if (chi != 0) gp += (pj * S.phi_side[k * M + i] - pbA * sc.seg_phibulk[k]) * chi;
+    }
+  }
+  GP = -gp * ks;
+}
+
+// site weight of sums over the lattice (Lattice::WeightedSum): 1 on planar
+// lattices, the site volume L on curved ones; ghosts are excluded by the
+// callers (interior work sites only).
+__device__ __forceinline__ double site_weight(const DevSys &S, int i) {
+  if (S.gradients == 3 || S.geometry == 0) return 1.0;
+  return S.L[i];
+}
+
+// Normalisations after the propagators (System::ComputePhis
+// src/system.cpp:2366-2558).  Single thread.
+__device__ inline void scalars_norms(const DevSys &S) {
+  DevScalars &sc = *S.sc;
+  double B = 0;
+  for (int i = 0; i < S.nmol; i++) {
+    double nrm = 0;
+    const int fr = S.mol_freedom[i];
+    if (fr == 0) {  // free
+      nrm = sc.phibulk[i] / S.mol_N[i];
+      sc.n[i] = nrm * sc.GN[i];
+      B += sc.phibulk[i];
+    }
+    if (fr == 1) {  // restricted
+      if (sc.GN[i] > 0) {
+        nrm = sc.n[i] / sc.GN[i];
+        if (S.mol_pinned[i]) {
+          sc.phibulk[i] = 0;
+        } else {
+          sc.phibulk[i] = S.mol_N[i] * nrm;
+          B += sc.phibulk[i];
+        }
+      } else {
+        sc.status = DS_GN;
+        return;
+      }
+    }
+    if (S.mol_pinned[i]) {
+      if (sc.GN[i] > 0) {
+        if (sc.n[i] == 0) sc.n[i] = 1;
+        nrm = sc.n[i] / sc.GN[i];
+      } else {
+        nrm = 0;
+      }
+      sc.phibulk[i] = 0;
+    }
+    sc.norm[i] = nrm;
+  }
+  if (S.solvent >= 0) {
+    const int sv = S.solvent;
+    sc.phibulk[sv] = 1.0 - B;
+    if (sc.phibulk[sv] < 0) {
+      sc.status = DS_PHIB;
+      return;
+    }
+    const double nrm = sc.phibulk[sv] / S.mol_N[sv];
+    sc.n[sv] = nrm * sc.GN[sv];
+    sc.norm[sv] = nrm;
+  }
+  for (int k = 0; k < S.nseg; k++) sc.seg_phibulk[k] = 0;
+  for (int i = 0; i < S.nmol; i++)
+    for (int mm = S.mol_mm_begin[i]; mm < S.mol_mm_begin[i + 1]; mm++) {
+      const int sg = S.mm_seg[mm];
+      sc.seg_phibulk[sg] += S.frac[i][sg] * sc.phibulk[i];
+    }
+}
+
+// Chemical potentials (System::CreateMu src/system.cpp:3246-3428), single thread.
+__device__ inline void scalars_mu(const DevSys &S) {
+  DevScalars &sc = *S.sc;
+  for (int i = 0; i < S.nmol; i++) {
+    const double NA = S.mol_N[i];
+    double mu = log(NA * sc.n[i] / sc.GN[i]) + 1;
+    double constant = 0;
+    for (int k = 0; k < S.nmol; k++) constant += sc.phibulk[k] / S.mol_N[k];
+    mu = mu - NA * constant;
+    for (int j = 0; j < S.nseg; j++) {
+      const double pbA = sc.seg_phibulk[j], FA = S.frac[i][j];
+      for (int k = 0; k < S.nseg; k++) {
+        const double chi = S.chi[j * kMaxSeg + k] / 2;
+        mu = mu - NA * chi * (pbA - FA) * (sc.seg_phibulk[k] - S.frac[i][k]);
+      }
+    }
+    sc.Mu[i] = mu;
+  }
+}
+
+// One DIIS solve on the recorded Gram row (SFNewton::DIIS
+// src/sfnewton.cpp:837-880): shifts the ring when full, writes row k_diis-1
+// from red[0..kd-1], solves for the coefficients C.  Single thread.
+__device__ __host__ inline bool diis_coefficients(DevScalars &sc, int m, int kd_in, const double *dots, int &kd) {
+  kd = kd_in;
+  double *Aij = sc.A;
+  if (kd > m) {
+    kd = m;
+    for (int i = 1; i < m; i++)
+      for (int j = 1; j < m; j++) Aij[m * (i - 1) + j - 1] = Aij[m * i + j];
+  }
+  double Ap[kMaxDIIS * kMaxDIIS];
+  for (int i = 0; i < kd; i++) {
+    Aij[i + m * (kd - 1)] = Aij[kd - 1 + m * i] = dots[i] + 1e-9;
+  }
+  for (int i = 0; i < kd; i++)
+    for (int j = 0; j < kd; j++) Ap[j + kd * i] = Aij[j + m * i];
+  return la_diis_coefficients(Ap, kd, sc.C);
+}
+
+}  // namespace nscf

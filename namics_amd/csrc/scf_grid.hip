@@ -1,0 +1,421 @@
+// Multi-workgroup SCF path for lattices too large for one workgroup
+// (3-gradient boxes such as 96^3): each phase of a residual evaluation is a
+// grid-stride kernel, each propagation step one launch over all sites, and
+// the DIIS loop is driven from the host with one small readback (status and
+// residual) per iteration.  The per-site arithmetic is shared with the fused
+// kernel (scf_device.h).  Reductions are deterministic: per-workgroup
+// partial sums in HBM, summed in workgroup order by one thread.
+#include <hip/hip_runtime.h>
+
+#include "scf_device.h"
+#include "scf_kernels.h"
+
+namespace nscf {
+namespace {
+
+constexpr int kGridThreads = 256;
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double t = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+  return t;
+}
+
+__device__ __forceinline__ double block_max(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double t = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t = fmax(t, red[w]);
+  return t;
+}
+
+#define GRID_LOOP(S) for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n_work_sites(S); w += gridDim.x * blockDim.x)
+
+__global__ void __launch_bounds__(kGridThreads) k_prepare(const DevSys *__restrict__ d) {
+  const DevSys &S = *d;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    site_prepare(S, i, x, y, z);
+    for (int mm = 0; mm < S.nmm; mm++) S.rho[mm * S.M + i] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_row_init(const DevSys *__restrict__ d, int seg) {
+  const DevSys &S = *d;
+  const double *g1 = S.G1 + seg * S.M;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    S.Gf[i] = g1[i];
+  }
+}
+
+// one forward step: Gf[s] = W(Gf[s-1]) * G1
+__global__ void __launch_bounds__(kGridThreads) k_step_fwd(const DevSys *__restrict__ d, int s, int seg) {
+  const DevSys &S = *d;
+  const int M = S.M;
+  const double *__restrict__ prev = S.Gf + (size_t)(s - 1) * M;
+  double *__restrict__ gs = S.Gf + (size_t)s * M;
+  const double *__restrict__ g1 = S.G1 + seg * M;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    gs[i] = stencil_prop<double>(S, prev, i, x, y, z) * g1[i];
+  }
+}
+
+// first backward row and its composition
+__global__ void __launch_bounds__(kGridThreads) k_bwd_first(const DevSys *__restrict__ d, int s, int seg, int mm) {
+  const DevSys &S = *d;
+  const int M = S.M;
+  const double *g1 = S.G1 + seg * M;
+  const double *gf = S.Gf + (size_t)s * M;
+  double *rho = S.rho + mm * M;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    const double v = g1[i];
+    S.Gb[i] = v;
+    rho[i] += gf[i] * v;
+  }
+}
+
+// backward step with fused composition: Gb[dst] = W(Gb[src]) * G1, rho += Gf[s] Gb[dst]
+__global__ void __launch_bounds__(kGridThreads) k_step_bwd(const DevSys *__restrict__ d, int s, int seg, int mm,
+                                                           int src) {
+  const DevSys &S = *d;
+  const int M = S.M;
+  const double *__restrict__ gb = S.Gb + (size_t)src * M;
+  double *__restrict__ gbn = S.Gb + (size_t)(1 - src) * M;
+  const double *__restrict__ g1 = S.G1 + seg * M;
+  const double *__restrict__ gf = S.Gf + (size_t)s * M;
+  double *__restrict__ rho = S.rho + mm * M;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    const double v = stencil_prop<double>(S, gb, i, x, y, z) * g1[i];
+    gbn[i] = v;
+    rho[i] += gf[i] * v;
+  }
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_gn_partial(const DevSys *__restrict__ d, int s, double *partial) {
+  const DevSys &S = *d;
+  __shared__ double red[kGridThreads / 64];
+  const double *gl = S.Gf + (size_t)s * S.M;
+  double acc = 0;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    if (is_interior(S, x, y, z)) acc += site_weight(S, i) * gl[i];
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// ordered sum of nparts partial sums (one per workgroup), stride `ncols`
+__global__ void k_finalize_gn(const DevSys *__restrict__ d, const double *partial, int nparts, int mo) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double t = 0;
+  for (int b = 0; b < nparts; b++) t += partial[b];
+  d->sc->GN[mo] = t;
+}
+
+__global__ void k_norms(const DevSys *__restrict__ d) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const DevSys &S = *d;
+  DevScalars &sc = *S.sc;
+  for (int i = 0; i < S.nmol; i++) {
+    if (S.mol_freedom[i] == 0) sc.phibulk[i] = S.mol_phibulk_in[i];
+    if (S.mol_freedom[i] == 1) sc.n[i] = S.mol_n_in[i];
+  }
+  scalars_norms(S);
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_densities(const DevSys *__restrict__ d) {
+  const DevSys &S = *d;
+  if (S.sc->status != DS_OK) return;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    site_densities(S, i);
+  }
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_residual_partial(const DevSys *__restrict__ d, double *partial) {
+  const DevSys &S = *d;
+  __shared__ double red[kGridThreads / 64];
+  if (S.sc->status != DS_OK) return;
+  double acc = 0;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    if (!is_interior(S, x, y, z)) continue;
+    const double g2 = site_residual(S, i, x, y, z);
+    if (S.max_g) {
+      for (int it = 0; it < S.nit; it++) acc = fmax(acc, fabs(S.g[it * S.M + i]));
+    } else {
+      acc += g2;
+    }
+  }
+  acc = S.max_g ? block_max(acc, red) : block_sum(acc, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+__global__ void k_finalize_residual(const DevSys *__restrict__ d, const double *partial, int nparts) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const DevSys &S = *d;
+  if (S.sc->status != DS_OK) return;
+  double t = 0;
+  for (int b = 0; b < nparts; b++) t = S.max_g ? fmax(t, partial[b]) : t + partial[b];
+  S.sc->residual = S.max_g ? t : sqrt(t);
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_thermo_partial(const DevSys *__restrict__ d, double *partial) {
+  const DevSys &S = *d;
+  __shared__ double red[kGridThreads / 64];
+  const DevScalars &sc = *S.sc;
+  double logc[kMaxMol];
+  for (int mo = 0; mo < S.nmol; mo++) {
+    const double N = S.mol_N[mo];
+    logc[mo] = log(N * sc.n[mo] / sc.GN[mo]) / N;
+  }
+  double f = 0, gp = 0, th[kMaxMol];
+  for (int mo = 0; mo < kMaxMol; mo++) th[mo] = 0;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    if (!is_interior(S, x, y, z)) continue;
+    double F, GP;
+    site_thermo(S, i, logc, F, GP);
+    const double wt = site_weight(S, i);
+    f += wt * F;
+    gp += wt * GP;
+    for (int mo = 0; mo < S.nmol; mo++) th[mo] += wt * S.mol_phitot[mo * S.M + i];
+  }
+  const int nc = 2 + S.nmol;
+  f = block_sum(f, red);
+  gp = block_sum(gp, red);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x * nc + 0] = f;
+    partial[blockIdx.x * nc + 1] = gp;
+  }
+  for (int mo = 0; mo < S.nmol; mo++) {
+    const double t = block_sum(th[mo], red);
+    if (threadIdx.x == 0) partial[blockIdx.x * nc + 2 + mo] = t;
+  }
+}
+
+__global__ void k_finalize_thermo(const DevSys *__restrict__ d, const double *partial, int nparts) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const DevSys &S = *d;
+  DevScalars &sc = *S.sc;
+  const int nc = 2 + S.nmol;
+  double f = 0, gp = 0;
+  for (int b = 0; b < nparts; b++) {
+    f += partial[b * nc];
+    gp += partial[b * nc + 1];
+  }
+  sc.F = f;
+  sc.GP = gp;
+  for (int mo = 0; mo < S.nmol; mo++) {
+    double t = 0;
+    for (int b = 0; b < nparts; b++) t += partial[b * nc + 2 + mo];
+    sc.theta[mo] = t;
+  }
+  scalars_mu(S);
+}
+
+// ---- DIIS element-wise parts (nvar = nit * M entries, all sites)
+#define VAR_LOOP(S) for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < S.nit * S.M; v += gridDim.x * blockDim.x)
+
+__global__ void __launch_bounds__(kGridThreads) k_copy_x0(const DevSys *__restrict__ d) {
+  const DevSys &S = *d;
+  VAR_LOOP(S) S.x0[v] = S.x[v];
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_axpy_first(const DevSys *__restrict__ d) {
+  const DevSys &S = *d;
+  const double dmax = S.deltamax;
+  VAR_LOOP(S) {
+    const double xv = S.x[v] + (-dmax) * S.g[v];
+    S.x[v] = xv;
+    S.xx0[v] = xv - S.x0[v];
+    S.xR[v] = xv;
+  }
+}
+
+// x -= dmax g; xR[k] = x; xx0[k] = x - x0; partial dots <xx0[posi_i], xx0[k]>
+__global__ void __launch_bounds__(kGridThreads) k_diis_step(const DevSys *__restrict__ d, int k, int kd,
+                                                             double *partial) {
+  const DevSys &S = *d;
+  __shared__ double red[kGridThreads / 64];
+  if (S.sc->status != DS_OK) return;
+  const int nvar = S.nit * S.M, m = S.m;
+  const double dmax = S.deltamax;
+  double *xRk = S.xR + (size_t)k * nvar, *xxk = S.xx0 + (size_t)k * nvar;
+  double acc[kMaxDIIS];
+  for (int i = 0; i < kMaxDIIS; i++) acc[i] = 0;
+  VAR_LOOP(S) {
+    const double xv = S.x[v] + (-dmax) * S.g[v];
+    S.x[v] = xv;
+    xRk[v] = xv;
+    const double e = xv - S.x0[v];
+    xxk[v] = e;
+    for (int i = 0; i < kd; i++) {
+      int posi = k - kd + 1 + i;
+      if (posi < 0) posi += m;
+      const double a = posi == k ? e : S.xx0[(size_t)posi * nvar + v];
+      acc[i] += a * e;
+    }
+  }
+  for (int i = 0; i < kd; i++) {
+    const double t = block_sum(acc[i], red);
+    if (threadIdx.x == 0) partial[blockIdx.x * kMaxDIIS + i] = t;
+  }
+}
+
+__global__ void k_diis_solve(const DevSys *__restrict__ d, const double *partial, int nparts, int k_diis, int kd) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const DevSys &S = *d;
+  DevScalars &sc = *S.sc;
+  if (sc.status != DS_OK) return;
+  double dots[kMaxDIIS];
+  for (int i = 0; i < kd; i++) {
+    double t = 0;
+    for (int b = 0; b < nparts; b++) t += partial[b * kMaxDIIS + i];
+    dots[i] = t;
+  }
+  int kdc;
+  if (!diis_coefficients(sc, S.m, k_diis, dots, kdc)) sc.status = DS_NAN;
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_diis_combine(const DevSys *__restrict__ d, int k, int kd) {
+  const DevSys &S = *d;
+  if (S.sc->status != DS_OK) return;
+  const int nvar = S.nit * S.M, m = S.m;
+  VAR_LOOP(S) {
+    int posi = k - kd + 1;
+    if (posi < 0) posi += m;
+    double xv = 0.0 + S.sc->C[0] * S.xR[(size_t)posi * nvar + v];
+    for (int i = 1; i < kd; i++) {
+      posi = k - kd + 1 + i;
+      if (posi < 0) posi += m;
+      xv += S.sc->C[i] * S.xR[(size_t)posi * nvar + v];
+    }
+    S.x[v] = xv;
+  }
+}
+
+__global__ void k_reset_status(const DevSys *__restrict__ d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) d->sc->status = DS_OK;
+}
+
+}  // namespace
+
+hipError_t grid_residual(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
+  const dim3 B(kGridThreads), G(nb);
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, st, d);
+  hipLaunchKernelGGL(k_prepare, G, B, 0, st, d);
+  for (int mo = 0; mo < h.nmol; mo++) {
+    if (h.mol_markov[mo] != 1) return hipErrorInvalidValue;
+    const int b0 = h.mol_block_begin[mo], b1 = h.mol_block_begin[mo + 1], N = h.mol_N[mo];
+    hipLaunchKernelGGL(k_row_init, G, B, 0, st, d, h.block_seg[b0]);
+    int s = 1;
+    for (int b = b0; b < b1; b++)
+      for (int j = (b == b0 ? 1 : 0); j < h.block_n[b]; j++, s++)
+        hipLaunchKernelGGL(k_step_fwd, G, B, 0, st, d, s, h.block_seg[b]);
+    hipLaunchKernelGGL(k_gn_partial, G, B, 0, st, d, N - 1, partial);
+    hipLaunchKernelGGL(k_finalize_gn, dim3(1), dim3(64), 0, st, d, partial, nb, mo);
+    hipLaunchKernelGGL(k_bwd_first, G, B, 0, st, d, N - 1, h.block_seg[b1 - 1], h.block_mm[b1 - 1]);
+    s = N - 2;
+    int src = 0;
+    for (int b = b1 - 1; b >= b0; b--)
+      for (int j = (b == b1 - 1 ? h.block_n[b] - 2 : h.block_n[b] - 1); j >= 0; j--, s--) {
+        hipLaunchKernelGGL(k_step_bwd, G, B, 0, st, d, s, h.block_seg[b], h.block_mm[b], src);
+        src = 1 - src;
+      }
+  }
+  hipLaunchKernelGGL(k_norms, dim3(1), dim3(64), 0, st, d);
+  hipLaunchKernelGGL(k_densities, G, B, 0, st, d);
+  hipLaunchKernelGGL(k_residual_partial, G, B, 0, st, d, partial);
+  hipLaunchKernelGGL(k_finalize_residual, dim3(1), dim3(64), 0, st, d, partial, nb);
+  return hipGetLastError();
+}
+
+hipError_t grid_thermo(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
+  (void)h;
+  hipLaunchKernelGGL(k_thermo_partial, dim3(nb), dim3(kGridThreads), 0, st, d, partial);
+  hipLaunchKernelGGL(k_finalize_thermo, dim3(1), dim3(64), 0, st, d, partial, nb);
+  return hipGetLastError();
+}
+
+hipError_t grid_copy_x0(const DevSys &h, DevSys *d, int nb, hipStream_t st) {
+  (void)h;
+  hipLaunchKernelGGL(k_copy_x0, dim3(nb), dim3(kGridThreads), 0, st, d);
+  return hipGetLastError();
+}
+
+hipError_t grid_axpy_first(const DevSys &h, DevSys *d, int nb, hipStream_t st) {
+  (void)h;
+  hipLaunchKernelGGL(k_axpy_first, dim3(nb), dim3(kGridThreads), 0, st, d);
+  return hipGetLastError();
+}
+
+hipError_t grid_diis_step(const DevSys &h, DevSys *d, double *partial, int nb, int k, int k_diis,
+                          hipStream_t st) {
+  const int kd = k_diis > h.m ? h.m : k_diis;
+  hipLaunchKernelGGL(k_diis_step, dim3(nb), dim3(kGridThreads), 0, st, d, k, kd, partial);
+  hipLaunchKernelGGL(k_diis_solve, dim3(1), dim3(64), 0, st, d, partial, nb, k_diis, kd);
+  return hipGetLastError();
+}
+
+hipError_t grid_diis_combine(const DevSys &h, DevSys *d, int nb, int k, int k_diis, hipStream_t st) {
+  const int kd = k_diis > h.m ? h.m : k_diis;
+  hipLaunchKernelGGL(k_diis_combine, dim3(nb), dim3(kGridThreads), 0, st, d, k, kd);
+  return hipGetLastError();
+}
+
+// ---- unit-test propagation: out[0] = G1, out[s] = W(out[s-1]) * G1
+namespace {
+__global__ void __launch_bounds__(kGridThreads) k_chain_step(const DevSys *__restrict__ d, const double *G1,
+                                                             const double *prev, double *out) {
+  const DevSys &S = *d;
+  if (S.gradients == 1) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < S.M; i += gridDim.x * blockDim.x)
+      out[i] = stencil_prop<double>(S, prev, i, i, 0, 0) * G1[i];
+    return;
+  }
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    out[i] = stencil_prop<double>(S, prev, i, x, y, z) * G1[i];
+  }
+}
+}  // namespace
+
+hipError_t launch_propagate_chain(DevSys *d, const double *G1, int N, double *out, hipStream_t st) {
+  DevSys h;
+  hipError_t e = hipMemcpyAsync(&h, d, sizeof(DevSys), hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(out, G1, sizeof(double) * h.M, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return e;
+  const int nb = (h.M + kGridThreads - 1) / kGridThreads;
+  for (int s = 1; s < N; s++)
+    hipLaunchKernelGGL(k_chain_step, dim3(nb), dim3(kGridThreads), 0, st, d, G1, out + (size_t)(s - 1) * h.M,
+                       out + (size_t)s * h.M);
+  return hipGetLastError();
+}
+
+}  // namespace nscf

@@ -1,0 +1,31 @@
+// Kernel launchers of the SCF hot path (host-callable).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "scf_device.h"
+
+namespace nscf {
+
+enum FusedMode : int { FUSED_RESIDUAL = 0, FUSED_SOLVE = 1 };
+
+// One workgroup per system: residual evaluation or full DIIS solve.
+// lds_bytes > 0 stages the propagator rows in LDS (2 rows of size*M doubles).
+hipError_t launch_fused(DevSys *const *d_systems, int nsys, int mode, int threads, size_t lds_bytes,
+                        hipStream_t stream);
+
+// Multi-workgroup path for large lattices (one launch per phase / per
+// propagation step).  Host-driven; see scf_grid.hip.
+struct GridPlan;
+hipError_t grid_residual(const DevSys &h, DevSys *d, double *d_partial, int nblocks, hipStream_t stream);
+hipError_t grid_thermo(const DevSys &h, DevSys *d, double *d_partial, int nblocks, hipStream_t stream);
+hipError_t grid_diis_step(const DevSys &h, DevSys *d, double *d_partial, int nblocks, int k, int kd,
+                          hipStream_t stream);
+hipError_t grid_diis_combine(const DevSys &h, DevSys *d, int nblocks, int k, int kd, hipStream_t stream);
+hipError_t grid_axpy_first(const DevSys &h, DevSys *d, int nblocks, hipStream_t stream);
+hipError_t grid_copy_x0(const DevSys &h, DevSys *d, int nblocks, hipStream_t stream);
+
+// Plain chain propagation for unit tests: out[s*M+i] = G_s(i) for a field
+// whose Boltzmann weight is G1 (Lattice::propagate repeated N-1 times).
+hipError_t launch_propagate_chain(DevSys *d, const double *G1, int N, double *out, hipStream_t stream);
+
+}  // namespace nscf

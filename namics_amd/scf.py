@@ -1,0 +1,217 @@
+"""ctypes binding of libnamics_scf.so (include/namics_scf.h).
+
+Mirrors the reference's entry points for the SCF hot path: a system is
+created from `.in` text (the reference's input format), `residual(x)` is
+Solve_scf::residuals / System::Classical_residual, `solve(x)` is
+Solve_scf::Solve with DIIS, `results()` and `profiles()` are what
+System::CheckResults and the `kal`/`pro` outputs report.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libnamics_scf.so")
+_lib = None
+_D = ctypes.POINTER(ctypes.c_double)
+
+SCF_OK = 0
+SCF_NOT_CONVERGED = 1
+MODE_AUTO, MODE_FUSED, MODE_GRID = 0, 1, 2
+MAX_MOL = 8
+MAX_SEG = 8
+
+
+class ScfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("scf error %d: %s" % (code, msg))
+        self.code = code
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("residual", ctypes.c_double),
+                ("kernel_ms", ctypes.c_double)]
+
+
+class Results(ctypes.Structure):
+    _fields_ = [("free_energy", ctypes.c_double), ("grand_potential", ctypes.c_double),
+                ("residual", ctypes.c_double), ("n_mol", ctypes.c_int), ("n_seg", ctypes.c_int),
+                ("GN", ctypes.c_double * MAX_MOL), ("n", ctypes.c_double * MAX_MOL),
+                ("theta", ctypes.c_double * MAX_MOL), ("phibulk", ctypes.c_double * MAX_MOL),
+                ("mu", ctypes.c_double * MAX_MOL), ("seg_phibulk", ctypes.c_double * MAX_SEG)]
+
+
+def library_path():
+    return _LIB
+
+
+def lib():
+    """Load the HIP library; raise if it was not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            raise ImportError("namics_amd/libnamics_scf.so is not built; run `python -m namics_amd.build`")
+        L = ctypes.CDLL(_LIB)
+        vp = ctypes.c_void_p
+        L.scf_system_create.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(vp), ctypes.c_char_p,
+                                        ctypes.c_size_t]
+        L.scf_system_create.restype = ctypes.c_int
+        L.scf_system_destroy.argtypes = [vp]
+        L.scf_system_describe.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+        L.scf_system_nvar.argtypes = [vp]
+        L.scf_system_nvar.restype = ctypes.c_int64
+        L.scf_system_sites.argtypes = [vp]
+        L.scf_system_sites.restype = ctypes.c_int64
+        L.scf_system_device_bytes.argtypes = [vp]
+        L.scf_system_device_bytes.restype = ctypes.c_size_t
+        L.scf_system_set_mode.argtypes = [vp, ctypes.c_int]
+        L.scf_system_set_solver.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.scf_residual.argtypes = [vp, _D, _D, _D]
+        L.scf_solve.argtypes = [vp, _D, ctypes.POINTER(SolveInfo)]
+        L.scf_solve_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(_D),
+                                      ctypes.POINTER(SolveInfo)]
+        L.scf_upload_fields.argtypes = [vp, _D]
+        L.scf_download_fields.argtypes = [vp, _D]
+        L.scf_carry_fields.argtypes = [vp, vp]
+        L.scf_solve_resident.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(SolveInfo)]
+        L.scf_get_results.argtypes = [vp, ctypes.POINTER(Results)]
+        L.scf_get_profiles.argtypes = [vp, _D, _D]
+        L.scf_propagate_chain.argtypes = [vp, _D, ctypes.c_int, _D]
+        L.scf_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _check(rc):
+    if rc < 0:
+        raise ScfError(rc, lib().scf_last_error().decode())
+    return rc
+
+
+class ScfSystem:
+    """One SCF system resident on the current GPU."""
+
+    def __init__(self, text, start=1, mode=MODE_AUTO):
+        L = lib()
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        rc = L.scf_system_create(text.encode(), start, ctypes.byref(h), err, 1024)
+        if rc != 0:
+            raise ScfError(rc, err.value.decode())
+        self.h = h
+        n = L.scf_system_describe(h, None, 0)
+        buf = ctypes.create_string_buffer(n)
+        L.scf_system_describe(h, buf, n)
+        self.desc = json.loads(buf.value.decode())
+        self.M = self.desc["M"]
+        self.nvar = self.desc["nvar"]
+        self.gradients = self.desc["gradients"]
+        self.dims = (self.desc["MX"], self.desc["MY"], self.desc["MZ"])
+        self.seg_names = [s["name"] for s in self.desc["segments"]]
+        self.mol_names = [m["name"] for m in self.desc["molecules"]]
+        self.field_names = list(self.desc["fields"])
+        if mode != MODE_AUTO:
+            self.set_mode(mode)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().scf_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def device_bytes(self):
+        return lib().scf_system_device_bytes(self.h)
+
+    def set_mode(self, mode):
+        _check(lib().scf_system_set_mode(self.h, mode))
+
+    def set_solver(self, tolerance=0.0, deltamax=0.0, m=0, iterationlimit=-1):
+        _check(lib().scf_system_set_solver(self.h, tolerance, deltamax, m, iterationlimit))
+
+    def residual(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.size == self.nvar
+        g = np.zeros(self.nvar)
+        r = ctypes.c_double(0)
+        _check(lib().scf_residual(self.h, _p(x), _p(g), ctypes.byref(r)))
+        return g, r.value
+
+    def solve(self, x=None):
+        x = np.zeros(self.nvar) if x is None else np.array(x, dtype=np.float64)
+        info = SolveInfo()
+        _check(lib().scf_solve(self.h, _p(x), ctypes.byref(info)))
+        return x, info.iterations, info.residual, info.status
+
+    def upload(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert x.size == self.nvar
+        _check(lib().scf_upload_fields(self.h, _p(x)))
+
+    def download(self):
+        x = np.zeros(self.nvar)
+        _check(lib().scf_download_fields(self.h, _p(x)))
+        return x
+
+    def carry_from(self, other):
+        """Start from `other`'s fields (by segment name), device to device."""
+        _check(lib().scf_carry_fields(self.h, other.h))
+
+    def results(self):
+        r = Results()
+        _check(lib().scf_get_results(self.h, ctypes.byref(r)))
+        out = {"free_energy": r.free_energy, "grand_potential": r.grand_potential, "residual": r.residual,
+               "mol": {}}
+        for i, m in enumerate(self.mol_names):
+            out["mol"][m] = dict(GN=r.GN[i], n=r.n[i], theta=r.theta[i], phibulk=r.phibulk[i], Mu=r.mu[i])
+        phi = np.zeros((len(self.seg_names), self.M))
+        mphi = np.zeros((len(self.mol_names), self.M))
+        _check(lib().scf_get_profiles(self.h, _p(phi), _p(mphi)))
+        out["seg_phi"] = {s: phi[i] for i, s in enumerate(self.seg_names)}
+        out["mol_phi"] = {m: mphi[i] for i, m in enumerate(self.mol_names)}
+        return out
+
+    def propagate_chain(self, G1, N):
+        G1 = np.ascontiguousarray(G1, dtype=np.float64)
+        out = np.zeros((N, self.M))
+        _check(lib().scf_propagate_chain(self.h, _p(G1), N, _p(out)))
+        return out
+
+    def interior(self, a):
+        mx, my, mz = self.dims
+        a = np.asarray(a)
+        if self.gradients == 1:
+            return a[1:mx + 1]
+        if self.gradients == 2:
+            return a.reshape(mx + 2, my + 2)[1:mx + 1, 1:my + 1].ravel()
+        return a.reshape(mx + 2, my + 2, mz + 2)[1:mx + 1, 1:my + 1, 1:mz + 1].ravel()
+
+
+def solve_batch(systems, xs=None):
+    """Solve several systems in one launch (one workgroup per system)."""
+    n = len(systems)
+    if xs is None:
+        xs = [np.zeros(s.nvar) for s in systems]
+    xs = [np.array(x, dtype=np.float64) for x in xs]
+    hs = (ctypes.c_void_p * n)(*[s.h.value for s in systems])
+    ps = (_D * n)(*[_p(x) for x in xs])
+    infos = (SolveInfo * n)()
+    _check(lib().scf_solve_batch(hs, n, ps, infos))
+    return xs, [(i.iterations, i.residual, i.status) for i in infos]
+
+
+def solve_resident(systems):
+    """Solve from device-resident fields; returns [(iterations, residual, status, kernel_ms)]."""
+    n = len(systems)
+    hs = (ctypes.c_void_p * n)(*[s.h.value for s in systems])
+    infos = (SolveInfo * n)()
+    _check(lib().scf_solve_resident(hs, n, infos))
+    return [(i.iterations, i.residual, i.status, i.kernel_ms) for i in infos]

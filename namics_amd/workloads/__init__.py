@@ -1,0 +1,128 @@
+"""Synthetic SCF workloads (the benchmark configurations), as `.in` text.
+
+Each generator returns input text in the reference's format plus seeded
+initial fields, so the same problem can be handed to the GPU library and to
+the reference program.
+"""
+import numpy as np
+
+PIE = 3.14159265
+
+
+def brush_cylinder(chi, n_r=64, n_z=128, N=200, sigma=0.05, lattice="simple_cubic", tol=1e-7):
+    """2-gradient cylindrical polymer brush on a rod: a frozen rod S fills the
+    first radial layer (radius 1), chains (X)1(A)N-1 are pinned with their X
+    end on the rod surface (second radial layer, every axial position),
+    monomeric solvent W, chi(A,W) = chi.  Grafting density sigma is chains
+    per unit rod area: n = sigma * 2 PIE R n_z with R = 1."""
+    n = sigma * 2 * PIE * 1.0 * n_z
+    text = """
+lat : cyl : gradients : 2
+lat : cyl : geometry : cylindrical
+lat : cyl : n_layers_x : {nr}
+lat : cyl : n_layers_y : {nz}
+lat : cyl : lattice_type : {lat}
+mon : S : freedom : frozen
+mon : S : frozen_range : 1,1;1,{nz}
+mon : X : freedom : pinned
+mon : X : pinned_range : 2,1;2,{nz}
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : {chi}
+mol : brush : composition : (X)1(A){na}
+mol : brush : freedom : restricted
+mol : brush : n : {n!r}
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+newton : isaac : method : DIIS
+newton : isaac : tolerance : {tol!r}
+newton : isaac : iterationlimit : 5000
+""".format(nr=n_r, nz=n_z, lat=lattice, chi=chi, na=N - 1, n=n, tol=tol)
+    return text
+
+
+def uniform_fields(nvar, seed, amp):
+    return np.random.default_rng(seed).uniform(-amp, amp, nvar)
+
+
+def homopolymer_adsorption(layers, N, chi, chi_s, phib, markov=1, k_stiff=1.0, tol=1e-7):
+    """1-gradient planar adsorption: frozen surface S on the lower ghost layer,
+    homopolymer (A)N free at phib, monomeric solvent W."""
+    extra = ""
+    if markov == 2:
+        extra = "mol : pol : Markov : 2\nmol : pol : k_stiff : %r\n" % k_stiff
+    return """
+lat : flat : gradients : 1
+lat : flat : n_layers : {L}
+lat : flat : lattice_type : simple_cubic
+lat : flat : lowerbound : surface
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : {cs!r}
+mon : A : chi_W : {c!r}
+mol : pol : composition : (A){N}
+mol : pol : freedom : free
+mol : pol : phibulk : {pb!r}
+{extra}mol : water : composition : (W)1
+mol : water : freedom : solvent
+newton : isaac : method : DIIS
+newton : isaac : tolerance : {tol!r}
+newton : isaac : iterationlimit : 5000
+""".format(L=layers, cs=chi_s, c=chi, N=N, pb=phib, extra=extra, tol=tol)
+
+
+def batch_draws(n, seed=12345):
+    """Configuration-4 style draws: layers U{50..400}, N log-uniform in
+    [20,2000], chi U[0.3,1.2], chi_S U[-1.5,0], phib log-uniform in
+    [1e-4,1e-1]; odd draws are second-order Markov with k_stiff = 1."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        L = int(rng.integers(50, 401))
+        N = int(round(float(np.exp(rng.uniform(np.log(20), np.log(2000))))))
+        chi = float(rng.uniform(0.3, 1.2))
+        chi_s = float(rng.uniform(-1.5, 0.0))
+        phib = float(np.exp(rng.uniform(np.log(1e-4), np.log(1e-1))))
+        markov = 2 if i % 2 == 1 else 1
+        out.append(dict(layers=L, N=N, chi=chi, chi_s=chi_s, phib=phib, markov=markov))
+    return out
+
+
+def carry_fields(x_prev, names_prev, names_new, M):
+    """Start a new solve from a previous result the way the reference chains
+    `start` blocks (Solve_scf::Guess, src/solve_scf.cpp:489-524): field blocks
+    are copied by segment name; blocks without a match start at zero."""
+    x = np.zeros(len(names_new) * M)
+    for j, n in enumerate(names_new):
+        if n in names_prev:
+            i = names_prev.index(n)
+            x[j * M:(j + 1) * M] = x_prev[i * M:(i + 1) * M]
+    return x
+
+
+def brush_chain_input(chis=(0.0, 0.5, 0.8), **kw):
+    """The brush solves as one reference input with one `start` block per
+    chi value (each solve starts from the previous result)."""
+    text = brush_cylinder(chis[0], **kw) + "start\n"
+    for c in chis[1:]:
+        text += "mon : A : chi_W : %r\nstart\n" % c
+    return text
+
+
+def write_guess(path, names, dims, x, method="DIIS"):
+    """Write fields in the reference's guess-file format (Lattice::StoreGuess,
+    src/lattice.cpp:971-1019; read back by Lattice::ReadGuess) at full
+    precision: method line, sizes, charge flag, field-block names, values."""
+    mx, my, mz = dims
+    with open(path, "w") as f:
+        f.write("%s\n" % method)
+        f.write(" %d\t%d\t%d\t%d\n" % (mx, my, mz, 1))
+        f.write("false\n")
+        f.write("%d\n" % len(names))
+        for n in names:
+            f.write("%s\n" % n)
+        f.write("0\n")
+        for v in x:
+            f.write("%.17e\n" % v)

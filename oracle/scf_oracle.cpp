@@ -1220,6 +1220,35 @@ int oracle_propagate_chain(void *h, const double *G1, int N, double *out) {
   return 0;
 }
 
+// One application of the lattice operator to X (M values, bounded in place
+// like the reference does): side=0 the propagator stencil with G1 = 1,
+// side=1 the site average used for the interaction term.
+int oracle_apply(void *h, double *X, double *out, int side) {
+  Oracle *o = (Oracle *)h;
+  Lat lat(o->s.lat);
+  if (side) {
+    lat.set_bounds(X);
+    lat.side(out, X);
+  } else {
+    std::vector<double> one(o->M, 1.0);
+    lat.propagate(out, X, one.data());
+  }
+  return 0;
+}
+
+// Markov-2 forward step (1 gradient): in/out hold three components (3M).
+int oracle_apply_markov2(void *h, double *X, double *out, int mol, int forward) {
+  Oracle *o = (Oracle *)h;
+  Lat lat(o->s.lat);
+  std::vector<double> one(o->M, 1.0);
+  const double *P = o->s.mol[mol].P.data();
+  if (forward)
+    lat.propagateF(out, X, one.data(), P);
+  else
+    lat.propagateB(out, X, one.data(), P);
+  return 0;
+}
+
 // A c = 1 by the SVD path (for tests of the DIIS least-squares step)
 int oracle_solve_ones(const double *A, int N, double *c) {
   vec Av(A, A + N * N), C;

@@ -32,6 +32,7 @@ REF = os.path.join(REPO, "oracle", "_ref", "namics")
 sys.path.insert(0, REPO)
 
 from tests.golden.cases import CASES  # noqa: E402
+from namics_amd.workloads import write_guess  # noqa: E402
 
 
 def oracle_layout(text):
@@ -39,20 +40,6 @@ def oracle_layout(text):
     from tests.oracle_lib import Oracle
     o = Oracle(text)
     return o
-
-
-def write_guess(path, names, dims, x):
-    mx, my, mz = dims
-    with open(path, "w") as f:
-        f.write("DIIS\n")
-        f.write(" %d\t%d\t%d\t%d\n" % (mx, my, mz, 1))
-        f.write("false\n")
-        f.write("%d\n" % len(names))
-        for n in names:
-            f.write("%s\n" % n)
-        f.write("0\n")
-        for v in x:
-            f.write("%.17e\n" % v)
 
 
 def read_kal(path):

@@ -38,6 +38,8 @@ def lib():
         L.oracle_mol_phi.argtypes = [ctypes.c_void_p, _D]
         L.oracle_propagate_chain.argtypes = [ctypes.c_void_p, _D, ctypes.c_int, _D]
         L.oracle_solve_ones.argtypes = [_D, ctypes.c_int, _D]
+        L.oracle_apply.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int]
+        L.oracle_apply_markov2.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -123,6 +125,19 @@ class Oracle:
         out = np.zeros((N, self.M))
         lib().oracle_propagate_chain(self.h, _p(G1), N, _p(out))
         return out
+
+    def apply(self, X, side=False):
+        """One lattice-operator application (propagator stencil with G1=1, or Side)."""
+        X = np.array(X, dtype=np.float64)
+        out = np.zeros(self.M)
+        lib().oracle_apply(self.h, _p(X), _p(out), 1 if side else 0)
+        return out
+
+    def apply_markov2(self, X, mol=0, forward=True):
+        X = np.array(X, dtype=np.float64).ravel()
+        out = np.zeros(3 * self.M)
+        lib().oracle_apply_markov2(self.h, _p(X), _p(out), mol, 1 if forward else 0)
+        return out.reshape(3, self.M)
 
     def interior(self, a):
         """Interior sites of a lattice array (ghost layers dropped), x-major."""

@@ -1,0 +1,261 @@
+"""Invariants that pin the oracle independently of the reference.
+
+Closed forms of the lattice operators (eigenmodes, row sums, detailed
+balance), composition-law identities, the chemical-potential closed form and
+the DIIS least-squares step.  Each would catch a dropped term, a wrong
+weight, a wrong boundary remap or a transposed operand.
+"""
+import numpy as np
+import pytest
+
+from tests.oracle_lib import Oracle, solve_ones
+
+LAT1 = """
+lat : L : gradients : 1
+lat : L : n_layers : {n}
+lat : L : lattice_type : {lt}
+lat : L : geometry : {geo}
+{extra}
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.5
+mol : pol : composition : (A)10
+mol : pol : freedom : free
+mol : pol : phibulk : 0.01
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+"""
+
+
+def lat1(n=24, lt="simple_cubic", geo="planar", extra=""):
+    return Oracle(LAT1.format(n=n, lt=lt, geo=geo, extra=extra))
+
+
+@pytest.mark.parametrize("lt,lam", [("simple_cubic", 1 / 6), ("hexagonal", 1 / 4)])
+def test_planar_mirror_eigenmodes(lt, lam):
+    """cos(k(x-1/2)), k = pi j / MX, are eigenvectors of the mirror-bounded
+    (lambda, 1-2 lambda, lambda) stencil with eigenvalue 1-2lambda+2lambda cos k."""
+    o = lat1(n=20, lt=lt)
+    MX = 20
+    xs = np.arange(1, MX + 1)
+    for j in (0, 1, 5, 19):
+        k = np.pi * j / MX
+        v = np.zeros(o.M)
+        v[1:MX + 1] = np.cos(k * (xs - 0.5))
+        w = o.apply(v)
+        ev = 1 - 2 * lam + 2 * lam * np.cos(k)
+        np.testing.assert_allclose(w[1:MX + 1], ev * v[1:MX + 1], atol=1e-13)
+        # Side uses the same weights
+        np.testing.assert_allclose(o.apply(v, side=True)[1:MX + 1], ev * v[1:MX + 1], atol=1e-13)
+
+
+def test_planar_periodic_eigenmodes():
+    o = lat1(n=16, extra="lat : L : lowerbound : periodic\nlat : L : upperbound : periodic")
+    MX, lam = 16, 1 / 6
+    xs = np.arange(1, MX + 1)
+    for j in (1, 3, 8):
+        k = 2 * np.pi * j / MX
+        v = np.zeros(o.M)
+        v[1:MX + 1] = np.sin(k * xs + 0.3)
+        ev = 1 - 2 * lam + 2 * lam * np.cos(k)
+        np.testing.assert_allclose(o.apply(v)[1:MX + 1], ev * v[1:MX + 1], atol=1e-13)
+
+
+@pytest.mark.parametrize("geo,extra", [("cylindrical", ""), ("cylindrical", "lat : L : offset_first_layer : 4"),
+                                       ("spherical", ""), ("spherical", "lat : L : offset_first_layer : 2.5")])
+def test_curved_rows_sum_to_one_and_detailed_balance(geo, extra):
+    """Curved-lattice weights: W 1 = 1 (mirror bounds) and W is symmetric in
+    the L-weighted inner product (L(x) lambda+(x) = L(x+1) lambda-(x+1))."""
+    o = lat1(n=18, geo=geo, extra=extra)
+    MX = 18
+    one = np.ones(o.M)
+    np.testing.assert_allclose(o.apply(one)[1:MX + 1], 1.0, atol=1e-14)
+    rng = np.random.default_rng(0)
+    u = np.zeros(o.M)
+    v = np.zeros(o.M)
+    u[1:MX + 1] = rng.standard_normal(MX)
+    v[1:MX + 1] = rng.standard_normal(MX)
+    # site volumes from the reference formulas with its pi constant
+    PIE = 3.14159265
+    off = 0.0 if not extra else float(extra.split(":")[-1])
+    r = off + np.arange(1, MX + 1)
+    L = PIE * (r ** 2 - (r - 1) ** 2) if geo == "cylindrical" else 4 / 3 * PIE * (r ** 3 - (r - 1) ** 3)
+    Wu = o.apply(u)[1:MX + 1]
+    Wv = o.apply(v)[1:MX + 1]
+    lhs = np.sum(L * u[1:MX + 1] * Wv)
+    rhs = np.sum(L * v[1:MX + 1] * Wu)
+    assert abs(lhs - rhs) < 1e-11 * np.sum(L)
+
+
+BOX = """
+lat : box : gradients : 3
+lat : box : n_layers_x : 6
+lat : box : n_layers_y : 5
+lat : box : n_layers_z : 4
+lat : box : lattice_type : simple_cubic
+lat : box : lowerbound_x : periodic
+lat : box : upperbound_x : periodic
+lat : box : lowerbound_y : {by}
+lat : box : upperbound_y : {by}
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : A : freedom : free
+mon : W : freedom : free
+mol : pol : composition : (A)5
+mol : pol : freedom : free
+mol : pol : phibulk : 0.1
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+"""
+
+
+@pytest.mark.parametrize("by", ["periodic", "mirror"])
+def test_3d_separable_eigenmodes(by):
+    """Products of 1-D modes are eigenvectors of the 6-point stencil with
+    eigenvalue (cos kx + cos ky + cos kz)/3."""
+    o = Oracle(BOX.format(by=by))
+    MX, MY, MZ = 6, 5, 4
+    x = np.arange(1, MX + 1)[:, None, None]
+    y = np.arange(1, MY + 1)[None, :, None]
+    z = np.arange(1, MZ + 1)[None, None, :]
+    kx, kz = 2 * np.pi * 2 / MX, 2 * np.pi * 1 / MZ
+    if by == "periodic":
+        ky = 2 * np.pi * 1 / MY
+        fy = np.cos(ky * y)
+    else:
+        ky = np.pi * 2 / MY
+        fy = np.cos(ky * (y - 0.5))
+    mode = np.cos(kx * x) * fy * np.sin(kz * z + 0.1)
+    v = np.zeros((MX + 2, MY + 2, MZ + 2))
+    v[1:-1, 1:-1, 1:-1] = mode
+    w = o.apply(v.ravel()).reshape(MX + 2, MY + 2, MZ + 2)[1:-1, 1:-1, 1:-1]
+    ev = (np.cos(kx) + np.cos(ky) + np.cos(kz)) / 3
+    np.testing.assert_allclose(w, ev * mode, atol=1e-13)
+
+
+PLANAR2D = """
+lat : L : gradients : 2
+lat : L : n_layers_x : 7
+lat : L : n_layers_y : 9
+lat : L : lattice_type : simple_cubic
+lat : L : lowerbound_y : periodic
+lat : L : upperbound_y : periodic
+mon : A : freedom : free
+mon : W : freedom : free
+mol : pol : composition : (A)5
+mol : pol : freedom : free
+mol : pol : phibulk : 0.1
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+"""
+
+
+def test_2d_planar_separable_weights():
+    """The 9-point stencil is (1,4,1)/6 (x) (1,4,1)/6: eigenvalue
+    (4+2cos kx)(4+2cos ky)/36 for mirror-x / periodic-y modes."""
+    o = Oracle(PLANAR2D)
+    MX, MY = 7, 9
+    kx, ky = np.pi * 3 / MX, 2 * np.pi * 2 / MY
+    x = np.arange(1, MX + 1)[:, None]
+    y = np.arange(1, MY + 1)[None, :]
+    mode = np.cos(kx * (x - 0.5)) * np.cos(ky * y + 0.2)
+    v = np.zeros((MX + 2, MY + 2))
+    v[1:-1, 1:-1] = mode
+    w = o.apply(v.ravel()).reshape(MX + 2, MY + 2)[1:-1, 1:-1]
+    np.testing.assert_allclose(w, (4 + 2 * np.cos(kx)) * (4 + 2 * np.cos(ky)) / 36 * mode, atol=1e-13)
+
+
+CYL2D = """
+lat : cyl : gradients : 2
+lat : cyl : geometry : cylindrical
+lat : cyl : n_layers_x : 8
+lat : cyl : n_layers_y : 6
+lat : cyl : lattice_type : simple_cubic
+mon : A : freedom : free
+mon : W : freedom : free
+mol : pol : composition : (A)5
+mol : pol : freedom : free
+mol : pol : phibulk : 0.1
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+"""
+
+
+def test_2d_cylindrical_is_radial_times_axial():
+    """The cylindrical 2-D stencil factorises into the 1-D cylindrical radial
+    operator (with lambda scaled so the radial row is (lambda-, 4/6, lambda+))
+    times the axial (1,4,1)/6: a product f(x) g(y) maps to (W_r f)(W_z g)."""
+    o2 = Oracle(CYL2D)
+    MX, MY = 8, 6
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal(MX)
+    ky = np.pi * 1 / MY
+    gy = np.cos(ky * (np.arange(1, MY + 1) - 0.5))
+    v = np.zeros((MX + 2, MY + 2))
+    v[1:-1, 1:-1] = f[:, None] * gy[None, :]
+    w = o2.apply(v.ravel()).reshape(MX + 2, MY + 2)[1:-1, 1:-1]
+    PIE = 3.14159265
+    r = np.arange(1, MX + 1).astype(float)
+    L = PIE * (r ** 2 - (r - 1) ** 2)
+    lp = 2 * PIE * r / L / 6
+    lm = 2 * PIE * (r - 1) / L / 6
+    fb = np.concatenate([[f[0]], f, [f[-1]]])  # mirror bounds
+    Wf = lm * fb[:-2] + (4 / 6) * fb[1:-1] + lp * fb[2:]
+    np.testing.assert_allclose(w, Wf[:, None] * ((4 + 2 * np.cos(ky)) / 6 * gy)[None, :], atol=1e-13)
+
+
+def test_markov2_straight_and_turn_weights():
+    """Markov-2 step on a uniform field: each direction component keeps the
+    total weight P0 + 4 P1 = 1 (normalised bending weights, no backfolding),
+    and with k_stiff = 0 P0 = P1 = 1/5."""
+    text = LAT1.format(n=12, lt="simple_cubic", geo="planar", extra="").replace(
+        "mol : pol : phibulk : 0.01", "mol : pol : phibulk : 0.01\nmol : pol : Markov : 2\nmol : pol : k_stiff : 0")
+    o = Oracle(text)
+    X = np.ones((3, o.M))
+    out = o.apply_markov2(X)
+    np.testing.assert_allclose(out[:, 1:13], 1.0, atol=1e-14)
+    # a walker in +x only: next +x weight P0 on layer x+1, lateral 4P1 there... P0 = P1 = 1/5
+    X = np.zeros((3, o.M))
+    X[0, 6] = 1.0
+    out = o.apply_markov2(X)
+    assert abs(out[0, 7] - 0.2) < 1e-15      # straight on
+    assert abs(out[1, 6] - 0.2) < 1e-15      # turn: lateral component at the same layer (one of 4 lateral)
+    assert abs(out[2, 5]) < 1e-15            # no backfolding
+
+
+def test_composition_identity_and_mu_closed_form(golden):
+    """For a homopolymer, sum_x Gf(x,s) Gb(x,s) / G1(x) = GN for every s, so
+    sum_x phi_pol(x) = n N = theta.  At zero field with no wall the chemical
+    potential is the Flory-Huggins value."""
+    fx = golden["cfg1_planar_adsorption"]
+    o = Oracle(fx["text"])
+    x = np.array(fx["evaluation"][0]["x"])
+    o.residual(x)
+    r = o.results()
+    theta = r["mol"]["pol"]["theta"]
+    assert abs(np.sum(o.interior(r["mol_phi"]["pol"])) - theta) < 1e-10 * theta
+    assert abs(theta - r["mol"]["pol"]["n"] * 100) < 1e-10 * theta
+    # Flory-Huggins mu of the bulk: ln(phi) + 1 - N(phi/N + phiW) + N chi phiW^2
+    phi, N, chi = 0.01, 100, 0.5
+    mu = np.log(phi) + 1 - N * (phi / N + (1 - phi)) + N * chi * (1 - phi) ** 2
+    assert abs(r["mol"]["pol"]["Mu"] - mu) < 1e-12 * abs(mu)
+
+
+def test_homogeneous_bulk_is_a_fixed_point():
+    """Without walls the homogeneous bulk (x = 0) has g = 0 and Omega = 0."""
+    o = lat1(n=30)
+    g, res = o.residual(np.zeros(o.nvar))
+    assert res < 1e-13
+    assert abs(o.results()["grand_potential"]) < 1e-12
+
+
+def test_diis_solve_matches_numpy():
+    """DIIS coefficients c = A^-1 1 / sum(A^-1 1) (the reference returns 1
+    directly for a 1 x 1 system)."""
+    rng = np.random.default_rng(7)
+    for n in (1, 2, 3, 6, 10):
+        B = rng.standard_normal((n + 3, n))
+        A = B.T @ B + 1e-3 * np.eye(n)
+        c = solve_ones(A)
+        ref = np.linalg.solve(A, np.ones(n))
+        np.testing.assert_allclose(c / c.sum(), ref / ref.sum(), rtol=1e-9, atol=1e-12)
