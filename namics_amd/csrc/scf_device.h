@@ -361,28 +361,32 @@ __device__ __forceinline__ void site_densities(const DevSys &S, int i) {
   const int M = S.M;
   const DevScalars &sc = *S.sc;
   double phitot = 0;
-  double segphi[kMaxSeg];
-  for (int k = 0; k < S.nseg; k++) {
-    segphi[k] = S.seg_frozen[k] ? S.seg_mask[k * M + i] : 0.0;
-    if (S.seg_frozen[k]) phitot += segphi[k];
-  }
+#pragma unroll
+  for (int k = 0; k < kMaxSeg; k++)
+    if (k < S.nseg && S.seg_frozen[k]) phitot += S.seg_mask[k * M + i];
   for (int mo = 0; mo < S.nmol; mo++) {
     const double nrm = sc.norm[mo];
     double mtot = 0;
     for (int mm = S.mol_mm_begin[mo]; mm < S.mol_mm_begin[mo + 1]; mm++) {
-      const int sg = S.mm_seg[mm];
-      const double g1 = S.G1[sg * M + i];
+      const double g1 = S.G1[S.mm_seg[mm] * M + i];
       double p = S.rho[mm * M + i];
       p = g1 != 0 ? p / g1 : 0.0;
       if (nrm > 0) p = p * nrm;
       S.rho[mm * M + i] = p;
-      segphi[sg] += p;
       phitot += p;
       mtot += p;
     }
     S.mol_phitot[mo * M + i] = mtot;
   }
-  for (int k = 0; k < S.nseg; k++) S.phi[k * M + i] = segphi[k];
+  // segment densities: frozen mask, then molecule contributions in order
+#pragma unroll
+  for (int k = 0; k < kMaxSeg; k++) {
+    if (k >= S.nseg) break;
+    double p = S.seg_frozen[k] ? S.seg_mask[k * M + i] : 0.0;
+    for (int mm = 0; mm < S.nmm; mm++)
+      if (S.mm_seg[mm] == k) p += S.rho[mm * M + i];
+    S.phi[k * M + i] = p;
+  }
   S.phitot[i] = phitot;
 }
 
@@ -393,32 +397,46 @@ __device__ __forceinline__ double site_residual(const DevSys &S, int i, int x, i
   const int M = S.M;
   const DevScalars &sc = *S.sc;
   double side[kMaxSeg];
-  for (int k = 0; k < S.nseg; k++) {
-    side[k] = stencil_side(S, S.phi + k * M, i, x, y, z);
-    S.phi_side[k * M + i] = side[k];
+#pragma unroll
+  for (int k = 0; k < kMaxSeg; k++) {
+    side[k] = 0.0;
+    if (k < S.nseg) {
+      side[k] = stencil_side(S, S.phi + k * M, i, x, y, z);
+      S.phi_side[k * M + i] = side[k];
+    }
   }
   const double pt = S.phitot[i];
   double gv[kMaxSeg];
   double alpha = 0;
-  for (int it = 0; it < S.nit; it++) {
-    const int si = S.itmon[it];
-    double gi = S.x[it * M + i];
-    for (int k = 0; k < S.nseg; k++) {
-      const double chi = S.chi[si * kMaxSeg + k];
-      if (chi != 0 && pt > 0) gi = gi - chi * (side[k] / pt - sc.seg_phibulk[k]);
+#pragma unroll
+  for (int it = 0; it < kMaxSeg; it++) {
+    gv[it] = 0.0;
+    if (it < S.nit) {
+      const int si = S.itmon[it];
+      double gi = S.x[it * M + i];
+#pragma unroll
+      for (int k = 0; k < kMaxSeg; k++) {
+        if (k < S.nseg) {
+          const double chi = S.chi[si * kMaxSeg + k];
+          if (chi != 0 && pt > 0) gi = gi - chi * (side[k] / pt - sc.seg_phibulk[k]);
+        }
+      }
+      gv[it] = gi;
+      alpha += gi;
     }
-    gv[it] = gi;
-    alpha += gi;
   }
   alpha = alpha * (1.0 / S.nit);
   S.alpha[i] = alpha;
   double g2 = 0;
   const double ks = S.ksam[i];
-  for (int it = 0; it < S.nit; it++) {
-    double gi = pt > 0 ? gv[it] - alpha + 1 / pt - 1.0 : 0.0;
-    gi = gi * ks;
-    S.g[it * M + i] = gi;
-    g2 += gi * gi;
+#pragma unroll
+  for (int it = 0; it < kMaxSeg; it++) {
+    if (it < S.nit) {
+      double gi = pt > 0 ? gv[it] - alpha + 1 / pt - 1.0 : 0.0;
+      gi = gi * ks;
+      S.g[it * M + i] = gi;
+      g2 += gi * gi;
+    }
   }
   return g2;
 }
@@ -431,7 +449,9 @@ __device__ __forceinline__ void site_thermo(const DevSys &S, int i, double logc[
   const int M = S.M;
   const DevScalars &sc = *S.sc;
   double f = 0;
-  for (int mo = 0; mo < S.nmol; mo++) f += logc[mo] * S.mol_phitot[mo * M + i];
+#pragma unroll
+  for (int mo = 0; mo < kMaxMol; mo++)
+    if (mo < S.nmol) f += logc[mo] * S.mol_phitot[mo * M + i];
   for (int j = 0; j < S.nseg; j++) {
     if (!S.sysmon[j]) continue;
     const double g1 = S.G1[j * M + i];
@@ -447,7 +467,9 @@ __device__ __forceinline__ void site_thermo(const DevSys &S, int i, double logc[
       if (chi != 0) f += pj * S.phi_side[k * M + i] * chi;
     }
   }
-  for (int mo = 0; mo < S.nmol; mo++) f += S.mol_constF[mo] * S.mol_phitot[mo * M + i];
+#pragma unroll
+  for (int mo = 0; mo < kMaxMol; mo++)
+    if (mo < S.nmol) f += S.mol_constF[mo] * S.mol_phitot[mo * M + i];
   const double ks = S.ksam[i];
   F = f * ks;
   double gp = 0;
@@ -551,24 +573,86 @@ __device__ inline void scalars_mu(const DevSys &S) {
   }
 }
 
-// One DIIS solve on the recorded Gram row (SFNewton::DIIS
-// src/sfnewton.cpp:837-880): shifts the ring when full, writes row k_diis-1
-// from red[0..kd-1], solves for the coefficients C.  Single thread.
-__device__ __host__ inline bool diis_coefficients(DevScalars &sc, int m, int kd_in, const double *dots, int &kd) {
-  kd = kd_in;
+// DIIS coefficients on the device by one wavefront: c = A^-1 (1..1) / sum,
+// A is the kd x kd Gram matrix (leading dimension kd).  Lane r holds row r;
+// Gauss-Jordan elimination with partial pivoting (pivot = largest |a_rk|
+// among rows not yet used).  Mathematically the reference's SVD solve
+// (SFNewton::Ax src/sfnewton.cpp:793-835) computes the same vector; a
+// parallel pivoted elimination takes microseconds where a single-lane SVD
+// would take a large part of an iteration.  Returns false on NaN / zero pivot.
+__device__ inline bool wave_diis_coefficients(const double *A, int n, double *C) {
+  const int lane = threadIdx.x & 63;
+  double row[kMaxDIIS];
+  double rhs = 1.0;
+  bool used = false, bad = false;
+  int pc = -1;  // column this lane's row became the pivot of
+#pragma unroll
+  for (int c = 0; c < kMaxDIIS; c++) row[c] = (lane < n && c < n) ? A[lane * n + c] : 0.0;
+  for (int k = 0; k < n; k++) {
+    double ak = 0.0;
+#pragma unroll
+    for (int c = 0; c < kMaxDIIS; c++)
+      if (c == k) ak = row[c];
+    // pivot: largest |a_rk| over rows not yet used (lowest lane on ties)
+    double cand = (lane < n && !used) ? fabs(ak) : -1.0;
+    if (cand != cand) cand = 1e308;  // a NaN is picked and reported
+    int who = lane;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double oc = __shfl_xor(cand, off, 64);
+      const int ow = __shfl_xor(who, off, 64);
+      if (oc > cand || (oc == cand && ow < who)) {
+        cand = oc;
+        who = ow;
+      }
+    }
+    if (lane == who) {
+      used = true;
+      pc = k;
+    }
+    const double piv = __shfl(ak, who, 64);
+    const double prhs = __shfl(rhs, who, 64);
+    if (!(piv != 0.0) || piv != piv) bad = true;
+    const bool upd = lane < n && lane != who && piv != 0.0;
+    const double f = upd ? ak / piv : 0.0;
+#pragma unroll
+    for (int c = 0; c < kMaxDIIS; c++) {
+      const double pcv = __shfl(row[c], who, 64);
+      if (upd) row[c] = (c == k) ? 0.0 : row[c] - f * pcv;
+    }
+    if (upd) rhs -= f * prhs;
+  }
+  double x = 0.0;
+  if (lane < n && pc >= 0) {
+    double d = 0.0;
+#pragma unroll
+    for (int c = 0; c < kMaxDIIS; c++)
+      if (c == pc) d = row[c];
+    x = rhs / d;
+  }
+  double sum = lane < n ? x : 0.0;
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  int anybad = (bad || (lane < n && (pc < 0 || x != x))) ? 1 : 0;
+  for (int off = 32; off > 0; off >>= 1) anybad |= __shfl_xor(anybad, off, 64);
+  if (lane < n && pc >= 0) C[pc] = x / sum;
+  return anybad == 0 && sum == sum && sum != 0.0;
+}
+
+// Record the new Gram row (SFNewton::DIIS src/sfnewton.cpp:837-867): shift
+// the ring when it is full, write row/column kd-1 from dots (+1e-9 on every
+// entry, as the reference does), and pack the kd x kd matrix into Ap.
+// Single thread.  Returns kd.
+__device__ __host__ inline int diis_gram_update(DevScalars &sc, int m, int k_diis, const double *dots, double *Ap) {
+  int kd = k_diis;
   double *Aij = sc.A;
   if (kd > m) {
     kd = m;
     for (int i = 1; i < m; i++)
       for (int j = 1; j < m; j++) Aij[m * (i - 1) + j - 1] = Aij[m * i + j];
   }
-  double Ap[kMaxDIIS * kMaxDIIS];
-  for (int i = 0; i < kd; i++) {
-    Aij[i + m * (kd - 1)] = Aij[kd - 1 + m * i] = dots[i] + 1e-9;
-  }
+  for (int i = 0; i < kd; i++) Aij[i + m * (kd - 1)] = Aij[kd - 1 + m * i] = dots[i] + 1e-9;
   for (int i = 0; i < kd; i++)
     for (int j = 0; j < kd; j++) Ap[j + kd * i] = Aij[j + m * i];
-  return la_diis_coefficients(Ap, kd, sc.C);
+  return kd;
 }
 
 }  // namespace nscf

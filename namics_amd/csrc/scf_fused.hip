@@ -49,6 +49,9 @@ __device__ __forceinline__ double wg_max(double v, double *red) {
 }
 
 // ------------------------------------------------------------- propagation
+constexpr int kStripX = 8;    // sites per thread along x in the 2-gradient step
+constexpr int kMaxLamX = 1026;  // radial layers + 2 held in the LDS weight table
+
 // Forward and backward recursion of one linear chain with the composition
 // law fused into the backward step.  rows0/rows1 are the two LDS row
 // buffers (size*M each) or null, in which case the rows are read from the
@@ -57,7 +60,7 @@ struct BlockWalk {
   int b, j;  // current block and position inside it
 };
 
-__device__ void chain_markov1(const DevSys &S, int mo, double *rows0, double *rows1, double *red) {
+__device__ __noinline__ void chain_markov1(const DevSys &S, int mo, double *rows0, double *rows1, double *red) {
   const int M = S.M, W = n_work_sites(S);
   const int tid = threadIdx.x, nt = blockDim.x;
   const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
@@ -149,8 +152,193 @@ __device__ void chain_markov1(const DevSys &S, int mo, double *rows0, double *ro
   }
 }
 
+// 2-gradient Markov-1 chain, register-blocked.  Thread t owns the strip of
+// SX sites (x0..x0+SX-1, y) with y = t % MY + 1: lanes of a wavefront walk the
+// contiguous y axis, so every LDS row read is unit-stride across lanes.  A
+// step reads the three rows y-1, y, y+1 over x0-1..x0+SX (3(SX+2) LDS reads
+// for SX sites instead of 9 per site), multiplies by the Boltzmann weights
+// held in registers for the current block, writes the end-point
+// distribution to HBM and the next LDS row.  The density of the current
+// block accumulates in registers and is flushed at block boundaries.
+// The 9-point weights are those of LGrad2::propagate (src/LGrad2.cpp:1058-1073);
+// the planar full stencil (src/LG2Planar.cpp:1085-1096) is the same form with
+// lambda- = lambda+ = 1/6.
+// One lattice step of a strip from row `src`: v = W(src) * g1 (rolling
+// 3x3 window over x, rows y-1, y, y+1).
+// Keeps a value opaque to the optimiser so address arithmetic derived from
+// it is recomputed inside the step loop instead of being hoisted into (and
+// spilling) registers.
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+template <int SX>
+__device__ __forceinline__ void strip_step(const double *__restrict__ src, int first, int JX, int last, int ym,
+                                           int y, int yp, int cnt, int x0, const double2 *lam, double C1,
+                                           double C2, double C3, const double (&g1)[SX], double (&v)[SX]) {
+  // columns: first (mapped x0-1), x0 .. x0+SX-1 at stride JX, last (mapped x0+cnt)
+  const int dm = opaque(ym - y), dp = opaque(yp - y);
+  const double *c0p = src + first + y;
+  double am = c0p[dm], bm = c0p[0], cm = c0p[dp];
+  const double *cp_ = src + x0 * JX + y;
+  double a0 = cp_[dm], b0 = cp_[0], c0 = cp_[dp];
+#pragma unroll
+  for (int k = 1; k <= SX; k++) {
+    const double *q = (k < SX) ? cp_ + k * JX : src + last + y;
+    const double ap = q[dm], bp = q[0], cp = q[dp];
+    const double2 lw = lam[opaque(x0 + k - 1)];
+    const double lmk = lw.x, lpk = lw.y;
+    double t = C1 * b0;
+    t += lmk * bm;
+    t += lpk * bp;
+    t += C2 * a0;
+    t += C2 * c0;
+    t *= C3;
+    t += lmk * am;
+    t += lmk * cm;
+    t += lpk * ap;
+    t += lpk * cp;
+    t *= C2;
+    v[k - 1] = t * g1[k - 1];
+    am = a0;
+    a0 = ap;
+    bm = b0;
+    b0 = bp;
+    cm = c0;
+    c0 = cp;
+  }
+}
+
+template <int SX>
+__device__ __noinline__ void chain2d_m1(const DevSys &S, int mo, double *rows0, double *rows1, double *red,
+                                        double2 *lam) {
+  const int MX = S.MX, MY = S.MY, JX = S.JX;
+  const int tid = threadIdx.x;
+  const int nsx = (MX + SX - 1) / SX;
+  const bool active = tid < nsx * MY;
+  const int y = active ? tid % MY + 1 : 1;
+  const int x0 = active ? (tid / MY) * SX + 1 : 1;
+  const int cnt = SX;  // the caller guarantees MX % SX == 0
+  const int ym = map_y(S, y - 1), yp = map_y(S, y + 1);
+  const bool cyl = S.geometry != 0;
+  const bool sc = S.lattice_type == 0;
+  const double C1 = sc ? 4.0 / 6.0 : 0.5, C2 = sc ? 1.0 / 6.0 : 0.25, C3 = sc ? 4.0 : 2.0;
+  const int first = map_x(S, x0 - 1) * JX, last = map_x(S, x0 + cnt) * JX;
+  const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
+  const int N = S.mol_N[mo];
+  const size_t M = S.M;
+  const bool lds = rows0 != nullptr;
+  double *Gf = S.Gf;
+  // radial weights per x (wave-uniform reads from LDS): (lambda-, lambda+)
+  for (int x = tid; x < MX + 2; x += blockDim.x)
+    lam[x] = cyl ? make_double2(S.lam_m[x * JX + 1], S.lam_p[x * JX + 1]) : make_double2(1.0 / 6.0, 1.0 / 6.0);
+  __syncthreads();
+  const int base = x0 * JX + y;  // site of k = 0; site k is base + k*JX
+  double g1[SX], v[SX], racc[SX];
+
+#define LOAD_G1(seg)                                              \
+  {                                                               \
+    const double *p_ = S.G1 + (seg) * M + base;                   \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) g1[k] = p_[k * JX]; \
+  }
+#define STORE_V(row)                                              \
+  {                                                               \
+    double *r_ = (row) + opaque(base);                            \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) r_[k * JX] = v[k]; \
+  }
+
+  // ---- forward
+  LOAD_G1(S.block_seg[b0]);
+#pragma unroll
+  for (int k = 0; k < SX; k++) v[k] = g1[k];
+  if (active) {
+    STORE_V(Gf);
+    if (lds) STORE_V(rows0);
+  }
+  __syncthreads();
+  const double *prev = lds ? rows0 : Gf;
+  double *next = rows1;
+  int s = 1;
+  for (int b = b0; b < b1; b++) {
+    LOAD_G1(S.block_seg[b]);
+    for (int j = (b == b0 ? 1 : 0); j < S.block_n[b]; j++, s++) {
+      double *gs = Gf + (size_t)s * M;
+      if (active) {
+        strip_step<SX>(prev, first, JX, last, ym, y, yp, cnt, x0, lam, C1, C2, C3, g1, v);
+        STORE_V(gs);
+        if (lds) STORE_V(next);
+      }
+      __syncthreads();
+      if (lds) {
+        double *t = (double *)prev;
+        prev = next;
+        next = t;
+      } else {
+        prev = gs;
+      }
+    }
+  }
+  // GN: weighted sum of the last distribution (held in v)
+  {
+    double acc = 0;
+    if (active) {
+      const double *Lw = S.L + base;
+#pragma unroll
+      for (int k = 0; k < SX; k++)
+        acc += (cyl ? Lw[k * JX] : 1.0) * v[k];
+    }
+    const double gn = wg_sum(acc, red);
+    if (tid == 0) S.sc->GN[mo] = gn;
+  }
+  // ---- backward with composition; rho of the current block in registers
+  double *ga = lds ? rows0 : S.Gb, *gb = lds ? rows1 : S.Gb + M;
+  LOAD_G1(S.block_seg[b1 - 1]);
+#pragma unroll
+  for (int k = 0; k < SX; k++) {
+    v[k] = g1[k];
+    racc[k] = 0.0;
+  }
+  if (active) {
+    const double *gf = Gf + (size_t)(N - 1) * M + base;
+#pragma unroll
+    for (int k = 0; k < SX; k++)
+      racc[k] += gf[k * JX] * v[k];
+    STORE_V(ga);
+  }
+  __syncthreads();
+  s = N - 2;
+  for (int b = b1 - 1; b >= b0; b--) {
+    if (b != b1 - 1) {
+      LOAD_G1(S.block_seg[b]);
+#pragma unroll
+      for (int k = 0; k < SX; k++) racc[k] = 0.0;
+    }
+    for (int j = (b == b1 - 1 ? S.block_n[b] - 2 : S.block_n[b] - 1); j >= 0; j--, s--) {
+      if (active) {
+        strip_step<SX>(ga, first, JX, last, ym, y, yp, cnt, x0, lam, C1, C2, C3, g1, v);
+        const double *gf = Gf + (size_t)s * M + base;
+#pragma unroll
+        for (int k = 0; k < SX; k++) racc[k] += gf[k * JX] * v[k];
+        STORE_V(gb);
+      }
+      __syncthreads();
+      double *t = ga;
+      ga = gb;
+      gb = t;
+    }
+    if (active) {
+      double *rho = S.rho + S.block_mm[b] * M + base;
+#pragma unroll
+      for (int k = 0; k < SX; k++) rho[k * JX] += racc[k];
+    }
+  }
+#undef LOAD_G1
+#undef STORE_V
+}
+
 // Second-order Markov chain (1 gradient, three direction components).
-__device__ void chain_markov2(const DevSys &S, int mo, double *rows0, double *rows1, double *red) {
+__device__ __noinline__ void chain_markov2(const DevSys &S, int mo, double *rows0, double *rows1, double *red) {
   const int M = S.M, W = S.M;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
@@ -260,6 +448,7 @@ __device__ void chain_markov2(const DevSys &S, int mo, double *rows0, double *ro
 // One Classical_residual: fields x -> g, densities, norms.  Returns ||g||
 // (or max|g|) to every thread, or a negative value on a status error.
 __device__ double fused_residual(const DevSys &S, double *rows0, double *rows1, double *red) {
+  __shared__ double2 lam2d[kMaxLamX];
   const int M = S.M, W = n_work_sites(S);
   const int tid = threadIdx.x, nt = blockDim.x;
   DevScalars &sc = *S.sc;
@@ -273,6 +462,12 @@ __device__ double fused_residual(const DevSys &S, double *rows0, double *rows1, 
   for (int mo = 0; mo < S.nmol; mo++) {
     if (S.mol_markov[mo] == 2)
       chain_markov2(S, mo, rows0, rows1, red);
+    else if (S.gradients == 2 && (S.geometry != 0 || S.stencil_full) && S.MX + 2 <= kMaxLamX &&
+             S.MX % 8 == 0 && (S.MX / 8) * S.MY <= (int)blockDim.x)
+      chain2d_m1<8>(S, mo, rows0, rows1, red, lam2d);
+    else if (S.gradients == 2 && (S.geometry != 0 || S.stencil_full) && S.MX + 2 <= kMaxLamX &&
+             S.MX % 4 == 0 && (S.MX / 4) * S.MY <= (int)blockDim.x)
+      chain2d_m1<4>(S, mo, rows0, rows1, red, lam2d);
     else
       chain_markov1(S, mo, rows0, rows1, red);
     __syncthreads();
@@ -311,16 +506,21 @@ __device__ double fused_residual(const DevSys &S, double *rows0, double *rows1, 
 }
 
 // Thermodynamic summary at the state of the last residual.
-__device__ void fused_thermo(const DevSys &S, double *red) {
+__device__ __noinline__ void fused_thermo(const DevSys &S, double *red) {
   const int W = n_work_sites(S);
   const int tid = threadIdx.x, nt = blockDim.x;
   DevScalars &sc = *S.sc;
   double logc[kMaxMol];
-  for (int mo = 0; mo < S.nmol; mo++) {
-    const double N = S.mol_N[mo];
-    logc[mo] = log(N * sc.n[mo] / sc.GN[mo]) / N;
+#pragma unroll
+  for (int mo = 0; mo < kMaxMol; mo++) {
+    logc[mo] = 0.0;
+    if (mo < S.nmol) {
+      const double N = S.mol_N[mo];
+      logc[mo] = log(N * sc.n[mo] / sc.GN[mo]) / N;
+    }
   }
   double f = 0, gp = 0, th[kMaxMol];
+#pragma unroll
   for (int mo = 0; mo < kMaxMol; mo++) th[mo] = 0;
   for (int w = tid; w < W; w += nt) {
     int x, y, z;
@@ -331,15 +531,21 @@ __device__ void fused_thermo(const DevSys &S, double *red) {
     const double wt = site_weight(S, i);
     f += wt * F;
     gp += wt * GP;
-    for (int mo = 0; mo < S.nmol; mo++) th[mo] += wt * S.mol_phitot[mo * S.M + i];
+#pragma unroll
+    for (int mo = 0; mo < kMaxMol; mo++)
+      if (mo < S.nmol) th[mo] += wt * S.mol_phitot[mo * S.M + i];
   }
   f = wg_sum(f, red);
   gp = wg_sum(gp, red);
-  for (int mo = 0; mo < S.nmol; mo++) th[mo] = wg_sum(th[mo], red);
+#pragma unroll
+  for (int mo = 0; mo < kMaxMol; mo++)
+    if (mo < S.nmol) th[mo] = wg_sum(th[mo], red);
   if (tid == 0) {
     sc.F = f;
     sc.GP = gp;
-    for (int mo = 0; mo < S.nmol; mo++) sc.theta[mo] = th[mo];
+#pragma unroll
+    for (int mo = 0; mo < kMaxMol; mo++)
+      if (mo < S.nmol) sc.theta[mo] = th[mo];
     scalars_mu(S);
   }
   __syncthreads();
@@ -347,6 +553,8 @@ __device__ void fused_thermo(const DevSys &S, double *red) {
 
 // ------------------------------------------------------------- DIIS solve
 __device__ void fused_solve(const DevSys &S, double *rows0, double *rows1, double *red) {
+  __shared__ double diis_dots[kMaxDIIS];
+  __shared__ double diis_A[kMaxDIIS * kMaxDIIS];
   const int nvar = S.nit * S.M;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int m = S.m;
@@ -382,18 +590,21 @@ __device__ void fused_solve(const DevSys &S, double *rows0, double *rows1, doubl
     }
     __syncthreads();
     const int kd = k_diis > m ? m : k_diis;
-    double dots[kMaxDIIS];
     for (int i = 0; i < kd; i++) {
       int posi = k - kd + 1 + i;
       if (posi < 0) posi += m;
       const double *a = xx0 + (size_t)posi * nvar;
       double acc = 0;
       for (int v = tid; v < nvar; v += nt) acc += a[v] * xxk[v];
-      dots[i] = wg_sum(acc, red);
+      acc = wg_sum(acc, red);
+      if (tid == 0) diis_dots[i] = acc;
     }
-    if (tid == 0) {
-      int kdc;
-      if (!diis_coefficients(sc, m, k_diis, dots, kdc)) sc.status = DS_NAN;
+    __syncthreads();
+    if (tid == 0) diis_gram_update(sc, m, k_diis, diis_dots, diis_A);
+    __syncthreads();
+    if (tid < 64) {
+      const bool ok = wave_diis_coefficients(diis_A, kd, sc.C);
+      if (tid == 0 && !ok) sc.status = DS_NAN;
     }
     __syncthreads();
     if (sc.status != DS_OK) break;

@@ -185,11 +185,16 @@ __global__ void __launch_bounds__(kGridThreads) k_thermo_partial(const DevSys *_
   __shared__ double red[kGridThreads / 64];
   const DevScalars &sc = *S.sc;
   double logc[kMaxMol];
-  for (int mo = 0; mo < S.nmol; mo++) {
-    const double N = S.mol_N[mo];
-    logc[mo] = log(N * sc.n[mo] / sc.GN[mo]) / N;
+#pragma unroll
+  for (int mo = 0; mo < kMaxMol; mo++) {
+    logc[mo] = 0.0;
+    if (mo < S.nmol) {
+      const double N = S.mol_N[mo];
+      logc[mo] = log(N * sc.n[mo] / sc.GN[mo]) / N;
+    }
   }
   double f = 0, gp = 0, th[kMaxMol];
+#pragma unroll
   for (int mo = 0; mo < kMaxMol; mo++) th[mo] = 0;
   GRID_LOOP(S) {
     int x, y, z;
@@ -200,7 +205,9 @@ __global__ void __launch_bounds__(kGridThreads) k_thermo_partial(const DevSys *_
     const double wt = site_weight(S, i);
     f += wt * F;
     gp += wt * GP;
-    for (int mo = 0; mo < S.nmol; mo++) th[mo] += wt * S.mol_phitot[mo * S.M + i];
+#pragma unroll
+    for (int mo = 0; mo < kMaxMol; mo++)
+      if (mo < S.nmol) th[mo] += wt * S.mol_phitot[mo * S.M + i];
   }
   const int nc = 2 + S.nmol;
   f = block_sum(f, red);
@@ -209,9 +216,12 @@ __global__ void __launch_bounds__(kGridThreads) k_thermo_partial(const DevSys *_
     partial[blockIdx.x * nc + 0] = f;
     partial[blockIdx.x * nc + 1] = gp;
   }
-  for (int mo = 0; mo < S.nmol; mo++) {
-    const double t = block_sum(th[mo], red);
-    if (threadIdx.x == 0) partial[blockIdx.x * nc + 2 + mo] = t;
+#pragma unroll
+  for (int mo = 0; mo < kMaxMol; mo++) {
+    if (mo < S.nmol) {
+      const double t = block_sum(th[mo], red);
+      if (threadIdx.x == 0) partial[blockIdx.x * nc + 2 + mo] = t;
+    }
   }
 }
 
@@ -264,6 +274,7 @@ __global__ void __launch_bounds__(kGridThreads) k_diis_step(const DevSys *__rest
   const double dmax = S.deltamax;
   double *xRk = S.xR + (size_t)k * nvar, *xxk = S.xx0 + (size_t)k * nvar;
   double acc[kMaxDIIS];
+#pragma unroll
   for (int i = 0; i < kMaxDIIS; i++) acc[i] = 0;
   VAR_LOOP(S) {
     const double xv = S.x[v] + (-dmax) * S.g[v];
@@ -271,32 +282,46 @@ __global__ void __launch_bounds__(kGridThreads) k_diis_step(const DevSys *__rest
     xRk[v] = xv;
     const double e = xv - S.x0[v];
     xxk[v] = e;
-    for (int i = 0; i < kd; i++) {
-      int posi = k - kd + 1 + i;
-      if (posi < 0) posi += m;
-      const double a = posi == k ? e : S.xx0[(size_t)posi * nvar + v];
-      acc[i] += a * e;
+#pragma unroll
+    for (int i = 0; i < kMaxDIIS; i++) {
+      if (i < kd) {
+        int posi = k - kd + 1 + i;
+        if (posi < 0) posi += m;
+        const double a = posi == k ? e : S.xx0[(size_t)posi * nvar + v];
+        acc[i] += a * e;
+      }
     }
   }
-  for (int i = 0; i < kd; i++) {
-    const double t = block_sum(acc[i], red);
-    if (threadIdx.x == 0) partial[blockIdx.x * kMaxDIIS + i] = t;
+#pragma unroll
+  for (int i = 0; i < kMaxDIIS; i++) {
+    if (i < kd) {
+      const double t = block_sum(acc[i], red);
+      if (threadIdx.x == 0) partial[blockIdx.x * kMaxDIIS + i] = t;
+    }
   }
 }
 
 __global__ void k_diis_solve(const DevSys *__restrict__ d, const double *partial, int nparts, int k_diis, int kd) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ double dots[kMaxDIIS];
+  __shared__ double Ap[kMaxDIIS * kMaxDIIS];
+  __shared__ int st;
   const DevSys &S = *d;
   DevScalars &sc = *S.sc;
-  if (sc.status != DS_OK) return;
-  double dots[kMaxDIIS];
-  for (int i = 0; i < kd; i++) {
-    double t = 0;
-    for (int b = 0; b < nparts; b++) t += partial[b * kMaxDIIS + i];
-    dots[i] = t;
+  if (threadIdx.x == 0) {
+    st = sc.status;
+    if (st == DS_OK) {
+      for (int i = 0; i < kd; i++) {
+        double t = 0;
+        for (int b = 0; b < nparts; b++) t += partial[b * kMaxDIIS + i];
+        dots[i] = t;
+      }
+      diis_gram_update(sc, S.m, k_diis, dots, Ap);
+    }
   }
-  int kdc;
-  if (!diis_coefficients(sc, S.m, k_diis, dots, kdc)) sc.status = DS_NAN;
+  __syncthreads();
+  if (st != DS_OK) return;
+  const bool ok = wave_diis_coefficients(Ap, kd, sc.C);
+  if (threadIdx.x == 0 && !ok) sc.status = DS_NAN;
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_diis_combine(const DevSys *__restrict__ d, int k, int kd) {
