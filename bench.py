@@ -34,13 +34,17 @@ CHIS = (0.0, 0.5, 0.8)
 
 
 def propagator_bytes(desc, iterations, w=8):
-    """Algorithmic propagator bytes of one solve: per residual evaluation each
-    chain of length N streams 3w per site forward (read G(s-1), read G1,
-    write G(s)) and 6w backward with composition (read Gb, G1, Gf; write Gb;
-    read+write rho): 9 w N per site; residual evaluations = iterations + 1."""
+    """Algorithmic propagator bytes of one solve.  Per residual evaluation a
+    first-order (Markov 1) chain of length N streams 3w per site forward (read
+    G(s-1), read G1, write G(s)) and 6w backward with composition (read Gb,
+    G1, Gf; write Gb; read+write rho): 9 w N per site.  A second-order chain
+    with 3 direction components streams 7w forward and 12w backward: 19 w N.
+    Residual evaluations per solve = iterations + 1."""
     g = desc["gradients"]
     W = desc["MX"] * (desc["MY"] if g >= 2 else 1) * (desc["MZ"] if g == 3 else 1)
-    per_eval = sum(9 * w * m["chainlength"] * m["size"] * W for m in desc["molecules"])
+    if g == 1:
+        W = desc["M"]  # 1-gradient kernels update the ghost sites too
+    per_eval = sum((19 if m["markov"] == 2 else 9) * w * m["chainlength"] * W for m in desc["molecules"])
     return per_eval * (iterations + 1)
 
 

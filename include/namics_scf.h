@@ -57,6 +57,8 @@ typedef struct {
   int status;      /* SCF_OK, SCF_NOT_CONVERGED or a negative error */
   double residual; /* ||g||_2 (or max|g| with stop_criterion max_of_element_of_|g|) */
   double kernel_ms; /* GPU time of the launch(es) that solved this system (HIP events) */
+  double solve_ms;  /* this system's own solve time on the device (device wall clock;
+                       equals kernel_ms for the grid path) */
 } scf_solve_info;
 
 typedef struct {
@@ -121,6 +123,9 @@ int scf_solve_batch(scf_system *const *systems, int n, double *const *x, scf_sol
 /* Device-resident fields: copy host fields in / out of the system's arena. */
 int scf_upload_fields(scf_system *sys, const double *x);
 int scf_download_fields(const scf_system *sys, double *x);
+
+/* Set the device-resident fields to zero (the reference's default start). */
+int scf_zero_fields(scf_system *sys);
 
 /* Start `dst` from the fields of `src` (same lattice), block by segment
  * name, zero where `src` has no block of that name, device to device.

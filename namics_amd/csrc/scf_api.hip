@@ -64,6 +64,21 @@ size_t lds_limit() {
   return g_lds_limit;
 }
 
+double g_wall_khz = 0;
+
+// frequency of the device wall clock (wall_clock64) in kHz
+double wall_clock_khz() {
+  if (g_wall_khz == 0) {
+    int dev = 0, khz = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+      g_wall_khz = khz;
+    else
+      g_wall_khz = 100000.0;
+  }
+  return g_wall_khz;
+}
+
 }  // namespace
 
 struct scf_system {
@@ -470,6 +485,13 @@ int scf_download_fields(const scf_system *S, double *x) {
   return SCF_OK;
 }
 
+int scf_zero_fields(scf_system *S) {
+  if (!S) return fail(SCF_E_ARG, "null system");
+  const size_t nvar = (size_t)S->h.nit * S->h.M;
+  HIP_TRY(hipMemsetAsync(S->h.x, 0, nvar * sizeof(double), S->stream));
+  return SCF_OK;
+}
+
 int scf_carry_fields(scf_system *dst, const scf_system *src) {
   if (!dst || !src) return fail(SCF_E_ARG, "null argument");
   const int M = dst->h.M;
@@ -539,6 +561,7 @@ int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info) 
       float ms = 0.f;
       HIP_TRY(hipEventElapsedTime(&ms, S->ev0, S->ev1));
       info[i].kernel_ms = ms;
+      info[i].solve_ms = ms;
     } else {
       DevScalars sc;
       int rc = read_scalars(S, sc);
@@ -547,6 +570,7 @@ int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info) 
       info[i].residual = sc.residual;
       info[i].status = sc.status != DS_OK ? sc.status : (sc.converged ? SCF_OK : SCF_NOT_CONVERGED);
       info[i].kernel_ms = fused_ms;
+      info[i].solve_ms = (double)(sc.t_end - sc.t_begin) / wall_clock_khz();
     }
     S->have_results = info[i].status >= 0;
   }

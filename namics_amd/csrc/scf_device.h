@@ -41,6 +41,7 @@ struct DevScalars {
   double C[kMaxDIIS];
   double red[kMaxDIIS + 4];       // reduction results handed between phases
   int status, iterations, k_diis, converged;
+  unsigned long long t_begin, t_end;  // device wall clock at solve start / end
 };
 
 struct DevSys {

@@ -640,7 +640,10 @@ __global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode
     rows0 = lds;
     rows1 = lds + R;
   }
-  if (threadIdx.x == 0) S.sc->status = DS_OK;
+  if (threadIdx.x == 0) {
+    S.sc->status = DS_OK;
+    S.sc->t_begin = wall_clock64();
+  }
   __syncthreads();
   if (mode == FUSED_RESIDUAL) {
     const double r = fused_residual(S, rows0, rows1, red);
@@ -649,6 +652,7 @@ __global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode
     fused_solve(S, rows0, rows1, red);
     if (S.sc->status == DS_OK) fused_thermo(S, red);
   }
+  if (threadIdx.x == 0) S.sc->t_end = wall_clock64();
 }
 
 hipError_t launch_fused(DevSys *const *d_systems, int nsys, int mode, int threads, size_t lds_bytes,

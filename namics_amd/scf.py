@@ -32,7 +32,7 @@ class ScfError(RuntimeError):
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("residual", ctypes.c_double),
-                ("kernel_ms", ctypes.c_double)]
+                ("kernel_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
 
 
 class Results(ctypes.Structure):
@@ -75,6 +75,7 @@ def lib():
         L.scf_upload_fields.argtypes = [vp, _D]
         L.scf_download_fields.argtypes = [vp, _D]
         L.scf_carry_fields.argtypes = [vp, vp]
+        L.scf_zero_fields.argtypes = [vp]
         L.scf_solve_resident.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(SolveInfo)]
         L.scf_get_results.argtypes = [vp, ctypes.POINTER(Results)]
         L.scf_get_profiles.argtypes = [vp, _D, _D]
@@ -161,6 +162,9 @@ class ScfSystem:
         _check(lib().scf_download_fields(self.h, _p(x)))
         return x
 
+    def zero_fields(self):
+        _check(lib().scf_zero_fields(self.h))
+
     def carry_from(self, other):
         """Start from `other`'s fields (by segment name), device to device."""
         _check(lib().scf_carry_fields(self.h, other.h))
@@ -208,10 +212,13 @@ def solve_batch(systems, xs=None):
     return xs, [(i.iterations, i.residual, i.status) for i in infos]
 
 
-def solve_resident(systems):
-    """Solve from device-resident fields; returns [(iterations, residual, status, kernel_ms)]."""
+def solve_resident(systems, with_solve_ms=False):
+    """Solve from device-resident fields; returns [(iterations, residual, status, kernel_ms)]
+    (plus each system's own device solve time in ms with with_solve_ms)."""
     n = len(systems)
     hs = (ctypes.c_void_p * n)(*[s.h.value for s in systems])
     infos = (SolveInfo * n)()
     _check(lib().scf_solve_resident(hs, n, infos))
+    if with_solve_ms:
+        return [(i.iterations, i.residual, i.status, i.kernel_ms, i.solve_ms) for i in infos]
     return [(i.iterations, i.residual, i.status, i.kernel_ms) for i in infos]
