@@ -51,7 +51,7 @@ size_t g_lds_limit = 0;  // bytes of dynamic LDS one workgroup may use
 size_t lds_limit() {
   if (g_lds_limit == 0) {
     int dev = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
       g_lds_limit = p.sharedMemPerBlock;
@@ -70,7 +70,7 @@ double g_wall_khz = 0;
 double wall_clock_khz() {
   if (g_wall_khz == 0) {
     int dev = 0, khz = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
       g_wall_khz = khz;
     else
@@ -90,6 +90,8 @@ struct scf_system {
   size_t arena_bytes = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t graph_exec = nullptr;  // captured grid-path iteration
+  bool use_graph = true;
   int mode = SCF_MODE_AUTO;
   int nb = 0;            // workgroups of the grid path
   int m_alloc = 0;       // DIIS history the arena was sized for
@@ -322,35 +324,45 @@ int read_scalars(const scf_system *S, DevScalars &sc) {
   return SCF_OK;
 }
 
-// host-driven DIIS over the grid path (SFNewton::iterate_DIIS src/sfnewton.cpp:1037-1130)
+// host-driven DIIS over the grid path (SFNewton::iterate_DIIS
+// src/sfnewton.cpp:1037-1130).  The loop body (one iteration: residual, DIIS
+// step, combine) is captured once into a hipGraph and replayed; the host
+// reads back the scalars after each replay to test convergence.
 int grid_solve(scf_system *S, scf_solve_info *info) {
   const DevSys &h = S->h;
   hipStream_t st = S->stream;
   DevScalars sc;
-  auto residual = [&]() -> int {
-    HIP_TRY(grid_copy_x0(h, S->d, S->nb, st));
-    HIP_TRY(grid_residual(h, S->d, h.partial, S->nb, st));
-    return SCF_OK;
-  };
-  int rc = residual();
-  if (rc) return rc;
+  HIP_TRY(grid_init(S->d, st));
+  HIP_TRY(grid_copy_x0(h, S->d, S->nb, st));
+  HIP_TRY(grid_residual(h, S->d, h.partial, S->nb, st));
   HIP_TRY(hipMemcpyAsync(&sc, h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   int it = 0;
   double res = sc.residual;
   if (sc.status == DS_OK) {
     HIP_TRY(grid_axpy_first(h, S->d, S->nb, st));
-    int k_diis = 0;
+    if (!S->graph_exec && S->use_graph) {
+      hipGraph_t graph;
+      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      hipError_t ce = grid_iteration(h, S->d, h.partial, S->nb, st);
+      hipError_t ee = hipStreamEndCapture(st, &graph);
+      if (ce != hipSuccess) return hip_fail(ce, "grid_iteration capture");
+      if (ee != hipSuccess) return hip_fail(ee, "hipStreamEndCapture");
+      hipError_t ie = hipGraphInstantiate(&S->graph_exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        S->graph_exec = nullptr;
+        return hip_fail(ie, "hipGraphInstantiate");
+      }
+    }
     while (res > h.tol && it < h.itlim) {
-      it++;
-      rc = residual();
-      if (rc) return rc;
-      const int k = it % h.m;
-      k_diis++;
-      HIP_TRY(grid_diis_step(h, S->d, h.partial, S->nb, k, k_diis, st));
-      HIP_TRY(grid_diis_combine(h, S->d, S->nb, k, k_diis, st));
+      if (S->graph_exec)
+        HIP_TRY(hipGraphLaunch(S->graph_exec, st));
+      else
+        HIP_TRY(grid_iteration(h, S->d, h.partial, S->nb, st));
       HIP_TRY(hipMemcpyAsync(&sc, h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
+      it = sc.iterations;
       if (sc.status != DS_OK) break;
       res = sc.residual;
     }
@@ -408,6 +420,7 @@ int scf_system_create(const char *input_text, int start, scf_system **out, char 
 void scf_system_destroy(scf_system *S) {
   if (!S) return;
   if (S->stream) hipStreamSynchronize(S->stream);
+  if (S->graph_exec) hipGraphExecDestroy(S->graph_exec);
   if (S->arena) hipFree(S->arena);
   if (S->ev0) hipEventDestroy(S->ev0);
   if (S->ev1) hipEventDestroy(S->ev1);

@@ -345,11 +345,119 @@ __global__ void k_reset_status(const DevSys *__restrict__ d) {
   if (threadIdx.x == 0 && blockIdx.x == 0) d->sc->status = DS_OK;
 }
 
+__global__ void k_grid_init(const DevSys *__restrict__ d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    d->sc->status = DS_OK;
+    d->sc->iterations = 0;
+    d->sc->k_diis = 0;
+  }
+}
+
+// start of one DIIS iteration: iterations++, k_diis++ (SFNewton::iterate_DIIS)
+__global__ void k_iter_begin(const DevSys *__restrict__ d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && d->sc->status == DS_OK) {
+    d->sc->iterations++;
+    d->sc->k_diis++;
+  }
+}
+
+__device__ __forceinline__ void diis_indices(const DevSys &S, int &k, int &kd, int &k_diis) {
+  k = S.sc->iterations % S.m;
+  k_diis = S.sc->k_diis;
+  kd = k_diis > S.m ? S.m : k_diis;
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_diis_step_dev(const DevSys *__restrict__ d, double *partial) {
+  const DevSys &S = *d;
+  __shared__ double red[kGridThreads / 64];
+  if (S.sc->status != DS_OK) return;
+  int k, kd, k_diis;
+  diis_indices(S, k, kd, k_diis);
+  const int nvar = S.nit * S.M, m = S.m;
+  const double dmax = S.deltamax;
+  double *xRk = S.xR + (size_t)k * nvar, *xxk = S.xx0 + (size_t)k * nvar;
+  double acc[kMaxDIIS];
+#pragma unroll
+  for (int i = 0; i < kMaxDIIS; i++) acc[i] = 0;
+  VAR_LOOP(S) {
+    const double xv = S.x[v] + (-dmax) * S.g[v];
+    S.x[v] = xv;
+    xRk[v] = xv;
+    const double e = xv - S.x0[v];
+    xxk[v] = e;
+#pragma unroll
+    for (int i = 0; i < kMaxDIIS; i++) {
+      if (i < kd) {
+        int posi = k - kd + 1 + i;
+        if (posi < 0) posi += m;
+        const double a = posi == k ? e : S.xx0[(size_t)posi * nvar + v];
+        acc[i] += a * e;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxDIIS; i++) {
+    if (i < kd) {
+      const double t = block_sum(acc[i], red);
+      if (threadIdx.x == 0) partial[blockIdx.x * kMaxDIIS + i] = t;
+    }
+  }
+}
+
+__global__ void k_diis_solve_dev(const DevSys *__restrict__ d, const double *partial, int nparts) {
+  __shared__ double dots[kMaxDIIS];
+  __shared__ double Ap[kMaxDIIS * kMaxDIIS];
+  __shared__ int st, kd_s;
+  const DevSys &S = *d;
+  DevScalars &sc = *S.sc;
+  if (threadIdx.x == 0) {
+    st = sc.status;
+    int k, kd, k_diis;
+    diis_indices(S, k, kd, k_diis);
+    kd_s = kd;
+    if (st == DS_OK) {
+      for (int i = 0; i < kd; i++) {
+        double t = 0;
+        for (int b = 0; b < nparts; b++) t += partial[b * kMaxDIIS + i];
+        dots[i] = t;
+      }
+      diis_gram_update(sc, S.m, k_diis, dots, Ap);
+    }
+  }
+  __syncthreads();
+  if (st != DS_OK) return;
+  const bool ok = wave_diis_coefficients(Ap, kd_s, sc.C);
+  if (threadIdx.x == 0 && !ok) sc.status = DS_NAN;
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_diis_combine_dev(const DevSys *__restrict__ d) {
+  const DevSys &S = *d;
+  if (S.sc->status != DS_OK) return;
+  int k, kd, k_diis;
+  diis_indices(S, k, kd, k_diis);
+  const int nvar = S.nit * S.M, m = S.m;
+  VAR_LOOP(S) {
+    int posi = k - kd + 1;
+    if (posi < 0) posi += m;
+    double xv = 0.0 + S.sc->C[0] * S.xR[(size_t)posi * nvar + v];
+    for (int i = 1; i < kd; i++) {
+      posi = k - kd + 1 + i;
+      if (posi < 0) posi += m;
+      xv += S.sc->C[i] * S.xR[(size_t)posi * nvar + v];
+    }
+    S.x[v] = xv;
+  }
+}
+
 }  // namespace
 
 hipError_t grid_residual(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
-  const dim3 B(kGridThreads), G(nb);
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, st, d);
+  return grid_residual_body(h, d, partial, nb, st);
+}
+
+hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
+  const dim3 B(kGridThreads), G(nb);
   hipLaunchKernelGGL(k_prepare, G, B, 0, st, d);
   for (int mo = 0; mo < h.nmol; mo++) {
     if (h.mol_markov[mo] != 1) return hipErrorInvalidValue;
@@ -381,6 +489,26 @@ hipError_t grid_thermo(const DevSys &h, DevSys *d, double *partial, int nb, hipS
   (void)h;
   hipLaunchKernelGGL(k_thermo_partial, dim3(nb), dim3(kGridThreads), 0, st, d, partial);
   hipLaunchKernelGGL(k_finalize_thermo, dim3(1), dim3(64), 0, st, d, partial, nb);
+  return hipGetLastError();
+}
+
+hipError_t grid_init(DevSys *d, hipStream_t st) {
+  hipLaunchKernelGGL(k_grid_init, dim3(1), dim3(1), 0, st, d);
+  return hipGetLastError();
+}
+
+// One DIIS iteration of the grid path with every index on the device, so the
+// launch sequence is identical from one iteration to the next and can be
+// captured once into a hipGraph.
+hipError_t grid_iteration(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
+  hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(1), 0, st, d);
+  hipLaunchKernelGGL(k_copy_x0, dim3(nb), dim3(kGridThreads), 0, st, d);
+  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, st, d);
+  hipError_t e = grid_residual_body(h, d, partial, nb, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_diis_step_dev, dim3(nb), dim3(kGridThreads), 0, st, d, partial);
+  hipLaunchKernelGGL(k_diis_solve_dev, dim3(1), dim3(64), 0, st, d, partial, nb);
+  hipLaunchKernelGGL(k_diis_combine_dev, dim3(nb), dim3(kGridThreads), 0, st, d);
   return hipGetLastError();
 }
 

@@ -23,6 +23,10 @@ hipError_t grid_diis_step(const DevSys &h, DevSys *d, double *d_partial, int nbl
 hipError_t grid_diis_combine(const DevSys &h, DevSys *d, int nblocks, int k, int kd, hipStream_t stream);
 hipError_t grid_axpy_first(const DevSys &h, DevSys *d, int nblocks, hipStream_t stream);
 hipError_t grid_copy_x0(const DevSys &h, DevSys *d, int nblocks, hipStream_t stream);
+hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *d_partial, int nblocks, hipStream_t stream);
+hipError_t grid_init(DevSys *d, hipStream_t stream);
+// One DIIS iteration with all indices on the device (capturable in a hipGraph).
+hipError_t grid_iteration(const DevSys &h, DevSys *d, double *d_partial, int nblocks, hipStream_t stream);
 
 // Plain chain propagation for unit tests: out[s*M+i] = G_s(i) for a field
 // whose Boltzmann weight is G1 (Lattice::propagate repeated N-1 times).
