@@ -48,6 +48,14 @@ __device__ __forceinline__ double wg_max(double v, double *red) {
   return t;
 }
 
+// Work is distributed over the system's own thread count (S.threads), not
+// the launch's: in a batch launched with more threads, the extra threads
+// take no sites and add exact zeros to the reductions, so a system does the
+// same arithmetic in the same order as when it is launched alone.
+__device__ __forceinline__ int work_tid(const DevSys &S) {
+  return (int)threadIdx.x < S.threads ? (int)threadIdx.x : (1 << 30);
+}
+
 // ------------------------------------------------------------- propagation
 constexpr int kStripX = 8;    // sites per thread along x in the 2-gradient step
 constexpr int kMaxLamX = 1026;  // radial layers + 2 held in the LDS weight table
@@ -62,7 +70,7 @@ struct BlockWalk {
 
 __device__ __noinline__ void chain_markov1(const DevSys &S, int mo, double *rows0, double *rows1, double *red) {
   const int M = S.M, W = n_work_sites(S);
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tid = work_tid(S), nt = S.threads;
   const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
   const int N = S.mol_N[mo];
   double *Gf = S.Gf;
@@ -340,7 +348,7 @@ __device__ __noinline__ void chain2d_m1(const DevSys &S, int mo, double *rows0, 
 // Second-order Markov chain (1 gradient, three direction components).
 __device__ __noinline__ void chain_markov2(const DevSys &S, int mo, double *rows0, double *rows1, double *red) {
   const int M = S.M, W = S.M;
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tid = work_tid(S), nt = S.threads;
   const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
   const int N = S.mol_N[mo];
   const int R = 3 * M;
@@ -450,7 +458,7 @@ __device__ __noinline__ void chain_markov2(const DevSys &S, int mo, double *rows
 __device__ double fused_residual(const DevSys &S, double *rows0, double *rows1, double *red) {
   __shared__ double2 lam2d[kMaxLamX];
   const int M = S.M, W = n_work_sites(S);
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tid = work_tid(S), nt = S.threads;
   DevScalars &sc = *S.sc;
   for (int w = tid; w < W; w += nt) {
     int x, y, z;
@@ -463,10 +471,10 @@ __device__ double fused_residual(const DevSys &S, double *rows0, double *rows1, 
     if (S.mol_markov[mo] == 2)
       chain_markov2(S, mo, rows0, rows1, red);
     else if (S.gradients == 2 && (S.geometry != 0 || S.stencil_full) && S.MX + 2 <= kMaxLamX &&
-             S.MX % 8 == 0 && (S.MX / 8) * S.MY <= (int)blockDim.x)
+             S.MX % 8 == 0 && (S.MX / 8) * S.MY <= S.threads)
       chain2d_m1<8>(S, mo, rows0, rows1, red, lam2d);
     else if (S.gradients == 2 && (S.geometry != 0 || S.stencil_full) && S.MX + 2 <= kMaxLamX &&
-             S.MX % 4 == 0 && (S.MX / 4) * S.MY <= (int)blockDim.x)
+             S.MX % 4 == 0 && (S.MX / 4) * S.MY <= S.threads)
       chain2d_m1<4>(S, mo, rows0, rows1, red, lam2d);
     else
       chain_markov1(S, mo, rows0, rows1, red);
@@ -508,7 +516,7 @@ __device__ double fused_residual(const DevSys &S, double *rows0, double *rows1, 
 // Thermodynamic summary at the state of the last residual.
 __device__ __noinline__ void fused_thermo(const DevSys &S, double *red) {
   const int W = n_work_sites(S);
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tid = work_tid(S), nt = S.threads;
   DevScalars &sc = *S.sc;
   double logc[kMaxMol];
 #pragma unroll
@@ -556,7 +564,7 @@ __device__ void fused_solve(const DevSys &S, double *rows0, double *rows1, doubl
   __shared__ double diis_dots[kMaxDIIS];
   __shared__ double diis_A[kMaxDIIS * kMaxDIIS];
   const int nvar = S.nit * S.M;
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tid = work_tid(S), nt = S.threads;
   const int m = S.m;
   const double dmax = S.deltamax;
   DevScalars &sc = *S.sc;
