@@ -56,7 +56,8 @@ size_t lds_limit() {
     if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
       g_lds_limit = p.sharedMemPerBlock;
       // leave room for the kernel's static shared memory
-      if (g_lds_limit > 1024) g_lds_limit -= 1024;
+      const size_t st = fused_static_lds() + 256;
+      g_lds_limit = g_lds_limit > st ? g_lds_limit - st : 0;
     } else {
       g_lds_limit = 64 * 1024 - 1024;
     }
@@ -302,11 +303,33 @@ int build(scf_system *S) {
   h.sc = (DevScalars *)(base + o_sc);
   h.partial = P(o_part);
   // execution choice for the fused path
-  const size_t rows = (size_t)2 * maxR * M * D;
-  h.use_lds = rows <= lds_limit() ? 1 : 0;
-  S->lds_bytes = h.use_lds ? rows : 0;
   int work = l.gradients == 1 ? M : W;
   h.threads = work <= 64 ? 64 : (work <= 128 ? 128 : (work <= 256 ? 256 : (work <= 512 ? 512 : 1024)));
+  h.strip_sx = 0;
+  h.strip_stride = 0;
+  size_t rows = (size_t)2 * maxR * M * D;
+  h.row_stride = maxR * M;
+  // 2-gradient strip path: every thread owns SX sites along x; needs
+  // Markov-1 chains, the 9-point stencil, ghosts that map onto interior
+  // sites, and the rows of one strip layout in LDS
+  const bool strip_ok = l.gradients == 2 && maxR == 1 && (l.geometry != 0 || l.stencil_full) &&
+                        l.MX + 2 <= 1026 && h.bx1 >= 1 && h.bx1 <= l.MX && h.bxm >= 1 && h.bxm <= l.MX &&
+                        h.by1 >= 1 && h.by1 <= l.MY && h.bym >= 1 && h.bym <= l.MY;
+  for (int sx : {8, 4}) {
+    if (!strip_ok || h.strip_sx || l.MX % sx) continue;
+    const int nact = (l.MX / sx) * l.MY;
+    if (nact > 1024) continue;
+    const int stride = nact <= 256 ? 256 : (nact <= 512 ? 512 : 1024);
+    const size_t srows = (size_t)2 * sx * stride * D;
+    if (srows > lds_limit()) continue;
+    h.strip_sx = sx;
+    h.strip_stride = stride;
+    h.row_stride = sx * stride;
+    rows = srows;
+    h.threads = std::max(h.threads, stride);
+  }
+  h.use_lds = rows <= lds_limit() ? 1 : 0;
+  S->lds_bytes = h.use_lds ? rows : 0;
   S->d = (DevSys *)(base + o_dev);
   S->d_list = (DevSys **)(base + o_list);
   HIP_TRY(hipMemcpy(S->d, &h, sizeof(DevSys), hipMemcpyHostToDevice));

@@ -89,6 +89,9 @@ struct DevSys {
   double deltamax, tol;
   // ---- execution
   int use_lds;            // propagator rows fit the LDS budget of the launch
+  int row_stride;         // doubles between the two LDS row buffers
+  int strip_sx;           // 2-gradient strip path: sites per thread along x (0: not used)
+  int strip_stride;       // 2-gradient strip path: LDS slots per strip column
   int threads;            // preferred workgroup size for the fused kernel
   // ---- scalars and reduction workspace
   DevScalars *sc;
@@ -96,24 +99,29 @@ struct DevSys {
 };
 
 // ------------------------------------------------------------ index helpers
-__device__ __forceinline__ int map_x(const DevSys &S, int x) {
+template <typename Sys>
+__device__ __forceinline__ int map_x(const Sys &S, int x) {
   return x < 1 ? S.bx1 : (x > S.MX ? S.bxm : x);
 }
-__device__ __forceinline__ int map_y(const DevSys &S, int y) {
+template <typename Sys>
+__device__ __forceinline__ int map_y(const Sys &S, int y) {
   return y < 1 ? S.by1 : (y > S.MY ? S.bym : y);
 }
-__device__ __forceinline__ int map_z(const DevSys &S, int z) {
+template <typename Sys>
+__device__ __forceinline__ int map_z(const Sys &S, int z) {
   return z < 1 ? S.bz1 : (z > S.MZ ? S.bzm : z);
 }
 
 // number of sites a phase visits: all sites in 1 gradient (ghosts included,
 // for surface boundaries), interior sites otherwise
-__device__ __forceinline__ int n_work_sites(const DevSys &S) {
+template <typename Sys>
+__device__ __forceinline__ int n_work_sites(const Sys &S) {
   return S.gradients == 1 ? S.M : S.MX * S.MY * (S.gradients == 3 ? S.MZ : 1);
 }
 
 // work index -> lattice site (and its coordinates)
-__device__ __forceinline__ int work_site(const DevSys &S, int w, int &x, int &y, int &z) {
+template <typename Sys>
+__device__ __forceinline__ int work_site(const Sys &S, int w, int &x, int &y, int &z) {
   if (S.gradients == 1) {
     x = w;
     y = z = 0;
@@ -133,7 +141,8 @@ __device__ __forceinline__ int work_site(const DevSys &S, int w, int &x, int &y,
   return x * S.JX + y * S.JY + z;
 }
 
-__device__ __forceinline__ bool is_interior(const DevSys &S, int x, int y, int z) {
+template <typename Sys>
+__device__ __forceinline__ bool is_interior(const Sys &S, int x, int y, int z) {
   if (x < 1 || x > S.MX) return false;
   if (S.gradients >= 2 && (y < 1 || y > S.MY)) return false;
   if (S.gradients == 3 && (z < 1 || z > S.MZ)) return false;
@@ -147,9 +156,8 @@ __device__ __forceinline__ bool is_interior(const DevSys &S, int x, int y, int z
 // LGrad1::propagate src/LGrad1.cpp:441-445, LGrad2::propagate
 // src/LGrad2.cpp:1058-1073, LG2Planar::propagate src/LG2Planar.cpp:1085-1096,
 // LGrad3::propagate src/LGrad3.cpp:733-743).
-template <typename T>
-__device__ __forceinline__ T stencil_prop(const DevSys &S, const T *__restrict__ g, int i, int x, int y,
-                                          int z) {
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int y, int z) {
   if (S.gradients == 1) {
     const int M = S.M;
     if (S.geometry == 0) {
@@ -240,8 +248,8 @@ __device__ __forceinline__ double stencil_side(const DevSys &S, const double *__
 // Markov-2 row at position x.  The reference bounds components 0 and 2 as a
 // pair, so a mirror reflection turns an outgoing walker into an incoming one
 // (LGrad1::set_bounds(X,Y) src/LGrad1.cpp:787-804).
-template <typename T>
-__device__ __forceinline__ T m2_val(const DevSys &S, const T *__restrict__ row, int c, int x) {
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ T m2_val(const Sys &S, Ptr row, int c, int x) {
   const int M = S.M;
   if (x >= 1 && x <= S.MX) return row[c * M + x];
   const bool lo = x < 1;
@@ -256,9 +264,8 @@ __device__ __forceinline__ T m2_val(const DevSys &S, const T *__restrict__ row, 
 // direction components (without the Boltzmann factor) to out[0..2].
 // Follows LG1Planar::propagateF/B (src/LG1Planar.cpp:57-136) and
 // LGrad1::propagateF/B (src/LGrad1.cpp:285-429), simple-cubic lattice.
-template <typename T>
-__device__ __forceinline__ void m2_step(const DevSys &S, const T *__restrict__ g, const double *P, bool forward,
-                                        int x, T out[3]) {
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ void m2_step(const Sys &S, Ptr g, const double *P, bool forward, int x, T out[3]) {
   const int M = S.M;
   const T P0 = (T)P[0], P1 = (T)P[1], P1x4 = (T)(4 * P[1]), Pm = (T)(2 * P[1] + P[0]);
   const bool hasm = x >= 1, hasp = x <= M - 2;
@@ -266,62 +273,62 @@ __device__ __forceinline__ void m2_step(const DevSys &S, const T *__restrict__ g
   if (S.geometry == 0) {
     if (forward) {
       if (hasm) {
-        a = P0 * m2_val(S, g, 0, x - 1);
-        a += P1x4 * m2_val(S, g, 1, x - 1);
+        a = P0 * m2_val<T>(S, g, 0, x - 1);
+        a += P1x4 * m2_val<T>(S, g, 1, x - 1);
       }
-      b = P1 * m2_val(S, g, 0, x);
-      b += Pm * m2_val(S, g, 1, x);
-      b += P1 * m2_val(S, g, 2, x);
+      b = P1 * m2_val<T>(S, g, 0, x);
+      b += Pm * m2_val<T>(S, g, 1, x);
+      b += P1 * m2_val<T>(S, g, 2, x);
       if (hasp) {
-        c = P1x4 * m2_val(S, g, 1, x + 1);
-        c += P0 * m2_val(S, g, 2, x + 1);
+        c = P1x4 * m2_val<T>(S, g, 1, x + 1);
+        c += P0 * m2_val<T>(S, g, 2, x + 1);
       }
     } else {
-      if (hasp) a = P0 * m2_val(S, g, 0, x + 1);
-      a += P1x4 * m2_val(S, g, 1, x);
-      if (hasp) b = P1 * m2_val(S, g, 0, x + 1);
-      b += Pm * m2_val(S, g, 1, x);
-      if (hasm) b += P1 * m2_val(S, g, 2, x - 1);
-      c = P1x4 * m2_val(S, g, 1, x);
-      if (hasm) c += P0 * m2_val(S, g, 2, x - 1);
+      if (hasp) a = P0 * m2_val<T>(S, g, 0, x + 1);
+      a += P1x4 * m2_val<T>(S, g, 1, x);
+      if (hasp) b = P1 * m2_val<T>(S, g, 0, x + 1);
+      b += Pm * m2_val<T>(S, g, 1, x);
+      if (hasm) b += P1 * m2_val<T>(S, g, 2, x - 1);
+      c = P1x4 * m2_val<T>(S, g, 1, x);
+      if (hasm) c += P0 * m2_val<T>(S, g, 2, x - 1);
     }
   } else {
     const T l1 = (T)S.l1[x], l11 = (T)S.l11[x], l_1 = (T)S.l_1[x], l_11 = (T)S.l_11[x];
     if (forward) {
       if (hasm) {
-        T h = l_1 * m2_val(S, g, 0, x - 1);
-        h += l_11 * m2_val(S, g, 2, x);
+        T h = l_1 * m2_val<T>(S, g, 0, x - 1);
+        h += l_11 * m2_val<T>(S, g, 2, x);
         a = P0 * h;
-        h = l_1 * m2_val(S, g, 1, x - 1);
-        h += l_11 * m2_val(S, g, 1, x);
+        h = l_1 * m2_val<T>(S, g, 1, x - 1);
+        h += l_11 * m2_val<T>(S, g, 1, x);
         a += P1x4 * h;
       }
-      b = P1 * m2_val(S, g, 0, x);
-      b += Pm * m2_val(S, g, 1, x);
-      b += P1 * m2_val(S, g, 2, x);
+      b = P1 * m2_val<T>(S, g, 0, x);
+      b += Pm * m2_val<T>(S, g, 1, x);
+      b += P1 * m2_val<T>(S, g, 2, x);
       if (hasp) {
-        T h = l1 * m2_val(S, g, 1, x + 1);
-        h += l11 * m2_val(S, g, 1, x);
+        T h = l1 * m2_val<T>(S, g, 1, x + 1);
+        h += l11 * m2_val<T>(S, g, 1, x);
         c = P1x4 * h;
-        h = l1 * m2_val(S, g, 2, x + 1);
-        h += l11 * m2_val(S, g, 0, x);
+        h = l1 * m2_val<T>(S, g, 2, x + 1);
+        h += l11 * m2_val<T>(S, g, 0, x);
         c += P0 * h;
       }
     } else {
       T hm = 0, hp = 0;
       if (hasm) {
-        hm = l_1 * m2_val(S, g, 2, x - 1);
-        hm += l_11 * m2_val(S, g, 0, x);
+        hm = l_1 * m2_val<T>(S, g, 2, x - 1);
+        hm += l_11 * m2_val<T>(S, g, 0, x);
       }
       if (hasp) {
-        hp = l1 * m2_val(S, g, 0, x + 1);
-        hp += l11 * m2_val(S, g, 2, x);
+        hp = l1 * m2_val<T>(S, g, 0, x + 1);
+        hp += l11 * m2_val<T>(S, g, 2, x);
       }
       b = P1 * hm;
       c = P0 * hm;
-      a = P1x4 * m2_val(S, g, 1, x);
-      b += Pm * m2_val(S, g, 1, x);
-      c += P1x4 * m2_val(S, g, 1, x);
+      a = P1x4 * m2_val<T>(S, g, 1, x);
+      b += Pm * m2_val<T>(S, g, 1, x);
+      c += P1x4 * m2_val<T>(S, g, 1, x);
       a += P0 * hp;
       b += P1 * hp;
     }

@@ -4,11 +4,12 @@
 //
 // Why one workgroup per system: the propagator recursion over the chain
 // contour s is strictly sequential, and for 1- and 2-gradient lattices
-// (M <= a few 10^4 sites) one step is ~1 us of work for one CU; splitting a
-// system over CUs would pay a grid-wide barrier per step.  Keeping the
-// previous end-point distribution in LDS and synchronising with s_barrier
-// makes a step a few hundred cycles, and many independent systems (the
-// batched and served workloads) fill the GPU one workgroup each.
+// (M <= a few 10^4 sites) one step is well under a microsecond of work for
+// one CU; splitting a system over CUs would pay a cross-CU synchronisation
+// per step.  Keeping the previous end-point distribution in LDS and
+// synchronising with s_barrier makes a step a few hundred cycles, and many
+// independent systems (the batched and served workloads) fill the GPU one
+// workgroup each.
 //
 // Reference path replaced per system (one call = one Solve_scf::Solve):
 //   SFNewton::iterate_DIIS        src/sfnewton.cpp:1037-1130
@@ -22,88 +23,274 @@
 
 namespace nscf {
 
+// ------------------------------------------------------------- memory spaces
+// Pointers are cast to their address space so that the compiler emits
+// global_* (HBM) and ds_* (LDS) instructions rather than flat ones: a flat
+// access counts against both the vector-memory and the LDS wait counters,
+// so every wait for an LDS read would also wait for HBM traffic in flight.
+typedef __attribute__((address_space(1))) double gdbl;
+typedef __attribute__((address_space(3))) double ldbl;
+// The system descriptor is read-only during a launch and the same for the
+// whole workgroup; read through the constant address space it is fetched
+// with scalar loads.
+typedef const __attribute__((address_space(4))) DevSys CSys;
+
+__device__ __forceinline__ CSys &uniform_sys(const DevSys &S) {
+  const unsigned long long p = (unsigned long long)&S;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)p);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(p >> 32));
+  return *(CSys *)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ gdbl *hbm(double *p) { return (gdbl *)p; }
+__device__ __forceinline__ const gdbl *hbm(const double *p) { return (const gdbl *)p; }
+__device__ __forceinline__ ldbl *lds_row(int offset) {
+  extern __shared__ double lds_dyn[];
+  return (ldbl *)(lds_dyn + offset);
+}
+
+// Barrier that orders LDS only.  HBM stores issued before it may still be in
+// flight: inside the step loops they are read back only by the thread that
+// wrote them, and every chain ends with a full __syncthreads().
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Propagator rows in LDS (kLds) or, when they do not fit, in HBM.
+template <bool kLds>
+struct Rows;
+template <>
+struct Rows<true> {
+  typedef ldbl T;
+  static __device__ __forceinline__ void barrier() { lds_barrier(); }
+};
+template <>
+struct Rows<false> {
+  typedef gdbl T;
+  static __device__ __forceinline__ void barrier() { __syncthreads(); }
+};
+
+constexpr int kMaxLamX = 1026;  // radial layers + 2 held in the LDS weight table
+
+__shared__ double s_red[16];                                // one value per wave
+__shared__ double s_lam[2 * kMaxLamX];                      // (lambda-, lambda+) per x
+__shared__ double s_diis[kMaxDIIS + kMaxDIIS * kMaxDIIS];  // dots, Gram matrix
+
+// Work is distributed over the system's own thread count (S.threads), not
+// the launch's: in a batch launched with more threads, the extra threads
+// take no sites and add exact zeros to the reductions, so a system does the
+// same arithmetic in the same order as when it is launched alone.
+template <typename Sys>
+__device__ __forceinline__ int work_tid(const Sys &S) {
+  return (int)threadIdx.x < S.threads ? (int)threadIdx.x : (1 << 30);
+}
+
 // ------------------------------------------------------------- reductions
 // Workgroup sum; every thread gets the same value (summed in a fixed order).
-__device__ __forceinline__ double wg_sum(double v, double *red) {
+__device__ __forceinline__ double wg_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  ldbl *red = (ldbl *)s_red;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
+  lds_barrier();
   if (lane == 0) red[wave] = v;
-  __syncthreads();
+  lds_barrier();
   double t = 0;
   const int nw = blockDim.x >> 6;
   for (int w = 0; w < nw; w++) t += red[w];
   return t;
 }
 
-__device__ __forceinline__ double wg_max(double v, double *red) {
+__device__ __forceinline__ double wg_max(double v) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  ldbl *red = (ldbl *)s_red;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
+  lds_barrier();
   if (lane == 0) red[wave] = v;
-  __syncthreads();
+  lds_barrier();
   double t = 0;
   const int nw = blockDim.x >> 6;
   for (int w = 0; w < nw; w++) t = fmax(t, red[w]);
   return t;
 }
 
-// Work is distributed over the system's own thread count (S.threads), not
-// the launch's: in a batch launched with more threads, the extra threads
-// take no sites and add exact zeros to the reductions, so a system does the
-// same arithmetic in the same order as when it is launched alone.
-__device__ __forceinline__ int work_tid(const DevSys &S) {
-  return (int)threadIdx.x < S.threads ? (int)threadIdx.x : (1 << 30);
+// ------------------------------------------------------------- propagation
+// Forward recursion of one linear chain, then the backward recursion with
+// the composition law fused into it: rho_block += Gf(s) * Gb(s)
+// (Molecule::propagate_forward/backward src/molecule.cpp:2300-2446,
+// mol_linear::ComputePhi src/mol_linear.cpp:131-149).  Gf keeps all N rows in
+// HBM; the row being read and the row being written live in LDS when they
+// fit.
+
+// 1-gradient Markov-1 chain, one site per thread (M <= threads): the
+// neighbour indices and lattice weights of the site stay in registers for
+// the whole chain, and the density of a block is accumulated in a register.
+// LG1Planar::propagate (src/LG1Planar.cpp:146-149) and LGrad1::propagate
+// (src/LGrad1.cpp:441-445); a neighbour that falls on a ghost reads the site
+// set_bounds copies there, a missing neighbour (the stencil at the outermost
+// sites) has weight zero.
+__device__ __noinline__ void chain1d_m1(const DevSys &S0, int mo) {
+  CSys &S = uniform_sys(S0);
+  const int M = S.M;
+  const int i = threadIdx.x;
+  const bool active = i < M;
+  const int ii = active ? i : 0;
+  const int i0 = map_x(S, ii);
+  const int im = ii >= 1 ? map_x(S, ii - 1) : i0;
+  const int ip = ii <= M - 2 ? map_x(S, ii + 1) : i0;
+  const bool planar = S.geometry == 0;
+  double wm, w0, wp;
+  if (planar) {
+    wm = S.lambda;
+    w0 = 1.0 - 2.0 * S.lambda;
+    wp = S.lambda;
+  } else {
+    wm = hbm(S.lam_m)[ii];
+    w0 = hbm(S.lam_0)[ii];
+    wp = hbm(S.lam_p)[ii];
+  }
+  if (ii < 1) wm = 0.0;
+  if (ii > M - 2) wp = 0.0;
+  const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
+  const int N = S.mol_N[mo];
+  gdbl *Gf = hbm(S.Gf);
+  const gdbl *G1 = hbm(S.G1);
+  ldbl *ra = lds_row(0), *rb = lds_row(S.row_stride);
+
+  double g1 = active ? G1[(size_t)S.block_seg[b0] * M + ii] : 0.0;
+  double v = g1;
+  if (active) {
+    Gf[ii] = v;
+    ra[ii] = v;
+  }
+  lds_barrier();
+  ldbl *prev = ra, *next = rb;
+  int s = 1;
+  for (int b = b0; b < b1; b++) {
+    if (b != b0 && active) g1 = G1[(size_t)S.block_seg[b] * M + ii];
+    const int n = S.block_n[b] - (b == b0 ? 1 : 0);
+    for (int j = 0; j < n; j++, s++) {
+      if (active) {
+        double t;
+        if (planar) {
+          t = wm * prev[im];
+          t += w0 * prev[i0];
+        } else {
+          t = w0 * prev[i0];
+          t += wm * prev[im];
+        }
+        t += wp * prev[ip];
+        v = t * g1;
+        Gf[(size_t)s * M + ii] = v;
+        next[ii] = v;
+      }
+      lds_barrier();
+      ldbl *tmp = prev;
+      prev = next;
+      next = tmp;
+    }
+  }
+  // GN = weighted sum of the last end-point distribution (Lattice::ComputeGN)
+  {
+    double acc = 0.0;
+    if (active && ii >= 1 && ii <= S.MX) acc = (planar ? 1.0 : hbm(S.L)[ii]) * v;
+    const double gn = wg_sum(acc);
+    if (threadIdx.x == 0) S0.sc->GN[mo] = gn;
+  }
+  // backward with composition, Gf(s-1) fetched one step ahead
+  ldbl *ga = ra, *gb = rb;
+  const int bl = b1 - 1;
+  g1 = active ? G1[(size_t)S.block_seg[bl] * M + ii] : 0.0;
+  v = g1;
+  double racc = 0.0, gf = 0.0;
+  if (active) {
+    racc = Gf[(size_t)(N - 1) * M + ii] * v;
+    ga[ii] = v;
+    if (N >= 2) gf = Gf[(size_t)(N - 2) * M + ii];
+  }
+  lds_barrier();
+  s = N - 2;
+  for (int b = bl; b >= b0; b--) {
+    if (b != bl) {
+      if (active) g1 = G1[(size_t)S.block_seg[b] * M + ii];
+      racc = 0.0;
+    }
+    const int n = S.block_n[b] - (b == bl ? 1 : 0);
+    for (int j = 0; j < n; j++, s--) {
+      if (active) {
+        double t;
+        if (planar) {
+          t = wm * ga[im];
+          t += w0 * ga[i0];
+        } else {
+          t = w0 * ga[i0];
+          t += wm * ga[im];
+        }
+        t += wp * ga[ip];
+        v = t * g1;
+        racc += gf * v;
+        gb[ii] = v;
+        if (s >= 1) gf = Gf[(size_t)(s - 1) * M + ii];
+      }
+      lds_barrier();
+      ldbl *tmp = ga;
+      ga = gb;
+      gb = tmp;
+    }
+    if (active) hbm(S.rho)[(size_t)S.block_mm[b] * M + ii] += racc;
+  }
+  __syncthreads();
 }
 
-// ------------------------------------------------------------- propagation
-constexpr int kStripX = 8;    // sites per thread along x in the 2-gradient step
-constexpr int kMaxLamX = 1026;  // radial layers + 2 held in the LDS weight table
-
-// Forward and backward recursion of one linear chain with the composition
-// law fused into the backward step.  rows0/rows1 are the two LDS row
-// buffers (size*M each) or null, in which case the rows are read from the
-// end-point store in HBM (Gf) and from the Gb ping-pong.
-struct BlockWalk {
-  int b, j;  // current block and position inside it
-};
-
-__device__ __noinline__ void chain_markov1(const DevSys &S, int mo, double *rows0, double *rows1, double *red) {
+// Markov-1 chain on any lattice, sites strided over the threads, rows in LDS
+// (kLds) or HBM.
+template <bool kLds>
+__device__ __noinline__ void chain_markov1(const DevSys &S0, int mo) {
+  typedef typename Rows<kLds>::T RowT;
+  CSys &S = uniform_sys(S0);
   const int M = S.M, W = n_work_sites(S);
   const int tid = work_tid(S), nt = S.threads;
   const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
   const int N = S.mol_N[mo];
-  double *Gf = S.Gf;
-  const bool lds = rows0 != nullptr;
+  gdbl *Gf = hbm(S.Gf);
+  const gdbl *G1 = hbm(S.G1);
+  RowT *ra, *rb;
+  if constexpr (kLds) {
+    ra = lds_row(0);
+    rb = lds_row(S.row_stride);
+  } else {
+    ra = Gf;
+    rb = Gf;
+  }
   // s = 0: G(.,1) = G1 (Lattice::Initiate)
   {
-    const double *g1 = S.G1 + S.block_seg[b0] * M;
+    const gdbl *g1 = G1 + (size_t)S.block_seg[b0] * M;
     for (int w = tid; w < W; w += nt) {
       int x, y, z;
       const int i = work_site(S, w, x, y, z);
       const double v = g1[i];
       Gf[i] = v;
-      if (lds) rows0[i] = v;
+      if constexpr (kLds) ra[i] = v;
     }
   }
-  __syncthreads();
-  const double *prev = lds ? rows0 : Gf;
-  double *next = rows1;
+  Rows<kLds>::barrier();
+  const RowT *prev = ra;
+  RowT *next = rb;
   int s = 1;
   for (int b = b0; b < b1; b++) {
-    const double *g1 = S.G1 + S.block_seg[b] * M;
+    const gdbl *g1 = G1 + (size_t)S.block_seg[b] * M;
     for (int j = (b == b0 ? 1 : 0); j < S.block_n[b]; j++, s++) {
-      double *gs = Gf + (size_t)s * M;
+      gdbl *gs = Gf + (size_t)s * M;
       for (int w = tid; w < W; w += nt) {
         int x, y, z;
         const int i = work_site(S, w, x, y, z);
         const double v = stencil_prop<double>(S, prev, i, x, y, z) * g1[i];
         gs[i] = v;
-        if (lds) next[i] = v;
+        if constexpr (kLds) next[i] = v;
       }
-      __syncthreads();
-      if (lds) {
-        double *t = (double *)prev;
+      Rows<kLds>::barrier();
+      if constexpr (kLds) {
+        RowT *t = (RowT *)prev;
         prev = next;
         next = t;
       } else {
@@ -111,25 +298,30 @@ __device__ __noinline__ void chain_markov1(const DevSys &S, int mo, double *rows
       }
     }
   }
-  // GN = weighted sum of the last end-point distribution (Lattice::ComputeGN)
   {
-    const double *gl = Gf + (size_t)(N - 1) * M;
+    const gdbl *gl = Gf + (size_t)(N - 1) * M;
     double acc = 0;
     for (int w = tid; w < W; w += nt) {
       int x, y, z;
       const int i = work_site(S, w, x, y, z);
-      if (is_interior(S, x, y, z)) acc += site_weight(S, i) * gl[i];
+      if (is_interior(S, x, y, z)) acc += site_weight(S0, i) * gl[i];
     }
-    const double gn = wg_sum(acc, red);
-    if (tid == 0) S.sc->GN[mo] = gn;
+    const double gn = wg_sum(acc);
+    if (threadIdx.x == 0) S0.sc->GN[mo] = gn;
   }
-  // backward with composition: rho_mm += Gf(s) * Gb(s)
-  double *gbA = lds ? rows0 : S.Gb, *gbB = lds ? rows1 : S.Gb + M;
+  RowT *gbA, *gbB;
+  if constexpr (kLds) {
+    gbA = ra;
+    gbB = rb;
+  } else {
+    gbA = hbm(S.Gb);
+    gbB = hbm(S.Gb) + M;
+  }
   {
     const int bl = b1 - 1;
-    const double *g1 = S.G1 + S.block_seg[bl] * M;
-    double *rho = S.rho + S.block_mm[bl] * M;
-    const double *gf = Gf + (size_t)(N - 1) * M;
+    const gdbl *g1 = G1 + (size_t)S.block_seg[bl] * M;
+    gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[bl] * M;
+    const gdbl *gf = Gf + (size_t)(N - 1) * M;
     for (int w = tid; w < W; w += nt) {
       int x, y, z;
       const int i = work_site(S, w, x, y, z);
@@ -138,13 +330,13 @@ __device__ __noinline__ void chain_markov1(const DevSys &S, int mo, double *rows
       rho[i] += gf[i] * v;
     }
   }
-  __syncthreads();
+  Rows<kLds>::barrier();
   s = N - 2;
   for (int b = b1 - 1; b >= b0; b--) {
-    const double *g1 = S.G1 + S.block_seg[b] * M;
-    double *rho = S.rho + S.block_mm[b] * M;
+    const gdbl *g1 = G1 + (size_t)S.block_seg[b] * M;
+    gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[b] * M;
     for (int j = (b == b1 - 1 ? S.block_n[b] - 2 : S.block_n[b] - 1); j >= 0; j--, s--) {
-      const double *gf = Gf + (size_t)s * M;
+      const gdbl *gf = Gf + (size_t)s * M;
       for (int w = tid; w < W; w += nt) {
         int x, y, z;
         const int i = work_site(S, w, x, y, z);
@@ -152,51 +344,46 @@ __device__ __noinline__ void chain_markov1(const DevSys &S, int mo, double *rows
         gbB[i] = v;
         rho[i] += gf[i] * v;
       }
-      __syncthreads();
-      double *t = gbA;
+      Rows<kLds>::barrier();
+      RowT *t = gbA;
       gbA = gbB;
       gbB = t;
     }
   }
+  __syncthreads();
 }
 
-// 2-gradient Markov-1 chain, register-blocked.  Thread t owns the strip of
-// SX sites (x0..x0+SX-1, y) with y = t % MY + 1: lanes of a wavefront walk the
-// contiguous y axis, so every LDS row read is unit-stride across lanes.  A
-// step reads the three rows y-1, y, y+1 over x0-1..x0+SX (3(SX+2) LDS reads
-// for SX sites instead of 9 per site), multiplies by the Boltzmann weights
-// held in registers for the current block, writes the end-point
-// distribution to HBM and the next LDS row.  The density of the current
-// block accumulates in registers and is flushed at block boundaries.
+// 2-gradient Markov-1 chain, register-blocked.  Thread t < nact owns the
+// strip of SX sites (x0 .. x0+SX-1, y) with x0 = (t / MY) SX + 1 and
+// y = t % MY + 1, and keeps its own previous values in registers.  The rows
+// live in LDS in a strip layout: slot(k, t) = k STRIDE + t holds site
+// (x0+k, y) of thread t, so the y-1 / y+1 neighbours of a column are the
+// adjacent slots (unit stride across the lanes of a wave, constant offsets
+// for the unrolled columns) and ghost sites are never stored.  Gf rows in HBM
+// use the same layout with the dense column stride nact.
 // The 9-point weights are those of LGrad2::propagate (src/LGrad2.cpp:1058-1073);
 // the planar full stencil (src/LG2Planar.cpp:1085-1096) is the same form with
 // lambda- = lambda+ = 1/6.
-// One lattice step of a strip from row `src`: v = W(src) * g1 (rolling
-// 3x3 window over x, rows y-1, y, y+1).
-// Keeps a value opaque to the optimiser so address arithmetic derived from
-// it is recomputed inside the step loop instead of being hoisted into (and
-// spilling) registers.
-__device__ __forceinline__ int opaque(int v) {
-  asm volatile("" : "+v"(v));
-  return v;
-}
-
-template <int SX>
-__device__ __forceinline__ void strip_step(const double *__restrict__ src, int first, int JX, int last, int ym,
-                                           int y, int yp, int cnt, int x0, const double2 *lam, double C1,
-                                           double C2, double C3, const double (&g1)[SX], double (&v)[SX]) {
-  // columns: first (mapped x0-1), x0 .. x0+SX-1 at stride JX, last (mapped x0+cnt)
-  const int dm = opaque(ym - y), dp = opaque(yp - y);
-  const double *c0p = src + first + y;
-  double am = c0p[dm], bm = c0p[0], cm = c0p[dp];
-  const double *cp_ = src + x0 * JX + y;
-  double a0 = cp_[dm], b0 = cp_[0], c0 = cp_[dp];
+template <int SX, int STRIDE>
+__device__ __forceinline__ void strip_step(const ldbl *__restrict__ p, int am0, int ap0, int sl, int sr, int dm,
+                                           int dp, const ldbl *__restrict__ lamx, double C1, double C2, double C3,
+                                           const double (&g1)[SX], double (&v)[SX]) {
+  // columns x0-1 (slot sl), x0 .. x0+SX-1 (own), x0+SX (slot sr); rows y-1 (a), y (b), y+1 (c)
+  double am = p[sl + dm], bm = p[sl], cm = p[sl + dp];
+  double a0 = p[am0], b0 = v[0], c0 = p[ap0];
 #pragma unroll
-  for (int k = 1; k <= SX; k++) {
-    const double *q = (k < SX) ? cp_ + k * JX : src + last + y;
-    const double ap = q[dm], bp = q[0], cp = q[dp];
-    const double2 lw = lam[opaque(x0 + k - 1)];
-    const double lmk = lw.x, lpk = lw.y;
+  for (int k = 0; k < SX; k++) {
+    double ap, bp, cp;
+    if (k + 1 < SX) {
+      ap = p[am0 + (k + 1) * STRIDE];
+      bp = v[k + 1];
+      cp = p[ap0 + (k + 1) * STRIDE];
+    } else {
+      ap = p[sr + dm];
+      bp = p[sr];
+      cp = p[sr + dp];
+    }
+    const double lmk = lamx[2 * k], lpk = lamx[2 * k + 1];
     double t = C1 * b0;
     t += lmk * bm;
     t += lpk * bp;
@@ -208,7 +395,7 @@ __device__ __forceinline__ void strip_step(const double *__restrict__ src, int f
     t += lpk * ap;
     t += lpk * cp;
     t *= C2;
-    v[k - 1] = t * g1[k - 1];
+    v[k] = t * g1[k];
     am = a0;
     a0 = ap;
     bm = b0;
@@ -218,161 +405,191 @@ __device__ __forceinline__ void strip_step(const double *__restrict__ src, int f
   }
 }
 
-template <int SX>
-__device__ __noinline__ void chain2d_m1(const DevSys &S, int mo, double *rows0, double *rows1, double *red,
-                                        double2 *lam) {
+template <int SX, int STRIDE>
+__device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo) {
+  CSys &S = uniform_sys(S0);
   const int MX = S.MX, MY = S.MY, JX = S.JX;
-  const int tid = threadIdx.x;
-  const int nsx = (MX + SX - 1) / SX;
-  const bool active = tid < nsx * MY;
-  const int y = active ? tid % MY + 1 : 1;
-  const int x0 = active ? (tid / MY) * SX + 1 : 1;
-  const int cnt = SX;  // the caller guarantees MX % SX == 0
-  const int ym = map_y(S, y - 1), yp = map_y(S, y + 1);
+  const int nact = (MX / SX) * MY;
+  const int t = threadIdx.x;
+  const bool active = t < nact;
+  const int xs = active ? t / MY : 0;
+  const int y = active ? t - xs * MY + 1 : 1;
+  const int x0 = xs * SX + 1;
+  // neighbours after set_bounds: y +- 1 as slot offsets inside the strip,
+  // x0-1 and x0+SX as slots of the strips that own them
+  const int dm = map_y(S, y - 1) - y, dp = map_y(S, y + 1) - y;
+  const int xl = map_x(S, x0 - 1) - 1, xr = map_x(S, x0 + SX) - 1;
+  const int sl = (xl % SX) * STRIDE + (xl / SX) * MY + (y - 1);
+  const int sr = (xr % SX) * STRIDE + (xr / SX) * MY + (y - 1);
+  const int tt = active ? t : 0;
+  const int am0 = tt + dm, ap0 = tt + dp;
   const bool cyl = S.geometry != 0;
   const bool sc = S.lattice_type == 0;
   const double C1 = sc ? 4.0 / 6.0 : 0.5, C2 = sc ? 1.0 / 6.0 : 0.25, C3 = sc ? 4.0 : 2.0;
-  const int first = map_x(S, x0 - 1) * JX, last = map_x(S, x0 + cnt) * JX;
   const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
   const int N = S.mol_N[mo];
   const size_t M = S.M;
-  const bool lds = rows0 != nullptr;
-  double *Gf = S.Gf;
-  // radial weights per x (wave-uniform reads from LDS): (lambda-, lambda+)
-  for (int x = tid; x < MX + 2; x += blockDim.x)
-    lam[x] = cyl ? make_double2(S.lam_m[x * JX + 1], S.lam_p[x * JX + 1]) : make_double2(1.0 / 6.0, 1.0 / 6.0);
-  __syncthreads();
-  const int base = x0 * JX + y;  // site of k = 0; site k is base + k*JX
-  double g1[SX], v[SX], racc[SX];
+  const size_t RG = (size_t)SX * nact;  // Gf row length (interior sites)
+  gdbl *Gf = hbm(S.Gf) + tt;
+  const gdbl *G1 = hbm(S.G1);
+  ldbl *ra = lds_row(0), *rb = lds_row(SX * STRIDE);
+  ldbl *lam = (ldbl *)s_lam;
+  for (int x = threadIdx.x; x < MX + 2; x += blockDim.x) {
+    lam[2 * x] = cyl ? hbm(S.lam_m)[x * JX + 1] : 1.0 / 6.0;
+    lam[2 * x + 1] = cyl ? hbm(S.lam_p)[x * JX + 1] : 1.0 / 6.0;
+  }
+  lds_barrier();
+  const ldbl *lamx = lam + 2 * x0;
+  const size_t site0 = (size_t)x0 * JX + y;  // site of column 0; column k is site0 + k JX
+  double g1[SX], v[SX], racc[SX], gfv[SX];
 
-#define LOAD_G1(seg)                                              \
-  {                                                               \
-    const double *p_ = S.G1 + (seg) * M + base;                   \
+#define LOAD_G1(seg)                                                   \
+  {                                                                    \
+    const gdbl *p_ = G1 + (size_t)(seg) * M + site0;                   \
     _Pragma("unroll") for (int k = 0; k < SX; k++) g1[k] = p_[k * JX]; \
   }
-#define STORE_V(row)                                              \
-  {                                                               \
-    double *r_ = (row) + opaque(base);                            \
-    _Pragma("unroll") for (int k = 0; k < SX; k++) r_[k * JX] = v[k]; \
+#define STORE_ROW(row)                                                            \
+  {                                                                               \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) (row)[k * STRIDE + tt] = v[k]; \
+  }
+#define STORE_GF(s_)                                                            \
+  {                                                                             \
+    gdbl *q_ = Gf + (size_t)(s_) * RG;                                          \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) q_[(size_t)k * nact] = v[k]; \
+  }
+#define LOAD_GF(s_)                                                               \
+  {                                                                               \
+    const gdbl *q_ = Gf + (size_t)(s_) * RG;                                      \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) gfv[k] = q_[(size_t)k * nact]; \
   }
 
   // ---- forward
-  LOAD_G1(S.block_seg[b0]);
+  if (active) LOAD_G1(S.block_seg[b0]);
 #pragma unroll
-  for (int k = 0; k < SX; k++) v[k] = g1[k];
+  for (int k = 0; k < SX; k++) v[k] = active ? g1[k] : 0.0;
   if (active) {
-    STORE_V(Gf);
-    if (lds) STORE_V(rows0);
+    STORE_GF(0);
+    STORE_ROW(ra);
   }
-  __syncthreads();
-  const double *prev = lds ? rows0 : Gf;
-  double *next = rows1;
+  lds_barrier();
+  ldbl *prev = ra, *next = rb;
   int s = 1;
   for (int b = b0; b < b1; b++) {
-    LOAD_G1(S.block_seg[b]);
-    for (int j = (b == b0 ? 1 : 0); j < S.block_n[b]; j++, s++) {
-      double *gs = Gf + (size_t)s * M;
+    if (b != b0 && active) LOAD_G1(S.block_seg[b]);
+    const int n = S.block_n[b] - (b == b0 ? 1 : 0);
+    for (int j = 0; j < n; j++, s++) {
       if (active) {
-        strip_step<SX>(prev, first, JX, last, ym, y, yp, cnt, x0, lam, C1, C2, C3, g1, v);
-        STORE_V(gs);
-        if (lds) STORE_V(next);
+        strip_step<SX, STRIDE>(prev, am0, ap0, sl, sr, dm, dp, lamx, C1, C2, C3, g1, v);
+        STORE_GF(s);
+        STORE_ROW(next);
       }
-      __syncthreads();
-      if (lds) {
-        double *t = (double *)prev;
-        prev = next;
-        next = t;
-      } else {
-        prev = gs;
-      }
+      lds_barrier();
+      ldbl *tmp = prev;
+      prev = next;
+      next = tmp;
     }
   }
   // GN: weighted sum of the last distribution (held in v)
   {
     double acc = 0;
     if (active) {
-      const double *Lw = S.L + base;
+      const gdbl *Lw = hbm(S.L) + site0;
 #pragma unroll
-      for (int k = 0; k < SX; k++)
-        acc += (cyl ? Lw[k * JX] : 1.0) * v[k];
+      for (int k = 0; k < SX; k++) acc += (cyl ? Lw[k * JX] : 1.0) * v[k];
     }
-    const double gn = wg_sum(acc, red);
-    if (tid == 0) S.sc->GN[mo] = gn;
+    const double gn = wg_sum(acc);
+    if (threadIdx.x == 0) S0.sc->GN[mo] = gn;
   }
-  // ---- backward with composition; rho of the current block in registers
-  double *ga = lds ? rows0 : S.Gb, *gb = lds ? rows1 : S.Gb + M;
-  LOAD_G1(S.block_seg[b1 - 1]);
-#pragma unroll
-  for (int k = 0; k < SX; k++) {
-    v[k] = g1[k];
-    racc[k] = 0.0;
-  }
+  // ---- backward with composition; the density of the current block in
+  // registers, Gf(s-1) fetched one step ahead
+  ldbl *ga = ra, *gb = rb;
+  const int bl = b1 - 1;
   if (active) {
-    const double *gf = Gf + (size_t)(N - 1) * M + base;
+    LOAD_G1(S.block_seg[bl]);
+    LOAD_GF(N - 1);
 #pragma unroll
-    for (int k = 0; k < SX; k++)
-      racc[k] += gf[k * JX] * v[k];
-    STORE_V(ga);
+    for (int k = 0; k < SX; k++) {
+      v[k] = g1[k];
+      racc[k] = gfv[k] * v[k];
+    }
+    STORE_ROW(ga);
+    if (N >= 2) LOAD_GF(N - 2);
   }
-  __syncthreads();
+  lds_barrier();
   s = N - 2;
-  for (int b = b1 - 1; b >= b0; b--) {
-    if (b != b1 - 1) {
-      LOAD_G1(S.block_seg[b]);
+  for (int b = bl; b >= b0; b--) {
+    if (b != bl) {
+      if (active) LOAD_G1(S.block_seg[b]);
 #pragma unroll
       for (int k = 0; k < SX; k++) racc[k] = 0.0;
     }
-    for (int j = (b == b1 - 1 ? S.block_n[b] - 2 : S.block_n[b] - 1); j >= 0; j--, s--) {
+    const int n = S.block_n[b] - (b == bl ? 1 : 0);
+    for (int j = 0; j < n; j++, s--) {
       if (active) {
-        strip_step<SX>(ga, first, JX, last, ym, y, yp, cnt, x0, lam, C1, C2, C3, g1, v);
-        const double *gf = Gf + (size_t)s * M + base;
+        strip_step<SX, STRIDE>(ga, am0, ap0, sl, sr, dm, dp, lamx, C1, C2, C3, g1, v);
 #pragma unroll
-        for (int k = 0; k < SX; k++) racc[k] += gf[k * JX] * v[k];
-        STORE_V(gb);
+        for (int k = 0; k < SX; k++) racc[k] += gfv[k] * v[k];
+        STORE_ROW(gb);
+        if (s >= 1) LOAD_GF(s - 1);
       }
-      __syncthreads();
-      double *t = ga;
+      lds_barrier();
+      ldbl *tmp = ga;
       ga = gb;
-      gb = t;
+      gb = tmp;
     }
     if (active) {
-      double *rho = S.rho + S.block_mm[b] * M + base;
+      gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[b] * M + site0;
 #pragma unroll
       for (int k = 0; k < SX; k++) rho[k * JX] += racc[k];
     }
   }
 #undef LOAD_G1
-#undef STORE_V
+#undef STORE_ROW
+#undef STORE_GF
+#undef LOAD_GF
+  __syncthreads();
 }
 
-// Second-order Markov chain (1 gradient, three direction components).
-__device__ __noinline__ void chain_markov2(const DevSys &S, int mo, double *rows0, double *rows1, double *red) {
+// Second-order Markov chain (1 gradient, three direction components), rows
+// of 3M in LDS (kLds) or HBM.
+template <bool kLds>
+__device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
+  typedef typename Rows<kLds>::T RowT;
+  CSys &S = uniform_sys(S0);
   const int M = S.M, W = S.M;
   const int tid = work_tid(S), nt = S.threads;
   const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
   const int N = S.mol_N[mo];
   const int R = 3 * M;
-  const double *P = S.mol_P[mo];
-  double *Gf = S.Gf;
-  const bool lds = rows0 != nullptr;
+  const double P[3] = {S.mol_P[mo][0], S.mol_P[mo][1], S.mol_P[mo][2]};
+  gdbl *Gf = hbm(S.Gf);
+  const gdbl *G1 = hbm(S.G1);
+  RowT *ra, *rb;
+  if constexpr (kLds) {
+    ra = lds_row(0);
+    rb = lds_row(S.row_stride);
+  } else {
+    ra = Gf;
+    rb = Gf;
+  }
   {
-    const double *g1 = S.G1 + S.block_seg[b0] * M;
+    const gdbl *g1 = G1 + (size_t)S.block_seg[b0] * M;
     for (int i = tid; i < W; i += nt) {
       const double v = g1[i];
       for (int c = 0; c < 3; c++) {
         Gf[c * M + i] = v;
-        if (lds) rows0[c * M + i] = v;
+        if constexpr (kLds) ra[c * M + i] = v;
       }
     }
   }
-  __syncthreads();
-  const double *prev = lds ? rows0 : Gf;
-  double *next = rows1;
+  Rows<kLds>::barrier();
+  const RowT *prev = ra;
+  RowT *next = rb;
   int s = 1;
   for (int b = b0; b < b1; b++) {
-    const double *g1 = S.G1 + S.block_seg[b] * M;
+    const gdbl *g1 = G1 + (size_t)S.block_seg[b] * M;
     for (int j = (b == b0 ? 1 : 0); j < S.block_n[b]; j++, s++) {
-      double *gs = Gf + (size_t)s * R;
+      gdbl *gs = Gf + (size_t)s * R;
       for (int i = tid; i < W; i += nt) {
         double o[3];
         m2_step<double>(S, prev, P, true, i, o);
@@ -380,12 +597,12 @@ __device__ __noinline__ void chain_markov2(const DevSys &S, int mo, double *rows
         for (int c = 0; c < 3; c++) {
           const double v = o[c] * w;
           gs[c * M + i] = v;
-          if (lds) next[c * M + i] = v;
+          if constexpr (kLds) next[c * M + i] = v;
         }
       }
-      __syncthreads();
-      if (lds) {
-        double *t = (double *)prev;
+      Rows<kLds>::barrier();
+      if constexpr (kLds) {
+        RowT *t = (RowT *)prev;
         prev = next;
         next = t;
       } else {
@@ -394,27 +611,34 @@ __device__ __noinline__ void chain_markov2(const DevSys &S, int mo, double *rows
     }
   }
   {
-    const double *gl = Gf + (size_t)(N - 1) * R;
+    const gdbl *gl = Gf + (size_t)(N - 1) * R;
     double a0 = 0, a1 = 0, a2 = 0;
     for (int i = tid; i < W; i += nt) {
       if (i >= 1 && i <= S.MX) {
-        const double wt = site_weight(S, i);
+        const double wt = site_weight(S0, i);
         a0 += wt * gl[i];
         a1 += wt * gl[M + i];
         a2 += wt * gl[2 * M + i];
       }
     }
-    double gn = wg_sum(a0, red);
-    gn += 4.0 * wg_sum(a1, red);
-    gn += wg_sum(a2, red);
-    if (tid == 0) S.sc->GN[mo] = gn / 6.0;
+    double gn = wg_sum(a0);
+    gn += 4.0 * wg_sum(a1);
+    gn += wg_sum(a2);
+    if (threadIdx.x == 0) S0.sc->GN[mo] = gn / 6.0;
   }
-  double *gbA = lds ? rows0 : S.Gb, *gbB = lds ? rows1 : S.Gb + R;
+  RowT *gbA, *gbB;
+  if constexpr (kLds) {
+    gbA = ra;
+    gbB = rb;
+  } else {
+    gbA = hbm(S.Gb);
+    gbB = hbm(S.Gb) + R;
+  }
   {
     const int bl = b1 - 1;
-    const double *g1 = S.G1 + S.block_seg[bl] * M;
-    double *rho = S.rho + S.block_mm[bl] * M;
-    const double *gf = Gf + (size_t)(N - 1) * R;
+    const gdbl *g1 = G1 + (size_t)S.block_seg[bl] * M;
+    gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[bl] * M;
+    const gdbl *gf = Gf + (size_t)(N - 1) * R;
     for (int i = tid; i < W; i += nt) {
       const double v = g1[i];
       double r = rho[i];
@@ -425,13 +649,13 @@ __device__ __noinline__ void chain_markov2(const DevSys &S, int mo, double *rows
       rho[i] = r;
     }
   }
-  __syncthreads();
+  Rows<kLds>::barrier();
   s = N - 2;
   for (int b = b1 - 1; b >= b0; b--) {
-    const double *g1 = S.G1 + S.block_seg[b] * M;
-    double *rho = S.rho + S.block_mm[b] * M;
+    const gdbl *g1 = G1 + (size_t)S.block_seg[b] * M;
+    gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[b] * M;
     for (int j = (b == b1 - 1 ? S.block_n[b] - 2 : S.block_n[b] - 1); j >= 0; j--, s--) {
-      const double *gf = Gf + (size_t)s * R;
+      const gdbl *gf = Gf + (size_t)s * R;
       for (int i = tid; i < W; i += nt) {
         double o[3];
         m2_step<double>(S, gbA, P, false, i, o);
@@ -444,19 +668,38 @@ __device__ __noinline__ void chain_markov2(const DevSys &S, int mo, double *rows
         }
         rho[i] = r;
       }
-      __syncthreads();
-      double *t = gbA;
+      Rows<kLds>::barrier();
+      RowT *t = gbA;
       gbA = gbB;
       gbB = t;
     }
+  }
+  __syncthreads();
+}
+
+__device__ __noinline__ void chain_strip_dispatch(const DevSys &S, int mo) {
+  const int sx = S.strip_sx, st = S.strip_stride;
+  if (sx == 8) {
+    if (st == 256)
+      chain2d_strip<8, 256>(S, mo);
+    else if (st == 512)
+      chain2d_strip<8, 512>(S, mo);
+    else
+      chain2d_strip<8, 1024>(S, mo);
+  } else {
+    if (st == 256)
+      chain2d_strip<4, 256>(S, mo);
+    else if (st == 512)
+      chain2d_strip<4, 512>(S, mo);
+    else
+      chain2d_strip<4, 1024>(S, mo);
   }
 }
 
 // ------------------------------------------------------------- residual
 // One Classical_residual: fields x -> g, densities, norms.  Returns ||g||
 // (or max|g|) to every thread, or a negative value on a status error.
-__device__ double fused_residual(const DevSys &S, double *rows0, double *rows1, double *red) {
-  __shared__ double2 lam2d[kMaxLamX];
+__device__ double fused_residual(const DevSys &S) {
   const int M = S.M, W = n_work_sites(S);
   const int tid = work_tid(S), nt = S.threads;
   DevScalars &sc = *S.sc;
@@ -468,19 +711,22 @@ __device__ double fused_residual(const DevSys &S, double *rows0, double *rows1, 
   }
   __syncthreads();
   for (int mo = 0; mo < S.nmol; mo++) {
-    if (S.mol_markov[mo] == 2)
-      chain_markov2(S, mo, rows0, rows1, red);
-    else if (S.gradients == 2 && (S.geometry != 0 || S.stencil_full) && S.MX + 2 <= kMaxLamX &&
-             S.MX % 8 == 0 && (S.MX / 8) * S.MY <= S.threads)
-      chain2d_m1<8>(S, mo, rows0, rows1, red, lam2d);
-    else if (S.gradients == 2 && (S.geometry != 0 || S.stencil_full) && S.MX + 2 <= kMaxLamX &&
-             S.MX % 4 == 0 && (S.MX / 4) * S.MY <= S.threads)
-      chain2d_m1<4>(S, mo, rows0, rows1, red, lam2d);
-    else
-      chain_markov1(S, mo, rows0, rows1, red);
-    __syncthreads();
+    if (S.mol_markov[mo] == 2) {
+      if (S.use_lds)
+        chain_markov2<true>(S, mo);
+      else
+        chain_markov2<false>(S, mo);
+    } else if (S.strip_sx > 0) {
+      chain_strip_dispatch(S, mo);
+    } else if (S.gradients == 1 && S.use_lds && S.M <= S.threads) {
+      chain1d_m1(S, mo);
+    } else if (S.use_lds) {
+      chain_markov1<true>(S, mo);
+    } else {
+      chain_markov1<false>(S, mo);
+    }
   }
-  if (tid == 0) {
+  if (threadIdx.x == 0) {
     for (int i = 0; i < S.nmol; i++) {
       if (S.mol_freedom[i] == 0) sc.phibulk[i] = S.mol_phibulk_in[i];
       if (S.mol_freedom[i] == 1) sc.n[i] = S.mol_n_in[i];
@@ -508,13 +754,14 @@ __device__ double fused_residual(const DevSys &S, double *rows0, double *rows1, 
       acc += site_residual(S, i, x, y, z);
     }
   }
-  const double r = S.max_g ? wg_max(acc, red) : sqrt(wg_sum(acc, red));
-  if (tid == 0) sc.residual = r;
+  const double r = S.max_g ? wg_max(acc) : sqrt(wg_sum(acc));
+  if (threadIdx.x == 0) sc.residual = r;
+  __syncthreads();
   return r;
 }
 
 // Thermodynamic summary at the state of the last residual.
-__device__ __noinline__ void fused_thermo(const DevSys &S, double *red) {
+__device__ __noinline__ void fused_thermo(const DevSys &S) {
   const int W = n_work_sites(S);
   const int tid = work_tid(S), nt = S.threads;
   DevScalars &sc = *S.sc;
@@ -543,12 +790,12 @@ __device__ __noinline__ void fused_thermo(const DevSys &S, double *red) {
     for (int mo = 0; mo < kMaxMol; mo++)
       if (mo < S.nmol) th[mo] += wt * S.mol_phitot[mo * S.M + i];
   }
-  f = wg_sum(f, red);
-  gp = wg_sum(gp, red);
+  f = wg_sum(f);
+  gp = wg_sum(gp);
 #pragma unroll
   for (int mo = 0; mo < kMaxMol; mo++)
-    if (mo < S.nmol) th[mo] = wg_sum(th[mo], red);
-  if (tid == 0) {
+    if (mo < S.nmol) th[mo] = wg_sum(th[mo]);
+  if (threadIdx.x == 0) {
     sc.F = f;
     sc.GP = gp;
 #pragma unroll
@@ -560,18 +807,21 @@ __device__ __noinline__ void fused_thermo(const DevSys &S, double *red) {
 }
 
 // ------------------------------------------------------------- DIIS solve
-__device__ void fused_solve(const DevSys &S, double *rows0, double *rows1, double *red) {
-  __shared__ double diis_dots[kMaxDIIS];
-  __shared__ double diis_A[kMaxDIIS * kMaxDIIS];
+__device__ void fused_solve(const DevSys &S0) {
+  CSys &S = uniform_sys(S0);
+  double *diis_dots = s_diis;
+  double *diis_A = s_diis + kMaxDIIS;
   const int nvar = S.nit * S.M;
   const int tid = work_tid(S), nt = S.threads;
   const int m = S.m;
-  const double dmax = S.deltamax;
-  DevScalars &sc = *S.sc;
-  double *x = S.x, *g = S.g, *x0 = S.x0, *xR = S.xR, *xx0 = S.xx0;
+  const double dmax = S.deltamax, tol = S.tol;
+  const int itlim = S.itlim;
+  DevScalars &sc = *S0.sc;
+  gdbl *x = hbm(S.x), *x0 = hbm(S.x0), *xR = hbm(S.xR), *xx0 = hbm(S.xx0);
+  const gdbl *g = hbm(S.g);
   for (int v = tid; v < nvar; v += nt) x0[v] = x[v];
   __syncthreads();
-  double res = fused_residual(S, rows0, rows1, red);
+  double res = fused_residual(S0);
   if (res < 0) return;
   for (int v = tid; v < nvar; v += nt) {
     const double xv = x[v] + (-dmax) * g[v];
@@ -581,15 +831,15 @@ __device__ void fused_solve(const DevSys &S, double *rows0, double *rows1, doubl
   }
   int it = 0, k_diis = 0;
   __syncthreads();
-  while (res > S.tol && it < S.itlim) {
+  while (res > tol && it < itlim) {
     it++;
     for (int v = tid; v < nvar; v += nt) x0[v] = x[v];
     __syncthreads();
-    res = fused_residual(S, rows0, rows1, red);
+    res = fused_residual(S0);
     if (res < 0) break;
     const int k = it % m;
     k_diis++;
-    double *xRk = xR + (size_t)k * nvar, *xxk = xx0 + (size_t)k * nvar;
+    gdbl *xRk = xR + (size_t)k * nvar, *xxk = xx0 + (size_t)k * nvar;
     for (int v = tid; v < nvar; v += nt) {
       const double xv = x[v] + (-dmax) * g[v];
       x[v] = xv;
@@ -601,64 +851,56 @@ __device__ void fused_solve(const DevSys &S, double *rows0, double *rows1, doubl
     for (int i = 0; i < kd; i++) {
       int posi = k - kd + 1 + i;
       if (posi < 0) posi += m;
-      const double *a = xx0 + (size_t)posi * nvar;
+      const gdbl *a = xx0 + (size_t)posi * nvar;
       double acc = 0;
       for (int v = tid; v < nvar; v += nt) acc += a[v] * xxk[v];
-      acc = wg_sum(acc, red);
-      if (tid == 0) diis_dots[i] = acc;
+      acc = wg_sum(acc);
+      if (threadIdx.x == 0) diis_dots[i] = acc;
     }
     __syncthreads();
-    if (tid == 0) diis_gram_update(sc, m, k_diis, diis_dots, diis_A);
+    if (threadIdx.x == 0) diis_gram_update(sc, m, k_diis, diis_dots, diis_A);
     __syncthreads();
-    if (tid < 64) {
+    if (threadIdx.x < 64) {
       const bool ok = wave_diis_coefficients(diis_A, kd, sc.C);
-      if (tid == 0 && !ok) sc.status = DS_NAN;
+      if (threadIdx.x == 0 && !ok) sc.status = DS_NAN;
     }
     __syncthreads();
     if (sc.status != DS_OK) break;
+    const gdbl *C = hbm((const double *)sc.C);
     for (int v = tid; v < nvar; v += nt) {
       int posi = k - kd + 1;
       if (posi < 0) posi += m;
-      double xv = 0.0 + sc.C[0] * xR[(size_t)posi * nvar + v];
+      double xv = 0.0 + C[0] * xR[(size_t)posi * nvar + v];
       for (int i = 1; i < kd; i++) {
         posi = k - kd + 1 + i;
         if (posi < 0) posi += m;
-        xv += sc.C[i] * xR[(size_t)posi * nvar + v];
+        xv += C[i] * xR[(size_t)posi * nvar + v];
       }
       x[v] = xv;
     }
     __syncthreads();
   }
-  if (tid == 0) {
+  if (threadIdx.x == 0) {
     sc.iterations = it;
-    sc.converged = (sc.status == DS_OK && res <= S.tol) ? 1 : 0;
+    sc.converged = (sc.status == DS_OK && res <= tol) ? 1 : 0;
   }
   __syncthreads();
 }
 
 // ------------------------------------------------------------- kernel
-__global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode, int use_lds) {
+__global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode) {
   const DevSys &S = *systems[blockIdx.x];
-  extern __shared__ double lds[];
-  __shared__ double red[32];
-  double *rows0 = nullptr, *rows1 = nullptr;
-  if (use_lds && S.use_lds) {
-    int R = 0;
-    for (int mo = 0; mo < S.nmol; mo++) R = R > S.mol_size[mo] * S.M ? R : S.mol_size[mo] * S.M;
-    rows0 = lds;
-    rows1 = lds + R;
-  }
   if (threadIdx.x == 0) {
     S.sc->status = DS_OK;
     S.sc->t_begin = wall_clock64();
   }
   __syncthreads();
   if (mode == FUSED_RESIDUAL) {
-    const double r = fused_residual(S, rows0, rows1, red);
-    if (r >= 0) fused_thermo(S, red);
+    const double r = fused_residual(S);
+    if (r >= 0) fused_thermo(S);
   } else {
-    fused_solve(S, rows0, rows1, red);
-    if (S.sc->status == DS_OK) fused_thermo(S, red);
+    fused_solve(S);
+    if (S.sc->status == DS_OK) fused_thermo(S);
   }
   if (threadIdx.x == 0) S.sc->t_end = wall_clock64();
 }
@@ -670,9 +912,14 @@ hipError_t launch_fused(DevSys *const *d_systems, int nsys, int mode, int thread
                                        (int)lds_bytes);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_fused, dim3(nsys), dim3(threads), lds_bytes, stream, d_systems, mode,
-                     lds_bytes > 0 ? 1 : 0);
+  hipLaunchKernelGGL(k_fused, dim3(nsys), dim3(threads), lds_bytes, stream, d_systems, mode);
   return hipGetLastError();
+}
+
+size_t fused_static_lds() {
+  hipFuncAttributes a;
+  if (hipFuncGetAttributes(&a, (const void *)k_fused) != hipSuccess) return 0;
+  return a.sharedSizeBytes;
 }
 
 }  // namespace nscf

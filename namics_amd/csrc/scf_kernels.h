@@ -12,6 +12,8 @@ enum FusedMode : int { FUSED_RESIDUAL = 0, FUSED_SOLVE = 1 };
 // lds_bytes > 0 stages the propagator rows in LDS (2 rows of size*M doubles).
 hipError_t launch_fused(DevSys *const *d_systems, int nsys, int mode, int threads, size_t lds_bytes,
                         hipStream_t stream);
+// Static LDS of the fused kernel (bytes), to size the dynamic part.
+size_t fused_static_lds();
 
 // Multi-workgroup path for large lattices (one launch per phase / per
 // propagation step).  Host-driven; see scf_grid.hip.
