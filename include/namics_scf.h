@@ -145,6 +145,14 @@ int scf_get_results(const scf_system *sys, scf_results *out);
  * Replaces: the `pro` output of mon/mol phi (src/output.cpp:385-708). */
 int scf_get_profiles(const scf_system *sys, double *seg_phi, double *mol_phi);
 
+/* Device time of the last fused residual / solve by phase, in ms:
+ * [0] Boltzmann weights (Segment::PrepareForCalculations), [1] propagator
+ * chains with composition (Molecule::ComputePhi), [2] normalisation,
+ * densities and residual (System::ComputePhis, Classical_residual),
+ * [3] the rest of the solve (DIIS update and thermodynamics).
+ * Profiling aid without a counterpart in the reference. */
+int scf_get_phase_ms(const scf_system *sys, double *ms);
+
 /* Unit-test entry: N-1 Markov-1 steps from G(1) = G1 (host arrays of M),
  * out = N x M end-point distributions.  Replaces Lattice::propagate
  * (src/lattice.h:133; src/LG1Planar.cpp:138-159, src/LGrad2.cpp:1053-1107,

@@ -659,6 +659,14 @@ int scf_get_profiles(const scf_system *S, double *seg_phi, double *mol_phi) {
   return SCF_OK;
 }
 
+int scf_get_phase_ms(const scf_system *S, double *ms) {
+  if (!S || !ms) return fail(SCF_E_ARG, "null argument");
+  DevScalars sc;
+  if (read_scalars(S, sc) != SCF_OK) return SCF_E_HIP;
+  for (int p = 0; p < 4; p++) ms[p] = (double)sc.t_phase[p] / wall_clock_khz();
+  return SCF_OK;
+}
+
 int scf_propagate_chain(scf_system *S, const double *G1, int N, double *out) {
   if (!S || !G1 || !out || N < 1) return fail(SCF_E_ARG, "bad argument");
   const size_t M = S->h.M;

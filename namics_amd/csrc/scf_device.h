@@ -42,6 +42,7 @@ struct DevScalars {
   double red[kMaxDIIS + 4];       // reduction results handed between phases
   int status, iterations, k_diis, converged;
   unsigned long long t_begin, t_end;  // device wall clock at solve start / end
+  unsigned long long t_phase[4];      // wall-clock ticks: prepare, chains, densities+residual, DIIS
 };
 
 struct DevSys {

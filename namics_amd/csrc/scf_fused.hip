@@ -703,6 +703,7 @@ __device__ double fused_residual(const DevSys &S) {
   const int M = S.M, W = n_work_sites(S);
   const int tid = work_tid(S), nt = S.threads;
   DevScalars &sc = *S.sc;
+  const unsigned long long t0 = wall_clock64();
   for (int w = tid; w < W; w += nt) {
     int x, y, z;
     const int i = work_site(S, w, x, y, z);
@@ -710,6 +711,7 @@ __device__ double fused_residual(const DevSys &S) {
     for (int mm = 0; mm < S.nmm; mm++) S.rho[mm * M + i] = 0.0;
   }
   __syncthreads();
+  const unsigned long long t1 = wall_clock64();
   for (int mo = 0; mo < S.nmol; mo++) {
     if (S.mol_markov[mo] == 2) {
       if (S.use_lds)
@@ -726,6 +728,7 @@ __device__ double fused_residual(const DevSys &S) {
       chain_markov1<false>(S, mo);
     }
   }
+  const unsigned long long t2 = wall_clock64();
   if (threadIdx.x == 0) {
     for (int i = 0; i < S.nmol; i++) {
       if (S.mol_freedom[i] == 0) sc.phibulk[i] = S.mol_phibulk_in[i];
@@ -755,7 +758,12 @@ __device__ double fused_residual(const DevSys &S) {
     }
   }
   const double r = S.max_g ? wg_max(acc) : sqrt(wg_sum(acc));
-  if (threadIdx.x == 0) sc.residual = r;
+  if (threadIdx.x == 0) {
+    sc.residual = r;
+    sc.t_phase[0] += t1 - t0;
+    sc.t_phase[1] += t2 - t1;
+    sc.t_phase[2] += wall_clock64() - t2;
+  }
   __syncthreads();
   return r;
 }
@@ -892,6 +900,7 @@ __global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode
   const DevSys &S = *systems[blockIdx.x];
   if (threadIdx.x == 0) {
     S.sc->status = DS_OK;
+    for (int p = 0; p < 4; p++) S.sc->t_phase[p] = 0;
     S.sc->t_begin = wall_clock64();
   }
   __syncthreads();
@@ -902,7 +911,13 @@ __global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode
     fused_solve(S);
     if (S.sc->status == DS_OK) fused_thermo(S);
   }
-  if (threadIdx.x == 0) S.sc->t_end = wall_clock64();
+  if (threadIdx.x == 0) {
+    DevScalars &sc = *S.sc;
+    sc.t_end = wall_clock64();
+    const unsigned long long in_res = sc.t_phase[0] + sc.t_phase[1] + sc.t_phase[2];
+    const unsigned long long total = sc.t_end - sc.t_begin;
+    sc.t_phase[3] = total > in_res ? total - in_res : 0;
+  }
 }
 
 hipError_t launch_fused(DevSys *const *d_systems, int nsys, int mode, int threads, size_t lds_bytes,

@@ -80,6 +80,7 @@ def lib():
         L.scf_get_results.argtypes = [vp, ctypes.POINTER(Results)]
         L.scf_get_profiles.argtypes = [vp, _D, _D]
         L.scf_propagate_chain.argtypes = [vp, _D, ctypes.c_int, _D]
+        L.scf_get_phase_ms.argtypes = [vp, _D]
         L.scf_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -182,6 +183,13 @@ class ScfSystem:
         out["seg_phi"] = {s: phi[i] for i, s in enumerate(self.seg_names)}
         out["mol_phi"] = {m: mphi[i] for i, m in enumerate(self.mol_names)}
         return out
+
+    def phase_ms(self):
+        """Device time of the last fused residual / solve by phase (ms):
+        prepare, chains, densities+residual, rest (DIIS, thermodynamics)."""
+        out = np.zeros(4)
+        _check(lib().scf_get_phase_ms(self.h, _p(out)))
+        return dict(zip(("prepare", "chains", "densities_residual", "diis_other"), out.tolist()))
 
     def propagate_chain(self, G1, N):
         G1 = np.ascontiguousarray(G1, dtype=np.float64)

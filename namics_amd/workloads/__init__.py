@@ -41,6 +41,57 @@ newton : isaac : iterationlimit : 5000
     return text
 
 
+def diblock_box(n=96, frac=0.05, chi_bw=1.6, chi_ab=1.6, tol=1e-7, save_memory=True, itlim=5000):
+    """3-gradient periodic simple-cubic box of n^3 sites: diblock (A)10(B)10,
+    restricted to theta = frac * n^3 monomers, in monomeric solvent W, with
+    chi(B,W) = chi(A,B) = chi_bw / chi_ab and chi(A,W) = 0."""
+    theta = frac * n ** 3
+    return """
+lat : box : gradients : 3
+lat : box : n_layers_x : {n}
+lat : box : n_layers_y : {n}
+lat : box : n_layers_z : {n}
+lat : box : lattice_type : simple_cubic
+lat : box : lowerbound_x : periodic
+lat : box : upperbound_x : periodic
+lat : box : lowerbound_y : periodic
+lat : box : upperbound_y : periodic
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : A : freedom : free
+mon : B : freedom : free
+mon : W : freedom : free
+mon : B : chi_W : {cbw!r}
+mon : A : chi_B : {cab!r}
+mol : surf : composition : (A)10(B)10
+mol : surf : freedom : restricted
+mol : surf : theta : {theta!r}
+mol : surf : save_memory : {sm}
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+newton : isaac : method : DIIS
+newton : isaac : tolerance : {tol!r}
+newton : isaac : iterationlimit : {itlim}
+""".format(n=n, cbw=chi_bw, cab=chi_ab, theta=theta, sm="true" if save_memory else "false", tol=tol,
+           itlim=itlim)
+
+
+def seeded_box_fields(names, n, seed, amp=0.1, well=1.0, width=4.0, well_seg="B"):
+    """Initial fields for `diblock_box`: uniform noise in [-amp, amp] on every
+    field block plus, on the block of `well_seg`, a Gaussian well of depth
+    `well` and width `width` (sites) at the box centre, which nucleates one
+    aggregate.  Layout: blocks in `names` order, each (n+2)^3 x-major with
+    ghost layers (ghost values are never read)."""
+    m = n + 2
+    x = uniform_fields(len(names) * m ** 3, seed, amp).reshape(len(names), m, m, m)
+    if well_seg in names:
+        c = (n + 1) / 2.0
+        r = np.arange(m) - c
+        r2 = r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2
+        x[names.index(well_seg)] -= well * np.exp(-r2 / (2.0 * width ** 2))
+    return x.ravel()
+
+
 def uniform_fields(nvar, seed, amp):
     return np.random.default_rng(seed).uniform(-amp, amp, nvar)
 
