@@ -59,6 +59,7 @@ typedef struct {
   double kernel_ms; /* GPU time of the launch(es) that solved this system (HIP events) */
   double solve_ms;  /* this system's own solve time on the device (device wall clock;
                        equals kernel_ms for the grid path) */
+  int fp32_iterations; /* iterations evaluated with FP32 propagators (SCF_PREC_MIXED) */
 } scf_solve_info;
 
 typedef struct {
@@ -98,6 +99,21 @@ size_t scf_system_device_bytes(const scf_system *sys);
 
 /* Force an execution strategy (SCF_MODE_*). */
 int scf_system_set_mode(scf_system *sys, int mode);
+
+/* propagator precision of a solve */
+enum {
+  SCF_PREC_FP64 = 0,  /* every evaluation in FP64 (the reference's arithmetic) */
+  SCF_PREC_MIXED = 1  /* FP32 propagators while ||g|| > switch, then FP64 */
+};
+
+/* Precision of the solves of this system.  SCF_PREC_MIXED evaluates the
+ * propagator recursion in FP32 (per-molecule gauge shift, FP64 composition
+ * sums, partition functions, DIIS and thermodynamics) until the residual
+ * drops below `switch_residual` (> the tolerance), then continues in FP64 to
+ * the tolerance, so the converged state is an FP64 one.  Applies to the
+ * 2-gradient strip path; other lattices stay FP64.  No counterpart in the
+ * reference (its arithmetic is FP64 throughout). */
+int scf_system_set_precision(scf_system *sys, int mode, double switch_residual);
 
 /* Override solver settings: tolerance, deltamax, DIIS history m, iteration
  * limit (values <= 0 keep the input's setting). */

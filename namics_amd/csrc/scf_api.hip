@@ -330,6 +330,19 @@ int build(scf_system *S) {
   }
   h.use_lds = rows <= lds_limit() ? 1 : 0;
   S->lds_bytes = h.use_lds ? rows : 0;
+  h.precision = SCF_PREC_FP64;
+  h.mixed_switch = 1e-5;
+  {
+    // weighted interior volume (Lattice::WeightedSum of ones)
+    double vol = 0;
+    for (int x = 1; x <= l.MX; x++)
+      for (int y = 1; y <= (l.gradients >= 2 ? l.MY : 1); y++)
+        for (int z = 1; z <= (l.gradients == 3 ? l.MZ : 1); z++) {
+          const int i = l.gradients == 1 ? x : (l.gradients == 2 ? x * h.JX + y : x * h.JX + y * h.JY + z);
+          vol += (l.gradients == 3 || l.geometry == GEOM_PLANAR) ? 1.0 : l.L[i];
+        }
+    h.volume = vol;
+  }
   S->d = (DevSys *)(base + o_dev);
   S->d_list = (DevSys **)(base + o_list);
   HIP_TRY(hipMemcpy(S->d, &h, sizeof(DevSys), hipMemcpyHostToDevice));
@@ -356,7 +369,6 @@ int grid_solve(scf_system *S, scf_solve_info *info) {
   hipStream_t st = S->stream;
   DevScalars sc;
   HIP_TRY(grid_init(S->d, st));
-  HIP_TRY(grid_copy_x0(h, S->d, S->nb, st));
   HIP_TRY(grid_residual(h, S->d, h.partial, S->nb, st));
   HIP_TRY(hipMemcpyAsync(&sc, h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -466,6 +478,18 @@ int scf_system_set_mode(scf_system *S, int mode) {
   if (!S || mode < SCF_MODE_AUTO || mode > SCF_MODE_GRID) return fail(SCF_E_ARG, "bad mode");
   S->mode = mode;
   return SCF_OK;
+}
+
+int scf_system_set_precision(scf_system *S, int mode, double switch_residual) {
+  if (!S) return fail(SCF_E_ARG, "null system");
+  if (mode != SCF_PREC_FP64 && mode != SCF_PREC_MIXED) return fail(SCF_E_ARG, "unknown precision mode");
+  if (mode == SCF_PREC_MIXED && !(switch_residual > S->h.tol))
+    return fail(SCF_E_ARG, "the FP64 switch level must lie above the tolerance");
+  if (mode == SCF_PREC_MIXED && (S->h.strip_sx == 0 || S->effective_mode() != SCF_MODE_FUSED))
+    return fail(SCF_E_UNSUPPORTED, "FP32 propagators need the 2-gradient register-strip path");
+  S->h.precision = mode;
+  if (mode == SCF_PREC_MIXED) S->h.mixed_switch = switch_residual;
+  return upload_solver(S);
 }
 
 int scf_system_set_solver(scf_system *S, double tolerance, double deltamax, int m, int iterationlimit) {
@@ -598,6 +622,7 @@ int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info) 
       HIP_TRY(hipEventElapsedTime(&ms, S->ev0, S->ev1));
       info[i].kernel_ms = ms;
       info[i].solve_ms = ms;
+      info[i].fp32_iterations = 0;
     } else {
       DevScalars sc;
       int rc = read_scalars(S, sc);
@@ -607,6 +632,7 @@ int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info) 
       info[i].status = sc.status != DS_OK ? sc.status : (sc.converged ? SCF_OK : SCF_NOT_CONVERGED);
       info[i].kernel_ms = fused_ms;
       info[i].solve_ms = (double)(sc.t_end - sc.t_begin) / wall_clock_khz();
+      info[i].fp32_iterations = sc.fp32_iterations;
     }
     S->have_results = info[i].status >= 0;
   }

@@ -43,6 +43,9 @@ struct DevScalars {
   int status, iterations, k_diis, converged;
   unsigned long long t_begin, t_end;  // device wall clock at solve start / end
   unsigned long long t_phase[4];      // wall-clock ticks: prepare, chains, densities+residual, DIIS
+  double lnshift[kMaxMol];            // FP32 propagators: gauge shift c per molecule (G1' = G1 e^c)
+  int fp32;                           // 1 while the solve evaluates propagators in FP32
+  int fp32_iterations;                // iterations of the last solve done with FP32 propagators
 };
 
 struct DevSys {
@@ -94,10 +97,35 @@ struct DevSys {
   int strip_sx;           // 2-gradient strip path: sites per thread along x (0: not used)
   int strip_stride;       // 2-gradient strip path: LDS slots per strip column
   int threads;            // preferred workgroup size for the fused kernel
+  int precision;          // 0: FP64 throughout; 1: FP32 propagators, FP64 refinement
+  double mixed_switch;    // residual below which a mixed solve continues in FP64
+  double volume;          // weighted interior volume (sum of site weights)
   // ---- scalars and reduction workspace
   DevScalars *sc;
   double *partial;        // per-workgroup partial sums (multi-workgroup path)
 };
+
+// ------------------------------------------------------------ address spaces
+// Device arrays are accessed through pointers cast to the global address
+// space (global_* instructions) and LDS rows through the LDS address space
+// (ds_*), rather than generic (flat) pointers: a flat access counts against
+// both the vector-memory and the LDS wait counters, so every wait for an
+// LDS value would also wait for HBM traffic in flight.
+typedef __attribute__((address_space(1))) double gdbl;
+typedef __attribute__((address_space(3))) double ldbl;
+// The system descriptor is read-only during a launch and the same for the
+// whole workgroup; read through the constant address space it is fetched
+// with scalar loads and kept in scalar registers.
+typedef const __attribute__((address_space(4))) DevSys CSys;
+
+__device__ __forceinline__ gdbl *hbm(double *p) { return (gdbl *)p; }
+__device__ __forceinline__ const gdbl *hbm(const double *p) { return (const gdbl *)p; }
+__device__ __forceinline__ CSys &uniform_sys(const DevSys &S) {
+  const unsigned long long p = (unsigned long long)&S;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)p);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(p >> 32));
+  return *(CSys *)(((unsigned long long)hi << 32) | lo);
+}
 
 // ------------------------------------------------------------ index helpers
 template <typename Sys>
@@ -169,9 +197,9 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
       if (x <= M - 2) v += lam * g[map_x(S, x + 1)];
       return v;
     }
-    T v = (T)S.lam_0[i] * g[map_x(S, x)];
-    if (x >= 1) v += (T)S.lam_m[i] * g[map_x(S, x - 1)];
-    if (x <= M - 2) v += (T)S.lam_p[i] * g[map_x(S, x + 1)];
+    T v = (T)hbm(S.lam_0)[i] * g[map_x(S, x)];
+    if (x >= 1) v += (T)hbm(S.lam_m)[i] * g[map_x(S, x - 1)];
+    if (x <= M - 2) v += (T)hbm(S.lam_p)[i] * g[map_x(S, x + 1)];
     return v;
   }
   if (S.gradients == 2) {
@@ -202,7 +230,7 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
     }
     const bool sc = S.lattice_type == 0;
     const T C1 = (T)(sc ? 4.0 / 6.0 : 0.5), C2 = (T)(sc ? 1.0 / 6.0 : 0.25), C3 = (T)(sc ? 4.0 : 2.0);
-    const T lm = (T)S.lam_m[i], lp = (T)S.lam_p[i];
+    const T lm = (T)hbm(S.lam_m)[i], lp = (T)hbm(S.lam_p)[i];
     T v = C1 * g[x0 + y];
     v += lm * g[xm + y];
     v += lp * g[xp + y];
@@ -230,8 +258,8 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
 // src/LG1Planar.cpp:28-55, src/LGrad1.cpp:251-281, src/LGrad2.cpp:207-240,
 // src/LG2Planar.cpp:27-100, src/LGrad3.cpp:156-166).  Identical weights to
 // the propagator; the 1-gradient planar form sums in a different order.
-__device__ __forceinline__ double stencil_side(const DevSys &S, const double *__restrict__ X, int i, int x,
-                                               int y, int z) {
+template <typename Sys, typename Ptr>
+__device__ __forceinline__ double stencil_side(const Sys &S, Ptr X, int i, int x, int y, int z) {
   if (S.gradients == 1 && S.geometry == 0) {
     const int M = S.M;
     const double lam = S.lambda;
@@ -294,7 +322,7 @@ __device__ __forceinline__ void m2_step(const Sys &S, Ptr g, const double *P, bo
       if (hasm) c += P0 * m2_val<T>(S, g, 2, x - 1);
     }
   } else {
-    const T l1 = (T)S.l1[x], l11 = (T)S.l11[x], l_1 = (T)S.l_1[x], l_11 = (T)S.l_11[x];
+    const T l1 = (T)hbm(S.l1)[x], l11 = (T)hbm(S.l11)[x], l_1 = (T)hbm(S.l_1)[x], l_11 = (T)hbm(S.l_11)[x];
     if (forward) {
       if (hasm) {
         T h = l_1 * m2_val<T>(S, g, 0, x - 1);
@@ -343,7 +371,8 @@ __device__ __forceinline__ void m2_step(const Sys &S, Ptr g, const double *P, bo
 // Boltzmann weights of every segment at site i (Segment::PrepareForCalculations
 // src/segment.cpp:835-879): u is the segment's field block with its ghosts
 // bounded, G1 = exp(-u), times the pinning mask, times KSAM.
-__device__ __forceinline__ void site_prepare(const DevSys &S, int i, int x, int y, int z) {
+template <typename Sys>
+__device__ __forceinline__ void site_prepare(const Sys &S, int i, int x, int y, int z) {
   int b;  // bounded source site for u
   if (S.gradients == 1) {
     b = map_x(S, x);
@@ -355,54 +384,56 @@ __device__ __forceinline__ void site_prepare(const DevSys &S, int i, int x, int 
   const int M = S.M;
   for (int k = 0; k < S.nseg; k++) {
     const int blk = S.seg_block[k];
-    const double u = blk >= 0 ? S.x[blk * M + b] : 0.0;
+    const double u = blk >= 0 ? hbm(S.x)[blk * M + b] : 0.0;
     double w = exp(-u);
-    if (S.seg_pinned[k]) w = w * S.seg_mask[k * M + i];
-    w = w * S.ksam[i];
-    S.G1[k * M + i] = w;
+    if (S.seg_pinned[k]) w = w * hbm(S.seg_mask)[k * M + i];
+    w = w * hbm(S.ksam)[i];
+    hbm(S.G1)[k * M + i] = w;
   }
 }
 
 // Densities at site i once GN and the normalisations are known
 // (System::ComputePhis src/system.cpp:2345-2610): molecule-segment densities
 // rho/G1*norm, segment densities, total density.
-__device__ __forceinline__ void site_densities(const DevSys &S, int i) {
+template <typename Sys>
+__device__ __forceinline__ void site_densities(const Sys &S, int i) {
   const int M = S.M;
   const DevScalars &sc = *S.sc;
   double phitot = 0;
 #pragma unroll
   for (int k = 0; k < kMaxSeg; k++)
-    if (k < S.nseg && S.seg_frozen[k]) phitot += S.seg_mask[k * M + i];
+    if (k < S.nseg && S.seg_frozen[k]) phitot += hbm(S.seg_mask)[k * M + i];
   for (int mo = 0; mo < S.nmol; mo++) {
     const double nrm = sc.norm[mo];
     double mtot = 0;
     for (int mm = S.mol_mm_begin[mo]; mm < S.mol_mm_begin[mo + 1]; mm++) {
-      const double g1 = S.G1[S.mm_seg[mm] * M + i];
-      double p = S.rho[mm * M + i];
+      const double g1 = hbm(S.G1)[S.mm_seg[mm] * M + i];
+      double p = hbm(S.rho)[mm * M + i];
       p = g1 != 0 ? p / g1 : 0.0;
       if (nrm > 0) p = p * nrm;
-      S.rho[mm * M + i] = p;
+      hbm(S.rho)[mm * M + i] = p;
       phitot += p;
       mtot += p;
     }
-    S.mol_phitot[mo * M + i] = mtot;
+    hbm(S.mol_phitot)[mo * M + i] = mtot;
   }
   // segment densities: frozen mask, then molecule contributions in order
 #pragma unroll
   for (int k = 0; k < kMaxSeg; k++) {
     if (k >= S.nseg) break;
-    double p = S.seg_frozen[k] ? S.seg_mask[k * M + i] : 0.0;
+    double p = S.seg_frozen[k] ? hbm(S.seg_mask)[k * M + i] : 0.0;
     for (int mm = 0; mm < S.nmm; mm++)
-      if (S.mm_seg[mm] == k) p += S.rho[mm * M + i];
-    S.phi[k * M + i] = p;
+      if (S.mm_seg[mm] == k) p += hbm(S.rho)[mm * M + i];
+    hbm(S.phi)[k * M + i] = p;
   }
-  S.phitot[i] = phitot;
+  hbm(S.phitot)[i] = phitot;
 }
 
 // Residual at an interior site (System::Classical_residual
 // src/system.cpp:2099-2157); also stores the site averages phi_side.
 // Returns g_i^2 summed over the field blocks (for the norm).
-__device__ __forceinline__ double site_residual(const DevSys &S, int i, int x, int y, int z) {
+template <typename Sys>
+__device__ __forceinline__ double site_residual(const Sys &S, int i, int x, int y, int z) {
   const int M = S.M;
   const DevScalars &sc = *S.sc;
   double side[kMaxSeg];
@@ -410,11 +441,11 @@ __device__ __forceinline__ double site_residual(const DevSys &S, int i, int x, i
   for (int k = 0; k < kMaxSeg; k++) {
     side[k] = 0.0;
     if (k < S.nseg) {
-      side[k] = stencil_side(S, S.phi + k * M, i, x, y, z);
-      S.phi_side[k * M + i] = side[k];
+      side[k] = stencil_side(S, hbm(S.phi) + k * M, i, x, y, z);
+      hbm(S.phi_side)[k * M + i] = side[k];
     }
   }
-  const double pt = S.phitot[i];
+  const double pt = hbm(S.phitot)[i];
   double gv[kMaxSeg];
   double alpha = 0;
 #pragma unroll
@@ -422,7 +453,7 @@ __device__ __forceinline__ double site_residual(const DevSys &S, int i, int x, i
     gv[it] = 0.0;
     if (it < S.nit) {
       const int si = S.itmon[it];
-      double gi = S.x[it * M + i];
+      double gi = hbm(S.x)[it * M + i];
 #pragma unroll
       for (int k = 0; k < kMaxSeg; k++) {
         if (k < S.nseg) {
@@ -435,15 +466,15 @@ __device__ __forceinline__ double site_residual(const DevSys &S, int i, int x, i
     }
   }
   alpha = alpha * (1.0 / S.nit);
-  S.alpha[i] = alpha;
+  hbm(S.alpha)[i] = alpha;
   double g2 = 0;
-  const double ks = S.ksam[i];
+  const double ks = hbm(S.ksam)[i];
 #pragma unroll
   for (int it = 0; it < kMaxSeg; it++) {
     if (it < S.nit) {
       double gi = pt > 0 ? gv[it] - alpha + 1 / pt - 1.0 : 0.0;
       gi = gi * ks;
-      S.g[it * M + i] = gi;
+      hbm(S.g)[it * M + i] = gi;
       g2 += gi * gi;
     }
   }
@@ -453,45 +484,45 @@ __device__ __forceinline__ double site_residual(const DevSys &S, int i, int x, i
 // Free-energy and grand-potential densities at an interior site
 // (System::GetFreeEnergy src/system.cpp:2809-3029, GetGrandPotential
 // src/system.cpp:3038-3244).  Returns both through F and GP.
-__device__ __forceinline__ void site_thermo(const DevSys &S, int i, double logc[kMaxMol], double &F,
-                                            double &GP) {
+template <typename Sys>
+__device__ __forceinline__ void site_thermo(const Sys &S, int i, double logc[kMaxMol], double &F, double &GP) {
   const int M = S.M;
   const DevScalars &sc = *S.sc;
   double f = 0;
 #pragma unroll
   for (int mo = 0; mo < kMaxMol; mo++)
-    if (mo < S.nmol) f += logc[mo] * S.mol_phitot[mo * M + i];
+    if (mo < S.nmol) f += logc[mo] * hbm(S.mol_phitot)[mo * M + i];
   for (int j = 0; j < S.nseg; j++) {
     if (!S.sysmon[j]) continue;
-    const double g1 = S.G1[j * M + i];
+    const double g1 = hbm(S.G1)[j * M + i];
     double t = g1 > 0 ? -log(g1) : 0.0;
-    f += -(S.phi[j * M + i] * t);
+    f += -(hbm(S.phi)[j * M + i] * t);
   }
   for (int j = 0; j < S.nseg; j++) {
     if (S.seg_frozen[j]) continue;
-    const double pj = S.phi[j * M + i];
+    const double pj = hbm(S.phi)[j * M + i];
     for (int k = 0; k < S.nseg; k++) {
       double chi = S.chi[j * kMaxSeg + k];
       if (!S.seg_frozen[k]) chi = chi / 2;
-      if (chi != 0) f += pj * S.phi_side[k * M + i] * chi;
+      if (chi != 0) f += pj * hbm(S.phi_side)[k * M + i] * chi;
     }
   }
 #pragma unroll
   for (int mo = 0; mo < kMaxMol; mo++)
-    if (mo < S.nmol) f += S.mol_constF[mo] * S.mol_phitot[mo * M + i];
-  const double ks = S.ksam[i];
+    if (mo < S.nmol) f += S.mol_constF[mo] * hbm(S.mol_phitot)[mo * M + i];
+  const double ks = hbm(S.ksam)[i];
   F = f * ks;
   double gp = 0;
   for (int mo = 0; mo < S.nmol; mo++)
-    gp += (S.mol_phitot[mo * M + i] - sc.phibulk[mo]) * (1.0 / S.mol_N[mo]);
-  gp += S.alpha[i];
+    gp += (hbm(S.mol_phitot)[mo * M + i] - sc.phibulk[mo]) * (1.0 / S.mol_N[mo]);
+  gp += hbm(S.alpha)[i];
   for (int j = 0; j < S.nseg; j++) {
     if (S.seg_frozen[j]) continue;
-    const double pj = S.phi[j * M + i], pbA = sc.seg_phibulk[j];
+    const double pj = hbm(S.phi)[j * M + i], pbA = sc.seg_phibulk[j];
     for (int k = 0; k < S.nseg; k++) {
       if (S.seg_frozen[k]) continue;
       const double chi = S.chi[j * kMaxSeg + k] / 2;
-      if (chi != 0) gp += (pj * S.phi_side[k * M + i] - pbA * sc.seg_phibulk[k]) * chi;
+      if (chi != 0) gp += (pj * hbm(S.phi_side)[k * M + i] - pbA * sc.seg_phibulk[k]) * chi;
     }
   }
   GP = -gp * ks;
@@ -500,9 +531,10 @@ __device__ __forceinline__ void site_thermo(const DevSys &S, int i, double logc[
 // site weight of sums over the lattice (Lattice::WeightedSum): 1 on planar
 // lattices, the site volume L on curved ones; ghosts are excluded by the
 // callers (interior work sites only).
-__device__ __forceinline__ double site_weight(const DevSys &S, int i) {
+template <typename Sys>
+__device__ __forceinline__ double site_weight(const Sys &S, int i) {
   if (S.gradients == 3 || S.geometry == 0) return 1.0;
-  return S.L[i];
+  return hbm(S.L)[i];
 }
 
 // Normalisations after the propagators (System::ComputePhis

@@ -23,26 +23,7 @@
 
 namespace nscf {
 
-// ------------------------------------------------------------- memory spaces
-// Pointers are cast to their address space so that the compiler emits
-// global_* (HBM) and ds_* (LDS) instructions rather than flat ones: a flat
-// access counts against both the vector-memory and the LDS wait counters,
-// so every wait for an LDS read would also wait for HBM traffic in flight.
-typedef __attribute__((address_space(1))) double gdbl;
-typedef __attribute__((address_space(3))) double ldbl;
-// The system descriptor is read-only during a launch and the same for the
-// whole workgroup; read through the constant address space it is fetched
-// with scalar loads.
-typedef const __attribute__((address_space(4))) DevSys CSys;
-
-__device__ __forceinline__ CSys &uniform_sys(const DevSys &S) {
-  const unsigned long long p = (unsigned long long)&S;
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)p);
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(p >> 32));
-  return *(CSys *)(((unsigned long long)hi << 32) | lo);
-}
-__device__ __forceinline__ gdbl *hbm(double *p) { return (gdbl *)p; }
-__device__ __forceinline__ const gdbl *hbm(const double *p) { return (const gdbl *)p; }
+// ------------------------------------------------------------- LDS helpers
 __device__ __forceinline__ ldbl *lds_row(int offset) {
   extern __shared__ double lds_dyn[];
   return (ldbl *)(lds_dyn + offset);
@@ -360,53 +341,60 @@ __device__ __noinline__ void chain_markov1(const DevSys &S0, int mo) {
 // (x0+k, y) of thread t, so the y-1 / y+1 neighbours of a column are the
 // adjacent slots (unit stride across the lanes of a wave, constant offsets
 // for the unrolled columns) and ghost sites are never stored.  Gf rows in HBM
-// use the same layout with the dense column stride nact.
-// The 9-point weights are those of LGrad2::propagate (src/LGrad2.cpp:1058-1073);
-// the planar full stencil (src/LG2Planar.cpp:1085-1096) is the same form with
-// lambda- = lambda+ = 1/6.
-template <int SX, int STRIDE>
-__device__ __forceinline__ void strip_step(const ldbl *__restrict__ p, int am0, int ap0, int sl, int sr, int dm,
-                                           int dp, const ldbl *__restrict__ lamx, double C1, double C2, double C3,
-                                           const double (&g1)[SX], double (&v)[SX]) {
-  // columns x0-1 (slot sl), x0 .. x0+SX-1 (own), x0+SX (slot sr); rows y-1 (a), y (b), y+1 (c)
-  double am = p[sl + dm], bm = p[sl], cm = p[sl + dp];
-  double a0 = p[am0], b0 = v[0], c0 = p[ap0];
+// are thread-major (the SX values of a thread contiguous), so a thread
+// writes and reads its strip with vector accesses.
+//
+// The 9-point weights of LGrad2::propagate (src/LGrad2.cpp:1058-1073) and of
+// the planar full stencil (src/LG2Planar.cpp:1085-1096) factor into an axial
+// and a radial pass: with column sums q(x) = C2 G(x,y-1) + C1 G(x,y) +
+// C2 G(x,y+1), the step is G'(x,y) = [C1 q(x) + lambda-(x) q(x-1) +
+// lambda+(x) q(x+1)] G1(x,y), C1 = 1 - 2 lambda (4/6 simple cubic, 1/2
+// hexagonal), C2 = lambda (1/6, 1/4); the planar stencil has lambda- =
+// lambda+ = C2.  The factored form takes 62 instead of 88 operations per
+// strip of 8 sites; it equals the reference's sum up to rounding.
+//
+// T is the propagator type.  With T = float the Boltzmann weights carry a
+// gauge factor e^c (c per molecule), which keeps the end-point
+// distributions near unity: G'(s) = G(s) e^{c s}.  It cancels in the
+// composition up to the constant e^{c(N+1)}, which is divided out in FP64
+// together with GN' = GN e^{c N}; the composition sums are FP64.
+template <typename T, int SX, int STRIDE>
+__device__ __forceinline__ void strip_step(const __attribute__((address_space(3))) T *__restrict__ p, int am0,
+                                           int ap0, int sl, int sr, int dm, int dp,
+                                           const __attribute__((address_space(3))) T *__restrict__ lamx, T C1,
+                                           T C2, const T (&g1)[SX], T (&v)[SX]) {
+  // column sums of columns x0-1 (slot sl), x0 .. x0+SX-1 (own), x0+SX (slot sr)
+  T qm = C1 * p[sl];
+  qm += C2 * p[sl + dm];
+  qm += C2 * p[sl + dp];
+  T q0 = C1 * v[0];
+  q0 += C2 * p[am0];
+  q0 += C2 * p[ap0];
 #pragma unroll
   for (int k = 0; k < SX; k++) {
-    double ap, bp, cp;
+    T qp;
     if (k + 1 < SX) {
-      ap = p[am0 + (k + 1) * STRIDE];
-      bp = v[k + 1];
-      cp = p[ap0 + (k + 1) * STRIDE];
+      qp = C1 * v[k + 1];
+      qp += C2 * p[am0 + (k + 1) * STRIDE];
+      qp += C2 * p[ap0 + (k + 1) * STRIDE];
     } else {
-      ap = p[sr + dm];
-      bp = p[sr];
-      cp = p[sr + dp];
+      qp = C1 * p[sr];
+      qp += C2 * p[sr + dm];
+      qp += C2 * p[sr + dp];
     }
-    const double lmk = lamx[2 * k], lpk = lamx[2 * k + 1];
-    double t = C1 * b0;
-    t += lmk * bm;
-    t += lpk * bp;
-    t += C2 * a0;
-    t += C2 * c0;
-    t *= C3;
-    t += lmk * am;
-    t += lmk * cm;
-    t += lpk * ap;
-    t += lpk * cp;
-    t *= C2;
+    T t = C1 * q0;
+    t += lamx[2 * k] * qm;
+    t += lamx[2 * k + 1] * qp;
     v[k] = t * g1[k];
-    am = a0;
-    a0 = ap;
-    bm = b0;
-    b0 = bp;
-    cm = c0;
-    c0 = cp;
+    qm = q0;
+    q0 = qp;
   }
 }
 
-template <int SX, int STRIDE>
-__device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo) {
+template <typename T, int SX, int STRIDE>
+__device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shift) {
+  typedef __attribute__((address_space(3))) T LT;
+  typedef __attribute__((address_space(1))) T GT;
   CSys &S = uniform_sys(S0);
   const int MX = S.MX, MY = S.MY, JX = S.JX;
   const int nact = (MX / SX) * MY;
@@ -425,66 +413,68 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo) {
   const int am0 = tt + dm, ap0 = tt + dp;
   const bool cyl = S.geometry != 0;
   const bool sc = S.lattice_type == 0;
-  const double C1 = sc ? 4.0 / 6.0 : 0.5, C2 = sc ? 1.0 / 6.0 : 0.25, C3 = sc ? 4.0 : 2.0;
+  const T C1 = sc ? (T)(4.0 / 6.0) : (T)0.5, C2 = sc ? (T)(1.0 / 6.0) : (T)0.25;
   const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
   const int N = S.mol_N[mo];
   const size_t M = S.M;
   const size_t RG = (size_t)SX * nact;  // Gf row length (interior sites)
-  gdbl *Gf = hbm(S.Gf) + tt;
+  GT *Gf = (GT *)S.Gf + (size_t)tt * SX;
   const gdbl *G1 = hbm(S.G1);
-  ldbl *ra = lds_row(0), *rb = lds_row(SX * STRIDE);
-  ldbl *lam = (ldbl *)s_lam;
+  LT *ra = (LT *)lds_row(0), *rb = ra + SX * STRIDE;
+  LT *lam = (LT *)s_lam;
   for (int x = threadIdx.x; x < MX + 2; x += blockDim.x) {
-    lam[2 * x] = cyl ? hbm(S.lam_m)[x * JX + 1] : 1.0 / 6.0;
-    lam[2 * x + 1] = cyl ? hbm(S.lam_p)[x * JX + 1] : 1.0 / 6.0;
+    lam[2 * x] = cyl ? (T)hbm(S.lam_m)[x * JX + 1] : C2;
+    lam[2 * x + 1] = cyl ? (T)hbm(S.lam_p)[x * JX + 1] : C2;
   }
   lds_barrier();
-  const ldbl *lamx = lam + 2 * x0;
+  const LT *lamx = lam + 2 * x0;
   const size_t site0 = (size_t)x0 * JX + y;  // site of column 0; column k is site0 + k JX
-  double g1[SX], v[SX], racc[SX], gfv[SX];
+  const double eshift = exp(shift);
+  T g1[SX], v[SX], gfv[SX];
+  double racc[SX];
 
-#define LOAD_G1(seg)                                                   \
-  {                                                                    \
-    const gdbl *p_ = G1 + (size_t)(seg) * M + site0;                   \
-    _Pragma("unroll") for (int k = 0; k < SX; k++) g1[k] = p_[k * JX]; \
+#define LOAD_G1(seg)                                                                  \
+  {                                                                                   \
+    const gdbl *p_ = G1 + (size_t)(seg) * M + site0;                                  \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) g1[k] = (T)(p_[k * JX] * eshift); \
   }
 #define STORE_ROW(row)                                                            \
   {                                                                               \
     _Pragma("unroll") for (int k = 0; k < SX; k++) (row)[k * STRIDE + tt] = v[k]; \
   }
-#define STORE_GF(s_)                                                            \
-  {                                                                             \
-    gdbl *q_ = Gf + (size_t)(s_) * RG;                                          \
-    _Pragma("unroll") for (int k = 0; k < SX; k++) q_[(size_t)k * nact] = v[k]; \
+#define STORE_GF(s_)                                                 \
+  {                                                                  \
+    GT *q_ = Gf + (size_t)(s_) * RG;                                 \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) q_[k] = v[k];     \
   }
-#define LOAD_GF(s_)                                                               \
-  {                                                                               \
-    const gdbl *q_ = Gf + (size_t)(s_) * RG;                                      \
-    _Pragma("unroll") for (int k = 0; k < SX; k++) gfv[k] = q_[(size_t)k * nact]; \
+#define LOAD_GF(s_)                                                  \
+  {                                                                  \
+    const GT *q_ = Gf + (size_t)(s_) * RG;                           \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) gfv[k] = q_[k];   \
   }
 
   // ---- forward
   if (active) LOAD_G1(S.block_seg[b0]);
 #pragma unroll
-  for (int k = 0; k < SX; k++) v[k] = active ? g1[k] : 0.0;
+  for (int k = 0; k < SX; k++) v[k] = active ? g1[k] : (T)0;
   if (active) {
     STORE_GF(0);
     STORE_ROW(ra);
   }
   lds_barrier();
-  ldbl *prev = ra, *next = rb;
+  LT *prev = ra, *next = rb;
   int s = 1;
   for (int b = b0; b < b1; b++) {
     if (b != b0 && active) LOAD_G1(S.block_seg[b]);
     const int n = S.block_n[b] - (b == b0 ? 1 : 0);
     for (int j = 0; j < n; j++, s++) {
       if (active) {
-        strip_step<SX, STRIDE>(prev, am0, ap0, sl, sr, dm, dp, lamx, C1, C2, C3, g1, v);
+        strip_step<T, SX, STRIDE>(prev, am0, ap0, sl, sr, dm, dp, lamx, C1, C2, g1, v);
         STORE_GF(s);
         STORE_ROW(next);
       }
       lds_barrier();
-      ldbl *tmp = prev;
+      LT *tmp = prev;
       prev = next;
       next = tmp;
     }
@@ -495,14 +485,15 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo) {
     if (active) {
       const gdbl *Lw = hbm(S.L) + site0;
 #pragma unroll
-      for (int k = 0; k < SX; k++) acc += (cyl ? Lw[k * JX] : 1.0) * v[k];
+      for (int k = 0; k < SX; k++) acc += (cyl ? Lw[k * JX] : 1.0) * (double)v[k];
     }
     const double gn = wg_sum(acc);
-    if (threadIdx.x == 0) S0.sc->GN[mo] = gn;
+    if (threadIdx.x == 0) S0.sc->GN[mo] = gn * exp(-shift * N);
   }
   // ---- backward with composition; the density of the current block in
   // registers, Gf(s-1) fetched one step ahead
-  ldbl *ga = ra, *gb = rb;
+  const double unshift = exp(-shift * (N + 1));
+  LT *ga = ra, *gb = rb;
   const int bl = b1 - 1;
   if (active) {
     LOAD_G1(S.block_seg[bl]);
@@ -510,7 +501,7 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo) {
 #pragma unroll
     for (int k = 0; k < SX; k++) {
       v[k] = g1[k];
-      racc[k] = gfv[k] * v[k];
+      racc[k] = (double)gfv[k] * (double)v[k];
     }
     STORE_ROW(ga);
     if (N >= 2) LOAD_GF(N - 2);
@@ -526,21 +517,21 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo) {
     const int n = S.block_n[b] - (b == bl ? 1 : 0);
     for (int j = 0; j < n; j++, s--) {
       if (active) {
-        strip_step<SX, STRIDE>(ga, am0, ap0, sl, sr, dm, dp, lamx, C1, C2, C3, g1, v);
+        strip_step<T, SX, STRIDE>(ga, am0, ap0, sl, sr, dm, dp, lamx, C1, C2, g1, v);
 #pragma unroll
-        for (int k = 0; k < SX; k++) racc[k] += gfv[k] * v[k];
+        for (int k = 0; k < SX; k++) racc[k] += (double)gfv[k] * (double)v[k];
         STORE_ROW(gb);
         if (s >= 1) LOAD_GF(s - 1);
       }
       lds_barrier();
-      ldbl *tmp = ga;
+      LT *tmp = ga;
       ga = gb;
       gb = tmp;
     }
     if (active) {
       gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[b] * M + site0;
 #pragma unroll
-      for (int k = 0; k < SX; k++) rho[k * JX] += racc[k];
+      for (int k = 0; k < SX; k++) rho[k * JX] += racc[k] * unshift;
     }
   }
 #undef LOAD_G1
@@ -677,64 +668,76 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
   __syncthreads();
 }
 
-__device__ __noinline__ void chain_strip_dispatch(const DevSys &S, int mo) {
+template <typename T>
+__device__ __forceinline__ void chain_strip_select(const DevSys &S, int mo, double shift) {
   const int sx = S.strip_sx, st = S.strip_stride;
   if (sx == 8) {
     if (st == 256)
-      chain2d_strip<8, 256>(S, mo);
+      chain2d_strip<T, 8, 256>(S, mo, shift);
     else if (st == 512)
-      chain2d_strip<8, 512>(S, mo);
+      chain2d_strip<T, 8, 512>(S, mo, shift);
     else
-      chain2d_strip<8, 1024>(S, mo);
+      chain2d_strip<T, 8, 1024>(S, mo, shift);
   } else {
     if (st == 256)
-      chain2d_strip<4, 256>(S, mo);
+      chain2d_strip<T, 4, 256>(S, mo, shift);
     else if (st == 512)
-      chain2d_strip<4, 512>(S, mo);
+      chain2d_strip<T, 4, 512>(S, mo, shift);
     else
-      chain2d_strip<4, 1024>(S, mo);
+      chain2d_strip<T, 4, 1024>(S, mo, shift);
   }
+}
+
+__device__ __noinline__ void chain_strip_dispatch(const DevSys &S, int mo, bool fp32, double shift) {
+  if (fp32)
+    chain_strip_select<float>(S, mo, shift);
+  else
+    chain_strip_select<double>(S, mo, 0.0);
 }
 
 // ------------------------------------------------------------- residual
 // One Classical_residual: fields x -> g, densities, norms.  Returns ||g||
 // (or max|g|) to every thread, or a negative value on a status error.
-__device__ double fused_residual(const DevSys &S) {
+__device__ double fused_residual(const DevSys &S0) {
+  CSys &S = uniform_sys(S0);
   const int M = S.M, W = n_work_sites(S);
   const int tid = work_tid(S), nt = S.threads;
-  DevScalars &sc = *S.sc;
+  DevScalars &sc = *S0.sc;
+  const bool fp32 = S.precision != 0 && sc.fp32 != 0;
   const unsigned long long t0 = wall_clock64();
+  gdbl *rho = hbm(S.rho);
+  const int nmm = S.nmm;
   for (int w = tid; w < W; w += nt) {
     int x, y, z;
     const int i = work_site(S, w, x, y, z);
     site_prepare(S, i, x, y, z);
-    for (int mm = 0; mm < S.nmm; mm++) S.rho[mm * M + i] = 0.0;
+    for (int mm = 0; mm < nmm; mm++) rho[mm * M + i] = 0.0;
   }
   __syncthreads();
   const unsigned long long t1 = wall_clock64();
   for (int mo = 0; mo < S.nmol; mo++) {
     if (S.mol_markov[mo] == 2) {
       if (S.use_lds)
-        chain_markov2<true>(S, mo);
+        chain_markov2<true>(S0, mo);
       else
-        chain_markov2<false>(S, mo);
+        chain_markov2<false>(S0, mo);
     } else if (S.strip_sx > 0) {
-      chain_strip_dispatch(S, mo);
+      chain_strip_dispatch(S0, mo, fp32, fp32 ? sc.lnshift[mo] : 0.0);
     } else if (S.gradients == 1 && S.use_lds && S.M <= S.threads) {
-      chain1d_m1(S, mo);
+      chain1d_m1(S0, mo);
     } else if (S.use_lds) {
-      chain_markov1<true>(S, mo);
+      chain_markov1<true>(S0, mo);
     } else {
-      chain_markov1<false>(S, mo);
+      chain_markov1<false>(S0, mo);
     }
   }
   const unsigned long long t2 = wall_clock64();
   if (threadIdx.x == 0) {
-    for (int i = 0; i < S.nmol; i++) {
-      if (S.mol_freedom[i] == 0) sc.phibulk[i] = S.mol_phibulk_in[i];
-      if (S.mol_freedom[i] == 1) sc.n[i] = S.mol_n_in[i];
+    for (int i = 0; i < S0.nmol; i++) {
+      if (S0.mol_freedom[i] == 0) sc.phibulk[i] = S0.mol_phibulk_in[i];
+      if (S0.mol_freedom[i] == 1) sc.n[i] = S0.mol_n_in[i];
     }
-    scalars_norms(S);
+    scalars_norms(S0);
   }
   __syncthreads();
   if (sc.status != DS_OK) return -1.0;
@@ -745,19 +748,21 @@ __device__ double fused_residual(const DevSys &S) {
   }
   __syncthreads();
   double acc = 0;
+  const bool max_g = S.max_g;
+  const gdbl *g = hbm(S.g);
   for (int w = tid; w < W; w += nt) {
     int x, y, z;
     const int i = work_site(S, w, x, y, z);
     if (!is_interior(S, x, y, z)) continue;
-    if (S.max_g) {
+    if (max_g) {
       const double g2 = site_residual(S, i, x, y, z);
       (void)g2;
-      for (int it = 0; it < S.nit; it++) acc = fmax(acc, fabs(S.g[it * M + i]));
+      for (int it = 0; it < S.nit; it++) acc = fmax(acc, fabs(g[it * M + i]));
     } else {
       acc += site_residual(S, i, x, y, z);
     }
   }
-  const double r = S.max_g ? wg_max(acc) : sqrt(wg_sum(acc));
+  const double r = max_g ? wg_max(acc) : sqrt(wg_sum(acc));
   if (threadIdx.x == 0) {
     sc.residual = r;
     sc.t_phase[0] += t1 - t0;
@@ -769,15 +774,18 @@ __device__ double fused_residual(const DevSys &S) {
 }
 
 // Thermodynamic summary at the state of the last residual.
-__device__ __noinline__ void fused_thermo(const DevSys &S) {
+__device__ __noinline__ void fused_thermo(const DevSys &S0) {
+  CSys &S = uniform_sys(S0);
   const int W = n_work_sites(S);
   const int tid = work_tid(S), nt = S.threads;
-  DevScalars &sc = *S.sc;
+  DevScalars &sc = *S0.sc;
+  const int nmol = S.nmol;
+  const gdbl *mphi = hbm(S.mol_phitot);
   double logc[kMaxMol];
 #pragma unroll
   for (int mo = 0; mo < kMaxMol; mo++) {
     logc[mo] = 0.0;
-    if (mo < S.nmol) {
+    if (mo < nmol) {
       const double N = S.mol_N[mo];
       logc[mo] = log(N * sc.n[mo] / sc.GN[mo]) / N;
     }
@@ -796,25 +804,53 @@ __device__ __noinline__ void fused_thermo(const DevSys &S) {
     gp += wt * GP;
 #pragma unroll
     for (int mo = 0; mo < kMaxMol; mo++)
-      if (mo < S.nmol) th[mo] += wt * S.mol_phitot[mo * S.M + i];
+      if (mo < nmol) th[mo] += wt * mphi[mo * S.M + i];
   }
   f = wg_sum(f);
   gp = wg_sum(gp);
 #pragma unroll
   for (int mo = 0; mo < kMaxMol; mo++)
-    if (mo < S.nmol) th[mo] = wg_sum(th[mo]);
+    if (mo < nmol) th[mo] = wg_sum(th[mo]);
   if (threadIdx.x == 0) {
     sc.F = f;
     sc.GP = gp;
 #pragma unroll
     for (int mo = 0; mo < kMaxMol; mo++)
-      if (mo < S.nmol) sc.theta[mo] = th[mo];
-    scalars_mu(S);
+      if (mo < nmol) sc.theta[mo] = th[mo];
+    scalars_mu(S0);
   }
   __syncthreads();
 }
 
 // ------------------------------------------------------------- DIIS solve
+// SFNewton::iterate_DIIS (src/sfnewton.cpp:1037-1130).  The residual never
+// changes x, so the iteration's x0 is x itself at the update: one pass per
+// iteration reads x and g, writes x, xR_k and e_k = x_new - x0, and
+// accumulates all kd Gram dots <e_i, e_k> (each thread sums its own entries
+// in index order, then the waves' sums are added in wave order, as for a
+// single dot); a second pass forms x = sum_i c_i xR_i.
+__shared__ double s_dred[kMaxDIIS * 16];  // per-wave partial dots
+
+// Mixed precision: after an evaluation with residual res, decide whether the
+// next ones use FP32 propagators (while res is above the switch level) and
+// set each molecule's gauge shift c = -ln(GN / V) / N from the latest GN, so
+// that the shifted end-point distributions average about one per site.
+__device__ void mixed_update(const DevSys &S, double res, bool first) {
+  if (threadIdx.x == 0) {
+    DevScalars &sc = *S.sc;
+    // FP32 from the first evaluation on, and never again once the solve
+    // has gone over to FP64
+    bool ok = res > S.mixed_switch && (first || sc.fp32 != 0);
+    for (int mo = 0; mo < S.nmol; mo++) {
+      const double gn = sc.GN[mo];
+      if (!(gn > 0) || !isfinite(gn)) ok = false;
+      else sc.lnshift[mo] = -log(gn / S.volume) / S.mol_N[mo];
+    }
+    sc.fp32 = ok ? 1 : 0;
+  }
+  __syncthreads();
+}
+
 __device__ void fused_solve(const DevSys &S0) {
   CSys &S = uniform_sys(S0);
   double *diis_dots = s_diis;
@@ -825,45 +861,102 @@ __device__ void fused_solve(const DevSys &S0) {
   const double dmax = S.deltamax, tol = S.tol;
   const int itlim = S.itlim;
   DevScalars &sc = *S0.sc;
-  gdbl *x = hbm(S.x), *x0 = hbm(S.x0), *xR = hbm(S.xR), *xx0 = hbm(S.xx0);
+  gdbl *x = hbm(S.x), *xR = hbm(S.xR), *xx0 = hbm(S.xx0);
   const gdbl *g = hbm(S.g);
-  for (int v = tid; v < nvar; v += nt) x0[v] = x[v];
-  __syncthreads();
+  ldbl *dred = (ldbl *)s_dred;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double res = fused_residual(S0);
   if (res < 0) return;
+  const bool mixed = S.precision != 0;
+  if (mixed) mixed_update(S0, res, true);
   for (int v = tid; v < nvar; v += nt) {
-    const double xv = x[v] + (-dmax) * g[v];
+    const double xo = x[v];
+    const double xv = xo + (-dmax) * g[v];
     x[v] = xv;
-    xx0[v] = xv - x0[v];
+    xx0[v] = xv - xo;
     xR[v] = xv;
   }
   int it = 0, k_diis = 0;
   __syncthreads();
   while (res > tol && it < itlim) {
     it++;
-    for (int v = tid; v < nvar; v += nt) x0[v] = x[v];
-    __syncthreads();
-    res = fused_residual(S0);
+    if (mixed && sc.fp32) {
+      if (threadIdx.x == 0) sc.fp32_iterations++;
+      res = fused_residual(S0);
+      if (res < 0) {
+        // an FP32 evaluation that failed (e.g. GN underflow) is redone in FP64
+        if (threadIdx.x == 0) {
+          sc.status = DS_OK;
+          sc.fp32 = 0;
+        }
+        __syncthreads();
+        res = fused_residual(S0);
+      }
+    } else {
+      res = fused_residual(S0);
+    }
     if (res < 0) break;
+    if (mixed) mixed_update(S0, res, false);
     const int k = it % m;
     k_diis++;
-    gdbl *xRk = xR + (size_t)k * nvar, *xxk = xx0 + (size_t)k * nvar;
-    for (int v = tid; v < nvar; v += nt) {
-      const double xv = x[v] + (-dmax) * g[v];
-      x[v] = xv;
-      xRk[v] = xv;
-      xxk[v] = xv - x0[v];
-    }
-    __syncthreads();
     const int kd = k_diis > m ? m : k_diis;
-    for (int i = 0; i < kd; i++) {
+    gdbl *xRk = xR + (size_t)k * nvar, *xxk = xx0 + (size_t)k * nvar;
+    // e vectors of the earlier kd-1 iterations, oldest first
+    const gdbl *ei[kMaxDIIS - 1];
+#pragma unroll
+    for (int i = 0; i < kMaxDIIS - 1; i++) {
       int posi = k - kd + 1 + i;
       if (posi < 0) posi += m;
-      const gdbl *a = xx0 + (size_t)posi * nvar;
-      double acc = 0;
-      for (int v = tid; v < nvar; v += nt) acc += a[v] * xxk[v];
-      acc = wg_sum(acc);
-      if (threadIdx.x == 0) diis_dots[i] = acc;
+      ei[i] = xx0 + (size_t)(i < kd - 1 ? posi : 0) * nvar;
+    }
+    double acc[kMaxDIIS - 1], self = 0.0;
+#pragma unroll
+    for (int i = 0; i < kMaxDIIS - 1; i++) acc[i] = 0.0;
+    // two entries per trip, all loads issued before the arithmetic; each
+    // accumulator still adds the thread's entries in index order
+    for (int v0 = tid; v0 < nvar; v0 += 2 * nt) {
+      const int v1 = v0 + nt;
+      const bool has1 = v1 < nvar;
+      const double xo0 = x[v0], g0 = g[v0];
+      const double xo1 = has1 ? x[v1] : 0.0, g1 = has1 ? g[v1] : 0.0;
+      double a0[kMaxDIIS - 1], a1[kMaxDIIS - 1];
+#pragma unroll
+      for (int i = 0; i < kMaxDIIS - 1; i++) {
+        a0[i] = i < kd - 1 ? ei[i][v0] : 0.0;
+        a1[i] = (i < kd - 1 && has1) ? ei[i][v1] : 0.0;
+      }
+      const double xv0 = xo0 + (-dmax) * g0, e0 = xv0 - xo0;
+      x[v0] = xv0;
+      xRk[v0] = xv0;
+      xxk[v0] = e0;
+#pragma unroll
+      for (int i = 0; i < kMaxDIIS - 1; i++)
+        if (i < kd - 1) acc[i] += a0[i] * e0;
+      self += e0 * e0;
+      if (has1) {
+        const double xv1 = xo1 + (-dmax) * g1, e1 = xv1 - xo1;
+        x[v1] = xv1;
+        xRk[v1] = xv1;
+        xxk[v1] = e1;
+#pragma unroll
+        for (int i = 0; i < kMaxDIIS - 1; i++)
+          if (i < kd - 1) acc[i] += a1[i] * e1;
+        self += e1 * e1;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxDIIS; i++) {
+      if (i < kd) {
+        double t = i < kd - 1 ? acc[i < kMaxDIIS - 1 ? i : 0] : self;
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if (lane == 0) dred[i * 16 + wave] = t;
+      }
+    }
+    lds_barrier();
+    if ((int)threadIdx.x < kd) {
+      double t = 0;
+      for (int w = 0; w < nw; w++) t += dred[threadIdx.x * 16 + w];
+      diis_dots[threadIdx.x] = t;
     }
     __syncthreads();
     if (threadIdx.x == 0) diis_gram_update(sc, m, k_diis, diis_dots, diis_A);
@@ -874,17 +967,42 @@ __device__ void fused_solve(const DevSys &S0) {
     }
     __syncthreads();
     if (sc.status != DS_OK) break;
-    const gdbl *C = hbm((const double *)sc.C);
-    for (int v = tid; v < nvar; v += nt) {
-      int posi = k - kd + 1;
-      if (posi < 0) posi += m;
-      double xv = 0.0 + C[0] * xR[(size_t)posi * nvar + v];
-      for (int i = 1; i < kd; i++) {
-        posi = k - kd + 1 + i;
+    // x = sum_i c_i xR_i (Xr_times_ci src/tools_host.h:405-416), four
+    // entries per trip with their loads issued together
+    {
+      const gdbl *C = hbm((const double *)sc.C);
+      double c[kMaxDIIS];
+      const gdbl *ri[kMaxDIIS];
+#pragma unroll
+      for (int i = 0; i < kMaxDIIS; i++) {
+        int posi = k - kd + 1 + i;
         if (posi < 0) posi += m;
-        xv += C[i] * xR[(size_t)posi * nvar + v];
+        c[i] = i < kd ? C[i] : 0.0;
+        ri[i] = xR + (size_t)(i < kd ? posi : 0) * nvar;
       }
-      x[v] = xv;
+      for (int v0 = tid; v0 < nvar; v0 += 4 * nt) {
+        double sum[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int v = v0 + u * nt;
+          sum[u] = 0.0 + c[0] * (v < nvar ? ri[0][v] : 0.0);
+        }
+#pragma unroll
+        for (int i = 1; i < kMaxDIIS; i++) {
+          if (i < kd) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int v = v0 + u * nt;
+              sum[u] += c[i] * (v < nvar ? ri[i][v] : 0.0);
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int v = v0 + u * nt;
+          if (v < nvar) x[v] = sum[u];
+        }
+      }
     }
     __syncthreads();
   }
@@ -900,6 +1018,8 @@ __global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode
   const DevSys &S = *systems[blockIdx.x];
   if (threadIdx.x == 0) {
     S.sc->status = DS_OK;
+    S.sc->fp32 = 0;
+    S.sc->fp32_iterations = 0;
     for (int p = 0; p < 4; p++) S.sc->t_phase[p] = 0;
     S.sc->t_begin = wall_clock64();
   }

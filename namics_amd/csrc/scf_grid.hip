@@ -40,28 +40,28 @@ __device__ __forceinline__ double block_max(double v, double *red) {
 #define GRID_LOOP(S) for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n_work_sites(S); w += gridDim.x * blockDim.x)
 
 __global__ void __launch_bounds__(kGridThreads) k_prepare(const DevSys *__restrict__ d) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   GRID_LOOP(S) {
     int x, y, z;
     const int i = work_site(S, w, x, y, z);
     site_prepare(S, i, x, y, z);
-    for (int mm = 0; mm < S.nmm; mm++) S.rho[mm * S.M + i] = 0.0;
+    for (int mm = 0; mm < S.nmm; mm++) hbm(S.rho)[mm * S.M + i] = 0.0;
   }
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_row_init(const DevSys *__restrict__ d, int seg) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   const double *g1 = S.G1 + seg * S.M;
   GRID_LOOP(S) {
     int x, y, z;
     const int i = work_site(S, w, x, y, z);
-    S.Gf[i] = g1[i];
+    hbm(S.Gf)[i] = g1[i];
   }
 }
 
 // one forward step: Gf[s] = W(Gf[s-1]) * G1
 __global__ void __launch_bounds__(kGridThreads) k_step_fwd(const DevSys *__restrict__ d, int s, int seg) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   const int M = S.M;
   const double *__restrict__ prev = S.Gf + (size_t)(s - 1) * M;
   double *__restrict__ gs = S.Gf + (size_t)s * M;
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(kGridThreads) k_step_fwd(const DevSys *__restr
 
 // first backward row and its composition
 __global__ void __launch_bounds__(kGridThreads) k_bwd_first(const DevSys *__restrict__ d, int s, int seg, int mm) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   const int M = S.M;
   const double *g1 = S.G1 + seg * M;
   const double *gf = S.Gf + (size_t)s * M;
@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(kGridThreads) k_bwd_first(const DevSys *__rest
     int x, y, z;
     const int i = work_site(S, w, x, y, z);
     const double v = g1[i];
-    S.Gb[i] = v;
+    hbm(S.Gb)[i] = v;
     rho[i] += gf[i] * v;
   }
 }
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(kGridThreads) k_bwd_first(const DevSys *__rest
 // backward step with fused composition: Gb[dst] = W(Gb[src]) * G1, rho += Gf[s] Gb[dst]
 __global__ void __launch_bounds__(kGridThreads) k_step_bwd(const DevSys *__restrict__ d, int s, int seg, int mm,
                                                            int src) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   const int M = S.M;
   const double *__restrict__ gb = S.Gb + (size_t)src * M;
   double *__restrict__ gbn = S.Gb + (size_t)(1 - src) * M;
@@ -108,8 +108,51 @@ __global__ void __launch_bounds__(kGridThreads) k_step_bwd(const DevSys *__restr
   }
 }
 
+// 3-gradient steps (LGrad3::propagate src/LGrad3.cpp:731-743): one thread
+// per site of an x-plane, blockIdx.y = x - 1, the (y, z) plane flattened
+// with z fastest, so no division per site and every load is coalesced.
+__device__ __forceinline__ double stencil3(CSys &S, const gdbl *__restrict__ g, int x, int y, int z) {
+  const int JX = S.JX, JY = S.JY;
+  const int bx = x * JX, by = y * JY;
+  double v = g[map_x(S, x - 1) * JX + by + z];
+  v += g[map_x(S, x + 1) * JX + by + z];
+  v += g[bx + map_y(S, y - 1) * JY + z];
+  v += g[bx + map_y(S, y + 1) * JY + z];
+  v += g[bx + by + map_z(S, z - 1)];
+  v += g[bx + by + map_z(S, z + 1)];
+  return v * (1.0 / 6.0);
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_step3_fwd(const DevSys *__restrict__ d, int s, int seg) {
+  CSys &S = uniform_sys(*d);
+  const int MZ = S.MZ, M = S.M;
+  const int p = blockIdx.x * kGridThreads + threadIdx.x;
+  if (p >= S.MY * MZ) return;
+  const int yy = p / MZ;
+  const int x = blockIdx.y + 1, y = yy + 1, z = p - yy * MZ + 1;
+  const int i = x * S.JX + y * S.JY + z;
+  const gdbl *prev = hbm(S.Gf) + (size_t)(s - 1) * M;
+  hbm(S.Gf)[(size_t)s * M + i] = stencil3(S, prev, x, y, z) * hbm(S.G1)[(size_t)seg * M + i];
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_step3_bwd(const DevSys *__restrict__ d, int s, int seg, int mm,
+                                                            int src) {
+  CSys &S = uniform_sys(*d);
+  const int MZ = S.MZ, M = S.M;
+  const int p = blockIdx.x * kGridThreads + threadIdx.x;
+  if (p >= S.MY * MZ) return;
+  const int yy = p / MZ;
+  const int x = blockIdx.y + 1, y = yy + 1, z = p - yy * MZ + 1;
+  const int i = x * S.JX + y * S.JY + z;
+  const gdbl *gb = hbm(S.Gb) + (size_t)src * M;
+  const double v = stencil3(S, gb, x, y, z) * hbm(S.G1)[(size_t)seg * M + i];
+  hbm(S.Gb)[(size_t)(1 - src) * M + i] = v;
+  gdbl *rho = hbm(S.rho) + (size_t)mm * M;
+  rho[i] += hbm(S.Gf)[(size_t)s * M + i] * v;
+}
+
 __global__ void __launch_bounds__(kGridThreads) k_gn_partial(const DevSys *__restrict__ d, int s, double *partial) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   __shared__ double red[kGridThreads / 64];
   const double *gl = S.Gf + (size_t)s * S.M;
   double acc = 0;
@@ -142,7 +185,7 @@ __global__ void k_norms(const DevSys *__restrict__ d) {
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_densities(const DevSys *__restrict__ d) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   if (S.sc->status != DS_OK) return;
   GRID_LOOP(S) {
     int x, y, z;
@@ -152,7 +195,7 @@ __global__ void __launch_bounds__(kGridThreads) k_densities(const DevSys *__rest
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_residual_partial(const DevSys *__restrict__ d, double *partial) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   __shared__ double red[kGridThreads / 64];
   if (S.sc->status != DS_OK) return;
   double acc = 0;
@@ -162,7 +205,7 @@ __global__ void __launch_bounds__(kGridThreads) k_residual_partial(const DevSys 
     if (!is_interior(S, x, y, z)) continue;
     const double g2 = site_residual(S, i, x, y, z);
     if (S.max_g) {
-      for (int it = 0; it < S.nit; it++) acc = fmax(acc, fabs(S.g[it * S.M + i]));
+      for (int it = 0; it < S.nit; it++) acc = fmax(acc, fabs(hbm(S.g)[it * S.M + i]));
     } else {
       acc += g2;
     }
@@ -173,7 +216,7 @@ __global__ void __launch_bounds__(kGridThreads) k_residual_partial(const DevSys 
 
 __global__ void k_finalize_residual(const DevSys *__restrict__ d, const double *partial, int nparts) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   if (S.sc->status != DS_OK) return;
   double t = 0;
   for (int b = 0; b < nparts; b++) t = S.max_g ? fmax(t, partial[b]) : t + partial[b];
@@ -181,7 +224,7 @@ __global__ void k_finalize_residual(const DevSys *__restrict__ d, const double *
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_thermo_partial(const DevSys *__restrict__ d, double *partial) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   __shared__ double red[kGridThreads / 64];
   const DevScalars &sc = *S.sc;
   double logc[kMaxMol];
@@ -207,7 +250,7 @@ __global__ void __launch_bounds__(kGridThreads) k_thermo_partial(const DevSys *_
     gp += wt * GP;
 #pragma unroll
     for (int mo = 0; mo < kMaxMol; mo++)
-      if (mo < S.nmol) th[mo] += wt * S.mol_phitot[mo * S.M + i];
+      if (mo < S.nmol) th[mo] += wt * hbm(S.mol_phitot)[mo * S.M + i];
   }
   const int nc = 2 + S.nmol;
   f = block_sum(f, red);
@@ -249,45 +292,48 @@ __global__ void k_finalize_thermo(const DevSys *__restrict__ d, const double *pa
 #define VAR_LOOP(S) for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < S.nit * S.M; v += gridDim.x * blockDim.x)
 
 __global__ void __launch_bounds__(kGridThreads) k_copy_x0(const DevSys *__restrict__ d) {
-  const DevSys &S = *d;
-  VAR_LOOP(S) S.x0[v] = S.x[v];
+  CSys &S = uniform_sys(*d);
+  VAR_LOOP(S) hbm(S.x0)[v] = hbm(S.x)[v];
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_axpy_first(const DevSys *__restrict__ d) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   const double dmax = S.deltamax;
   VAR_LOOP(S) {
-    const double xv = S.x[v] + (-dmax) * S.g[v];
-    S.x[v] = xv;
-    S.xx0[v] = xv - S.x0[v];
-    S.xR[v] = xv;
+    const double xo = hbm(S.x)[v];
+    const double xv = xo + (-dmax) * hbm(S.g)[v];
+    hbm(S.x)[v] = xv;
+    hbm(S.xx0)[v] = xv - xo;
+    hbm(S.xR)[v] = xv;
   }
 }
 
-// x -= dmax g; xR[k] = x; xx0[k] = x - x0; partial dots <xx0[posi_i], xx0[k]>
+// x -= dmax g; xR[k] = x; xx0[k] = x_new - x_old; partial dots <xx0[posi_i], xx0[k]>.
+// The residual never changes x, so x_old is the iteration's x0.
 __global__ void __launch_bounds__(kGridThreads) k_diis_step(const DevSys *__restrict__ d, int k, int kd,
                                                              double *partial) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   __shared__ double red[kGridThreads / 64];
   if (S.sc->status != DS_OK) return;
   const int nvar = S.nit * S.M, m = S.m;
   const double dmax = S.deltamax;
-  double *xRk = S.xR + (size_t)k * nvar, *xxk = S.xx0 + (size_t)k * nvar;
+  gdbl *xRk = hbm(S.xR) + (size_t)k * nvar, *xxk = hbm(S.xx0) + (size_t)k * nvar;
   double acc[kMaxDIIS];
 #pragma unroll
   for (int i = 0; i < kMaxDIIS; i++) acc[i] = 0;
   VAR_LOOP(S) {
-    const double xv = S.x[v] + (-dmax) * S.g[v];
-    S.x[v] = xv;
+    const double xo = hbm(S.x)[v];
+    const double xv = xo + (-dmax) * hbm(S.g)[v];
+    hbm(S.x)[v] = xv;
     xRk[v] = xv;
-    const double e = xv - S.x0[v];
+    const double e = xv - xo;
     xxk[v] = e;
 #pragma unroll
     for (int i = 0; i < kMaxDIIS; i++) {
       if (i < kd) {
         int posi = k - kd + 1 + i;
         if (posi < 0) posi += m;
-        const double a = posi == k ? e : S.xx0[(size_t)posi * nvar + v];
+        const double a = posi == k ? e : hbm(S.xx0)[(size_t)posi * nvar + v];
         acc[i] += a * e;
       }
     }
@@ -305,7 +351,7 @@ __global__ void k_diis_solve(const DevSys *__restrict__ d, const double *partial
   __shared__ double dots[kMaxDIIS];
   __shared__ double Ap[kMaxDIIS * kMaxDIIS];
   __shared__ int st;
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   DevScalars &sc = *S.sc;
   if (threadIdx.x == 0) {
     st = sc.status;
@@ -325,19 +371,19 @@ __global__ void k_diis_solve(const DevSys *__restrict__ d, const double *partial
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_diis_combine(const DevSys *__restrict__ d, int k, int kd) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   if (S.sc->status != DS_OK) return;
   const int nvar = S.nit * S.M, m = S.m;
   VAR_LOOP(S) {
     int posi = k - kd + 1;
     if (posi < 0) posi += m;
-    double xv = 0.0 + S.sc->C[0] * S.xR[(size_t)posi * nvar + v];
+    double xv = 0.0 + S.sc->C[0] * hbm(S.xR)[(size_t)posi * nvar + v];
     for (int i = 1; i < kd; i++) {
       posi = k - kd + 1 + i;
       if (posi < 0) posi += m;
-      xv += S.sc->C[i] * S.xR[(size_t)posi * nvar + v];
+      xv += S.sc->C[i] * hbm(S.xR)[(size_t)posi * nvar + v];
     }
-    S.x[v] = xv;
+    hbm(S.x)[v] = xv;
   }
 }
 
@@ -361,36 +407,38 @@ __global__ void k_iter_begin(const DevSys *__restrict__ d) {
   }
 }
 
-__device__ __forceinline__ void diis_indices(const DevSys &S, int &k, int &kd, int &k_diis) {
+template <typename Sys>
+__device__ __forceinline__ void diis_indices(const Sys &S, int &k, int &kd, int &k_diis) {
   k = S.sc->iterations % S.m;
   k_diis = S.sc->k_diis;
   kd = k_diis > S.m ? S.m : k_diis;
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_diis_step_dev(const DevSys *__restrict__ d, double *partial) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   __shared__ double red[kGridThreads / 64];
   if (S.sc->status != DS_OK) return;
   int k, kd, k_diis;
   diis_indices(S, k, kd, k_diis);
   const int nvar = S.nit * S.M, m = S.m;
   const double dmax = S.deltamax;
-  double *xRk = S.xR + (size_t)k * nvar, *xxk = S.xx0 + (size_t)k * nvar;
+  gdbl *xRk = hbm(S.xR) + (size_t)k * nvar, *xxk = hbm(S.xx0) + (size_t)k * nvar;
   double acc[kMaxDIIS];
 #pragma unroll
   for (int i = 0; i < kMaxDIIS; i++) acc[i] = 0;
   VAR_LOOP(S) {
-    const double xv = S.x[v] + (-dmax) * S.g[v];
-    S.x[v] = xv;
+    const double xo = hbm(S.x)[v];
+    const double xv = xo + (-dmax) * hbm(S.g)[v];
+    hbm(S.x)[v] = xv;
     xRk[v] = xv;
-    const double e = xv - S.x0[v];
+    const double e = xv - xo;
     xxk[v] = e;
 #pragma unroll
     for (int i = 0; i < kMaxDIIS; i++) {
       if (i < kd) {
         int posi = k - kd + 1 + i;
         if (posi < 0) posi += m;
-        const double a = posi == k ? e : S.xx0[(size_t)posi * nvar + v];
+        const double a = posi == k ? e : hbm(S.xx0)[(size_t)posi * nvar + v];
         acc[i] += a * e;
       }
     }
@@ -408,7 +456,7 @@ __global__ void k_diis_solve_dev(const DevSys *__restrict__ d, const double *par
   __shared__ double dots[kMaxDIIS];
   __shared__ double Ap[kMaxDIIS * kMaxDIIS];
   __shared__ int st, kd_s;
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   DevScalars &sc = *S.sc;
   if (threadIdx.x == 0) {
     st = sc.status;
@@ -431,7 +479,7 @@ __global__ void k_diis_solve_dev(const DevSys *__restrict__ d, const double *par
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_diis_combine_dev(const DevSys *__restrict__ d) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   if (S.sc->status != DS_OK) return;
   int k, kd, k_diis;
   diis_indices(S, k, kd, k_diis);
@@ -439,13 +487,13 @@ __global__ void __launch_bounds__(kGridThreads) k_diis_combine_dev(const DevSys 
   VAR_LOOP(S) {
     int posi = k - kd + 1;
     if (posi < 0) posi += m;
-    double xv = 0.0 + S.sc->C[0] * S.xR[(size_t)posi * nvar + v];
+    double xv = 0.0 + S.sc->C[0] * hbm(S.xR)[(size_t)posi * nvar + v];
     for (int i = 1; i < kd; i++) {
       posi = k - kd + 1 + i;
       if (posi < 0) posi += m;
-      xv += S.sc->C[i] * S.xR[(size_t)posi * nvar + v];
+      xv += S.sc->C[i] * hbm(S.xR)[(size_t)posi * nvar + v];
     }
-    S.x[v] = xv;
+    hbm(S.x)[v] = xv;
   }
 }
 
@@ -458,6 +506,7 @@ hipError_t grid_residual(const DevSys &h, DevSys *d, double *partial, int nb, hi
 
 hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
   const dim3 B(kGridThreads), G(nb);
+  const dim3 G3((h.MY * h.MZ + kGridThreads - 1) / kGridThreads, h.MX);
   hipLaunchKernelGGL(k_prepare, G, B, 0, st, d);
   for (int mo = 0; mo < h.nmol; mo++) {
     if (h.mol_markov[mo] != 1) return hipErrorInvalidValue;
@@ -466,7 +515,10 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
     int s = 1;
     for (int b = b0; b < b1; b++)
       for (int j = (b == b0 ? 1 : 0); j < h.block_n[b]; j++, s++)
-        hipLaunchKernelGGL(k_step_fwd, G, B, 0, st, d, s, h.block_seg[b]);
+        if (h.gradients == 3)
+          hipLaunchKernelGGL(k_step3_fwd, G3, B, 0, st, d, s, h.block_seg[b]);
+        else
+          hipLaunchKernelGGL(k_step_fwd, G, B, 0, st, d, s, h.block_seg[b]);
     hipLaunchKernelGGL(k_gn_partial, G, B, 0, st, d, N - 1, partial);
     hipLaunchKernelGGL(k_finalize_gn, dim3(1), dim3(64), 0, st, d, partial, nb, mo);
     hipLaunchKernelGGL(k_bwd_first, G, B, 0, st, d, N - 1, h.block_seg[b1 - 1], h.block_mm[b1 - 1]);
@@ -474,7 +526,10 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
     int src = 0;
     for (int b = b1 - 1; b >= b0; b--)
       for (int j = (b == b1 - 1 ? h.block_n[b] - 2 : h.block_n[b] - 1); j >= 0; j--, s--) {
-        hipLaunchKernelGGL(k_step_bwd, G, B, 0, st, d, s, h.block_seg[b], h.block_mm[b], src);
+        if (h.gradients == 3)
+          hipLaunchKernelGGL(k_step3_bwd, G3, B, 0, st, d, s, h.block_seg[b], h.block_mm[b], src);
+        else
+          hipLaunchKernelGGL(k_step_bwd, G, B, 0, st, d, s, h.block_seg[b], h.block_mm[b], src);
         src = 1 - src;
       }
   }
@@ -502,7 +557,6 @@ hipError_t grid_init(DevSys *d, hipStream_t st) {
 // captured once into a hipGraph.
 hipError_t grid_iteration(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
   hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(1), 0, st, d);
-  hipLaunchKernelGGL(k_copy_x0, dim3(nb), dim3(kGridThreads), 0, st, d);
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, st, d);
   hipError_t e = grid_residual_body(h, d, partial, nb, st);
   if (e != hipSuccess) return e;
@@ -542,7 +596,7 @@ hipError_t grid_diis_combine(const DevSys &h, DevSys *d, int nb, int k, int k_di
 namespace {
 __global__ void __launch_bounds__(kGridThreads) k_chain_step(const DevSys *__restrict__ d, const double *G1,
                                                              const double *prev, double *out) {
-  const DevSys &S = *d;
+  CSys &S = uniform_sys(*d);
   if (S.gradients == 1) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < S.M; i += gridDim.x * blockDim.x)
       out[i] = stencil_prop<double>(S, prev, i, i, 0, 0) * G1[i];

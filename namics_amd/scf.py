@@ -32,7 +32,8 @@ class ScfError(RuntimeError):
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("residual", ctypes.c_double),
-                ("kernel_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+                ("kernel_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
+                ("fp32_iterations", ctypes.c_int)]
 
 
 class Results(ctypes.Structure):
@@ -81,6 +82,7 @@ def lib():
         L.scf_get_profiles.argtypes = [vp, _D, _D]
         L.scf_propagate_chain.argtypes = [vp, _D, ctypes.c_int, _D]
         L.scf_get_phase_ms.argtypes = [vp, _D]
+        L.scf_system_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.scf_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -184,6 +186,11 @@ class ScfSystem:
         out["mol_phi"] = {m: mphi[i] for i, m in enumerate(self.mol_names)}
         return out
 
+    def set_precision(self, mixed, switch_residual=1e-5):
+        """FP64 throughout (mixed=False) or FP32 propagators until the
+        residual falls below `switch_residual`, then FP64 (mixed=True)."""
+        _check(lib().scf_system_set_precision(self.h, 1 if mixed else 0, float(switch_residual)))
+
     def phase_ms(self):
         """Device time of the last fused residual / solve by phase (ms):
         prepare, chains, densities+residual, rest (DIIS, thermodynamics)."""
@@ -218,6 +225,16 @@ def solve_batch(systems, xs=None):
     infos = (SolveInfo * n)()
     _check(lib().scf_solve_batch(hs, n, ps, infos))
     return xs, [(i.iterations, i.residual, i.status) for i in infos]
+
+
+def solve_resident_info(systems):
+    """Solve from device-resident fields in one launch; returns one dict per
+    system with every field of scf_solve_info."""
+    n = len(systems)
+    hs = (ctypes.c_void_p * n)(*[s.h.value for s in systems])
+    infos = (SolveInfo * n)()
+    _check(lib().scf_solve_resident(hs, n, infos))
+    return [{f: getattr(i, f) for f, _ in SolveInfo._fields_} for i in infos]
 
 
 def solve_resident(systems, with_solve_ms=False):

@@ -162,6 +162,54 @@ mol : water : freedom : solvent
         eval=[(8, 0.05), (9, 0.5)],
         converged=[dict(method="DIIS", tol=1e-10, seed=10, amp=0.05)],
     ),
+    # 2-gradient cylindrical brush on a rod, radial size a multiple of 8 (the
+    # register-strip propagator path)
+    "brush2d_rod_strip": dict(
+        text="""
+lat : cyl : gradients : 2
+lat : cyl : geometry : cylindrical
+lat : cyl : n_layers_x : 16
+lat : cyl : n_layers_y : 24
+lat : cyl : lattice_type : simple_cubic
+mon : S : freedom : frozen
+mon : S : frozen_range : 1,1;1,24
+mon : X : freedom : pinned
+mon : X : pinned_range : 2,1;2,24
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.8
+mon : S : chi_A : -0.3
+mol : brush : composition : (X)1(A)39
+mol : brush : freedom : restricted
+mol : brush : n : 7.5398223686
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(21, 0.05), (22, 0.4)],
+        converged=[dict(method="DIIS", tol=1e-10, seed=23, amp=0.05)],
+    ),
+    # 2-gradient cylindrical, hexagonal lattice, homopolymer near a wall
+    "hex2d_cyl_strip": dict(
+        text="""
+lat : cyl : gradients : 2
+lat : cyl : geometry : cylindrical
+lat : cyl : n_layers_x : 8
+lat : cyl : n_layers_y : 12
+lat : cyl : lattice_type : hexagonal
+mon : S : freedom : frozen
+mon : S : frozen_range : 8,1;8,12
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -0.8
+mon : A : chi_W : 0.6
+mol : pol : composition : (A)25
+mol : pol : freedom : free
+mol : pol : phibulk : 0.01
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(24, 0.3)],
+    ),
     # 2-gradient planar, frozen particle, periodic y
     "planar2d_particle": dict(
         text="""

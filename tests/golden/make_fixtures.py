@@ -16,7 +16,8 @@ Two kinds of fixture:
   DIIS elsewhere) or to the path tolerance 1e-7 with DIIS, and the iteration
   count plus the converged values are recorded.
 
-Usage: python tests/golden/make_fixtures.py   (writes tests/golden/*.json)
+Usage: python tests/golden/make_fixtures.py [case ...]   (writes tests/golden/<case>.json;
+all cases when none is named)
 """
 import json
 import os
@@ -176,7 +177,10 @@ def converged_fixture(case, method, tol, itlim=5000, seed=None, amp=0.0):
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
+    only = set(sys.argv[1:])
     for name, case in CASES.items():
+        if only and name not in only:
+            continue
         fx = {"case": name, "text": case["text"], "evaluation": [], "converged": []}
         for seed, amp in case.get("eval", [(1, 0.3)]):
             fx["evaluation"].append(evaluation_fixture(case, seed, amp))

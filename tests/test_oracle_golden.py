@@ -11,9 +11,9 @@ import pytest
 from tests.helpers import rel_err
 from tests.oracle_lib import Oracle
 
-CASES = ["cfg1_planar_adsorption", "planar_markov2", "cyl1d_micelle", "sph1d_particle",
-         "hex_periodic_restricted", "cyl1d_markov2", "brush2d_cyl", "planar2d_particle",
-         "diblock3d_periodic", "particle3d_mirror"]
+from tests.golden.cases import CASES as _ALL
+
+CASES = list(_ALL)
 
 EVAL_TOL = 1e-12
 
