@@ -7,13 +7,17 @@ chi(A,W) = 0.0, 0.5, 0.8 in sequence (each solve starts from the previous
 one's fields, as the reference chains `start` blocks; the first starts from
 seeded uniform random fields in [-0.05, 0.05]), DIIS to ||g|| <= 1e-7, FP64.
 
-One step = the whole job (three solves).  The timed region holds K steps
-with fields and state already resident in HBM.  Prints one JSON line with
-solves/sec, per-solve p50/p99 latency, the fused kernel's algorithmic
-propagator bandwidth against HBM peak, and the reference CPU program timed
-on this host as the baseline.
+One job = those three solves.  One step = J independent jobs (one seed
+each; J defaults to the number of CUs), run as three launches of J
+workgroups (one per chi), with fields and solver state resident in HBM.
+The timed region holds K steps.  Prints one JSON line: converged solves/sec
+(FP64), per-solve p50/p99 device latency, the chi = 0.8 launch's algorithmic
+propagator bandwidth against HBM peak, the same jobs with FP32 propagators
+and FP64 refinement, one job alone, and the reference CPU program timed on
+this host on one job as the baseline.
 
-    python bench.py [--steps K] [--warmup W] [--workload brush|batch1d] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--jobs J] [--workload brush|batch1d]
+                    [--no-cpu] [--no-mixed] [--switch R]
 """
 import argparse
 import json
@@ -83,47 +87,59 @@ def cpu_baseline_brush(x0, names, dims, budget_s=60.0):
                 free_energy=[float(rw["sys_NN_free_energy"]) for rw in rows])
 
 
-def run_brush(args):
-    import torch
-    from namics_amd import ScfSystem, solve_resident
+def _job_systems(J):
+    """Per chi, J brush systems; plus J seed holders with the seeded fields."""
+    from namics_amd import ScfSystem
     from namics_amd.workloads import brush_cylinder, uniform_fields
+    systems = [[ScfSystem(brush_cylinder(c)) for _ in range(J)] for c in CHIS]
+    seeds = []
+    for j in range(J):
+        sd = ScfSystem(brush_cylinder(CHIS[0]))
+        sd.upload(uniform_fields(sd.nvar, SEED + j, 0.05))
+        seeds.append(sd)
+    return systems, seeds
 
-    systems = [ScfSystem(brush_cylinder(c)) for c in CHIS]
-    seed_sys = ScfSystem(brush_cylinder(CHIS[0]))
-    x0 = uniform_fields(seed_sys.nvar, SEED, 0.05)
-    seed_sys.upload(x0)
-    torch.cuda.synchronize()
 
-    def step(record):
-        systems[0].carry_from(seed_sys)
+def _run_jobs(systems, seeds, steps, warmup):
+    """steps x (three launches of J workgroups, chi chained); returns the
+    per-step records and the timed seconds."""
+    import torch
+    from namics_amd import solve_resident_info
+
+    def step():
         out = []
-        for i, s in enumerate(systems):
-            if i > 0:
-                s.carry_from(systems[i - 1])
-            it, res, st, kms = solve_resident([s])[0]
-            out.append((it, res, st, kms))
-        record.append(out)
+        for c, row in enumerate(systems):
+            for j, s in enumerate(row):
+                s.carry_from(seeds[j] if c == 0 else systems[c - 1][j])
+            out.append(solve_resident_info(row))
+        return out
 
-    warm = []
-    for _ in range(args.warmup):
-        step(warm)
-    rec = []
+    for _ in range(warmup):
+        step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    lat = []
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        step(rec)
-        lat.append(time.perf_counter() - ts)
+    rec = [step() for _ in range(steps)]
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    return rec, time.perf_counter() - t0
 
-    solves = [s for stp in rec for s in stp]
-    converged = sum(1 for s in solves if s[2] == 0)
-    kernel_ms = [s[3] for s in solves]
-    nbytes = [propagator_bytes(systems[i].desc, rec[k][i][0]) for k in range(len(rec)) for i in range(3)]
-    achieved = sum(nbytes) / (sum(kernel_ms) * 1e-3) / 1e9
-    results = [s.results() for s in systems]
+
+def run_brush(args):
+    import torch
+    J = args.jobs if args.jobs > 0 else torch.cuda.get_device_properties(0).multi_processor_count
+    systems, seeds = _job_systems(J)
+    desc = systems[0][0].desc
+    rec, elapsed = _run_jobs(systems, seeds, args.steps, args.warmup)
+    solves = [i for stp in rec for launch in stp for i in launch]
+    converged = sum(1 for i in solves if i["status"] == 0)
+    launch_ms = [launch[0]["kernel_ms"] for stp in rec for launch in stp]
+    nbytes = [sum(propagator_bytes(desc, i["iterations"]) for i in launch) for stp in rec for launch in stp]
+    # the dominant kernel: the chi = 0.8 launches
+    dom = [(b, ms) for k, (b, ms) in enumerate(zip(nbytes, launch_ms)) if k % len(CHIS) == len(CHIS) - 1]
+    dom_bytes = float(np.mean([b for b, _ in dom]))
+    dom_ms = float(np.mean([ms for _, ms in dom]))
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    solve_ms = [i["solve_ms"] for i in solves]
+    F64 = [[s.results()["free_energy"] for s in row] for row in systems]
     out = {
         "metric": "converged SCF solves/sec + propagator HBM GB/s at 1e-7 residual; p50/p99 latency",
         "value": converged / elapsed,
@@ -136,39 +152,73 @@ def run_brush(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded uniform random initial fields)",
+        "data": "synthetic (seeded uniform random initial fields, one seed per job)",
         "config": {"workload": "2-gradient cylindrical brush 64x128, (X)1(A)199 grafted 0.05/area on a rod, "
-                               "chi 0/0.5/0.8 chained, DIIS 1e-7, FP64",
-                   "solves_per_step": 3, "seed": SEED},
-        "latency_ms": {"p50": float(np.percentile(kernel_ms, 50)), "p99": float(np.percentile(kernel_ms, 99)),
-                       "job_p50": 1e3 * float(np.percentile(lat, 50)), "job_p99": 1e3 * float(np.percentile(lat, 99))},
-        "iterations": [s[0] for s in rec[-1]],
+                               "chi 0/0.5/0.8 chained (each solve starts from the previous one's fields), "
+                               "DIIS 1e-7, FP64",
+                   "jobs_per_step": J, "solves_per_step": J * len(CHIS), "seeds": [SEED, SEED + J - 1]},
+        "latency_ms": {"p50": float(np.percentile(solve_ms, 50)), "p99": float(np.percentile(solve_ms, 99)),
+                       "what": "device time of one solve inside the batched launch"},
+        "iterations_p50_p99": [float(np.percentile([i["iterations"] for i in solves], q)) for q in (50, 99)],
         "converged_fraction": converged / max(1, len(solves)),
-        "free_energy": [r["free_energy"] for r in results],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fused", "avg_launch_ms": float(np.mean(kernel_ms)),
-                     "algorithmic_bytes_per_launch": float(np.mean(nbytes))},
+                     "kernel": "k_fused (chi = 0.8 launch)", "avg_launch_ms": dom_ms,
+                     "algorithmic_bytes_per_launch": dom_bytes},
     }
+    # the same jobs with FP32 propagators and FP64 refinement
+    if not args.no_mixed:
+        for row in systems:
+            for s in row:
+                s.set_precision(True, args.switch)
+        recm, elm = _run_jobs(systems, seeds, args.steps, args.warmup)
+        sm = [i for stp in recm for launch in stp for i in launch]
+        convm = sum(1 for i in sm if i["status"] == 0)
+        Fm = [[s.results()["free_energy"] for s in row] for row in systems]
+        dF = max(abs(a - b) / abs(b) for ra, rb in zip(Fm, F64) for a, b in zip(ra, rb))
+        out["mixed_precision"] = {
+            "value": convm / elm, "unit": "solves/s", "ms_per_step": 1e3 * elm / args.steps,
+            "converged_fraction": convm / max(1, len(sm)), "switch_residual": args.switch,
+            "fp32_iteration_fraction": sum(i["fp32_iterations"] for i in sm) / max(1, sum(i["iterations"] for i in sm)),
+            "max_rel_free_energy_diff_vs_fp64": dF,
+            "latency_ms": {"p50": float(np.percentile([i["solve_ms"] for i in sm], 50)),
+                           "p99": float(np.percentile([i["solve_ms"] for i in sm], 99))}}
+        for row in systems:
+            for s in row:
+                s.set_precision(False)
+    # one job alone (the latency of the chained job on an otherwise idle GPU)
+    one, el1 = _run_jobs([[row[0]] for row in systems], seeds[:1], 1, 0)
+    out["single_job"] = {"ms": 1e3 * el1, "solves_per_s": sum(1 for launch in one[0] for i in launch
+                                                                if i["status"] == 0) / el1,
+                         "iterations": [launch[0]["iterations"] for launch in one[0]]}
     if not args.no_cpu:
-        cb = cpu_baseline_brush(x0, seed_sys.field_names, seed_sys.dims)
+        x0 = seeds[0].download()
+        cb = cpu_baseline_brush(x0, seeds[0].field_names, seeds[0].dims)
         if cb is not None:
             out["cpu_baseline"] = {"value": cb["converged"] / cb["seconds"], "unit": "solves/s", "cores": 1,
                                    "kind": "reference",
-                                   "sample": "the full job (3 chained solves) by the reference program built from "
-                                             "its sources, one core, %.1f s; iterations %s" % (cb["seconds"],
-                                                                                               cb["iterations"]),
+                                   "sample": "one job (3 chained solves, seed %d) by the reference program built "
+                                             "from its sources, one core, %.1f s; iterations %s"
+                                             % (SEED, cb["seconds"], cb["iterations"]),
                                    "free_energy": cb["free_energy"]}
+    for row in systems:
+        for s in row:
+            s.close()
+    for s in seeds:
+        s.close()
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="brush")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--jobs", type=int, default=0, help="independent brush jobs per step (0: one per CU)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision rerun")
+    ap.add_argument("--switch", type=float, default=1e-4, help="mixed precision: FP64 below this residual")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

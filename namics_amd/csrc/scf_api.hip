@@ -331,7 +331,7 @@ int build(scf_system *S) {
   h.use_lds = rows <= lds_limit() ? 1 : 0;
   S->lds_bytes = h.use_lds ? rows : 0;
   h.precision = SCF_PREC_FP64;
-  h.mixed_switch = 1e-5;
+  h.mixed_switch = 1e-4;
   {
     // weighted interior volume (Lattice::WeightedSum of ones)
     double vol = 0;

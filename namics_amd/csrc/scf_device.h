@@ -46,6 +46,8 @@ struct DevScalars {
   double lnshift[kMaxMol];            // FP32 propagators: gauge shift c per molecule (G1' = G1 e^c)
   int fp32;                           // 1 while the solve evaluates propagators in FP32
   int fp32_iterations;                // iterations of the last solve done with FP32 propagators
+  int mixed_since, mixed_restart;     // evaluations since the best FP32 residual; DIIS restart flag
+  double mixed_best;                  // best residual of the FP32 phase
 };
 
 struct DevSys {

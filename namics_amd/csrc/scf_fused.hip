@@ -341,8 +341,8 @@ __device__ __noinline__ void chain_markov1(const DevSys &S0, int mo) {
 // (x0+k, y) of thread t, so the y-1 / y+1 neighbours of a column are the
 // adjacent slots (unit stride across the lanes of a wave, constant offsets
 // for the unrolled columns) and ghost sites are never stored.  Gf rows in HBM
-// are thread-major (the SX values of a thread contiguous), so a thread
-// writes and reads its strip with vector accesses.
+// use the same layout with the dense column stride nact, so each store or
+// load instruction of a wave covers one contiguous span.
 //
 // The 9-point weights of LGrad2::propagate (src/LGrad2.cpp:1058-1073) and of
 // the planar full stencil (src/LG2Planar.cpp:1085-1096) factor into an axial
@@ -418,7 +418,7 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shif
   const int N = S.mol_N[mo];
   const size_t M = S.M;
   const size_t RG = (size_t)SX * nact;  // Gf row length (interior sites)
-  GT *Gf = (GT *)S.Gf + (size_t)tt * SX;
+  GT *Gf = (GT *)S.Gf + tt;
   const gdbl *G1 = hbm(S.G1);
   LT *ra = (LT *)lds_row(0), *rb = ra + SX * STRIDE;
   LT *lam = (LT *)s_lam;
@@ -442,15 +442,15 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shif
   {                                                                               \
     _Pragma("unroll") for (int k = 0; k < SX; k++) (row)[k * STRIDE + tt] = v[k]; \
   }
-#define STORE_GF(s_)                                                 \
-  {                                                                  \
-    GT *q_ = Gf + (size_t)(s_) * RG;                                 \
-    _Pragma("unroll") for (int k = 0; k < SX; k++) q_[k] = v[k];     \
+#define STORE_GF(s_)                                                                \
+  {                                                                                 \
+    GT *q_ = Gf + (size_t)(s_) * RG;                                                \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) q_[(size_t)k * nact] = v[k];     \
   }
-#define LOAD_GF(s_)                                                  \
-  {                                                                  \
-    const GT *q_ = Gf + (size_t)(s_) * RG;                           \
-    _Pragma("unroll") for (int k = 0; k < SX; k++) gfv[k] = q_[k];   \
+#define LOAD_GF(s_)                                                                 \
+  {                                                                                 \
+    const GT *q_ = Gf + (size_t)(s_) * RG;                                          \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) gfv[k] = q_[(size_t)k * nact];   \
   }
 
   // ---- forward
@@ -823,6 +823,8 @@ __device__ __noinline__ void fused_thermo(const DevSys &S0) {
 }
 
 // ------------------------------------------------------------- DIIS solve
+constexpr int kMixedPatience = 25;  // FP32 evaluations without a new best residual before FP64
+
 // SFNewton::iterate_DIIS (src/sfnewton.cpp:1037-1130).  The residual never
 // changes x, so the iteration's x0 is x itself at the update: one pass per
 // iteration reads x and g, writes x, xR_k and e_k = x_new - x0, and
@@ -830,27 +832,140 @@ __device__ __noinline__ void fused_thermo(const DevSys &S0) {
 // in index order, then the waves' sums are added in wave order, as for a
 // single dot); a second pass forms x = sum_i c_i xR_i.
 __shared__ double s_dred[kMaxDIIS * 16];  // per-wave partial dots
+__shared__ double s_coef[kMaxDIIS];        // DIIS coefficients of the combine
 
 // Mixed precision: after an evaluation with residual res, decide whether the
 // next ones use FP32 propagators (while res is above the switch level) and
 // set each molecule's gauge shift c = -ln(GN / V) / N from the latest GN, so
 // that the shifted end-point distributions average about one per site.
-__device__ void mixed_update(const DevSys &S, double res, bool first) {
+__device__ __noinline__ void mixed_update(const DevSys &S, double res, bool first) {
   if (threadIdx.x == 0) {
     DevScalars &sc = *S.sc;
-    // FP32 from the first evaluation on, and never again once the solve
-    // has gone over to FP64
-    bool ok = res > S.mixed_switch && (first || sc.fp32 != 0);
+    // FP32 from the first evaluation on while the residual is above the
+    // switch level and still falling (best value improved within the last
+    // kMixedPatience evaluations); never again once the solve has gone over
+    // to FP64.  At the switch the DIIS history restarts, so the FP64
+    // iterations combine FP64 residuals only.
+    if (first) {
+      sc.mixed_best = res;
+      sc.mixed_since = 0;
+    } else if (res < sc.mixed_best) {
+      sc.mixed_best = res;
+      sc.mixed_since = 0;
+    } else {
+      sc.mixed_since++;
+    }
+    bool ok = res > S.mixed_switch && (first || sc.fp32 != 0) && sc.mixed_since < kMixedPatience;
     for (int mo = 0; mo < S.nmol; mo++) {
       const double gn = sc.GN[mo];
       if (!(gn > 0) || !isfinite(gn)) ok = false;
       else sc.lnshift[mo] = -log(gn / S.volume) / S.mol_N[mo];
     }
+    if (!ok && sc.fp32) sc.mixed_restart = 1;
     sc.fp32 = ok ? 1 : 0;
   }
   __syncthreads();
 }
 
+// One DIIS update (iteration `it`, ring slot k, kd vectors in use): x -= dmax
+// g, xR_k = x, e_k = x_new - x_old, and the kd Gram dots <e_i, e_k> into
+// dots[0..kd-1] (each thread sums its entries in index order, the waves'
+// sums are added in wave order).
+__device__ __noinline__ void diis_update_dots(const DevSys &S0, int k, int kd, double *dots) {
+  CSys &S = uniform_sys(S0);
+  k = __builtin_amdgcn_readfirstlane(k);
+  kd = __builtin_amdgcn_readfirstlane(kd);
+  const int nvar = S.nit * S.M, m = S.m;
+  const int tid = work_tid(S), nt = S.threads;
+  const double dmax = S.deltamax;
+  gdbl *x = hbm(S.x), *xR = hbm(S.xR), *xx0 = hbm(S.xx0);
+  const gdbl *g = hbm(S.g);
+  ldbl *dred = (ldbl *)s_dred;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  gdbl *xRk = xR + (size_t)k * nvar, *xxk = xx0 + (size_t)k * nvar;
+  // e vectors of the earlier kd-1 iterations, oldest first
+  const gdbl *ei[kMaxDIIS - 1];
+#pragma unroll
+  for (int i = 0; i < kMaxDIIS - 1; i++) {
+    int posi = k - kd + 1 + i;
+    if (posi < 0) posi += m;
+    ei[i] = xx0 + (size_t)(i < kd - 1 ? posi : 0) * nvar;
+  }
+  double acc[kMaxDIIS - 1], self = 0.0;
+#pragma unroll
+  for (int i = 0; i < kMaxDIIS - 1; i++) acc[i] = 0.0;
+  for (int v = tid; v < nvar; v += nt) {
+    double a[kMaxDIIS - 1];
+#pragma unroll
+    for (int i = 0; i < kMaxDIIS - 1; i++) a[i] = i < kd - 1 ? ei[i][v] : 0.0;
+    const double xo = x[v];
+    const double xv = xo + (-dmax) * g[v];
+    const double e = xv - xo;
+    x[v] = xv;
+    xRk[v] = xv;
+    xxk[v] = e;
+#pragma unroll
+    for (int i = 0; i < kMaxDIIS - 1; i++)
+      if (i < kd - 1) acc[i] += a[i] * e;
+    self += e * e;
+  }
+#pragma unroll
+  for (int i = 0; i < kMaxDIIS; i++) {
+    if (i < kd) {
+      double t = i < kd - 1 ? acc[i < kMaxDIIS - 1 ? i : 0] : self;
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+      if (lane == 0) dred[i * 16 + wave] = t;
+    }
+  }
+  lds_barrier();
+  if ((int)threadIdx.x < kd) {
+    double t = 0;
+    for (int w = 0; w < nw; w++) t += dred[threadIdx.x * 16 + w];
+    dots[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+// x = sum_i c_i xR_i (Xr_times_ci src/tools_host.h:405-416); the
+// coefficients are broadcast from LDS, two entries per trip with their loads
+// issued together.
+__device__ __noinline__ void diis_combine(const DevSys &S0, int k, int kd) {
+  CSys &S = uniform_sys(S0);
+  k = __builtin_amdgcn_readfirstlane(k);
+  kd = __builtin_amdgcn_readfirstlane(kd);
+  const int nvar = S.nit * S.M, m = S.m;
+  const int tid = work_tid(S), nt = S.threads;
+  gdbl *x = hbm(S.x);
+  const gdbl *xR = hbm(S.xR);
+  ldbl *c = (ldbl *)s_coef;
+  if ((int)threadIdx.x < kd) c[threadIdx.x] = S0.sc->C[threadIdx.x];
+  lds_barrier();
+  int p0 = k - kd + 1;
+  if (p0 < 0) p0 += m;
+  for (int v0 = tid; v0 < nvar; v0 += 2 * nt) {
+    const int v1 = v0 + nt;
+    const bool has1 = v1 < nvar;
+    const gdbl *r = xR + (size_t)p0 * nvar;
+    double s0 = 0.0 + c[0] * r[v0];
+    double s1 = has1 ? 0.0 + c[0] * r[v1] : 0.0;
+    for (int i = 1; i < kd; i++) {
+      int posi = k - kd + 1 + i;
+      if (posi < 0) posi += m;
+      r = xR + (size_t)posi * nvar;
+      const double ci = c[i];
+      s0 += ci * r[v0];
+      if (has1) s1 += ci * r[v1];
+    }
+    x[v0] = s0;
+    if (has1) x[v1] = s1;
+  }
+  __syncthreads();
+}
+
+// SFNewton::iterate_DIIS (src/sfnewton.cpp:1037-1130).  The residual never
+// changes x, so the iteration's x0 is x itself at the update: one pass per
+// iteration reads x and g, writes x, xR_k and e_k = x_new - x0, and
+// accumulates all kd Gram dots; a second pass forms x = sum_i c_i xR_i.
 __device__ void fused_solve(const DevSys &S0) {
   CSys &S = uniform_sys(S0);
   double *diis_dots = s_diis;
@@ -861,20 +976,20 @@ __device__ void fused_solve(const DevSys &S0) {
   const double dmax = S.deltamax, tol = S.tol;
   const int itlim = S.itlim;
   DevScalars &sc = *S0.sc;
-  gdbl *x = hbm(S.x), *xR = hbm(S.xR), *xx0 = hbm(S.xx0);
-  const gdbl *g = hbm(S.g);
-  ldbl *dred = (ldbl *)s_dred;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double res = fused_residual(S0);
   if (res < 0) return;
   const bool mixed = S.precision != 0;
   if (mixed) mixed_update(S0, res, true);
-  for (int v = tid; v < nvar; v += nt) {
-    const double xo = x[v];
-    const double xv = xo + (-dmax) * g[v];
-    x[v] = xv;
-    xx0[v] = xv - xo;
-    xR[v] = xv;
+  {
+    gdbl *x = hbm(S.x), *xR = hbm(S.xR), *xx0 = hbm(S.xx0);
+    const gdbl *g = hbm(S.g);
+    for (int v = tid; v < nvar; v += nt) {
+      const double xo = x[v];
+      const double xv = xo + (-dmax) * g[v];
+      x[v] = xv;
+      xx0[v] = xv - xo;
+      xR[v] = xv;
+    }
   }
   int it = 0, k_diis = 0;
   __syncthreads();
@@ -896,69 +1011,18 @@ __device__ void fused_solve(const DevSys &S0) {
       res = fused_residual(S0);
     }
     if (res < 0) break;
-    if (mixed) mixed_update(S0, res, false);
+    if (mixed) {
+      mixed_update(S0, res, false);
+      if (sc.mixed_restart) {
+        k_diis = 0;
+        __syncthreads();
+        if (threadIdx.x == 0) sc.mixed_restart = 0;
+      }
+    }
     const int k = it % m;
     k_diis++;
     const int kd = k_diis > m ? m : k_diis;
-    gdbl *xRk = xR + (size_t)k * nvar, *xxk = xx0 + (size_t)k * nvar;
-    // e vectors of the earlier kd-1 iterations, oldest first
-    const gdbl *ei[kMaxDIIS - 1];
-#pragma unroll
-    for (int i = 0; i < kMaxDIIS - 1; i++) {
-      int posi = k - kd + 1 + i;
-      if (posi < 0) posi += m;
-      ei[i] = xx0 + (size_t)(i < kd - 1 ? posi : 0) * nvar;
-    }
-    double acc[kMaxDIIS - 1], self = 0.0;
-#pragma unroll
-    for (int i = 0; i < kMaxDIIS - 1; i++) acc[i] = 0.0;
-    // two entries per trip, all loads issued before the arithmetic; each
-    // accumulator still adds the thread's entries in index order
-    for (int v0 = tid; v0 < nvar; v0 += 2 * nt) {
-      const int v1 = v0 + nt;
-      const bool has1 = v1 < nvar;
-      const double xo0 = x[v0], g0 = g[v0];
-      const double xo1 = has1 ? x[v1] : 0.0, g1 = has1 ? g[v1] : 0.0;
-      double a0[kMaxDIIS - 1], a1[kMaxDIIS - 1];
-#pragma unroll
-      for (int i = 0; i < kMaxDIIS - 1; i++) {
-        a0[i] = i < kd - 1 ? ei[i][v0] : 0.0;
-        a1[i] = (i < kd - 1 && has1) ? ei[i][v1] : 0.0;
-      }
-      const double xv0 = xo0 + (-dmax) * g0, e0 = xv0 - xo0;
-      x[v0] = xv0;
-      xRk[v0] = xv0;
-      xxk[v0] = e0;
-#pragma unroll
-      for (int i = 0; i < kMaxDIIS - 1; i++)
-        if (i < kd - 1) acc[i] += a0[i] * e0;
-      self += e0 * e0;
-      if (has1) {
-        const double xv1 = xo1 + (-dmax) * g1, e1 = xv1 - xo1;
-        x[v1] = xv1;
-        xRk[v1] = xv1;
-        xxk[v1] = e1;
-#pragma unroll
-        for (int i = 0; i < kMaxDIIS - 1; i++)
-          if (i < kd - 1) acc[i] += a1[i] * e1;
-        self += e1 * e1;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < kMaxDIIS; i++) {
-      if (i < kd) {
-        double t = i < kd - 1 ? acc[i < kMaxDIIS - 1 ? i : 0] : self;
-        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-        if (lane == 0) dred[i * 16 + wave] = t;
-      }
-    }
-    lds_barrier();
-    if ((int)threadIdx.x < kd) {
-      double t = 0;
-      for (int w = 0; w < nw; w++) t += dred[threadIdx.x * 16 + w];
-      diis_dots[threadIdx.x] = t;
-    }
-    __syncthreads();
+    diis_update_dots(S0, k, kd, diis_dots);
     if (threadIdx.x == 0) diis_gram_update(sc, m, k_diis, diis_dots, diis_A);
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -967,44 +1031,7 @@ __device__ void fused_solve(const DevSys &S0) {
     }
     __syncthreads();
     if (sc.status != DS_OK) break;
-    // x = sum_i c_i xR_i (Xr_times_ci src/tools_host.h:405-416), four
-    // entries per trip with their loads issued together
-    {
-      const gdbl *C = hbm((const double *)sc.C);
-      double c[kMaxDIIS];
-      const gdbl *ri[kMaxDIIS];
-#pragma unroll
-      for (int i = 0; i < kMaxDIIS; i++) {
-        int posi = k - kd + 1 + i;
-        if (posi < 0) posi += m;
-        c[i] = i < kd ? C[i] : 0.0;
-        ri[i] = xR + (size_t)(i < kd ? posi : 0) * nvar;
-      }
-      for (int v0 = tid; v0 < nvar; v0 += 4 * nt) {
-        double sum[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int v = v0 + u * nt;
-          sum[u] = 0.0 + c[0] * (v < nvar ? ri[0][v] : 0.0);
-        }
-#pragma unroll
-        for (int i = 1; i < kMaxDIIS; i++) {
-          if (i < kd) {
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int v = v0 + u * nt;
-              sum[u] += c[i] * (v < nvar ? ri[i][v] : 0.0);
-            }
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int v = v0 + u * nt;
-          if (v < nvar) x[v] = sum[u];
-        }
-      }
-    }
-    __syncthreads();
+    diis_combine(S0, k, kd);
   }
   if (threadIdx.x == 0) {
     sc.iterations = it;
@@ -1020,6 +1047,7 @@ __global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode
     S.sc->status = DS_OK;
     S.sc->fp32 = 0;
     S.sc->fp32_iterations = 0;
+    S.sc->mixed_restart = 0;
     for (int p = 0; p < 4; p++) S.sc->t_phase[p] = 0;
     S.sc->t_begin = wall_clock64();
   }

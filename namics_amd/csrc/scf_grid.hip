@@ -37,6 +37,24 @@ __device__ __forceinline__ double block_max(double v, double *red) {
   return t;
 }
 
+// Sum (or max) of one column of per-workgroup partials by one workgroup:
+// thread t adds entries t, t + blockDim, ... in order, then the threads'
+// sums are combined by block_sum's fixed tree, so the result does not depend
+// on timing.
+__device__ __forceinline__ double ordered_sum(const double *partial, int nparts, int stride, int col,
+                                              double *red) {
+  double t = 0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) t += partial[(size_t)b * stride + col];
+  return block_sum(t, red);
+}
+
+__device__ __forceinline__ double ordered_max(const double *partial, int nparts, int stride, int col,
+                                              double *red) {
+  double t = 0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) t = fmax(t, partial[(size_t)b * stride + col]);
+  return block_max(t, red);
+}
+
 #define GRID_LOOP(S) for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n_work_sites(S); w += gridDim.x * blockDim.x)
 
 __global__ void __launch_bounds__(kGridThreads) k_prepare(const DevSys *__restrict__ d) {
@@ -165,12 +183,12 @@ __global__ void __launch_bounds__(kGridThreads) k_gn_partial(const DevSys *__res
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
-// ordered sum of nparts partial sums (one per workgroup), stride `ncols`
-__global__ void k_finalize_gn(const DevSys *__restrict__ d, const double *partial, int nparts, int mo) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double t = 0;
-  for (int b = 0; b < nparts; b++) t += partial[b];
-  d->sc->GN[mo] = t;
+// GN of molecule mo from the per-workgroup partials
+__global__ void __launch_bounds__(kGridThreads) k_finalize_gn(const DevSys *__restrict__ d, const double *partial,
+                                                               int nparts, int mo) {
+  __shared__ double red[kGridThreads / 64];
+  const double t = ordered_sum(partial, nparts, 1, 0, red);
+  if (threadIdx.x == 0) d->sc->GN[mo] = t;
 }
 
 __global__ void k_norms(const DevSys *__restrict__ d) {
@@ -214,13 +232,13 @@ __global__ void __launch_bounds__(kGridThreads) k_residual_partial(const DevSys 
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
-__global__ void k_finalize_residual(const DevSys *__restrict__ d, const double *partial, int nparts) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(kGridThreads) k_finalize_residual(const DevSys *__restrict__ d,
+                                                                     const double *partial, int nparts) {
+  __shared__ double red[kGridThreads / 64];
   CSys &S = uniform_sys(*d);
   if (S.sc->status != DS_OK) return;
-  double t = 0;
-  for (int b = 0; b < nparts; b++) t = S.max_g ? fmax(t, partial[b]) : t + partial[b];
-  S.sc->residual = S.max_g ? t : sqrt(t);
+  const double t = S.max_g ? ordered_max(partial, nparts, 1, 0, red) : ordered_sum(partial, nparts, 1, 0, red);
+  if (threadIdx.x == 0) S.sc->residual = S.max_g ? t : sqrt(t);
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_thermo_partial(const DevSys *__restrict__ d, double *partial) {
@@ -268,24 +286,23 @@ __global__ void __launch_bounds__(kGridThreads) k_thermo_partial(const DevSys *_
   }
 }
 
-__global__ void k_finalize_thermo(const DevSys *__restrict__ d, const double *partial, int nparts) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(kGridThreads) k_finalize_thermo(const DevSys *__restrict__ d,
+                                                                   const double *partial, int nparts) {
+  __shared__ double red[kGridThreads / 64];
   const DevSys &S = *d;
   DevScalars &sc = *S.sc;
   const int nc = 2 + S.nmol;
-  double f = 0, gp = 0;
-  for (int b = 0; b < nparts; b++) {
-    f += partial[b * nc];
-    gp += partial[b * nc + 1];
+  const double f = ordered_sum(partial, nparts, nc, 0, red);
+  const double gp = ordered_sum(partial, nparts, nc, 1, red);
+  if (threadIdx.x == 0) {
+    sc.F = f;
+    sc.GP = gp;
   }
-  sc.F = f;
-  sc.GP = gp;
   for (int mo = 0; mo < S.nmol; mo++) {
-    double t = 0;
-    for (int b = 0; b < nparts; b++) t += partial[b * nc + 2 + mo];
-    sc.theta[mo] = t;
+    const double t = ordered_sum(partial, nparts, nc, 2 + mo, red);
+    if (threadIdx.x == 0) sc.theta[mo] = t;
   }
-  scalars_mu(S);
+  if (threadIdx.x == 0) scalars_mu(S);
 }
 
 // ---- DIIS element-wise parts (nvar = nit * M entries, all sites)
@@ -347,27 +364,25 @@ __global__ void __launch_bounds__(kGridThreads) k_diis_step(const DevSys *__rest
   }
 }
 
-__global__ void k_diis_solve(const DevSys *__restrict__ d, const double *partial, int nparts, int k_diis, int kd) {
+__global__ void __launch_bounds__(kGridThreads) k_diis_solve(const DevSys *__restrict__ d, const double *partial,
+                                                              int nparts, int k_diis, int kd) {
   __shared__ double dots[kMaxDIIS];
   __shared__ double Ap[kMaxDIIS * kMaxDIIS];
-  __shared__ int st;
-  CSys &S = uniform_sys(*d);
+  __shared__ double red[kGridThreads / 64];
+  const DevSys &S = *d;
   DevScalars &sc = *S.sc;
-  if (threadIdx.x == 0) {
-    st = sc.status;
-    if (st == DS_OK) {
-      for (int i = 0; i < kd; i++) {
-        double t = 0;
-        for (int b = 0; b < nparts; b++) t += partial[b * kMaxDIIS + i];
-        dots[i] = t;
-      }
-      diis_gram_update(sc, S.m, k_diis, dots, Ap);
-    }
-  }
-  __syncthreads();
+  const int st = sc.status;
   if (st != DS_OK) return;
-  const bool ok = wave_diis_coefficients(Ap, kd, sc.C);
-  if (threadIdx.x == 0 && !ok) sc.status = DS_NAN;
+  for (int i = 0; i < kd; i++) {
+    const double t = ordered_sum(partial, nparts, kMaxDIIS, i, red);
+    if (threadIdx.x == 0) dots[i] = t;
+  }
+  if (threadIdx.x == 0) diis_gram_update(sc, S.m, k_diis, dots, Ap);
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const bool ok = wave_diis_coefficients(Ap, kd, sc.C);
+    if (threadIdx.x == 0 && !ok) sc.status = DS_NAN;
+  }
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_diis_combine(const DevSys *__restrict__ d, int k, int kd) {
@@ -452,30 +467,26 @@ __global__ void __launch_bounds__(kGridThreads) k_diis_step_dev(const DevSys *__
   }
 }
 
-__global__ void k_diis_solve_dev(const DevSys *__restrict__ d, const double *partial, int nparts) {
+__global__ void __launch_bounds__(kGridThreads) k_diis_solve_dev(const DevSys *__restrict__ d,
+                                                                  const double *partial, int nparts) {
   __shared__ double dots[kMaxDIIS];
   __shared__ double Ap[kMaxDIIS * kMaxDIIS];
-  __shared__ int st, kd_s;
-  CSys &S = uniform_sys(*d);
+  __shared__ double red[kGridThreads / 64];
+  const DevSys &S = *d;
   DevScalars &sc = *S.sc;
-  if (threadIdx.x == 0) {
-    st = sc.status;
-    int k, kd, k_diis;
-    diis_indices(S, k, kd, k_diis);
-    kd_s = kd;
-    if (st == DS_OK) {
-      for (int i = 0; i < kd; i++) {
-        double t = 0;
-        for (int b = 0; b < nparts; b++) t += partial[b * kMaxDIIS + i];
-        dots[i] = t;
-      }
-      diis_gram_update(sc, S.m, k_diis, dots, Ap);
-    }
+  if (sc.status != DS_OK) return;
+  int k, kd, k_diis;
+  diis_indices(S, k, kd, k_diis);
+  for (int i = 0; i < kd; i++) {
+    const double t = ordered_sum(partial, nparts, kMaxDIIS, i, red);
+    if (threadIdx.x == 0) dots[i] = t;
   }
+  if (threadIdx.x == 0) diis_gram_update(sc, S.m, k_diis, dots, Ap);
   __syncthreads();
-  if (st != DS_OK) return;
-  const bool ok = wave_diis_coefficients(Ap, kd_s, sc.C);
-  if (threadIdx.x == 0 && !ok) sc.status = DS_NAN;
+  if (threadIdx.x < 64) {
+    const bool ok = wave_diis_coefficients(Ap, kd, sc.C);
+    if (threadIdx.x == 0 && !ok) sc.status = DS_NAN;
+  }
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_diis_combine_dev(const DevSys *__restrict__ d) {
@@ -520,7 +531,7 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
         else
           hipLaunchKernelGGL(k_step_fwd, G, B, 0, st, d, s, h.block_seg[b]);
     hipLaunchKernelGGL(k_gn_partial, G, B, 0, st, d, N - 1, partial);
-    hipLaunchKernelGGL(k_finalize_gn, dim3(1), dim3(64), 0, st, d, partial, nb, mo);
+    hipLaunchKernelGGL(k_finalize_gn, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb, mo);
     hipLaunchKernelGGL(k_bwd_first, G, B, 0, st, d, N - 1, h.block_seg[b1 - 1], h.block_mm[b1 - 1]);
     s = N - 2;
     int src = 0;
@@ -536,14 +547,14 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
   hipLaunchKernelGGL(k_norms, dim3(1), dim3(64), 0, st, d);
   hipLaunchKernelGGL(k_densities, G, B, 0, st, d);
   hipLaunchKernelGGL(k_residual_partial, G, B, 0, st, d, partial);
-  hipLaunchKernelGGL(k_finalize_residual, dim3(1), dim3(64), 0, st, d, partial, nb);
+  hipLaunchKernelGGL(k_finalize_residual, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb);
   return hipGetLastError();
 }
 
 hipError_t grid_thermo(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
   (void)h;
   hipLaunchKernelGGL(k_thermo_partial, dim3(nb), dim3(kGridThreads), 0, st, d, partial);
-  hipLaunchKernelGGL(k_finalize_thermo, dim3(1), dim3(64), 0, st, d, partial, nb);
+  hipLaunchKernelGGL(k_finalize_thermo, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb);
   return hipGetLastError();
 }
 
@@ -561,7 +572,7 @@ hipError_t grid_iteration(const DevSys &h, DevSys *d, double *partial, int nb, h
   hipError_t e = grid_residual_body(h, d, partial, nb, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_diis_step_dev, dim3(nb), dim3(kGridThreads), 0, st, d, partial);
-  hipLaunchKernelGGL(k_diis_solve_dev, dim3(1), dim3(64), 0, st, d, partial, nb);
+  hipLaunchKernelGGL(k_diis_solve_dev, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb);
   hipLaunchKernelGGL(k_diis_combine_dev, dim3(nb), dim3(kGridThreads), 0, st, d);
   return hipGetLastError();
 }
@@ -582,7 +593,7 @@ hipError_t grid_diis_step(const DevSys &h, DevSys *d, double *partial, int nb, i
                           hipStream_t st) {
   const int kd = k_diis > h.m ? h.m : k_diis;
   hipLaunchKernelGGL(k_diis_step, dim3(nb), dim3(kGridThreads), 0, st, d, k, kd, partial);
-  hipLaunchKernelGGL(k_diis_solve, dim3(1), dim3(64), 0, st, d, partial, nb, k_diis, kd);
+  hipLaunchKernelGGL(k_diis_solve, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb, k_diis, kd);
   return hipGetLastError();
 }
 

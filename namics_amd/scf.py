@@ -186,7 +186,7 @@ class ScfSystem:
         out["mol_phi"] = {m: mphi[i] for i, m in enumerate(self.mol_names)}
         return out
 
-    def set_precision(self, mixed, switch_residual=1e-5):
+    def set_precision(self, mixed, switch_residual=1e-4):
         """FP64 throughout (mixed=False) or FP32 propagators until the
         residual falls below `switch_residual`, then FP64 (mixed=True)."""
         _check(lib().scf_system_set_precision(self.h, 1 if mixed else 0, float(switch_residual)))
