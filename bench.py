@@ -8,15 +8,16 @@ one's fields, as the reference chains `start` blocks; the first starts from
 seeded uniform random fields in [-0.05, 0.05]), DIIS to ||g|| <= 1e-7, FP64.
 
 One job = those three solves.  One step = J independent jobs (one seed
-each; J defaults to the number of CUs), run as three launches of J
-workgroups (one per chi), with fields and solver state resident in HBM.
-The timed region holds K steps.  Prints one JSON line: converged solves/sec
-(FP64), per-solve p50/p99 device latency, the chi = 0.8 launch's algorithmic
-propagator bandwidth against HBM peak, the same jobs with FP32 propagators
-and FP64 refinement, one job alone, and the reference CPU program timed on
-this host on one job as the baseline.
+each, J = twice the number of CUs by default) in one launch of the
+persistent job queue: one workgroup per resident slot (one slot per CU),
+each taking the next job when it finishes one, with fields, seeds and
+solver state resident in HBM.  The timed region holds K steps.  Prints one
+JSON line: converged solves/sec (FP64), per-solve p50/p99 device latency,
+the launch's algorithmic propagator bandwidth against HBM peak, the same
+jobs with FP32 propagators and FP64 refinement, one job alone, and the
+reference CPU program timed on this host on one job as the baseline.
 
-    python bench.py [--steps K] [--warmup W] [--jobs J] [--workload brush|batch1d]
+    python bench.py [--steps K] [--warmup W] [--jobs J] [--slots S] [--workload brush|batch1d]
                     [--no-cpu] [--no-mixed] [--switch R]
 """
 import argparse
@@ -87,59 +88,42 @@ def cpu_baseline_brush(x0, names, dims, budget_s=60.0):
                 free_energy=[float(rw["sys_NN_free_energy"]) for rw in rows])
 
 
-def _job_systems(J):
-    """Per chi, J brush systems; plus J seed holders with the seeded fields."""
-    from namics_amd import ScfSystem
-    from namics_amd.workloads import brush_cylinder, uniform_fields
-    systems = [[ScfSystem(brush_cylinder(c)) for _ in range(J)] for c in CHIS]
-    seeds = []
-    for j in range(J):
-        sd = ScfSystem(brush_cylinder(CHIS[0]))
-        sd.upload(uniform_fields(sd.nvar, SEED + j, 0.05))
-        seeds.append(sd)
-    return systems, seeds
-
-
-def _run_jobs(systems, seeds, steps, warmup):
-    """steps x (three launches of J workgroups, chi chained); returns the
-    per-step records and the timed seconds."""
-    import torch
-    from namics_amd import solve_resident_info
-
-    def step():
-        out = []
-        for c, row in enumerate(systems):
-            for j, s in enumerate(row):
-                s.carry_from(seeds[j] if c == 0 else systems[c - 1][j])
-            out.append(solve_resident_info(row))
-        return out
-
-    for _ in range(warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rec = [step() for _ in range(steps)]
-    torch.cuda.synchronize()
-    return rec, time.perf_counter() - t0
-
-
 def run_brush(args):
-    import torch
-    J = args.jobs if args.jobs > 0 else torch.cuda.get_device_properties(0).multi_processor_count
-    systems, seeds = _job_systems(J)
-    desc = systems[0][0].desc
-    rec, elapsed = _run_jobs(systems, seeds, args.steps, args.warmup)
-    solves = [i for stp in rec for launch in stp for i in launch]
-    converged = sum(1 for i in solves if i["status"] == 0)
-    launch_ms = [launch[0]["kernel_ms"] for stp in rec for launch in stp]
-    nbytes = [sum(propagator_bytes(desc, i["iterations"]) for i in launch) for stp in rec for launch in stp]
-    # the dominant kernel: the chi = 0.8 launches
-    dom = [(b, ms) for k, (b, ms) in enumerate(zip(nbytes, launch_ms)) if k % len(CHIS) == len(CHIS) - 1]
-    dom_bytes = float(np.mean([b for b, _ in dom]))
-    dom_ms = float(np.mean([ms for _, ms in dom]))
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    solve_ms = [i["solve_ms"] for i in solves]
-    F64 = [[s.results()["free_energy"] for s in row] for row in systems]
+    from namics_amd import DeviceBuffer, ScfSystem, device_info, solve_jobs, synchronize
+    from namics_amd.workloads import brush_cylinder, uniform_fields
+    ncu = device_info()[0]
+    nslots = args.slots if args.slots > 0 else ncu
+    J = args.jobs if args.jobs > 0 else 2 * nslots
+    t_setup = time.perf_counter()
+    slots = [[ScfSystem(brush_cylinder(c)) for c in CHIS] for _ in range(nslots)]
+    descs = [s.desc for s in slots[0]]
+    nvar0 = slots[0][0].nvar
+    seeds = np.stack([uniform_fields(nvar0, SEED + j, 0.05) for j in range(J)])
+    seeds_dev = DeviceBuffer(seeds)
+    synchronize()
+    t_setup = time.perf_counter() - t_setup
+
+    def run(steps, warmup):
+        for _ in range(warmup):
+            solve_jobs(slots, seeds_dev, J)
+        synchronize()
+        t0 = time.perf_counter()
+        rec = [solve_jobs(slots, seeds_dev, J) for _ in range(steps)]
+        synchronize()
+        return rec, time.perf_counter() - t0
+
+    def summary(rec, elapsed):
+        solves = [r for res, _ in rec for job in res for r in job]
+        conv = sum(1 for r in solves if r["status"] == 0)
+        launch_ms = float(np.mean([ms for _, ms in rec]))
+        nbytes = float(np.mean([sum(propagator_bytes(descs[c], job[c]["iterations"])
+                                    for job in res for c in range(len(CHIS))) for res, _ in rec]))
+        return solves, conv, launch_ms, nbytes
+
+    rec, elapsed = run(args.steps, args.warmup)
+    solves, converged, launch_ms, nbytes = summary(rec, elapsed)
+    achieved = nbytes / (launch_ms * 1e-3) / 1e9
+    F64 = [[r["free_energy"] for r in job] for job in rec[-1][0]]
     out = {
         "metric": "converged SCF solves/sec + propagator HBM GB/s at 1e-7 residual; p50/p99 latency",
         "value": converged / elapsed,
@@ -156,44 +140,50 @@ def run_brush(args):
         "config": {"workload": "2-gradient cylindrical brush 64x128, (X)1(A)199 grafted 0.05/area on a rod, "
                                "chi 0/0.5/0.8 chained (each solve starts from the previous one's fields), "
                                "DIIS 1e-7, FP64",
-                   "jobs_per_step": J, "solves_per_step": J * len(CHIS), "seeds": [SEED, SEED + J - 1]},
-        "latency_ms": {"p50": float(np.percentile(solve_ms, 50)), "p99": float(np.percentile(solve_ms, 99)),
-                       "what": "device time of one solve inside the batched launch"},
-        "iterations_p50_p99": [float(np.percentile([i["iterations"] for i in solves], q)) for q in (50, 99)],
+                   "jobs_per_step": J, "solves_per_step": J * len(CHIS), "seeds": [SEED, SEED + J - 1],
+                   "slots": nslots},
+        "latency_ms": {"p50": float(np.percentile([r["solve_ms"] for r in solves], 50)),
+                       "p99": float(np.percentile([r["solve_ms"] for r in solves], 99)),
+                       "what": "device time of one solve inside the persistent launch"},
+        "iterations_p50_p99": [float(np.percentile([r["iterations"] for r in solves], q)) for q in (50, 99)],
         "converged_fraction": converged / max(1, len(solves)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fused (chi = 0.8 launch)", "avg_launch_ms": dom_ms,
-                     "algorithmic_bytes_per_launch": dom_bytes},
+                     "kernel": "k_fused_jobs (one launch per step)", "avg_launch_ms": launch_ms,
+                     "algorithmic_bytes_per_launch": nbytes},
+        "setup_s": t_setup,
     }
     # the same jobs with FP32 propagators and FP64 refinement
     if not args.no_mixed:
-        for row in systems:
+        for row in slots:
             for s in row:
                 s.set_precision(True, args.switch)
-        recm, elm = _run_jobs(systems, seeds, args.steps, args.warmup)
-        sm = [i for stp in recm for launch in stp for i in launch]
-        convm = sum(1 for i in sm if i["status"] == 0)
-        Fm = [[s.results()["free_energy"] for s in row] for row in systems]
-        dF = max(abs(a - b) / abs(b) for ra, rb in zip(Fm, F64) for a, b in zip(ra, rb))
+        recm, elm = run(args.steps, args.warmup)
+        sm, convm, msm, _ = summary(recm, elm)
+        Fm = [[r["free_energy"] for r in job] for job in recm[-1][0]]
+        both = [(a, b) for ja, jb, jfa, jfb in zip(Fm, F64, recm[-1][0], rec[-1][0])
+                for a, b, ra, rb in zip(ja, jb, jfa, jfb) if ra["status"] == 0 and rb["status"] == 0]
         out["mixed_precision"] = {
             "value": convm / elm, "unit": "solves/s", "ms_per_step": 1e3 * elm / args.steps,
             "converged_fraction": convm / max(1, len(sm)), "switch_residual": args.switch,
-            "fp32_iteration_fraction": sum(i["fp32_iterations"] for i in sm) / max(1, sum(i["iterations"] for i in sm)),
-            "max_rel_free_energy_diff_vs_fp64": dF,
-            "latency_ms": {"p50": float(np.percentile([i["solve_ms"] for i in sm], 50)),
-                           "p99": float(np.percentile([i["solve_ms"] for i in sm], 99))}}
-        for row in systems:
+            "fp32_iteration_fraction": sum(r["fp32_iterations"] for r in sm) / max(1, sum(r["iterations"] for r in sm)),
+            "median_rel_free_energy_diff_vs_fp64": float(np.median([abs(a - b) / abs(b) for a, b in both]))
+            if both else None,
+            "latency_ms": {"p50": float(np.percentile([r["solve_ms"] for r in sm], 50)),
+                           "p99": float(np.percentile([r["solve_ms"] for r in sm], 99))}}
+        for row in slots:
             for s in row:
                 s.set_precision(False)
     # one job alone (the latency of the chained job on an otherwise idle GPU)
-    one, el1 = _run_jobs([[row[0]] for row in systems], seeds[:1], 1, 0)
-    out["single_job"] = {"ms": 1e3 * el1, "solves_per_s": sum(1 for launch in one[0] for i in launch
-                                                                if i["status"] == 0) / el1,
-                         "iterations": [launch[0]["iterations"] for launch in one[0]]}
+    one, el1 = None, 0.0
+    t1 = time.perf_counter()
+    one, _ = solve_jobs(slots[:1], seeds_dev, 1)
+    synchronize()
+    el1 = time.perf_counter() - t1
+    out["single_job"] = {"ms": 1e3 * el1, "solves_per_s": sum(1 for r in one[0] if r["status"] == 0) / el1,
+                         "iterations": [r["iterations"] for r in one[0]]}
     if not args.no_cpu:
-        x0 = seeds[0].download()
-        cb = cpu_baseline_brush(x0, seeds[0].field_names, seeds[0].dims)
+        cb = cpu_baseline_brush(seeds[0], slots[0][0].field_names, slots[0][0].dims)
         if cb is not None:
             out["cpu_baseline"] = {"value": cb["converged"] / cb["seconds"], "unit": "solves/s", "cores": 1,
                                    "kind": "reference",
@@ -201,22 +191,22 @@ def run_brush(args):
                                              "from its sources, one core, %.1f s; iterations %s"
                                              % (SEED, cb["seconds"], cb["iterations"]),
                                    "free_energy": cb["free_energy"]}
-    for row in systems:
+    for row in slots:
         for s in row:
             s.close()
-    for s in seeds:
-        s.close()
+    seeds_dev.close()
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="brush")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--jobs", type=int, default=0, help="independent brush jobs per step (0: one per CU)")
+    ap.add_argument("--jobs", type=int, default=0, help="independent brush jobs per step (0: twice the slots)")
+    ap.add_argument("--slots", type=int, default=0, help="resident job slots (0: one per CU)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision rerun")
     ap.add_argument("--switch", type=float, default=1e-4, help="mixed precision: FP64 below this residual")
     args = ap.parse_args()

@@ -153,6 +153,32 @@ int scf_carry_fields(scf_system *dst, const scf_system *src);
  * fused ones; no host copies of fields.  info[i] per system. */
 int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info);
 
+/* Result of one solve of a queued job (scf_solve_jobs). */
+typedef struct {
+  int iterations;
+  int status;          /* SCF_OK, SCF_NOT_CONVERGED or a negative error */
+  int fp32_iterations;
+  double residual;
+  double free_energy;
+  double grand_potential;
+  double solve_ms;     /* device time of this solve */
+} scf_job_result;
+
+/* Persistent job queue (parameter sweeps, many seeds of one problem).
+ * `slots` holds nslots x chain systems: slot s owns systems
+ * slots[s*chain .. s*chain+chain-1], all slots alike (same inputs up to
+ * parameters; every system of chain position c has the same field layout).
+ * Job j solves the chain on whichever slot takes it: position 0 from the
+ * seed fields `seeds_dev + j*nvar0` (DEVICE memory, nvar0 = fields of
+ * position 0), position c > 0 from position c-1's converged fields mapped
+ * by segment name (initial_guess : previous_result).  One launch, one
+ * workgroup per slot, jobs taken from a counter until none is left, so
+ * uneven jobs balance across the GPU.  results: njobs x chain (host).
+ * Replaces: running the reference once per parameter set / start block
+ * (src/namics.cpp:407-470 `start` loop). */
+int scf_solve_jobs(scf_system *const *slots, int nslots, int chain, const double *seeds_dev, int njobs,
+                   scf_job_result *results, double *kernel_ms);
+
 /* Scalars of the last residual / solve. */
 int scf_get_results(const scf_system *sys, scf_results *out);
 
@@ -174,6 +200,16 @@ int scf_get_phase_ms(const scf_system *sys, double *ms);
  * (src/lattice.h:133; src/LG1Planar.cpp:138-159, src/LGrad2.cpp:1053-1107,
  * src/LGrad3.cpp:673-743). */
 int scf_propagate_chain(scf_system *sys, const double *G1, int N, double *out);
+
+/* Device plumbing for callers without their own GPU runtime (the job
+ * queue's seed fields): allocate / fill / free device memory, the current
+ * device's compute-unit count and HBM size, and a device-wide
+ * synchronisation.  No counterpart in the reference (CPU only). */
+int scf_buffer_alloc(size_t bytes, void **dev);
+int scf_buffer_upload(void *dev, const void *host, size_t bytes);
+int scf_buffer_free(void *dev);
+int scf_device_info(int *compute_units, size_t *hbm_bytes);
+int scf_device_synchronize(void);
 
 /* Last error message of this thread (static storage). */
 const char *scf_last_error(void);

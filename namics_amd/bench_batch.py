@@ -59,12 +59,12 @@ def cpu_sample(draws, budget_s=25.0):
 
 
 def run_batch(args, n=512, seed=12345):
-    import torch
+    from namics_amd import synchronize
     from namics_amd import ScfSystem, solve_resident
     from namics_amd.workloads import batch_draws
     draws = batch_draws(n, seed)
     systems = [ScfSystem(batch_input(d)) for d in draws]
-    torch.cuda.synchronize()
+    synchronize()
 
     def step():
         for s in systems:
@@ -73,12 +73,12 @@ def run_batch(args, n=512, seed=12345):
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    synchronize()
     t0 = time.perf_counter()
     recs = []
     for _ in range(args.steps):
         recs.append(step())
-    torch.cuda.synchronize()
+    synchronize()
     elapsed = time.perf_counter() - t0
     flat = [r for rec in recs for r in rec]
     conv = [r for r in flat if r[2] == 0]

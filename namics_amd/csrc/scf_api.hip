@@ -573,6 +573,87 @@ int scf_carry_fields(scf_system *dst, const scf_system *src) {
   return SCF_OK;
 }
 
+// carry map of `dst` from `src` by segment name (initial_guess : previous_result)
+static int set_carry_map(scf_system *dst, const scf_system *src) {
+  bool changed = false;
+  for (size_t j = 0; j < dst->spec.itmon.size(); j++) {
+    const std::string &name = dst->spec.seg[dst->spec.itmon[j]].name;
+    int found = -1;
+    for (size_t i = 0; i < src->spec.itmon.size(); i++)
+      if (src->spec.seg[src->spec.itmon[i]].name == name) found = (int)i;
+    if (dst->h.carry_src[j] != found) changed = true;
+    dst->h.carry_src[j] = found;
+  }
+  if (changed) return upload_solver(dst);
+  return SCF_OK;
+}
+
+int scf_solve_jobs(scf_system *const *slots, int nslots, int chain, const double *seeds_dev, int njobs,
+                   scf_job_result *results, double *kernel_ms) {
+  if (!slots || nslots <= 0 || chain <= 0 || !seeds_dev || njobs < 0 || !results)
+    return fail(SCF_E_ARG, "bad argument");
+  size_t lds = 0;
+  int threads = 64;
+  for (int s = 0; s < nslots; s++)
+    for (int c = 0; c < chain; c++) {
+      scf_system *S = slots[(size_t)s * chain + c];
+      if (!S) return fail(SCF_E_ARG, "null system");
+      if (S->effective_mode() != SCF_MODE_FUSED) return fail(SCF_E_UNSUPPORTED, "job queue needs fused systems");
+      scf_system *R = slots[c];  // slot 0 fixes the layout of position c
+      if (S->h.nit != R->h.nit || S->h.M != R->h.M) return fail(SCF_E_ARG, "slots differ in field layout");
+      if (c > 0) {
+        scf_system *P = slots[(size_t)s * chain + c - 1];
+        if (P->h.M != S->h.M) return fail(SCF_E_ARG, "chained systems differ in lattice size");
+        int rc = set_carry_map(S, P);
+        if (rc != SCF_OK) return rc;
+      }
+      HIP_TRY(hipStreamSynchronize(S->stream));
+      threads = std::max(threads, S->h.threads);
+      lds = std::max(lds, S->lds_bytes);
+    }
+  if (njobs == 0) return SCF_OK;
+  scf_system *S0 = slots[0];
+  hipStream_t st = S0->stream;
+  const size_t nsys = (size_t)nslots * chain, nrec = (size_t)njobs * chain;
+  const size_t bytes = 256 + nsys * sizeof(DevSys *) + nrec * sizeof(JobRecord);
+  char *tmp = nullptr;
+  HIP_TRY(hipMalloc((void **)&tmp, bytes));
+  int *counter = (int *)tmp;
+  DevSys **d_slots = (DevSys **)(tmp + 256);
+  JobRecord *rec = (JobRecord *)(tmp + 256 + nsys * sizeof(DevSys *));
+  std::vector<DevSys *> list(nsys);
+  for (size_t i = 0; i < nsys; i++) list[i] = slots[i]->d;
+  int rc = SCF_OK;
+  float ms = 0.f;
+  hipError_t e = hipMemcpy(d_slots, list.data(), nsys * sizeof(DevSys *), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(counter, 0, sizeof(int));
+  if (e == hipSuccess) e = hipEventRecord(S0->ev0, st);
+  if (e == hipSuccess)
+    e = launch_fused_jobs(d_slots, nslots, chain, seeds_dev, S0->h.nit * S0->h.M, njobs, counter, rec, threads,
+                          lds, st);
+  if (e == hipSuccess) e = hipEventRecord(S0->ev1, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, S0->ev0, S0->ev1);
+  std::vector<JobRecord> h(nrec);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), rec, nrec * sizeof(JobRecord), hipMemcpyDeviceToHost);
+  hipFree(tmp);
+  if (e != hipSuccess) return hip_fail(e, "job queue");
+  const double khz = wall_clock_khz();
+  for (size_t i = 0; i < nrec; i++) {
+    scf_job_result &r = results[i];
+    r.iterations = h[i].iterations;
+    r.status = h[i].status != DS_OK ? h[i].status : (h[i].converged ? SCF_OK : SCF_NOT_CONVERGED);
+    r.fp32_iterations = h[i].fp32_iterations;
+    r.residual = h[i].residual;
+    r.free_energy = h[i].F;
+    r.grand_potential = h[i].GP;
+    r.solve_ms = (double)(h[i].t_end - h[i].t_begin) / khz;
+  }
+  if (kernel_ms) *kernel_ms = ms;
+  for (size_t i = 0; i < nsys; i++) slots[i]->have_results = 1;
+  return rc;
+}
+
 int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info) {
   if (!systems || n <= 0 || !info) return fail(SCF_E_ARG, "null argument");
   std::vector<int> fused;
@@ -682,6 +763,39 @@ int scf_get_profiles(const scf_system *S, double *seg_phi, double *mol_phi) {
   const size_t M = S->h.M;
   if (seg_phi) HIP_TRY(hipMemcpy(seg_phi, S->h.phi, S->h.nseg * M * sizeof(double), hipMemcpyDeviceToHost));
   if (mol_phi) HIP_TRY(hipMemcpy(mol_phi, S->h.mol_phitot, S->h.nmol * M * sizeof(double), hipMemcpyDeviceToHost));
+  return SCF_OK;
+}
+
+int scf_buffer_alloc(size_t bytes, void **dev) {
+  if (!dev || bytes == 0) return fail(SCF_E_ARG, "bad argument");
+  hipError_t e = hipMalloc(dev, bytes);
+  if (e != hipSuccess) return fail(SCF_E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  return SCF_OK;
+}
+
+int scf_buffer_upload(void *dev, const void *host, size_t bytes) {
+  if (!dev || !host) return fail(SCF_E_ARG, "null argument");
+  HIP_TRY(hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice));
+  return SCF_OK;
+}
+
+int scf_buffer_free(void *dev) {
+  if (dev) HIP_TRY(hipFree(dev));
+  return SCF_OK;
+}
+
+int scf_device_info(int *compute_units, size_t *hbm_bytes) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  HIP_TRY(hipGetDeviceProperties(&p, dev));
+  if (compute_units) *compute_units = p.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
+  return SCF_OK;
+}
+
+int scf_device_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
   return SCF_OK;
 }
 

@@ -50,6 +50,13 @@ struct DevScalars {
   double mixed_best;                  // best residual of the FP32 phase
 };
 
+// Result of one solve of a queued job (persistent job queue).
+struct JobRecord {
+  int iterations, status, fp32_iterations, converged;
+  double residual, F, GP;
+  unsigned long long t_begin, t_end;
+};
+
 struct DevSys {
   // ---- lattice
   int gradients, geometry, lattice_type, stencil_full;
@@ -100,6 +107,7 @@ struct DevSys {
   int strip_stride;       // 2-gradient strip path: LDS slots per strip column
   int threads;            // preferred workgroup size for the fused kernel
   int precision;          // 0: FP64 throughout; 1: FP32 propagators, FP64 refinement
+  int carry_src[kMaxSeg]; // job queue: field block of the preceding system for each block (-1: zero)
   double mixed_switch;    // residual below which a mixed solve continues in FP64
   double volume;          // weighted interior volume (sum of site weights)
   // ---- scalars and reduction workspace

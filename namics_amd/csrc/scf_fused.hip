@@ -1040,9 +1040,10 @@ __device__ void fused_solve(const DevSys &S0) {
   __syncthreads();
 }
 
-// ------------------------------------------------------------- kernel
-__global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode) {
-  const DevSys &S = *systems[blockIdx.x];
+// ------------------------------------------------------------- kernels
+// One residual evaluation or one full solve of system S by this workgroup,
+// with the thermodynamic summary and the device-clock bookkeeping.
+__device__ void run_system(const DevSys &S, int mode) {
   if (threadIdx.x == 0) {
     S.sc->status = DS_OK;
     S.sc->fp32 = 0;
@@ -1066,6 +1067,89 @@ __global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode
     const unsigned long long total = sc.t_end - sc.t_begin;
     sc.t_phase[3] = total > in_res ? total - in_res : 0;
   }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode) {
+  run_system(*systems[blockIdx.x], mode);
+}
+
+// Fields of the first system of a job from its seed (nvar values).
+__device__ __noinline__ void seed_fields(const DevSys &D0, const double *seed) {
+  CSys &D = uniform_sys(D0);
+  const int nvar = D.nit * D.M;
+  gdbl *x = hbm(D.x);
+  const gdbl *sd = hbm(seed);
+  for (int v = threadIdx.x; v < nvar; v += blockDim.x) x[v] = sd[v];
+  __syncthreads();
+}
+
+// Fields of a chained system from the preceding one, block by block as the
+// host mapped them by segment name (initial_guess : previous_result);
+// blocks without a counterpart start at zero.
+__device__ __noinline__ void carry_fields(const DevSys &D0, const DevSys &S0) {
+  CSys &D = uniform_sys(D0);
+  const int M = D.M;
+  gdbl *x = hbm(D.x);
+  const gdbl *xs = hbm(S0.x);
+  for (int j = 0; j < D.nit; j++) {
+    const int sj = D.carry_src[j];
+    for (int i = threadIdx.x; i < M; i += blockDim.x)
+      x[(size_t)j * M + i] = sj >= 0 ? xs[(size_t)sj * M + i] : 0.0;
+  }
+  __syncthreads();
+}
+
+// Persistent job queue.  Workgroup w owns slot w: `chain` systems solved one
+// after another, the first from the job's seed fields, each later one from
+// its predecessor's fields.  A workgroup takes the next job number from the
+// shared counter until the counter passes njobs, so uneven jobs balance over
+// the slots; every workgroup leaves the loop once the queue is empty.
+__global__ void __launch_bounds__(1024) k_fused_jobs(DevSys *const *slots, int chain, const double *seeds,
+                                                     int seed_stride, int njobs, int *counter, JobRecord *rec) {
+  __shared__ int job_s;
+  DevSys *const *mine = slots + (size_t)blockIdx.x * chain;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) job_s = atomicAdd(counter, 1);
+    __syncthreads();
+    const int job = job_s;
+    if (job >= njobs) break;
+    for (int c = 0; c < chain; c++) {
+      const DevSys &S = *mine[c];
+      if (c == 0)
+        seed_fields(S, seeds + (size_t)job * seed_stride);
+      else
+        carry_fields(S, *mine[c - 1]);
+      run_system(S, FUSED_SOLVE);
+      if (threadIdx.x == 0) {
+        const DevScalars &sc = *S.sc;
+        JobRecord &r = rec[(size_t)job * chain + c];
+        r.iterations = sc.iterations;
+        r.status = sc.status;
+        r.fp32_iterations = sc.fp32_iterations;
+        r.converged = sc.converged;
+        r.residual = sc.residual;
+        r.F = sc.F;
+        r.GP = sc.GP;
+        r.t_begin = sc.t_begin;
+        r.t_end = sc.t_end;
+      }
+    }
+  }
+}
+
+hipError_t launch_fused_jobs(DevSys *const *slots, int nslots, int chain, const double *seeds, int seed_stride,
+                             int njobs, int *counter, JobRecord *rec, int threads, size_t lds_bytes,
+                             hipStream_t stream) {
+  if (lds_bytes > 0) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_fused_jobs, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_fused_jobs, dim3(nslots), dim3(threads), lds_bytes, stream, slots, chain, seeds,
+                     seed_stride, njobs, counter, rec);
+  return hipGetLastError();
 }
 
 hipError_t launch_fused(DevSys *const *d_systems, int nsys, int mode, int threads, size_t lds_bytes,

@@ -12,6 +12,10 @@ enum FusedMode : int { FUSED_RESIDUAL = 0, FUSED_SOLVE = 1 };
 // lds_bytes > 0 stages the propagator rows in LDS (2 rows of size*M doubles).
 hipError_t launch_fused(DevSys *const *d_systems, int nsys, int mode, int threads, size_t lds_bytes,
                         hipStream_t stream);
+// Persistent job queue over nslots slots of `chain` systems (see k_fused_jobs).
+hipError_t launch_fused_jobs(DevSys *const *slots, int nslots, int chain, const double *seeds, int seed_stride,
+                             int njobs, int *counter, JobRecord *rec, int threads, size_t lds_bytes,
+                             hipStream_t stream);
 // Static LDS of the fused kernel (bytes), to size the dynamic part.
 size_t fused_static_lds();
 

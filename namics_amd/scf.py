@@ -36,6 +36,12 @@ class SolveInfo(ctypes.Structure):
                 ("fp32_iterations", ctypes.c_int)]
 
 
+class JobResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("fp32_iterations", ctypes.c_int),
+                ("residual", ctypes.c_double), ("free_energy", ctypes.c_double),
+                ("grand_potential", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+
+
 class Results(ctypes.Structure):
     _fields_ = [("free_energy", ctypes.c_double), ("grand_potential", ctypes.c_double),
                 ("residual", ctypes.c_double), ("n_mol", ctypes.c_int), ("n_seg", ctypes.c_int),
@@ -83,6 +89,13 @@ def lib():
         L.scf_propagate_chain.argtypes = [vp, _D, ctypes.c_int, _D]
         L.scf_get_phase_ms.argtypes = [vp, _D]
         L.scf_system_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.scf_buffer_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
+        L.scf_buffer_upload.argtypes = [vp, vp, ctypes.c_size_t]
+        L.scf_buffer_free.argtypes = [vp]
+        L.scf_device_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)]
+        L.scf_device_synchronize.argtypes = []
+        L.scf_solve_jobs.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, vp, ctypes.c_int,
+                                     ctypes.POINTER(JobResult), ctypes.POINTER(ctypes.c_double)]
         L.scf_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -225,6 +238,56 @@ def solve_batch(systems, xs=None):
     infos = (SolveInfo * n)()
     _check(lib().scf_solve_batch(hs, n, ps, infos))
     return xs, [(i.iterations, i.residual, i.status) for i in infos]
+
+
+class DeviceBuffer:
+    """Device memory owned by the library (e.g. a job queue's seed fields)."""
+
+    def __init__(self, array):
+        a = np.ascontiguousarray(array, dtype=np.float64)
+        self.nbytes = a.nbytes
+        self.ptr = ctypes.c_void_p()
+        _check(lib().scf_buffer_alloc(self.nbytes, ctypes.byref(self.ptr)))
+        _check(lib().scf_buffer_upload(self.ptr, a.ctypes.data_as(ctypes.c_void_p), self.nbytes))
+
+    def close(self):
+        if self.ptr:
+            lib().scf_buffer_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_info():
+    """(compute units, HBM bytes) of the current device."""
+    cu = ctypes.c_int(0)
+    hb = ctypes.c_size_t(0)
+    _check(lib().scf_device_info(ctypes.byref(cu), ctypes.byref(hb)))
+    return cu.value, hb.value
+
+
+def synchronize():
+    _check(lib().scf_device_synchronize())
+
+
+def solve_jobs(slots, seeds, njobs):
+    """Persistent job queue: `slots` is a list of equal chains (lists of
+    systems), `seeds` a DeviceBuffer holding njobs x nvar seed fields of chain
+    position 0.  Returns (results as a list of per-job lists of dicts,
+    kernel_ms)."""
+    nslots, chain = len(slots), len(slots[0])
+    flat = [s for row in slots for s in row]
+    hs = (ctypes.c_void_p * len(flat))(*[s.h.value for s in flat])
+    res = (JobResult * (njobs * chain))()
+    ms = ctypes.c_double(0.0)
+    _check(lib().scf_solve_jobs(hs, nslots, chain, seeds.ptr, njobs, res, ctypes.byref(ms)))
+    out = [[{f: getattr(res[j * chain + c], f) for f, _ in JobResult._fields_} for c in range(chain)]
+           for j in range(njobs)]
+    return out, ms.value
 
 
 def solve_resident_info(systems):

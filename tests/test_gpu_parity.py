@@ -194,3 +194,43 @@ def test_mixed_precision_reaches_the_fp64_state(golden, case):
         assert scaled_err(smx.interior(rmx["seg_phi"][sn]), s64.interior(r64["seg_phi"][sn])) < 1e-6
     s64.close()
     smx.close()
+
+
+@pytest.mark.parametrize("case", ["cfg1_planar_adsorption", "brush2d_rod_strip"])
+def test_job_queue_equals_alone(golden, case):
+    """Persistent job queue: 5 seeded jobs (chain of two chi values) over two
+    slots give, per job and chain position, exactly the iterations, status and
+    free energy of solving that chain alone."""
+    from namics_amd import DeviceBuffer, ScfSystem, solve_jobs, solve_resident_info
+    fx = golden[case]
+    o0 = Oracle(fx["text"])
+    base = fx["text"] + "\nnewton : %s : method : DIIS\nnewton : %s : tolerance : 1e-08\n" \
+        "newton : %s : iterationlimit : 3000\n" % (o0.newton_name, o0.newton_name, o0.newton_name)
+    texts = [base, base + "\nmon : A : chi_W : 0.65\n"]
+    njobs = 5
+    seeds = np.random.default_rng(9).uniform(-0.05, 0.05, (njobs, o0.nvar))
+    alone = []
+    a = [ScfSystem(t) for t in texts]
+    for j in range(njobs):
+        a[0].upload(seeds[j])
+        row = []
+        for c, s in enumerate(a):
+            if c:
+                s.carry_from(a[c - 1])
+            info = solve_resident_info([s])[0]
+            row.append((info["iterations"], info["status"], s.results()["free_energy"]))
+        alone.append(row)
+    slots = [[ScfSystem(t) for t in texts] for _ in range(2)]
+    dev = DeviceBuffer(seeds)
+    print("case", case, "jobs", njobs, flush=True)
+    res, ms = solve_jobs(slots, dev, njobs)
+    dev.close()
+    assert ms > 0
+    for j in range(njobs):
+        for c in range(len(texts)):
+            it, st, F = alone[j][c]
+            r = res[j][c]
+            assert (r["iterations"], r["status"]) == (it, st), (j, c)
+            assert r["free_energy"] == F, (j, c)
+    for s in a + [s for row in slots for s in row]:
+        s.close()
