@@ -17,7 +17,7 @@ the launch's algorithmic propagator bandwidth against HBM peak, the same
 jobs with FP32 propagators and FP64 refinement, one job alone, and the
 reference CPU program timed on this host on one job as the baseline.
 
-    python bench.py [--steps K] [--warmup W] [--jobs J] [--slots S] [--workload brush|batch1d]
+    python bench.py [--steps K] [--warmup W] [--jobs J] [--slots S] [--workload brush|batch1d|serve]
                     [--no-cpu] [--no-mixed] [--switch R]
 """
 import argparse
@@ -209,6 +209,10 @@ def main():
     ap.add_argument("--slots", type=int, default=0, help="resident job slots (0: one per CU)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision rerun")
     ap.add_argument("--switch", type=float, default=1e-4, help="mixed precision: FP64 below this residual")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: requests per second")
+    ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals")
+    ap.add_argument("--pool-gb", type=float, default=64.0, help="serve: HBM pool for requests")
+    ap.add_argument("--seed", type=int, default=7, help="serve: request-stream seed")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -216,6 +220,9 @@ def main():
         sys.exit(2)
     if args.workload == "brush":
         out = run_brush(args)
+    elif args.workload == "serve":
+        from namics_amd.bench_serve import run_serve
+        out = run_serve(args)
     else:
         from namics_amd.bench_batch import run_batch
         out = run_batch(args)

@@ -88,6 +88,10 @@ void scf_system_destroy(scf_system *sys);
  * sizes (M, MX, MY, MZ), nvar, solver settings.  Returns bytes needed. */
 int scf_system_describe(const scf_system *sys, char *buf, size_t len);
 
+/* The same description for input text without creating a system (no
+ * device work): the field layout a caller needs to prepare starting fields. */
+int scf_describe_input(const char *input_text, int start, char *buf, size_t len);
+
 /* Number of field values (n_field_blocks * M), the length of x and g. */
 int64_t scf_system_nvar(const scf_system *sys);
 
@@ -200,6 +204,48 @@ int scf_get_phase_ms(const scf_system *sys, double *ms);
  * (src/lattice.h:133; src/LG1Planar.cpp:138-159, src/LGrad2.cpp:1053-1107,
  * src/LGrad3.cpp:673-743). */
 int scf_propagate_chain(scf_system *sys, const double *G1, int N, double *out);
+
+/* ---------------------------------------------------------------------
+ * Request server: continuous batching of independent solves.
+ * Requests are parsed and sized on submission and admitted first come,
+ * first served while their HBM footprint fits in the server's pool (one
+ * device allocation, sub-allocated, so finished requests return memory
+ * without synchronising the device).  Admitted 1- and 2-gradient requests
+ * are solved in time slices, one workgroup each: a launch runs the active
+ * requests of one workgroup shape for a device-clock budget, unfinished
+ * solves save their state and resume in the next launch together with newly
+ * admitted requests (iteration-level continuous batching); several streams
+ * carry launches at once.  3-gradient requests run the grid path in a
+ * worker thread.  scf_server_progress() drives admission,
+ * launches and completions without blocking; scf_server_results() hands
+ * over finished requests.  Latency = device completion time of the
+ * request's own solve (device clock mapped to host time) - submission time.
+ * No counterpart in the reference (one system per process run). */
+typedef struct scf_server scf_server;
+
+typedef struct {
+  long long id;
+  int status;          /* SCF_OK, SCF_NOT_CONVERGED or a negative error */
+  int iterations;
+  int gradients;
+  double residual;
+  double free_energy;
+  double grand_potential;
+  double latency_ms;   /* submission -> end of the request's solve */
+  double queue_ms;     /* submission -> admission (waiting for HBM) */
+  double solve_ms;     /* the solve on the device */
+  size_t device_bytes; /* HBM footprint */
+} scf_served;
+
+int scf_server_create(size_t pool_bytes, int streams, scf_server **out);
+/* Device-clock budget of one time slice of the fused-path launches
+ * (default 20 ms): the longest a newly admitted request waits for a start. */
+int scf_server_set_slice(scf_server *srv, double slice_ms);
+/* x0: starting fields (nx0 values, the system's nvar) or NULL for zeros. */
+int scf_server_submit(scf_server *srv, const char *input_text, const double *x0, size_t nx0, long long *id);
+int scf_server_progress(scf_server *srv, int *queued, int *running);
+int scf_server_results(scf_server *srv, scf_served *out, int max, int *n);
+void scf_server_destroy(scf_server *srv);
 
 /* Device plumbing for callers without their own GPU runtime (the job
  * queue's seed fields): allocate / fill / free device memory, the current

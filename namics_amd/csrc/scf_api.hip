@@ -10,14 +10,11 @@
 #include <string>
 #include <vector>
 
-#include "../../include/namics_scf.h"
-#include "scf_device.h"
-#include "scf_kernels.h"
-#include "scf_spec.h"
+#include "scf_internal.h"
 
 using namespace nscf;
 
-namespace {
+namespace nscf {
 
 thread_local std::string g_last_error;
 
@@ -30,11 +27,7 @@ int hip_fail(hipError_t e, const char *where) {
   return fail(SCF_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 
-#define HIP_TRY(expr)                                 \
-  do {                                                \
-    hipError_t _e = (expr);                           \
-    if (_e != hipSuccess) return hip_fail(_e, #expr); \
-  } while (0)
+namespace {
 
 // Bump allocator over one device allocation.
 struct Arena {
@@ -47,6 +40,8 @@ struct Arena {
 };
 
 size_t g_lds_limit = 0;  // bytes of dynamic LDS one workgroup may use
+
+}  // namespace
 
 size_t lds_limit() {
   if (g_lds_limit == 0) {
@@ -65,6 +60,7 @@ size_t lds_limit() {
   return g_lds_limit;
 }
 
+
 double g_wall_khz = 0;
 
 // frequency of the device wall clock (wall_clock64) in kHz
@@ -79,33 +75,6 @@ double wall_clock_khz() {
   }
   return g_wall_khz;
 }
-
-}  // namespace
-
-struct scf_system {
-  SystemSpec spec;
-  DevSys h;              // host copy holding device pointers
-  DevSys *d = nullptr;   // device copy
-  DevSys **d_list = nullptr;
-  char *arena = nullptr;
-  size_t arena_bytes = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipGraphExec_t graph_exec = nullptr;  // captured grid-path iteration
-  bool use_graph = true;
-  int mode = SCF_MODE_AUTO;
-  int nb = 0;            // workgroups of the grid path
-  int m_alloc = 0;       // DIIS history the arena was sized for
-  size_t lds_bytes = 0;  // dynamic LDS the fused path asks for (0: rows in HBM)
-  bool have_results = false;
-
-  int effective_mode() const {
-    if (mode != SCF_MODE_AUTO) return mode;
-    return spec.lat.gradients == 3 ? SCF_MODE_GRID : SCF_MODE_FUSED;
-  }
-};
-
-namespace {
 
 int validate(const SystemSpec &s, std::string &why) {
   const LatticeSpec &l = s.lat;
@@ -140,7 +109,7 @@ double fraction(const MoleculeSpec &m, int seg) {
   return 1.0 * n / m.chainlength;
 }
 
-int build(scf_system *S) {
+int build(scf_system *S, const Placement *pl) {
   const SystemSpec &s = S->spec;
   const LatticeSpec &l = s.lat;
   DevSys &h = S->h;
@@ -251,30 +220,38 @@ int build(scf_system *S) {
   size_t o_dev = a.take(sizeof(DevSys));
   size_t o_list = a.take(sizeof(DevSys *));
   S->arena_bytes = a.used;
-  hipError_t e = hipMalloc((void **)&S->arena, a.used);
-  if (e != hipSuccess) return fail(SCF_E_OOM, "device allocation of " + std::to_string(a.used) + " bytes failed");
-  char *base = S->arena;
-  HIP_TRY(hipMemset(base, 0, a.used));
+  if (pl && pl->dry) return SCF_OK;
+  char *base = nullptr;
+  if (pl && pl->region) {
+    if (a.used > pl->capacity) return fail(SCF_E_OOM, "system needs " + std::to_string(a.used) + " bytes");
+    base = S->region = pl->region;
+  } else {
+    hipError_t e = hipMalloc((void **)&S->arena, a.used);
+    if (e != hipSuccess) return fail(SCF_E_OOM, "device allocation of " + std::to_string(a.used) + " bytes failed");
+    base = S->arena;
+  }
+  hipStream_t st = S->stream;
+  HIP_TRY(hipMemsetAsync(base, 0, a.used, st));
   auto P = [&](size_t off) { return (double *)(base + off); };
   if (curved) {
     h.L = P(o_L);
     h.lam_m = P(o_L) + M;
     h.lam_0 = P(o_L) + 2 * M;
     h.lam_p = P(o_L) + 3 * M;
-    HIP_TRY(hipMemcpy((void *)h.L, l.L.data(), M * D, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy((void *)h.lam_m, l.lambda_1.data(), M * D, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy((void *)h.lam_0, l.lambda0.data(), M * D, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy((void *)h.lam_p, l.lambda1.data(), M * D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync((void *)h.L, l.L.data(), M * D, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync((void *)h.lam_m, l.lambda_1.data(), M * D, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync((void *)h.lam_0, l.lambda0.data(), M * D, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync((void *)h.lam_p, l.lambda1.data(), M * D, hipMemcpyHostToDevice, st));
   }
   if (m2curved) {
     h.l1 = P(o_l1);
     h.l11 = P(o_l1) + M;
     h.l_1 = P(o_l1) + 2 * M;
     h.l_11 = P(o_l1) + 3 * M;
-    HIP_TRY(hipMemcpy((void *)h.l1, l.l1.data(), M * D, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy((void *)h.l11, l.l11.data(), M * D, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy((void *)h.l_1, l.l_1.data(), M * D, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy((void *)h.l_11, l.l_11.data(), M * D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync((void *)h.l1, l.l1.data(), M * D, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync((void *)h.l11, l.l11.data(), M * D, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync((void *)h.l_1, l.l_1.data(), M * D, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync((void *)h.l_11, l.l_11.data(), M * D, hipMemcpyHostToDevice, st));
   }
   {
     std::vector<double> mask((size_t)nseg * M, 0.0), ks(M, 0.0);
@@ -283,8 +260,9 @@ int build(scf_system *S) {
     for (int i = 0; i < M; i++) ks[i] = s.ksam[i];
     h.seg_mask = P(o_mask);
     h.ksam = P(o_ksam);
-    HIP_TRY(hipMemcpy((void *)h.seg_mask, mask.data(), mask.size() * D, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy((void *)h.ksam, ks.data(), M * D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync((void *)h.seg_mask, mask.data(), mask.size() * D, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync((void *)h.ksam, ks.data(), M * D, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));  // the host vectors end here
   }
   h.x = P(o_x);
   h.g = P(o_g);
@@ -345,18 +323,21 @@ int build(scf_system *S) {
   }
   S->d = (DevSys *)(base + o_dev);
   S->d_list = (DevSys **)(base + o_list);
-  HIP_TRY(hipMemcpy(S->d, &h, sizeof(DevSys), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(S->d_list, &S->d, sizeof(DevSys *), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(S->d, &h, sizeof(DevSys), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(S->d_list, &S->d, sizeof(DevSys *), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
   return SCF_OK;
 }
 
 int upload_solver(scf_system *S) {
-  HIP_TRY(hipMemcpy(S->d, &S->h, sizeof(DevSys), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(S->d, &S->h, sizeof(DevSys), hipMemcpyHostToDevice, S->stream));
+  HIP_TRY(hipStreamSynchronize(S->stream));
   return SCF_OK;
 }
 
 int read_scalars(const scf_system *S, DevScalars &sc) {
-  HIP_TRY(hipMemcpy(&sc, S->h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpyAsync(&sc, S->h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, S->stream));
+  HIP_TRY(hipStreamSynchronize(S->stream));
   return SCF_OK;
 }
 
@@ -459,13 +440,23 @@ void scf_system_destroy(scf_system *S) {
   if (S->arena) hipFree(S->arena);
   if (S->ev0) hipEventDestroy(S->ev0);
   if (S->ev1) hipEventDestroy(S->ev1);
-  if (S->stream) hipStreamDestroy(S->stream);
+  if (S->stream && S->owns_stream) hipStreamDestroy(S->stream);
   delete S;
 }
 
 int scf_system_describe(const scf_system *S, char *buf, size_t len) {
   if (!S) return fail(SCF_E_ARG, "null system");
   std::string d = describe_json(S->spec);
+  if (buf && len > 0) snprintf(buf, len, "%s", d.c_str());
+  return (int)d.size() + 1;
+}
+
+int scf_describe_input(const char *input_text, int start, char *buf, size_t len) {
+  if (!input_text) return fail(SCF_E_ARG, "null input");
+  SystemSpec spec;
+  std::string why;
+  if (!parse_input(input_text, start, spec, why)) return fail(SCF_E_PARSE, why);
+  std::string d = describe_json(spec);
   if (buf && len > 0) snprintf(buf, len, "%s", d.c_str());
   return (int)d.size() + 1;
 }

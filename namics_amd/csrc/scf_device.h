@@ -47,6 +47,7 @@ struct DevScalars {
   int fp32;                           // 1 while the solve evaluates propagators in FP32
   int fp32_iterations;                // iterations of the last solve done with FP32 propagators
   int mixed_since, mixed_restart;     // evaluations since the best FP32 residual; DIIS restart flag
+  int phase;                          // time-sliced solve: 0 not started, 1 running, 2 finished
   double mixed_best;                  // best residual of the FP32 phase
 };
 

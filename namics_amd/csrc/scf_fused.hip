@@ -966,7 +966,15 @@ __device__ __noinline__ void diis_combine(const DevSys &S0, int k, int kd) {
 // changes x, so the iteration's x0 is x itself at the update: one pass per
 // iteration reads x and g, writes x, xR_k and e_k = x_new - x0, and
 // accumulates all kd Gram dots; a second pass forms x = sum_i c_i xR_i.
-__device__ void fused_solve(const DevSys &S0) {
+__shared__ int s_slice_stop;  // time-sliced solve: stop flag broadcast by thread 0
+
+// Returns true when the solve has finished (converged, iteration limit or
+// error).  With a budget (device-clock ticks) it may instead stop between
+// iterations once the budget is used, with its state (iteration count,
+// DIIS ring position and Gram matrix, residual, precision flags) in the
+// system's scalars, and return false; called again with fresh = false it
+// resumes exactly where it stopped.
+__device__ bool fused_solve(const DevSys &S0, bool fresh, unsigned long long budget) {
   CSys &S = uniform_sys(S0);
   double *diis_dots = s_diis;
   double *diis_A = s_diis + kMaxDIIS;
@@ -976,11 +984,13 @@ __device__ void fused_solve(const DevSys &S0) {
   const double dmax = S.deltamax, tol = S.tol;
   const int itlim = S.itlim;
   DevScalars &sc = *S0.sc;
-  double res = fused_residual(S0);
-  if (res < 0) return;
   const bool mixed = S.precision != 0;
-  if (mixed) mixed_update(S0, res, true);
-  {
+  double res;
+  int it, k_diis;
+  if (fresh) {
+    res = fused_residual(S0);
+    if (res < 0) return true;
+    if (mixed) mixed_update(S0, res, true);
     gdbl *x = hbm(S.x), *xR = hbm(S.xR), *xx0 = hbm(S.xx0);
     const gdbl *g = hbm(S.g);
     for (int v = tid; v < nvar; v += nt) {
@@ -990,10 +1000,25 @@ __device__ void fused_solve(const DevSys &S0) {
       xx0[v] = xv - xo;
       xR[v] = xv;
     }
+    it = 0;
+    k_diis = 0;
+  } else {
+    res = sc.residual;
+    it = sc.iterations;
+    k_diis = sc.k_diis;
   }
-  int it = 0, k_diis = 0;
+  const unsigned long long t_slice = wall_clock64();
+  bool stopped = false;
   __syncthreads();
   while (res > tol && it < itlim) {
+    if (budget) {
+      if (threadIdx.x == 0) s_slice_stop = (wall_clock64() - t_slice) > budget ? 1 : 0;
+      __syncthreads();
+      if (s_slice_stop) {
+        stopped = true;
+        break;
+      }
+    }
     it++;
     if (mixed && sc.fp32) {
       if (threadIdx.x == 0) sc.fp32_iterations++;
@@ -1035,43 +1060,53 @@ __device__ void fused_solve(const DevSys &S0) {
   }
   if (threadIdx.x == 0) {
     sc.iterations = it;
+    sc.k_diis = k_diis;
     sc.converged = (sc.status == DS_OK && res <= tol) ? 1 : 0;
   }
   __syncthreads();
+  return !stopped;
 }
 
 // ------------------------------------------------------------- kernels
 // One residual evaluation or one full solve of system S by this workgroup,
 // with the thermodynamic summary and the device-clock bookkeeping.
-__device__ void run_system(const DevSys &S, int mode) {
-  if (threadIdx.x == 0) {
-    S.sc->status = DS_OK;
-    S.sc->fp32 = 0;
-    S.sc->fp32_iterations = 0;
-    S.sc->mixed_restart = 0;
-    for (int p = 0; p < 4; p++) S.sc->t_phase[p] = 0;
-    S.sc->t_begin = wall_clock64();
+__device__ void run_system(const DevSys &S, int mode, unsigned long long budget = 0) {
+  DevScalars &sc = *S.sc;
+  const bool fresh = mode != FUSED_SLICE || sc.phase == 0;
+  if (fresh && threadIdx.x == 0) {
+    sc.status = DS_OK;
+    sc.fp32 = 0;
+    sc.fp32_iterations = 0;
+    sc.mixed_restart = 0;
+    for (int p = 0; p < 4; p++) sc.t_phase[p] = 0;
+    sc.t_begin = wall_clock64();
   }
   __syncthreads();
+  bool fin = true;
   if (mode == FUSED_RESIDUAL) {
     const double r = fused_residual(S);
     if (r >= 0) fused_thermo(S);
   } else {
-    fused_solve(S);
-    if (S.sc->status == DS_OK) fused_thermo(S);
+    fin = fused_solve(S, fresh, mode == FUSED_SLICE ? budget : 0);
+    if (fin && sc.status == DS_OK) fused_thermo(S);
   }
   if (threadIdx.x == 0) {
-    DevScalars &sc = *S.sc;
-    sc.t_end = wall_clock64();
-    const unsigned long long in_res = sc.t_phase[0] + sc.t_phase[1] + sc.t_phase[2];
-    const unsigned long long total = sc.t_end - sc.t_begin;
-    sc.t_phase[3] = total > in_res ? total - in_res : 0;
+    sc.phase = fin ? 2 : 1;
+    if (fin) {
+      sc.t_end = wall_clock64();
+      const unsigned long long in_res = sc.t_phase[0] + sc.t_phase[1] + sc.t_phase[2];
+      const unsigned long long total = sc.t_end - sc.t_begin;
+      sc.t_phase[3] = total > in_res ? total - in_res : 0;
+    }
   }
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode) {
-  run_system(*systems[blockIdx.x], mode);
+__global__ void __launch_bounds__(1024) k_fused(DevSys *const *systems, int mode, unsigned long long budget,
+                                                int *phase_out) {
+  const DevSys &S = *systems[blockIdx.x];
+  run_system(S, mode, budget);
+  if (phase_out && threadIdx.x == 0) phase_out[blockIdx.x] = S.sc->phase;
 }
 
 // Fields of the first system of a job from its seed (nvar values).
@@ -1153,13 +1188,14 @@ hipError_t launch_fused_jobs(DevSys *const *slots, int nslots, int chain, const 
 }
 
 hipError_t launch_fused(DevSys *const *d_systems, int nsys, int mode, int threads, size_t lds_bytes,
-                        hipStream_t stream) {
+                        hipStream_t stream, unsigned long long budget_ticks, int *phase_out) {
   if (lds_bytes > 0) {
     hipError_t e = hipFuncSetAttribute((const void *)k_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds_bytes);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_fused, dim3(nsys), dim3(threads), lds_bytes, stream, d_systems, mode);
+  hipLaunchKernelGGL(k_fused, dim3(nsys), dim3(threads), lds_bytes, stream, d_systems, mode, budget_ticks,
+                     phase_out);
   return hipGetLastError();
 }
 

@@ -1,0 +1,67 @@
+// Host-side internals shared by the C ABI (scf_api.hip) and the request
+// server (scf_server.hip): the system handle and the helpers that build it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/namics_scf.h"
+#include "scf_device.h"
+#include "scf_kernels.h"
+#include "scf_spec.h"
+
+// One SCF system: its parsed input, device description and HBM arena.
+struct scf_system {
+  nscf::SystemSpec spec;
+  nscf::DevSys h;              // host copy holding device pointers
+  nscf::DevSys *d = nullptr;   // device copy
+  nscf::DevSys **d_list = nullptr;
+  char *arena = nullptr;
+  size_t arena_bytes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t graph_exec = nullptr;  // captured grid-path iteration
+  bool use_graph = true;
+  int mode = SCF_MODE_AUTO;
+  int nb = 0;            // workgroups of the grid path
+  int m_alloc = 0;       // DIIS history the arena was sized for
+  size_t lds_bytes = 0;  // dynamic LDS the fused path asks for (0: rows in HBM)
+  bool have_results = false;
+  bool owns_stream = true;  // false when a server lends its stream
+  char *region = nullptr;   // arena placed in a caller's device region (not owned)
+
+  int effective_mode() const {
+    if (mode != SCF_MODE_AUTO) return mode;
+    return spec.lat.gradients == 3 ? SCF_MODE_GRID : SCF_MODE_FUSED;
+  }
+};
+
+namespace nscf {
+
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *where);
+
+#define HIP_TRY(expr)                                 \
+  do {                                                \
+    hipError_t _e = (expr);                           \
+    if (_e != hipSuccess) return hip_fail(_e, #expr); \
+  } while (0)
+
+// Where build() places a system's arena: its own allocation (default), a
+// caller's device region of `capacity` bytes, or nowhere (dry: compute the
+// footprint only, into scf_system::arena_bytes).
+struct Placement {
+  char *region = nullptr;
+  size_t capacity = 0;
+  bool dry = false;
+};
+
+int validate(const SystemSpec &s, std::string &why);
+int build(scf_system *S, const Placement *pl = nullptr);
+int upload_solver(scf_system *S);
+int read_scalars(const scf_system *S, DevScalars &sc);
+int grid_solve(scf_system *S, scf_solve_info *info);
+double wall_clock_khz();
+size_t lds_limit();
+
+}  // namespace nscf

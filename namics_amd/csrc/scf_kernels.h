@@ -6,12 +6,14 @@
 
 namespace nscf {
 
-enum FusedMode : int { FUSED_RESIDUAL = 0, FUSED_SOLVE = 1 };
+// FUSED_SLICE: a solve that stops after a device-clock budget with its state
+// saved (phase 1) and resumes on the next launch, until it finishes (phase 2).
+enum FusedMode : int { FUSED_RESIDUAL = 0, FUSED_SOLVE = 1, FUSED_SLICE = 2 };
 
 // One workgroup per system: residual evaluation or full DIIS solve.
 // lds_bytes > 0 stages the propagator rows in LDS (2 rows of size*M doubles).
 hipError_t launch_fused(DevSys *const *d_systems, int nsys, int mode, int threads, size_t lds_bytes,
-                        hipStream_t stream);
+                        hipStream_t stream, unsigned long long budget_ticks = 0, int *phase_out = nullptr);
 // Persistent job queue over nslots slots of `chain` systems (see k_fused_jobs).
 hipError_t launch_fused_jobs(DevSys *const *slots, int nslots, int chain, const double *seeds, int seed_stride,
                              int njobs, int *counter, JobRecord *rec, int threads, size_t lds_bytes,

@@ -42,6 +42,13 @@ class JobResult(ctypes.Structure):
                 ("grand_potential", ctypes.c_double), ("solve_ms", ctypes.c_double)]
 
 
+class Served(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_longlong), ("status", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("gradients", ctypes.c_int), ("residual", ctypes.c_double), ("free_energy", ctypes.c_double),
+                ("grand_potential", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("queue_ms", ctypes.c_double), ("solve_ms", ctypes.c_double), ("device_bytes", ctypes.c_size_t)]
+
+
 class Results(ctypes.Structure):
     _fields_ = [("free_energy", ctypes.c_double), ("grand_potential", ctypes.c_double),
                 ("residual", ctypes.c_double), ("n_mol", ctypes.c_int), ("n_seg", ctypes.c_int),
@@ -89,6 +96,14 @@ def lib():
         L.scf_propagate_chain.argtypes = [vp, _D, ctypes.c_int, _D]
         L.scf_get_phase_ms.argtypes = [vp, _D]
         L.scf_system_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.scf_describe_input.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+        L.scf_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(vp)]
+        L.scf_server_set_slice.argtypes = [vp, ctypes.c_double]
+        L.scf_server_submit.argtypes = [vp, ctypes.c_char_p, _D, ctypes.c_size_t, ctypes.POINTER(ctypes.c_longlong)]
+        L.scf_server_progress.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.scf_server_results.argtypes = [vp, ctypes.POINTER(Served), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.scf_server_destroy.argtypes = [vp]
+        L.scf_server_destroy.restype = None
         L.scf_buffer_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
         L.scf_buffer_upload.argtypes = [vp, vp, ctypes.c_size_t]
         L.scf_buffer_free.argtypes = [vp]
@@ -238,6 +253,58 @@ def solve_batch(systems, xs=None):
     infos = (SolveInfo * n)()
     _check(lib().scf_solve_batch(hs, n, ps, infos))
     return xs, [(i.iterations, i.residual, i.status) for i in infos]
+
+
+def describe_input(text, start=1):
+    """Layout of an input (fields, sizes, nvar) without creating a system."""
+    n = lib().scf_describe_input(text.encode(), start, None, 0)
+    if n < 0:
+        raise ScfError(n, lib().scf_last_error().decode())
+    buf = ctypes.create_string_buffer(n)
+    lib().scf_describe_input(text.encode(), start, buf, n)
+    return json.loads(buf.value.decode())
+
+
+class Server:
+    """Request server (continuous batching, HBM-footprint admission)."""
+
+    def __init__(self, pool_bytes, streams=4):
+        self.h = ctypes.c_void_p()
+        _check(lib().scf_server_create(int(pool_bytes), int(streams), ctypes.byref(self.h)))
+
+    def set_slice(self, ms):
+        _check(lib().scf_server_set_slice(self.h, float(ms)))
+
+    def submit(self, text, x0=None):
+        rid = ctypes.c_longlong(0)
+        if x0 is None:
+            _check(lib().scf_server_submit(self.h, text.encode(), None, 0, ctypes.byref(rid)))
+        else:
+            x = np.ascontiguousarray(x0, dtype=np.float64)
+            _check(lib().scf_server_submit(self.h, text.encode(), _p(x), x.size, ctypes.byref(rid)))
+        return rid.value
+
+    def progress(self):
+        q, r = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib().scf_server_progress(self.h, ctypes.byref(q), ctypes.byref(r)))
+        return q.value, r.value
+
+    def results(self, max_n=4096):
+        buf = (Served * max_n)()
+        n = ctypes.c_int(0)
+        _check(lib().scf_server_results(self.h, buf, max_n, ctypes.byref(n)))
+        return [{f: getattr(buf[i], f) for f, _ in Served._fields_} for i in range(n.value)]
+
+    def close(self):
+        if self.h:
+            lib().scf_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:

@@ -234,3 +234,68 @@ def test_job_queue_equals_alone(golden, case):
             assert r["free_energy"] == F, (j, c)
     for s in a + [s for row in slots for s in row]:
         s.close()
+
+
+def test_server_results_equal_alone_and_admission_waits(golden):
+    """Request server: a mix of 1-, 2- and 3-gradient requests, submitted in
+    two waves into a pool that holds only a few of them at a time, gives
+    every request exactly the iterations, status and free energy it gets
+    solved alone -- with 20 ms time slices and with 0.02 ms slices (every
+    solve cut into many launches and resumed); requests that did not fit
+    waited (queue time > 0)."""
+    import time
+    from namics_amd import ScfSystem, Server
+    mix = []
+    for case, chis in (("cfg1_planar_adsorption", (0.5, 0.7, 0.9)), ("planar_markov2", (None,)),
+                       ("brush2d_rod_strip", (0.6,)), ("diblock3d_periodic", (None,))):
+        fx = golden[case]
+        o0 = Oracle(fx["text"])
+        for k, chi in enumerate(chis):
+            text = fx["text"] + "\nnewton : %s : method : DIIS\nnewton : %s : tolerance : 1e-08\n" \
+                "newton : %s : iterationlimit : 3000\n" % (o0.newton_name, o0.newton_name, o0.newton_name)
+            if chi is not None:
+                text += "mon : A : chi_W : %r\n" % chi
+            x0 = np.random.default_rng(100 + k).uniform(-0.05, 0.05, o0.nvar)
+            mix.append((case, text, x0))
+    alone, sizes = [], []
+    for case, text, x0 in mix:
+        s = ScfSystem(text)
+        x, it, res, st = s.solve(x0)
+        alone.append((it, st, s.results()["free_energy"]))
+        sizes.append(s.device_bytes)
+        s.close()
+    pool = int(1.05 * max(sizes))  # the largest request alone fills it: the others wait
+    for slice_ms in (20.0, 0.02):
+        got = serve_mix(Server, pool, slice_ms, mix)
+        assert len(got) == len(mix)
+        for k, (it, st, F) in enumerate(alone):
+            r = got[k]
+            print("slice", slice_ms, "request", k, mix[k][0], r["iterations"], it, flush=True)
+            assert (r["iterations"], r["status"]) == (it, st), k
+            assert r["free_energy"] == F, k
+            assert r["solve_ms"] > 0 and r["latency_ms"] >= r["solve_ms"] - 0.5
+        assert any(got[k]["queue_ms"] > 0.5 for k in got)
+
+
+def serve_mix(Server, pool, slice_ms, mix):
+    """Submit `mix` in two waves to a server and collect every result."""
+    import time
+    srv = Server(pool, streams=3)
+    srv.set_slice(slice_ms)
+    ids = {}
+    half = len(mix) // 2
+    for k, (case, text, x0) in enumerate(mix[:half]):
+        ids[srv.submit(text, x0)] = k
+    for _ in range(5):
+        srv.progress()
+    for k, (case, text, x0) in enumerate(mix[half:], start=half):
+        ids[srv.submit(text, x0)] = k
+    got = {}
+    t_end = time.time() + 120
+    while len(got) < len(mix) and time.time() < t_end:
+        srv.progress()
+        for r in srv.results():
+            got[ids[r["id"]]] = r
+        time.sleep(0.001)
+    srv.close()
+    return got
