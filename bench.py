@@ -23,6 +23,7 @@ reference CPU program timed on this host on one job as the baseline.
 import argparse
 import json
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -51,6 +52,23 @@ def propagator_bytes(desc, iterations, w=8):
         W = desc["M"]  # 1-gradient kernels update the ghost sites too
     per_eval = sum((19 if m["markov"] == 2 else 9) * w * m["chainlength"] * W for m in desc["molecules"])
     return per_eval * (iterations + 1)
+
+
+def profiled_traffic(jobs, slots):
+    """HBM bytes per launch of the batched-brush job kernel measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-calibrated), from the committed
+    profile summary, when this run has the profiled configuration."""
+    path = os.path.join(REPO, "profiles", "k_fused_jobs_brush512.md")
+    try:
+        with open(path) as f:
+            text = f.read()
+    except OSError:
+        return None, None
+    m = re.search(r"traffic_bytes_per_launch = ([0-9.eE+-]+)", text)
+    c = re.search(r"traffic_config = jobs (\d+), slots (\d+)", text)
+    if m and c and int(c.group(1)) == jobs and int(c.group(2)) == slots:
+        return float(m.group(1)), os.path.relpath(path, REPO)
+    return None, None
 
 
 def cpu_baseline_brush(x0, names, dims, budget_s=60.0):
@@ -153,6 +171,11 @@ def run_brush(args):
                      "algorithmic_bytes_per_launch": nbytes},
         "setup_s": t_setup,
     }
+    traffic, src = profiled_traffic(J, nslots)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic
+        out["roofline"]["traffic_source"] = src + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)"
+        out["roofline"]["hbm_gbs_measured_traffic"] = traffic / (launch_ms * 1e-3) / 1e9
     # the same jobs with FP32 propagators and FP64 refinement
     if not args.no_mixed:
         for row in slots:

@@ -256,6 +256,10 @@ int scf_buffer_upload(void *dev, const void *host, size_t bytes);
 int scf_buffer_free(void *dev);
 int scf_device_info(int *compute_units, size_t *hbm_bytes);
 int scf_device_synchronize(void);
+/* A plain streaming copy kernel of `bytes` (a multiple of 16) between device
+ * buffers: reads and writes exactly `bytes`, the reference point for
+ * hardware memory-traffic counters. */
+int scf_device_copy(void *dst, const void *src, size_t bytes);
 
 /* Last error message of this thread (static storage). */
 const char *scf_last_error(void);

@@ -785,6 +785,25 @@ int scf_device_info(int *compute_units, size_t *hbm_bytes) {
   return SCF_OK;
 }
 
+namespace {
+// Plain streaming copy, 16 bytes per thread per trip (HBM counter calibration).
+__global__ void __launch_bounds__(256) k_stream_copy(double2 *__restrict__ dst, const double2 *__restrict__ src,
+                                                      size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+int scf_device_copy(void *dst, const void *src, size_t bytes) {
+  if (!dst || !src || bytes % 16) return fail(SCF_E_ARG, "bad argument (16-byte multiples)");
+  const size_t n = bytes / 16;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, 0, (double2 *)dst, (const double2 *)src, n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  return SCF_OK;
+}
+
 int scf_device_synchronize(void) {
   HIP_TRY(hipDeviceSynchronize());
   return SCF_OK;

@@ -13,8 +13,8 @@
 // non-blocking streams carry launches of different shapes at once.  So a new
 // request waits at most about one slice, long solves cannot hold others back,
 // and a solve's arithmetic -- hence its result -- is the same as alone.
-// 3-gradient requests run the grid path (host-driven DIIS) in a worker thread
-// on their own stream.  A request's completion time comes from the device
+// 3-gradient requests run the grid path (host-driven DIIS) in worker threads
+// (a few at once), each on its own stream.  A request's completion time comes from the device
 // clock at the end of its own solve, converted to host time.
 //
 // No counterpart in the reference, which solves one system per process.
@@ -35,6 +35,8 @@
 using namespace nscf;
 
 namespace {
+
+constexpr int kGridWorkers = 3;  // 3-gradient requests solved at once (each a grid-wide launch sequence)
 
 double host_now() {
   using namespace std::chrono;
@@ -125,7 +127,7 @@ struct scf_server {
   std::deque<Request *> grid_queue;
   int grid_busy = 0;
   bool stop = false;
-  std::thread worker;
+  std::vector<std::thread> workers;  // grid-path solves, each on its own stream
 };
 
 namespace {
@@ -229,7 +231,7 @@ int scf_server_create(size_t pool_bytes, int streams, scf_server **out) {
     scf_server_destroy(srv);
     return rc;
   }
-  srv->worker = std::thread(grid_worker, srv);
+  for (int w = 0; w < kGridWorkers; w++) srv->workers.emplace_back(grid_worker, srv);
   *out = srv;
   return SCF_OK;
 }
@@ -386,7 +388,8 @@ void scf_server_destroy(scf_server *srv) {
     srv->stop = true;
     srv->cv.notify_all();
   }
-  if (srv->worker.joinable()) srv->worker.join();
+  for (auto &w : srv->workers)
+    if (w.joinable()) w.join();
   for (auto &L : srv->launches) {
     if (L.stream) hipStreamSynchronize(L.stream);
     for (Request *r : L.reqs) delete r;

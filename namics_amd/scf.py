@@ -109,6 +109,7 @@ def lib():
         L.scf_buffer_free.argtypes = [vp]
         L.scf_device_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)]
         L.scf_device_synchronize.argtypes = []
+        L.scf_device_copy.argtypes = [vp, vp, ctypes.c_size_t]
         L.scf_solve_jobs.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, vp, ctypes.c_int,
                                      ctypes.POINTER(JobResult), ctypes.POINTER(ctypes.c_double)]
         L.scf_last_error.restype = ctypes.c_char_p
@@ -335,6 +336,11 @@ def device_info():
     hb = ctypes.c_size_t(0)
     _check(lib().scf_device_info(ctypes.byref(cu), ctypes.byref(hb)))
     return cu.value, hb.value
+
+
+def device_copy(dst, src, nbytes):
+    """Streaming copy kernel between two DeviceBuffers."""
+    _check(lib().scf_device_copy(dst.ptr, src.ptr, int(nbytes)))
 
 
 def synchronize():
