@@ -119,6 +119,18 @@ enum {
  * reference (its arithmetic is FP64 throughout). */
 int scf_system_set_precision(scf_system *sys, int mode, double switch_residual);
 
+/* Checkpointed propagators (the reference's `save_memory`, src/molecule.cpp
+ * :2300-2446): the forward pass keeps only every `interval`-th end-point
+ * distribution; the backward pass recomputes the rows between two
+ * checkpoints from the earlier one just before it composes them.  Densities,
+ * partition functions and every iterate are bit-identical to full storage;
+ * the end-point traffic falls about interval-fold for up to 50% more
+ * propagation steps.  interval 0 (or 1) stores every row.  Applies to the
+ * 2-gradient strip path; SCF_E_UNSUPPORTED elsewhere.  The schedule is
+ * uniform rather than the reference's per-block round(sqrt(2 N_b)) plan,
+ * which changes no output. */
+int scf_system_set_checkpoint(scf_system *sys, int interval);
+
 /* Override solver settings: tolerance, deltamax, DIIS history m, iteration
  * limit (values <= 0 keep the input's setting). */
 int scf_system_set_solver(scf_system *sys, double tolerance, double deltamax, int m, int iterationlimit);
@@ -166,6 +178,7 @@ typedef struct {
   double free_energy;
   double grand_potential;
   double solve_ms;     /* device time of this solve */
+  double phase_ms[4];  /* of which: Boltzmann weights, propagator chains, densities + residual, the rest */
 } scf_job_result;
 
 /* Persistent job queue (parameter sweeps, many seeds of one problem).

@@ -483,6 +483,19 @@ int scf_system_set_precision(scf_system *S, int mode, double switch_residual) {
   return upload_solver(S);
 }
 
+int scf_system_set_checkpoint(scf_system *S, int interval) {
+  if (!S) return fail(SCF_E_ARG, "null system");
+  if (interval < 0) return fail(SCF_E_ARG, "negative checkpoint interval");
+  if (interval <= 1) {
+    S->h.ckpt = 0;
+    return upload_solver(S);
+  }
+  if (S->h.strip_sx == 0 || S->effective_mode() != SCF_MODE_FUSED)
+    return fail(SCF_E_UNSUPPORTED, "checkpointed propagators need the 2-gradient register-strip path");
+  S->h.ckpt = interval;
+  return upload_solver(S);
+}
+
 int scf_system_set_solver(scf_system *S, double tolerance, double deltamax, int m, int iterationlimit) {
   if (!S) return fail(SCF_E_ARG, "null system");
   if (m > S->m_alloc) return fail(SCF_E_UNSUPPORTED, "DIIS history m above the input's setting");
@@ -639,6 +652,7 @@ int scf_solve_jobs(scf_system *const *slots, int nslots, int chain, const double
     r.free_energy = h[i].F;
     r.grand_potential = h[i].GP;
     r.solve_ms = (double)(h[i].t_end - h[i].t_begin) / khz;
+    for (int q = 0; q < 4; q++) r.phase_ms[q] = (double)h[i].t_phase[q] / khz;
   }
   if (kernel_ms) *kernel_ms = ms;
   for (size_t i = 0; i < nsys; i++) slots[i]->have_results = 1;

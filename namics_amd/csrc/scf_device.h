@@ -56,6 +56,7 @@ struct JobRecord {
   int iterations, status, fp32_iterations, converged;
   double residual, F, GP;
   unsigned long long t_begin, t_end;
+  unsigned long long t_phase[4];  // prepare, chains, densities+residual, rest (device ticks)
 };
 
 struct DevSys {
@@ -111,6 +112,7 @@ struct DevSys {
   int carry_src[kMaxSeg]; // job queue: field block of the preceding system for each block (-1: zero)
   double mixed_switch;    // residual below which a mixed solve continues in FP64
   double volume;          // weighted interior volume (sum of site weights)
+  int ckpt;               // 2-gradient strip path: keep every ckpt-th end-point row (0: all rows)
   // ---- scalars and reduction workspace
   DevScalars *sc;
   double *partial;        // per-workgroup partial sums (multi-workgroup path)

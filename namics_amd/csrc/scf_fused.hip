@@ -391,48 +391,27 @@ __device__ __forceinline__ void strip_step(const __attribute__((address_space(3)
   }
 }
 
-template <typename T, int SX, int STRIDE>
-__device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shift) {
-  typedef __attribute__((address_space(3))) T LT;
-  typedef __attribute__((address_space(1))) T GT;
-  CSys &S = uniform_sys(S0);
-  const int MX = S.MX, MY = S.MY, JX = S.JX;
-  const int nact = (MX / SX) * MY;
-  const int t = threadIdx.x;
-  const bool active = t < nact;
-  const int xs = active ? t / MY : 0;
-  const int y = active ? t - xs * MY + 1 : 1;
-  const int x0 = xs * SX + 1;
-  // neighbours after set_bounds: y +- 1 as slot offsets inside the strip,
-  // x0-1 and x0+SX as slots of the strips that own them
-  const int dm = map_y(S, y - 1) - y, dp = map_y(S, y + 1) - y;
-  const int xl = map_x(S, x0 - 1) - 1, xr = map_x(S, x0 + SX) - 1;
-  const int sl = (xl % SX) * STRIDE + (xl / SX) * MY + (y - 1);
-  const int sr = (xr % SX) * STRIDE + (xr / SX) * MY + (y - 1);
-  const int tt = active ? t : 0;
-  const int am0 = tt + dm, ap0 = tt + dp;
-  const bool cyl = S.geometry != 0;
-  const bool sc = S.lattice_type == 0;
-  const T C1 = sc ? (T)(4.0 / 6.0) : (T)0.5, C2 = sc ? (T)(1.0 / 6.0) : (T)0.25;
-  const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
-  const int N = S.mol_N[mo];
-  const size_t M = S.M;
-  const size_t RG = (size_t)SX * nact;  // Gf row length (interior sites)
-  GT *Gf = (GT *)S.Gf + tt;
-  const gdbl *G1 = hbm(S.G1);
-  LT *ra = (LT *)lds_row(0), *rb = ra + SX * STRIDE;
-  LT *lam = (LT *)s_lam;
-  for (int x = threadIdx.x; x < MX + 2; x += blockDim.x) {
-    lam[2 * x] = cyl ? (T)hbm(S.lam_m)[x * JX + 1] : C2;
-    lam[2 * x + 1] = cyl ? (T)hbm(S.lam_p)[x * JX + 1] : C2;
-  }
-  lds_barrier();
-  const LT *lamx = lam + 2 * x0;
-  const size_t site0 = (size_t)x0 * JX + y;  // site of column 0; column k is site0 + k JX
-  const double eshift = exp(shift);
-  T g1[SX], v[SX], gfv[SX];
-  double racc[SX];
+// Per-thread geometry of the strip layout: thread t < nact owns the SX sites
+// (x0 .. x0+SX-1, y) of strip column t, held in LDS slots k*STRIDE + t.
+// Neighbours after set_bounds: y +- 1 as slot offsets inside the strip,
+// x0-1 and x0+SX as slots of the strips that own them.  site0 is the site of
+// column 0; column k is site0 + k JX.
+#define STRIP_GEOM                                                     \
+  const int nact = (S.MX / SX) * S.MY;                                 \
+  const bool active = (int)threadIdx.x < nact;                         \
+  const int xs_ = active ? (int)threadIdx.x / S.MY : 0;                \
+  const int y_ = active ? (int)threadIdx.x - xs_ * S.MY + 1 : 1;       \
+  const int x0 = xs_ * SX + 1;                                         \
+  const int dm = map_y(S, y_ - 1) - y_, dp = map_y(S, y_ + 1) - y_;    \
+  const int xl_ = map_x(S, x0 - 1) - 1, xr_ = map_x(S, x0 + SX) - 1;   \
+  const int sl = (xl_ % SX) * STRIDE + (xl_ / SX) * S.MY + (y_ - 1);   \
+  const int sr = (xr_ % SX) * STRIDE + (xr_ / SX) * S.MY + (y_ - 1);   \
+  const int tt = active ? (int)threadIdx.x : 0;                        \
+  const int am0 = tt + dm, ap0 = tt + dp;                              \
+  const size_t site0 = (size_t)x0 * S.JX + y_;
 
+// Row access of the strip recursion (local names: g1, v, gfv, G1, Gf, M,
+// site0, JX, eshift, tt, nact, RG)
 #define LOAD_G1(seg)                                                                  \
   {                                                                                   \
     const gdbl *p_ = G1 + (size_t)(seg) * M + site0;                                  \
@@ -453,7 +432,182 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shif
     _Pragma("unroll") for (int k = 0; k < SX; k++) gfv[k] = q_[(size_t)k * nact];   \
   }
 
-  // ---- forward
+// Checkpointed backward pass with composition (save_memory, the
+// reference's src/molecule.cpp:2375-2410): Gf holds rows 0, K, 2K, ... of
+// the forward recursion; the rows of segment q (qK+1 .. qK+K-1) are
+// recomputed from checkpoint q into a buffer of K-1 rows (Gf rows nck ..
+// nck+K-2) just before the backward pass needs them.  The recomputation
+// repeats the forward arithmetic exactly, so the densities are bit-identical
+// to full storage.  It borrows gfv as its state, g1 for its weights and both
+// LDS rows; afterwards every thread rewrites its backward values (v) into the
+// current row.  Expects the LDS weights (s_lam) of the forward pass.
+template <typename T, int SX, int STRIDE>
+__device__ __noinline__ void strip_backward_ckpt(const DevSys &S0, int mo, double shift) {
+  typedef __attribute__((address_space(3))) T LT;
+  typedef __attribute__((address_space(1))) T GT;
+  CSys &S = uniform_sys(S0);
+  const int JX = S.JX;
+  STRIP_GEOM;
+  const bool sc = S.lattice_type == 0;
+  const T C1 = sc ? (T)(4.0 / 6.0) : (T)0.5, C2 = sc ? (T)(1.0 / 6.0) : (T)0.25;
+  const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
+  const int N = S.mol_N[mo];
+  const size_t M = S.M;
+  const size_t RG = (size_t)SX * nact;
+  GT *Gf = (GT *)S.Gf + tt;
+  const gdbl *G1 = hbm(S.G1);
+  LT *ra = (LT *)lds_row(0), *rb = ra + SX * STRIDE;
+  const LT *lamx = (LT *)s_lam + 2 * x0;
+  const double eshift = exp(shift);
+  const double unshift = exp(-shift * (N + 1));
+  T g1[SX], v[SX], gfv[SX];
+  double racc[SX];
+  const int K = S.ckpt;
+  LT *ga = ra, *gb = rb;
+  const int bl = b1 - 1;
+  const int nck = (N + K - 1) / K;
+  GT *Bf = Gf + (size_t)nck * RG;
+  // row r of the forward recursion: a checkpoint or a recomputed row
+#define LOAD_ROWF(r_)                                                                   \
+  {                                                                                     \
+    const int rr_ = (r_);                                                               \
+    const GT *q_ = (rr_ % K == 0) ? Gf + (size_t)(rr_ / K) * RG                         \
+                                  : Bf + (size_t)(rr_ % K - 1) * RG;                    \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) gfv[k] = q_[(size_t)k * nact];      \
+  }
+  int b = bl, bstart = N - S.block_n[bl], q = -1;
+  if (active) LOAD_G1(S.block_seg[bl]);
+#pragma unroll
+  for (int k = 0; k < SX; k++) {
+    racc[k] = 0.0;
+    v[k] = g1[k];
+  }
+  for (int s2 = N - 1; s2 >= 0; s2--) {
+    if (s2 < bstart) {
+      if (active) {
+        gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[b] * M + site0;
+#pragma unroll
+        for (int k = 0; k < SX; k++) rho[k * JX] += racc[k] * unshift;
+      }
+      b--;
+      bstart -= S.block_n[b];
+      if (active) LOAD_G1(S.block_seg[b]);
+#pragma unroll
+      for (int k = 0; k < SX; k++) racc[k] = 0.0;
+    }
+    if (s2 / K != q) {
+      q = s2 / K;
+      const int a0 = q * K, e0 = (a0 + K < N ? a0 + K : N) - 1;
+      if (e0 > a0) {
+        // forward state from checkpoint q, weights of the block of row a0+1
+        int rb0 = b0, rst = 0;
+        while (rst + S.block_n[rb0] <= a0 + 1) {
+          rst += S.block_n[rb0];
+          rb0++;
+        }
+        if (active) {
+          const GT *c_ = Gf + (size_t)q * RG;
+#pragma unroll
+          for (int k = 0; k < SX; k++) gfv[k] = c_[(size_t)k * nact];
+          LOAD_G1(S.block_seg[rb0]);
+        }
+        lds_barrier();
+        if (active) {
+#pragma unroll
+          for (int k = 0; k < SX; k++) ra[k * STRIDE + tt] = gfv[k];
+        }
+        lds_barrier();
+        LT *pr = ra, *nx = rb;
+        for (int r = a0 + 1; r <= e0; r++) {
+          if (r >= rst + S.block_n[rb0]) {
+            rst += S.block_n[rb0];
+            rb0++;
+            if (active) LOAD_G1(S.block_seg[rb0]);
+          }
+          if (active) {
+            strip_step<T, SX, STRIDE>(pr, am0, ap0, sl, sr, dm, dp, lamx, C1, C2, g1, gfv);
+            GT *w_ = Bf + (size_t)(r - a0 - 1) * RG;
+#pragma unroll
+            for (int k = 0; k < SX; k++) {
+              w_[(size_t)k * nact] = gfv[k];
+              nx[k * STRIDE + tt] = gfv[k];
+            }
+          }
+          lds_barrier();
+          LT *tmp = pr;
+          pr = nx;
+          nx = tmp;
+        }
+        // back to the backward pass: its current row and weights
+        if (active) {
+          STORE_ROW(ga);
+          LOAD_G1(S.block_seg[b]);
+        }
+        lds_barrier();
+      }
+      if (active) LOAD_ROWF(s2);
+    }
+    if (active) {
+      if (s2 == N - 1) {
+#pragma unroll
+        for (int k = 0; k < SX; k++) v[k] = g1[k];
+      } else {
+        strip_step<T, SX, STRIDE>(ga, am0, ap0, sl, sr, dm, dp, lamx, C1, C2, g1, v);
+      }
+#pragma unroll
+      for (int k = 0; k < SX; k++) racc[k] += (double)gfv[k] * (double)v[k];
+      if (s2 == N - 1) {
+        STORE_ROW(ga);
+      } else {
+        STORE_ROW(gb);
+      }
+      if (s2 >= 1 && (s2 - 1) / K == q) LOAD_ROWF(s2 - 1);
+    }
+    lds_barrier();
+    if (s2 != N - 1) {
+      LT *tmp = ga;
+      ga = gb;
+      gb = tmp;
+    }
+  }
+  if (active) {
+    gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[b] * M + site0;
+#pragma unroll
+    for (int k = 0; k < SX; k++) rho[k * JX] += racc[k] * unshift;
+  }
+#undef LOAD_ROWF
+}
+
+template <typename T, int SX, int STRIDE, bool kCk>
+__device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shift) {
+  typedef __attribute__((address_space(3))) T LT;
+  typedef __attribute__((address_space(1))) T GT;
+  CSys &S = uniform_sys(S0);
+  const int MX = S.MX, JX = S.JX;
+  STRIP_GEOM;
+  const bool cyl = S.geometry != 0;
+  const bool sc = S.lattice_type == 0;
+  const T C1 = sc ? (T)(4.0 / 6.0) : (T)0.5, C2 = sc ? (T)(1.0 / 6.0) : (T)0.25;
+  const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
+  const int N = S.mol_N[mo];
+  const size_t M = S.M;
+  const size_t RG = (size_t)SX * nact;  // Gf row length (interior sites)
+  GT *Gf = (GT *)S.Gf + tt;
+  const gdbl *G1 = hbm(S.G1);
+  LT *ra = (LT *)lds_row(0), *rb = ra + SX * STRIDE;
+  LT *lam = (LT *)s_lam;
+  for (int x = threadIdx.x; x < MX + 2; x += blockDim.x) {
+    lam[2 * x] = cyl ? (T)hbm(S.lam_m)[x * JX + 1] : C2;
+    lam[2 * x + 1] = cyl ? (T)hbm(S.lam_p)[x * JX + 1] : C2;
+  }
+  lds_barrier();
+  const LT *lamx = lam + 2 * x0;
+  const double eshift = exp(shift);
+  T g1[SX], v[SX], gfv[SX];
+  double racc[SX];
+
+  // ---- forward (kCk: keep only every K-th row, K = S.ckpt > 1)
+  const int K = kCk ? S.ckpt : 1;
   if (active) LOAD_G1(S.block_seg[b0]);
 #pragma unroll
   for (int k = 0; k < SX; k++) v[k] = active ? g1[k] : (T)0;
@@ -470,7 +624,11 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shif
     for (int j = 0; j < n; j++, s++) {
       if (active) {
         strip_step<T, SX, STRIDE>(prev, am0, ap0, sl, sr, dm, dp, lamx, C1, C2, g1, v);
-        STORE_GF(s);
+        if constexpr (!kCk) {
+          STORE_GF(s);
+        } else if (s % K == 0) {
+          STORE_GF(s / K);
+        }
         STORE_ROW(next);
       }
       lds_barrier();
@@ -495,6 +653,9 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shif
   const double unshift = exp(-shift * (N + 1));
   LT *ga = ra, *gb = rb;
   const int bl = b1 - 1;
+  if constexpr (kCk) {
+    strip_backward_ckpt<T, SX, STRIDE>(S0, mo, shift);
+  } else {
   if (active) {
     LOAD_G1(S.block_seg[bl]);
     LOAD_GF(N - 1);
@@ -534,6 +695,8 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shif
       for (int k = 0; k < SX; k++) rho[k * JX] += racc[k] * unshift;
     }
   }
+  }
+#undef STRIP_GEOM
 #undef LOAD_G1
 #undef STORE_ROW
 #undef STORE_GF
@@ -671,20 +834,38 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
 template <typename T>
 __device__ __forceinline__ void chain_strip_select(const DevSys &S, int mo, double shift) {
   const int sx = S.strip_sx, st = S.strip_stride;
+  if (S.ckpt > 1) {
+    if (sx == 8) {
+      if (st == 256)
+        chain2d_strip<T, 8, 256, true>(S, mo, shift);
+      else if (st == 512)
+        chain2d_strip<T, 8, 512, true>(S, mo, shift);
+      else
+        chain2d_strip<T, 8, 1024, true>(S, mo, shift);
+    } else {
+      if (st == 256)
+        chain2d_strip<T, 4, 256, true>(S, mo, shift);
+      else if (st == 512)
+        chain2d_strip<T, 4, 512, true>(S, mo, shift);
+      else
+        chain2d_strip<T, 4, 1024, true>(S, mo, shift);
+    }
+    return;
+  }
   if (sx == 8) {
     if (st == 256)
-      chain2d_strip<T, 8, 256>(S, mo, shift);
+      chain2d_strip<T, 8, 256, false>(S, mo, shift);
     else if (st == 512)
-      chain2d_strip<T, 8, 512>(S, mo, shift);
+      chain2d_strip<T, 8, 512, false>(S, mo, shift);
     else
-      chain2d_strip<T, 8, 1024>(S, mo, shift);
+      chain2d_strip<T, 8, 1024, false>(S, mo, shift);
   } else {
     if (st == 256)
-      chain2d_strip<T, 4, 256>(S, mo, shift);
+      chain2d_strip<T, 4, 256, false>(S, mo, shift);
     else if (st == 512)
-      chain2d_strip<T, 4, 512>(S, mo, shift);
+      chain2d_strip<T, 4, 512, false>(S, mo, shift);
     else
-      chain2d_strip<T, 4, 1024>(S, mo, shift);
+      chain2d_strip<T, 4, 1024, false>(S, mo, shift);
   }
 }
 
@@ -1169,6 +1350,7 @@ __global__ void __launch_bounds__(1024) k_fused_jobs(DevSys *const *slots, int c
         r.GP = sc.GP;
         r.t_begin = sc.t_begin;
         r.t_end = sc.t_end;
+        for (int q = 0; q < 4; q++) r.t_phase[q] = sc.t_phase[q];
       }
     }
   }

@@ -39,7 +39,8 @@ class SolveInfo(ctypes.Structure):
 class JobResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("fp32_iterations", ctypes.c_int),
                 ("residual", ctypes.c_double), ("free_energy", ctypes.c_double),
-                ("grand_potential", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+                ("grand_potential", ctypes.c_double), ("solve_ms", ctypes.c_double),
+                ("phase_ms", ctypes.c_double * 4)]
 
 
 class Served(ctypes.Structure):
@@ -96,6 +97,7 @@ def lib():
         L.scf_propagate_chain.argtypes = [vp, _D, ctypes.c_int, _D]
         L.scf_get_phase_ms.argtypes = [vp, _D]
         L.scf_system_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.scf_system_set_checkpoint.argtypes = [vp, ctypes.c_int]
         L.scf_describe_input.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
         L.scf_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(vp)]
         L.scf_server_set_slice.argtypes = [vp, ctypes.c_double]
@@ -219,6 +221,11 @@ class ScfSystem:
         """FP64 throughout (mixed=False) or FP32 propagators until the
         residual falls below `switch_residual`, then FP64 (mixed=True)."""
         _check(lib().scf_system_set_precision(self.h, 1 if mixed else 0, float(switch_residual)))
+
+    def set_checkpoint(self, interval):
+        """Keep every `interval`-th end-point distribution and recompute the
+        rest in the backward pass (save_memory); 0 stores every row."""
+        _check(lib().scf_system_set_checkpoint(self.h, int(interval)))
 
     def phase_ms(self):
         """Device time of the last fused residual / solve by phase (ms):
@@ -358,8 +365,11 @@ def solve_jobs(slots, seeds, njobs):
     res = (JobResult * (njobs * chain))()
     ms = ctypes.c_double(0.0)
     _check(lib().scf_solve_jobs(hs, nslots, chain, seeds.ptr, njobs, res, ctypes.byref(ms)))
-    out = [[{f: getattr(res[j * chain + c], f) for f, _ in JobResult._fields_} for c in range(chain)]
-           for j in range(njobs)]
+    def conv(r):
+        d = {f: getattr(r, f) for f, _ in JobResult._fields_}
+        d["phase_ms"] = list(d["phase_ms"])
+        return d
+    out = [[conv(res[j * chain + c]) for c in range(chain)] for j in range(njobs)]
     return out, ms.value
 
 

@@ -5,6 +5,8 @@ FP64 mode must reproduce the reference at given fields to 1e-10 relative
 oracle restates the reference bit-exactly, so the GPU is also held to 1e-12
 against the oracle where no cancellation is involved.
 """
+import re
+
 import numpy as np
 import pytest
 
@@ -234,6 +236,73 @@ def test_job_queue_equals_alone(golden, case):
             assert r["free_energy"] == F, (j, c)
     for s in a + [s for row in slots for s in row]:
         s.close()
+
+
+@pytest.mark.parametrize("case", ["brush2d_rod_strip", "hex2d_cyl_strip"])
+def test_checkpointed_propagators_equal_full_storage(golden, case):
+    """save_memory: keeping every K-th end-point distribution and recomputing
+    the rows between checkpoints in the backward pass repeats the forward
+    arithmetic exactly, so residuals, densities and partition functions are
+    bit-identical to full storage for every interval (2, 3, 5, N-1, N, N+4),
+    also for a chain of four blocks whose boundaries fall inside the
+    recomputed segments; and FP64 and mixed-precision solves follow the same
+    trajectory to the last bit.  Lattices off the strip path refuse the
+    option."""
+    from namics_amd import ScfSystem, ScfError, solve_resident_info
+    fx = golden[case]
+    o0 = Oracle(fx["text"])
+    texts = [fx["text"]]
+    if "(X)1(A)39" in fx["text"]:
+        texts.append(fx["text"].replace("(X)1(A)39", "(X)1(A)12(W)5(A)22"))
+    else:
+        texts.append(fx["text"].replace("(A)25", "(A)7(W)4(A)14"))
+    for text in texts:
+        ref = ScfSystem(text)
+        N = max(sum(int(n) for n in re.findall(r"\)(\d+)", c)) for c in re.findall(r"composition : (\S+)", text))
+        rng = np.random.default_rng(17)
+        xs = [rng.uniform(-0.3, 0.3, ref.nvar) for _ in range(2)]
+        full = []
+        for x in xs:
+            g, r = ref.residual(x)
+            res = ref.results()
+            full.append((g, r, res))
+        for K in (2, 3, 5, N - 1, N, N + 4):
+            s = ScfSystem(text)
+            s.set_checkpoint(K)
+            print("case", case, "N", N, "K", K, flush=True)
+            for x, (g0, r0, res0) in zip(xs, full):
+                g, r = s.residual(x)
+                res = s.results()
+                assert np.array_equal(g, g0) and r == r0, K
+                for m in res0["mol"]:
+                    assert res["mol"][m]["GN"] == res0["mol"][m]["GN"], (K, m)
+                for sn in res0["seg_phi"]:
+                    assert np.array_equal(res["seg_phi"][sn], res0["seg_phi"][sn]), (K, sn)
+            s.close()
+        ref.close()
+    solve_text = texts[1] + "\nnewton : %s : method : DIIS\nnewton : %s : tolerance : 1e-09\n" \
+        "newton : %s : iterationlimit : 3000\n" % (o0.newton_name, o0.newton_name, o0.newton_name)
+    x0 = np.random.default_rng(5).uniform(-0.05, 0.05, o0.nvar)
+    for mixed in (False, True):
+        out = []
+        for K in (0, 4):
+            s = ScfSystem(solve_text)
+            s.set_precision(mixed, 1e-5)
+            s.set_checkpoint(K)
+            print("case", case, "solve mixed", mixed, "K", K, flush=True)
+            s.upload(x0)
+            info = solve_resident_info([s])[0]
+            out.append((info, s.download(), s.results()["free_energy"]))
+            s.close()
+        (i0, x_0, F0), (i1, x_1, F1) = out
+        assert (i1["iterations"], i1["status"], i1["fp32_iterations"], i1["residual"]) == \
+            (i0["iterations"], i0["status"], i0["fp32_iterations"], i0["residual"])
+        assert np.array_equal(x_1, x_0) and F1 == F0
+    off = ScfSystem(golden["brush2d_cyl"]["text"])
+    with pytest.raises(ScfError):
+        off.set_checkpoint(4)
+    off.set_checkpoint(0)
+    off.close()
 
 
 def test_server_results_equal_alone_and_admission_waits(golden):
