@@ -135,6 +135,13 @@ int build(scf_system *S, const Placement *pl) {
   h.bzm = l.BZM;
   h.surface_lo = l.bc[0] == BC_SURFACE;
   h.surface_hi = l.bc[3] == BC_SURFACE;
+  static_assert(kMaxFjc >= kMaxFjcSpec, "refinement table too small");
+  h.fjc = l.fjc;
+  for (int k = 0; k < l.fjc; k++) {
+    h.bx1f[k] = l.bx1f[k];
+    h.bxmf[k] = l.bxmf[k];
+  }
+  h.wplanar = 1.0 / l.fjc;
   h.lambda = l.lambda;
   h.nseg = nseg;
   h.nit = nit;
@@ -195,6 +202,7 @@ int build(scf_system *S, const Placement *pl) {
   const bool m2curved = !l.l1.empty();
   size_t o_L = curved ? a.take(4 * M * D) : 0;
   size_t o_l1 = m2curved ? a.take(4 * M * D) : 0;
+  size_t o_LAM = l.fjc > 1 ? a.take((size_t)l.FJC * M * D) : 0;
   size_t o_mask = a.take((size_t)nseg * M * D);
   size_t o_ksam = a.take(M * D);
   size_t o_x = a.take(nvar * D);
@@ -242,6 +250,10 @@ int build(scf_system *S, const Placement *pl) {
     HIP_TRY(hipMemcpyAsync((void *)h.lam_m, l.lambda_1.data(), M * D, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync((void *)h.lam_0, l.lambda0.data(), M * D, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync((void *)h.lam_p, l.lambda1.data(), M * D, hipMemcpyHostToDevice, st));
+  }
+  if (l.fjc > 1) {
+    h.LAM = P(o_LAM);
+    HIP_TRY(hipMemcpyAsync((void *)h.LAM, l.LAMBDA.data(), (size_t)l.FJC * M * D, hipMemcpyHostToDevice, st));
   }
   if (m2curved) {
     h.l1 = P(o_l1);
@@ -313,7 +325,8 @@ int build(scf_system *S, const Placement *pl) {
   {
     // weighted interior volume (Lattice::WeightedSum of ones)
     double vol = 0;
-    for (int x = 1; x <= l.MX; x++)
+    if (l.gradients == 1 && l.fjc > 1) vol = l.volume;
+    for (int x = 1; l.fjc == 1 && x <= l.MX; x++)
       for (int y = 1; y <= (l.gradients >= 2 ? l.MY : 1); y++)
         for (int z = 1; z <= (l.gradients == 3 ? l.MZ : 1); z++) {
           const int i = l.gradients == 1 ? x : (l.gradients == 2 ? x * h.JX + y : x * h.JX + y * h.JY + z);

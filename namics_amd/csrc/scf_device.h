@@ -23,6 +23,7 @@ constexpr int kMaxSeg = 8;
 constexpr int kMaxMol = 8;
 constexpr int kMaxBlocks = 16;
 constexpr int kMaxMM = 16;
+constexpr int kMaxFjc = 4;   // lattice refinement up to FJC_choices 9
 
 enum DevStatus : int {
   DS_OK = 0,
@@ -65,6 +66,10 @@ struct DevSys {
   int MX, MY, MZ, M, JX, JY;
   int bx1, bxm, by1, bym, bz1, bzm;
   int surface_lo, surface_hi;  // 1-gradient surface boundaries
+  int fjc;                     // lattice refinement (grid layers per bond; > 1 only with 1 gradient)
+  int bx1f[kMaxFjc], bxmf[kMaxFjc];  // refined 1-gradient ghost sources
+  double wplanar;              // site weight of planar sums: 1 / fjc
+  const double *LAM;           // refined 1-gradient weights: (2 fjc + 1) rows of M
   double lambda;
   const double *L, *lam_m, *lam_0, *lam_p;  // M each (1 and 2 gradients)
   const double *l1, *l11, *l_1, *l_11;      // Markov-2 curved weights (1 gradient)
@@ -145,6 +150,15 @@ template <typename Sys>
 __device__ __forceinline__ int map_x(const Sys &S, int x) {
   return x < 1 ? S.bx1 : (x > S.MX ? S.bxm : x);
 }
+// 1-gradient site x (0..M-1) -> the site set_bounds copies there; on a
+// refined lattice the ghosts are fjc layers per side (src/LGrad1.cpp:823-835)
+template <typename Sys>
+__device__ __forceinline__ int map_x1(const Sys &S, int x) {
+  if (S.fjc == 1) return map_x(S, x);
+  if (x < S.fjc) return S.bx1f[x];
+  if (x >= S.MX + S.fjc) return S.bxmf[x - S.MX - S.fjc];
+  return x;
+}
 template <typename Sys>
 __device__ __forceinline__ int map_y(const Sys &S, int y) {
   return y < 1 ? S.by1 : (y > S.MY ? S.bym : y);
@@ -185,6 +199,7 @@ __device__ __forceinline__ int work_site(const Sys &S, int w, int &x, int &y, in
 
 template <typename Sys>
 __device__ __forceinline__ bool is_interior(const Sys &S, int x, int y, int z) {
+  if (S.gradients == 1) return x >= S.fjc && x < S.MX + S.fjc;
   if (x < 1 || x > S.MX) return false;
   if (S.gradients >= 2 && (y < 1 || y > S.MY)) return false;
   if (S.gradients == 3 && (z < 1 || z > S.MZ)) return false;
@@ -198,10 +213,31 @@ __device__ __forceinline__ bool is_interior(const Sys &S, int x, int y, int z) {
 // LGrad1::propagate src/LGrad1.cpp:441-445, LGrad2::propagate
 // src/LGrad2.cpp:1058-1073, LG2Planar::propagate src/LG2Planar.cpp:1085-1096,
 // LGrad3::propagate src/LGrad3.cpp:733-743).
+// Refined 1-gradient stencil (fjc > 1): 2 fjc + 1 taps weighted by the
+// LAMBDA rows, summed in the order of LG1Planar::propagate / LGrad1::
+// propagate / Side (src/LG1Planar.cpp:151-156, src/LGrad1.cpp:447-452,
+// src/LG1Planar.cpp:47-52): for distance fjc down to 1 the lower then the
+// upper neighbour, the site itself last.  A neighbour outside 0..M-1 is
+// missing (the reference's shifted passes stop at the array ends).
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ T refined_taps(const Sys &S, Ptr g, int x) {
+  const int M = S.M, f = S.fjc, F = 2 * f + 1;
+  const gdbl *LA = hbm(S.LAM);
+  T v = 0;
+  for (int j = 0; j < f; j++) {
+    const int kk = f - j;
+    if (x - kk >= 0) v += (T)LA[j * M + x] * g[map_x1(S, x - kk)];
+    if (x + kk <= M - 1) v += (T)LA[(F - 1 - j) * M + x] * g[map_x1(S, x + kk)];
+  }
+  v += (T)LA[f * M + x] * g[map_x1(S, x)];
+  return v;
+}
+
 template <typename T, typename Sys, typename Ptr>
 __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int y, int z) {
   if (S.gradients == 1) {
     const int M = S.M;
+    if (S.fjc > 1) return refined_taps<T>(S, g, x);
     if (S.geometry == 0) {
       const T lam = (T)S.lambda, c0 = (T)(1.0 - 2.0 * S.lambda);
       T v = 0;
@@ -273,6 +309,7 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
 // the propagator; the 1-gradient planar form sums in a different order.
 template <typename Sys, typename Ptr>
 __device__ __forceinline__ double stencil_side(const Sys &S, Ptr X, int i, int x, int y, int z) {
+  if (S.gradients == 1 && S.fjc > 1) return refined_taps<double>(S, X, x);
   if (S.gradients == 1 && S.geometry == 0) {
     const int M = S.M;
     const double lam = S.lambda;
@@ -388,7 +425,7 @@ template <typename Sys>
 __device__ __forceinline__ void site_prepare(const Sys &S, int i, int x, int y, int z) {
   int b;  // bounded source site for u
   if (S.gradients == 1) {
-    b = map_x(S, x);
+    b = map_x1(S, x);
   } else if (S.gradients == 2) {
     b = map_x(S, x) * S.JX + map_y(S, y);
   } else {
@@ -541,12 +578,12 @@ __device__ __forceinline__ void site_thermo(const Sys &S, int i, double logc[kMa
   GP = -gp * ks;
 }
 
-// site weight of sums over the lattice (Lattice::WeightedSum): 1 on planar
-// lattices, the site volume L on curved ones; ghosts are excluded by the
-// callers (interior work sites only).
+// site weight of sums over the lattice (Lattice::WeightedSum): 1 / fjc on
+// planar lattices, the site volume L on curved ones; ghosts are excluded by
+// the callers (interior work sites only).
 template <typename Sys>
 __device__ __forceinline__ double site_weight(const Sys &S, int i) {
-  if (S.gradients == 3 || S.geometry == 0) return 1.0;
+  if (S.gradients == 3 || S.geometry == 0) return S.wplanar;
   return hbm(S.L)[i];
 }
 

@@ -904,7 +904,7 @@ __device__ double fused_residual(const DevSys &S0) {
         chain_markov2<false>(S0, mo);
     } else if (S.strip_sx > 0) {
       chain_strip_dispatch(S0, mo, fp32, fp32 ? sc.lnshift[mo] : 0.0);
-    } else if (S.gradients == 1 && S.use_lds && S.M <= S.threads) {
+    } else if (S.gradients == 1 && S.fjc == 1 && S.use_lds && S.M <= S.threads) {
       chain1d_m1(S0, mo);
     } else if (S.use_lds) {
       chain_markov1<true>(S0, mo);

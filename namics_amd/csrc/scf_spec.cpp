@@ -157,6 +157,80 @@ void boundary_indices(LatticeSpec &l) {
   l.BYM = upper(l.bc[4], l.MY);
   l.BZ1 = lower(l.bc[2], l.MZ);
   l.BZM = upper(l.bc[5], l.MZ);
+  // refined 1-gradient lattice: fjc ghost layers per side
+  const int f = l.fjc;
+  l.bx1f.assign(f, 0);
+  l.bxmf.assign(f, 0);
+  for (int k = 0; k < f; k++) {
+    l.bx1f[k] = l.bc[0] == BC_MIRROR ? 2 * f - 1 - k : (l.bc[0] == BC_PERIODIC ? l.MX + k : k);
+    l.bxmf[k] = l.bc[3] == BC_MIRROR ? l.MX + f - k - 1 : (l.bc[3] == BC_PERIODIC ? f + k : l.MX + f + k);
+  }
+}
+
+// Weights of the refined (fjc > 1) 1-gradient lattices: LG1Planar::
+// ComputeLambdas (src/LG1Planar.cpp:14-25) and the fjc > 1 branch of
+// LGrad1::ComputeLambdas (src/LGrad1.cpp:72-163).  The curved weights split
+// the bond directions over the FJC sub-layers by the radii of their shells;
+// the expressions keep the reference's operation order so that the weights
+// are the same doubles.
+void refined_lambdas(LatticeSpec &l) {
+  const int M = l.M, f = l.fjc, F = l.FJC, MX = l.MX;
+  const double off = l.offset_first_layer;
+  l.L.assign(M, 0.0);
+  l.lambda_1.assign(M, 0.0);  // the fjc = 1 weights stay zero (unused)
+  l.lambda0.assign(M, 0.0);
+  l.lambda1.assign(M, 0.0);
+  l.LAMBDA.assign((size_t)F * M, 0.0);
+  double *LA = l.LAMBDA.data();
+  if (l.geometry == GEOM_PLANAR) {
+    for (int i = 0; i < M; i++) {
+      l.L[i] = 1.0 / f;
+      LA[i] = 1.0 / (2 * (F - 1));
+      LA[i + (F - 1) * M] = 1.0 / (2 * (F - 1));
+      LA[i + (F - 1) / 2 * M] = 1.0 / (F - 1);
+      for (int j = 1; j < F / 2; j++) {
+        LA[i + j * M] = 1.0 / (F - 1);
+        LA[i + (F - j - 1) * M] = 1.0 / (F - 1);
+      }
+    }
+    return;
+  }
+  const bool cyl = l.geometry == GEOM_CYLINDRICAL;
+  const double c = 1.0 / (1.0 * F - 1.0), top = 1.0 * MX / f;
+  // shell weight of a sub-layer boundary at radius q (outer / inner rows)
+  auto wo = [&](double q, double VL) { return cyl ? c * q / VL : 0.5 / (1.0 * F - 1.0) * 4.0 * q * q / VL; };
+  auto wi = [&](double q, double VL) { return cyl ? c * 2.0 * q / VL : c * 4.0 * q * q / VL; };
+  for (int i = f; i < M - f; i++) {
+    const double r = cyl ? off + 1.0 * (i - f + 1.0) / f : off + 1.0 * (1.0 * i - 1.0 * f + 1.0) / f;
+    double rlow = r - 0.5, rhigh = r + 0.5;
+    l.L[i] = cyl ? kPIE * (2.0 * r) / f : kPIE * 4.0 / 3.0 * (rhigh * rhigh * rhigh - rlow * rlow * rlow) / f;
+    const double VL = l.L[i] / kPIE * f;
+    if ((rlow - r) * 2 + r > 0.0) LA[i] += wo(rlow, VL);
+    if ((rhigh - r) * 2 + r < top) {
+      LA[i + (F - 1) * M] += wo(rhigh, VL);
+    } else {
+      const double d = 2 * rhigh - r - top;
+      if (d > -0.001 && d < 0.001) LA[i + (F - 1) * M] += wo(rhigh, VL);
+      for (int j = 1; j <= f; j++)
+        if (d > 0.99 * j / f && d < 1.01 * j / f) LA[i + (F - 1) * M] += wo(rhigh - 1.0 * j / f, VL);
+    }
+    for (int j = 1; j < f; j++) {
+      rlow += 0.5 / (f);
+      rhigh -= 0.5 / (f);
+      if ((rlow - r) * 2 + r > 0.0) LA[i + j * M] += wi(rlow, VL);
+      if ((rhigh - r) * 2 + r < off + top) {
+        LA[i + (F - 1 - j) * M] += wi(rhigh, VL);
+      } else {
+        const double d = 2 * rhigh - r - top;
+        if (d > -0.001 && d < 0.001) LA[i + (F - 1 - j) * M] += wi(rhigh, VL);
+        for (int k = 1; k <= f; k++)
+          if (d > 0.99 * k / f && d < 1.01 * k / f) LA[i + (F - 1 - j) * M] += wi(rhigh - 1.0 * k / f, VL);
+      }
+    }
+    double LS = 0;
+    for (int j = 0; j < F; j++) LS += LA[i + j * M];
+    LA[i + (F / 2) * M] += 1.0 - LS;
+  }
 }
 
 // Site volumes and lambda weights (LG1Planar::ComputeLambdas
@@ -165,6 +239,10 @@ void boundary_indices(LatticeSpec &l) {
 void compute_lambdas(LatticeSpec &l, bool markov2) {
   const int M = l.M;
   if (l.gradients == 3) return;
+  if (l.fjc > 1) {
+    refined_lambdas(l);
+    return;
+  }
   l.L.assign(M, 0.0);
   l.lambda_1.assign(M, 0.0);
   l.lambda0.assign(M, 0.0);
@@ -223,7 +301,8 @@ void compute_lambdas(LatticeSpec &l, bool markov2) {
 // Parse a coordinate token of a range (firstlayer / lastlayer / lowerbound /
 // upperbound / number) for dimension d.
 bool coord(const LatticeSpec &l, const std::string &tok, int d, int &v) {
-  int n = d == 0 ? l.MX : (d == 1 ? l.MY : l.MZ);
+  // coordinates count bond-length layers: n_layers = MX / fjc
+  int n = d == 0 ? l.MX / l.fjc : (d == 1 ? l.MY : l.MZ);
   if (tok == "firstlayer") {
     v = 1;
     return true;
@@ -293,6 +372,9 @@ bool parse_range(const LatticeSpec &l, std::string r, std::vector<int> &mask, st
       err = "range '" + r + "' has its first point above its second";
       return false;
     }
+  // a bond-length layer spans fjc grid layers (src/segment.cpp:145-146)
+  lo[0] *= l.fjc;
+  hi[0] = (hi[0] + 1) * l.fjc - 1;
   for (int x = lo[0]; x <= hi[0]; x++)
     for (int y = (g > 1 ? lo[1] : 0); y <= (g > 1 ? hi[1] : 0); y++)
       for (int z = (g > 2 ? lo[2] : 0); z <= (g > 2 ? hi[2] : 0); z++) mask[site(l, x, y, z)] = 1;
@@ -367,20 +449,36 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
   }
   if (get(p, "lat", ln, "gradients", v) && !to_int(v, l.gradients)) l.gradients = 1;
   if (l.gradients < 1 || l.gradients > 3) l.gradients = 1;
+  // lattice refinement (src/lattice.cpp:296-329): FJC_choices = 3 + 2i or
+  // b/l = fjc, FJC = 2 fjc + 1; inconsistent pairs keep b/l when it is the
+  // smaller, else FJC_choices
   l.fjc = 1;
   l.FJC = 3;
-  if (get(p, "lat", ln, "FJC_choices", v)) {
-    if (!to_int(v, l.FJC) || l.FJC != 3) {
-      err = "FJC_choices other than 3 (lattice refinement) is not supported";
+  const bool has_fjc = get(p, "lat", ln, "FJC_choices", v);
+  if (has_fjc) {
+    if (!to_int(v, l.FJC) || l.FJC < 3 || (l.FJC - 3) % 2 != 0) {
+      err = "FJC_choices can adopt only the values 3 + 2i, i = 0, 1, 2, ...";
       return false;
     }
+    l.fjc = (l.FJC - 1) / 2;
   }
   if (get(p, "lat", ln, "b/l", v)) {
     int b;
-    if (!to_int(v, b) || b != 1) {
-      err = "b/l other than 1 (lattice refinement) is not supported";
+    if (!to_int(v, b) || b < 1) {
+      err = "b/l must be a positive integer";
       return false;
     }
+    if (has_fjc && b != l.fjc && !(b < l.fjc)) b = l.fjc;
+    l.fjc = b;
+    l.FJC = 2 * b + 1;
+  }
+  if (l.fjc > kMaxFjcSpec) {
+    err = "lattice refinement above FJC_choices 9 is not supported";
+    return false;
+  }
+  if (l.fjc > 1 && l.gradients != 1) {
+    err = "lattice refinement (FJC_choices > 3) is supported with 1 gradient";
+    return false;
   }
   if (!get(p, "lat", ln, "lattice_type", v)) {
     err = "'lattice_type' is required";
@@ -394,6 +492,11 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     l.lambda = 1.0 / 4.0;
   } else {
     err = "lattice_type '" + v + "' not recognised";
+    return false;
+  }
+  // src/lattice.cpp:550
+  if (l.fjc > 1 && l.lattice_type != LAT_HEXAGONAL) {
+    err = "for FJC_choices > 3 the lattice_type must be hexagonal";
     return false;
   }
   l.geometry = GEOM_PLANAR;
@@ -430,6 +533,8 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
   bool ok;
   if (l.gradients == 1) {
     if (!need_int("n_layers", l.MX)) return false;
+    l.MX *= l.fjc;
+    l.offset_first_layer *= l.fjc;
     l.bc[0] = get(p, "lat", ln, "lowerbound", v) ? bc_of(v, ok) : BC_MIRROR;
     if (get(p, "lat", ln, "lowerbound", v) && !ok) {
       err = "lowerbound '" + v + "' not recognised";
@@ -486,12 +591,14 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     l.JX = 1;
     l.JY = 0;
     l.JZ = 0;
-    l.M = l.MX + 2;
-    if (l.geometry == GEOM_PLANAR) l.volume = l.MX;
+    const int f = l.fjc;
+    l.M = l.MX + 2 * f;
+    if (l.geometry == GEOM_PLANAR) l.volume = l.MX / f;
     if (l.geometry == GEOM_SPHERICAL)
-      l.volume = 4.0 / 3.0 * kPIE * (std::pow(l.MX + l.offset_first_layer, 3) - std::pow(l.offset_first_layer, 3));
+      l.volume = 4.0 / 3.0 * kPIE * (std::pow(l.MX + l.offset_first_layer, 3) - std::pow(l.offset_first_layer, 3)) /
+                 f / f / f;
     if (l.geometry == GEOM_CYLINDRICAL)
-      l.volume = kPIE * (std::pow(l.MX + l.offset_first_layer, 2) - std::pow(l.offset_first_layer, 2));
+      l.volume = kPIE * (std::pow(l.MX + l.offset_first_layer, 2) - std::pow(l.offset_first_layer, 2)) / f / f;
   } else if (l.gradients == 2) {
     l.JX = l.MY + 2;
     l.JY = 1;
@@ -685,8 +792,8 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     }
     m.size = 1;
     if (m.markov == 2) {
-      if (l.gradients != 1 || l.lattice_type != LAT_SIMPLE_CUBIC) {
-        err = "mol '" + m.name + "': Markov 2 is supported on 1-gradient simple-cubic lattices";
+      if (l.gradients != 1 || l.lattice_type != LAT_SIMPLE_CUBIC || l.fjc != 1) {
+        err = "mol '" + m.name + "': Markov 2 is supported on 1-gradient simple-cubic lattices without refinement";
         return false;
       }
       any_markov2 = true;
@@ -790,7 +897,7 @@ std::string describe_json(const SystemSpec &s) {
   o << ",\"newton_name\":";
   str(s.newton_name);
   const LatticeSpec &l = s.lat;
-  o << ",\"gradients\":" << l.gradients << ",\"geometry\":" << l.geometry << ",\"MX\":" << l.MX
+  o << ",\"gradients\":" << l.gradients << ",\"geometry\":" << l.geometry << ",\"fjc\":" << l.fjc << ",\"MX\":" << l.MX
     << ",\"MY\":" << l.MY << ",\"MZ\":" << l.MZ << ",\"M\":" << l.M << ",\"JX\":" << l.JX
     << ",\"JY\":" << l.JY << ",\"nvar\":" << s.nvar() << ",\"solvent\":" << s.solvent;
   o << ",\"segments\":[";

@@ -42,7 +42,7 @@ struct LatticeSpec {
   double lambda = 1.0 / 6.0;
   // bc[0..2] lower x,y,z; bc[3..5] upper x,y,z (reference BC[6] order)
   int bc[6] = {BC_MIRROR, BC_MIRROR, BC_MIRROR, BC_MIRROR, BC_MIRROR, BC_MIRROR};
-  double offset_first_layer = 0.0;
+  double offset_first_layer = 0.0;  // in grid units (times fjc)
   bool stencil_full = true;
   // derived
   int M = 0, JX = 0, JY = 0, JZ = 1;
@@ -54,7 +54,16 @@ struct LatticeSpec {
   std::vector<double> L, lambda_1, lambda0, lambda1;
   // Markov-2 reflection weights for curved 1-gradient lattices.
   std::vector<double> l1, l11, l_1, l_11;
+  // Lattice refinement (fjc > 1, 1 gradient): ghost layers 0..fjc-1 and
+  // MX+fjc..MX+2fjc-1 take their values from bx1f[k] / bxmf[k]; LAMBDA holds
+  // FJC rows of M weights, row j (j < fjc) toward the site fjc-j below, row
+  // FJC-1-j toward the site fjc-j above, row fjc the site itself.
+  std::vector<int> bx1f, bxmf;
+  std::vector<double> LAMBDA;
 };
+
+// Largest refinement the device path handles (FJC_choices up to 9).
+constexpr int kMaxFjcSpec = 4;
 
 struct SegmentSpec {
   std::string name;

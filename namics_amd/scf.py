@@ -148,6 +148,7 @@ class ScfSystem:
         self.nvar = self.desc["nvar"]
         self.gradients = self.desc["gradients"]
         self.dims = (self.desc["MX"], self.desc["MY"], self.desc["MZ"])
+        self.fjc = self.desc.get("fjc", 1)
         self.seg_names = [s["name"] for s in self.desc["segments"]]
         self.mol_names = [m["name"] for m in self.desc["molecules"]]
         self.field_names = list(self.desc["fields"])
@@ -244,7 +245,7 @@ class ScfSystem:
         mx, my, mz = self.dims
         a = np.asarray(a)
         if self.gradients == 1:
-            return a[1:mx + 1]
+            return a[self.fjc:mx + self.fjc]
         if self.gradients == 2:
             return a.reshape(mx + 2, my + 2)[1:mx + 1, 1:my + 1].ravel()
         return a.reshape(mx + 2, my + 2, mz + 2)[1:mx + 1, 1:my + 1, 1:mz + 1].ravel()

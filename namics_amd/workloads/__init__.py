@@ -162,14 +162,15 @@ def brush_chain_input(chis=(0.0, 0.5, 0.8), **kw):
     return text
 
 
-def write_guess(path, names, dims, x, method="DIIS"):
+def write_guess(path, names, dims, x, method="DIIS", fjc=1):
     """Write fields in the reference's guess-file format (Lattice::StoreGuess,
     src/lattice.cpp:971-1019; read back by Lattice::ReadGuess) at full
-    precision: method line, sizes, charge flag, field-block names, values."""
+    precision: method line, sizes (grid layers) and refinement, charge flag,
+    field-block names, values."""
     mx, my, mz = dims
     with open(path, "w") as f:
         f.write("%s\n" % method)
-        f.write(" %d\t%d\t%d\t%d\n" % (mx, my, mz, 1))
+        f.write(" %d\t%d\t%d\t%d\n" % (mx, my, mz, fjc))
         f.write("false\n")
         f.write("%d\n" % len(names))
         for n in names:

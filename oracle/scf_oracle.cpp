@@ -88,7 +88,13 @@ struct Lat {
 
   void set_bounds(double *X) const {
     const int MX = l.MX, MY = l.MY, MZ = l.MZ, JX = l.JX, JY = l.JY;
-    if (l.gradients == 1) {
+    if (l.gradients == 1 && l.fjc > 1) {
+      // refined lattice: fjc ghost layers per side (src/LGrad1.cpp:823-835)
+      for (int k = 0; k < l.fjc; k++) {
+        X[k] = X[l.bx1f[k]];
+        X[MX + l.fjc + k] = X[l.bxmf[k]];
+      }
+    } else if (l.gradients == 1) {
       X[0] = X[l.BX1];
       X[MX + 1] = X[l.BXM];
     } else if (l.gradients == 2) {
@@ -158,7 +164,12 @@ struct Lat {
 
   void remove_bounds(double *X) const {
     const int MX = l.MX, MY = l.MY, MZ = l.MZ, JX = l.JX, JY = l.JY;
-    if (l.gradients == 1) {
+    if (l.gradients == 1 && l.fjc > 1) {
+      for (int k = 0; k < l.fjc; k++) {
+        X[k] = 0;
+        X[MX + l.fjc + k] = 0;
+      }
+    } else if (l.gradients == 1) {
       X[0] = 0;
       X[MX + 1] = 0;
     } else if (l.gradients == 2) {
@@ -198,8 +209,22 @@ struct Lat {
     remove_bounds(X);
     const int M = l.M;
     if (l.gradients == 3) return Sum(X, M);
-    if (l.geometry == GEOM_PLANAR) return Sum(X, M);
+    if (l.geometry == GEOM_PLANAR) return l.fjc > 1 ? Sum(X, M) / l.fjc : Sum(X, M);
     return Dot(X, l.L.data(), M);
+  }
+
+  // Refined 1-gradient stencil (fjc > 1): FJC taps with the LAMBDA rows,
+  // in the order of src/LG1Planar.cpp:151-156 / src/LGrad1.cpp:447-452
+  // (also Side, src/LG1Planar.cpp:47-52, src/LGrad1.cpp:272-277).
+  void refined_taps(double *out, const double *X) const {
+    const int M = l.M, f = l.fjc, F = l.FJC;
+    const double *LA = l.LAMBDA.data();
+    for (int j = 0; j < F / 2; j++) {
+      const int kk = (F - 1) / 2 - j;
+      AddTimes(out + kk, X, LA + j * M + kk, M - kk);
+      AddTimes(out, X + kk, LA + (F - j - 1) * M, M - kk);
+    }
+    AddTimes(out, X, LA + f * M, M);
   }
 
   // Markov-1 step: gs <- G1 * W(gs_1); gs_1 gets its ghosts set in place.
@@ -207,7 +232,10 @@ struct Lat {
     const int M = l.M, JX = l.JX;
     Zero(gs, M);
     set_bounds(gs_1);
-    if (l.gradients == 1) {
+    if (l.gradients == 1 && l.fjc > 1) {
+      refined_taps(gs, gs_1);
+      Times(gs, gs, G1, M);
+    } else if (l.gradients == 1) {
       if (l.geometry == GEOM_PLANAR) {
         const double lam = l.lambda;
         YplusisCtimesX(gs + 1, gs_1, lam, M - 1);
@@ -368,7 +396,9 @@ struct Lat {
   void side(double *Xs, double *X) const {
     const int M = l.M, JX = l.JX;
     Zero(Xs, M);
-    if (l.gradients == 1) {
+    if (l.gradients == 1 && l.fjc > 1) {
+      refined_taps(Xs, X);
+    } else if (l.gradients == 1) {
       if (l.geometry == GEOM_PLANAR) {
         const double lam = l.lambda;
         YplusisCtimesX(Xs + 1, X, lam, M - 1);

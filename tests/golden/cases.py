@@ -119,6 +119,118 @@ mol : water : freedom : solvent
 """,
         eval=[(6, 0.4)],
     ),
+    # lattice refinement (FJC_choices 5: two grid layers per bond length),
+    # planar adsorption next to a surface
+    "fjc5_planar_adsorption": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : n_layers : 40
+lat : flat : lattice_type : hexagonal
+lat : flat : FJC_choices : 5
+lat : flat : lowerbound : surface
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -1.0
+mon : A : chi_W : 0.5
+mol : pol : composition : (A)50
+mol : pol : freedom : free
+mol : pol : phibulk : 0.01
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(31, 0.3), (32, 0.8)],
+        converged=[dict(method="DIIS", tol=1e-10, seed=33, amp=0.05)],
+    ),
+    # refinement through b/l (three grid layers per bond), grafted brush on a
+    # mirror, hexagonal lattice
+    "fjc7_planar_brush": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : n_layers : 30
+lat : flat : lattice_type : hexagonal
+lat : flat : b/l : 3
+mon : X : freedom : pinned
+mon : X : pinned_range : firstlayer
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.4
+mol : brush : composition : (X)1(A)40
+mol : brush : freedom : restricted
+mol : brush : theta : 4.1
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(34, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-10, seed=35, amp=0.05)],
+    ),
+    # refined cylindrical micelle of a restricted diblock (refinement needs
+    # the hexagonal lattice)
+    "fjc5_cyl_micelle": dict(
+        text="""
+lat : L : gradients : 1
+lat : L : n_layers : 25
+lat : L : geometry : cylindrical
+lat : L : lattice_type : hexagonal
+lat : L : FJC_choices : 5
+mon : A : freedom : free
+mon : B : freedom : free
+mon : W : freedom : free
+mon : B : chi_W : 1.5
+mon : A : chi_B : 1.0
+mol : surf : composition : (A)6(B)6
+mol : surf : freedom : restricted
+mol : surf : theta : 150
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(36, 0.5)],
+    ),
+    # refined spherical particle with an offset and an adsorbing surface
+    "fjc5_sph_particle": dict(
+        text="""
+lat : L : gradients : 1
+lat : L : n_layers : 30
+lat : L : geometry : spherical
+lat : L : offset_first_layer : 4
+lat : L : lattice_type : hexagonal
+lat : L : FJC_choices : 5
+lat : L : lowerbound : surface
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -1.5
+mon : A : chi_W : 0.4
+mol : pol : composition : (A)30
+mol : pol : freedom : free
+mol : pol : phibulk : 0.02
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(37, 0.3)],
+    ),
+    # refined periodic lattice (FJC_choices 9), restricted homopolymer
+    "fjc9_periodic_restricted": dict(
+        text="""
+lat : L : gradients : 1
+lat : L : n_layers : 24
+lat : L : lattice_type : hexagonal
+lat : L : FJC_choices : 9
+lat : L : lowerbound : periodic
+lat : L : upperbound : periodic
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.7
+mol : pol : composition : (A)20
+mol : pol : freedom : restricted
+mol : pol : theta : 6
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(38, 0.4)],
+    ),
     # second-order Markov on a curved 1-gradient lattice
     "cyl1d_markov2": dict(
         text="""

@@ -122,7 +122,7 @@ def evaluation_fixture(case, seed, amp):
     rng = np.random.default_rng(seed)
     x = rng.uniform(-amp, amp, o.nvar)
     with tempfile.TemporaryDirectory() as d:
-        write_guess(os.path.join(d, "x.ig"), o.field_names, o.dims, x)
+        write_guess(os.path.join(d, "x.ig"), o.field_names, o.dims, x, fjc=o.fjc)
         text = case["text"] + "\n" + output_lines(o) + (
             "sys : %s : initial_guess : file\n"
             "sys : %s : guess_inputfile : x.ig\n"
@@ -150,7 +150,7 @@ def converged_fixture(case, method, tol, itlim=5000, seed=None, amp=0.0):
         if seed is not None:
             rng = np.random.default_rng(seed)
             x = rng.uniform(-amp, amp, o.nvar)
-            write_guess(os.path.join(d, "x.ig"), o.field_names, o.dims, x)
+            write_guess(os.path.join(d, "x.ig"), o.field_names, o.dims, x, fjc=o.fjc)
             guess = ("sys : %s : initial_guess : file\n"
                      "sys : %s : guess_inputfile : x.ig\n") % (o.sys_name, o.sys_name)
         text = case["text"] + "\n" + output_lines(o) + guess + (

@@ -65,6 +65,7 @@ class Oracle:
         self.nvar = d["nvar"]
         self.gradients = d["gradients"]
         self.dims = (d["MX"], d["MY"], d["MZ"])
+        self.fjc = d.get("fjc", 1)
         self.seg_names = [s["name"] for s in d["segments"]]
         self.mol_names = [m["name"] for m in d["molecules"]]
         self.field_names = list(d["fields"])
@@ -144,7 +145,7 @@ class Oracle:
         mx, my, mz = self.dims
         a = np.asarray(a)
         if self.gradients == 1:
-            return a[1:mx + 1]
+            return a[self.fjc:mx + self.fjc]
         if self.gradients == 2:
             return a.reshape(mx + 2, my + 2)[1:mx + 1, 1:my + 1].ravel()
         return a.reshape(mx + 2, my + 2, mz + 2)[1:mx + 1, 1:my + 1, 1:mz + 1].ravel()
