@@ -174,7 +174,7 @@ int build(scf_system *S, const Placement *pl) {
       h.mm_seg[nmm++] = sg;
       h.sysmon[sg] = 1;
     }
-    for (int k = 0; k < 3 && k < (int)m.P.size(); k++) h.mol_P[i][k] = m.P[k];
+    for (int k = 0; k < kMaxComp && k < (int)m.P.size(); k++) h.mol_P[i][k] = m.P[k];
     h.mol_phibulk_in[i] = m.phibulk;
     h.mol_n_in[i] = m.n;
     double cf = 0;

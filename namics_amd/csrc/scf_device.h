@@ -24,6 +24,7 @@ constexpr int kMaxMol = 8;
 constexpr int kMaxBlocks = 16;
 constexpr int kMaxMM = 16;
 constexpr int kMaxFjc = 4;   // lattice refinement up to FJC_choices 9
+constexpr int kMaxComp = 2 * kMaxFjc + 1;  // Markov-2 direction components (FJC)
 
 enum DevStatus : int {
   DS_OK = 0,
@@ -89,7 +90,7 @@ struct DevSys {
   int mol_mm_begin[kMaxMol + 1];
   int block_seg[kMaxBlocks], block_n[kMaxBlocks], block_mm[kMaxBlocks];
   int mm_seg[kMaxMM];
-  double mol_P[kMaxMol][3];
+  double mol_P[kMaxMol][kMaxComp];  // Markov-2 bending weights, FJC entries
   double mol_phibulk_in[kMaxMol], mol_n_in[kMaxMol];
   double frac[kMaxMol][kMaxSeg];
   double mol_constF[kMaxMol];
@@ -415,6 +416,58 @@ __device__ __forceinline__ void m2_step(const Sys &S, Ptr g, const double *P, bo
   out[0] = a;
   out[1] = b;
   out[2] = c;
+}
+
+// Bounded value of direction component c of a Markov-2 row at position x on
+// a planar hexagonal lattice with FJC = 2 fjc + 1 components.  Components c
+// and FJC-1-c (c < fjc) are bounded as a pair and the lateral component fjc
+// alone (LG1Planar::propagateF/B src/LG1Planar.cpp:64-65, LGrad1::set_bounds
+// src/LGrad1.cpp:787-821): a ghost of component c < fjc takes the partner's
+// value at the source site, a ghost of c > fjc the partner's value there
+// unless the boundary is a surface, where it keeps its own.
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ T m2h_val(const Sys &S, Ptr row, int c, int x) {
+  const int M = S.M, f = S.fjc, F = 2 * S.fjc + 1;
+  if (x >= f && x < S.MX + f) return row[c * M + x];
+  const int b = map_x1(S, x);
+  const bool surf = x < f ? S.surface_lo : S.surface_hi;
+  if (c == f) return row[c * M + b];
+  if (c < f) return row[(F - 1 - c) * M + b];
+  return surf ? row[c * M + x] : row[(F - 1 - c) * M + b];
+}
+
+// Forward or backward Markov-2 step at site x on a planar hexagonal lattice
+// (LG1Planar::propagateF/B src/LG1Planar.cpp:82-94,122-134): component p
+// advances fjc - p sites per bond; the weight from component q is
+// P[|p-q|] + P[FJC-1-|FJC-1-p-q|] (the second term only for q off the x
+// axis).  Forward, G'_p(x) = sum_q w G_q(x - fjc + p), q ascending; backward,
+// G'_p(x) = sum_q w G_q(x + fjc - q), q descending.  Writes FJC components
+// (without the Boltzmann factor) to out.
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ void m2h_step(const Sys &S, Ptr g, const double *P, bool forward, int x,
+                                         T out[kMaxComp]) {
+  const int M = S.M, f = S.fjc, F = 2 * S.fjc + 1;
+  for (int p = 0; p < F; p++) {
+    T acc = 0;
+    if (forward) {
+      const int src = x - f + p;
+      if (src >= 0 && src <= M - 1)
+        for (int q = 0; q < F; q++) {
+          double c = P[p > q ? p - q : q - p];
+          if (q > 0 && q < F - 1) c += P[F - 1 - abs(F - 1 - p - q)];
+          if (c != 0) acc += (T)c * m2h_val<T>(S, g, q, src);
+        }
+    } else {
+      for (int q = F - 1; q >= 0; q--) {
+        const int src = x + f - q;
+        if (src < 0 || src > M - 1) continue;
+        double c = P[p > q ? p - q : q - p];
+        if (q > 0 && q < F - 1) c += P[F - 1 - abs(F - 1 - p - q)];
+        if (c != 0) acc += (T)c * m2h_val<T>(S, g, q, src);
+      }
+    }
+    out[p] = acc;
+  }
 }
 
 // ----------------------------------------------------- per-site phases

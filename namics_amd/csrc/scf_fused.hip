@@ -714,8 +714,19 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
   const int tid = work_tid(S), nt = S.threads;
   const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
   const int N = S.mol_N[mo];
-  const int R = 3 * M;
-  const double P[3] = {S.mol_P[mo][0], S.mol_P[mo][1], S.mol_P[mo][2]};
+  // direction components: 3 on the simple-cubic lattice, FJC = 2 fjc + 1 on
+  // the hexagonal one
+  const int C = S.mol_size[mo];
+  const int R = C * M;
+  const bool hex = S.lattice_type != 0;
+  double P[kMaxComp];
+#pragma unroll
+  for (int k = 0; k < kMaxComp; k++) P[k] = S.mol_P[mo][k];
+  // composition weights (LGrad1::AddPhiS src/LGrad1.cpp:876-910): ends and
+  // inner components
+  const int F = C;
+  const double wend = hex ? (S.fjc == 1 ? 1.0 / 4.0 : 0.5 / (F - 1.0)) : 1.0 / 6.0;
+  const double wmid = hex ? (S.fjc == 1 ? 2.0 / 4.0 : 1.0 / (F - 1.0)) : 4.0 / 6.0;
   gdbl *Gf = hbm(S.Gf);
   const gdbl *G1 = hbm(S.G1);
   RowT *ra, *rb;
@@ -730,7 +741,7 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
     const gdbl *g1 = G1 + (size_t)S.block_seg[b0] * M;
     for (int i = tid; i < W; i += nt) {
       const double v = g1[i];
-      for (int c = 0; c < 3; c++) {
+      for (int c = 0; c < C; c++) {
         Gf[c * M + i] = v;
         if constexpr (kLds) ra[c * M + i] = v;
       }
@@ -745,10 +756,15 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
     for (int j = (b == b0 ? 1 : 0); j < S.block_n[b]; j++, s++) {
       gdbl *gs = Gf + (size_t)s * R;
       for (int i = tid; i < W; i += nt) {
-        double o[3];
-        m2_step<double>(S, prev, P, true, i, o);
+        double o[kMaxComp];
+        if (hex)
+          m2h_step<double>(S, prev, P, true, i, o);
+        else
+          m2_step<double>(S, prev, P, true, i, o);
         const double w = g1[i];
-        for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int c = 0; c < kMaxComp; c++) {
+          if (c >= C) break;
           const double v = o[c] * w;
           gs[c * M + i] = v;
           if constexpr (kLds) next[c * M + i] = v;
@@ -764,21 +780,30 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
       }
     }
   }
+  // GN (LGrad1::ComputeGN src/LGrad1.cpp:943-955): simple cubic
+  // (G0 + 4 G1 + G2) / 6, hexagonal (G0 + 2 sum G_inner + G_last) / (4 fjc)
   {
     const gdbl *gl = Gf + (size_t)(N - 1) * R;
-    double a0 = 0, a1 = 0, a2 = 0;
+    double a[kMaxComp];
+#pragma unroll
+    for (int c = 0; c < kMaxComp; c++) a[c] = 0;
     for (int i = tid; i < W; i += nt) {
-      if (i >= 1 && i <= S.MX) {
+      if (is_interior(S, i, 0, 0)) {
         const double wt = site_weight(S0, i);
-        a0 += wt * gl[i];
-        a1 += wt * gl[M + i];
-        a2 += wt * gl[2 * M + i];
+#pragma unroll
+        for (int c = 0; c < kMaxComp; c++)
+          if (c < C) a[c] += wt * gl[c * M + i];
       }
     }
-    double gn = wg_sum(a0);
-    gn += 4.0 * wg_sum(a1);
-    gn += wg_sum(a2);
-    if (threadIdx.x == 0) S0.sc->GN[mo] = gn / 6.0;
+    double gn = wg_sum(a[0]);
+    const double inner = hex ? 2.0 : 4.0;
+#pragma unroll
+    for (int c = 1; c < kMaxComp - 1; c++)
+      if (c < C - 1) gn += inner * wg_sum(a[c]);
+#pragma unroll
+    for (int c = 2; c < kMaxComp; c++)
+      if (c == C - 1) gn += wg_sum(a[c]);
+    if (threadIdx.x == 0) S0.sc->GN[mo] = hex ? gn / (4.0 * S.fjc) : gn / 6.0;
   }
   RowT *gbA, *gbB;
   if constexpr (kLds) {
@@ -796,10 +821,12 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
     for (int i = tid; i < W; i += nt) {
       const double v = g1[i];
       double r = rho[i];
-      for (int c = 0; c < 3; c++) gbA[c * M + i] = v;
-      r += (1.0 / 6.0) * gf[i] * v;
-      r += (4.0 / 6.0) * gf[M + i] * v;
-      r += (1.0 / 6.0) * gf[2 * M + i] * v;
+#pragma unroll
+      for (int c = 0; c < kMaxComp; c++) {
+        if (c >= C) break;
+        gbA[c * M + i] = v;
+        r += (c == 0 || c == C - 1 ? wend : wmid) * gf[c * M + i] * v;
+      }
       rho[i] = r;
     }
   }
@@ -811,14 +838,19 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
     for (int j = (b == b1 - 1 ? S.block_n[b] - 2 : S.block_n[b] - 1); j >= 0; j--, s--) {
       const gdbl *gf = Gf + (size_t)s * R;
       for (int i = tid; i < W; i += nt) {
-        double o[3];
-        m2_step<double>(S, gbA, P, false, i, o);
+        double o[kMaxComp];
+        if (hex)
+          m2h_step<double>(S, gbA, P, false, i, o);
+        else
+          m2_step<double>(S, gbA, P, false, i, o);
         const double w = g1[i];
         double r = rho[i];
-        for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int c = 0; c < kMaxComp; c++) {
+          if (c >= C) break;
           const double v = o[c] * w;
           gbB[c * M + i] = v;
-          r += (c == 1 ? 4.0 / 6.0 : 1.0 / 6.0) * gf[c * M + i] * v;
+          r += (c == 0 || c == C - 1 ? wend : wmid) * gf[c * M + i] * v;
         }
         rho[i] = r;
       }

@@ -792,8 +792,11 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     }
     m.size = 1;
     if (m.markov == 2) {
-      if (l.gradients != 1 || l.lattice_type != LAT_SIMPLE_CUBIC || l.fjc != 1) {
-        err = "mol '" + m.name + "': Markov 2 is supported on 1-gradient simple-cubic lattices without refinement";
+      const bool sc1 = l.lattice_type == LAT_SIMPLE_CUBIC && l.fjc == 1;
+      const bool hexp = l.lattice_type == LAT_HEXAGONAL && l.geometry == GEOM_PLANAR;
+      if (l.gradients != 1 || !(sc1 || hexp)) {
+        err = "mol '" + m.name + "': Markov 2 is supported on 1-gradient simple-cubic lattices without refinement "
+              "and on planar 1-gradient hexagonal lattices";
         return false;
       }
       any_markov2 = true;

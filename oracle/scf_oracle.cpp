@@ -154,12 +154,70 @@ struct Lat {
     }
   }
 
-  // two-array form used for Markov-2 direction components (src/LGrad1.cpp:787-804)
+  // two-array form used for Markov-2 direction components (src/LGrad1.cpp:787-821)
   void set_bounds2(double *X, double *Y) const {
+    if (l.fjc > 1) {
+      const int f = l.fjc;
+      for (int k = 0; k < f; k++) {
+        X[k] = Y[l.bx1f[k]];
+        X[l.MX + f + k] = Y[l.bxmf[k]];
+        Y[k] = X[l.bx1f[k]];
+        Y[l.MX + f + k] = X[l.bxmf[k]];
+      }
+      return;
+    }
     X[0] = Y[l.BX1];
     X[l.MX + 1] = Y[l.BXM];
     Y[0] = X[l.BX1];
     Y[l.MX + 1] = X[l.BXM];
+  }
+
+  // Markov-2 step on a planar 1-gradient hexagonal lattice (any fjc), FJC
+  // direction components: component p moves fjc - p sites per bond; the
+  // weight from component q is P[|p-q|] plus, for q not along x,
+  // P[FJC-1-|FJC-1-p-q|] (LG1Planar::propagateF/B src/LG1Planar.cpp:82-94,
+  // 122-134).
+  void propagate_hex(double *gs, double *gs_1, const double *G1, const double *P, bool forward) const {
+    const int M = l.M, f = l.fjc, F = l.FJC;
+    Zero(gs, M * F);
+    for (int k = 0; k < (F - 1) / 2; k++) set_bounds2(gs_1 + k * M, gs_1 + (F - k - 1) * M);
+    set_bounds(gs_1 + (F - 1) / 2 * M);
+    auto cw = [&](int p, int q) {
+      double c = P[std::abs(-p + q)];
+      if (q > 0 && q < F - 1) c += P[F - 1 - std::abs(F - 1 - p - q)];
+      return c;
+    };
+    if (forward) {
+      for (int p = 0; p < F; p++) {
+        int a = p - f, b;
+        if (a < 0) {
+          b = 0;
+          a = -a;
+        } else {
+          b = a;
+          a = 0;
+        }
+        for (int q = 0; q < F; q++) {
+          const double c = cw(p, q);
+          if (c != 0) YplusisCtimesX(gs + p * M + a, gs_1 + q * M + b, c, M - a - b);
+        }
+      }
+    } else {
+      for (int q = F - 1; q > -1; q--) {
+        int a = q - f, b;
+        if (a > 0) {
+          b = 0;
+        } else {
+          b = -a;
+          a = 0;
+        }
+        for (int p = F - 1; p > -1; p--) {
+          const double c = cw(p, q);
+          if (c != 0) YplusisCtimesX(gs + p * M + a, gs_1 + q * M + b, c, M - a - b);
+        }
+      }
+    }
+    for (int k = 0; k < F; k++) Times(gs + k * M, gs + k * M, G1, M);
   }
 
   void remove_bounds(double *X) const {
@@ -322,6 +380,7 @@ struct Lat {
   // Markov-2 forward/backward steps on a 1-gradient simple-cubic lattice.
   // gs, gs_1 hold the three direction components back to back (stride M).
   void propagateF(double *gs, double *gs_1, const double *G1, const double *P) const {
+    if (l.lattice_type == LAT_HEXAGONAL) return propagate_hex(gs, gs_1, G1, P, true);
     const int M = l.M;
     Zero(gs, 3 * M);
     set_bounds2(gs_1, gs_1 + 2 * M);
@@ -360,6 +419,7 @@ struct Lat {
   }
 
   void propagateB(double *gs, double *gs_1, const double *G1, const double *P) const {
+    if (l.lattice_type == LAT_HEXAGONAL) return propagate_hex(gs, gs_1, G1, P, false);
     const int M = l.M;
     Zero(gs, 3 * M);
     set_bounds2(gs_1, gs_1 + 2 * M);
@@ -795,7 +855,13 @@ struct Oracle {
     }
     // GN (LGrad1::ComputeGN src/LGrad1.cpp:943-955 and analogues)
     double *Glast = Gf[N - 1].data();
-    if (m.markov == 2) {
+    if (m.markov == 2 && s.lat.lattice_type == LAT_HEXAGONAL) {
+      const int F = s.lat.FJC;
+      double g = lat.weighted_sum(Glast);
+      for (int k = 1; k < F - 1; k++) g += 2.0 * lat.weighted_sum(Glast + k * M);
+      g += lat.weighted_sum(Glast + (F - 1) * M);
+      GN[mi] = g / (4.0 * s.lat.fjc);
+    } else if (m.markov == 2) {
       double g = lat.weighted_sum(Glast);
       g += 4.0 * lat.weighted_sum(Glast + M);
       g += lat.weighted_sum(Glast + 2 * M);
@@ -816,7 +882,15 @@ struct Oracle {
       }
       double *r = rho.data() + mm_of[k] * M;
       const double *gf = Gf[k].data();
-      if (m.markov == 2) {
+      if (m.markov == 2 && s.lat.lattice_type == LAT_HEXAGONAL) {
+        // LGrad1::AddPhiS (src/LGrad1.cpp:876-898)
+        const int F = s.lat.FJC;
+        const double C1 = s.lat.fjc == 1 ? 1.0 / 4.0 : 0.5 / (F - 1.0);
+        const double C2 = s.lat.fjc == 1 ? 2.0 / 4.0 : 1.0 / (F - 1.0);
+        YplusisCtimesAtimesB(r, gf, gb, C1, M);
+        for (int k = 1; k < F - 1; k++) YplusisCtimesAtimesB(r, gf + k * M, gb + k * M, C2, M);
+        YplusisCtimesAtimesB(r, gf + (F - 1) * M, gb + (F - 1) * M, C1, M);
+      } else if (m.markov == 2) {
         YplusisCtimesAtimesB(r, gf, gb, 1.0 / 6.0, M);
         YplusisCtimesAtimesB(r, gf + M, gb + M, 4.0 / 6.0, M);
         YplusisCtimesAtimesB(r, gf + 2 * M, gb + 2 * M, 1.0 / 6.0, M);

@@ -231,6 +231,81 @@ mol : water : freedom : solvent
 """,
         eval=[(38, 0.4)],
     ),
+    # second-order Markov on the hexagonal lattice (the general direction
+    # loop with FJC = 3 components)
+    "hex_markov2_planar": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : n_layers : 50
+lat : flat : lattice_type : hexagonal
+lat : flat : lowerbound : surface
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -1.2
+mon : A : chi_W : 0.5
+mol : pol : composition : (A)40
+mol : pol : freedom : free
+mol : pol : phibulk : 0.005
+mol : pol : Markov : 2
+mol : pol : k_stiff : 1
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(41, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-10, seed=42, amp=0.05)],
+    ),
+    # second-order Markov on a refined hexagonal lattice (FJC = 5
+    # components), adsorption next to a surface
+    "fjc5_markov2_adsorption": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : n_layers : 30
+lat : flat : lattice_type : hexagonal
+lat : flat : FJC_choices : 5
+lat : flat : lowerbound : surface
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -1.0
+mon : A : chi_W : 0.6
+mol : pol : composition : (A)30
+mol : pol : freedom : free
+mol : pol : phibulk : 0.01
+mol : pol : Markov : 2
+mol : pol : k_stiff : 2
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(43, 0.3), (44, 0.6)],
+        converged=[dict(method="DIIS", tol=1e-10, seed=45, amp=0.05)],
+    ),
+    # stiff grafted brush on a refined lattice (b/l 3, FJC = 7 components),
+    # mirror boundaries
+    "fjc7_markov2_brush": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : n_layers : 25
+lat : flat : lattice_type : hexagonal
+lat : flat : b/l : 3
+mon : X : freedom : pinned
+mon : X : pinned_range : firstlayer
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.3
+mol : brush : composition : (X)1(A)30
+mol : brush : freedom : restricted
+mol : brush : theta : 3.1
+mol : brush : Markov : 2
+mol : brush : k_stiff : 0.5
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(46, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-9, seed=47, amp=0.05)],
+    ),
     # second-order Markov on a curved 1-gradient lattice
     "cyl1d_markov2": dict(
         text="""
