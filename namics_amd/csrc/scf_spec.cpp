@@ -618,6 +618,7 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
   boundary_indices(l);
 
   // ---------------- segments ----------------
+  bool frozen_at_bound[6] = {false, false, false, false, false, false};
   const int nseg = (int)p.mons.size();
   if (nseg == 0) {
     err = "no 'mon' defined";
@@ -648,6 +649,13 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
         return false;
       }
       if (!parse_range(l, v, sg.mask, err)) return false;
+      // a frozen layer given by a boundary keyword backs that surface
+      // (Segment::ParseFreedoms src/segment.cpp:418-440)
+      static const char *axes[4] = {"", "_x", "_y", "_z"};
+      for (int a = 0; a < 4; a++) {
+        if (v == std::string("lowerbound") + axes[a]) frozen_at_bound[a == 0 ? 0 : a - 1] = true;
+        if (v == std::string("upperbound") + axes[a]) frozen_at_bound[a == 0 ? 3 : a + 2] = true;
+      }
     }
     if (sg.freedom == SEG_PINNED) {
       if (!get(p, "mon", sg.name, "pinned_range", v)) {
@@ -657,6 +665,15 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       if (!parse_range(l, v, sg.mask, err)) return false;
     }
   }
+  // a surface boundary needs a frozen segment on it (System::CheckInput
+  // src/system.cpp:1193-1209)
+  static const char *side_name[6] = {"lowerboundary in x", "lowerboundary in y", "lowerboundary in z",
+                                     "upperboundary in x", "upperboundary in y", "upperboundary in z"};
+  for (int b = 0; b < 6; b++)
+    if (l.bc[b] == BC_SURFACE && !frozen_at_bound[b]) {
+      err = std::string("lonely 'surface': specify a segment with frozen_range including the ") + side_name[b];
+      return false;
+    }
   // chi matrix (Segment::CheckInput src/segment.cpp:1105-1119,
   // System::CheckChi_values src/system.cpp:1718-1800): unset entries take
   // the transposed value, otherwise 0.

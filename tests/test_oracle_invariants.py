@@ -259,3 +259,66 @@ def test_diis_solve_matches_numpy():
         c = solve_ones(A)
         ref = np.linalg.solve(A, np.ones(n))
         np.testing.assert_allclose(c / c.sum(), ref / ref.sum(), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("fjc", [2, 3, 4])
+def test_refined_periodic_eigenmodes(fjc):
+    """Refined planar lattice (FJC = 2 fjc + 1): the stencil has weight
+    1/(2(FJC-1)) at distance fjc and 1/(FJC-1) at distances 0..fjc-1, so
+    sin(kx + c) with k = 2 pi j / MX (MX = fjc n) is an eigenvector of the
+    periodic operator with eigenvalue w0 + 2 sum_d w_d cos(k d); the same
+    weights serve the site average."""
+    n = 12
+    o = lat1(n=n, lt="hexagonal",
+             extra="lat : L : FJC_choices : %d\nlat : L : lowerbound : periodic\nlat : L : upperbound : periodic"
+             % (2 * fjc + 1))
+    F, MX = 2 * fjc + 1, fjc * n
+    assert o.M == MX + 2 * fjc
+    w = [1.0 / (F - 1)] * fjc + [0.5 / (F - 1)]          # distances 0..fjc
+    xs = np.arange(fjc, MX + fjc)
+    for j in (1, 4, n):
+        k = 2 * np.pi * j / MX
+        v = np.zeros(o.M)
+        v[fjc:MX + fjc] = np.sin(k * xs + 0.3)
+        ev = w[0] + 2 * sum(w[d] * np.cos(k * d) for d in range(1, fjc + 1))
+        np.testing.assert_allclose(o.apply(v)[fjc:MX + fjc], ev * v[fjc:MX + fjc], atol=1e-13)
+        np.testing.assert_allclose(o.apply(v, side=True)[fjc:MX + fjc], ev * v[fjc:MX + fjc], atol=1e-13)
+
+
+@pytest.mark.parametrize("geo,extra", [("planar", ""), ("cylindrical", ""), ("spherical", ""),
+                                       ("cylindrical", "lat : L : offset_first_layer : 3")])
+def test_refined_rows_sum_to_one_and_zero_field_gn(geo, extra):
+    """Refined lattices (FJC 5): with mirror bounds the operator keeps a
+    constant (every row of weights sums to one, the self weight absorbing the
+    remainder), and on the planar lattice a chain in zero field has
+    GN = volume = n_layers (sums carry the 1/fjc site weight)."""
+    n = 15
+    o = lat1(n=n, lt="hexagonal", geo=geo, extra="lat : L : FJC_choices : 5\n" + extra)
+    f = 2
+    one = np.ones(o.M)
+    np.testing.assert_allclose(o.apply(one)[f:f * n + f], 1.0, atol=1e-14)
+    if geo == "planar":
+        o.residual(np.zeros(o.nvar))
+        assert abs(o.results()["mol"]["pol"]["GN"] - n) < 1e-12 * n
+
+
+@pytest.mark.parametrize("fjc,kst", [(1, 0.0), (2, 0.0), (2, 1.5), (3, 0.7)])
+def test_hexagonal_markov2_keeps_a_constant(fjc, kst):
+    """Planar hexagonal Markov 2 with FJC components: the bending weights are
+    normalised so that every component's incoming weights sum to one, hence
+    in zero field with mirror bounds all components stay 1 and
+    GN = (G_0 + 2 sum G_inner + G_last) / (4 fjc) equals the volume."""
+    n = 14
+    text = LAT1.format(n=n, lt="hexagonal", geo="planar",
+                       extra="lat : L : FJC_choices : %d" % (2 * fjc + 1)).replace(
+        "mol : pol : phibulk : 0.01", "mol : pol : phibulk : 0.01\nmol : pol : Markov : 2\nmol : pol : k_stiff : %g" % kst)
+    o = Oracle(text)
+    o.residual(np.zeros(o.nvar))
+    assert abs(o.results()["mol"]["pol"]["GN"] - n) < 1e-11 * n
+
+
+def test_surface_needs_a_frozen_layer():
+    """A surface boundary without a frozen segment on it is refused, as the
+    reference refuses it (System::CheckInput)."""
+    with pytest.raises(ValueError, match="lonely"):
+        lat1(extra="lat : L : lowerbound : surface")
