@@ -113,6 +113,15 @@ def run_brush(args):
     nslots = args.slots if args.slots > 0 else ncu
     J = args.jobs if args.jobs > 0 else 2 * nslots
     t_setup = time.perf_counter()
+    # launch layout: tuned once on one system of the brush's shape; every
+    # slot system built afterwards starts from the tuned layout
+    tune = None
+    if not args.no_tune:
+        probe = ScfSystem(brush_cylinder(CHIS[-1]))
+        probe.upload(uniform_fields(probe.nvar, SEED, 0.05))
+        tune = probe.autotune(evals=5)
+        tune["layout"] = probe.layout()
+        probe.close()
     slots = [[ScfSystem(brush_cylinder(c)) for c in CHIS] for _ in range(nslots)]
     descs = [s.desc for s in slots[0]]
     nvar0 = slots[0][0].nvar
@@ -170,6 +179,7 @@ def run_brush(args):
                      "kernel": "k_fused_jobs (one launch per step)", "avg_launch_ms": launch_ms,
                      "algorithmic_bytes_per_launch": nbytes},
         "setup_s": t_setup,
+        "autotune": tune,
     }
     traffic, src = profiled_traffic(J, nslots)
     if traffic is not None:
@@ -177,7 +187,10 @@ def run_brush(args):
         out["roofline"]["traffic_source"] = src + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)"
         out["roofline"]["hbm_gbs_measured_traffic"] = traffic / (launch_ms * 1e-3) / 1e9
     # the same jobs with FP32 propagators and FP64 refinement
-    if not args.no_mixed:
+    mixed_ok = not args.no_mixed and slots[0][0].layout()["strip_sx"] > 0
+    if not args.no_mixed and not mixed_ok:
+        out["mixed_precision"] = {"skipped": "the tuned layout is not a register-strip layout"}
+    if mixed_ok:
         for row in slots:
             for s in row:
                 s.set_precision(True, args.switch)
@@ -231,6 +244,7 @@ def main():
     ap.add_argument("--jobs", type=int, default=0, help="independent brush jobs per step (0: twice the slots)")
     ap.add_argument("--slots", type=int, default=0, help="resident job slots (0: one per CU)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision rerun")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch layout (no autotuning)")
     ap.add_argument("--switch", type=float, default=1e-4, help="mixed precision: FP64 below this residual")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: requests per second")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals")

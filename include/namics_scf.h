@@ -131,6 +131,32 @@ int scf_system_set_precision(scf_system *sys, int mode, double switch_residual);
  * which changes no output. */
 int scf_system_set_checkpoint(scf_system *sys, int interval);
 
+/* Launch-layout autotuner.  Times `evals` residual evaluations (at the
+ * system's current device fields) for every layout that applies to the
+ * lattice -- on 2-gradient lattices the register-strip widths SX = 8 and 4
+ * and the generic site-strided chain; on 3-gradient lattices one workgroup
+ * per system against one launch per phase -- keeps the fastest for this
+ * system, and remembers it for the lattice shape, so systems built later in
+ * the process with the same shape start from it.  Every layout computes the
+ * same residual (to rounding).  A layout the system's options exclude (FP32
+ * propagators and checkpointing need a strip layout; an explicit
+ * scf_system_set_mode) is not tried.  No counterpart in the reference, which
+ * has one code path per lattice. */
+#define SCF_TUNE_MAX 4
+typedef struct {
+  int candidates;                   /* layouts timed */
+  int chosen;                       /* index of the fastest */
+  int mode[SCF_TUNE_MAX];           /* SCF_MODE_FUSED / SCF_MODE_GRID */
+  int strip_sx[SCF_TUNE_MAX];       /* fused path: 8, 4, or 0 for the generic chain */
+  double ms_per_eval[SCF_TUNE_MAX]; /* device time of one residual evaluation */
+} scf_tune_report;
+int scf_system_autotune(scf_system *sys, int evals, scf_tune_report *report);
+/* Forget the layouts remembered per lattice shape (tests, benchmarks). */
+void scf_autotune_forget(void);
+/* The system's current launch layout: execution mode, strip width (0 when
+ * not on the strip path) and threads per system. */
+int scf_system_layout(const scf_system *sys, int *mode, int *strip_sx, int *threads);
+
 /* Override solver settings: tolerance, deltamax, DIIS history m, iteration
  * limit (values <= 0 keep the input's setting). */
 int scf_system_set_solver(scf_system *sys, double tolerance, double deltamax, int m, int iterationlimit);

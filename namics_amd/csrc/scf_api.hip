@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -107,6 +109,76 @@ double fraction(const MoleculeSpec &m, int seg) {
   for (auto &b : m.blocks)
     if (b.seg == seg) n += b.n;
   return 1.0 * n / m.chainlength;
+}
+
+// ---------------------------------------------------------------- layouts
+// Launch layout of the fused path: threads per system, whether the
+// propagator rows live in LDS, and on 2-gradient lattices the register-strip
+// width SX (8 or 4 sites per thread along x; 0 = the generic site-strided
+// chain).  want_sx < 0 picks the first strip width that fits.
+void plan_layout(scf_system *S, int want_sx) {
+  const LatticeSpec &l = S->spec.lat;
+  DevSys &h = S->h;
+  const size_t D = sizeof(double);
+  const int M = l.M;
+  const int W = l.gradients == 1 ? M : l.MX * l.MY * (l.gradients == 3 ? l.MZ : 1);
+  int maxR = 1;
+  for (auto &m : S->spec.mol) maxR = std::max(maxR, m.size);
+  const int work = W;
+  h.threads = work <= 64 ? 64 : (work <= 128 ? 128 : (work <= 256 ? 256 : (work <= 512 ? 512 : 1024)));
+  h.strip_sx = 0;
+  h.strip_stride = 0;
+  size_t rows = (size_t)2 * maxR * M * D;
+  h.row_stride = maxR * M;
+  // 2-gradient strip path: every thread owns SX sites along x; needs
+  // Markov-1 chains, the 9-point stencil, ghosts that map onto interior
+  // sites, and the rows of one strip layout in LDS
+  const bool strip_ok = l.gradients == 2 && maxR == 1 && (l.geometry != 0 || l.stencil_full) &&
+                        l.MX + 2 <= 1026 && h.bx1 >= 1 && h.bx1 <= l.MX && h.bxm >= 1 && h.bxm <= l.MX &&
+                        h.by1 >= 1 && h.by1 <= l.MY && h.bym >= 1 && h.bym <= l.MY;
+  for (int sx : {8, 4}) {
+    if (!strip_ok || h.strip_sx || l.MX % sx || (want_sx >= 0 && want_sx != sx)) continue;
+    const int nact = (l.MX / sx) * l.MY;
+    if (nact > 1024) continue;
+    const int stride = nact <= 256 ? 256 : (nact <= 512 ? 512 : 1024);
+    const size_t srows = (size_t)2 * sx * stride * D;
+    if (srows > lds_limit()) continue;
+    h.strip_sx = sx;
+    h.strip_stride = stride;
+    h.row_stride = sx * stride;
+    rows = srows;
+    h.threads = std::max(h.threads, stride);
+  }
+  h.use_lds = rows <= lds_limit() ? 1 : 0;
+  S->lds_bytes = h.use_lds ? rows : 0;
+}
+
+// Layout choices the autotuner made, per lattice shape, for this process:
+// systems built later with the same shape start from the tuned layout.
+namespace {
+struct TunedLayout {
+  int mode;      // SCF_MODE_FUSED or SCF_MODE_GRID
+  int strip_sx;  // fused path: 8, 4 or 0
+};
+std::mutex g_tune_mu;
+std::map<std::string, TunedLayout> g_tuned;
+}  // namespace
+
+std::string shape_key(const scf_system *S) {
+  const LatticeSpec &l = S->spec.lat;
+  std::string k = std::to_string(l.gradients) + "/" + std::to_string(l.geometry) + "/" +
+                  std::to_string(l.lattice_type) + "/" + std::to_string(l.MX) + "x" + std::to_string(l.MY) + "x" +
+                  std::to_string(l.MZ) + "/f" + std::to_string(l.fjc);
+  for (auto &m : S->spec.mol) k += "/" + std::to_string(m.chainlength) + "m" + std::to_string(m.markov);
+  return k;
+}
+
+int tuned_strip(scf_system *S) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tuned.find(shape_key(S));
+  if (it == g_tuned.end()) return -1;
+  S->tuned_mode = it->second.mode;
+  return it->second.strip_sx;
 }
 
 int build(scf_system *S, const Placement *pl) {
@@ -292,34 +364,7 @@ int build(scf_system *S, const Placement *pl) {
   h.x0 = P(o_x0);
   h.sc = (DevScalars *)(base + o_sc);
   h.partial = P(o_part);
-  // execution choice for the fused path
-  int work = l.gradients == 1 ? M : W;
-  h.threads = work <= 64 ? 64 : (work <= 128 ? 128 : (work <= 256 ? 256 : (work <= 512 ? 512 : 1024)));
-  h.strip_sx = 0;
-  h.strip_stride = 0;
-  size_t rows = (size_t)2 * maxR * M * D;
-  h.row_stride = maxR * M;
-  // 2-gradient strip path: every thread owns SX sites along x; needs
-  // Markov-1 chains, the 9-point stencil, ghosts that map onto interior
-  // sites, and the rows of one strip layout in LDS
-  const bool strip_ok = l.gradients == 2 && maxR == 1 && (l.geometry != 0 || l.stencil_full) &&
-                        l.MX + 2 <= 1026 && h.bx1 >= 1 && h.bx1 <= l.MX && h.bxm >= 1 && h.bxm <= l.MX &&
-                        h.by1 >= 1 && h.by1 <= l.MY && h.bym >= 1 && h.bym <= l.MY;
-  for (int sx : {8, 4}) {
-    if (!strip_ok || h.strip_sx || l.MX % sx) continue;
-    const int nact = (l.MX / sx) * l.MY;
-    if (nact > 1024) continue;
-    const int stride = nact <= 256 ? 256 : (nact <= 512 ? 512 : 1024);
-    const size_t srows = (size_t)2 * sx * stride * D;
-    if (srows > lds_limit()) continue;
-    h.strip_sx = sx;
-    h.strip_stride = stride;
-    h.row_stride = sx * stride;
-    rows = srows;
-    h.threads = std::max(h.threads, stride);
-  }
-  h.use_lds = rows <= lds_limit() ? 1 : 0;
-  S->lds_bytes = h.use_lds ? rows : 0;
+  plan_layout(S, tuned_strip(S));
   h.precision = SCF_PREC_FP64;
   h.mixed_switch = 1e-4;
   {
@@ -506,6 +551,96 @@ int scf_system_set_checkpoint(scf_system *S, int interval) {
   if (S->h.strip_sx == 0 || S->effective_mode() != SCF_MODE_FUSED)
     return fail(SCF_E_UNSUPPORTED, "checkpointed propagators need the 2-gradient register-strip path");
   S->h.ckpt = interval;
+  return upload_solver(S);
+}
+
+int scf_system_layout(const scf_system *S, int *mode, int *strip_sx, int *threads) {
+  if (!S) return fail(SCF_E_ARG, "null system");
+  if (mode) *mode = S->effective_mode();
+  if (strip_sx) *strip_sx = S->h.strip_sx;
+  if (threads) *threads = S->h.threads;
+  return SCF_OK;
+}
+
+void scf_autotune_forget(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned.clear();
+}
+
+int scf_system_autotune(scf_system *S, int evals, scf_tune_report *rep) {
+  if (!S) return fail(SCF_E_ARG, "null system");
+  if (evals < 1) evals = 1;
+  scf_tune_report tmp;
+  if (!rep) rep = &tmp;
+  std::memset(rep, 0, sizeof(*rep));
+  const LatticeSpec &l = S->spec.lat;
+  const bool need_strip = S->h.precision != SCF_PREC_FP64 || S->h.ckpt > 1;
+  // candidate layouts
+  bool markov2 = false;
+  for (auto &m : S->spec.mol) markov2 = markov2 || m.markov == 2;
+  std::vector<TunedLayout> cand;
+  if (S->mode == SCF_MODE_AUTO && !markov2 && !need_strip) cand.push_back({SCF_MODE_GRID, 0});
+  if (S->mode != SCF_MODE_GRID) {
+    if (l.gradients == 2) {
+      for (int sx : {8, 4}) {
+        plan_layout(S, sx);
+        if (S->h.strip_sx == sx) cand.push_back({SCF_MODE_FUSED, sx});
+      }
+      if (!need_strip) cand.push_back({SCF_MODE_FUSED, 0});
+    } else {
+      cand.push_back({SCF_MODE_FUSED, 0});
+    }
+  }
+  if (cand.empty()) cand.push_back({SCF_MODE_GRID, 0});
+  const int saved_mode = S->mode, saved_tuned = S->tuned_mode;
+  double best = 1e300;
+  int besti = 0;
+  DevSys &h = S->h;
+  for (size_t c = 0; c < cand.size() && c < SCF_TUNE_MAX; c++) {
+    S->tuned_mode = cand[c].mode;
+    plan_layout(S, cand[c].mode == SCF_MODE_FUSED && l.gradients == 2 ? cand[c].strip_sx : -1);
+    int rc = upload_solver(S);
+    if (rc != SCF_OK) return rc;
+    auto one = [&]() -> hipError_t {
+      if (S->effective_mode() == SCF_MODE_FUSED)
+        return launch_fused(S->d_list, 1, FUSED_RESIDUAL, h.threads, S->lds_bytes, S->stream);
+      hipError_t e = grid_residual(h, S->d, h.partial, S->nb, S->stream);
+      return e != hipSuccess ? e : grid_thermo(h, S->d, h.partial, S->nb, S->stream);
+    };
+    HIP_TRY(one());  // warm-up (and the cost bound for slow layouts)
+    HIP_TRY(hipEventRecord(S->ev0, S->stream));
+    HIP_TRY(one());
+    HIP_TRY(hipEventRecord(S->ev1, S->stream));
+    HIP_TRY(hipEventSynchronize(S->ev1));
+    float ms1 = 0;
+    HIP_TRY(hipEventElapsedTime(&ms1, S->ev0, S->ev1));
+    double ms = ms1;
+    if (ms1 < 4 * best && evals > 1) {  // a clearly slower layout is not timed further
+      HIP_TRY(hipEventRecord(S->ev0, S->stream));
+      for (int e = 0; e < evals; e++) HIP_TRY(one());
+      HIP_TRY(hipEventRecord(S->ev1, S->stream));
+      HIP_TRY(hipEventSynchronize(S->ev1));
+      HIP_TRY(hipEventElapsedTime(&ms1, S->ev0, S->ev1));
+      ms = ms1 / evals;
+    }
+    rep->mode[c] = cand[c].mode;
+    rep->strip_sx[c] = l.gradients == 2 && cand[c].mode == SCF_MODE_FUSED ? h.strip_sx : 0;
+    rep->ms_per_eval[c] = ms;
+    rep->candidates = (int)c + 1;
+    if (ms < best) {
+      best = ms;
+      besti = (int)c;
+    }
+  }
+  rep->chosen = besti;
+  TunedLayout pick{rep->mode[besti], rep->strip_sx[besti]};
+  S->mode = saved_mode;
+  S->tuned_mode = saved_mode == SCF_MODE_AUTO ? pick.mode : saved_tuned;
+  plan_layout(S, pick.mode == SCF_MODE_FUSED && l.gradients == 2 ? pick.strip_sx : -1);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[shape_key(S)] = pick;
+  }
   return upload_solver(S);
 }
 

@@ -23,6 +23,7 @@ struct scf_system {
   hipGraphExec_t graph_exec = nullptr;  // captured grid-path iteration
   bool use_graph = true;
   int mode = SCF_MODE_AUTO;
+  int tuned_mode = SCF_MODE_AUTO;  // the autotuner's choice for this shape (used under AUTO)
   int nb = 0;            // workgroups of the grid path
   int m_alloc = 0;       // DIIS history the arena was sized for
   size_t lds_bytes = 0;  // dynamic LDS the fused path asks for (0: rows in HBM)
@@ -32,6 +33,7 @@ struct scf_system {
 
   int effective_mode() const {
     if (mode != SCF_MODE_AUTO) return mode;
+    if (tuned_mode != SCF_MODE_AUTO) return tuned_mode;
     return spec.lat.gradients == 3 ? SCF_MODE_GRID : SCF_MODE_FUSED;
   }
 };
@@ -63,5 +65,8 @@ int read_scalars(const scf_system *S, DevScalars &sc);
 int grid_solve(scf_system *S, scf_solve_info *info);
 double wall_clock_khz();
 size_t lds_limit();
+void plan_layout(scf_system *S, int want_sx);
+int tuned_strip(scf_system *S);
+std::string shape_key(const scf_system *S);
 
 }  // namespace nscf

@@ -36,6 +36,11 @@ class SolveInfo(ctypes.Structure):
                 ("fp32_iterations", ctypes.c_int)]
 
 
+class TuneReport(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_int), ("chosen", ctypes.c_int), ("mode", ctypes.c_int * 4),
+                ("strip_sx", ctypes.c_int * 4), ("ms_per_eval", ctypes.c_double * 4)]
+
+
 class JobResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("fp32_iterations", ctypes.c_int),
                 ("residual", ctypes.c_double), ("free_energy", ctypes.c_double),
@@ -98,6 +103,11 @@ def lib():
         L.scf_get_phase_ms.argtypes = [vp, _D]
         L.scf_system_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.scf_system_set_checkpoint.argtypes = [vp, ctypes.c_int]
+        L.scf_system_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(TuneReport)]
+        L.scf_system_layout.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int)]
+        L.scf_autotune_forget.argtypes = []
+        L.scf_autotune_forget.restype = None
         L.scf_describe_input.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
         L.scf_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(vp)]
         L.scf_server_set_slice.argtypes = [vp, ctypes.c_double]
@@ -222,6 +232,22 @@ class ScfSystem:
         """FP64 throughout (mixed=False) or FP32 propagators until the
         residual falls below `switch_residual`, then FP64 (mixed=True)."""
         _check(lib().scf_system_set_precision(self.h, 1 if mixed else 0, float(switch_residual)))
+
+    def autotune(self, evals=5):
+        """Time every applicable launch layout and keep the fastest (also for
+        systems of the same lattice shape built later in this process)."""
+        r = TuneReport()
+        _check(lib().scf_system_autotune(self.h, int(evals), ctypes.byref(r)))
+        names = {MODE_FUSED: "fused", MODE_GRID: "grid"}
+        cands = [dict(mode=names.get(r.mode[i], r.mode[i]), strip_sx=r.strip_sx[i], ms_per_eval=r.ms_per_eval[i])
+                 for i in range(r.candidates)]
+        return dict(candidates=cands, chosen=r.chosen)
+
+    def layout(self):
+        m, sx, th = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().scf_system_layout(self.h, ctypes.byref(m), ctypes.byref(sx), ctypes.byref(th)))
+        return dict(mode={MODE_FUSED: "fused", MODE_GRID: "grid"}.get(m.value, m.value), strip_sx=sx.value,
+                    threads=th.value)
 
     def set_checkpoint(self, interval):
         """Keep every `interval`-th end-point distribution and recompute the
@@ -394,3 +420,8 @@ def solve_resident(systems, with_solve_ms=False):
     if with_solve_ms:
         return [(i.iterations, i.residual, i.status, i.kernel_ms, i.solve_ms) for i in infos]
     return [(i.iterations, i.residual, i.status, i.kernel_ms) for i in infos]
+
+
+def autotune_forget():
+    """Forget the launch layouts remembered per lattice shape."""
+    lib().scf_autotune_forget()

@@ -305,6 +305,44 @@ def test_checkpointed_propagators_equal_full_storage(golden, case):
     off.close()
 
 
+@pytest.mark.parametrize("case", ["brush2d_rod_strip", "planar2d_particle", "diblock3d_periodic",
+                                  "cfg1_planar_adsorption"])
+def test_autotuned_layout_gives_the_same_residual(golden, case):
+    """The autotuner times every applicable layout (strip widths 8 and 4, the
+    generic chain, the grid path), keeps the fastest, and a system built
+    afterwards with the same lattice shape starts from it; every layout gives
+    the default layout's residual and results to rounding (1e-12)."""
+    from namics_amd import ScfSystem, autotune_forget
+    fx = golden[case]
+    x = np.random.default_rng(61).uniform(-0.3, 0.3, Oracle(fx["text"]).nvar)
+    autotune_forget()
+    try:
+        s = ScfSystem(fx["text"])
+        default = s.layout()
+        g0, r0 = s.residual(x)
+        F0 = s.results()["free_energy"]
+        print("case", case, "autotune from", default, flush=True)
+        rep = s.autotune(evals=3)
+        print("case", case, "candidates", rep, flush=True)
+        assert 1 <= len(rep["candidates"]) <= 4 and 0 <= rep["chosen"] < len(rep["candidates"])
+        assert all(c["ms_per_eval"] > 0 for c in rep["candidates"])
+        best = rep["candidates"][rep["chosen"]]
+        assert best["ms_per_eval"] == min(c["ms_per_eval"] for c in rep["candidates"])
+        lay = s.layout()
+        assert lay["mode"] == best["mode"] and lay["strip_sx"] == best["strip_sx"]
+        g1, r1 = s.residual(x)
+        assert rel_err(r1, r0) < 1e-12 and scaled_err(g1, g0) < 1e-12
+        assert rel_err(s.results()["free_energy"], F0) < 1e-12
+        s2 = ScfSystem(fx["text"])
+        assert s2.layout() == lay
+        g2, r2 = s2.residual(x)
+        assert np.array_equal(g2, g1) and r2 == r1
+        s.close()
+        s2.close()
+    finally:
+        autotune_forget()
+
+
 def test_server_results_equal_alone_and_admission_waits(golden):
     """Request server: a mix of 1-, 2- and 3-gradient requests, submitted in
     two waves into a pool that holds only a few of them at a time, gives
