@@ -93,12 +93,8 @@ int validate(const SystemSpec &s, std::string &why) {
   if (l.gradients >= 2)
     for (int d = 0; d < 6; d++)
       if (l.bc[d] == BC_SURFACE) return why = "surface boundaries are supported with 1 gradient only", SCF_E_UNSUPPORTED;
-  if (l.gradients == 2 && l.geometry == GEOM_PLANAR && l.stencil_full && l.lattice_type != LAT_SIMPLE_CUBIC)
-    return why = "2-gradient planar hexagonal stencil is not supported", SCF_E_UNSUPPORTED;
   if (l.gradients == 2 && l.geometry == GEOM_PLANAR && !l.stencil_full && l.lattice_type != LAT_SIMPLE_CUBIC)
-    return why = "2-gradient planar hexagonal stencil is not supported", SCF_E_UNSUPPORTED;
-  if (l.gradients == 3 && !(l.stencil_full && l.lattice_type == LAT_SIMPLE_CUBIC))
-    return why = "3 gradients: only the simple-cubic 6-point stencil is supported", SCF_E_UNSUPPORTED;
+    return why = "2-gradient planar hexagonal stencil without stencil_full is not supported", SCF_E_UNSUPPORTED;
   for (auto &m : s.mol)
     if (m.markov == 2 && l.gradients != 1) return why = "Markov 2 needs 1 gradient", SCF_E_UNSUPPORTED;
   return SCF_OK;

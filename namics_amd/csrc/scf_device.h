@@ -214,6 +214,67 @@ __device__ __forceinline__ bool is_interior(const Sys &S, int x, int y, int z) {
 // LGrad1::propagate src/LGrad1.cpp:441-445, LGrad2::propagate
 // src/LGrad2.cpp:1058-1073, LG2Planar::propagate src/LG2Planar.cpp:1085-1096,
 // LGrad3::propagate src/LGrad3.cpp:733-743).
+// 3-gradient stencils over the 26 neighbours (LGrad3::propagate
+// src/LGrad3.cpp:686-729 without stencil_full: faces, edges and corners with
+// the weights 16 : 4 : 1 / 152 on the simple-cubic lattice and 4 : 2 : 1 / 56
+// on the hexagonal one; the hexagonal stencil_full form, src/LGrad3.cpp:
+// 746-782, adds the site twice and sums to 64/40, as the reference does).
+// Neighbours are summed in the reference's pass order; ghosts map
+// coordinate by coordinate.
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ T stencil3_wide(const Sys &S, Ptr g, int x, int y, int z) {
+  const int JX = S.JX, JY = S.JY;
+  const int xm = map_x(S, x - 1) * JX, x0 = x * JX, xp = map_x(S, x + 1) * JX;
+  const int ym = map_y(S, y - 1) * JY, y0 = y * JY, yp = map_y(S, y + 1) * JY;
+  const int zm = map_z(S, z - 1), z0 = z, zp = map_z(S, z + 1);
+  const bool hexfull = S.stencil_full;  // only the hexagonal lattice reaches here with stencil_full
+  const T f1 = hexfull ? (T)2.0 : (S.lattice_type == 0 ? (T)4.0 : (T)2.0);
+  T v = hexfull ? (T)2.0 * g[x0 + y0 + z0] : (T)0;
+  v += g[xm + y0 + z0];
+  v += g[xp + y0 + z0];
+  v += g[x0 + ym + z0];
+  v += g[x0 + yp + z0];
+  v += g[x0 + y0 + zm];
+  v += g[x0 + y0 + zp];
+  v *= f1;
+  v += g[xm + ym + z0];
+  v += g[xp + yp + z0];
+  if (hexfull) {
+    v += g[xm + yp + z0];
+    v += g[xp + ym + z0];
+    v += g[xm + y0 + zm];
+    v += g[xp + y0 + zm];
+    v += g[x0 + ym + zm];
+    v += g[x0 + yp + zm];
+    v += g[xm + y0 + zp];
+    v += g[xp + y0 + zp];
+    v += g[x0 + ym + zp];
+    v += g[x0 + yp + zp];
+  } else {
+    v += g[xp + ym + z0];
+    v += g[xm + yp + z0];
+    v += g[xm + y0 + zm];
+    v += g[xp + y0 + zp];
+    v += g[xm + y0 + zp];
+    v += g[xp + y0 + zm];
+    v += g[x0 + ym + zm];
+    v += g[x0 + yp + zp];
+    v += g[x0 + ym + zp];
+    v += g[x0 + yp + zm];
+  }
+  v *= f1;
+  v += g[xm + ym + zm];
+  v += g[xp + yp + zp];
+  v += g[xm + ym + zp];
+  v += g[xp + yp + zm];
+  v += g[xm + yp + zm];
+  v += g[xp + ym + zp];
+  v += g[xp + ym + zm];
+  v += g[xm + yp + zp];
+  const T c = hexfull ? (T)(1.0 / 40.0) : (S.lattice_type == 0 ? (T)(1.0 / 152.0) : (T)(1.0 / 56.0));
+  return v * c;
+}
+
 // Refined 1-gradient stencil (fjc > 1): 2 fjc + 1 taps weighted by the
 // LAMBDA rows, summed in the order of LG1Planar::propagate / LGrad1::
 // propagate / Side (src/LG1Planar.cpp:151-156, src/LGrad1.cpp:447-452,
@@ -257,6 +318,21 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
     const int xm = map_x(S, x - 1) * JX, x0 = x * JX, xp = map_x(S, x + 1) * JX;
     const int ym = map_y(S, y - 1), yp = map_y(S, y + 1);
     if (S.geometry == 0) {
+      if (S.stencil_full && S.lattice_type != 0) {
+        // hexagonal 9-point, LG2Planar::propagate src/LG2Planar.cpp:1092-1108:
+        // (2 (2 centre + 4 nearest) + 4 diagonals) / 16
+        T v = (T)2.0 * g[x0 + y];
+        v += g[xm + y];
+        v += g[xp + y];
+        v += g[x0 + ym];
+        v += g[x0 + yp];
+        v *= (T)2.0;
+        v += g[xm + ym];
+        v += g[xp + yp];
+        v += g[xm + yp];
+        v += g[xp + ym];
+        return v * (T)(1.0 / 16.0);
+      }
       if (S.stencil_full) {
         const T C1 = (T)(16.0 / 36.0), C2 = (T)(4.0 / 36.0), C3 = (T)(1.0 / 36.0);
         T v = C1 * g[x0 + y];
@@ -294,14 +370,17 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
     return v * C2;
   }
   const int JX = S.JX, JY = S.JY;
-  const int bx = x * JX, by = y * JY;
-  T v = g[map_x(S, x - 1) * JX + by + z];
-  v += g[map_x(S, x + 1) * JX + by + z];
-  v += g[bx + map_y(S, y - 1) * JY + z];
-  v += g[bx + map_y(S, y + 1) * JY + z];
-  v += g[bx + by + map_z(S, z - 1)];
-  v += g[bx + by + map_z(S, z + 1)];
-  return v * (T)(1.0 / 6.0);
+  if (S.stencil_full && S.lattice_type == 0) {
+    const int bx = x * JX, by = y * JY;
+    T v = g[map_x(S, x - 1) * JX + by + z];
+    v += g[map_x(S, x + 1) * JX + by + z];
+    v += g[bx + map_y(S, y - 1) * JY + z];
+    v += g[bx + map_y(S, y + 1) * JY + z];
+    v += g[bx + by + map_z(S, z - 1)];
+    v += g[bx + by + map_z(S, z + 1)];
+    return v * (T)(1.0 / 6.0);
+  }
+  return stencil3_wide<T>(S, g, x, y, z);
 }
 
 // Site average <X> for the Flory-Huggins term (Lattice::Side,

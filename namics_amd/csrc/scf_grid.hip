@@ -518,6 +518,9 @@ hipError_t grid_residual(const DevSys &h, DevSys *d, double *partial, int nb, hi
 hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
   const dim3 B(kGridThreads), G(nb);
   const dim3 G3((h.MY * h.MZ + kGridThreads - 1) / kGridThreads, h.MX);
+  // the plane-mapped 6-point kernels serve the default simple-cubic 3-D
+  // stencil; the 26-neighbour stencils go through the generic step
+  const bool six = h.gradients == 3 && h.stencil_full && h.lattice_type == 0;
   hipLaunchKernelGGL(k_prepare, G, B, 0, st, d);
   for (int mo = 0; mo < h.nmol; mo++) {
     if (h.mol_markov[mo] != 1) return hipErrorInvalidValue;
@@ -526,7 +529,7 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
     int s = 1;
     for (int b = b0; b < b1; b++)
       for (int j = (b == b0 ? 1 : 0); j < h.block_n[b]; j++, s++)
-        if (h.gradients == 3)
+        if (six)
           hipLaunchKernelGGL(k_step3_fwd, G3, B, 0, st, d, s, h.block_seg[b]);
         else
           hipLaunchKernelGGL(k_step_fwd, G, B, 0, st, d, s, h.block_seg[b]);
@@ -537,7 +540,7 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
     int src = 0;
     for (int b = b1 - 1; b >= b0; b--)
       for (int j = (b == b1 - 1 ? h.block_n[b] - 2 : h.block_n[b] - 1); j >= 0; j--, s--) {
-        if (h.gradients == 3)
+        if (six)
           hipLaunchKernelGGL(k_step3_bwd, G3, B, 0, st, d, s, h.block_seg[b], h.block_mm[b], src);
         else
           hipLaunchKernelGGL(k_step_bwd, G, B, 0, st, d, s, h.block_seg[b], h.block_mm[b], src);

@@ -285,6 +285,71 @@ struct Lat {
     AddTimes(out, X, LA + f * M, M);
   }
 
+  // 3-gradient stencils over the 26 neighbours, the passes of
+  // LGrad3::propagate / Side without stencil_full (src/LGrad3.cpp:686-729,
+  // 111-152; simple cubic 4, 4, 1/152; hexagonal 2, 2, 1/56) and the
+  // hexagonal stencil_full form (src/LGrad3.cpp:746-782, 166-205), with the
+  // reference's pass lengths.
+  void wide3(double *gs, double *gs_1) const {
+    const int M = l.M, JX = l.JX, JY = l.JY;
+    const bool sc = l.lattice_type == LAT_SIMPLE_CUBIC;
+    if (!l.stencil_full) {
+      const double n1 = sc ? 4.0 : 2.0;
+      Add(gs + JX, gs_1, M - JX);
+      Add(gs, gs_1 + JX, M - JX);
+      Add(gs + JY, gs_1, M - JY);
+      Add(gs, gs_1 + JY, M - JY);
+      Add(gs + 1, gs_1, M - 1);
+      Add(gs, gs_1 + 1, M - 1);
+      Norm(gs, n1, M);
+      Add(gs + JX + JY, gs_1, M - JX - JY);
+      Add(gs, gs_1 + JX + JY, M - JX - JY);
+      Add(gs + JY, gs_1 + JX, M - JY - JX);
+      Add(gs + JX, gs_1 + JY, M - JY - JX);
+      Add(gs + JX + 1, gs_1, M - JX - 1);
+      Add(gs, gs_1 + JX + 1, M - JX - 1);
+      Add(gs + JX, gs_1 + 1, M - JX);
+      Add(gs + 1, gs_1 + JX, M - JX);
+      Add(gs + JY + 1, gs_1, M - JY - 1);
+      Add(gs, gs_1 + JY + 1, M - JX - 1);
+      Add(gs + JY, gs_1 + 1, M - JY);
+      Add(gs + 1, gs_1 + JY, M - JY);
+      Norm(gs, n1, M);
+    } else {
+      const double Two = 2.0;
+      YplusisCtimesX(gs, gs_1, Two, M);
+      Add(gs + JX, gs_1, M - JX);
+      Add(gs, gs_1 + JX, M - JX);
+      Add(gs + JY, gs_1, M - JY);
+      Add(gs, gs_1 + JY, M - JY);
+      Add(gs + 1, gs_1, M - 1);
+      Add(gs, gs_1 + 1, M - 1);
+      Norm(gs, Two, M);
+      Add(gs + JX + JY, gs_1, M - JX - JY);
+      Add(gs, gs_1 + JX + JY, M - JX - JY);
+      Add(gs + JX, gs_1 + JY, M - JX - JY);
+      Add(gs + JY, gs_1 + JX, M - JX - JY);
+      Add(gs + JX + 1, gs_1, M - JX - 1);
+      Add(gs + 1, gs_1 + JX, M - JX - 1);
+      Add(gs + JY + 1, gs_1, M - JY - 1);
+      Add(gs + 1, gs_1 + JY, M - JY - 1);
+      Add(gs + JX, gs_1 + 1, M - JX - 1);
+      Add(gs, gs_1 + JX + 1, M - JX - 1);
+      Add(gs + JY, gs_1 + 1, M - JY - 1);
+      Add(gs, gs_1 + JY + 1, M - JY - 1);
+      Norm(gs, Two, M);
+    }
+    Add(gs + JX + JY + 1, gs_1, M - JX - JY - 1);
+    Add(gs, gs_1 + JX + JY + 1, M - JX - JY - 1);
+    Add(gs + JX + JY, gs_1 + 1, M - JX - JY - 1);
+    Add(gs + 1, gs_1 + JX + JY, M - JX - JY - 1);
+    Add(gs + JX + 1, gs_1 + JY, M - JX - JY - 1);
+    Add(gs + JY, gs_1 + JX + 1, M - JX - JY - 1);
+    Add(gs + JY + 1, gs_1 + JX, M - JX - JY - 1);
+    Add(gs + JX, gs_1 + JY + 1, M - JX - JY - 1);
+    Norm(gs, l.stencil_full ? 1.0 / 40.0 : (sc ? 1.0 / 152.0 : 1.0 / 56.0), M);
+  }
+
   // Markov-1 step: gs <- G1 * W(gs_1); gs_1 gets its ghosts set in place.
   void propagate(double *gs, double *gs_1, const double *G1) const {
     const int M = l.M, JX = l.JX;
@@ -364,8 +429,11 @@ struct Lat {
       Times(gs, gs, G1, M);
     } else {
       const int JY = l.JY;
-      if (!(l.stencil_full && l.lattice_type == LAT_SIMPLE_CUBIC))
-        throw std::string("unsupported 3-D stencil");
+      if (!(l.stencil_full && l.lattice_type == LAT_SIMPLE_CUBIC)) {
+        wide3(gs, gs_1);
+        Times(gs, gs, G1, M);
+        return;
+      }
       Add(gs + JX, gs_1, M - JX);
       Add(gs, gs_1 + JX, M - JX);
       Add(gs + JY, gs_1, M - JY);
@@ -524,6 +592,8 @@ struct Lat {
         AddTimes(Xs, X + JX + 1, lp, M - JX - 1);
         Norm(Xs, C2, M);
       }
+    } else if (!(l.stencil_full && l.lattice_type == LAT_SIMPLE_CUBIC)) {
+      wide3(Xs, X);
     } else {
       const int JY = l.JY;
       Add(Xs + JX, X, M - JX);

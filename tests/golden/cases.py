@@ -420,6 +420,55 @@ mol : water : freedom : solvent
 """,
         eval=[(11, 0.3)],
     ),
+    # 2-gradient planar, hexagonal 9-point stencil, frozen particle, periodic
+    # y (radial size a multiple of 4: register-strip path)
+    "hex2d_planar_particle": dict(
+        text="""
+lat : L : gradients : 2
+lat : L : n_layers_x : 12
+lat : L : n_layers_y : 16
+lat : L : lattice_type : hexagonal
+lat : L : lowerbound_y : periodic
+lat : L : upperbound_y : periodic
+mon : S : freedom : frozen
+mon : S : frozen_range : 5,6;7,9
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -1.0
+mon : A : chi_W : 0.5
+mol : pol : composition : (A)20
+mol : pol : freedom : free
+mol : pol : phibulk : 0.02
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(51, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-10, seed=52, amp=0.05)],
+    ),
+    # 2-gradient planar hexagonal brush grafted on a frozen wall (14 layers:
+    # the site-strided path)
+    "hex2d_planar_brush": dict(
+        text="""
+lat : L : gradients : 2
+lat : L : n_layers_x : 14
+lat : L : n_layers_y : 10
+lat : L : lattice_type : hexagonal
+mon : S : freedom : frozen
+mon : S : frozen_range : 1,1;1,10
+mon : X : freedom : pinned
+mon : X : pinned_range : 2,1;2,10
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.4
+mon : S : chi_A : -0.2
+mol : brush : composition : (X)1(A)24
+mol : brush : freedom : restricted
+mol : brush : n : 3
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(53, 0.3)],
+    ),
     # 3-gradient periodic diblock box (small version of configuration 3)
     "diblock3d_periodic": dict(
         text="""
@@ -469,5 +518,82 @@ mol : water : composition : (W)1
 mol : water : freedom : solvent
 """,
         eval=[(15, 0.3)],
+    ),
+    # 3-gradient 27-point stencil (stencil_full false): periodic diblock box
+    "diblock3d_27pt": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 8
+lat : box : n_layers_y : 7
+lat : box : n_layers_z : 9
+lat : box : lattice_type : simple_cubic
+lat : box : stencil_full : false
+lat : box : lowerbound_x : periodic
+lat : box : upperbound_x : periodic
+lat : box : lowerbound_y : periodic
+lat : box : upperbound_y : periodic
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : A : freedom : free
+mon : B : freedom : free
+mon : W : freedom : free
+mon : B : chi_W : 1.6
+mon : A : chi_B : 1.6
+mol : surf : composition : (A)10(B)10
+mol : surf : freedom : restricted
+mol : surf : theta : 30
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(55, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-9, seed=56, amp=0.1)],
+    ),
+    # 3-gradient hexagonal lattice (the reference's unnormalised 64/40
+    # stencil), mirror box with a frozen cube
+    "hex3d_particle": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 7
+lat : box : n_layers_y : 8
+lat : box : n_layers_z : 6
+lat : box : lattice_type : hexagonal
+mon : S : freedom : frozen
+mon : S : frozen_range : 1,1,1;2,3,2
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -0.8
+mon : A : chi_W : 0.4
+mol : pol : composition : (A)12
+mol : pol : freedom : free
+mol : pol : phibulk : 0.03
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(57, 0.2)],
+    ),
+    # 3-gradient hexagonal lattice without stencil_full (4 : 2 : 1 / 56),
+    # mirror in x, periodic in y and z
+    "hex3d_27pt": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 6
+lat : box : n_layers_y : 7
+lat : box : n_layers_z : 8
+lat : box : lattice_type : hexagonal
+lat : box : stencil_full : false
+lat : box : lowerbound_y : periodic
+lat : box : upperbound_y : periodic
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.7
+mol : pol : composition : (A)10
+mol : pol : freedom : restricted
+mol : pol : theta : 20
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(58, 0.3)],
     ),
 }

@@ -322,3 +322,51 @@ def test_surface_needs_a_frozen_layer():
     reference refuses it (System::CheckInput)."""
     with pytest.raises(ValueError, match="lonely"):
         lat1(extra="lat : L : lowerbound : surface")
+
+
+BOX27 = """
+lat : box : gradients : 3
+lat : box : n_layers_x : 8
+lat : box : n_layers_y : 6
+lat : box : n_layers_z : 10
+lat : box : lattice_type : {lt}
+lat : box : stencil_full : {full}
+lat : box : lowerbound_x : periodic
+lat : box : upperbound_x : periodic
+lat : box : lowerbound_y : periodic
+lat : box : upperbound_y : periodic
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : A : freedom : free
+mon : W : freedom : free
+mol : pol : composition : (A)5
+mol : pol : freedom : free
+mol : pol : phibulk : 0.1
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+"""
+
+
+@pytest.mark.parametrize("lt,full", [("simple_cubic", "false"), ("hexagonal", "true"), ("hexagonal", "false")])
+def test_3d_wide_stencil_symbols(lt, full):
+    """The 26-neighbour 3-D stencils on a periodic box: a product of cosines
+    is an eigenvector with eigenvalue (w0 + 2 wf sum c + 4 we sum cc + 8 wc ccc)
+    -- simple cubic without stencil_full (0, 16, 4, 1)/152, hexagonal
+    (8, 4, 2, 1)/40 (summing to 64/40, as in the reference), hexagonal
+    without stencil_full (0, 4, 2, 1)/56 -- for the propagator and the site
+    average alike."""
+    o = Oracle(BOX27.format(lt=lt, full=full))
+    MX, MY, MZ = 8, 6, 10
+    w0, wf, we, wc, nrm = {("simple_cubic", "false"): (0, 16, 4, 1, 152), ("hexagonal", "true"): (8, 4, 2, 1, 40),
+                           ("hexagonal", "false"): (0, 4, 2, 1, 56)}[(lt, full)]
+    JX, JY = (MY + 2) * (MZ + 2), MZ + 2
+    x, y, z = np.meshgrid(np.arange(1, MX + 1), np.arange(1, MY + 1), np.arange(1, MZ + 1), indexing="ij")
+    idx = (x * JX + y * JY + z).ravel()
+    for jx, jy, jz in ((1, 0, 0), (1, 2, 3), (4, 3, 5), (2, 1, 0)):
+        kx, ky, kz = 2 * np.pi * jx / MX, 2 * np.pi * jy / MY, 2 * np.pi * jz / MZ
+        v = np.zeros(o.M)
+        v[idx] = (np.cos(kx * x + 0.2) * np.cos(ky * y + 0.5) * np.cos(kz * z - 0.3)).ravel()
+        cx, cy, cz = np.cos(kx), np.cos(ky), np.cos(kz)
+        ev = (w0 + 2 * wf * (cx + cy + cz) + 4 * we * (cx * cy + cx * cz + cy * cz) + 8 * wc * cx * cy * cz) / nrm
+        for side in (False, True):
+            np.testing.assert_allclose(o.apply(v, side=side)[idx], ev * v[idx], atol=1e-13)
