@@ -129,7 +129,7 @@ void plan_layout(scf_system *S, int want_sx) {
   // 2-gradient strip path: every thread owns SX sites along x; needs
   // Markov-1 chains, the 9-point stencil, ghosts that map onto interior
   // sites, and the rows of one strip layout in LDS
-  const bool strip_ok = l.gradients == 2 && maxR == 1 && (l.geometry != 0 || l.stencil_full) &&
+  const bool strip_ok = l.gradients == 2 && l.fjc == 1 && maxR == 1 && (l.geometry != 0 || l.stencil_full) &&
                         l.MX + 2 <= 1026 && h.bx1 >= 1 && h.bx1 <= l.MX && h.bxm >= 1 && h.bxm <= l.MX &&
                         h.by1 >= 1 && h.by1 <= l.MY && h.bym >= 1 && h.bym <= l.MY;
   for (int sx : {8, 4}) {
@@ -208,8 +208,10 @@ int build(scf_system *S, const Placement *pl) {
   for (int k = 0; k < l.fjc; k++) {
     h.bx1f[k] = l.bx1f[k];
     h.bxmf[k] = l.bxmf[k];
+    h.by1f[k] = l.by1f[k];
+    h.bymf[k] = l.bymf[k];
   }
-  h.wplanar = 1.0 / l.fjc;
+  h.wplanar = l.gradients == 2 ? 1.0 / (l.fjc * l.fjc) : 1.0 / l.fjc;
   h.lambda = l.lambda;
   h.nseg = nseg;
   h.nit = nit;
@@ -366,7 +368,7 @@ int build(scf_system *S, const Placement *pl) {
   {
     // weighted interior volume (Lattice::WeightedSum of ones)
     double vol = 0;
-    if (l.gradients == 1 && l.fjc > 1) vol = l.volume;
+    if (l.fjc > 1) vol = l.volume;
     for (int x = 1; l.fjc == 1 && x <= l.MX; x++)
       for (int y = 1; y <= (l.gradients >= 2 ? l.MY : 1); y++)
         for (int z = 1; z <= (l.gradients == 3 ? l.MZ : 1); z++) {

@@ -68,8 +68,9 @@ struct DevSys {
   int bx1, bxm, by1, bym, bz1, bzm;
   int surface_lo, surface_hi;  // 1-gradient surface boundaries
   int fjc;                     // lattice refinement (grid layers per bond; > 1 only with 1 gradient)
-  int bx1f[kMaxFjc], bxmf[kMaxFjc];  // refined 1-gradient ghost sources
-  double wplanar;              // site weight of planar sums: 1 / fjc
+  int bx1f[kMaxFjc], bxmf[kMaxFjc];  // refined ghost sources along x
+  int by1f[kMaxFjc], bymf[kMaxFjc];  // and along y (2 gradients)
+  double wplanar;              // site weight of planar sums: 1 / fjc^(gradients)
   const double *LAM;           // refined 1-gradient weights: (2 fjc + 1) rows of M
   double lambda;
   const double *L, *lam_m, *lam_0, *lam_p;  // M each (1 and 2 gradients)
@@ -149,19 +150,27 @@ __device__ __forceinline__ CSys &uniform_sys(const DevSys &S) {
 // ------------------------------------------------------------ index helpers
 template <typename Sys>
 __device__ __forceinline__ int map_x(const Sys &S, int x) {
+  if (S.fjc > 1) {
+    if (x < S.fjc) return S.bx1f[x];
+    if (x >= S.MX + S.fjc) return S.bxmf[x - S.MX - S.fjc];
+    return x;
+  }
   return x < 1 ? S.bx1 : (x > S.MX ? S.bxm : x);
 }
 // 1-gradient site x (0..M-1) -> the site set_bounds copies there; on a
-// refined lattice the ghosts are fjc layers per side (src/LGrad1.cpp:823-835)
+// refined lattice the ghosts are fjc layers per side (src/LGrad1.cpp:823-835,
+// src/LGrad2.cpp:1674-1687)
 template <typename Sys>
 __device__ __forceinline__ int map_x1(const Sys &S, int x) {
-  if (S.fjc == 1) return map_x(S, x);
-  if (x < S.fjc) return S.bx1f[x];
-  if (x >= S.MX + S.fjc) return S.bxmf[x - S.MX - S.fjc];
-  return x;
+  return map_x(S, x);
 }
 template <typename Sys>
 __device__ __forceinline__ int map_y(const Sys &S, int y) {
+  if (S.fjc > 1) {
+    if (y < S.fjc) return S.by1f[y];
+    if (y >= S.MY + S.fjc) return S.bymf[y - S.MY - S.fjc];
+    return y;
+  }
   return y < 1 ? S.by1 : (y > S.MY ? S.bym : y);
 }
 template <typename Sys>
@@ -185,8 +194,9 @@ __device__ __forceinline__ int work_site(const Sys &S, int w, int &x, int &y, in
     return w;
   }
   if (S.gradients == 2) {
-    x = w / S.MY + 1;
-    y = w - (x - 1) * S.MY + 1;
+    const int xs = w / S.MY;
+    x = xs + S.fjc;
+    y = w - xs * S.MY + S.fjc;
     z = 0;
     return x * S.JX + y;
   }
@@ -201,6 +211,7 @@ __device__ __forceinline__ int work_site(const Sys &S, int w, int &x, int &y, in
 template <typename Sys>
 __device__ __forceinline__ bool is_interior(const Sys &S, int x, int y, int z) {
   if (S.gradients == 1) return x >= S.fjc && x < S.MX + S.fjc;
+  if (S.gradients == 2) return x >= S.fjc && x < S.MX + S.fjc && y >= S.fjc && y < S.MY + S.fjc;
   if (x < 1 || x > S.MX) return false;
   if (S.gradients >= 2 && (y < 1 || y > S.MY)) return false;
   if (S.gradients == 3 && (z < 1 || z > S.MZ)) return false;
@@ -295,6 +306,41 @@ __device__ __forceinline__ T refined_taps(const Sys &S, Ptr g, int x) {
   return v;
 }
 
+// Refined 2-gradient stencils (fjc > 1) at an interior site, block sums
+// over the (2 fjc + 1)^2 offsets in the reference's pass order.  Cylindrical
+// (LGrad2::propagate / Side src/LGrad2.cpp:1091-1106, 264-278): offsets
+// with |dy| < fjc weighted by the LAMBDA row of dx, times 2, then those with
+// |dy| = fjc, times 1 / (2 (fjc + 2)).  Planar (LG2Planar src/LG2Planar.cpp:
+// 1110-1124, 117-130): plain sums of the offsets with none, one, two
+// coordinates at +-fjc, times 2, 2 and 1 / (4 (FJC - 2) FJC + 1).
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ T refined2(const Sys &S, Ptr g, int i, int x, int y) {
+  const int f = S.fjc, F = 2 * S.fjc + 1, M = S.M, JX = S.JX;
+  const bool planar = S.geometry == 0;
+  const gdbl *LA = hbm(S.LAM);
+  const int nblocks = planar ? 3 : 2;
+  T v = 0;
+  for (int block = 0; block < nblocks; block++) {
+    for (int dx = -f; dx <= f; dx++) {
+      const int sx = map_x(S, x + dx) * JX;
+      for (int dy = -f; dy <= f; dy++) {
+        const int bk = (planar && (dx == -f || dx == f) ? 1 : 0) + (dy == -f || dy == f ? 1 : 0);
+        if (bk != block) continue;
+        const T gv = g[sx + map_y(S, y + dy)];
+        if (planar)
+          v += gv;
+        else
+          v += (T)LA[(f + dx) * M + i] * gv;
+      }
+    }
+    if (planar)
+      v *= block != 2 ? (T)2.0 : (T)(1.0 / (4.0 * (F - 2) * F + 1));
+    else
+      v *= block != 1 ? (T)2.0 : (T)(1.0 / (2 * (f + 2)));
+  }
+  return v;
+}
+
 template <typename T, typename Sys, typename Ptr>
 __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int y, int z) {
   if (S.gradients == 1) {
@@ -313,6 +359,7 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
     if (x <= M - 2) v += (T)hbm(S.lam_p)[i] * g[map_x(S, x + 1)];
     return v;
   }
+  if (S.gradients == 2 && S.fjc > 1) return refined2<T>(S, g, i, x, y);
   if (S.gradients == 2) {
     const int JX = S.JX;
     const int xm = map_x(S, x - 1) * JX, x0 = x * JX, xp = map_x(S, x + 1) * JX;

@@ -165,6 +165,68 @@ void boundary_indices(LatticeSpec &l) {
     l.bx1f[k] = l.bc[0] == BC_MIRROR ? 2 * f - 1 - k : (l.bc[0] == BC_PERIODIC ? l.MX + k : k);
     l.bxmf[k] = l.bc[3] == BC_MIRROR ? l.MX + f - k - 1 : (l.bc[3] == BC_PERIODIC ? f + k : l.MX + f + k);
   }
+  l.by1f.assign(f, 0);
+  l.bymf.assign(f, 0);
+  for (int k = 0; k < f; k++) {
+    l.by1f[k] = l.bc[1] == BC_MIRROR ? 2 * f - 1 - k : (l.bc[1] == BC_PERIODIC ? l.MY + k : k);
+    l.bymf[k] = l.bc[4] == BC_MIRROR ? l.MY + f - k - 1 : (l.bc[4] == BC_PERIODIC ? f + k : l.MY + f + k);
+  }
+}
+
+// Weights of the refined (fjc > 1) 2-gradient lattices (LGrad2::
+// ComputeLambdas src/LGrad2.cpp:51-97, LG2Planar::ComputeLambdas
+// src/LG2Planar.cpp:15-24).  Cylindrical: the radial rows of the 1-gradient
+// construction at every interior site (x, y), site volume L = 2 PIE r / fjc;
+// planar: no LAMBDA rows (the planar stencil is a plain block sum) and
+// L = 1/fjc for fjc = 2 only, as in the reference.
+void refined_lambdas2(LatticeSpec &l) {
+  const int M = l.M, f = l.fjc, F = l.FJC, MX = l.MX, MY = l.MY, JX = l.JX;
+  const double off = l.offset_first_layer;
+  l.L.assign(M, 0.0);
+  l.lambda_1.assign(M, 0.0);
+  l.lambda0.assign(M, 0.0);
+  l.lambda1.assign(M, 0.0);
+  l.LAMBDA.assign((size_t)F * M, 0.0);
+  if (l.geometry == GEOM_PLANAR) {
+    if (f == 2)
+      for (int i = 0; i < M; i++) l.L[i] = 1.0 / f;
+    return;
+  }
+  double *LA = l.LAMBDA.data();
+  const double c = 1.0 / (1.0 * F - 1.0), top = 1.0 * MX / f;
+  for (int y = f; y < MY + f; y++)
+    for (int x = f; x < MX + f; x++) {
+      const int i = x * JX + y;
+      const double r = off + 1.0 * (x - f + 1.0) / f;
+      double rlow = r - 0.5, rhigh = r + 0.5;
+      l.L[i] = kPIE * (2.0 * r) / f;
+      const double VL = l.L[i] / kPIE * f;
+      if ((rlow - r) * 2 + r > 0.0) LA[i] += c * rlow / VL;
+      if ((rhigh - r) * 2 + r < top) {
+        LA[i + (F - 1) * M] += c * rhigh / VL;
+      } else {
+        const double d = 2 * rhigh - r - top;
+        if (d > -0.001 && d < 0.001) LA[i + (F - 1) * M] += c * rhigh / VL;
+        for (int j = 1; j <= f; j++)
+          if (d > 0.99 * j / f && d < 1.01 * j / f) LA[i + (F - 1) * M] += c * (rhigh - 1.0 * j / f) / VL;
+      }
+      for (int j = 1; j < f; j++) {
+        rlow += 0.5 / (f);
+        rhigh -= 0.5 / (f);
+        if ((rlow - r) * 2 + r > 0.0) LA[i + j * M] += c * 2.0 * rlow / VL;
+        if ((rhigh - r) * 2 + r < off + top) {
+          LA[i + (F - 1 - j) * M] += c * 2.0 * rhigh / VL;
+        } else {
+          const double d = 2 * rhigh - r - top;
+          if (d > -0.001 && d < 0.001) LA[i + (F - 1 - j) * M] += c * 2.0 * rhigh / VL;
+          for (int k = 1; k <= f; k++)
+            if (d > 0.99 * k / f && d < 1.01 * k / f) LA[i + (F - 1 - j) * M] += c * 2.0 * (rhigh - 1.0 * k / f) / VL;
+        }
+      }
+      double LS = 0;
+      for (int j = 0; j < F; j++) LS += LA[i + j * M];
+      LA[i + (F / 2) * M] += 1.0 - LS;
+    }
 }
 
 // Weights of the refined (fjc > 1) 1-gradient lattices: LG1Planar::
@@ -176,6 +238,10 @@ void boundary_indices(LatticeSpec &l) {
 void refined_lambdas(LatticeSpec &l) {
   const int M = l.M, f = l.fjc, F = l.FJC, MX = l.MX;
   const double off = l.offset_first_layer;
+  if (l.gradients == 2) {
+    refined_lambdas2(l);
+    return;
+  }
   l.L.assign(M, 0.0);
   l.lambda_1.assign(M, 0.0);  // the fjc = 1 weights stay zero (unused)
   l.lambda0.assign(M, 0.0);
@@ -302,7 +368,7 @@ void compute_lambdas(LatticeSpec &l, bool markov2) {
 // upperbound / number) for dimension d.
 bool coord(const LatticeSpec &l, const std::string &tok, int d, int &v) {
   // coordinates count bond-length layers: n_layers = MX / fjc
-  int n = d == 0 ? l.MX / l.fjc : (d == 1 ? l.MY : l.MZ);
+  int n = d == 0 ? l.MX / l.fjc : (d == 1 ? l.MY / l.fjc : l.MZ / l.fjc);
   if (tok == "firstlayer") {
     v = 1;
     return true;
@@ -332,7 +398,7 @@ bool parse_range(const LatticeSpec &l, std::string r, std::vector<int> &mask, st
     // build "a;b" where dimension d is fixed to which and others span 1..n
     std::string lo, hi;
     for (int k = 0; k < g; k++) {
-      int n = k == 0 ? l.MX : (k == 1 ? l.MY : l.MZ);
+      int n = (k == 0 ? l.MX : (k == 1 ? l.MY : l.MZ)) / l.fjc;
       std::string a = k == d ? which : "1";
       std::string b = k == d ? which : std::to_string(n);
       lo += (k ? "," : "") + a;
@@ -373,8 +439,10 @@ bool parse_range(const LatticeSpec &l, std::string r, std::vector<int> &mask, st
       return false;
     }
   // a bond-length layer spans fjc grid layers (src/segment.cpp:145-146)
-  lo[0] *= l.fjc;
-  hi[0] = (hi[0] + 1) * l.fjc - 1;
+  for (int d = 0; d < g; d++) {
+    lo[d] *= l.fjc;
+    hi[d] = (hi[d] + 1) * l.fjc - 1;
+  }
   for (int x = lo[0]; x <= hi[0]; x++)
     for (int y = (g > 1 ? lo[1] : 0); y <= (g > 1 ? hi[1] : 0); y++)
       for (int z = (g > 2 ? lo[2] : 0); z <= (g > 2 ? hi[2] : 0); z++) mask[site(l, x, y, z)] = 1;
@@ -476,8 +544,13 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     err = "lattice refinement above FJC_choices 9 is not supported";
     return false;
   }
-  if (l.fjc > 1 && l.gradients != 1) {
-    err = "lattice refinement (FJC_choices > 3) is supported with 1 gradient";
+  // src/lattice.cpp:551-552 (3-gradient refinement is not restated here)
+  if (l.fjc > 3 && l.gradients == 2) {
+    err = "with 2 gradients FJC_choices are limited to 7";
+    return false;
+  }
+  if (l.fjc > 1 && l.gradients == 3) {
+    err = "lattice refinement (FJC_choices > 3) is supported with 1 and 2 gradients";
     return false;
   }
   if (!get(p, "lat", ln, "lattice_type", v)) {
@@ -547,6 +620,10 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     }
   } else {
     if (!need_int("n_layers_x", l.MX) || !need_int("n_layers_y", l.MY)) return false;
+    if (l.gradients == 2) {
+      l.MX *= l.fjc;
+      l.MY *= l.fjc;
+    }
     if (l.gradients == 3 && !need_int("n_layers_z", l.MZ)) return false;
     static const char *lo[3] = {"lowerbound_x", "lowerbound_y", "lowerbound_z"};
     static const char *hi[3] = {"upperbound_x", "upperbound_y", "upperbound_z"};
@@ -600,10 +677,10 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     if (l.geometry == GEOM_CYLINDRICAL)
       l.volume = kPIE * (std::pow(l.MX + l.offset_first_layer, 2) - std::pow(l.offset_first_layer, 2)) / f / f;
   } else if (l.gradients == 2) {
-    l.JX = l.MY + 2;
+    l.JX = l.MY + 2 * l.fjc;
     l.JY = 1;
     l.JZ = 0;
-    l.M = (l.MX + 2) * (l.MY + 2);
+    l.M = (l.MX + 2 * l.fjc) * (l.MY + 2 * l.fjc);
     if (l.geometry == GEOM_CYLINDRICAL)
       l.volume = l.MY * kPIE * (std::pow(l.MX + l.offset_first_layer, 2) - std::pow(l.offset_first_layer, 2));
     else

@@ -54,11 +54,12 @@ struct LatticeSpec {
   std::vector<double> L, lambda_1, lambda0, lambda1;
   // Markov-2 reflection weights for curved 1-gradient lattices.
   std::vector<double> l1, l11, l_1, l_11;
-  // Lattice refinement (fjc > 1, 1 gradient): ghost layers 0..fjc-1 and
+  // Lattice refinement (fjc > 1, 1 or 2 gradients): ghost layers 0..fjc-1 and
   // MX+fjc..MX+2fjc-1 take their values from bx1f[k] / bxmf[k]; LAMBDA holds
   // FJC rows of M weights, row j (j < fjc) toward the site fjc-j below, row
   // FJC-1-j toward the site fjc-j above, row fjc the site itself.
   std::vector<int> bx1f, bxmf;
+  std::vector<int> by1f, bymf;  // the same for y on refined 2-gradient lattices
   std::vector<double> LAMBDA;
 };
 

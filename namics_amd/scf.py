@@ -273,7 +273,8 @@ class ScfSystem:
         if self.gradients == 1:
             return a[self.fjc:mx + self.fjc]
         if self.gradients == 2:
-            return a.reshape(mx + 2, my + 2)[1:mx + 1, 1:my + 1].ravel()
+            f = self.fjc
+            return a.reshape(mx + 2 * f, my + 2 * f)[f:mx + f, f:my + f].ravel()
         return a.reshape(mx + 2, my + 2, mz + 2)[1:mx + 1, 1:my + 1, 1:mz + 1].ravel()
 
 

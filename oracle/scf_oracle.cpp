@@ -97,6 +97,20 @@ struct Lat {
     } else if (l.gradients == 1) {
       X[0] = X[l.BX1];
       X[MX + 1] = X[l.BXM];
+    } else if (l.gradients == 2 && l.fjc > 1) {
+      // src/LGrad2.cpp:1674-1687: y ghosts of the interior x, then x ghosts
+      // of every y (corners included)
+      const int f = l.fjc;
+      for (int x = f; x < MX + f; x++)
+        for (int k = 0; k < f; k++) {
+          X[x * JX + k] = X[x * JX + l.by1f[k]];
+          X[x * JX + MY + f + k] = X[x * JX + l.bymf[k]];
+        }
+      for (int y = 0; y < MY + 2 * f; y++)
+        for (int k = 0; k < f; k++) {
+          X[k * JX + y] = X[l.bx1f[k] * JX + y];
+          X[(MX + f + k) * JX + y] = X[l.bxmf[k] * JX + y];
+        }
     } else if (l.gradients == 2) {
       for (int x = 1; x < MX + 1; x++) {
         X[x * JX + 0] = X[x * JX + l.BY1];
@@ -230,6 +244,18 @@ struct Lat {
     } else if (l.gradients == 1) {
       X[0] = 0;
       X[MX + 1] = 0;
+    } else if (l.gradients == 2 && l.fjc > 1) {
+      const int f = l.fjc;
+      for (int x = f; x < MX + f; x++)
+        for (int k = 0; k < f; k++) {
+          X[x * JX + k] = 0;
+          X[x * JX + MY + f + k] = 0;
+        }
+      for (int y = 0; y < MY + 2 * f; y++)
+        for (int k = 0; k < f; k++) {
+          X[k * JX + y] = 0;
+          X[(MX + f + k) * JX + y] = 0;
+        }
     } else if (l.gradients == 2) {
       for (int x = 1; x < MX + 1; x++) {
         X[x * JX + 0] = 0;
@@ -267,6 +293,7 @@ struct Lat {
     remove_bounds(X);
     const int M = l.M;
     if (l.gradients == 3) return Sum(X, M);
+    if (l.geometry == GEOM_PLANAR && l.gradients == 2) return l.fjc > 1 ? Sum(X, M) / (l.fjc * l.fjc) : Sum(X, M);
     if (l.geometry == GEOM_PLANAR) return l.fjc > 1 ? Sum(X, M) / l.fjc : Sum(X, M);
     return Dot(X, l.L.data(), M);
   }
@@ -350,6 +377,39 @@ struct Lat {
     Norm(gs, l.stencil_full ? 1.0 / 40.0 : (sc ? 1.0 / 152.0 : 1.0 / 56.0), M);
   }
 
+  // Refined 2-gradient stencils (fjc > 1), block sums over the
+  // (2 fjc + 1)^2 offsets.  Cylindrical (LGrad2::propagate / Side
+  // src/LGrad2.cpp:1091-1106, 264-278): offsets with |dy| < fjc, weighted
+  // by the LAMBDA row of dx, times 2, plus those with |dy| = fjc, times
+  // 1/(2(fjc+2)).  Planar (LG2Planar src/LG2Planar.cpp:1110-1124, 117-130):
+  // plain sums of the offsets with no, one, two coordinates at +-fjc, with
+  // factors 2, 2 and 1/(4(FJC-2)FJC+1).
+  void refined2(double *gs, double *gs_1) const {
+    const int M = l.M, JX = l.JX, JY = l.JY, f = l.fjc, F = l.FJC;
+    const double *LA = l.LAMBDA.data();
+    const bool planar = l.geometry == GEOM_PLANAR;
+    const int nblocks = planar ? 3 : 2;
+    for (int block = 0; block < nblocks; block++) {
+      for (int x = -f; x < f + 1; x++)
+        for (int y = -f; y < f + 1; y++) {
+          int bk = 0, a = 0, b = 0;
+          if (planar && (x == -f || x == f)) bk++;
+          if (y == -f || y == f) bk++;
+          if (bk != block) continue;
+          if (x < 0) a = -x * JX; else b = x * JX;
+          if (y < 0) a -= y * JY; else b += y * JY;
+          if (planar)
+            Add(gs + a, gs_1 + b, M - a - b);
+          else
+            AddTimes(gs + a, gs_1 + b, LA + (f + x) * M + a, M - a - b);
+        }
+      if (planar)
+        Norm(gs, block != 2 ? 2.0 : 1.0 / (4.0 * (F - 2) * F + 1), M);
+      else
+        Norm(gs, block != 1 ? 2.0 : 1.0 / (2 * (f + 2)), M);
+    }
+  }
+
   // Markov-1 step: gs <- G1 * W(gs_1); gs_1 gets its ghosts set in place.
   void propagate(double *gs, double *gs_1, const double *G1) const {
     const int M = l.M, JX = l.JX;
@@ -357,6 +417,9 @@ struct Lat {
     set_bounds(gs_1);
     if (l.gradients == 1 && l.fjc > 1) {
       refined_taps(gs, gs_1);
+      Times(gs, gs, G1, M);
+    } else if (l.gradients == 2 && l.fjc > 1) {
+      refined2(gs, gs_1);
       Times(gs, gs, G1, M);
     } else if (l.gradients == 1) {
       if (l.geometry == GEOM_PLANAR) {
@@ -526,6 +589,9 @@ struct Lat {
     Zero(Xs, M);
     if (l.gradients == 1 && l.fjc > 1) {
       refined_taps(Xs, X);
+    } else if (l.gradients == 2 && l.fjc > 1) {
+      if (l.geometry == GEOM_PLANAR) set_bounds(X);
+      refined2(Xs, X);
     } else if (l.gradients == 1) {
       if (l.geometry == GEOM_PLANAR) {
         const double lam = l.lambda;

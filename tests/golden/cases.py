@@ -596,4 +596,96 @@ mol : water : freedom : solvent
 """,
         eval=[(58, 0.3)],
     ),
+    # refined 2-gradient cylindrical brush on a rod (FJC_choices 5,
+    # hexagonal: the refined stencils need it)
+    "fjc5_cyl2d_brush": dict(
+        text="""
+lat : cyl : gradients : 2
+lat : cyl : geometry : cylindrical
+lat : cyl : n_layers_x : 10
+lat : cyl : n_layers_y : 8
+lat : cyl : lattice_type : hexagonal
+lat : cyl : FJC_choices : 5
+mon : S : freedom : frozen
+mon : S : frozen_range : 1,1;1,8
+mon : X : freedom : pinned
+mon : X : pinned_range : 2,1;2,8
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.5
+mon : S : chi_A : -0.3
+mol : brush : composition : (X)1(A)20
+mol : brush : freedom : restricted
+mol : brush : n : 5
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(61, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-9, seed=62, amp=0.05)],
+    ),
+    # refined 2-gradient planar lattice (FJC_choices 5), frozen particle,
+    # periodic y
+    "fjc5_planar2d_particle": dict(
+        text="""
+lat : L : gradients : 2
+lat : L : n_layers_x : 8
+lat : L : n_layers_y : 9
+lat : L : lattice_type : hexagonal
+lat : L : FJC_choices : 5
+lat : L : lowerbound_y : periodic
+lat : L : upperbound_y : periodic
+mon : S : freedom : frozen
+mon : S : frozen_range : 3,3;4,5
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -0.8
+mon : A : chi_W : 0.5
+mol : pol : composition : (A)15
+mol : pol : freedom : free
+mol : pol : phibulk : 0.02
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(63, 0.2)],
+    ),
+    # FJC_choices 7 on a 2-gradient cylindrical lattice (b/l 3)
+    "fjc7_cyl2d": dict(
+        text="""
+lat : cyl : gradients : 2
+lat : cyl : geometry : cylindrical
+lat : cyl : n_layers_x : 6
+lat : cyl : n_layers_y : 5
+lat : cyl : offset_first_layer : 2
+lat : cyl : lattice_type : hexagonal
+lat : cyl : b/l : 3
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.6
+mol : pol : composition : (A)8
+mol : pol : freedom : restricted
+mol : pol : theta : 40
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(64, 0.2)],
+    ),
+    # FJC_choices 7 on a 2-gradient planar lattice, mirror boundaries
+    "fjc7_planar2d": dict(
+        text="""
+lat : L : gradients : 2
+lat : L : n_layers_x : 5
+lat : L : n_layers_y : 6
+lat : L : lattice_type : hexagonal
+lat : L : FJC_choices : 7
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.6
+mol : pol : composition : (A)6
+mol : pol : freedom : free
+mol : pol : phibulk : 0.05
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(65, 0.2)],
+    ),
 }
