@@ -210,8 +210,12 @@ int build(scf_system *S, const Placement *pl) {
     h.bxmf[k] = l.bxmf[k];
     h.by1f[k] = l.by1f[k];
     h.bymf[k] = l.bymf[k];
+    h.bz1f[k] = l.bz1f[k];
+    h.bzmf[k] = l.bzmf[k];
   }
-  h.wplanar = l.gradients == 2 ? 1.0 / (l.fjc * l.fjc) : 1.0 / l.fjc;
+  // planar weighted sums (src/LGrad1.cpp:213-221, src/LGrad2.cpp:147-157,
+  // src/LGrad3.cpp:40-46): divided by fjc, fjc^2, and not at all in 3-D
+  h.wplanar = l.gradients == 1 ? 1.0 / l.fjc : (l.gradients == 2 ? 1.0 / (l.fjc * l.fjc) : 1.0);
   h.lambda = l.lambda;
   h.nseg = nseg;
   h.nit = nit;
@@ -272,7 +276,8 @@ int build(scf_system *S, const Placement *pl) {
   const bool m2curved = !l.l1.empty();
   size_t o_L = curved ? a.take(4 * M * D) : 0;
   size_t o_l1 = m2curved ? a.take(4 * M * D) : 0;
-  size_t o_LAM = l.fjc > 1 ? a.take((size_t)l.FJC * M * D) : 0;
+  const bool has_LAM = l.fjc > 1 && l.LAMBDA.size() == (size_t)l.FJC * M;  // 1 and 2 gradients
+  size_t o_LAM = has_LAM ? a.take((size_t)l.FJC * M * D) : 0;
   size_t o_mask = a.take((size_t)nseg * M * D);
   size_t o_ksam = a.take(M * D);
   size_t o_x = a.take(nvar * D);
@@ -321,7 +326,7 @@ int build(scf_system *S, const Placement *pl) {
     HIP_TRY(hipMemcpyAsync((void *)h.lam_0, l.lambda0.data(), M * D, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync((void *)h.lam_p, l.lambda1.data(), M * D, hipMemcpyHostToDevice, st));
   }
-  if (l.fjc > 1) {
+  if (has_LAM) {
     h.LAM = P(o_LAM);
     HIP_TRY(hipMemcpyAsync((void *)h.LAM, l.LAMBDA.data(), (size_t)l.FJC * M * D, hipMemcpyHostToDevice, st));
   }

@@ -69,7 +69,8 @@ struct DevSys {
   int surface_lo, surface_hi;  // 1-gradient surface boundaries
   int fjc;                     // lattice refinement (grid layers per bond; > 1 only with 1 gradient)
   int bx1f[kMaxFjc], bxmf[kMaxFjc];  // refined ghost sources along x
-  int by1f[kMaxFjc], bymf[kMaxFjc];  // and along y (2 gradients)
+  int by1f[kMaxFjc], bymf[kMaxFjc];  // along y (2 and 3 gradients)
+  int bz1f[kMaxFjc], bzmf[kMaxFjc];  // along z (3 gradients)
   double wplanar;              // site weight of planar sums: 1 / fjc^(gradients)
   const double *LAM;           // refined 1-gradient weights: (2 fjc + 1) rows of M
   double lambda;
@@ -175,6 +176,11 @@ __device__ __forceinline__ int map_y(const Sys &S, int y) {
 }
 template <typename Sys>
 __device__ __forceinline__ int map_z(const Sys &S, int z) {
+  if (S.fjc > 1) {
+    if (z < S.fjc) return S.bz1f[z];
+    if (z >= S.MZ + S.fjc) return S.bzmf[z - S.MZ - S.fjc];
+    return z;
+  }
   return z < 1 ? S.bz1 : (z > S.MZ ? S.bzm : z);
 }
 
@@ -200,11 +206,13 @@ __device__ __forceinline__ int work_site(const Sys &S, int w, int &x, int &y, in
     z = 0;
     return x * S.JX + y;
   }
-  int yz = S.MY * S.MZ;
-  x = w / yz + 1;
-  int r = w - (x - 1) * yz;
-  y = r / S.MZ + 1;
-  z = r - (y - 1) * S.MZ + 1;
+  const int yz = S.MY * S.MZ, f = S.fjc;
+  const int xs = w / yz;
+  const int r = w - xs * yz;
+  const int ys = r / S.MZ;
+  x = xs + f;
+  y = ys + f;
+  z = r - ys * S.MZ + f;
   return x * S.JX + y * S.JY + z;
 }
 
@@ -212,6 +220,8 @@ template <typename Sys>
 __device__ __forceinline__ bool is_interior(const Sys &S, int x, int y, int z) {
   if (S.gradients == 1) return x >= S.fjc && x < S.MX + S.fjc;
   if (S.gradients == 2) return x >= S.fjc && x < S.MX + S.fjc && y >= S.fjc && y < S.MY + S.fjc;
+  if (S.fjc > 1)
+    return x >= S.fjc && x < S.MX + S.fjc && y >= S.fjc && y < S.MY + S.fjc && z >= S.fjc && z < S.MZ + S.fjc;
   if (x < 1 || x > S.MX) return false;
   if (S.gradients >= 2 && (y < 1 || y > S.MY)) return false;
   if (S.gradients == 3 && (z < 1 || z > S.MZ)) return false;
@@ -341,6 +351,31 @@ __device__ __forceinline__ T refined2(const Sys &S, Ptr g, int i, int x, int y) 
   return v;
 }
 
+// Refined 3-gradient hexagonal stencil (fjc = 2; LGrad3::propagate / Side
+// src/LGrad3.cpp:808-830, 206-226): plain sums of the (2 fjc + 1)^3 offsets
+// with none, one, two, three coordinates at +-fjc, times 2, 2, 2 and
+// 1/8/((FJC-2)^3 + 3 (FJC-2)^2 + 3 (FJC-2) + 1), in the reference's order.
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ T refined3(const Sys &S, Ptr g, int x, int y, int z) {
+  const int f = S.fjc, F = 2 * S.fjc + 1, JX = S.JX, JY = S.JY;
+  T v = 0;
+  for (int block = 0; block < 4; block++) {
+    for (int dx = -f; dx <= f; dx++) {
+      const int sx = map_x(S, x + dx) * JX;
+      for (int dy = -f; dy <= f; dy++) {
+        const int sy = sx + map_y(S, y + dy) * JY;
+        for (int dz = -f; dz <= f; dz++) {
+          const int bk = (dx == -f || dx == f) + (dy == -f || dy == f) + (dz == -f || dz == f);
+          if (bk == block) v += g[sy + map_z(S, z + dz)];
+        }
+      }
+    }
+    v *= block != 3 ? (T)2.0
+                    : (T)(1.0 / 8.0 / ((F - 2) * (F - 2) * (F - 2) + 3 * (F - 2) * (F - 2) + 3 * (F - 2) + 1));
+  }
+  return v;
+}
+
 template <typename T, typename Sys, typename Ptr>
 __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int y, int z) {
   if (S.gradients == 1) {
@@ -417,6 +452,7 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
     return v * C2;
   }
   const int JX = S.JX, JY = S.JY;
+  if (S.fjc > 1) return refined3<T>(S, g, x, y, z);
   if (S.stencil_full && S.lattice_type == 0) {
     const int bx = x * JX, by = y * JY;
     T v = g[map_x(S, x - 1) * JX + by + z];

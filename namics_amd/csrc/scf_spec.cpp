@@ -171,6 +171,12 @@ void boundary_indices(LatticeSpec &l) {
     l.by1f[k] = l.bc[1] == BC_MIRROR ? 2 * f - 1 - k : (l.bc[1] == BC_PERIODIC ? l.MY + k : k);
     l.bymf[k] = l.bc[4] == BC_MIRROR ? l.MY + f - k - 1 : (l.bc[4] == BC_PERIODIC ? f + k : l.MY + f + k);
   }
+  l.bz1f.assign(f, 0);
+  l.bzmf.assign(f, 0);
+  for (int k = 0; k < f; k++) {
+    l.bz1f[k] = l.bc[2] == BC_MIRROR ? 2 * f - 1 - k : (l.bc[2] == BC_PERIODIC ? l.MZ + k : k);
+    l.bzmf[k] = l.bc[5] == BC_MIRROR ? l.MZ + f - k - 1 : (l.bc[5] == BC_PERIODIC ? f + k : l.MZ + f + k);
+  }
 }
 
 // Weights of the refined (fjc > 1) 2-gradient lattices (LGrad2::
@@ -549,8 +555,8 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     err = "with 2 gradients FJC_choices are limited to 7";
     return false;
   }
-  if (l.fjc > 1 && l.gradients == 3) {
-    err = "lattice refinement (FJC_choices > 3) is supported with 1 and 2 gradients";
+  if (l.fjc > 2 && l.gradients == 3) {
+    err = "with 3 gradients FJC_choices are limited to 5";
     return false;
   }
   if (!get(p, "lat", ln, "lattice_type", v)) {
@@ -620,11 +626,10 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     }
   } else {
     if (!need_int("n_layers_x", l.MX) || !need_int("n_layers_y", l.MY)) return false;
-    if (l.gradients == 2) {
-      l.MX *= l.fjc;
-      l.MY *= l.fjc;
-    }
     if (l.gradients == 3 && !need_int("n_layers_z", l.MZ)) return false;
+    l.MX *= l.fjc;
+    l.MY *= l.fjc;
+    if (l.gradients == 3) l.MZ *= l.fjc;
     static const char *lo[3] = {"lowerbound_x", "lowerbound_y", "lowerbound_z"};
     static const char *hi[3] = {"upperbound_x", "upperbound_y", "upperbound_z"};
     for (int d = 0; d < l.gradients; d++) {
@@ -663,6 +668,10 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     bool b;
     if (to_bool(v, b)) l.stencil_full = b;
   }
+  if (l.fjc > 1 && l.gradients == 3 && !l.stencil_full) {
+    err = "refined 3-gradient lattices need stencil_full";
+    return false;
+  }
   // PutM (src/LGrad1.cpp:180-190, src/LGrad2.cpp:101-111, src/LGrad3.cpp:15-23)
   if (l.gradients == 1) {
     l.JX = 1;
@@ -686,10 +695,11 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     else
       l.volume = (double)l.MX * l.MY;
   } else {
-    l.JX = (l.MZ + 2) * (l.MY + 2);
-    l.JY = l.MZ + 2;
+    const int f = l.fjc;
+    l.JX = (l.MZ + 2 * f) * (l.MY + 2 * f);
+    l.JY = l.MZ + 2 * f;
     l.JZ = 1;
-    l.M = (l.MX + 2) * (l.MY + 2) * (l.MZ + 2);
+    l.M = (l.MX + 2 * f) * (l.MY + 2 * f) * (l.MZ + 2 * f);
     l.volume = (double)l.MX * l.MY * l.MZ;
   }
   boundary_indices(l);

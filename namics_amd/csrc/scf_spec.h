@@ -59,7 +59,8 @@ struct LatticeSpec {
   // FJC rows of M weights, row j (j < fjc) toward the site fjc-j below, row
   // FJC-1-j toward the site fjc-j above, row fjc the site itself.
   std::vector<int> bx1f, bxmf;
-  std::vector<int> by1f, bymf;  // the same for y on refined 2-gradient lattices
+  std::vector<int> by1f, bymf;  // the same for y (2 and 3 gradients)
+  std::vector<int> bz1f, bzmf;  // and for z (3 gradients)
   std::vector<double> LAMBDA;
 };
 

@@ -124,6 +124,64 @@ struct Lat {
       X[MY + 1] = X[l.BYM];
       X[(MX + 1) * JX + 0] = X[(MX + 1) * JX + l.BY1];
       X[(MX + 1) * JX + MY + 1] = X[(MX + 1) * JX + l.BYM];
+    } else if (l.fjc > 1) {
+      // src/LGrad3.cpp:1618-1672: faces of the interior, then the edges of the
+      // x-ghost planes and of the y-ghost rows, then the corners
+      const int f = l.fjc;
+      auto P3 = [&](int x, int y, int z) { return x * JX + y * JY + z; };
+      for (int x = f; x < MX + f; x++)
+        for (int y = f; y < MY + f; y++)
+          for (int k = 0; k < f; k++) {
+            X[P3(x, y, k)] = X[P3(x, y, l.bz1f[k])];
+            X[P3(x, y, MZ + f + k)] = X[P3(x, y, l.bzmf[k])];
+          }
+      for (int y = f; y < MY + f; y++)
+        for (int z = f; z < MZ + f; z++)
+          for (int k = 0; k < f; k++) {
+            X[P3(k, y, z)] = X[P3(l.bx1f[k], y, z)];
+            X[P3(MX + f + k, y, z)] = X[P3(l.bxmf[k], y, z)];
+          }
+      for (int z = f; z < MZ + f; z++)
+        for (int x = f; x < MX + f; x++)
+          for (int k = 0; k < f; k++) {
+            X[P3(x, k, z)] = X[P3(x, l.by1f[k], z)];
+            X[P3(x, MY + f + k, z)] = X[P3(x, l.bymf[k], z)];
+          }
+      auto xplane = [&](int x) {
+        for (int y = f; y < MY + f; y++)
+          for (int k = 0; k < f; k++) {
+            X[P3(x, y, k)] = X[P3(x, y, l.bz1f[k])];
+            X[P3(x, y, MZ + f + k)] = X[P3(x, y, l.bzmf[k])];
+          }
+        for (int z = f; z < MZ + f; z++)
+          for (int k = 0; k < f; k++) {
+            X[P3(x, k, z)] = X[P3(x, l.by1f[k], z)];
+            X[P3(x, MY + f + k, z)] = X[P3(x, l.bymf[k], z)];
+          }
+      };
+      for (int x = 0; x < f; x++) xplane(x);
+      for (int x = MX + f; x < MX + 2 * f; x++) xplane(x);
+      auto yrow = [&](int y) {
+        for (int x = f; x < MX + f; x++)
+          for (int k = 0; k < f; k++) {
+            X[P3(x, y, k)] = X[P3(x, y, l.bz1f[k])];
+            X[P3(x, y, MZ + f + k)] = X[P3(x, y, l.bzmf[k])];
+          }
+      };
+      for (int y = 0; y < f; y++) yrow(y);
+      for (int y = MY + f; y < MY + 2 * f; y++) yrow(y);
+      for (int k = 0; k < f; k++)
+        for (int q = 0; q < f; q++)
+          for (int m = 0; m < f; m++) {
+            X[P3(k, q, m)] = X[P3(l.bx1f[k], l.by1f[q], l.bz1f[m])];
+            X[P3(MX + f + k, q, m)] = X[P3(l.bxmf[k], l.by1f[q], l.bz1f[m])];
+            X[P3(k, MY + f + q, m)] = X[P3(l.bx1f[k], l.bymf[q], l.bz1f[m])];
+            X[P3(k, q, MZ + f + m)] = X[P3(l.bx1f[k], l.by1f[q], l.bzmf[m])];
+            X[P3(MX + f + k, MY + f + q, m)] = X[P3(l.bxmf[k], l.bymf[q], l.bz1f[m])];
+            X[P3(MX + f + k, q, MZ + f + m)] = X[P3(l.bxmf[k], l.by1f[q], l.bzmf[m])];
+            X[P3(k, MY + f + q, MZ + f + m)] = X[P3(l.bx1f[k], l.bymf[q], l.bzmf[m])];
+            X[P3(MX + f + k, MY + f + q, MZ + f + m)] = X[P3(l.bxmf[k], l.bymf[q], l.bzmf[m])];
+          }
     } else {
       const int JZ = 1;
       const int BX1 = l.BX1, BXM = l.BXM, BY1 = l.BY1, BYM = l.BYM, BZ1 = l.BZ1, BZM = l.BZM;
@@ -269,6 +327,26 @@ struct Lat {
       X[MY + 1] = 0;
       X[(MX + 1) * JX + 0] = 0;
       X[(MX + 1) * JX + MY + 1] = 0;
+    } else if (l.fjc > 1) {
+      const int f = l.fjc;
+      for (int x = 0; x < MX + 2 * f; x++)
+        for (int y = 0; y < MY + 2 * f; y++)
+          for (int k = 0; k < f; k++) {
+            X[x * JX + y * JY + k] = 0;
+            X[x * JX + y * JY + MZ + f + k] = 0;
+          }
+      for (int y = 0; y < MY + 2 * f; y++)
+        for (int z = 0; z < MZ + 2 * f; z++)
+          for (int k = 0; k < f; k++) {
+            X[k * JX + y * JY + z] = 0;
+            X[(MX + f + k) * JX + y * JY + z] = 0;
+          }
+      for (int z = 0; z < MZ + 2 * f; z++)
+        for (int x = 0; x < MX + 2 * f; x++)
+          for (int k = 0; k < f; k++) {
+            X[x * JX + k * JY + z] = 0;
+            X[x * JX + (MY + f + k) * JY + z] = 0;
+          }
     } else {
       // RemoveBoundaries (src/tools_host.h:305-310): zero every ghost face
       for (int y = 0; y < MY + 2; y++)
@@ -410,6 +488,31 @@ struct Lat {
     }
   }
 
+  // Refined 3-gradient hexagonal stencil (fjc = 2; LGrad3::propagate / Side
+  // src/LGrad3.cpp:808-830, 206-226): plain sums of the (2 fjc + 1)^3
+  // offsets with none, one, two, three coordinates at +-fjc, with factors 2,
+  // 2, 2 and 1/8/((FJC-2)^3 + 3(FJC-2)^2 + 3(FJC-2) + 1).
+  void refined3(double *gs, double *gs_1) const {
+    const int M = l.M, JX = l.JX, JY = l.JY, f = l.fjc, F = l.FJC;
+    for (int block = 0; block < 4; block++) {
+      for (int x = -f; x < f + 1; x++)
+        for (int y = -f; y < f + 1; y++)
+          for (int z = -f; z < f + 1; z++) {
+            int bk = 0, a = 0, b = 0;
+            if (x == -f || x == f) bk++;
+            if (y == -f || y == f) bk++;
+            if (z == -f || z == f) bk++;
+            if (bk != block) continue;
+            if (x < 0) a = -x * JX; else b = x * JX;
+            if (y < 0) a -= y * JY; else b += y * JY;
+            if (z < 0) a -= z; else b += z;
+            Add(gs + a, gs_1 + b, M - a - b);
+          }
+      Norm(gs, block != 3 ? 2.0 : 1.0 / 8.0 / ((F - 2) * (F - 2) * (F - 2) + 3 * (F - 2) * (F - 2) + 3 * (F - 2) + 1),
+           M);
+    }
+  }
+
   // Markov-1 step: gs <- G1 * W(gs_1); gs_1 gets its ghosts set in place.
   void propagate(double *gs, double *gs_1, const double *G1) const {
     const int M = l.M, JX = l.JX;
@@ -492,6 +595,11 @@ struct Lat {
       Times(gs, gs, G1, M);
     } else {
       const int JY = l.JY;
+      if (l.fjc > 1) {
+        refined3(gs, gs_1);
+        Times(gs, gs, G1, M);
+        return;
+      }
       if (!(l.stencil_full && l.lattice_type == LAT_SIMPLE_CUBIC)) {
         wide3(gs, gs_1);
         Times(gs, gs, G1, M);
@@ -658,6 +766,8 @@ struct Lat {
         AddTimes(Xs, X + JX + 1, lp, M - JX - 1);
         Norm(Xs, C2, M);
       }
+    } else if (l.fjc > 1) {
+      refined3(Xs, X);
     } else if (!(l.stencil_full && l.lattice_type == LAT_SIMPLE_CUBIC)) {
       wide3(Xs, X);
     } else {

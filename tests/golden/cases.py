@@ -688,4 +688,58 @@ mol : water : freedom : solvent
 """,
         eval=[(65, 0.2)],
     ),
+    # refined 3-gradient hexagonal lattice (FJC_choices 5): a frozen wall
+    # layer with an adsorbing polymer, periodic in y and z.  Evaluated at zero
+    # fields and solved from zero fields (the reference misreads guess files
+    # of refined 3-gradient lattices with several field blocks)
+    "fjc5_wall3d": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 6
+lat : box : n_layers_y : 3
+lat : box : n_layers_z : 3
+lat : box : lattice_type : hexagonal
+lat : box : FJC_choices : 5
+lat : box : lowerbound_y : periodic
+lat : box : upperbound_y : periodic
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : S : freedom : frozen
+mon : S : frozen_range : 1,1,1;1,3,3
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -1.0
+mon : A : chi_W : 0.5
+mol : pol : composition : (A)10
+mol : pol : freedom : free
+mol : pol : phibulk : 0.02
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(66, 0.0)],
+        converged=[dict(method="DIIS", tol=1e-9)],
+    ),
+    # refined 3-gradient hexagonal box with a frozen cube, mirror boundaries
+    "fjc5_particle3d": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 5
+lat : box : n_layers_y : 4
+lat : box : n_layers_z : 4
+lat : box : lattice_type : hexagonal
+lat : box : FJC_choices : 5
+mon : S : freedom : frozen
+mon : S : frozen_range : 1,1,1;2,2,2
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -0.6
+mon : A : chi_W : 0.4
+mol : pol : composition : (A)8
+mol : pol : freedom : free
+mol : pol : phibulk : 0.03
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(68, 0.0)],
+    ),
 }

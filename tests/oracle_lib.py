@@ -149,7 +149,8 @@ class Oracle:
         if self.gradients == 2:
             f = self.fjc
             return a.reshape(mx + 2 * f, my + 2 * f)[f:mx + f, f:my + f].ravel()
-        return a.reshape(mx + 2, my + 2, mz + 2)[1:mx + 1, 1:my + 1, 1:mz + 1].ravel()
+        f = self.fjc
+        return a.reshape(mx + 2 * f, my + 2 * f, mz + 2 * f)[f:mx + f, f:my + f, f:mz + f].ravel()
 
 
 def solve_ones(A):

@@ -370,3 +370,31 @@ def test_3d_wide_stencil_symbols(lt, full):
         ev = (w0 + 2 * wf * (cx + cy + cz) + 4 * we * (cx * cy + cx * cz + cy * cz) + 8 * wc * cx * cy * cz) / nrm
         for side in (False, True):
             np.testing.assert_allclose(o.apply(v, side=side)[idx], ev * v[idx], atol=1e-13)
+
+
+def test_refined_2d_3d_block_sums():
+    """Refined 2- and 3-gradient stencils (FJC 5, hexagonal) on mirror
+    lattices: the 3-D block sum (weights 8/4/2/1 over 512) and the 2-D
+    cylindrical one (2 LAMBDA / 2(fjc+2) with |dy| < fjc, LAMBDA / 2(fjc+2) at
+    |dy| = fjc) keep a constant at every interior site; the 2-D planar block
+    sum is normalised by 16 fjc^2 - 3 instead of 16 fjc^2 in the reference,
+    so a constant grows by 64/61 there (fjc = 2)."""
+    base = """
+mon : A : freedom : free
+mon : W : freedom : free
+mol : pol : composition : (A)5
+mol : pol : freedom : free
+mol : pol : phibulk : 0.1
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+"""
+    o3 = Oracle("lat : b : gradients : 3\nlat : b : n_layers_x : 3\nlat : b : n_layers_y : 4\n"
+                "lat : b : n_layers_z : 3\nlat : b : lattice_type : hexagonal\nlat : b : FJC_choices : 5\n" + base)
+    w = o3.interior(o3.apply(np.ones(o3.M)))
+    np.testing.assert_allclose(w, 1.0, atol=1e-14)
+    oc = Oracle("lat : c : gradients : 2\nlat : c : geometry : cylindrical\nlat : c : n_layers_x : 5\n"
+                "lat : c : n_layers_y : 4\nlat : c : lattice_type : hexagonal\nlat : c : FJC_choices : 5\n" + base)
+    np.testing.assert_allclose(oc.interior(oc.apply(np.ones(oc.M))), 1.0, atol=1e-14)
+    op = Oracle("lat : p : gradients : 2\nlat : p : n_layers_x : 5\nlat : p : n_layers_y : 4\n"
+                "lat : p : lattice_type : hexagonal\nlat : p : FJC_choices : 5\n" + base)
+    np.testing.assert_allclose(op.interior(op.apply(np.ones(op.M))), 64.0 / 61.0, atol=1e-14)
