@@ -96,7 +96,7 @@ int validate(const SystemSpec &s, std::string &why) {
   if (l.gradients == 2 && l.geometry == GEOM_PLANAR && !l.stencil_full && l.lattice_type != LAT_SIMPLE_CUBIC)
     return why = "2-gradient planar hexagonal stencil without stencil_full is not supported", SCF_E_UNSUPPORTED;
   for (auto &m : s.mol)
-    if (m.markov == 2 && l.gradients != 1) return why = "Markov 2 needs 1 gradient", SCF_E_UNSUPPORTED;
+    if (m.markov == 2 && l.gradients == 3) return why = "Markov 2 needs 1 or 2 gradients", SCF_E_UNSUPPORTED;
   return SCF_OK;
 }
 

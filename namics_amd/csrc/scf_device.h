@@ -632,6 +632,113 @@ __device__ __forceinline__ void m2h_step(const Sys &S, Ptr g, const double *P, b
   }
 }
 
+// Bounded value of direction component c (0: +x, 1: +y, 2: out of plane,
+// 3: -y, 4: -x) of a 2-gradient Markov-2 row at (x, y): at a reflecting x
+// boundary components 0 and 4 swap, at a reflecting y boundary 1 and 3, the
+// rest (and everything at a periodic boundary) takes its own value at the
+// source site (LGrad2::set_bounds_x/y src/LGrad2.cpp:1555-1650).  Ghost
+// corners are never read by the axis-aligned Markov-2 steps.
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ T m2d_val(const Sys &S, Ptr row, int c, int x, int y) {
+  const int M = S.M, JX = S.JX;
+  if (x < 1 || x > S.MX) {
+    const int b = x < 1 ? S.bx1 : S.bxm;
+    const int cc = (S.bx1 <= S.bxm && (c == 0 || c == 4)) ? 4 - c : c;
+    return row[cc * M + b * JX + y];
+  }
+  if (y < 1 || y > S.MY) {
+    const int b = y < 1 ? S.by1 : S.bym;
+    const int cc = (S.by1 <= S.bym && (c == 1 || c == 3)) ? 4 - c : c;
+    return row[cc * M + x * JX + b];
+  }
+  return row[c * M + x * JX + y];
+}
+
+// Forward or backward 2-gradient Markov-2 step at interior site (x, y),
+// simple cubic (LGrad2::propagateF/B src/LGrad2.cpp:645-681,1003-1039 with
+// the curved reflection weights; LG2Planar::propagateF/B
+// src/LG2Planar.cpp:445-481,930-966 without them).  Straight continuation
+// has weight P0, a turn P1, the out-of-plane component counts both
+// perpendicular directions (2 P1).  Writes the five components without the
+// Boltzmann factor, each summed in the reference's pass order.
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ void m2d_step(const Sys &S, Ptr g, const double *P, bool forward, int i, int x, int y,
+                                         T out[5]) {
+  const T P0 = (T)P[0], P1 = (T)P[1], P2 = (T)(2 * P[1]);
+  const bool cyl = S.geometry != 0;
+  auto z = [&](int c, int xx, int yy) { return m2d_val<T>(S, g, c, xx, yy); };
+  if (forward) {
+    T hl = z(0, x - 1, y), hu = z(4, x + 1, y);
+    if (cyl) {
+      hl = (T)hbm(S.l_1)[i] * z(0, x - 1, y);
+      hl += (T)hbm(S.l_11)[i] * z(4, x, y);
+      hu = (T)hbm(S.l1)[i] * z(4, x + 1, y);
+      hu += (T)hbm(S.l11)[i] * z(0, x, y);
+    }
+    T o = P0 * hl;
+    o += P1 * z(1, x - 1, y);
+    o += P2 * z(2, x - 1, y);
+    o += P1 * z(3, x - 1, y);
+    out[0] = o;
+    o = P1 * z(0, x, y - 1);
+    o += P0 * z(1, x, y - 1);
+    o += P2 * z(2, x, y - 1);
+    o += P1 * z(4, x, y - 1);
+    out[1] = o;
+    o = P1 * z(0, x, y);
+    o += P1 * z(1, x, y);
+    o += P0 * z(2, x, y);
+    o += P1 * z(3, x, y);
+    o += P1 * z(4, x, y);
+    out[2] = o;
+    o = P1 * z(0, x, y + 1);
+    o += P2 * z(2, x, y + 1);
+    o += P0 * z(3, x, y + 1);
+    o += P1 * z(4, x, y + 1);
+    out[3] = o;
+    o = P1 * z(1, x + 1, y);
+    o += P2 * z(2, x + 1, y);
+    o += P1 * z(3, x + 1, y);
+    o += P0 * hu;
+    out[4] = o;
+  } else {
+    T hl = z(4, x - 1, y), hu = z(0, x + 1, y);
+    if (cyl) {
+      hl = (T)hbm(S.l_1)[i] * z(4, x - 1, y);
+      hl += (T)hbm(S.l_11)[i] * z(0, x, y);
+      hu = (T)hbm(S.l1)[i] * z(0, x + 1, y);
+      hu += (T)hbm(S.l11)[i] * z(4, x, y);
+    }
+    const T z3m = z(3, x, y - 1), z2 = z(2, x, y), z1p = z(1, x, y + 1);
+    T o = P1 * z3m;
+    o += P2 * z2;
+    o += P1 * z1p;
+    o += P0 * hu;
+    out[0] = o;
+    o = P1 * hl;
+    o += P2 * z2;
+    o += P0 * z1p;
+    o += P1 * hu;
+    out[1] = o;
+    o = P1 * hl;
+    o += P1 * z3m;
+    o += P0 * z2;
+    o += P1 * z1p;
+    o += P1 * hu;
+    out[2] = o;
+    o = P1 * hl;
+    o += P0 * z3m;
+    o += P2 * z2;
+    o += P1 * hu;
+    out[3] = o;
+    o = P0 * hl;
+    o += P1 * z3m;
+    o += P2 * z2;
+    o += P1 * z1p;
+    out[4] = o;
+  }
+}
+
 // ----------------------------------------------------- per-site phases
 // Boltzmann weights of every segment at site i (Segment::PrepareForCalculations
 // src/segment.cpp:835-879): u is the segment's field block with its ghosts

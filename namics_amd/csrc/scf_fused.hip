@@ -710,7 +710,7 @@ template <bool kLds>
 __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
   typedef typename Rows<kLds>::T RowT;
   CSys &S = uniform_sys(S0);
-  const int M = S.M, W = S.M;
+  const int M = S.M, W = n_work_sites(S);
   const int tid = work_tid(S), nt = S.threads;
   const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
   const int N = S.mol_N[mo];
@@ -719,6 +719,7 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
   const int C = S.mol_size[mo];
   const int R = C * M;
   const bool hex = S.lattice_type != 0;
+  const bool two = S.gradients == 2;  // 5 components: +x, +y, out of plane, -y, -x
   double P[kMaxComp];
 #pragma unroll
   for (int k = 0; k < kMaxComp; k++) P[k] = S.mol_P[mo][k];
@@ -727,6 +728,9 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
   const int F = C;
   const double wend = hex ? (S.fjc == 1 ? 1.0 / 4.0 : 0.5 / (F - 1.0)) : 1.0 / 6.0;
   const double wmid = hex ? (S.fjc == 1 ? 2.0 / 4.0 : 1.0 / (F - 1.0)) : 4.0 / 6.0;
+  // composition weight of component c (LGrad2::AddPhiS src/LGrad2.cpp:1849-1882:
+  // 1/3 for the out-of-plane component, 1/6 for the others)
+  auto wcomp = [&](int c) { return two ? (c == 2 ? 1.0 / 3.0 : 1.0 / 6.0) : (c == 0 || c == C - 1 ? wend : wmid); };
   gdbl *Gf = hbm(S.Gf);
   const gdbl *G1 = hbm(S.G1);
   RowT *ra, *rb;
@@ -739,7 +743,9 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
   }
   {
     const gdbl *g1 = G1 + (size_t)S.block_seg[b0] * M;
-    for (int i = tid; i < W; i += nt) {
+    for (int w = tid; w < W; w += nt) {
+      int x, y, z;
+      const int i = work_site(S, w, x, y, z);
       const double v = g1[i];
       for (int c = 0; c < C; c++) {
         Gf[c * M + i] = v;
@@ -755,17 +761,21 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
     const gdbl *g1 = G1 + (size_t)S.block_seg[b] * M;
     for (int j = (b == b0 ? 1 : 0); j < S.block_n[b]; j++, s++) {
       gdbl *gs = Gf + (size_t)s * R;
-      for (int i = tid; i < W; i += nt) {
+      for (int w = tid; w < W; w += nt) {
+        int x, y, z;
+        const int i = work_site(S, w, x, y, z);
         double o[kMaxComp];
-        if (hex)
+        if (two)
+          m2d_step<double>(S, prev, P, true, i, x, y, o);
+        else if (hex)
           m2h_step<double>(S, prev, P, true, i, o);
         else
           m2_step<double>(S, prev, P, true, i, o);
-        const double w = g1[i];
+        const double wg = g1[i];
 #pragma unroll
         for (int c = 0; c < kMaxComp; c++) {
           if (c >= C) break;
-          const double v = o[c] * w;
+          const double v = o[c] * wg;
           gs[c * M + i] = v;
           if constexpr (kLds) next[c * M + i] = v;
         }
@@ -787,8 +797,10 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
     double a[kMaxComp];
 #pragma unroll
     for (int c = 0; c < kMaxComp; c++) a[c] = 0;
-    for (int i = tid; i < W; i += nt) {
-      if (is_interior(S, i, 0, 0)) {
+    for (int w = tid; w < W; w += nt) {
+      int x, y, z;
+      const int i = work_site(S, w, x, y, z);
+      if (is_interior(S, x, y, z)) {
         const double wt = site_weight(S0, i);
 #pragma unroll
         for (int c = 0; c < kMaxComp; c++)
@@ -796,13 +808,21 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
       }
     }
     double gn = wg_sum(a[0]);
-    const double inner = hex ? 2.0 : 4.0;
+    if (two) {
+      // LGrad2::ComputeGN (src/LGrad2.cpp:1814-1840): (G0 + G1 + 2 G2 + G3 + G4) / 6
+      gn += wg_sum(a[1]);
+      gn += 2 * wg_sum(a[2]);
+      gn += wg_sum(a[3]);
+      gn += wg_sum(a[4]);
+    } else {
+      const double inner = hex ? 2.0 : 4.0;
 #pragma unroll
-    for (int c = 1; c < kMaxComp - 1; c++)
-      if (c < C - 1) gn += inner * wg_sum(a[c]);
+      for (int c = 1; c < kMaxComp - 1; c++)
+        if (c < C - 1) gn += inner * wg_sum(a[c]);
 #pragma unroll
-    for (int c = 2; c < kMaxComp; c++)
-      if (c == C - 1) gn += wg_sum(a[c]);
+      for (int c = 2; c < kMaxComp; c++)
+        if (c == C - 1) gn += wg_sum(a[c]);
+    }
     if (threadIdx.x == 0) S0.sc->GN[mo] = hex ? gn / (4.0 * S.fjc) : gn / 6.0;
   }
   RowT *gbA, *gbB;
@@ -818,14 +838,16 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
     const gdbl *g1 = G1 + (size_t)S.block_seg[bl] * M;
     gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[bl] * M;
     const gdbl *gf = Gf + (size_t)(N - 1) * R;
-    for (int i = tid; i < W; i += nt) {
+    for (int w = tid; w < W; w += nt) {
+      int x, y, z;
+      const int i = work_site(S, w, x, y, z);
       const double v = g1[i];
       double r = rho[i];
 #pragma unroll
       for (int c = 0; c < kMaxComp; c++) {
         if (c >= C) break;
         gbA[c * M + i] = v;
-        r += (c == 0 || c == C - 1 ? wend : wmid) * gf[c * M + i] * v;
+        r += wcomp(c) * gf[c * M + i] * v;
       }
       rho[i] = r;
     }
@@ -837,20 +859,24 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
     gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[b] * M;
     for (int j = (b == b1 - 1 ? S.block_n[b] - 2 : S.block_n[b] - 1); j >= 0; j--, s--) {
       const gdbl *gf = Gf + (size_t)s * R;
-      for (int i = tid; i < W; i += nt) {
+      for (int w = tid; w < W; w += nt) {
+        int x, y, z;
+        const int i = work_site(S, w, x, y, z);
         double o[kMaxComp];
-        if (hex)
+        if (two)
+          m2d_step<double>(S, gbA, P, false, i, x, y, o);
+        else if (hex)
           m2h_step<double>(S, gbA, P, false, i, o);
         else
           m2_step<double>(S, gbA, P, false, i, o);
-        const double w = g1[i];
+        const double wg = g1[i];
         double r = rho[i];
 #pragma unroll
         for (int c = 0; c < kMaxComp; c++) {
           if (c >= C) break;
-          const double v = o[c] * w;
+          const double v = o[c] * wg;
           gbB[c * M + i] = v;
-          r += (c == 0 || c == C - 1 ? wend : wmid) * gf[c * M + i] * v;
+          r += wcomp(c) * gf[c * M + i] * v;
         }
         rho[i] = r;
       }

@@ -363,6 +363,19 @@ void compute_lambdas(LatticeSpec &l, bool markov2) {
           l.lambda_1[p] = 2.0 * kPIE * (r - 1) / l.L[p] * lam;
           l.lambda0[p] = 1.0 - 2.0 * lam;
         }
+      if (markov2) {
+        // Markov-2 reflection weights at every site (src/LGrad2.cpp:42-47)
+        l.l1.assign(M, 0.0);
+        l.l11.assign(M, 0.0);
+        l.l_1.assign(M, 0.0);
+        l.l_11.assign(M, 0.0);
+        for (int i = 0; i < M; i++) {
+          l.l1[i] = l.lambda1[i] / lam;
+          l.l11[i] = 1.0 - l.l1[i];
+          l.l_1[i] = l.lambda_1[i] / lam;
+          l.l_11[i] = 1.0 - l.l_1[i];
+        }
+      }
     } else {
       // planar 2-D: L=1 on every site (LG2Planar::ComputeLambdas src/LG2Planar.cpp:15-20)
       for (int i = 0; i < M; i++) l.L[i] = 1.0;
@@ -898,13 +911,19 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     if (m.markov == 2) {
       const bool sc1 = l.lattice_type == LAT_SIMPLE_CUBIC && l.fjc == 1;
       const bool hexp = l.lattice_type == LAT_HEXAGONAL && l.geometry == GEOM_PLANAR;
-      if (l.gradients != 1 || !(sc1 || hexp)) {
-        err = "mol '" + m.name + "': Markov 2 is supported on 1-gradient simple-cubic lattices without refinement "
-              "and on planar 1-gradient hexagonal lattices";
+      const bool ok1 = l.gradients == 1 && (sc1 || hexp);
+      const bool ok2 = l.gradients == 2 && sc1;
+      if (!(ok1 || ok2)) {
+        err = "mol '" + m.name + "': Markov 2 is supported on 1-gradient simple-cubic lattices without refinement, "
+              "planar 1-gradient hexagonal lattices and 2-gradient simple-cubic lattices without refinement";
         return false;
       }
       any_markov2 = true;
-      m.size = l.FJC;
+      // direction components (Molecule::CheckInput src/molecule.cpp:648-663):
+      // FJC in 1 gradient, 2 FJC - 1 in 2 gradients, where Markov 2 also
+      // turns stencil_full off for the whole lattice
+      m.size = l.gradients == 1 ? l.FJC : 2 * l.FJC - 1;
+      if (l.gradients == 2) l.stencil_full = false;
       // Bending weights (Molecule::AllocateMemory src/molecule.cpp:133-153)
       m.P.assign(l.FJC, 0.0);
       double Q = 0;

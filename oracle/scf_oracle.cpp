@@ -618,7 +618,156 @@ struct Lat {
 
   // Markov-2 forward/backward steps on a 1-gradient simple-cubic lattice.
   // gs, gs_1 hold the three direction components back to back (stride M).
+  // 2-gradient Markov-2 bounds (LGrad2::set_bounds_x/y src/LGrad2.cpp:1555-1650,
+  // fjc = 1, no shift): a pair of opposite directions swaps at a reflecting
+  // boundary, every other component is bounded by itself; periodic x or y
+  // bounds each component by itself.  Ghost corners are not touched.
+  void sbx(double *X) const {
+    const int MX = l.MX, MY = l.MY, JX = l.JX;
+    for (int y = 1; y < MY + 1; y++) {
+      X[0 + y] = X[l.BX1 * JX + y];
+      X[(MX + 1) * JX + y] = X[l.BXM * JX + y];
+    }
+  }
+  void sbx(double *X, double *Y) const {
+    const int MX = l.MX, MY = l.MY, JX = l.JX;
+    if (l.BX1 > l.BXM) {
+      sbx(X);
+      sbx(Y);
+      return;
+    }
+    for (int y = 1; y < MY + 1; y++) {
+      X[0 + y] = Y[l.BX1 * JX + y];
+      X[(MX + 1) * JX + y] = Y[l.BXM * JX + y];
+      Y[0 + y] = X[l.BX1 * JX + y];
+      Y[(MX + 1) * JX + y] = X[l.BXM * JX + y];
+    }
+  }
+  void sby(double *X) const {
+    const int MX = l.MX, MY = l.MY, JX = l.JX;
+    for (int x = 1; x < MX + 1; x++) {
+      X[x * JX + 0] = X[x * JX + l.BY1];
+      X[x * JX + MY + 1] = X[x * JX + l.BYM];
+    }
+  }
+  void sby(double *X, double *Y) const {
+    const int MX = l.MX, MY = l.MY, JX = l.JX;
+    if (l.BY1 > l.BYM) {
+      sby(X);
+      sby(Y);
+      return;
+    }
+    for (int x = 1; x < MX + 1; x++) {
+      X[x * JX + 0] = Y[x * JX + l.BY1];
+      X[x * JX + MY + 1] = Y[x * JX + l.BYM];
+      Y[x * JX + 0] = X[x * JX + l.BY1];
+      Y[x * JX + MY + 1] = X[x * JX + l.BYM];
+    }
+  }
+
+  // 2-gradient Markov-2 step, simple cubic, 5 direction components (+x, +y,
+  // out of plane, -y, -x): LGrad2::propagateF/B (src/LGrad2.cpp:645-681,
+  // 1003-1039, with the reflection weights l_1, l_11, l1, l11 of the curved
+  // lattice) and LG2Planar::propagateF/B (src/LG2Planar.cpp:445-481,
+  // 930-966).
+  void propagate2d(double *gs, double *gs_1, const double *G1, const double *P, bool forward) const {
+    const int M = l.M, JX = l.JX, JY = l.JY;
+    double *z0 = gs_1, *z1 = gs_1 + M, *z2 = gs_1 + 2 * M, *z3 = gs_1 + 3 * M, *z4 = gs_1 + 4 * M;
+    sbx(z0, z4);
+    sbx(z1);
+    sbx(z2);
+    sbx(z3);
+    sby(z1, z3);
+    sby(z0);
+    sby(z2);
+    sby(z4);
+    double *x0 = gs, *x1 = gs + M, *x2 = gs + 2 * M, *x3 = gs + 3 * M, *x4 = gs + 4 * M;
+    Zero(gs, 5 * M);
+    const bool cyl = l.geometry != GEOM_PLANAR;
+    vec Hv(M, 0.0);
+    double *H = Hv.data();
+    const double *l1 = cyl ? l.l1.data() : nullptr, *l11 = cyl ? l.l11.data() : nullptr;
+    const double *l_1 = cyl ? l.l_1.data() : nullptr, *l_11 = cyl ? l.l_11.data() : nullptr;
+    // LReflect(H, P, Q): H = l_1(+JX) P + l_11(+JX) Q(+JX); UReflect(H, P, Q):
+    // H(+JX) = l1 P(+JX) + l11 Q
+    auto lref = [&](const double *Pp, const double *Qq) {
+      Times(H, l_1 + JX, Pp, M - JX);
+      AddTimes(H, l_11 + JX, Qq + JX, M - JX);
+    };
+    auto uref = [&](const double *Pp, const double *Qq) {
+      Times(H + JX, l1, Pp + JX, M - JX);
+      AddTimes(H + JX, l11, Qq, M - JX);
+    };
+    if (forward) {
+      if (cyl) {
+        lref(z0, z4);
+        YplusisCtimesX(x0 + JX, H, P[0], M - JX);
+      } else {
+        YplusisCtimesX(x0 + JX, z0, P[0], M - JX);
+      }
+      YplusisCtimesX(x0 + JX, z1, P[1], M - JX);
+      YplusisCtimesX(x0 + JX, z2, 2 * P[1], M - JX);
+      YplusisCtimesX(x0 + JX, z3, P[1], M - JX);
+      YplusisCtimesX(x1 + JY, z0, P[1], M - JY);
+      YplusisCtimesX(x1 + JY, z1, P[0], M - JY);
+      YplusisCtimesX(x1 + JY, z2, 2 * P[1], M - JY);
+      YplusisCtimesX(x1 + JY, z4, P[1], M - JY);
+      YplusisCtimesX(x2, z0, P[1], M);
+      YplusisCtimesX(x2, z1, P[1], M);
+      YplusisCtimesX(x2, z2, P[0], M);
+      YplusisCtimesX(x2, z3, P[1], M);
+      YplusisCtimesX(x2, z4, P[1], M);
+      YplusisCtimesX(x3, z0 + JY, P[1], M - JY);
+      YplusisCtimesX(x3, z2 + JY, 2 * P[1], M - JY);
+      YplusisCtimesX(x3, z3 + JY, P[0], M - JY);
+      YplusisCtimesX(x3, z4 + JY, P[1], M - JY);
+      YplusisCtimesX(x4, z1 + JX, P[1], M - JX);
+      YplusisCtimesX(x4, z2 + JX, 2 * P[1], M - JX);
+      YplusisCtimesX(x4, z3 + JX, P[1], M - JX);
+      if (cyl) {
+        uref(z4, z0);
+        YplusisCtimesX(x4, H + JX, P[0], M - JX);
+      } else {
+        YplusisCtimesX(x4, z4 + JX, P[0], M - JX);
+      }
+    } else {
+      const double *h4 = z4;
+      if (cyl) {
+        lref(z4, z0);
+        h4 = H;
+      }
+      YplusisCtimesX(x1 + JX, h4, P[1], M - JX);
+      YplusisCtimesX(x2 + JX, h4, P[1], M - JX);
+      YplusisCtimesX(x3 + JX, h4, P[1], M - JX);
+      YplusisCtimesX(x4 + JX, h4, P[0], M - JX);
+      YplusisCtimesX(x0 + JY, z3, P[1], M - JY);
+      YplusisCtimesX(x2 + JY, z3, P[1], M - JY);
+      YplusisCtimesX(x3 + JY, z3, P[0], M - JY);
+      YplusisCtimesX(x4 + JY, z3, P[1], M - JY);
+      YplusisCtimesX(x0, z2, 2 * P[1], M);
+      YplusisCtimesX(x1, z2, 2 * P[1], M);
+      YplusisCtimesX(x2, z2, P[0], M);
+      YplusisCtimesX(x3, z2, 2 * P[1], M);
+      YplusisCtimesX(x4, z2, 2 * P[1], M);
+      YplusisCtimesX(x0, z1 + JY, P[1], M - JY);
+      YplusisCtimesX(x1, z1 + JY, P[0], M - JY);
+      YplusisCtimesX(x2, z1 + JY, P[1], M - JY);
+      YplusisCtimesX(x4, z1 + JY, P[1], M - JY);
+      const double *h0 = z0 + JX;
+      if (cyl) {
+        uref(z0, z4);
+        h0 = H + JX;
+      }
+      YplusisCtimesX(x0, h0, P[0], M - JX);
+      YplusisCtimesX(x1, h0, P[1], M - JX);
+      YplusisCtimesX(x2, h0, P[1], M - JX);
+      YplusisCtimesX(x3, h0, P[1], M - JX);
+    }
+    for (int k = 0; k < 5; k++) Times(gs + k * M, gs + k * M, G1, M);
+  }
+
   void propagateF(double *gs, double *gs_1, const double *G1, const double *P) const {
+    if (l.gradients == 2) return propagate2d(gs, gs_1, G1, P, true);
     if (l.lattice_type == LAT_HEXAGONAL) return propagate_hex(gs, gs_1, G1, P, true);
     const int M = l.M;
     Zero(gs, 3 * M);
@@ -658,6 +807,7 @@ struct Lat {
   }
 
   void propagateB(double *gs, double *gs_1, const double *G1, const double *P) const {
+    if (l.gradients == 2) return propagate2d(gs, gs_1, G1, P, false);
     if (l.lattice_type == LAT_HEXAGONAL) return propagate_hex(gs, gs_1, G1, P, false);
     const int M = l.M;
     Zero(gs, 3 * M);
@@ -1101,7 +1251,12 @@ struct Oracle {
     }
     // GN (LGrad1::ComputeGN src/LGrad1.cpp:943-955 and analogues)
     double *Glast = Gf[N - 1].data();
-    if (m.markov == 2 && s.lat.lattice_type == LAT_HEXAGONAL) {
+    if (m.markov == 2 && s.lat.gradients == 2) {
+      // LGrad2::ComputeGN (src/LGrad2.cpp:1814-1840): (G0 + G1 + 2 G2 + G3 + G4) / 6
+      double g = 0;
+      for (int k = 0; k < 5; k++) g += (k == 2 ? 2 * lat.weighted_sum(Glast + k * M) : lat.weighted_sum(Glast + k * M));
+      GN[mi] = g / 6.0;
+    } else if (m.markov == 2 && s.lat.lattice_type == LAT_HEXAGONAL) {
       const int F = s.lat.FJC;
       double g = lat.weighted_sum(Glast);
       for (int k = 1; k < F - 1; k++) g += 2.0 * lat.weighted_sum(Glast + k * M);
@@ -1128,7 +1283,11 @@ struct Oracle {
       }
       double *r = rho.data() + mm_of[k] * M;
       const double *gf = Gf[k].data();
-      if (m.markov == 2 && s.lat.lattice_type == LAT_HEXAGONAL) {
+      if (m.markov == 2 && s.lat.gradients == 2) {
+        // LGrad2::AddPhiS (src/LGrad2.cpp:1849-1882)
+        for (int k = 0; k < 5; k++)
+          YplusisCtimesAtimesB(r, gf + k * M, gb + k * M, k == 2 ? 1.0 / 3.0 : 1.0 / 6.0, M);
+      } else if (m.markov == 2 && s.lat.lattice_type == LAT_HEXAGONAL) {
         // LGrad1::AddPhiS (src/LGrad1.cpp:876-898)
         const int F = s.lat.FJC;
         const double C1 = s.lat.fjc == 1 ? 1.0 / 4.0 : 0.5 / (F - 1.0);

@@ -742,4 +742,60 @@ mol : water : freedom : solvent
 """,
         eval=[(68, 0.0)],
     ),
+    # second-order Markov on a 2-gradient cylindrical lattice (5 direction
+    # components with the curved reflection weights): stiff brush on a rod
+    "markov2_cyl2d_brush": dict(
+        text="""
+lat : cyl : gradients : 2
+lat : cyl : geometry : cylindrical
+lat : cyl : n_layers_x : 10
+lat : cyl : n_layers_y : 12
+lat : cyl : lattice_type : simple_cubic
+mon : S : freedom : frozen
+mon : S : frozen_range : 1,1;1,12
+mon : X : freedom : pinned
+mon : X : pinned_range : 2,1;2,12
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.5
+mon : S : chi_A : -0.3
+mol : brush : composition : (X)1(A)20
+mol : brush : freedom : restricted
+mol : brush : n : 6
+mol : brush : Markov : 2
+mol : brush : k_stiff : 1
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(71, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-9, seed=72, amp=0.05)],
+    ),
+    # second-order Markov on a 2-gradient planar lattice next to a frozen
+    # particle, periodic y, mixed with a first-order solvent (stencil_full
+    # goes off for the whole lattice)
+    "markov2_planar2d": dict(
+        text="""
+lat : L : gradients : 2
+lat : L : n_layers_x : 9
+lat : L : n_layers_y : 8
+lat : L : lattice_type : simple_cubic
+lat : L : lowerbound_y : periodic
+lat : L : upperbound_y : periodic
+mon : S : freedom : frozen
+mon : S : frozen_range : 4,3;5,5
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -0.9
+mon : A : chi_W : 0.5
+mol : pol : composition : (A)15
+mol : pol : freedom : free
+mol : pol : phibulk : 0.02
+mol : pol : Markov : 2
+mol : pol : k_stiff : 2
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(73, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-9, seed=74, amp=0.05)],
+    ),
 }
