@@ -95,8 +95,6 @@ int validate(const SystemSpec &s, std::string &why) {
       if (l.bc[d] == BC_SURFACE) return why = "surface boundaries are supported with 1 gradient only", SCF_E_UNSUPPORTED;
   if (l.gradients == 2 && l.geometry == GEOM_PLANAR && !l.stencil_full && l.lattice_type != LAT_SIMPLE_CUBIC)
     return why = "2-gradient planar hexagonal stencil without stencil_full is not supported", SCF_E_UNSUPPORTED;
-  for (auto &m : s.mol)
-    if (m.markov == 2 && l.gradients == 3) return why = "Markov 2 needs 1 or 2 gradients", SCF_E_UNSUPPORTED;
   return SCF_OK;
 }
 
@@ -528,6 +526,9 @@ size_t scf_system_device_bytes(const scf_system *S) { return S ? S->arena_bytes 
 
 int scf_system_set_mode(scf_system *S, int mode) {
   if (!S || mode < SCF_MODE_AUTO || mode > SCF_MODE_GRID) return fail(SCF_E_ARG, "bad mode");
+  if (mode == SCF_MODE_GRID)
+    for (auto &m : S->spec.mol)
+      if (m.markov == 2) return fail(SCF_E_UNSUPPORTED, "second-order chains run on the fused path only");
   S->mode = mode;
   return SCF_OK;
 }

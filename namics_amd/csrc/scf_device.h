@@ -739,6 +739,64 @@ __device__ __forceinline__ void m2d_step(const Sys &S, Ptr g, const double *P, b
   }
 }
 
+// Bounded value of direction component c (0: +x, 1: +y, 2: +z, 3: -z,
+// 4: -y, 5: -x) of a 3-gradient Markov-2 row at (x, y, z): at a reflecting
+// boundary of an axis that axis' pair of directions swaps, other components
+// and periodic boundaries take their own values (LGrad3::set_bounds_x/y/z
+// src/LGrad3.cpp:1376-1522).  Only face ghosts are read.
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ T m3_val(const Sys &S, Ptr row, int c, int x, int y, int z) {
+  const int M = S.M, JX = S.JX, JY = S.JY;
+  if (x < 1 || x > S.MX) {
+    const int b = x < 1 ? S.bx1 : S.bxm;
+    const int cc = (S.bx1 <= S.bxm && (c == 0 || c == 5)) ? 5 - c : c;
+    return row[cc * M + b * JX + y * JY + z];
+  }
+  if (y < 1 || y > S.MY) {
+    const int b = y < 1 ? S.by1 : S.bym;
+    const int cc = (S.by1 <= S.bym && (c == 1 || c == 4)) ? 5 - c : c;
+    return row[cc * M + x * JX + b * JY + z];
+  }
+  if (z < 1 || z > S.MZ) {
+    const int b = z < 1 ? S.bz1 : S.bzm;
+    const int cc = (S.bz1 <= S.bzm && (c == 2 || c == 3)) ? 5 - c : c;
+    return row[cc * M + x * JX + y * JY + b];
+  }
+  return row[c * M + x * JX + y * JY + z];
+}
+
+// Forward or backward 3-gradient Markov-2 step at interior site (x, y, z),
+// simple cubic (LGrad3::propagateF/B src/LGrad3.cpp:399-446, 605-663): a
+// walker never reverses, straight on weighs P0 and a turn P1.  Forward,
+// component p arrives from the site behind it and sums the incoming
+// components q != 5 - p in increasing q; backward, the components are
+// summed in decreasing q from the site ahead of each incoming direction.
+template <typename T, typename Sys, typename Ptr>
+__device__ __forceinline__ void m3_step(const Sys &S, Ptr g, const double *P, bool forward, int x, int y, int z,
+                                        T out[6]) {
+  const T P0 = (T)P[0], P1 = (T)P[1];
+  // displacement of direction d
+  const int dx[6] = {1, 0, 0, 0, 0, -1}, dy[6] = {0, 1, 0, 0, -1, 0}, dz[6] = {0, 0, 1, -1, 0, 0};
+#pragma unroll
+  for (int p = 0; p < 6; p++) {
+    T o = 0;
+    if (forward) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        if (q == 5 - p) continue;
+        o += (q == p ? P0 : P1) * m3_val<T>(S, g, q, x - dx[p], y - dy[p], z - dz[p]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 5; q >= 0; q--) {
+        if (p == 5 - q) continue;
+        o += (q == p ? P0 : P1) * m3_val<T>(S, g, q, x + dx[q], y + dy[q], z + dz[q]);
+      }
+    }
+    out[p] = o;
+  }
+}
+
 // ----------------------------------------------------- per-site phases
 // Boltzmann weights of every segment at site i (Segment::PrepareForCalculations
 // src/segment.cpp:835-879): u is the segment's field block with its ghosts

@@ -719,7 +719,8 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
   const int C = S.mol_size[mo];
   const int R = C * M;
   const bool hex = S.lattice_type != 0;
-  const bool two = S.gradients == 2;  // 5 components: +x, +y, out of plane, -y, -x
+  const bool two = S.gradients == 2;    // 5 components: +x, +y, out of plane, -y, -x
+  const bool three = S.gradients == 3;  // 6 components: +x, +y, +z, -z, -y, -x
   double P[kMaxComp];
 #pragma unroll
   for (int k = 0; k < kMaxComp; k++) P[k] = S.mol_P[mo][k];
@@ -730,7 +731,9 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
   const double wmid = hex ? (S.fjc == 1 ? 2.0 / 4.0 : 1.0 / (F - 1.0)) : 4.0 / 6.0;
   // composition weight of component c (LGrad2::AddPhiS src/LGrad2.cpp:1849-1882:
   // 1/3 for the out-of-plane component, 1/6 for the others)
-  auto wcomp = [&](int c) { return two ? (c == 2 ? 1.0 / 3.0 : 1.0 / 6.0) : (c == 0 || c == C - 1 ? wend : wmid); };
+  auto wcomp = [&](int c) {
+    return three ? 1.0 / 6.0 : (two ? (c == 2 ? 1.0 / 3.0 : 1.0 / 6.0) : (c == 0 || c == C - 1 ? wend : wmid));
+  };
   gdbl *Gf = hbm(S.Gf);
   const gdbl *G1 = hbm(S.G1);
   RowT *ra, *rb;
@@ -765,7 +768,9 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
         int x, y, z;
         const int i = work_site(S, w, x, y, z);
         double o[kMaxComp];
-        if (two)
+        if (three)
+          m3_step<double>(S, prev, P, true, x, y, z, o);
+        else if (two)
           m2d_step<double>(S, prev, P, true, i, x, y, o);
         else if (hex)
           m2h_step<double>(S, prev, P, true, i, o);
@@ -808,7 +813,12 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
       }
     }
     double gn = wg_sum(a[0]);
-    if (two) {
+    if (three) {
+      // LGrad3::ComputeGN (src/LGrad3.cpp:1883-1895): the mean of the components
+#pragma unroll
+      for (int c = 1; c < 6; c++) gn += wg_sum(a[c]);
+      gn = gn / (1.0 * 6);
+    } else if (two) {
       // LGrad2::ComputeGN (src/LGrad2.cpp:1814-1840): (G0 + G1 + 2 G2 + G3 + G4) / 6
       gn += wg_sum(a[1]);
       gn += 2 * wg_sum(a[2]);
@@ -823,7 +833,7 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
       for (int c = 2; c < kMaxComp; c++)
         if (c == C - 1) gn += wg_sum(a[c]);
     }
-    if (threadIdx.x == 0) S0.sc->GN[mo] = hex ? gn / (4.0 * S.fjc) : gn / 6.0;
+    if (threadIdx.x == 0) S0.sc->GN[mo] = three ? gn : (hex ? gn / (4.0 * S.fjc) : gn / 6.0);
   }
   RowT *gbA, *gbB;
   if constexpr (kLds) {
@@ -863,7 +873,9 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
         int x, y, z;
         const int i = work_site(S, w, x, y, z);
         double o[kMaxComp];
-        if (two)
+        if (three)
+          m3_step<double>(S, gbA, P, false, x, y, z, o);
+        else if (two)
           m2d_step<double>(S, gbA, P, false, i, x, y, o);
         else if (hex)
           m2h_step<double>(S, gbA, P, false, i, o);

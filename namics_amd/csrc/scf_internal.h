@@ -34,7 +34,9 @@ struct scf_system {
   int effective_mode() const {
     if (mode != SCF_MODE_AUTO) return mode;
     if (tuned_mode != SCF_MODE_AUTO) return tuned_mode;
-    return spec.lat.gradients == 3 ? SCF_MODE_GRID : SCF_MODE_FUSED;
+    bool markov2 = false;  // second-order chains run on the fused path only
+    for (auto &m : spec.mol) markov2 = markov2 || m.markov == 2;
+    return spec.lat.gradients == 3 && !markov2 ? SCF_MODE_GRID : SCF_MODE_FUSED;
   }
 };
 

@@ -912,18 +912,18 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       const bool sc1 = l.lattice_type == LAT_SIMPLE_CUBIC && l.fjc == 1;
       const bool hexp = l.lattice_type == LAT_HEXAGONAL && l.geometry == GEOM_PLANAR;
       const bool ok1 = l.gradients == 1 && (sc1 || hexp);
-      const bool ok2 = l.gradients == 2 && sc1;
-      if (!(ok1 || ok2)) {
-        err = "mol '" + m.name + "': Markov 2 is supported on 1-gradient simple-cubic lattices without refinement, "
-              "planar 1-gradient hexagonal lattices and 2-gradient simple-cubic lattices without refinement";
+      const bool ok23 = l.gradients >= 2 && sc1;
+      if (!(ok1 || ok23)) {
+        err = "mol '" + m.name + "': Markov 2 is supported on simple-cubic lattices without refinement and on "
+              "planar 1-gradient hexagonal lattices";
         return false;
       }
       any_markov2 = true;
-      // direction components (Molecule::CheckInput src/molecule.cpp:648-663):
-      // FJC in 1 gradient, 2 FJC - 1 in 2 gradients, where Markov 2 also
-      // turns stencil_full off for the whole lattice
-      m.size = l.gradients == 1 ? l.FJC : 2 * l.FJC - 1;
-      if (l.gradients == 2) l.stencil_full = false;
+      // direction components (Molecule::CheckInput src/molecule.cpp:648-670):
+      // FJC in 1 gradient, 2 FJC - 1 in 2 and 6 in 3 gradients, where Markov 2
+      // also turns stencil_full off for the whole lattice
+      m.size = l.gradients == 1 ? l.FJC : (l.gradients == 2 ? 2 * l.FJC - 1 : 6);
+      if (l.gradients >= 2) l.stencil_full = false;
       // Bending weights (Molecule::AllocateMemory src/molecule.cpp:133-153)
       m.P.assign(l.FJC, 0.0);
       double Q = 0;

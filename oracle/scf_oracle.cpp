@@ -766,7 +766,88 @@ struct Lat {
     for (int k = 0; k < 5; k++) Times(gs + k * M, gs + k * M, G1, M);
   }
 
+  // 3-gradient Markov-2 face bounds (LGrad3::set_bounds_x/y/z
+  // src/LGrad3.cpp:1376-1522, fjc = 1, no shift): axis d's pair of opposite
+  // directions swaps at a reflecting boundary of that axis; periodic bounds
+  // and the other components take their own values.  Only face ghosts.
+  void sb3(double *X, double *Y, int axis) const {
+    const int MX = l.MX, MY = l.MY, MZ = l.MZ, JX = l.JX, JY = l.JY;
+    const int B1 = axis == 0 ? l.BX1 : (axis == 1 ? l.BY1 : l.BZ1);
+    const int BM = axis == 0 ? l.BXM : (axis == 1 ? l.BYM : l.BZM);
+    const int n = axis == 0 ? MX : (axis == 1 ? MY : MZ);
+    const int J = axis == 0 ? JX : (axis == 1 ? JY : 1);
+    const bool pair = Y != nullptr && B1 <= BM;
+    // the two other axes span their interior
+    const int na = axis == 0 ? MY : MX, nb = axis == 2 ? MY : MZ;
+    const int Ja = axis == 0 ? JY : JX, Jb = axis == 2 ? JY : 1;
+    for (int a = 1; a < na + 1; a++)
+      for (int b = 1; b < nb + 1; b++) {
+        const int base = a * Ja + b * Jb;
+        if (pair) {
+          X[base] = Y[base + B1 * J];
+          X[base + (n + 1) * J] = Y[base + BM * J];
+          Y[base] = X[base + B1 * J];
+          Y[base + (n + 1) * J] = X[base + BM * J];
+        } else {
+          X[base] = X[base + B1 * J];
+          X[base + (n + 1) * J] = X[base + BM * J];
+          if (Y) {
+            Y[base] = Y[base + B1 * J];
+            Y[base + (n + 1) * J] = Y[base + BM * J];
+          }
+        }
+      }
+  }
+
+  // 3-gradient Markov-2 step, simple cubic, 6 direction components (+x, +y,
+  // +z, -z, -y, -x): LGrad3::propagateF/B (src/LGrad3.cpp:399-446, 605-663);
+  // a walker never reverses, straight on weighs P0, a turn P1.
+  void propagate3d(double *gs, double *gs_1, const double *G1, const double *P, bool forward) const {
+    const int M = l.M, JX = l.JX, JY = l.JY, JZ = 1;
+    double *z[6];
+    double *o[6];
+    for (int k = 0; k < 6; k++) {
+      z[k] = gs_1 + k * M;
+      o[k] = gs + k * M;
+    }
+    Zero(gs, 6 * M);
+    if (forward) {
+      sb3(z[0], z[5], 0);
+      for (int k : {1, 2, 3, 4}) sb3(z[k], nullptr, 0);
+      sb3(z[1], z[4], 1);
+      for (int k : {0, 2, 3, 5}) sb3(z[k], nullptr, 1);
+      sb3(z[2], z[3], 2);
+      for (int k : {0, 1, 4, 5}) sb3(z[k], nullptr, 2);
+      // component p arrives from the site behind it and takes every incoming
+      // direction except its reverse 5 - p
+      const int sh[6] = {JX, JY, JZ, -JZ, -JY, -JX};
+      for (int p = 0; p < 6; p++) {
+        const int a = sh[p] > 0 ? sh[p] : 0, b = sh[p] < 0 ? -sh[p] : 0;
+        for (int q = 0; q < 6; q++) {
+          if (q == 5 - p) continue;
+          YplusisCtimesX(o[p] + a, z[q] + b, q == p ? P[0] : P[1], M - a - b);
+        }
+      }
+    } else {
+      sb3(z[0], z[5], 0);
+      sb3(z[1], z[4], 1);
+      sb3(z[2], z[3], 2);
+      // incoming direction q (reverse order 5..0) feeds every component p
+      // except its reverse
+      const int sh[6] = {JX, JY, JZ, -JZ, -JY, -JX};
+      for (int q = 5; q >= 0; q--) {
+        const int a = sh[5 - q] > 0 ? sh[5 - q] : 0, b = sh[5 - q] < 0 ? -sh[5 - q] : 0;
+        for (int p = 0; p < 6; p++) {
+          if (p == 5 - q) continue;
+          YplusisCtimesX(o[p] + a, z[q] + b, q == p ? P[0] : P[1], M - a - b);
+        }
+      }
+    }
+    for (int k = 0; k < 6; k++) Times(gs + k * M, gs + k * M, G1, M);
+  }
+
   void propagateF(double *gs, double *gs_1, const double *G1, const double *P) const {
+    if (l.gradients == 3) return propagate3d(gs, gs_1, G1, P, true);
     if (l.gradients == 2) return propagate2d(gs, gs_1, G1, P, true);
     if (l.lattice_type == LAT_HEXAGONAL) return propagate_hex(gs, gs_1, G1, P, true);
     const int M = l.M;
@@ -807,6 +888,7 @@ struct Lat {
   }
 
   void propagateB(double *gs, double *gs_1, const double *G1, const double *P) const {
+    if (l.gradients == 3) return propagate3d(gs, gs_1, G1, P, false);
     if (l.gradients == 2) return propagate2d(gs, gs_1, G1, P, false);
     if (l.lattice_type == LAT_HEXAGONAL) return propagate_hex(gs, gs_1, G1, P, false);
     const int M = l.M;
@@ -1251,7 +1333,12 @@ struct Oracle {
     }
     // GN (LGrad1::ComputeGN src/LGrad1.cpp:943-955 and analogues)
     double *Glast = Gf[N - 1].data();
-    if (m.markov == 2 && s.lat.gradients == 2) {
+    if (m.markov == 2 && s.lat.gradients == 3) {
+      // LGrad3::ComputeGN (src/LGrad3.cpp:1883-1895): the mean of the 6 components
+      double g = 0;
+      for (int k = 0; k < 6; k++) g += lat.weighted_sum(Glast + k * M);
+      GN[mi] = g / (1.0 * 6);
+    } else if (m.markov == 2 && s.lat.gradients == 2) {
       // LGrad2::ComputeGN (src/LGrad2.cpp:1814-1840): (G0 + G1 + 2 G2 + G3 + G4) / 6
       double g = 0;
       for (int k = 0; k < 5; k++) g += (k == 2 ? 2 * lat.weighted_sum(Glast + k * M) : lat.weighted_sum(Glast + k * M));
@@ -1283,7 +1370,10 @@ struct Oracle {
       }
       double *r = rho.data() + mm_of[k] * M;
       const double *gf = Gf[k].data();
-      if (m.markov == 2 && s.lat.gradients == 2) {
+      if (m.markov == 2 && s.lat.gradients == 3) {
+        // LGrad3::AddPhiS (src/LGrad3.cpp:1897-1905)
+        for (int k = 0; k < 6; k++) YplusisCtimesAtimesB(r, gf + k * M, gb + k * M, 1.0 / 6.0, M);
+      } else if (m.markov == 2 && s.lat.gradients == 2) {
         // LGrad2::AddPhiS (src/LGrad2.cpp:1849-1882)
         for (int k = 0; k < 5; k++)
           YplusisCtimesAtimesB(r, gf + k * M, gb + k * M, k == 2 ? 1.0 / 3.0 : 1.0 / 6.0, M);

@@ -798,4 +798,35 @@ mol : water : freedom : solvent
         eval=[(73, 0.2)],
         converged=[dict(method="DIIS", tol=1e-9, seed=74, amp=0.05)],
     ),
+    # second-order Markov on a 3-gradient simple-cubic lattice (6 direction
+    # components; stencil_full goes off for the whole lattice): polymer near
+    # a frozen cube, mirror in x, periodic in y and z
+    "markov2_box3d": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 7
+lat : box : n_layers_y : 6
+lat : box : n_layers_z : 5
+lat : box : lattice_type : simple_cubic
+lat : box : lowerbound_y : periodic
+lat : box : upperbound_y : periodic
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : S : freedom : frozen
+mon : S : frozen_range : 1,1,1;2,3,5
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -0.8
+mon : A : chi_W : 0.5
+mol : pol : composition : (A)12
+mol : pol : freedom : free
+mol : pol : phibulk : 0.03
+mol : pol : Markov : 2
+mol : pol : k_stiff : 1.5
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(75, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-9, seed=76, amp=0.05)],
+    ),
 }

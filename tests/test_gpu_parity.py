@@ -26,10 +26,10 @@ TOL_ORC = 1e-12
 
 def modes_for(gradients, markov2):
     from namics_amd import scf
-    if gradients == 3:
-        return [scf.MODE_GRID, scf.MODE_FUSED]
     if markov2:
         return [scf.MODE_FUSED]
+    if gradients == 3:
+        return [scf.MODE_GRID, scf.MODE_FUSED]
     return [scf.MODE_FUSED, scf.MODE_GRID]
 
 
