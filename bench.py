@@ -245,6 +245,7 @@ def main():
     ap.add_argument("--slots", type=int, default=0, help="resident job slots (0: one per CU)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision rerun")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch layout (no autotuning)")
+    ap.add_argument("--no-graph", action="store_true", help="box: launch the grid-path kernels one by one")
     ap.add_argument("--switch", type=float, default=1e-4, help="mixed precision: FP64 below this residual")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: requests per second")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals")
@@ -260,6 +261,9 @@ def main():
     elif args.workload == "serve":
         from namics_amd.bench_serve import run_serve
         out = run_serve(args)
+    elif args.workload == "box":
+        from namics_amd.bench_box import run_box
+        out = run_box(args)
     else:
         from namics_amd.bench_batch import run_batch
         out = run_batch(args)

@@ -131,6 +131,12 @@ int scf_system_set_precision(scf_system *sys, int mode, double switch_residual);
  * which changes no output. */
 int scf_system_set_checkpoint(scf_system *sys, int interval);
 
+/* Grid-path iteration replay: on (default), the first grid-path DIIS
+ * iteration of a system is captured into a hipGraph and replayed; off, every
+ * iteration launches its kernels one by one (same kernels, same results).
+ * No counterpart in the reference. */
+int scf_system_set_graph(scf_system *sys, int on);
+
 /* Launch-layout autotuner.  Times `evals` residual evaluations (at the
  * system's current device fields) for every layout that applies to the
  * lattice -- on 2-gradient lattices the register-strip widths SX = 8 and 4

@@ -545,6 +545,17 @@ int scf_system_set_precision(scf_system *S, int mode, double switch_residual) {
   return upload_solver(S);
 }
 
+int scf_system_set_graph(scf_system *S, int on) {
+  if (!S) return fail(SCF_E_ARG, "null system");
+  S->use_graph = on != 0;
+  if (!S->use_graph && S->graph_exec) {
+    HIP_TRY(hipStreamSynchronize(S->stream));
+    hipGraphExecDestroy(S->graph_exec);
+    S->graph_exec = nullptr;
+  }
+  return SCF_OK;
+}
+
 int scf_system_set_checkpoint(scf_system *S, int interval) {
   if (!S) return fail(SCF_E_ARG, "null system");
   if (interval < 0) return fail(SCF_E_ARG, "negative checkpoint interval");

@@ -1126,4 +1126,41 @@ __device__ __host__ inline int diis_gram_update(DevScalars &sc, int m, int k_dii
   return kd;
 }
 
+// Workgroup-parallel form of diis_gram_update for device code; every thread
+// of the workgroup calls it.  Thread t owns Gram entries t, t + blockDim.x:
+// it reads the entry's new value (the shifted one, or the new row/column from
+// dots) before a barrier and stores it after, which equals the in-place
+// row-major shift above (each source entry lies after its target and is read
+// before it is overwritten).  dots must be visible to the workgroup on entry;
+// sc.A and Ap are on return.  Returns kd.
+__device__ inline int diis_gram_update_block(DevScalars &sc, int m, int k_diis, const double *dots, double *Ap) {
+  constexpr int kPer = (kMaxDIIS * kMaxDIIS + 63) / 64;
+  const int kd = k_diis > m ? m : k_diis;
+  const bool shift = k_diis > m;
+  double v[kPer];
+#pragma unroll
+  for (int r = 0; r < kPer; r++) {
+    const int t = threadIdx.x + r * blockDim.x;
+    v[r] = 0.0;
+    if (t < m * m) {
+      const int i = t / m, j = t - (t / m) * m;
+      v[r] = sc.A[(shift && i < m - 1 && j < m - 1) ? t + m + 1 : t];
+      if (i == kd - 1 && j < kd) v[r] = dots[j] + 1e-9;
+      if (j == kd - 1 && i < kd) v[r] = dots[i] + 1e-9;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kPer; r++) {
+    const int t = threadIdx.x + r * blockDim.x;
+    if (t < m * m) {
+      const int i = t / m, j = t - (t / m) * m;
+      sc.A[t] = v[r];
+      if (i < kd && j < kd) Ap[j + kd * i] = v[r];
+    }
+  }
+  __syncthreads();
+  return kd;
+}
+
 }  // namespace nscf

@@ -1299,8 +1299,7 @@ __device__ bool fused_solve(const DevSys &S0, bool fresh, unsigned long long bud
     k_diis++;
     const int kd = k_diis > m ? m : k_diis;
     diis_update_dots(S0, k, kd, diis_dots);
-    if (threadIdx.x == 0) diis_gram_update(sc, m, k_diis, diis_dots, diis_A);
-    __syncthreads();
+    diis_gram_update_block(sc, m, k_diis, diis_dots, diis_A);
     if (threadIdx.x < 64) {
       const bool ok = wave_diis_coefficients(diis_A, kd, sc.C);
       if (threadIdx.x == 0 && !ok) sc.status = DS_NAN;

@@ -55,6 +55,27 @@ __device__ __forceinline__ double ordered_max(const double *partial, int nparts,
   return block_max(t, red);
 }
 
+// ordered_sum of columns 0..ncols-1 of partial[nparts][kMaxDIIS] in one
+// pass (each column summed in the same order as ordered_sum); the sums land
+// in out[] and are visible to the workgroup on return.
+__device__ __forceinline__ void ordered_sums(const double *partial, int nparts, int ncols, double *out,
+                                             double *red) {
+  double acc[kMaxDIIS];
+#pragma unroll
+  for (int i = 0; i < kMaxDIIS; i++) acc[i] = 0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < kMaxDIIS; i++)
+      if (i < ncols) acc[i] += partial[(size_t)b * kMaxDIIS + i];
+#pragma unroll
+  for (int i = 0; i < kMaxDIIS; i++)
+    if (i < ncols) {
+      const double t = block_sum(acc[i], red);
+      if (threadIdx.x == 0) out[i] = t;
+    }
+  __syncthreads();
+}
+
 #define GRID_LOOP(S) for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n_work_sites(S); w += gridDim.x * blockDim.x)
 
 __global__ void __launch_bounds__(kGridThreads) k_prepare(const DevSys *__restrict__ d) {
@@ -373,12 +394,8 @@ __global__ void __launch_bounds__(kGridThreads) k_diis_solve(const DevSys *__res
   DevScalars &sc = *S.sc;
   const int st = sc.status;
   if (st != DS_OK) return;
-  for (int i = 0; i < kd; i++) {
-    const double t = ordered_sum(partial, nparts, kMaxDIIS, i, red);
-    if (threadIdx.x == 0) dots[i] = t;
-  }
-  if (threadIdx.x == 0) diis_gram_update(sc, S.m, k_diis, dots, Ap);
-  __syncthreads();
+  ordered_sums(partial, nparts, kd, dots, red);
+  diis_gram_update_block(sc, S.m, k_diis, dots, Ap);
   if (threadIdx.x < 64) {
     const bool ok = wave_diis_coefficients(Ap, kd, sc.C);
     if (threadIdx.x == 0 && !ok) sc.status = DS_NAN;
@@ -477,12 +494,8 @@ __global__ void __launch_bounds__(kGridThreads) k_diis_solve_dev(const DevSys *_
   if (sc.status != DS_OK) return;
   int k, kd, k_diis;
   diis_indices(S, k, kd, k_diis);
-  for (int i = 0; i < kd; i++) {
-    const double t = ordered_sum(partial, nparts, kMaxDIIS, i, red);
-    if (threadIdx.x == 0) dots[i] = t;
-  }
-  if (threadIdx.x == 0) diis_gram_update(sc, S.m, k_diis, dots, Ap);
-  __syncthreads();
+  ordered_sums(partial, nparts, kd, dots, red);
+  diis_gram_update_block(sc, S.m, k_diis, dots, Ap);
   if (threadIdx.x < 64) {
     const bool ok = wave_diis_coefficients(Ap, kd, sc.C);
     if (threadIdx.x == 0 && !ok) sc.status = DS_NAN;

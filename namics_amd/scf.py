@@ -103,6 +103,7 @@ def lib():
         L.scf_get_phase_ms.argtypes = [vp, _D]
         L.scf_system_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.scf_system_set_checkpoint.argtypes = [vp, ctypes.c_int]
+        L.scf_system_set_graph.argtypes = [vp, ctypes.c_int]
         L.scf_system_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(TuneReport)]
         L.scf_system_layout.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_int)]
@@ -253,6 +254,11 @@ class ScfSystem:
         """Keep every `interval`-th end-point distribution and recompute the
         rest in the backward pass (save_memory); 0 stores every row."""
         _check(lib().scf_system_set_checkpoint(self.h, int(interval)))
+
+    def set_graph(self, on):
+        """Replay the grid-path iteration from a captured hipGraph (default)
+        or launch its kernels one by one."""
+        _check(lib().scf_system_set_graph(self.h, 1 if on else 0))
 
     def phase_ms(self):
         """Device time of the last fused residual / solve by phase (ms):

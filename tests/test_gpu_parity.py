@@ -105,6 +105,32 @@ def test_solve_matches_reference_trajectory_and_fixed_point(golden, case):
         s.close()
 
 
+def test_grid_iteration_without_graph_equals_graph_replay(golden):
+    """The grid-path DIIS iteration launched kernel by kernel gives the same
+    iterate as its hipGraph replay: same iteration count, bit-identical
+    fields and free energy."""
+    from namics_amd import ScfSystem, scf
+    fx = golden["diblock3d_periodic"]
+    cv = fx["converged"][0]
+    o0 = Oracle(fx["text"])
+    text = fx["text"] + "\nnewton : %s : method : DIIS\nnewton : %s : tolerance : %r\n" \
+        "newton : %s : iterationlimit : 5000\n" % (o0.newton_name, o0.newton_name, cv["tolerance"],
+                                                  o0.newton_name)
+    out = []
+    for graph in (True, False):
+        print("diblock3d_periodic graph", graph, flush=True)
+        s = ScfSystem(text)
+        s.set_mode(scf.MODE_GRID)
+        s.set_graph(graph)
+        x0 = np.array(cv["x0"]) if "x0" in cv else np.zeros(s.nvar)
+        x, it, res, st = s.solve(x0)
+        out.append((x, it, st, s.results()["free_energy"]))
+        s.close()
+    (xa, ia, sa, fa), (xb, ib, sb, fb) = out
+    assert sa == 0 and sb == 0 and ia == ib
+    assert np.array_equal(xa, xb) and fa == fb
+
+
 def test_tight_solve_matches_oracle_fixed_point(golden):
     """FP64 GPU solve at tolerance 1e-11 (the tightest the reference's DIIS
     reaches before its Gram regulariser makes it fail) equals the oracle's:
