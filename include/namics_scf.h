@@ -174,9 +174,19 @@ int scf_system_set_solver(scf_system *sys, double tolerance, double deltamax, in
  * summary of System::CheckResults (src/system.cpp:2719-2808). */
 int scf_residual(scf_system *sys, const double *x, double *g, double *residual);
 
-/* Solve the SCF equations by DIIS from the host fields x (updated in place
- * with the final fields).  Replaces: Solve_scf::Solve with method DIIS ->
- * SFNewton::iterate_DIIS (src/solve_scf.cpp:558-651, src/sfnewton.cpp:1037-1130). */
+/* Solve the SCF equations from the host fields x (updated in place with the
+ * final fields), by the input's `newton : method`.
+ * Replaces: Solve_scf::Solve (src/solve_scf.cpp:558-651) with
+ *   - DIIS -> SFNewton::iterate_DIIS (src/sfnewton.cpp:1037-1130), restated
+ *     on the device (same iterates up to rounding);
+ *   - pseudohessian / hessian (the reference's default) -> SFNewton::iterate
+ *     (src/sfnewton.cpp:636-747): solved by a Jacobian-free Newton-Krylov
+ *     method of this library instead (GMRES on forward-difference J v,
+ *     backtracking line search) that reaches the same fixed point by a
+ *     different path; `iterations` counts its Newton steps.
+ * The same rule holds for scf_solve_batch and scf_solve_resident (Newton
+ * systems are solved one after another, DIIS systems in one launch); the job
+ * queue and the request server solve every input by DIIS. */
 int scf_solve(scf_system *sys, double *x, scf_solve_info *info);
 
 /* Solve n independent systems in one launch (one workgroup per system);

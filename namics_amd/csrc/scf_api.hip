@@ -831,6 +831,7 @@ int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info) 
     scf_system *S = systems[i];
     if (!S) return fail(SCF_E_ARG, "null system");
     HIP_TRY(hipStreamSynchronize(S->stream));
+    if (S->newton()) continue;
     if (S->effective_mode() == SCF_MODE_FUSED) {
       fused.push_back(i);
       threads = std::max(threads, S->h.threads);
@@ -861,7 +862,10 @@ int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info) 
   }
   for (int i = 0; i < n; i++) {
     scf_system *S = systems[i];
-    if (S->effective_mode() == SCF_MODE_GRID) {
+    if (S->newton()) {
+      int rc = newton_solve(S, &info[i]);
+      if (rc != SCF_OK) return rc;
+    } else if (S->effective_mode() == SCF_MODE_GRID) {
       HIP_TRY(hipEventRecord(S->ev0, S->stream));
       int rc = grid_solve(S, &info[i]);
       if (rc != SCF_OK) return rc;

@@ -31,6 +31,8 @@ struct scf_system {
   bool owns_stream = true;  // false when a server lends its stream
   char *region = nullptr;   // arena placed in a caller's device region (not owned)
 
+  bool newton() const { return spec.solver.method != nscf::SOLVER_DIIS; }
+
   int effective_mode() const {
     if (mode != SCF_MODE_AUTO) return mode;
     if (tuned_mode != SCF_MODE_AUTO) return tuned_mode;
@@ -65,6 +67,8 @@ int build(scf_system *S, const Placement *pl = nullptr);
 int upload_solver(scf_system *S);
 int read_scalars(const scf_system *S, DevScalars &sc);
 int grid_solve(scf_system *S, scf_solve_info *info);
+// Newton-Krylov solve of a pseudohessian / hessian input (scf_newton.hip)
+int newton_solve(scf_system *S, scf_solve_info *info);
 double wall_clock_khz();
 size_t lds_limit();
 void plan_layout(scf_system *S, int want_sx);

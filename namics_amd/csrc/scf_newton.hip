@@ -1,0 +1,253 @@
+// Newton solver for `newton : method : pseudohessian | hessian` inputs.
+//
+// The reference solves these inputs with a dense quasi-Newton method
+// (SFNewton::iterate, src/sfnewton.cpp:636-747, selected in
+// src/solve_scf.cpp:191-231).  This is a different, self-contained Newton
+// method with the same fixed point: a Jacobian-free inexact Newton iteration
+// (Knoll & Keyes, J. Comput. Phys. 193 (2004) 357) --
+//
+//   * the Newton system J d = -F(x) is solved by GMRES (Saad & Schultz 1986)
+//     with modified Gram-Schmidt applied twice and Givens rotations, up to
+//     kRestarts cycles of kKrylov vectors;
+//   * J v is the forward difference (F(x + h v) - F(x)) / h with
+//     h = sqrt(eps (1 + ||x||)) / ||v||;
+//   * the GMRES tolerance follows Eisenstat & Walker's choice 2
+//     (SIAM J. Sci. Comput. 17 (1996) 16) with their safeguards, kept in
+//     [kEtaMin, kEtaMax] and no tighter than the final tolerance asks;
+//   * the step is capped at kMaxStep in max-norm and shortened by halving
+//     until ||F|| drops (Armijo condition, Dennis & Schnabel 6.3).
+//
+// F is the residual of the SCF equations (the hot path: every evaluation
+// runs on the device, fused or grid path); the Krylov vectors and the small
+// least-squares problem live on the host, as the reference keeps its Newton
+// on the host.  Masked variables (frozen sites, ghosts) have F = 0 for every
+// x, so J v vanishes there and the iteration never moves them.  The stop
+// test is the system's own: ||g||_2 (or max|g|) <= tolerance.  `iterations`
+// counts Newton steps, bounded by the input's iterationlimit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "scf_internal.h"
+
+namespace nscf {
+namespace {
+
+constexpr int kKrylov = 40;        // Krylov vectors per GMRES cycle
+constexpr int kRestarts = 3;       // GMRES cycles per Newton step
+constexpr int kHalvings = 30;      // step halvings before the step is given up
+constexpr double kMaxStep = 2.0;   // max |dx_i| of one Newton step
+constexpr double kEtaMax = 0.1;
+constexpr double kEtaMin = 1e-8;
+constexpr double kArmijo = 1e-4;
+
+double norm2(const std::vector<double> &v) {
+  double s = 0;
+  for (double a : v) s += a * a;
+  return std::sqrt(s);
+}
+
+double dot(const std::vector<double> &a, const std::vector<double> &b) {
+  double s = 0;
+  for (size_t i = 0; i < a.size(); i++) s += a[i] * b[i];
+  return s;
+}
+
+// Residual of the SCF equations at host fields x on the device.  Returns
+// SCF_OK or a HIP error; *status is the evaluation's own status (DS_OK or a
+// failure such as a vanishing partition function).  With `thermo`, the grid
+// path also computes F, Omega and mu (the fused path always does).
+int evaluate(scf_system *S, const double *x, double *g, double *res, int *status, bool thermo) {
+  const DevSys &h = S->h;
+  const size_t nvar = (size_t)h.nit * h.M;
+  hipStream_t st = S->stream;
+  HIP_TRY(hipMemcpyAsync(h.x, x, nvar * sizeof(double), hipMemcpyHostToDevice, st));
+  if (S->effective_mode() == SCF_MODE_FUSED) {
+    HIP_TRY(launch_fused(S->d_list, 1, FUSED_RESIDUAL, h.threads, S->lds_bytes, st));
+  } else {
+    HIP_TRY(grid_residual(h, S->d, h.partial, S->nb, st));
+    if (thermo) HIP_TRY(grid_thermo(h, S->d, h.partial, S->nb, st));
+  }
+  DevScalars sc;
+  HIP_TRY(hipMemcpyAsync(&sc, h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
+  if (g) HIP_TRY(hipMemcpyAsync(g, h.g, nvar * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  *res = sc.residual;
+  *status = sc.status;
+  return SCF_OK;
+}
+
+struct Newton {
+  scf_system *S;
+  size_t n;
+  std::vector<double> x, F, xt, Ft, w;
+  double fnorm = 0;
+  int hip_rc = SCF_OK;
+
+  // J v by a forward difference; false when the perturbed evaluation fails
+  bool jv(const std::vector<double> &v, std::vector<double> &out) {
+    const double vn = norm2(v);
+    if (!(vn > 0)) {
+      std::fill(out.begin(), out.end(), 0.0);
+      return true;
+    }
+    const double h = std::sqrt(2.220446049250313e-16 * (1.0 + norm2(x))) / vn;
+    for (size_t i = 0; i < n; i++) xt[i] = x[i] + h * v[i];
+    double r;
+    int status;
+    hip_rc = evaluate(S, xt.data(), Ft.data(), &r, &status, false);
+    if (hip_rc != SCF_OK || status != DS_OK) return false;
+    for (size_t i = 0; i < n; i++) out[i] = (Ft[i] - F[i]) / h;
+    return true;
+  }
+
+  // GMRES for J d = -F to relative residual eta; d holds the best iterate
+  bool gmres(double eta, std::vector<double> &d) {
+    std::fill(d.begin(), d.end(), 0.0);
+    std::vector<std::vector<double>> V(kKrylov + 1, std::vector<double>(n));
+    std::vector<double> H((kKrylov + 1) * kKrylov), cs(kKrylov), sn(kKrylov), gv(kKrylov + 1), y(kKrylov);
+    std::vector<double> r(n);
+    const double target = eta * fnorm;
+    for (int cycle = 0; cycle < kRestarts; cycle++) {
+      if (cycle == 0) {
+        for (size_t i = 0; i < n; i++) r[i] = -F[i];
+      } else {
+        if (!jv(d, w)) return false;
+        for (size_t i = 0; i < n; i++) r[i] = -F[i] - w[i];
+      }
+      const double beta = norm2(r);
+      if (beta <= target || !(beta > 0)) return true;
+      for (size_t i = 0; i < n; i++) V[0][i] = r[i] / beta;
+      std::fill(gv.begin(), gv.end(), 0.0);
+      gv[0] = beta;
+      int k = 0;
+      for (; k < kKrylov; k++) {
+        if (!jv(V[k], w)) return false;
+        for (int i = 0; i <= k; i++) H[i * kKrylov + k] = 0.0;
+        for (int pass = 0; pass < 2; pass++)
+          for (int i = 0; i <= k; i++) {
+            const double c = dot(w, V[i]);
+            H[i * kKrylov + k] += c;
+            for (size_t j = 0; j < n; j++) w[j] -= c * V[i][j];
+          }
+        const double hn = norm2(w);
+        H[(k + 1) * kKrylov + k] = hn;
+        if (hn > 0)
+          for (size_t j = 0; j < n; j++) V[k + 1][j] = w[j] / hn;
+        for (int i = 0; i < k; i++) {
+          const double a = H[i * kKrylov + k], b = H[(i + 1) * kKrylov + k];
+          H[i * kKrylov + k] = cs[i] * a + sn[i] * b;
+          H[(i + 1) * kKrylov + k] = -sn[i] * a + cs[i] * b;
+        }
+        const double a = H[k * kKrylov + k], b = H[(k + 1) * kKrylov + k];
+        const double rr = std::hypot(a, b);
+        cs[k] = rr > 0 ? a / rr : 1.0;
+        sn[k] = rr > 0 ? b / rr : 0.0;
+        H[k * kKrylov + k] = rr;
+        H[(k + 1) * kKrylov + k] = 0.0;
+        gv[k + 1] = -sn[k] * gv[k];
+        gv[k] = cs[k] * gv[k];
+        if (std::fabs(gv[k + 1]) <= target || !(hn > 0)) {
+          k++;
+          break;
+        }
+      }
+      // back substitution and update of d
+      for (int i = k - 1; i >= 0; i--) {
+        double s = gv[i];
+        for (int j = i + 1; j < k; j++) s -= H[i * kKrylov + j] * y[j];
+        y[i] = H[i * kKrylov + i] != 0.0 ? s / H[i * kKrylov + i] : 0.0;
+      }
+      for (int i = 0; i < k; i++)
+        for (size_t j = 0; j < n; j++) d[j] += y[i] * V[i][j];
+      if (std::fabs(gv[k]) <= target) return true;
+    }
+    return true;
+  }
+};
+
+}  // namespace
+
+int newton_solve(scf_system *S, scf_solve_info *info) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const DevSys &h = S->h;
+  const size_t n = (size_t)h.nit * h.M;
+  Newton N;
+  N.S = S;
+  N.n = n;
+  N.x.resize(n);
+  N.F.resize(n);
+  N.xt.resize(n);
+  N.Ft.resize(n);
+  N.w.resize(n);
+  HIP_TRY(hipMemcpyAsync(N.x.data(), h.x, n * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  HIP_TRY(hipStreamSynchronize(S->stream));
+  const double tol = h.tol;
+  const int itlim = h.itlim;
+  double res;
+  int status;
+  int rc = evaluate(S, N.x.data(), N.F.data(), &res, &status, false);
+  if (rc != SCF_OK) return rc;
+  N.fnorm = norm2(N.F);
+  int it = 0;
+  double eta = kEtaMax, fprev = N.fnorm;
+  std::vector<double> d(n), xbest(n);
+  while (status == DS_OK && res > tol && it < itlim) {
+    // forcing term (Eisenstat-Walker choice 2, safeguarded)
+    if (it > 0) {
+      const double ratio = N.fnorm / fprev;
+      double e = 0.9 * ratio * ratio;
+      const double safe = 0.9 * eta * eta;
+      if (safe > 0.1) e = std::max(e, safe);
+      eta = std::min(kEtaMax, std::max(kEtaMin, e));
+      // no tighter than what the remaining distance to the tolerance needs
+      eta = std::max(eta, std::min(kEtaMax, 0.5 * tol / N.fnorm));
+    }
+    if (!N.gmres(eta, d)) {
+      if (N.hip_rc != SCF_OK) return N.hip_rc;
+      break;  // a perturbed evaluation failed: stop where the iterate is
+    }
+    double dmax = 0;
+    for (double v : d) dmax = std::max(dmax, std::fabs(v));
+    double lambda = dmax > kMaxStep ? kMaxStep / dmax : 1.0;
+    bool accepted = false;
+    double rt = 0;
+    int st = DS_OK;
+    for (int k = 0; k < kHalvings; k++, lambda *= 0.5) {
+      for (size_t i = 0; i < n; i++) N.xt[i] = N.x[i] + lambda * d[i];
+      rc = evaluate(S, N.xt.data(), N.Ft.data(), &rt, &st, false);
+      if (rc != SCF_OK) return rc;
+      if (st != DS_OK) continue;
+      const double ft = norm2(N.Ft);
+      if (ft <= (1.0 - kArmijo * lambda) * N.fnorm) {
+        accepted = true;
+        break;
+      }
+    }
+    if (!accepted) break;  // no decrease along d: stop where the iterate is
+    fprev = N.fnorm;
+    N.x.swap(N.xt);
+    N.F.swap(N.Ft);
+    N.fnorm = norm2(N.F);
+    res = rt;
+    status = st;
+    it++;
+  }
+  // leave the device state (fields, densities, F, Omega, mu) at the final x
+  rc = evaluate(S, N.x.data(), nullptr, &res, &status, true);
+  if (rc != SCF_OK) return rc;
+  info->iterations = it;
+  info->residual = res;
+  info->status = status != DS_OK ? status : (res <= tol ? SCF_OK : SCF_NOT_CONVERGED);
+  info->fp32_iterations = 0;
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  info->kernel_ms = ms;
+  info->solve_ms = ms;
+  S->have_results = status == DS_OK;
+  return SCF_OK;
+}
+
+}  // namespace nscf

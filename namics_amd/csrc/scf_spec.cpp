@@ -1000,8 +1000,10 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
         so.method = SOLVER_DIIS;
       else if (v == "pseudohessian")
         so.method = SOLVER_PSEUDOHESSIAN;
+      else if (v == "hessian")
+        so.method = SOLVER_HESSIAN;
       else {
-        err = "newton method '" + v + "' is not supported (DIIS, pseudohessian)";
+        err = "newton method '" + v + "' is not supported (DIIS, pseudohessian, hessian)";
         return false;
       }
     }

@@ -193,6 +193,9 @@ class ScfSystem:
         return g, r.value
 
     def solve(self, x=None):
+        """Solve from fields x (zeros by default) with the input's method:
+        DIIS on the device, or Newton-Krylov over device residuals for
+        `pseudohessian` / `hessian`.  Returns (x, iterations, residual, status)."""
         x = np.zeros(self.nvar) if x is None else np.array(x, dtype=np.float64)
         info = SolveInfo()
         _check(lib().scf_solve(self.h, _p(x), ctypes.byref(info)))

@@ -131,6 +131,64 @@ def test_grid_iteration_without_graph_equals_graph_replay(golden):
     assert np.array_equal(xa, xb) and fa == fb
 
 
+def _newton_runs():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "newton_fixed_points.json")) as f:
+        return json.load(f)["runs"]
+
+
+@pytest.mark.parametrize("rec", _newton_runs(), ids=lambda r: "%s-%s" % (r["case"], r["method"]))
+def test_newton_reaches_the_reference_newton_fixed_point(golden, rec):
+    """`method : pseudohessian` / `hessian` inputs are solved by the
+    Newton-Krylov solver over device residuals; from seeded fields
+    (uniform in [-0.05, 0.05]; several of these systems have the homogeneous
+    state, x = 0, as their fixed point) it reaches the tolerance and the
+    fixed point the reference's own Newton reports:
+    F, Omega, per-molecule GN, n, theta, phibulk, mu and the segment profiles
+    to 1e-10 (or to what the reference's final residual fixes, when its
+    solver stopped above 1e-11)."""
+    from namics_amd import ScfSystem
+    text = golden[rec["case"]]["text"]
+    o0 = Oracle(text)
+    tol = rec["tolerance"]
+    text += "\nnewton : %s : method : %s\nnewton : %s : tolerance : %r\nnewton : %s : iterationlimit : 200\n" % (
+        o0.newton_name, rec["method"], o0.newton_name, tol, o0.newton_name)
+    print("newton", rec["case"], rec["method"], tol, flush=True)
+    s = ScfSystem(text)
+    x0 = np.random.default_rng(5).uniform(-0.05, 0.05, s.nvar)
+    x, it, res, st = s.solve(x0)
+    print("newton steps", it, "residual", res, "(reference: %d iterations)" % rec["iterations"], flush=True)
+    assert st == 0 and res <= tol, (st, res, it)
+    r = s.results()
+    bar = max(1e-10, 100 * rec["residual"])
+    scale = abs(rec["free_energy"]) + abs(rec["grand_potential"])
+    assert abs(r["free_energy"] - rec["free_energy"]) < bar * scale
+    assert abs(r["grand_potential"] - rec["grand_potential"]) < bar * scale
+    for m, d in rec["mol"].items():
+        for k in ("GN", "n", "theta", "phibulk"):
+            assert rel_err(r["mol"][m][k], d[k]) < bar, (m, k)
+        assert abs(r["mol"][m]["Mu"] - d["Mu"]) < bar * max(1.0, abs(d["Mu"])), (m, "Mu")
+    for sn, p in rec["seg_phi"].items():
+        assert scaled_err(s.interior(r["seg_phi"][sn]), p) < 10 * bar, sn
+    s.close()
+
+
+def test_newton_stops_at_the_iteration_limit(golden):
+    """A Newton solve cut short by the iteration limit reports the steps it
+    took and SCF_NOT_CONVERGED, and leaves the results of its last iterate."""
+    from namics_amd import ScfSystem
+    text = golden["cfg1_planar_adsorption"]["text"]
+    nn = Oracle(text).newton_name
+    s = ScfSystem(text + "\nnewton : %s : method : pseudohessian\nnewton : %s : tolerance : 1e-12\n"
+                         "newton : %s : iterationlimit : 2\n" % (nn, nn, nn))
+    x, it, res, st = s.solve()
+    assert it == 2 and st == 1 and res > 1e-12
+    g, res2 = s.residual(x)
+    assert res2 == res and s.results()["residual"] == res
+    s.close()
+
+
 def test_tight_solve_matches_oracle_fixed_point(golden):
     """FP64 GPU solve at tolerance 1e-11 (the tightest the reference's DIIS
     reaches before its Gram regulariser makes it fail) equals the oracle's:
@@ -156,8 +214,8 @@ def test_tight_solve_matches_oracle_fixed_point(golden):
 def test_batch_equals_alone(golden):
     """One launch over several systems gives each system the result it gets alone."""
     from namics_amd import ScfSystem, solve_batch
-    texts = [golden[c]["text"] for c in ("cfg1_planar_adsorption", "planar_markov2", "hex_periodic_restricted",
-                                         "brush2d_cyl")]
+    texts = [golden[c]["text"] + "\nnewton : %s : method : DIIS\n" % Oracle(golden[c]["text"]).newton_name
+             for c in ("cfg1_planar_adsorption", "planar_markov2", "hex_periodic_restricted", "brush2d_cyl")]
     alone = []
     for t in texts:
         s = ScfSystem(t)

@@ -74,3 +74,33 @@ def test_fixed_point_matches_pseudohessian(golden):
     assert abs(r["grand_potential"] - cv["grand_potential"]) < 1e-9 * abs(cv["free_energy"])
     for s, p in cv["seg_phi"].items():
         assert rel_err(o.interior(r["seg_phi"][s]), p, 1e-200) < 1e-8
+
+
+def _newton_runs():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "newton_fixed_points.json")) as f:
+        return json.load(f)["runs"]
+
+
+@pytest.mark.parametrize("rec", _newton_runs(), ids=lambda r: "%s-%s" % (r["case"], r["method"]))
+def test_reference_newton_fixed_points_are_diis_fixed_points(golden, rec):
+    """Every fixed point the reference's Newton solver (pseudohessian /
+    hessian) reports is the fixed point the DIIS restatement converges to
+    from zero fields at 1e-11: free energy and grand potential on the scale
+    of F, segment profiles on their own scale, each to what the two
+    residuals fix."""
+    text = golden[rec["case"]]["text"]
+    o0 = Oracle(text)
+    o = Oracle(text + "\nnewton : %s : method : DIIS\nnewton : %s : tolerance : 1e-11\n"
+               "newton : %s : iterationlimit : 20000\n" % (o0.newton_name, o0.newton_name, o0.newton_name))
+    x, it, res, st = o.solve(np.zeros(o.nvar))
+    assert st == 0
+    r = o.results()
+    tol = max(1e-9, 100 * rec["residual"])
+    scale = abs(rec["free_energy"]) + abs(rec["grand_potential"])
+    assert abs(r["free_energy"] - rec["free_energy"]) < tol * scale
+    assert abs(r["grand_potential"] - rec["grand_potential"]) < tol * scale
+    for s, p in rec["seg_phi"].items():
+        a = o.interior(r["seg_phi"][s])
+        assert np.max(np.abs(a - np.asarray(p))) < 10 * tol * max(1.0, np.max(np.abs(p))), s
