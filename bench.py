@@ -39,22 +39,26 @@ SEED = 3
 CHIS = (0.0, 0.5, 0.8)
 
 
-def propagator_bytes(desc, iterations, w=8):
+def propagator_bytes(desc, iterations, w=8, ckpt=1):
     """Algorithmic propagator bytes of one solve.  Per residual evaluation a
     first-order (Markov 1) chain of length N streams 3w per site forward (read
     G(s-1), read G1, write G(s)) and 6w backward with composition (read Gb,
     G1, Gf; write Gb; read+write rho): 9 w N per site.  A second-order chain
     with 3 direction components streams 7w forward and 12w backward: 19 w N.
-    Residual evaluations per solve = iterations + 1."""
+    With save_memory (every ckpt-th end-point row kept), each row not kept
+    is recomputed once more: 3w per site for each of N - ceil(N / ckpt)
+    rows.  Residual evaluations per solve = iterations + 1."""
     g = desc["gradients"]
     W = desc["MX"] * (desc["MY"] if g >= 2 else 1) * (desc["MZ"] if g == 3 else 1)
     if g == 1:
         W = desc["M"]  # 1-gradient kernels update the ghost sites too
     per_eval = sum((19 if m["markov"] == 2 else 9) * w * m["chainlength"] * W for m in desc["molecules"])
+    if ckpt > 1:
+        per_eval += sum(3 * w * (m["chainlength"] - -(-m["chainlength"] // ckpt)) * W for m in desc["molecules"])
     return per_eval * (iterations + 1)
 
 
-def profiled_traffic(jobs, slots):
+def profiled_traffic(jobs, slots, ckpt):
     """HBM bytes per launch of the batched-brush job kernel measured with
     rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-calibrated), from the committed
     profile summary, when this run has the profiled configuration."""
@@ -65,8 +69,8 @@ def profiled_traffic(jobs, slots):
     except OSError:
         return None, None
     m = re.search(r"traffic_bytes_per_launch = ([0-9.eE+-]+)", text)
-    c = re.search(r"traffic_config = jobs (\d+), slots (\d+)", text)
-    if m and c and int(c.group(1)) == jobs and int(c.group(2)) == slots:
+    c = re.search(r"traffic_config = jobs (\d+), slots (\d+), checkpoint (\d+)", text)
+    if m and c and int(c.group(1)) == jobs and int(c.group(2)) == slots and int(c.group(3)) == ckpt:
         return float(m.group(1)), os.path.relpath(path, REPO)
     return None, None
 
@@ -107,7 +111,7 @@ def cpu_baseline_brush(x0, names, dims, budget_s=60.0):
 
 
 def run_brush(args):
-    from namics_amd import DeviceBuffer, ScfSystem, device_info, solve_jobs, synchronize
+    from namics_amd import DeviceBuffer, ScfSystem, autotune_batch, device_info, solve_jobs, synchronize
     from namics_amd.workloads import brush_cylinder, uniform_fields
     ncu = device_info()[0]
     nslots = args.slots if args.slots > 0 else ncu
@@ -123,6 +127,17 @@ def run_brush(args):
         tune["layout"] = probe.layout()
         probe.close()
     slots = [[ScfSystem(brush_cylinder(c)) for c in CHIS] for _ in range(nslots)]
+    # end-point storage: tuned with all slots in flight (every row, or every
+    # other row with the odd rows recomputed on chip), unless given
+    K = args.checkpoint
+    if K == 0 and not args.no_tune:
+        tune_batch = autotune_batch([row[0] for row in slots])
+        tune = dict(tune or {}, batch=tune_batch)
+        K = tune_batch["chosen"]
+    K = max(1, K)
+    for row in slots:
+        for s in row:
+            s.set_checkpoint(K)
     descs = [s.desc for s in slots[0]]
     nvar0 = slots[0][0].nvar
     seeds = np.stack([uniform_fields(nvar0, SEED + j, 0.05) for j in range(J)])
@@ -143,7 +158,7 @@ def run_brush(args):
         solves = [r for res, _ in rec for job in res for r in job]
         conv = sum(1 for r in solves if r["status"] == 0)
         launch_ms = float(np.mean([ms for _, ms in rec]))
-        nbytes = float(np.mean([sum(propagator_bytes(descs[c], job[c]["iterations"])
+        nbytes = float(np.mean([sum(propagator_bytes(descs[c], job[c]["iterations"], ckpt=K)
                                     for job in res for c in range(len(CHIS))) for res, _ in rec]))
         return solves, conv, launch_ms, nbytes
 
@@ -168,7 +183,7 @@ def run_brush(args):
                                "chi 0/0.5/0.8 chained (each solve starts from the previous one's fields), "
                                "DIIS 1e-7, FP64",
                    "jobs_per_step": J, "solves_per_step": J * len(CHIS), "seeds": [SEED, SEED + J - 1],
-                   "slots": nslots},
+                   "slots": nslots, "checkpoint": K},
         "latency_ms": {"p50": float(np.percentile([r["solve_ms"] for r in solves], 50)),
                        "p99": float(np.percentile([r["solve_ms"] for r in solves], 99)),
                        "what": "device time of one solve inside the persistent launch"},
@@ -181,7 +196,7 @@ def run_brush(args):
         "setup_s": t_setup,
         "autotune": tune,
     }
-    traffic, src = profiled_traffic(J, nslots)
+    traffic, src = profiled_traffic(J, nslots, K)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
         out["roofline"]["traffic_source"] = src + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)"
@@ -210,14 +225,20 @@ def run_brush(args):
         for row in slots:
             for s in row:
                 s.set_precision(False)
-    # one job alone (the latency of the chained job on an otherwise idle GPU)
+    # one job alone (the latency of the chained job on an otherwise idle GPU),
+    # with the end-point storage tuned for one system in flight
+    K1 = K
+    if args.checkpoint == 0 and not args.no_tune:
+        K1 = autotune_batch(slots[0][:1])["chosen"]
+        for s in slots[0]:
+            s.set_checkpoint(K1)
     one, el1 = None, 0.0
     t1 = time.perf_counter()
     one, _ = solve_jobs(slots[:1], seeds_dev, 1)
     synchronize()
     el1 = time.perf_counter() - t1
     out["single_job"] = {"ms": 1e3 * el1, "solves_per_s": sum(1 for r in one[0] if r["status"] == 0) / el1,
-                         "iterations": [r["iterations"] for r in one[0]]}
+                         "iterations": [r["iterations"] for r in one[0]], "checkpoint": K1}
     if not args.no_cpu:
         cb = cpu_baseline_brush(seeds[0], slots[0][0].field_names, slots[0][0].dims)
         if cb is not None:
@@ -246,6 +267,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision rerun")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch layout (no autotuning)")
     ap.add_argument("--no-graph", action="store_true", help="box: launch the grid-path kernels one by one")
+    ap.add_argument("--checkpoint", type=int, default=0,
+                    help="brush: keep every K-th end-point row (save_memory); 0 tunes it with all slots in flight")
     ap.add_argument("--switch", type=float, default=1e-4, help="mixed precision: FP64 below this residual")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: requests per second")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals")

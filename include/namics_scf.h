@@ -157,6 +157,23 @@ typedef struct {
   double ms_per_eval[SCF_TUNE_MAX]; /* device time of one residual evaluation */
 } scf_tune_report;
 int scf_system_autotune(scf_system *sys, int evals, scf_tune_report *report);
+/* Concurrency autotuner for systems solved together (one launch, one
+ * workgroup per system, as scf_solve_resident and the job queue run them).
+ * Times `evals` residual evaluations of all n systems in one launch for each
+ * end-point storage that applies -- every row (interval 1) and, when every
+ * system is on the 2-gradient strip path, every other row with the odd rows
+ * recomputed on chip (interval 2, scf_system_set_checkpoint) -- and applies
+ * the fastest to all n.  Alone, a system's propagation is bound by its own
+ * LDS traffic and recomputing rows costs time; with one system per CU the
+ * stored rows saturate HBM and halving them pays.  Results are bit-identical
+ * either way.  No counterpart in the reference. */
+typedef struct {
+  int candidates;
+  int chosen;
+  int checkpoint[SCF_TUNE_MAX];      /* end-point row interval */
+  double ms_per_launch[SCF_TUNE_MAX]; /* one residual evaluation of all n systems */
+} scf_batch_tune_report;
+int scf_autotune_batch(scf_system *const *systems, int n, int evals, scf_batch_tune_report *report);
 /* Forget the layouts remembered per lattice shape (tests, benchmarks). */
 void scf_autotune_forget(void);
 /* The system's current launch layout: execution mode, strip width (0 when

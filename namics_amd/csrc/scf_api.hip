@@ -659,6 +659,67 @@ int scf_system_autotune(scf_system *S, int evals, scf_tune_report *rep) {
   return upload_solver(S);
 }
 
+int scf_autotune_batch(scf_system *const *systems, int n, int evals, scf_batch_tune_report *rep) {
+  if (!systems || n <= 0) return fail(SCF_E_ARG, "null argument");
+  if (evals < 1) evals = 1;
+  scf_batch_tune_report tmp;
+  if (!rep) rep = &tmp;
+  std::memset(rep, 0, sizeof(*rep));
+  bool strip = true;
+  int threads = 64;
+  size_t lds = 0;
+  for (int i = 0; i < n; i++) {
+    scf_system *S = systems[i];
+    if (!S) return fail(SCF_E_ARG, "null system");
+    if (S->effective_mode() != SCF_MODE_FUSED || S->newton())
+      return fail(SCF_E_UNSUPPORTED, "batch tuning times the one-workgroup-per-system launch");
+    strip = strip && S->h.strip_sx > 0;
+    threads = std::max(threads, S->h.threads);
+    lds = std::max(lds, S->lds_bytes);
+    HIP_TRY(hipStreamSynchronize(S->stream));
+  }
+  std::vector<int> cand{1};
+  if (strip) cand.push_back(2);
+  scf_system *S0 = systems[0];
+  hipStream_t st = S0->stream;
+  std::vector<DevSys *> list;
+  for (int i = 0; i < n; i++) list.push_back(systems[i]->d);
+  DevSys **d_list = nullptr;
+  HIP_TRY(hipMalloc((void **)&d_list, n * sizeof(DevSys *)));
+  auto body = [&]() -> int {
+    HIP_TRY(hipMemcpy(d_list, list.data(), n * sizeof(DevSys *), hipMemcpyHostToDevice));
+    double best = 1e300;
+    for (size_t c = 0; c < cand.size(); c++) {
+      for (int i = 0; i < n; i++) {
+        int rc = scf_system_set_checkpoint(systems[i], cand[c]);
+        if (rc != SCF_OK) return rc;
+      }
+      HIP_TRY(launch_fused(d_list, n, FUSED_RESIDUAL, threads, lds, st));  // warm-up
+      HIP_TRY(hipEventRecord(S0->ev0, st));
+      for (int e = 0; e < evals; e++) HIP_TRY(launch_fused(d_list, n, FUSED_RESIDUAL, threads, lds, st));
+      HIP_TRY(hipEventRecord(S0->ev1, st));
+      HIP_TRY(hipEventSynchronize(S0->ev1));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, S0->ev0, S0->ev1));
+      rep->checkpoint[c] = cand[c];
+      rep->ms_per_launch[c] = ms / evals;
+      rep->candidates = (int)c + 1;
+      if (ms < best) {
+        best = ms;
+        rep->chosen = (int)c;
+      }
+    }
+    for (int i = 0; i < n; i++) {
+      int rc = scf_system_set_checkpoint(systems[i], rep->checkpoint[rep->chosen]);
+      if (rc != SCF_OK) return rc;
+    }
+    return SCF_OK;
+  };
+  const int rc = body();
+  hipFree(d_list);
+  return rc;
+}
+
 int scf_system_set_solver(scf_system *S, double tolerance, double deltamax, int m, int iterationlimit) {
   if (!S) return fail(SCF_E_ARG, "null system");
   if (m > S->m_alloc) return fail(SCF_E_UNSUPPORTED, "DIIS history m above the input's setting");

@@ -421,6 +421,10 @@ __device__ __forceinline__ void strip_step(const __attribute__((address_space(3)
   {                                                                               \
     _Pragma("unroll") for (int k = 0; k < SX; k++) (row)[k * STRIDE + tt] = v[k]; \
   }
+#define STORE_ROW_FROM(row, a)                                                    \
+  {                                                                               \
+    _Pragma("unroll") for (int k = 0; k < SX; k++) (row)[k * STRIDE + tt] = a[k]; \
+  }
 #define STORE_GF(s_)                                                                \
   {                                                                                 \
     GT *q_ = Gf + (size_t)(s_) * RG;                                                \
@@ -578,6 +582,93 @@ __device__ __noinline__ void strip_backward_ckpt(const DevSys &S0, int mo, doubl
 #undef LOAD_ROWF
 }
 
+// Checkpointed backward pass with composition for K = 2: Gf holds the even
+// rows of the forward recursion (row 2q at index q).  An odd row 2q+1 is
+// recomputed on chip just before the backward step that composes it: its
+// base row 2q goes into the free LDS row and one forward step (the same
+// arithmetic, bit-identical) turns it into row 2q+1 in registers.  The HBM
+// traffic of the end-point rows halves for one extra propagation step per
+// two rows; every Gf load is issued one step ahead.  Expects the LDS weights
+// (s_lam) of the forward pass.
+template <typename T, int SX, int STRIDE>
+__device__ __noinline__ void strip_backward_pair(const DevSys &S0, int mo, double shift) {
+  typedef __attribute__((address_space(3))) T LT;
+  typedef __attribute__((address_space(1))) T GT;
+  CSys &S = uniform_sys(S0);
+  const int JX = S.JX;
+  STRIP_GEOM;
+  const bool sc = S.lattice_type == 0;
+  const T C1 = sc ? (T)(4.0 / 6.0) : (T)0.5, C2 = sc ? (T)(1.0 / 6.0) : (T)0.25;
+  const int b1 = S.mol_block_begin[mo + 1];
+  const int N = S.mol_N[mo];
+  const size_t M = S.M;
+  const size_t RG = (size_t)SX * nact;
+  GT *Gf = (GT *)S.Gf + tt;
+  const gdbl *G1 = hbm(S.G1);
+  LT *ga = (LT *)lds_row(0), *gb = ga + SX * STRIDE;
+  const LT *lamx = (LT *)s_lam + 2 * x0;
+  const double eshift = exp(shift);
+  const double unshift = exp(-shift * (N + 1));
+  T g1[SX], v[SX], gfv[SX];
+  double racc[SX];
+  int b = b1 - 1, bstart = N - S.block_n[b1 - 1];
+  if (active) {
+    LOAD_G1(S.block_seg[b]);
+    LOAD_GF((N - 1) / 2);  // row N-1 itself (even) or the base of its pair (odd)
+  }
+#pragma unroll
+  for (int k = 0; k < SX; k++) racc[k] = 0.0;
+  for (int s2 = N - 1; s2 >= 0; s2--) {
+    if (s2 < bstart) {
+      if (active) {
+        gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[b] * M + site0;
+#pragma unroll
+        for (int k = 0; k < SX; k++) rho[k * JX] += racc[k] * unshift;
+      }
+      b--;
+      bstart -= S.block_n[b];
+      if (active) LOAD_G1(S.block_seg[b]);
+#pragma unroll
+      for (int k = 0; k < SX; k++) racc[k] = 0.0;
+    }
+    if (s2 & 1) {
+      // gfv holds row s2-1: one forward step through the free LDS row
+      if (active) STORE_ROW_FROM(gb, gfv);
+      lds_barrier();
+      if (active) strip_step<T, SX, STRIDE>(gb, am0, ap0, sl, sr, dm, dp, lamx, C1, C2, g1, gfv);
+      lds_barrier();
+    }
+    if (active) {
+      if (s2 == N - 1) {
+#pragma unroll
+        for (int k = 0; k < SX; k++) v[k] = g1[k];
+      } else {
+        strip_step<T, SX, STRIDE>(ga, am0, ap0, sl, sr, dm, dp, lamx, C1, C2, g1, v);
+      }
+#pragma unroll
+      for (int k = 0; k < SX; k++) racc[k] += (double)gfv[k] * (double)v[k];
+      if (s2 == N - 1) {
+        STORE_ROW(ga);
+      } else {
+        STORE_ROW(gb);
+      }
+      // row s2-1 (even) or the base of the pair ending at s2-1 (odd)
+      if (s2 >= 1) LOAD_GF((s2 - 1) / 2);
+    }
+    lds_barrier();
+    if (s2 != N - 1) {
+      LT *tmp = ga;
+      ga = gb;
+      gb = tmp;
+    }
+  }
+  if (active) {
+    gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[b] * M + site0;
+#pragma unroll
+    for (int k = 0; k < SX; k++) rho[k * JX] += racc[k] * unshift;
+  }
+}
+
 template <typename T, int SX, int STRIDE, bool kCk>
 __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shift) {
   typedef __attribute__((address_space(3))) T LT;
@@ -654,7 +745,10 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shif
   LT *ga = ra, *gb = rb;
   const int bl = b1 - 1;
   if constexpr (kCk) {
-    strip_backward_ckpt<T, SX, STRIDE>(S0, mo, shift);
+    if (S.ckpt == 2)
+      strip_backward_pair<T, SX, STRIDE>(S0, mo, shift);
+    else
+      strip_backward_ckpt<T, SX, STRIDE>(S0, mo, shift);
   } else {
   if (active) {
     LOAD_G1(S.block_seg[bl]);
@@ -699,6 +793,7 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shif
 #undef STRIP_GEOM
 #undef LOAD_G1
 #undef STORE_ROW
+#undef STORE_ROW_FROM
 #undef STORE_GF
 #undef LOAD_GF
   __syncthreads();

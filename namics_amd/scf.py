@@ -41,6 +41,11 @@ class TuneReport(ctypes.Structure):
                 ("strip_sx", ctypes.c_int * 4), ("ms_per_eval", ctypes.c_double * 4)]
 
 
+class BatchTuneReport(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_int), ("chosen", ctypes.c_int), ("checkpoint", ctypes.c_int * 4),
+                ("ms_per_launch", ctypes.c_double * 4)]
+
+
 class JobResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("fp32_iterations", ctypes.c_int),
                 ("residual", ctypes.c_double), ("free_energy", ctypes.c_double),
@@ -107,6 +112,8 @@ def lib():
         L.scf_system_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(TuneReport)]
         L.scf_system_layout.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_int)]
+        L.scf_autotune_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(BatchTuneReport)]
         L.scf_autotune_forget.argtypes = []
         L.scf_autotune_forget.restype = None
         L.scf_describe_input.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
@@ -431,6 +438,19 @@ def solve_resident(systems, with_solve_ms=False):
     if with_solve_ms:
         return [(i.iterations, i.residual, i.status, i.kernel_ms, i.solve_ms) for i in infos]
     return [(i.iterations, i.residual, i.status, i.kernel_ms) for i in infos]
+
+
+def autotune_batch(systems, evals=3):
+    """Time one residual launch of all `systems` together for each end-point
+    storage that applies (every row; every other row, odd rows recomputed on
+    chip), apply the fastest to all of them and report the timings."""
+    n = len(systems)
+    hs = (ctypes.c_void_p * n)(*[s.h.value for s in systems])
+    rep = BatchTuneReport()
+    _check(lib().scf_autotune_batch(hs, n, int(evals), ctypes.byref(rep)))
+    return {"candidates": [{"checkpoint": rep.checkpoint[i], "ms_per_launch": rep.ms_per_launch[i]}
+                           for i in range(rep.candidates)],
+            "chosen": rep.checkpoint[rep.chosen]}
 
 
 def autotune_forget():

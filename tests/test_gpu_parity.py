@@ -327,9 +327,10 @@ def test_checkpointed_propagators_equal_full_storage(golden, case):
     """save_memory: keeping every K-th end-point distribution and recomputing
     the rows between checkpoints in the backward pass repeats the forward
     arithmetic exactly, so residuals, densities and partition functions are
-    bit-identical to full storage for every interval (2, 3, 5, N-1, N, N+4),
-    also for a chain of four blocks whose boundaries fall inside the
-    recomputed segments; and FP64 and mixed-precision solves follow the same
+    bit-identical to full storage for every interval (2, 3, 5, N-1, N, N+4;
+    K = 2 recomputes each odd row on chip), also for chains of four blocks,
+    of even and odd length, whose boundaries fall inside the recomputed
+    segments; and FP64 and mixed-precision solves follow the same
     trajectory to the last bit.  Lattices off the strip path refuse the
     option."""
     from namics_amd import ScfSystem, ScfError, solve_resident_info
@@ -338,6 +339,7 @@ def test_checkpointed_propagators_equal_full_storage(golden, case):
     texts = [fx["text"]]
     if "(X)1(A)39" in fx["text"]:
         texts.append(fx["text"].replace("(X)1(A)39", "(X)1(A)12(W)5(A)22"))
+        texts.append(fx["text"].replace("(X)1(A)39", "(X)1(A)11(W)5(A)22"))  # odd N
     else:
         texts.append(fx["text"].replace("(A)25", "(A)7(W)4(A)14"))
     for text in texts:
@@ -425,6 +427,37 @@ def test_autotuned_layout_gives_the_same_residual(golden, case):
         s2.close()
     finally:
         autotune_forget()
+
+
+def test_batch_autotune_applies_one_storage_to_all(golden):
+    """The concurrency autotuner times both end-point storages with every
+    system in flight, applies the faster to all of them, and the solves that
+    follow are bit-identical to full storage."""
+    from namics_amd import ScfSystem, autotune_batch, solve_resident_info
+    fx = golden["brush2d_rod_strip"]
+    nn = Oracle(fx["text"]).newton_name
+    text = fx["text"] + "\nnewton : %s : method : DIIS\nnewton : %s : tolerance : 1e-09\n" % (nn, nn)
+    probe = ScfSystem(text)
+    nvar = probe.nvar
+    probe.close()
+    rng = np.random.default_rng(3)
+    x0 = [rng.uniform(-0.05, 0.05, nvar) for _ in range(6)]
+    out = []
+    for tuned in (False, True):
+        systems = [ScfSystem(text) for _ in range(6)]
+        if tuned:
+            rep = autotune_batch(systems, evals=2)
+            assert [c["checkpoint"] for c in rep["candidates"]] == [1, 2]
+            assert rep["chosen"] in (1, 2) and all(c["ms_per_launch"] > 0 for c in rep["candidates"])
+            print("batch tune", rep, flush=True)
+        for s, x in zip(systems, x0):
+            s.upload(x)
+        infos = solve_resident_info(systems)
+        out.append([(i["iterations"], s.download(), s.results()["free_energy"]) for i, s in zip(infos, systems)])
+        for s in systems:
+            s.close()
+    for (ia, xa, fa), (ib, xb, fb) in zip(*out):
+        assert ia == ib and np.array_equal(xa, xb) and fa == fb
 
 
 def test_server_results_equal_alone_and_admission_waits(golden):
