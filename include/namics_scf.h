@@ -10,8 +10,7 @@
  *
  * The reference (leermakers/Namics) has no plugin API; its seams are C++
  * virtuals.  Each entry point names the reference interface it replaces, so a
- * maintainer can route the reference's own classes through this library
- * (see INTEGRATION.md).
+ * maintainer can route the reference's own classes through this library.
  */
 #ifndef NAMICS_SCF_H
 #define NAMICS_SCF_H
@@ -179,6 +178,42 @@ void scf_autotune_forget(void);
 /* The system's current launch layout: execution mode, strip width (0 when
  * not on the strip path) and threads per system. */
 int scf_system_layout(const scf_system *sys, int *mode, int *strip_sx, int *threads);
+
+/* ---- output and format layer (the reference's files) ----
+ * The `kal` / `pro` requests of the system's input (`output : kal : ...`,
+ * `kal : mol : pol : GN`, `pro : mon : A : phi`, names `*` expanded) are
+ * written in the reference's formats.  kal quantities: sys free_energy,
+ * grand_potential, start; newton iterations, residual, tolerance,
+ * iterationlimit, m, delta_max, method, stop_criterion; mol GN, n, theta,
+ * theta_exc, phibulk, Mu, chainlength, composition, freedom, Markov,
+ * k_stiff (1-D also phi1, phiM, Dphi); mon freedom, valence, phibulk, theta;
+ * lat gradients, volume, lattice_type, FJC_choices, offset_first_layer
+ * (when > 0), bond_length, n_layers, n_layers_y, n_layers_z.  pro profiles: mon and mol phi.  A quantity the
+ * library does not know prints "NiN" in a kal row and is left out of a pro
+ * file, as the reference does for unknown names.
+ *
+ * Append this system's kal row to `path` (header first when the file is new
+ * or `append` is 0).  Replaces: Output::WriteOutput for kal
+ * (src/output.cpp:511-557). */
+int scf_write_kal(scf_system *sys, const char *path, int append, const scf_solve_info *info);
+/* Write the requested profiles.  Replaces: Output::WriteOutput for pro and
+ * Lattice::PutProfiles (src/output.cpp:487-510, src/LGrad1.cpp:228-249,
+ * src/LGrad2.cpp:185-205, src/LGrad3.cpp:81-102). */
+int scf_write_pro(scf_system *sys, const char *path);
+/* Write the system's fields as a guess file.  Replaces: Lattice::StoreGuess
+ * (src/lattice.cpp:971-1019). */
+int scf_store_guess(scf_system *sys, const char *path);
+/* Read a guess file and upload the field blocks it names (same lattice
+ * size).  Replaces: Lattice::ReadGuess + Solve_scf::Guess
+ * (src/lattice.cpp:926-968, src/solve_scf.cpp:489-524). */
+int scf_read_guess(scf_system *sys, const char *path);
+/* Run every start block of an input file like the reference program: initial
+ * guess (file, previous_result or none), solve with the input's method, kal /
+ * pro / final-guess files in the output folder (./output/ unless
+ * `out_info : folder : path` is given, relative to the working directory).
+ * Replaces: the start loop of main (src/namics.cpp:163-737) for SCF inputs
+ * without `var` scans. */
+int scf_run_input_file(const char *path, int *n_starts, int *n_converged);
 
 /* Override solver settings: tolerance, deltamax, DIIS history m, iteration
  * limit (values <= 0 keep the input's setting). */

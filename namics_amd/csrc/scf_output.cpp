@@ -1,0 +1,417 @@
+// Output and format layer: the reference's `kal` and `pro` files, its guess
+// files, and a runner that takes an input file through its start blocks the
+// way the reference program does.
+//
+//   kal   one row per solve: the requested scalars, `%.16e` for reals, `%i`
+//         for integers, strings as they are, "NiN" for a quantity that is not
+//         known; a header of `key<sep>name<sep>prop` names when the file is
+//         new (Output::WriteOutput, src/output.cpp:511-557).
+//   pro   the requested profiles with the coordinates of every interior site
+//         (or every site with write_bounds): 1-D rows `%e` then `%.20g`
+//         values, 2-D / 3-D rows `%e` coordinates then `%.20e` values, each
+//         followed by a tab (LGrad1/LGrad2/LGrad3::PutProfiles,
+//         src/LGrad1.cpp:228-249, src/LGrad2.cpp:185-205, src/LGrad3.cpp:81-102).
+//   guess method, sizes and refinement, charge flag, field-block names, then
+//         one `%e` value per line (Lattice::StoreGuess,
+//         src/lattice.cpp:971-1019); read back by name (Lattice::ReadGuess and
+//         Solve_scf::Guess, src/lattice.cpp:926-968, src/solve_scf.cpp:489-524).
+//
+// File naming follows the reference: the stem is the input file name up to
+// its first '.', without directories (use_output_folder); `kal` files are
+// <stem>.kal, profiles <stem>.pro for a single start block and
+// <stem>_<start>.pro otherwise, guesses <stem>_<start>.outiv (or the
+// guess_outputfile name with _<start> before its extension), all inside the
+// output folder (./output/ unless `out_info : folder : path` says otherwise).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "scf_internal.h"
+
+namespace nscf {
+namespace {
+
+// Lattice::WeightedSum with the ghosts removed (src/LGrad1.cpp:213-221,
+// src/LGrad2.cpp:147-157, src/LGrad3.cpp:40-46): sums in site order.
+double weighted_sum(const LatticeSpec &l, const std::vector<double> &X) {
+  double s = 0;
+  if (l.gradients == 3) {
+    for (int i = 0; i < l.M; i++) s += X[i];
+    return s;
+  }
+  if (l.geometry == GEOM_PLANAR) {
+    for (int i = 0; i < l.M; i++) s += X[i];
+    return l.gradients == 1 ? s / l.fjc : s / (l.fjc * l.fjc);
+  }
+  for (int i = 0; i < l.M; i++) s += X[i] * l.L[i];
+  return s;
+}
+
+// zero the ghost layers (the reference's remove_bounds before its sums and
+// before a profile is written)
+void remove_ghosts(const LatticeSpec &l, std::vector<double> &X) {
+  const int f = l.fjc;
+  const int nx = l.MX + 2 * f, ny = l.gradients >= 2 ? l.MY + 2 * f : 1, nz = l.gradients == 3 ? l.MZ + 2 * f : 1;
+  for (int x = 0; x < nx; x++)
+    for (int y = 0; y < ny; y++)
+      for (int z = 0; z < nz; z++) {
+        const bool in = x >= f && x < l.MX + f && (l.gradients < 2 || (y >= f && y < l.MY + f)) &&
+                        (l.gradients < 3 || (z >= f && z < l.MZ + f));
+        if (!in) X[(size_t)x * l.JX + (size_t)y * l.JY + z * l.JZ] = 0.0;
+      }
+}
+
+// ghost layers as the boundary conditions fill them (set_bounds: mirror and
+// periodic ghosts copy their interior partner, a surface ghost keeps its own
+// value), the state of a profile when the reference writes it
+int ghost_map(int x, int n, int f, const std::vector<int> &lo, const std::vector<int> &hi) {
+  if (x < f) return lo[x];
+  if (x >= n + f) return hi[x - n - f];
+  return x;
+}
+
+void set_ghosts(const LatticeSpec &l, std::vector<double> &X) {
+  const int f = l.fjc;
+  const int nx = l.MX + 2 * f, ny = l.gradients >= 2 ? l.MY + 2 * f : 1, nz = l.gradients == 3 ? l.MZ + 2 * f : 1;
+  std::vector<int> lx = l.bx1f, hx = l.bxmf, ly = l.by1f, hy = l.bymf, lz = l.bz1f, hz = l.bzmf;
+  if (f == 1) {
+    lx = {l.BX1};
+    hx = {l.BXM};
+    ly = {l.BY1};
+    hy = {l.BYM};
+    lz = {l.BZ1};
+    hz = {l.BZM};
+  }
+  const std::vector<double> Y = X;
+  for (int x = 0; x < nx; x++)
+    for (int y = 0; y < ny; y++)
+      for (int z = 0; z < nz; z++) {
+        const int sx = ghost_map(x, l.MX, f, lx, hx);
+        const int sy = l.gradients >= 2 ? ghost_map(y, l.MY, f, ly, hy) : y;
+        const int sz = l.gradients == 3 ? ghost_map(z, l.MZ, f, lz, hz) : z;
+        X[(size_t)x * l.JX + (size_t)y * l.JY + z * l.JZ] = Y[(size_t)sx * l.JX + (size_t)sy * l.JY + sz * l.JZ];
+      }
+}
+
+struct Value {
+  int kind = 0;  // 0 unknown, 1 int, 2 real, 3 string
+  int i = 0;
+  double r = 0;
+  std::string s;
+};
+Value vint(int v) {
+  Value o;
+  o.kind = 1;
+  o.i = v;
+  return o;
+}
+Value vreal(double v) {
+  Value o;
+  o.kind = 2;
+  o.r = v;
+  return o;
+}
+Value vstr(const std::string &v) {
+  Value o;
+  o.kind = 3;
+  o.s = v;
+  return o;
+}
+
+// Everything the writers read from one solved system.
+struct State {
+  scf_results r;
+  std::vector<double> seg_phi, mol_phi;  // [n][M]
+};
+
+int load_state(const scf_system *S, State &st) {
+  int rc = scf_get_results(S, &st.r);
+  if (rc != SCF_OK) return rc;
+  const size_t M = S->h.M;
+  st.seg_phi.assign(S->h.nseg * M, 0.0);
+  st.mol_phi.assign(S->h.nmol * M, 0.0);
+  return scf_get_profiles(S, st.seg_phi.data(), st.mol_phi.data());
+}
+
+// One kal quantity (the scalars the reference's objects push for output,
+// src/system.cpp PushOutput, src/molecule.cpp:1939-2075,
+// src/segment.cpp:1741-1800, src/solve_scf.cpp:325-345, lattice PushOutput)
+Value kal_value(const scf_system *S, const State &st, const scf_solve_info *info, const OutputItem &it) {
+  const SystemSpec &s = S->spec;
+  const LatticeSpec &l = s.lat;
+  const std::string &p = it.prop;
+  if (it.key == "sys" && it.name == s.sys_name) {
+    if (p == "free_energy") return vreal(st.r.free_energy);
+    if (p == "grand_potential") return vreal(st.r.grand_potential);
+    if (p == "start") return vint(s.start);
+  } else if (it.key == "newton" && it.name == s.newton_name) {
+    if (p == "iterations" && info) return vint(info->iterations);
+    if (p == "residual") return vreal(info ? info->residual : st.r.residual);
+    if (p == "tolerance") return vreal(S->h.tol);
+    if (p == "iterationlimit") return vint(S->h.itlim);
+    if (p == "m") return vint(s.solver.m);
+    if (p == "delta_max") return vreal(s.solver.deltamax);
+    if (p == "method") return vstr(s.method_name);
+    if (p == "stop_criterion") return vstr(s.stop_criterion_name);
+  } else if (it.key == "mol") {
+    for (size_t i = 0; i < s.mol.size(); i++) {
+      if (s.mol[i].name != it.name) continue;
+      const MoleculeSpec &m = s.mol[i];
+      if (p == "GN") return vreal(st.r.GN[i]);
+      if (p == "n") return vreal(st.r.n[i]);
+      if (p == "theta") return vreal(st.r.theta[i]);
+      if (p == "theta_exc" || p == "thetaexc") return vreal(st.r.theta[i] - l.volume * st.r.phibulk[i]);
+      if (p == "phibulk") return vreal(st.r.phibulk[i]);
+      if (p == "Mu") return vreal(st.r.mu[i]);
+      if (p == "chainlength") return vint(m.chainlength);
+      if (p == "composition") return vstr(s.mol_composition[i]);
+      if (p == "freedom") return vstr(s.mol_freedom_name[i]);
+      if (p == "Markov") return vint(m.markov);
+      if (p == "k_stiff") return vreal(m.k_stiff);
+      if (l.gradients == 1 && (p == "phi1" || p == "phiM" || p == "Dphi")) {
+        const double *ph = st.mol_phi.data() + i * (size_t)l.M;
+        const double a = ph[l.fjc], b = ph[l.M - 2 * l.fjc];
+        return vreal(p == "phi1" ? a : (p == "phiM" ? b : a - b));
+      }
+    }
+  } else if (it.key == "mon") {
+    for (size_t k = 0; k < s.seg.size(); k++) {
+      if (s.seg[k].name != it.name) continue;
+      if (p == "freedom") return vstr(s.seg_freedom_name[k]);
+      if (p == "valence") return vreal(s.seg_valence[k]);
+      if (p == "phibulk") return vreal(st.r.seg_phibulk[k]);
+      if (p == "theta") {
+        std::vector<double> X(st.seg_phi.begin() + k * (size_t)l.M, st.seg_phi.begin() + (k + 1) * (size_t)l.M);
+        remove_ghosts(l, X);
+        return vreal(weighted_sum(l, X));
+      }
+    }
+  } else if (it.key == "lat") {
+    if (p == "gradients") return vint(l.gradients);
+    if (p == "volume") return vreal(l.volume);
+    if (p == "lattice_type") return vstr(s.lattice_type_name);
+    if (p == "FJC_choices") return vint(l.FJC);
+    if (p == "offset_first_layer" && l.offset_first_layer > 0) return vreal(l.offset_first_layer);
+    if (p == "bond_length") return vreal(l.bond_length / l.fjc);
+    if (p == "n_layers") return vint(l.MX / l.fjc);
+    if (p == "n_layers_y" && l.gradients >= 2) return vint(l.MY / l.fjc);
+    if (p == "n_layers_z" && l.gradients == 3) return vint(l.MZ / l.fjc);
+  }
+  return Value();
+}
+
+std::string header_name(const OutputItem &it, const std::string &sep) { return it.key + sep + it.name + sep + it.prop; }
+
+bool file_exists(const std::string &path) {
+  std::ifstream f(path.c_str());
+  return (bool)f;
+}
+
+}  // namespace
+
+// Stem of the output files from the input file name (Output::WriteOutput,
+// src/output.cpp:392-419): the part before the first '.', without the
+// directories when use_output_folder is set.
+std::string output_stem(const std::string &input_path, bool use_output_folder) {
+  std::string stem = input_path.substr(0, input_path.find('.'));
+  if (use_output_folder) {
+    const size_t k = stem.find_last_of("/\\");
+    if (k != std::string::npos) stem = stem.substr(k + 1);
+  }
+  return stem;
+}
+
+}  // namespace nscf
+
+using namespace nscf;
+
+extern "C" {
+
+int scf_write_kal(scf_system *S, const char *path, int append, const scf_solve_info *info) {
+  if (!S || !path) return fail(SCF_E_ARG, "null argument");
+  const OutputRequest &req = S->spec.out.kal;
+  State st;
+  int rc = load_state(S, st);
+  if (rc != SCF_OK) return rc;
+  const bool header = !(file_exists(path) && append);
+  FILE *fp = std::fopen(path, header ? "w" : "a");
+  if (!fp) return fail(SCF_E_ARG, std::string("cannot open ") + path);
+  const size_t n = req.items.size();
+  if (header) {
+    for (size_t i = 0; i < n; i++)
+      std::fprintf(fp, i + 1 < n ? "%s\t" : "%s", header_name(req.items[i], req.sep).c_str());
+    std::fprintf(fp, "\n");
+  }
+  for (size_t i = 0; i < n; i++) {
+    const Value v = kal_value(S, st, info, req.items[i]);
+    const char *tab = i + 1 < n ? "\t" : "";
+    if (v.kind == 0) std::fprintf(fp, "NiN%s", tab);
+    if (v.kind == 1) std::fprintf(fp, "%i%s", v.i, tab);
+    if (v.kind == 2) std::fprintf(fp, "%.16e%s", v.r, tab);
+    if (v.kind == 3) std::fprintf(fp, "%s%s", v.s.c_str(), tab);
+  }
+  std::fprintf(fp, "\n");
+  std::fclose(fp);
+  return SCF_OK;
+}
+
+int scf_write_pro(scf_system *S, const char *path) {
+  if (!S || !path) return fail(SCF_E_ARG, "null argument");
+  const SystemSpec &s = S->spec;
+  const LatticeSpec &l = s.lat;
+  const OutputRequest &req = s.out.pro;
+  State st;
+  int rc = load_state(S, st);
+  if (rc != SCF_OK) return rc;
+  // the profiles the request names that exist: segment and molecule densities
+  std::vector<std::string> names;
+  std::vector<std::vector<double>> cols;
+  for (const OutputItem &it : req.items) {
+    const double *src = nullptr;
+    if (it.prop == "phi" && it.key == "mon")
+      for (size_t k = 0; k < s.seg.size(); k++)
+        if (s.seg[k].name == it.name) src = st.seg_phi.data() + k * (size_t)l.M;
+    if (it.prop == "phi" && it.key == "mol")
+      for (size_t i = 0; i < s.mol.size(); i++)
+        if (s.mol[i].name == it.name) src = st.mol_phi.data() + i * (size_t)l.M;
+    if (!src) continue;  // not a profile this system knows: left out, as the reference does
+    std::vector<double> X(src, src + l.M);
+    if (req.write_bounds) set_ghosts(l, X);
+    cols.push_back(std::move(X));
+    names.push_back(header_name(it, req.sep));
+  }
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return fail(SCF_E_ARG, std::string("cannot open ") + path);
+  std::fprintf(fp, l.gradients == 1 ? "x\t" : (l.gradients == 2 ? "x\ty\t" : "x\ty\tz\t"));
+  for (size_t i = 0; i < names.size(); i++)
+    std::fprintf(fp, i + 1 < names.size() ? "%s\t" : "%s", names[i].c_str());
+  std::fprintf(fp, "\n");
+  const int f = l.fjc, a = req.write_bounds ? 0 : f;
+  const double off = l.offset_first_layer / f;
+  auto coord = [&](int x) { return 1.0 * (x - f + 1) / f - 0.5 / f; };
+  if (l.gradients == 1) {
+    for (int x = a; x < l.MX + 2 * f - a; x++) {
+      std::fprintf(fp, "%e\t", off + 1.0 * (x - f + 1) / f - 0.5 / f);
+      for (size_t c = 0; c < cols.size(); c++)
+        std::fprintf(fp, c + 1 < cols.size() ? "%.20g\t" : "%.20g", cols[c][x]);
+      std::fprintf(fp, "\n");
+    }
+  } else if (l.gradients == 2) {
+    for (int x = a; x < l.MX + 2 * f - a; x++)
+      for (int y = a; y < l.MY + 2 * f - a; y++) {
+        std::fprintf(fp, "%e\t%e\t", off + 1.0 * (x - f + 1) / f - 0.5 / f, coord(y));
+        for (auto &col : cols) std::fprintf(fp, "%.20e\t", col[(size_t)x * l.JX + y]);
+        std::fprintf(fp, "\n");
+      }
+  } else {
+    for (int x = a; x < l.MX + 2 * f - a; x++)
+      for (int y = a; y < l.MY + 2 * f - a; y++)
+        for (int z = a; z < l.MZ + 2 * f - a; z++) {
+          std::fprintf(fp, "%e\t%e\t%e\t", coord(x), coord(y), coord(z));
+          for (auto &col : cols) std::fprintf(fp, "%.20e\t", col[(size_t)x * l.JX + (size_t)y * l.JY + z]);
+          std::fprintf(fp, "\n");
+        }
+  }
+  std::fclose(fp);
+  return SCF_OK;
+}
+
+int scf_store_guess(scf_system *S, const char *path) {
+  if (!S || !path) return fail(SCF_E_ARG, "null argument");
+  const SystemSpec &s = S->spec;
+  const LatticeSpec &l = s.lat;
+  std::vector<double> x((size_t)s.nvar());
+  int rc = scf_download_fields(S, x.data());
+  if (rc != SCF_OK) return rc;
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return fail(SCF_E_ARG, std::string("cannot open ") + path);
+  std::fprintf(fp, "%s\n", s.method_name.c_str());
+  std::fprintf(fp, " %i\t%i\t%i\t%i\n", l.MX, l.gradients >= 2 ? l.MY : 0, l.gradients == 3 ? l.MZ : 0, l.fjc);
+  std::fprintf(fp, "%s\n", "false");
+  std::fprintf(fp, "%i\n", (int)s.itmon.size());
+  for (int k : s.itmon) std::fprintf(fp, "%s\n", s.seg[k].name.c_str());
+  std::fprintf(fp, "%i\n", 0);
+  for (double v : x) std::fprintf(fp, "%e\n", v);
+  std::fclose(fp);
+  return SCF_OK;
+}
+
+int scf_read_guess(scf_system *S, const char *path) {
+  if (!S || !path) return fail(SCF_E_ARG, "null argument");
+  std::ifstream f(path);
+  if (!f) return fail(SCF_E_ARG, std::string("guess file ") + path + " not found");
+  std::stringstream ss;
+  ss << f.rdbuf();
+  std::vector<double> x;
+  std::string err;
+  if (!read_guess_text(ss.str(), S->spec, x, err)) return fail(SCF_E_ARG, err);
+  return scf_upload_fields(S, x.data());
+}
+
+int scf_run_input_file(const char *path, int *n_starts_out, int *n_converged_out) {
+  if (!path) return fail(SCF_E_ARG, "null argument");
+  std::ifstream f(path);
+  if (!f) return fail(SCF_E_ARG, std::string("input file ") + path + " not found");
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string text = ss.str();
+  int n_starts = count_starts(text);
+  if (n_starts < 1) n_starts = 1;
+  int converged = 0;
+  scf_system *prev = nullptr;
+  int rc = SCF_OK;
+  bool kal_started = false;
+  for (int start = 1; start <= n_starts && rc == SCF_OK; start++) {
+    scf_system *S = nullptr;
+    char err[512] = {0};
+    rc = scf_system_create(text.c_str(), start, &S, err, sizeof(err));
+    if (rc != SCF_OK) break;
+    const SystemSpec &s = S->spec;
+    // initial guess (namics main loop, src/namics.cpp:419-466,530-570)
+    if (s.initial_guess == "file") {
+      rc = scf_read_guess(S, s.guess_inputfile.c_str());
+    } else if (s.initial_guess == "previous_result" && prev) {
+      rc = scf_carry_fields(S, prev);
+    }
+    scf_solve_info info{};
+    if (rc == SCF_OK) rc = scf_solve_resident(&S, 1, &info);
+    if (rc == SCF_OK && info.status == SCF_OK) converged++;
+    const std::string stem = output_stem(path, s.out.use_output_folder);
+    if (rc == SCF_OK && s.out.kal.present && s.out.kal.write) {
+      // the first kal request of a run appends only when asked (or when the
+      // request sets no append option); later starts always append
+      const bool append = kal_started || s.out.kal.append || !s.out.kal.append_given;
+      rc = scf_write_kal(S, (s.out.path + stem + ".kal").c_str(), append ? 1 : 0, &info);
+      kal_started = true;
+    }
+    if (rc == SCF_OK && s.out.pro.present && s.out.pro.write) {
+      const std::string name = n_starts == 1 ? stem + ".pro" : stem + "_" + std::to_string(start) + ".pro";
+      rc = scf_write_pro(S, (s.out.path + name).c_str());
+    }
+    if (rc == SCF_OK && s.final_guess == "file") {
+      std::string name;
+      if (s.guess_outputfile.empty()) {
+        name = stem + "_" + std::to_string(start) + ".outiv";
+      } else {
+        const std::string &g = s.guess_outputfile;
+        const size_t d = g.find('.');
+        const std::string ext = d == std::string::npos ? "" : g.substr(d + 1, g.find('.', d + 1) - d - 1);
+        name = g.substr(0, d) + "_" + std::to_string(start) + "." + ext;
+      }
+      rc = scf_store_guess(S, (s.out.path + name).c_str());
+    }
+    if (prev) scf_system_destroy(prev);
+    prev = S;
+  }
+  if (prev) scf_system_destroy(prev);
+  if (n_starts_out) *n_starts_out = n_starts;
+  if (n_converged_out) *n_converged_out = converged;
+  return rc;
+}
+
+}  // extern "C"

@@ -576,6 +576,12 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     err = "'lattice_type' is required";
     return false;
   }
+  s.lattice_type_name = v;
+  {
+    std::string bl;
+    double b = 0;
+    if (get(p, "lat", ln, "bondlength", bl) && to_real(bl, b)) l.bond_length = b;
+  }
   if (v == "simple_cubic") {
     l.lattice_type = LAT_SIMPLE_CUBIC;
     l.lambda = 1.0 / 6.0;
@@ -733,6 +739,12 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       err = "mon '" + sg.name + "' has no 'freedom'";
       return false;
     }
+    s.seg_freedom_name.push_back(v);
+    {
+      std::string vv;
+      double val = 0;
+      s.seg_valence.push_back(get(p, "mon", sg.name, "valence", vv) && to_real(vv, val) ? val : 0.0);
+    }
     if (v == "free")
       sg.freedom = SEG_FREE;
     else if (v == "frozen")
@@ -821,6 +833,7 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       err = "mol '" + m.name + "' has no composition";
       return false;
     }
+    s.mol_composition.push_back(v);
     if (!parse_composition(v, p.mons, m.blocks, err)) return false;
     m.chainlength = 0;
     for (auto &b : m.blocks) {
@@ -842,6 +855,7 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       err = "mol '" + m.name + "' has no freedom";
       return false;
     }
+    s.mol_freedom_name.push_back(v);
     if (v == "free")
       m.freedom = MOL_FREE;
     else if (v == "restricted")
@@ -996,6 +1010,7 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       if (so.tolerance < 1e-16 || so.tolerance > 10) so.tolerance = 1e-5;
     }
     if (get(p, "newton", nn, "method", v)) {
+      s.method_name = v;
       if (v == "DIIS")
         so.method = SOLVER_DIIS;
       else if (v == "pseudohessian")
@@ -1010,10 +1025,52 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     if (get(p, "newton", nn, "m", v)) {
       if (!to_int(v, so.m) || so.m < 0 || so.m > 100) so.m = 10;
     }
-    if (get(p, "newton", nn, "stop_criterion", v)) so.max_g = (v == "max_of_element_of_|g|");
+    if (get(p, "newton", nn, "stop_criterion", v)) {
+      so.max_g = (v == "max_of_element_of_|g|");
+      if (so.max_g) s.stop_criterion_name = v;
+    }
   }
   if (get(p, "sys", p.sys_name, "initial_guess", v)) s.initial_guess = v;
   if (get(p, "sys", p.sys_name, "guess_inputfile", v)) s.guess_inputfile = v;
+  if (get(p, "sys", p.sys_name, "final_guess", v) && (v == "next_problem" || v == "file")) s.final_guess = v;
+  if (get(p, "sys", p.sys_name, "guess_outputfile", v)) s.guess_outputfile = v;
+  s.start = start;
+  // output requests (Output::CheckInput / Load, src/output.cpp:32-178):
+  // `output : kal | pro : option : value` and `kal | pro : key : name : prop`
+  // item lines in input order; name `*` expands to every molecule / segment
+  OutputSpec &out = s.out;
+  for (auto &ln : p.lines) {
+    if (ln.key == "out_info" && ln.name == "folder" && ln.prop == "path" && !ln.value.empty())
+      out.path = ln.value.back() == '/' ? ln.value : ln.value + "/";
+    if (ln.key != "output" || (ln.name != "kal" && ln.name != "pro")) continue;
+    OutputRequest &r = ln.name == "kal" ? out.kal : out.pro;
+    r.present = true;
+    bool b;
+    if (ln.prop == "append" && to_bool(ln.value, b)) {
+      r.append = b;
+      r.append_given = true;
+    }
+    if (ln.prop == "write_bounds" && to_bool(ln.value, b)) r.write_bounds = b;
+    if (ln.prop == "write" && to_bool(ln.value, b)) r.write = b;
+    if (ln.prop == "use_output_folder" && to_bool(ln.value, b)) out.use_output_folder = b;
+    if (ln.prop == "header_separator")
+      r.sep = ln.value == "classic" ? ":" : (ln.value.size() == 1 ? ln.value : "_");
+  }
+  out.pro.append = false;  // a profile file is always rewritten
+  for (auto &ln : p.lines) {
+    if (ln.key != "kal" && ln.key != "pro") continue;
+    OutputRequest &r = ln.key == "kal" ? out.kal : out.pro;
+    if (!r.present) continue;
+    const std::string &k = ln.name;
+    if (ln.prop == "*" && (k == "mol" || k == "mon")) {
+      if (k == "mol")
+        for (auto &m : s.mol) r.items.push_back({k, m.name, ln.value});
+      else
+        for (auto &g : s.seg) r.items.push_back({k, g.name, ln.value});
+    } else {
+      r.items.push_back({k, ln.prop == "*" && k == "sys" ? s.sys_name : ln.prop, ln.value});
+    }
+  }
   return true;
 }
 

@@ -43,6 +43,7 @@ struct LatticeSpec {
   // bc[0..2] lower x,y,z; bc[3..5] upper x,y,z (reference BC[6] order)
   int bc[6] = {BC_MIRROR, BC_MIRROR, BC_MIRROR, BC_MIRROR, BC_MIRROR, BC_MIRROR};
   double offset_first_layer = 0.0;  // in grid units (times fjc)
+  double bond_length = 0.0;         // `bondlength` (m), 0 when not given
   bool stencil_full = true;
   // derived
   int M = 0, JX = 0, JY = 0, JZ = 1;
@@ -107,6 +108,30 @@ struct SolverSpec {
   bool max_g = false;          // stop on max|g| instead of ||g||_2
 };
 
+// One requested output quantity: `<type> : key : name : prop`, e.g.
+// `kal : mol : pol : GN` (key sys / mol / mon / newton / lat).
+struct OutputItem {
+  std::string key, name, prop;
+};
+
+// Requests of one output type (Output::CheckInput / Load,
+// src/output.cpp:32-178): present when an `output : <type> : ...` line exists.
+struct OutputRequest {
+  bool present = false;
+  bool write = true;
+  bool append = false;         // kal: rows of later starts are always appended
+  bool append_given = false;
+  bool write_bounds = false;   // pro: include the ghost layers
+  std::string sep = "_";       // header_separator (`classic` gives ':')
+  std::vector<OutputItem> items;
+};
+
+struct OutputSpec {
+  OutputRequest kal, pro;
+  bool use_output_folder = true;     // strip directories from the file stem
+  std::string path = "./output/";    // `out_info : folder : path : ...`
+};
+
 struct SystemSpec {
   std::string sys_name = "NN";
   std::string newton_name = "NN";
@@ -119,6 +144,13 @@ struct SystemSpec {
   std::vector<int> ksam;       // M: 1 on accessible sites, 0 on frozen sites
   std::string initial_guess = "previous_result";
   std::string guess_inputfile;
+  std::string final_guess = "next_problem";
+  std::string guess_outputfile;
+  int start = 1;               // the start block this system was built for
+  OutputSpec out;
+  std::string lattice_type_name, method_name = "pseudohessian", stop_criterion_name = "norm_of_g";
+  std::vector<std::string> mol_composition, seg_freedom_name, mol_freedom_name;
+  std::vector<double> seg_valence;
   int nvar() const { return (int)itmon.size() * lat.M; }
   int max_chain() const {
     int n = 0;

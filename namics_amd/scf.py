@@ -112,6 +112,11 @@ def lib():
         L.scf_system_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(TuneReport)]
         L.scf_system_layout.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_int)]
+        L.scf_write_kal.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(SolveInfo)]
+        L.scf_write_pro.argtypes = [vp, ctypes.c_char_p]
+        L.scf_store_guess.argtypes = [vp, ctypes.c_char_p]
+        L.scf_read_guess.argtypes = [vp, ctypes.c_char_p]
+        L.scf_run_input_file.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.scf_autotune_batch.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(BatchTuneReport)]
         L.scf_autotune_forget.argtypes = []
@@ -269,6 +274,28 @@ class ScfSystem:
         """Replay the grid-path iteration from a captured hipGraph (default)
         or launch its kernels one by one."""
         _check(lib().scf_system_set_graph(self.h, 1 if on else 0))
+
+    def write_kal(self, path, append=True, iterations=None, residual=None):
+        """Append the kal row the input requests (reference format)."""
+        info = None
+        if iterations is not None or residual is not None:
+            info = SolveInfo()
+            info.iterations = int(iterations or 0)
+            info.residual = float(residual if residual is not None else 0.0)
+        _check(lib().scf_write_kal(self.h, str(path).encode(), 1 if append else 0,
+                                   ctypes.byref(info) if info is not None else None))
+
+    def write_pro(self, path):
+        """Write the profiles the input requests (reference format)."""
+        _check(lib().scf_write_pro(self.h, str(path).encode()))
+
+    def store_guess(self, path):
+        """Write the current fields as a reference guess file."""
+        _check(lib().scf_store_guess(self.h, str(path).encode()))
+
+    def read_guess(self, path):
+        """Upload the field blocks a reference guess file holds."""
+        _check(lib().scf_read_guess(self.h, str(path).encode()))
 
     def phase_ms(self):
         """Device time of the last fused residual / solve by phase (ms):
@@ -438,6 +465,15 @@ def solve_resident(systems, with_solve_ms=False):
     if with_solve_ms:
         return [(i.iterations, i.residual, i.status, i.kernel_ms, i.solve_ms) for i in infos]
     return [(i.iterations, i.residual, i.status, i.kernel_ms) for i in infos]
+
+
+def run_input_file(path):
+    """Run every start block of a reference input file (guesses, solves,
+    kal / pro / guess files in the output folder, relative to the working
+    directory).  Returns (start blocks, converged solves)."""
+    n, c = ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().scf_run_input_file(str(path).encode(), ctypes.byref(n), ctypes.byref(c)))
+    return n.value, c.value
 
 
 def autotune_batch(systems, evals=3):
