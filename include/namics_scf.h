@@ -211,9 +211,12 @@ int scf_read_guess(scf_system *sys, const char *path);
  * guess (file, previous_result or none), solve with the input's method, kal /
  * pro / final-guess files in the output folder (./output/ unless
  * `out_info : folder : path` is given, relative to the working directory).
- * Replaces: the start loop of main (src/namics.cpp:163-737) for SCF inputs
- * without `var` scans. */
-int scf_run_input_file(const char *path, int *n_starts, int *n_converged);
+ * A `var : <kind>-<name> : scan : <property>` block (mol theta, n, phibulk;
+ * mon chi_X) solves every scan point in turn, each from the previous one.
+ * *n_solves counts the solves (start blocks times scan points), *n_converged
+ * those that converged.  Replaces: the start and scan loops of main
+ * (src/namics.cpp:163-737, Variate::PutVarScan src/variate.cpp:361-400). */
+int scf_run_input_file(const char *path, int *n_solves, int *n_converged);
 
 /* Override solver settings: tolerance, deltamax, DIIS history m, iteration
  * limit (values <= 0 keep the input's setting). */

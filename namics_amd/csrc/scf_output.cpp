@@ -362,54 +362,81 @@ int scf_run_input_file(const char *path, int *n_starts_out, int *n_converged_out
   const std::string text = ss.str();
   int n_starts = count_starts(text);
   if (n_starts < 1) n_starts = 1;
-  int converged = 0;
+  int converged = 0, solves = 0;
   scf_system *prev = nullptr;
   int rc = SCF_OK;
   bool kal_started = false;
   for (int start = 1; start <= n_starts && rc == SCF_OK; start++) {
-    scf_system *S = nullptr;
-    char err[512] = {0};
-    rc = scf_system_create(text.c_str(), start, &S, err, sizeof(err));
-    if (rc != SCF_OK) break;
-    const SystemSpec &s = S->spec;
-    // initial guess (namics main loop, src/namics.cpp:419-466,530-570)
-    if (s.initial_guess == "file") {
-      rc = scf_read_guess(S, s.guess_inputfile.c_str());
-    } else if (s.initial_guess == "previous_result" && prev) {
-      rc = scf_carry_fields(S, prev);
+    // the start block as written, and its var scan (points 0 .. num_of_steps)
+    SystemSpec spec0;
+    std::string why;
+    if (!parse_input(text, start, spec0, why)) {
+      rc = fail(SCF_E_PARSE, why);
+      break;
     }
-    scf_solve_info info{};
-    if (rc == SCF_OK) rc = scf_solve_resident(&S, 1, &info);
-    if (rc == SCF_OK && info.status == SCF_OK) converged++;
-    const std::string stem = output_stem(path, s.out.use_output_folder);
-    if (rc == SCF_OK && s.out.kal.present && s.out.kal.write) {
-      // the first kal request of a run appends only when asked (or when the
-      // request sets no append option); later starts always append
-      const bool append = kal_started || s.out.kal.append || !s.out.kal.append_given;
-      rc = scf_write_kal(S, (s.out.path + stem + ".kal").c_str(), append ? 1 : 0, &info);
-      kal_started = true;
-    }
-    if (rc == SCF_OK && s.out.pro.present && s.out.pro.write) {
-      const std::string name = n_starts == 1 ? stem + ".pro" : stem + "_" + std::to_string(start) + ".pro";
-      rc = scf_write_pro(S, (s.out.path + name).c_str());
-    }
-    if (rc == SCF_OK && s.final_guess == "file") {
-      std::string name;
-      if (s.guess_outputfile.empty()) {
-        name = stem + "_" + std::to_string(start) + ".outiv";
-      } else {
-        const std::string &g = s.guess_outputfile;
-        const size_t d = g.find('.');
-        const std::string ext = d == std::string::npos ? "" : g.substr(d + 1, g.find('.', d + 1) - d - 1);
-        name = g.substr(0, d) + "_" + std::to_string(start) + "." + ext;
+    const VarScan &vs = spec0.scan;
+    const int nsub = vs.present ? vs.num_of_steps : 0;
+    for (int subl = 0; subl <= nsub && rc == SCF_OK; subl++) {
+      std::string t = text;
+      if (vs.present) {
+        char num[64];
+        std::snprintf(num, sizeof(num), "%.17g", scan_value(vs, subl));
+        if (vs.kind == "mol") {
+          t = insert_before_start(t, start, "mol : " + vs.name + " : " + vs.prop + " : " + num);
+        } else {
+          const std::string other = vs.prop.substr(4);
+          t = insert_before_start(t, start, "mon : " + vs.name + " : chi_" + other + " : " + num);
+          t = insert_before_start(t, start, "mon : " + other + " : chi_" + vs.name + " : " + num);
+        }
       }
-      rc = scf_store_guess(S, (s.out.path + name).c_str());
+      scf_system *S = nullptr;
+      char err[512] = {0};
+      rc = scf_system_create(t.c_str(), start, &S, err, sizeof(err));
+      if (rc != SCF_OK) break;
+      const SystemSpec &s = S->spec;
+      // initial guess (namics main loop, src/namics.cpp:419-466,530-570):
+      // the guess file is read once per start block; every later solve
+      // starts from the one before it
+      if (s.initial_guess == "file" && subl == 0) {
+        rc = scf_read_guess(S, s.guess_inputfile.c_str());
+      } else if (s.initial_guess != "none" && prev) {
+        rc = scf_carry_fields(S, prev);
+      }
+      scf_solve_info info{};
+      if (rc == SCF_OK) rc = scf_solve_resident(&S, 1, &info);
+      solves++;
+      if (rc == SCF_OK && info.status == SCF_OK) converged++;
+      const std::string stem = output_stem(path, s.out.use_output_folder);
+      const std::string tag = (n_starts > 1 ? "_" + std::to_string(start) : std::string()) +
+                              (subl > 0 ? "_" + std::to_string(subl) : std::string());
+      if (rc == SCF_OK && s.out.kal.present && s.out.kal.write) {
+        // the first kal row of a run appends only when asked (or when the
+        // request sets no append option); every later row appends
+        const bool append = kal_started || s.out.kal.append || !s.out.kal.append_given;
+        rc = scf_write_kal(S, (s.out.path + stem + ".kal").c_str(), append ? 1 : 0, &info);
+        kal_started = true;
+      }
+      if (rc == SCF_OK && s.out.pro.present && s.out.pro.write)
+        rc = scf_write_pro(S, (s.out.path + stem + tag + ".pro").c_str());
+      if (rc == SCF_OK && s.final_guess == "file") {
+        std::string name;
+        const std::string gtag = "_" + std::to_string(start) + (subl > 0 ? "_" + std::to_string(subl) : std::string());
+        if (s.guess_outputfile.empty()) {
+          name = stem + gtag + ".outiv";
+        } else {
+          const std::string &g = s.guess_outputfile;
+          const size_t d = g.find('.');
+          const std::string ext = d == std::string::npos ? "" : g.substr(d + 1, g.find('.', d + 1) - d - 1);
+          name = g.substr(0, d) + gtag + "." + ext;
+        }
+        rc = scf_store_guess(S, (s.out.path + name).c_str());
+      }
+      if (prev) scf_system_destroy(prev);
+      prev = S;
     }
-    if (prev) scf_system_destroy(prev);
-    prev = S;
   }
   if (prev) scf_system_destroy(prev);
-  if (n_starts_out) *n_starts_out = n_starts;
+  if (n_starts_out) *n_starts_out = solves;
   if (n_converged_out) *n_converged_out = converged;
   return rc;
 }

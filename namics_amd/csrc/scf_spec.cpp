@@ -519,6 +519,34 @@ int count_starts(const std::string &text) {
   return n;
 }
 
+std::string insert_before_start(const std::string &text, int start, const std::string &line) {
+  std::istringstream is(text);
+  std::string raw, out;
+  int n = 0;
+  bool done = false;
+  while (std::getline(is, raw)) {
+    std::string t;
+    for (char ch : raw)
+      if (ch != ' ' && ch != '\t' && ch != '\r') t.push_back(ch);
+    if (!done && t.compare(0, 5, "start") == 0 && ++n == start) {
+      out += line + "\n";
+      done = true;
+    }
+    out += raw + "\n";
+  }
+  if (!done) out += line + "\nstart\n";
+  return out;
+}
+
+double scan_value(const VarScan &v, int k) {
+  if (v.exponential) {
+    const double f = 1.0 * k / v.num_of_steps;
+    const double sgn = v.start_value < 0 ? -1.0 : 1.0;
+    return sgn * std::pow(10.0, (1 - f) * std::log10(sgn * v.start_value) + f * std::log10(sgn * v.end_value));
+  }
+  return v.start_value + k * v.step;
+}
+
 bool parse_input(const std::string &text, int start, SystemSpec &s, std::string &err) {
   Parsed p;
   if (start < 1) start = 1;
@@ -1035,6 +1063,73 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
   if (get(p, "sys", p.sys_name, "final_guess", v) && (v == "next_problem" || v == "file")) s.final_guess = v;
   if (get(p, "sys", p.sys_name, "guess_outputfile", v)) s.guess_outputfile = v;
   s.start = start;
+  // var scan (Variate::CheckInput, src/variate.cpp:64-345; the scan counts
+  // of Molecule::PutVarScan / Segment::PutVarScan, src/molecule.cpp:785-816,
+  // src/segment.cpp:1402-1438, truncated to an integer as there)
+  for (auto &ln : p.lines) {
+    if (ln.key != "var" || ln.prop != "scan" || s.scan.present) continue;
+    VarScan &vs = s.scan;
+    const size_t dash = ln.name.find('-');
+    if (dash == std::string::npos) {
+      err = "var '" + ln.name + "': expected <kind>-<name>";
+      return false;
+    }
+    vs.kind = ln.name.substr(0, dash);
+    vs.name = ln.name.substr(dash + 1);
+    vs.prop = ln.value;
+    bool found = false;
+    if (vs.kind == "mol") {
+      for (auto &m : s.mol)
+        if (m.name == vs.name && (vs.prop == "theta" || vs.prop == "n" || vs.prop == "phibulk")) {
+          found = true;
+          vs.start_value = vs.prop == "theta" ? m.theta : (vs.prop == "n" ? m.n : m.phibulk);
+        }
+    } else if (vs.kind == "mon" && vs.prop.compare(0, 4, "chi_") == 0) {
+      const std::string other = vs.prop.substr(4);
+      for (size_t i = 0; i < s.seg.size(); i++)
+        if (s.seg[i].name == vs.name)
+          for (size_t j = 0; j < s.seg.size(); j++)
+            if (s.seg[j].name == other) {
+              found = true;
+              vs.start_value = s.seg[i].chi[j];
+            }
+    }
+    if (!found) {
+      err = "var scan of " + ln.name + " : " + vs.prop + " is not supported (mol theta, n, phibulk; mon chi_X)";
+      return false;
+    }
+    std::string sv;
+    if (!get(p, "var", ln.name, "end_value", sv) || !to_real(sv, vs.end_value)) {
+      err = "var scan needs an end_value";
+      return false;
+    }
+    vs.exponential = get(p, "var", ln.name, "scale", sv) && sv == "exponential";
+    if (vs.exponential) {
+      if (vs.kind == "mon") {
+        err = "a chi scan is linear only";
+        return false;
+      }
+      if (get(p, "var", ln.name, "steps", sv) && !to_int(sv, vs.steps)) vs.steps = 1;
+      if (vs.steps < 1 || vs.end_value * vs.start_value <= 0) {
+        err = "exponential var scan needs steps >= 1 and end_value, start value of one sign";
+        return false;
+      }
+      const double r = vs.end_value > vs.start_value ? vs.end_value / vs.start_value : vs.start_value / vs.end_value;
+      vs.num_of_steps = (int)(vs.steps * std::log10(r));
+    } else {
+      if (!get(p, "var", ln.name, "step", sv) || !to_real(sv, vs.step) || vs.step == 0) {
+        err = "linear var scan needs a nonzero step";
+        return false;
+      }
+      const double q = (vs.end_value - vs.start_value) / vs.step;
+      if (q < 0) {
+        err = "var scan: (end_value - start value) / step is negative";
+        return false;
+      }
+      vs.num_of_steps = (int)q;
+    }
+    vs.present = true;
+  }
   // output requests (Output::CheckInput / Load, src/output.cpp:32-178):
   // `output : kal | pro : option : value` and `kal | pro : key : name : prop`
   // item lines in input order; name `*` expands to every molecule / segment

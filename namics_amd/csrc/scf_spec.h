@@ -132,6 +132,19 @@ struct OutputSpec {
   std::string path = "./output/";    // `out_info : folder : path : ...`
 };
 
+// A `var : <kind>-<name> : scan : <property>` request (Variate::CheckInput,
+// src/variate.cpp:64-345): the property is stepped from its value in the
+// input to end_value, linearly by `step` or exponentially with `steps` per
+// decade.
+struct VarScan {
+  bool present = false;
+  std::string kind, name, prop;  // kind mol (theta, n, phibulk) or mon (chi_X)
+  double start_value = 0, end_value = 0, step = 0;
+  int steps = 1;
+  bool exponential = false;
+  int num_of_steps = 0;          // scan points after the first
+};
+
 struct SystemSpec {
   std::string sys_name = "NN";
   std::string newton_name = "NN";
@@ -148,6 +161,7 @@ struct SystemSpec {
   std::string guess_outputfile;
   int start = 1;               // the start block this system was built for
   OutputSpec out;
+  VarScan scan;
   std::string lattice_type_name, method_name = "pseudohessian", stop_criterion_name = "norm_of_g";
   std::vector<std::string> mol_composition, seg_freedom_name, mol_freedom_name;
   std::vector<double> seg_valence;
@@ -165,6 +179,14 @@ bool parse_input(const std::string &text, int start, SystemSpec &out, std::strin
 
 // Number of `start` blocks in the text.
 int count_starts(const std::string &text);
+
+// The text with `line` inserted just before the start-th `start` line, so it
+// overrides the settings of that block only from there on.
+std::string insert_before_start(const std::string &text, int start, const std::string &line);
+
+// Value of the scanned property at scan point k (Molecule::UpdateVarInfo,
+// Segment::UpdateVarInfo, src/molecule.cpp:884-934, src/segment.cpp:1440-1480).
+double scan_value(const VarScan &v, int k);
 
 // Read a reference guess file (Lattice::ReadGuess format, src/lattice.cpp:926-968)
 // and map its field blocks onto x by segment name (Solve_scf::Guess,

@@ -470,7 +470,8 @@ def solve_resident(systems, with_solve_ms=False):
 def run_input_file(path):
     """Run every start block of a reference input file (guesses, solves,
     kal / pro / guess files in the output folder, relative to the working
-    directory).  Returns (start blocks, converged solves)."""
+    directory); `var` scans solve every scan point.  Returns (solves,
+    converged solves)."""
     n, c = ctypes.c_int(0), ctypes.c_int(0)
     _check(lib().scf_run_input_file(str(path).encode(), ctypes.byref(n), ctypes.byref(c)))
     return n.value, c.value

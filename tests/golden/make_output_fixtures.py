@@ -96,6 +96,29 @@ output : pro : append : false
 pro : mon : A : phi
 pro : mol : brush : phi
 """ + DIIS + "start\n",
+    # var scans: phibulk exponentially (2 points per decade), chi linearly
+    # ((0.8 - 0.5) / 0.1 truncates to 2 in floating point: 3 points)
+    "scan_phibulk": CASES["cfg1_planar_adsorption"]["text"] + """output : kal : append : false
+kal : sys : NN : free_energy
+kal : mol : pol : phibulk
+kal : mol : pol : theta_exc
+kal : newton : isaac : iterations
+output : pro : append : false
+pro : mol : pol : phi
+var : mol-pol : scan : phibulk
+var : mol-pol : scale : exponential
+var : mol-pol : steps : 2
+var : mol-pol : end_value : 0.1
+""" + DIIS + "start\n",
+    "scan_chi": CASES["sph1d_particle"]["text"] + """output : kal : append : false
+kal : sys : NN : free_energy
+kal : sys : NN : grand_potential
+kal : mol : pol : theta
+var : mon-A : scan : chi_W
+var : mon-A : step : 0.1
+var : mon-A : end_value : 0.7
+sys : NN : final_guess : file
+""" + DIIS + "start\n",
     "box3d": CASES["diblock3d_periodic"]["text"] + """output : kal : append : false
 kal : sys : NN : free_energy
 kal : mol : surf : GN
