@@ -201,6 +201,9 @@ def run_brush(args):
         out["roofline"]["traffic"] = traffic
         out["roofline"]["traffic_source"] = src + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)"
         out["roofline"]["hbm_gbs_measured_traffic"] = traffic / (launch_ms * 1e-3) / 1e9
+        # measured HBM bytes over the launch against the peak (frac counts the
+        # algorithmic bytes, most of which stay on chip)
+        out["roofline"]["frac_measured_traffic"] = out["roofline"]["hbm_gbs_measured_traffic"] / HBM_PEAK_GBS
     # the same jobs with FP32 propagators and FP64 refinement
     mixed_ok = not args.no_mixed and slots[0][0].layout()["strip_sx"] > 0
     if not args.no_mixed and not mixed_ok:

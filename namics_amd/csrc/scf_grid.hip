@@ -55,24 +55,45 @@ __device__ __forceinline__ double ordered_max(const double *partial, int nparts,
   return block_max(t, red);
 }
 
-// ordered_sum of columns 0..ncols-1 of partial[nparts][kMaxDIIS] in one
-// pass (each column summed in the same order as ordered_sum); the sums land
-// in out[] and are visible to the workgroup on return.
+// Column sums of partial[nparts][kMaxDIIS], columns 0..ncols-1, in one pass:
+// thread t adds rows t, t + blockDim.x, ... in order (loads of two rows in
+// flight), the waves reduce by lane exchange, and the per-wave sums of all
+// columns meet in LDS behind one barrier (red needs kMaxDIIS * waves
+// doubles).  The sums land in out[] and are visible to the workgroup on
+// return.
 __device__ __forceinline__ void ordered_sums(const double *partial, int nparts, int ncols, double *out,
                                              double *red) {
   double acc[kMaxDIIS];
 #pragma unroll
   for (int i = 0; i < kMaxDIIS; i++) acc[i] = 0;
-  for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+  int b = threadIdx.x;
+  for (; b + (int)blockDim.x < nparts; b += 2 * blockDim.x) {
+    double a0[kMaxDIIS], a1[kMaxDIIS];
+#pragma unroll
+    for (int i = 0; i < kMaxDIIS; i++) {
+      a0[i] = i < ncols ? partial[(size_t)b * kMaxDIIS + i] : 0.0;
+      a1[i] = i < ncols ? partial[(size_t)(b + blockDim.x) * kMaxDIIS + i] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxDIIS; i++) acc[i] = (acc[i] + a0[i]) + a1[i];
+  }
+  if (b < nparts)
 #pragma unroll
     for (int i = 0; i < kMaxDIIS; i++)
       if (i < ncols) acc[i] += partial[(size_t)b * kMaxDIIS + i];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
-  for (int i = 0; i < kMaxDIIS; i++)
-    if (i < ncols) {
-      const double t = block_sum(acc[i], red);
-      if (threadIdx.x == 0) out[i] = t;
-    }
+  for (int i = 0; i < kMaxDIIS; i++) {
+    double v = acc[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[i * nw + wave] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < ncols) {
+    double t = 0;
+    for (int w = 0; w < nw; w++) t += red[threadIdx.x * nw + w];
+    out[threadIdx.x] = t;
+  }
   __syncthreads();
 }
 
@@ -389,7 +410,7 @@ __global__ void __launch_bounds__(kGridThreads) k_diis_solve(const DevSys *__res
                                                               int nparts, int k_diis, int kd) {
   __shared__ double dots[kMaxDIIS];
   __shared__ double Ap[kMaxDIIS * kMaxDIIS];
-  __shared__ double red[kGridThreads / 64];
+  __shared__ double red[kMaxDIIS * (kGridThreads / 64)];
   const DevSys &S = *d;
   DevScalars &sc = *S.sc;
   const int st = sc.status;
@@ -488,7 +509,7 @@ __global__ void __launch_bounds__(kGridThreads) k_diis_solve_dev(const DevSys *_
                                                                   const double *partial, int nparts) {
   __shared__ double dots[kMaxDIIS];
   __shared__ double Ap[kMaxDIIS * kMaxDIIS];
-  __shared__ double red[kGridThreads / 64];
+  __shared__ double red[kMaxDIIS * (kGridThreads / 64)];
   const DevSys &S = *d;
   DevScalars &sc = *S.sc;
   if (sc.status != DS_OK) return;
