@@ -279,8 +279,9 @@ def main():
     ap.add_argument("--seed", type=int, default=7, help="serve: request-stream seed")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(independent systems shard trivially across GPUs; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(independent systems shard across GPUs with no communication; run one process per GPU)",
+              file=sys.stderr)
         sys.exit(2)
     if args.workload == "brush":
         out = run_brush(args)
