@@ -189,6 +189,12 @@ def run_brush(args):
                        "what": "device time of one solve inside the persistent launch"},
         "iterations_p50_p99": [float(np.percentile([r["iterations"] for r in solves], q)) for q in (50, 99)],
         "converged_fraction": converged / max(1, len(solves)),
+        # device time per DIIS iteration by phase, over every solve of the
+        # timed steps (device clock inside the persistent launch)
+        "phase_ms_per_iteration": dict(zip(
+            ("prepare", "chains", "densities_residual", "diis_other"),
+            (np.sum([r["phase_ms"] for r in solves], axis=0) /
+             max(1, sum(r["iterations"] + 1 for r in solves))).tolist())),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_fused_jobs (one launch per step)", "avg_launch_ms": launch_ms,
