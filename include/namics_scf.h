@@ -200,6 +200,10 @@ int scf_write_kal(scf_system *sys, const char *path, int append, const scf_solve
  * Lattice::PutProfiles (src/output.cpp:487-510, src/LGrad1.cpp:228-249,
  * src/LGrad2.cpp:185-205, src/LGrad3.cpp:81-102). */
 int scf_write_pro(scf_system *sys, const char *path);
+/* Write the first requested `vtk` profile (2 and 3 gradients; the reference
+ * writes no 1-gradient vtk file).  Replaces: LGrad2::vtk / LGrad3::vtk
+ * (src/LGrad2.cpp:159-183, src/LGrad3.cpp:48-79). */
+int scf_write_vtk(scf_system *sys, const char *path);
 /* Write the system's fields as a guess file.  Replaces: Lattice::StoreGuess
  * (src/lattice.cpp:971-1019). */
 int scf_store_guess(scf_system *sys, const char *path);

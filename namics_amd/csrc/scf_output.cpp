@@ -321,6 +321,59 @@ int scf_write_pro(scf_system *S, const char *path) {
   return SCF_OK;
 }
 
+int scf_write_vtk(scf_system *S, const char *path) {
+  if (!S || !path) return fail(SCF_E_ARG, "null argument");
+  const SystemSpec &s = S->spec;
+  const LatticeSpec &l = s.lat;
+  const OutputRequest &req = s.out.vtk;
+  if (l.gradients == 1 || req.items.empty()) return SCF_OK;  // the reference writes no 1-gradient vtk file
+  State st;
+  int rc = load_state(S, st);
+  if (rc != SCF_OK) return rc;
+  // one member: the first requested profile (Output::WriteOutput src/output.cpp:560-571)
+  const OutputItem &it = req.items[0];
+  const double *src = nullptr;
+  if (it.prop == "phi" && it.key == "mon")
+    for (size_t k = 0; k < s.seg.size(); k++)
+      if (s.seg[k].name == it.name) src = st.seg_phi.data() + k * (size_t)l.M;
+  if (it.prop == "phi" && it.key == "mol")
+    for (size_t i = 0; i < s.mol.size(); i++)
+      if (s.mol[i].name == it.name) src = st.mol_phi.data() + i * (size_t)l.M;
+  if (!src) return SCF_OK;  // not a profile: no file, as in the reference
+  std::vector<double> X(src, src + l.M);
+  if (req.write_bounds) set_ghosts(l, X);
+  FILE *fp = std::fopen(path, "w+");
+  if (!fp) return fail(SCF_E_ARG, std::string("cannot open ") + path);
+  const int f = l.fjc;
+  const std::string id = header_name(it, req.sep);
+  // LGrad2::vtk / LGrad3::vtk (src/LGrad2.cpp:159-183, src/LGrad3.cpp:48-79)
+  if (l.gradients == 2)
+    std::fprintf(fp, "# vtk DataFile Version 3.0\nvtk output\nASCII\nDATASET STRUCTURED_POINTS\nDIMENSIONS %i %i %i\n",
+                 l.MY, l.MX, 1);
+  else
+    std::fprintf(fp, "# vtk DataFile Version 3.0\nvtk output\nASCII\nDATASET STRUCTURED_POINTS\nDIMENSIONS %i %i %i\n",
+                 l.MZ, l.MY, l.MX);
+  const int npts = req.write_bounds ? l.M : l.MX * l.MY * (l.gradients == 3 ? l.MZ : 1);
+  std::fprintf(fp, "SPACING 1 1 1\nORIGIN 0 0 0\nPOINT_DATA %i\n", npts);
+  std::fprintf(fp, "SCALARS %s double\nLOOKUP_TABLE default\n", id.c_str());
+  if (l.gradients == 2) {
+    // with write_bounds the 2-gradient writer prints every site and then
+    // the interior again, as the reference does
+    if (req.write_bounds)
+      for (int i = 0; i < l.M; i++) std::fprintf(fp, "%f\n", X[i]);
+    for (int x = f; x < l.MX + f; x++)
+      for (int y = f; y < l.MY + f; y++) std::fprintf(fp, "%f\n", X[(size_t)x * l.JX + y]);
+  } else if (req.write_bounds) {
+    for (int i = 0; i < l.M; i++) std::fprintf(fp, "%f\n", X[i]);
+  } else {
+    for (int x = f; x < l.MX + f; x++)
+      for (int y = f; y < l.MY + f; y++)
+        for (int z = f; z < l.MZ + f; z++) std::fprintf(fp, "%e\n", X[(size_t)x * l.JX + (size_t)y * l.JY + z]);
+  }
+  std::fclose(fp);
+  return SCF_OK;
+}
+
 int scf_store_guess(scf_system *S, const char *path) {
   if (!S || !path) return fail(SCF_E_ARG, "null argument");
   const SystemSpec &s = S->spec;
@@ -418,6 +471,8 @@ int scf_run_input_file(const char *path, int *n_starts_out, int *n_converged_out
       }
       if (rc == SCF_OK && s.out.pro.present && s.out.pro.write)
         rc = scf_write_pro(S, (s.out.path + stem + tag + ".pro").c_str());
+      if (rc == SCF_OK && s.out.vtk.present && s.out.vtk.write)
+        rc = scf_write_vtk(S, (s.out.path + stem + tag + ".vtk").c_str());
       if (rc == SCF_OK && s.final_guess == "file") {
         std::string name;
         const std::string gtag = "_" + std::to_string(start) + (subl > 0 ? "_" + std::to_string(subl) : std::string());

@@ -1137,8 +1137,8 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
   for (auto &ln : p.lines) {
     if (ln.key == "out_info" && ln.name == "folder" && ln.prop == "path" && !ln.value.empty())
       out.path = ln.value.back() == '/' ? ln.value : ln.value + "/";
-    if (ln.key != "output" || (ln.name != "kal" && ln.name != "pro")) continue;
-    OutputRequest &r = ln.name == "kal" ? out.kal : out.pro;
+    if (ln.key != "output" || (ln.name != "kal" && ln.name != "pro" && ln.name != "vtk")) continue;
+    OutputRequest &r = ln.name == "kal" ? out.kal : (ln.name == "pro" ? out.pro : out.vtk);
     r.present = true;
     bool b;
     if (ln.prop == "append" && to_bool(ln.value, b)) {
@@ -1152,9 +1152,10 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       r.sep = ln.value == "classic" ? ":" : (ln.value.size() == 1 ? ln.value : "_");
   }
   out.pro.append = false;  // a profile file is always rewritten
+  out.vtk.append = false;
   for (auto &ln : p.lines) {
-    if (ln.key != "kal" && ln.key != "pro") continue;
-    OutputRequest &r = ln.key == "kal" ? out.kal : out.pro;
+    if (ln.key != "kal" && ln.key != "pro" && ln.key != "vtk") continue;
+    OutputRequest &r = ln.key == "kal" ? out.kal : (ln.key == "pro" ? out.pro : out.vtk);
     if (!r.present) continue;
     const std::string &k = ln.name;
     if (ln.prop == "*" && (k == "mol" || k == "mon")) {

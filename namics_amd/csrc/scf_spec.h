@@ -127,7 +127,7 @@ struct OutputRequest {
 };
 
 struct OutputSpec {
-  OutputRequest kal, pro;
+  OutputRequest kal, pro, vtk;
   bool use_output_folder = true;     // strip directories from the file stem
   std::string path = "./output/";    // `out_info : folder : path : ...`
 };

@@ -95,6 +95,8 @@ kal : lat : cyl : n_layers_y
 output : pro : append : false
 pro : mon : A : phi
 pro : mol : brush : phi
+output : vtk : write_bounds : true
+vtk : mon : A : phi
 """ + DIIS + "start\n",
     # var scans: phibulk exponentially (2 points per decade), chi linearly
     # ((0.8 - 0.5) / 0.1 truncates to 2 in floating point: 3 points)
@@ -127,6 +129,8 @@ kal : mon : B : theta
 output : pro : append : false
 pro : mon : B : phi
 pro : mol : water : phi
+output : vtk : append : false
+vtk : mol : surf : phi
 """ + DIIS + "start\n",
 }
 

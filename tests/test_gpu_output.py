@@ -3,7 +3,8 @@
 Each fixture input (tests/golden/make_output_fixtures.py) was run by the
 reference program; here the same input file is run by scf_run_input_file in
 a scratch directory and every file the reference wrote is compared with the
-one written here: the same file names, the same header lines and row counts,
+one written here (kal, pro, vtk and guess files): the same file names, the
+same header lines and row counts,
 the same coordinate columns (text), the same strings, integers (iteration
 counts within the drift DIIS shows between builds) and "NiN" cells, and
 numbers within what the solves' tolerance (DIIS, 1e-9) fixes.  Guess files
@@ -85,6 +86,13 @@ def test_output_files_match_the_reference(name, tmp_path, monkeypatch):
         elif fn.endswith(".pro"):
             g = 1 + ref.split("\n")[0].count("\ty\t") + ref.split("\n")[0].count("\tz\t")
             _compare_table(ours, ref, rtol=0.0, atol=1e-7, ncoord=g, scale_cols=True)
+        elif fn.endswith(".vtk"):  # 10 header lines, then %f / %e values
+            lo, lr = ours.split("\n"), ref.split("\n")
+            assert lo[:10] == lr[:10]
+            vo = np.array([float(v) for v in lo[10:] if v])
+            vr = np.array([float(v) for v in lr[10:] if v])
+            assert vo.shape == vr.shape
+            assert np.all(np.abs(vo - vr) <= 2e-6 * np.abs(vr) + 1.5e-6)
         else:  # guess file: header lines, then %e values
             lo, lr = ours.split("\n"), ref.split("\n")
             k = 4 + int(lr[3]) + 1
