@@ -799,10 +799,16 @@ __device__ __noinline__ void chain2d_strip(const DevSys &S0, int mo, double shif
   __syncthreads();
 }
 
-// Second-order Markov chain (1 gradient, three direction components), rows
-// of 3M in LDS (kLds) or HBM.
-template <bool kLds>
+// Second-order Markov chain, rows of C M values (C direction components) in
+// LDS (kLds) or HBM.  kKind picks the lattice at compile time, so each
+// instance holds only its own step and a component count fixed where it
+// can be: M2_SC1 1-gradient simple cubic (3 components), M2_HEX1
+// 1-gradient hexagonal (FJC components, at most kMaxComp), M2_D2
+// 2-gradient (5), M2_D3 3-gradient (6).
+enum M2Kind : int { M2_SC1 = 0, M2_HEX1 = 1, M2_D2 = 2, M2_D3 = 3 };
+template <bool kLds, int kKind>
 __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
+  constexpr int kC = kKind == M2_SC1 ? 3 : (kKind == M2_D2 ? 5 : (kKind == M2_D3 ? 6 : kMaxComp));
   typedef typename Rows<kLds>::T RowT;
   CSys &S = uniform_sys(S0);
   const int M = S.M, W = n_work_sites(S);
@@ -813,9 +819,9 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
   // the hexagonal one
   const int C = S.mol_size[mo];
   const int R = C * M;
-  const bool hex = S.lattice_type != 0;
-  const bool two = S.gradients == 2;    // 5 components: +x, +y, out of plane, -y, -x
-  const bool three = S.gradients == 3;  // 6 components: +x, +y, +z, -z, -y, -x
+  constexpr bool hex = kKind == M2_HEX1;
+  constexpr bool two = kKind == M2_D2;    // 5 components: +x, +y, out of plane, -y, -x
+  constexpr bool three = kKind == M2_D3;  // 6 components: +x, +y, +z, -z, -y, -x
   double P[kMaxComp];
 #pragma unroll
   for (int k = 0; k < kMaxComp; k++) P[k] = S.mol_P[mo][k];
@@ -845,7 +851,7 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
       int x, y, z;
       const int i = work_site(S, w, x, y, z);
       const double v = g1[i];
-      for (int c = 0; c < C; c++) {
+      for (int c = 0; c < kC && c < C; c++) {
         Gf[c * M + i] = v;
         if constexpr (kLds) ra[c * M + i] = v;
       }
@@ -862,18 +868,18 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
       for (int w = tid; w < W; w += nt) {
         int x, y, z;
         const int i = work_site(S, w, x, y, z);
-        double o[kMaxComp];
-        if (three)
+        double o[kC];
+        if constexpr (three)
           m3_step<double>(S, prev, P, true, x, y, z, o);
-        else if (two)
+        else if constexpr (two)
           m2d_step<double>(S, prev, P, true, i, x, y, o);
-        else if (hex)
+        else if constexpr (hex)
           m2h_step<double>(S, prev, P, true, i, o);
         else
           m2_step<double>(S, prev, P, true, i, o);
         const double wg = g1[i];
 #pragma unroll
-        for (int c = 0; c < kMaxComp; c++) {
+        for (int c = 0; c < kC; c++) {
           if (c >= C) break;
           const double v = o[c] * wg;
           gs[c * M + i] = v;
@@ -894,26 +900,26 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
   // (G0 + 4 G1 + G2) / 6, hexagonal (G0 + 2 sum G_inner + G_last) / (4 fjc)
   {
     const gdbl *gl = Gf + (size_t)(N - 1) * R;
-    double a[kMaxComp];
+    double a[kC];
 #pragma unroll
-    for (int c = 0; c < kMaxComp; c++) a[c] = 0;
+    for (int c = 0; c < kC; c++) a[c] = 0;
     for (int w = tid; w < W; w += nt) {
       int x, y, z;
       const int i = work_site(S, w, x, y, z);
       if (is_interior(S, x, y, z)) {
         const double wt = site_weight(S0, i);
 #pragma unroll
-        for (int c = 0; c < kMaxComp; c++)
+        for (int c = 0; c < kC; c++)
           if (c < C) a[c] += wt * gl[c * M + i];
       }
     }
     double gn = wg_sum(a[0]);
-    if (three) {
+    if constexpr (three) {
       // LGrad3::ComputeGN (src/LGrad3.cpp:1883-1895): the mean of the components
 #pragma unroll
       for (int c = 1; c < 6; c++) gn += wg_sum(a[c]);
       gn = gn / (1.0 * 6);
-    } else if (two) {
+    } else if constexpr (two) {
       // LGrad2::ComputeGN (src/LGrad2.cpp:1814-1840): (G0 + G1 + 2 G2 + G3 + G4) / 6
       gn += wg_sum(a[1]);
       gn += 2 * wg_sum(a[2]);
@@ -922,10 +928,10 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
     } else {
       const double inner = hex ? 2.0 : 4.0;
 #pragma unroll
-      for (int c = 1; c < kMaxComp - 1; c++)
+      for (int c = 1; c < kC - 1; c++)
         if (c < C - 1) gn += inner * wg_sum(a[c]);
 #pragma unroll
-      for (int c = 2; c < kMaxComp; c++)
+      for (int c = 2; c < kC; c++)
         if (c == C - 1) gn += wg_sum(a[c]);
     }
     if (threadIdx.x == 0) S0.sc->GN[mo] = three ? gn : (hex ? gn / (4.0 * S.fjc) : gn / 6.0);
@@ -949,7 +955,7 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
       const double v = g1[i];
       double r = rho[i];
 #pragma unroll
-      for (int c = 0; c < kMaxComp; c++) {
+      for (int c = 0; c < kC; c++) {
         if (c >= C) break;
         gbA[c * M + i] = v;
         r += wcomp(c) * gf[c * M + i] * v;
@@ -967,19 +973,19 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
       for (int w = tid; w < W; w += nt) {
         int x, y, z;
         const int i = work_site(S, w, x, y, z);
-        double o[kMaxComp];
-        if (three)
+        double o[kC];
+        if constexpr (three)
           m3_step<double>(S, gbA, P, false, x, y, z, o);
-        else if (two)
+        else if constexpr (two)
           m2d_step<double>(S, gbA, P, false, i, x, y, o);
-        else if (hex)
+        else if constexpr (hex)
           m2h_step<double>(S, gbA, P, false, i, o);
         else
           m2_step<double>(S, gbA, P, false, i, o);
         const double wg = g1[i];
         double r = rho[i];
 #pragma unroll
-        for (int c = 0; c < kMaxComp; c++) {
+        for (int c = 0; c < kC; c++) {
           if (c >= C) break;
           const double v = o[c] * wg;
           gbB[c * M + i] = v;
@@ -1063,10 +1069,26 @@ __device__ double fused_residual(const DevSys &S0) {
   const unsigned long long t1 = wall_clock64();
   for (int mo = 0; mo < S.nmol; mo++) {
     if (S.mol_markov[mo] == 2) {
-      if (S.use_lds)
-        chain_markov2<true>(S0, mo);
-      else
-        chain_markov2<false>(S0, mo);
+      const int kind = S.gradients == 3 ? M2_D3 : (S.gradients == 2 ? M2_D2 : (S.lattice_type != 0 ? M2_HEX1 : M2_SC1));
+      if (S.use_lds) {
+        if (kind == M2_SC1)
+          chain_markov2<true, M2_SC1>(S0, mo);
+        else if (kind == M2_HEX1)
+          chain_markov2<true, M2_HEX1>(S0, mo);
+        else if (kind == M2_D2)
+          chain_markov2<true, M2_D2>(S0, mo);
+        else
+          chain_markov2<true, M2_D3>(S0, mo);
+      } else {
+        if (kind == M2_SC1)
+          chain_markov2<false, M2_SC1>(S0, mo);
+        else if (kind == M2_HEX1)
+          chain_markov2<false, M2_HEX1>(S0, mo);
+        else if (kind == M2_D2)
+          chain_markov2<false, M2_D2>(S0, mo);
+        else
+          chain_markov2<false, M2_D3>(S0, mo);
+      }
     } else if (S.strip_sx > 0) {
       chain_strip_dispatch(S0, mo, fp32, fp32 ? sc.lnshift[mo] : 0.0);
     } else if (S.gradients == 1 && S.fjc == 1 && S.use_lds && S.M <= S.threads) {
