@@ -866,16 +866,22 @@ template <typename Sys>
 __device__ __forceinline__ double site_residual(const Sys &S, int i, int x, int y, int z) {
   const int M = S.M;
   const DevScalars &sc = *S.sc;
-  double side[kMaxSeg];
+  // every load first (site averages of all segments, fields, phitot, KSAM),
+  // then the stores: a store between them would keep the later loads from
+  // being issued early
+  double side[kMaxSeg], xin[kMaxSeg];
 #pragma unroll
   for (int k = 0; k < kMaxSeg; k++) {
     side[k] = 0.0;
-    if (k < S.nseg) {
-      side[k] = stencil_side(S, hbm(S.phi) + k * M, i, x, y, z);
-      hbm(S.phi_side)[k * M + i] = side[k];
-    }
+    if (k < S.nseg) side[k] = stencil_side(S, hbm(S.phi) + k * M, i, x, y, z);
   }
+#pragma unroll
+  for (int it = 0; it < kMaxSeg; it++) xin[it] = it < S.nit ? hbm(S.x)[it * M + i] : 0.0;
   const double pt = hbm(S.phitot)[i];
+  const double ks = hbm(S.ksam)[i];
+#pragma unroll
+  for (int k = 0; k < kMaxSeg; k++)
+    if (k < S.nseg) hbm(S.phi_side)[k * M + i] = side[k];
   double gv[kMaxSeg];
   double alpha = 0;
 #pragma unroll
@@ -883,7 +889,7 @@ __device__ __forceinline__ double site_residual(const Sys &S, int i, int x, int 
     gv[it] = 0.0;
     if (it < S.nit) {
       const int si = S.itmon[it];
-      double gi = hbm(S.x)[it * M + i];
+      double gi = xin[it];
 #pragma unroll
       for (int k = 0; k < kMaxSeg; k++) {
         if (k < S.nseg) {
@@ -898,7 +904,6 @@ __device__ __forceinline__ double site_residual(const Sys &S, int i, int x, int 
   alpha = alpha * (1.0 / S.nit);
   hbm(S.alpha)[i] = alpha;
   double g2 = 0;
-  const double ks = hbm(S.ksam)[i];
 #pragma unroll
   for (int it = 0; it < kMaxSeg; it++) {
     if (it < S.nit) {
