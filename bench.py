@@ -75,6 +75,25 @@ def profiled_traffic(jobs, slots, ckpt):
     return None, None
 
 
+def slot_occupancy(jobs, launch_ms, nslots):
+    """How full the persistent launch keeps its slots: busy share of
+    slots x launch time, the time at the end of the launch during which fewer
+    than a tenth of the slots still run a job, and the jobs whose chain has
+    more than one solve that stops at the iteration limit."""
+    spans = [(job[0]["start_ms"], max(r["start_ms"] + r["solve_ms"] for r in job)) for job in jobs]
+    end = max(e for _, e in spans)
+    events = sorted([(b, 1) for b, _ in spans] + [(e, -1) for _, e in spans])
+    few = max(1, nslots // 10)
+    active, drop = 0, 0.0
+    for t, d in events:
+        if active >= few and active + d < few:
+            drop = t
+        active += d
+    return {"busy_fraction": sum(r["solve_ms"] for job in jobs for r in job) / (nslots * launch_ms),
+            "tail_ms_below_10pct_slots": end - drop,
+            "jobs_with_2plus_unconverged": sum(1 for job in jobs if sum(r["status"] != 0 for r in job) >= 2)}
+
+
 def host_cores():
     """Cores this process may run on (at most 16, the host share of one GPU)."""
     try:
@@ -222,6 +241,7 @@ def run_brush(args):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_fused_jobs (one launch per step)", "avg_launch_ms": launch_ms,
                      "algorithmic_bytes_per_launch": nbytes},
+        "occupancy": slot_occupancy(rec[-1][0], rec[-1][1], nslots),
         "setup_s": t_setup,
         "autotune": tune,
     }

@@ -280,6 +280,7 @@ typedef struct {
   double grand_potential;
   double solve_ms;     /* device time of this solve */
   double phase_ms[4];  /* of which: Boltzmann weights, propagator chains, densities + residual, the rest */
+  double start_ms;     /* device time from the earliest solve start of the launch to this solve's start */
 } scf_job_result;
 
 /* Persistent job queue (parameter sweeps, many seeds of one problem).

@@ -867,6 +867,8 @@ int scf_solve_jobs(scf_system *const *slots, int nslots, int chain, const double
   hipFree(tmp);
   if (e != hipSuccess) return hip_fail(e, "job queue");
   const double khz = wall_clock_khz();
+  unsigned long long t0 = ~0ull;
+  for (size_t i = 0; i < nrec; i++) t0 = h[i].t_begin < t0 ? h[i].t_begin : t0;
   for (size_t i = 0; i < nrec; i++) {
     scf_job_result &r = results[i];
     r.iterations = h[i].iterations;
@@ -877,6 +879,7 @@ int scf_solve_jobs(scf_system *const *slots, int nslots, int chain, const double
     r.grand_potential = h[i].GP;
     r.solve_ms = (double)(h[i].t_end - h[i].t_begin) / khz;
     for (int q = 0; q < 4; q++) r.phase_ms[q] = (double)h[i].t_phase[q] / khz;
+    r.start_ms = (double)(h[i].t_begin - t0) / khz;
   }
   if (kernel_ms) *kernel_ms = ms;
   for (size_t i = 0; i < nsys; i++) slots[i]->have_results = 1;

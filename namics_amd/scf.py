@@ -50,7 +50,7 @@ class JobResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("fp32_iterations", ctypes.c_int),
                 ("residual", ctypes.c_double), ("free_energy", ctypes.c_double),
                 ("grand_potential", ctypes.c_double), ("solve_ms", ctypes.c_double),
-                ("phase_ms", ctypes.c_double * 4)]
+                ("phase_ms", ctypes.c_double * 4), ("start_ms", ctypes.c_double)]
 
 
 class Served(ctypes.Structure):
