@@ -47,3 +47,34 @@ def test_binding_fails_loudly_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(scf, "_lib", None)
     with pytest.raises(ImportError):
         scf.lib()
+
+
+def test_ctypes_structs_match_header(tmp_path):
+    """The ctypes mirrors in namics_amd.scf have the size and field offsets of
+    the header's structs (compiled with the host C compiler)."""
+    import shutil
+    import subprocess
+    from namics_amd import scf
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    pairs = {"scf_solve_info": scf.SolveInfo, "scf_results": scf.Results, "scf_tune_report": scf.TuneReport,
+             "scf_batch_tune_report": scf.BatchTuneReport, "scf_job_result": scf.JobResult,
+             "scf_served": scf.Served}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "namics_scf.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "sizes.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "sizes"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run([cc, "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                 text=True).stdout.splitlines())
+    for cname, py in pairs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[cname + "." + f]) == getattr(py, f).offset, (cname, f)
