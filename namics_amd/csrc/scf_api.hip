@@ -417,12 +417,14 @@ int grid_solve(scf_system *S, scf_solve_info *info) {
   if (sc.status == DS_OK) {
     HIP_TRY(grid_axpy_first(h, S->d, S->nb, st));
     if (!S->graph_exec && S->use_graph) {
-      hipGraph_t graph;
+      hipGraph_t graph = nullptr;
       HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       hipError_t ce = grid_iteration(h, S->d, h.partial, S->nb, st);
       hipError_t ee = hipStreamEndCapture(st, &graph);
-      if (ce != hipSuccess) return hip_fail(ce, "grid_iteration capture");
-      if (ee != hipSuccess) return hip_fail(ee, "hipStreamEndCapture");
+      if (ce != hipSuccess || ee != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return ce != hipSuccess ? hip_fail(ce, "grid_iteration capture") : hip_fail(ee, "hipStreamEndCapture");
+      }
       hipError_t ie = hipGraphInstantiate(&S->graph_exec, graph, nullptr, nullptr, 0);
       hipGraphDestroy(graph);
       if (ie != hipSuccess) {
