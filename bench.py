@@ -5,20 +5,28 @@ cylindrical coordinates, 64 radial x 128 axial layers, chains (X)1(A)199
 grafted at 0.05 chains per unit rod area, monomeric solvent, solved for
 chi(A,W) = 0.0, 0.5, 0.8 in sequence (each solve starts from the previous
 one's fields, as the reference chains `start` blocks; the first starts from
-seeded uniform random fields in [-0.05, 0.05]), DIIS to ||g|| <= 1e-7, FP64.
+seeded uniform random fields in [-0.05, 0.05]), DIIS to ||g|| <= 1e-7
+(iteration limit 5000, the input's), FP64.
 
-One job = those three solves.  One step = J independent jobs (one seed
-each, J = twice the number of CUs by default) in one launch of the
-persistent job queue: one workgroup per resident slot (one slot per CU),
-each taking the next job when it finishes one, with fields, seeds and
-solver state resident in HBM.  The timed region holds K steps.  Prints one
-JSON line: converged solves/sec (FP64), per-solve p50/p99 device latency,
-the launch's algorithmic propagator bandwidth against HBM peak, the same
-jobs with FP32 propagators and FP64 refinement, one job alone, and the
-reference CPU program timed on this host on one job as the baseline.
+One job = those three solves for one seed.  One step = one batch of J jobs
+(seeds 3 .. 3+J-1, J = 128 by default, 3J solves).  The K timed steps (and,
+before them, the W warm-up steps) are handed to the persistent job queue as
+one launch of K*J jobs: one workgroup per resident slot (one slot per CU),
+each taking the next job when it finishes one, so the stall tail of the
+slowest solves is paid once per launch, not once per step.  Fields, seeds
+and solver state are resident in HBM before the timed region.
 
-    python bench.py [--steps K] [--warmup W] [--jobs J] [--slots S] [--workload brush|batch1d|serve]
-                    [--no-cpu] [--no-mixed] [--switch R]
+`value` counts the solves that reach ||g|| <= 1e-7 in both this library and
+the reference program (tests/golden/brush_seeds.json holds the reference's
+own runs of the same 128 seeded jobs), per second of the timed launch.  The
+line also states the library's free-energy error against the reference on
+those solves, per-solve p50/p99 device latency, the launch's HBM roofline,
+the same batch with FP32 propagators and FP64 refinement, one job alone, and
+the reference program timed on this host's cores on distinct seeds of the
+same batch.
+
+    python bench.py [--steps K] [--warmup W] [--jobs J] [--slots S]
+                    [--workload brush|batch1d|serve|box] [--no-cpu] [--no-mixed]
 """
 import argparse
 import json
@@ -37,61 +45,47 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 SEED = 3
 CHIS = (0.0, 0.5, 0.8)
+TOL = 1e-7
+REF_SEEDS = os.path.join(REPO, "tests", "golden", "brush_seeds.json")
 
 
-def propagator_bytes(desc, iterations, w=8, ckpt=1):
-    """Algorithmic propagator bytes of one solve.  Per residual evaluation a
-    first-order (Markov 1) chain of length N streams 3w per site forward (read
-    G(s-1), read G1, write G(s)) and 6w backward with composition (read Gb,
-    G1, Gf; write Gb; read+write rho): 9 w N per site.  A second-order chain
-    with 3 direction components streams 7w forward and 12w backward: 19 w N.
-    With save_memory (every ckpt-th end-point row kept), each row not kept
-    is recomputed once more: 3w per site for each of N - ceil(N / ckpt)
-    rows.  Residual evaluations per solve = iterations + 1."""
+def stored_rows(N, ckpt):
+    """End-point rows of a chain of length N kept in HBM: all N, or rows
+    0, K, 2K, ... with checkpoint interval K > 1."""
+    return N if ckpt <= 1 else -(-N // ckpt)
+
+
+def hbm_bytes_per_evaluation(desc, ckpt=1, w=8, kd=10):
+    """Bytes one residual evaluation plus one DIIS update must move through
+    HBM on the fused path, each array read or written once (what a perfect
+    cache would move).  The propagator recursion itself runs in LDS and
+    registers; what must reach HBM is
+
+    * end-point rows: every stored row written once in the forward pass and
+      read once in the backward pass, 2 w per stored site and row
+      (w = 4 B with FP32 propagators), over the sites the chain kernel stores
+      (interior sites on 2-gradient lattices, all sites on 1 gradient);
+    * per-site arrays of the evaluation (8 B per site each): fields x read by
+      the Boltzmann-weight pass and the residual (2 nit rows), G1 written
+      (nseg) and read by every block of the forward and backward passes
+      (2 nblocks) and by the density pass (nmm), molecule-segment densities
+      written by the chains and read by the density pass (2 nmm), segment
+      densities and site averages written and read (4 nseg), the total
+      density (2 rows), the residual g written (nit);
+    * the DIIS update with kd history entries: xR_k, e_k and x written, x
+      and g read, kd entries of both histories read for the Gram row and kd
+      of xR for the combination: (3 kd + 5) nvar values."""
     g = desc["gradients"]
-    W = desc["MX"] * (desc["MY"] if g >= 2 else 1) * (desc["MZ"] if g == 3 else 1)
-    if g == 1:
-        W = desc["M"]  # 1-gradient kernels update the ghost sites too
-    per_eval = sum((19 if m["markov"] == 2 else 9) * w * m["chainlength"] * W for m in desc["molecules"])
-    if ckpt > 1:
-        per_eval += sum(3 * w * (m["chainlength"] - -(-m["chainlength"] // ckpt)) * W for m in desc["molecules"])
-    return per_eval * (iterations + 1)
-
-
-def profiled_traffic(jobs, slots, ckpt):
-    """HBM bytes per launch of the batched-brush job kernel measured with
-    rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-calibrated), from the committed
-    profile summary, when this run has the profiled configuration."""
-    path = os.path.join(REPO, "profiles", "k_fused_jobs_brush512.md")
-    try:
-        with open(path) as f:
-            text = f.read()
-    except OSError:
-        return None, None
-    m = re.search(r"traffic_bytes_per_launch = ([0-9.eE+-]+)", text)
-    c = re.search(r"traffic_config = jobs (\d+), slots (\d+), checkpoint (\d+)", text)
-    if m and c and int(c.group(1)) == jobs and int(c.group(2)) == slots and int(c.group(3)) == ckpt:
-        return float(m.group(1)), os.path.relpath(path, REPO)
-    return None, None
-
-
-def slot_occupancy(jobs, launch_ms, nslots):
-    """How full the persistent launch keeps its slots: busy share of
-    slots x launch time, the time at the end of the launch during which fewer
-    than a tenth of the slots still run a job, and the jobs whose chain has
-    more than one solve that stops at the iteration limit."""
-    spans = [(job[0]["start_ms"], max(r["start_ms"] + r["solve_ms"] for r in job)) for job in jobs]
-    end = max(e for _, e in spans)
-    events = sorted([(b, 1) for b, _ in spans] + [(e, -1) for _, e in spans])
-    few = max(1, nslots // 10)
-    active, drop = 0, 0.0
-    for t, d in events:
-        if active >= few and active + d < few:
-            drop = t
-        active += d
-    return {"busy_fraction": sum(r["solve_ms"] for job in jobs for r in job) / (nslots * launch_ms),
-            "tail_ms_below_10pct_slots": end - drop,
-            "jobs_with_2plus_unconverged": sum(1 for job in jobs if sum(r["status"] != 0 for r in job) >= 2)}
+    M = desc["M"]
+    W = M if g == 1 else desc["MX"] * (desc["MY"] if g >= 2 else 1) * (desc["MZ"] if g == 3 else 1)
+    gf = sum(2 * w * m.get("size", 1) * W * stored_rows(m["chainlength"], ckpt) for m in desc["molecules"])
+    nit = len(desc["fields"])
+    nseg = len(desc["segments"])
+    nblocks = sum(m["blocks"] for m in desc["molecules"])
+    nmm = sum(m["molmon"] for m in desc["molecules"])
+    rows = 3 * nit + nseg + 2 * nblocks + 3 * nmm + 4 * nseg + 2
+    diis = (3 * kd + 5) * desc["nvar"]
+    return gf + 8 * M * rows + 8 * diis
 
 
 def host_cores():
@@ -103,53 +97,106 @@ def host_cores():
     return max(1, min(16, n))
 
 
-def cpu_baseline_brush(x0, names, dims, budget_s=60.0, procs=1):
+def load_reference_seeds():
+    """The reference program's own runs of the benchmark jobs: per seed, the
+    three solves' iterations, residual and free energy."""
+    try:
+        with open(REF_SEEDS) as f:
+            fx = json.load(f)
+    except OSError:
+        return {}
+    return {j["seed"]: j["solves"] for j in fx["jobs"] if j["rc"] == 0 and len(j["solves"]) == len(CHIS)}
+
+
+def profiled_traffic(jobs, slots, ckpt):
+    """HBM bytes per step of the batched-brush job kernel measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-calibrated), from the committed
+    profile summary, when this run has the profiled configuration."""
+    path = os.path.join(REPO, "profiles", "k_fused_jobs_brush128x20.md")
+    try:
+        with open(path) as f:
+            text = f.read()
+    except OSError:
+        return None, None
+    m = re.search(r"traffic_bytes_per_step = ([0-9.eE+-]+)", text)
+    c = re.search(r"traffic_config = jobs (\d+), slots (\d+), checkpoint (\d+)", text)
+    if m and c and int(c.group(1)) == jobs and int(c.group(2)) == slots and int(c.group(3)) == ckpt:
+        return float(m.group(1)), os.path.relpath(path, REPO)
+    return None, None
+
+
+def slot_occupancy(jobs, launch_ms, nslots):
+    """How full the persistent launch keeps its slots: busy share of
+    slots x launch time, and the time at the end of the launch during which
+    fewer than a tenth of the slots still run a job."""
+    spans = [(job[0]["start_ms"], max(r["start_ms"] + r["solve_ms"] for r in job)) for job in jobs]
+    end = max(e for _, e in spans)
+    events = sorted([(b, 1) for b, _ in spans] + [(e, -1) for _, e in spans])
+    few = max(1, nslots // 10)
+    active, drop = 0, 0.0
+    for t, d in events:
+        if active >= few and active + d < few:
+            drop = t
+        active += d
+    return {"busy_fraction": sum(r["solve_ms"] for job in jobs for r in job) / (nslots * launch_ms),
+            "tail_ms_below_10pct_slots": end - drop}
+
+
+def compare_with_reference(results, seeds, ref):
+    """Solves of `results` (per-job lists of dicts, job j on seed seeds[j])
+    whose seed the reference ran: counts converged in the library, in the
+    reference and in both, and the relative free-energy difference on the
+    solves converged in both."""
+    n = conv_lib = conv_ref = both = 0
+    rel = []
+    for job, seed in zip(results, seeds):
+        rs = ref.get(seed)
+        if rs is None:
+            continue
+        for r, q in zip(job, rs):
+            n += 1
+            a = r["status"] == 0
+            b = q["residual"] <= TOL
+            conv_lib += a
+            conv_ref += b
+            if a and b:
+                both += 1
+                rel.append(abs(r["free_energy"] - q["free_energy"]) / abs(q["free_energy"]))
+    return {"solves": n, "converged_library": conv_lib, "converged_reference": conv_ref, "converged_both": both,
+            "rel_free_energy_vs_reference": {"max": float(np.max(rel)), "median": float(np.median(rel))}
+            if rel else None}
+
+
+def cpu_baseline_brush(seeds, names, dims, nvar):
     """Time the reference program (oracle/_ref/namics, built from the
-    reference's own sources) on the same job: three start blocks, the first
-    from the same seeded fields, the rest from the previous result; `procs`
-    copies run at once, one per core, and the result counts all of them over
-    the wall time of the slowest."""
+    reference's own sources) on the benchmark's jobs: one process per host
+    core, each on its own seed of the batch (three start blocks, the first
+    from the seeded fields, the rest from the previous result), all started
+    together; the throughput counts the converged solves over the wall time
+    of the slowest process."""
     ref = os.path.join(REPO, "oracle", "_ref", "namics")
     if not os.path.exists(ref):
         return None
-    from namics_amd.workloads import brush_cylinder, write_guess
+    from namics_amd.workloads import brush_job_input, read_job_kal, uniform_fields, write_guess
     d = tempfile.mkdtemp(prefix="scfbench")
-    os.makedirs(os.path.join(d, "output"))
-    write_guess(os.path.join(d, "x.ig"), names, dims, x0)
-    text = brush_cylinder(CHIS[0])
-    text += ("sys : NN : initial_guess : file\nsys : NN : guess_inputfile : x.ig\n"
-             "newton : isaac : e_info : false\n"
-             "output : kal : append : true\nkal : newton : isaac : iterations\n"
-             "kal : newton : isaac : residual\nkal : sys : NN : free_energy\nstart\n")
-    for c in CHIS[1:]:
-        text += "mon : A : chi_W : %r\nsys : NN : initial_guess : previous_result\nstart\n" % c
     dirs = []
-    for k in range(procs):
-        dk = os.path.join(d, "p%d" % k)
+    for seed in seeds:
+        dk = os.path.join(d, "s%d" % seed)
         os.makedirs(os.path.join(dk, "output"))
+        write_guess(os.path.join(dk, "x.ig"), names, dims, uniform_fields(nvar, seed, 0.05))
         with open(os.path.join(dk, "job.in"), "w") as f:
-            f.write(text)
-        os.link(os.path.join(d, "x.ig"), os.path.join(dk, "x.ig"))
+            f.write(brush_job_input("x.ig", CHIS))
         dirs.append(dk)
     t0 = time.perf_counter()
     ps = [subprocess.Popen([ref, "job.in"], cwd=dk, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
           for dk in dirs]
-    rcs = [p.wait(timeout=budget_s * 10) for p in ps]
+    rcs = [p.wait(timeout=900) for p in ps]
     dt = time.perf_counter() - t0
-    runs = []
-    for dk in dirs:
-        rows = []
+    runs = {}
+    for seed, dk in zip(seeds, dirs):
         kal = os.path.join(dk, "output", "job.kal")
-        if os.path.exists(kal):
-            with open(kal) as f:
-                lines = [l.split("\t") for l in f.read().strip().splitlines()]
-            rows = [dict(zip(lines[0], l)) for l in lines[1:]]
-        runs.append(rows)
-    rows = runs[0]
-    conv = sum(1 for rs in runs for rw in rs if float(rw["newton_isaac_residual"]) <= 1e-7)
-    return dict(seconds=dt, rc=max(rcs), solves=sum(len(rs) for rs in runs), converged=conv, procs=procs,
-                iterations=[int(float(rw["newton_isaac_iterations"])) for rw in rows],
-                free_energy=[float(rw["sys_NN_free_energy"]) for rw in rows])
+        runs[seed] = read_job_kal(kal) if os.path.exists(kal) else []
+    return dict(seconds=dt, rc=max(rcs), runs=runs)
 
 
 def run_brush(args):
@@ -157,7 +204,9 @@ def run_brush(args):
     from namics_amd.workloads import brush_cylinder, uniform_fields
     ncu = device_info()[0]
     nslots = args.slots if args.slots > 0 else ncu
-    J = args.jobs if args.jobs > 0 else 2 * nslots
+    J = args.jobs if args.jobs > 0 else 128
+    K, W = args.steps, args.warmup
+    ref = load_reference_seeds()
     t_setup = time.perf_counter()
     # launch layout: tuned once on one system of the brush's shape; every
     # slot system built afterwards starts from the tuned layout
@@ -171,136 +220,151 @@ def run_brush(args):
     slots = [[ScfSystem(brush_cylinder(c)) for c in CHIS] for _ in range(nslots)]
     # end-point storage: tuned with all slots in flight (every row, or every
     # other row with the odd rows recomputed on chip), unless given
-    K = args.checkpoint
-    if K == 0 and not args.no_tune:
+    ck = args.checkpoint
+    if ck == 0 and not args.no_tune:
         tune_batch = autotune_batch([row[0] for row in slots])
         tune = dict(tune or {}, batch=tune_batch)
-        K = tune_batch["chosen"]
-    K = max(1, K)
+        ck = tune_batch["chosen"]
+    ck = max(1, ck)
     for row in slots:
         for s in row:
-            s.set_checkpoint(K)
+            s.set_checkpoint(ck)
     descs = [s.desc for s in slots[0]]
     nvar0 = slots[0][0].nvar
-    seeds = np.stack([uniform_fields(nvar0, SEED + j, 0.05) for j in range(J)])
-    seeds_dev = DeviceBuffer(seeds)
+    batch_seeds = list(range(SEED, SEED + J))
+    seeds_one = np.stack([uniform_fields(nvar0, s, 0.05) for s in batch_seeds])
+    # K steps (and W warm-up steps) of the same batch, one launch each
+    seeds_dev = DeviceBuffer(np.tile(seeds_one, (max(K, W, 1), 1)))
     synchronize()
     t_setup = time.perf_counter() - t_setup
 
-    def run(steps, warmup):
-        for _ in range(warmup):
-            solve_jobs(slots, seeds_dev, J)
+    def launch(steps):
+        """One persistent launch of `steps` batches; per-job results of the
+        last batch, the launch's HIP-event time, and the wall time between
+        device synchronisations."""
         synchronize()
         t0 = time.perf_counter()
-        rec = [solve_jobs(slots, seeds_dev, J) for _ in range(steps)]
+        res, ms = solve_jobs(slots, seeds_dev, steps * J)
         synchronize()
-        return rec, time.perf_counter() - t0
+        return res, ms, time.perf_counter() - t0
 
-    def summary(rec, elapsed):
-        solves = [r for res, _ in rec for job in res for r in job]
-        conv = sum(1 for r in solves if r["status"] == 0)
-        launch_ms = float(np.mean([ms for _, ms in rec]))
-        nbytes = float(np.mean([sum(propagator_bytes(descs[c], job[c]["iterations"], ckpt=K)
-                                    for job in res for c in range(len(CHIS))) for res, _ in rec]))
-        return solves, conv, launch_ms, nbytes
-
-    rec, elapsed = run(args.steps, args.warmup)
-    solves, converged, launch_ms, nbytes = summary(rec, elapsed)
+    if W > 0:
+        launch(W)
+    res, launch_ms, elapsed = launch(K)
+    solves = [r for job in res for r in job]
+    last = res[(K - 1) * J:]
+    conv = sum(1 for r in solves if r["status"] == 0)
+    cmp = compare_with_reference(last, batch_seeds, ref)
+    # every step solves the same batch with the same arithmetic: the count of
+    # solves converged in both paths is K times that of one batch
+    both = cmp["converged_both"] * K if ref else None
+    nbytes = sum(hbm_bytes_per_evaluation(descs[c], ck, kd=min(r["iterations"], descs[c]["solver"]["m"]))
+                 * (r["iterations"] + 1) for job in res for c, r in enumerate(job))
     achieved = nbytes / (launch_ms * 1e-3) / 1e9
-    F64 = [[r["free_energy"] for r in job] for job in rec[-1][0]]
     out = {
         "metric": "converged SCF solves/sec + propagator HBM GB/s at 1e-7 residual; p50/p99 latency",
-        "value": converged / elapsed,
+        "value": (both if both is not None else conv) / elapsed,
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * elapsed / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded uniform random initial fields, one seed per job)",
-        "config": {"workload": "2-gradient cylindrical brush 64x128, (X)1(A)199 grafted 0.05/area on a rod, "
-                               "chi 0/0.5/0.8 chained (each solve starts from the previous one's fields), "
-                               "DIIS 1e-7, FP64",
+        "config": {"workload": "configuration 2: 2-gradient cylindrical brush 64x128, (X)1(A)199 grafted "
+                               "0.05/area on a rod, chi 0/0.5/0.8 chained (each solve from the previous one's "
+                               "fields), DIIS to 1e-7, FP64",
                    "jobs_per_step": J, "solves_per_step": J * len(CHIS), "seeds": [SEED, SEED + J - 1],
-                   "slots": nslots, "checkpoint": K},
+                   "slots": nslots, "checkpoint": ck,
+                   "launch": "the K steps run as one persistent job-queue launch of K*J jobs"},
+        "value_counts": "solves converged (||g|| <= 1e-7) in both the library and the reference program"
+                        if both is not None else "solves converged in the library (no reference runs found)",
+        "converged_library_per_s": conv / elapsed,
+        "vs_reference": cmp if ref else None,
         "latency_ms": {"p50": float(np.percentile([r["solve_ms"] for r in solves], 50)),
                        "p99": float(np.percentile([r["solve_ms"] for r in solves], 99)),
                        "what": "device time of one solve inside the persistent launch"},
         "iterations_p50_p99": [float(np.percentile([r["iterations"] for r in solves], q)) for q in (50, 99)],
-        "converged_fraction": converged / max(1, len(solves)),
-        # device time per DIIS iteration by phase, over every solve of the
-        # timed steps (device clock inside the persistent launch)
+        "converged_fraction": conv / max(1, len(solves)),
         "phase_ms_per_iteration": dict(zip(
             ("prepare", "chains", "densities_residual", "diis_other"),
             (np.sum([r["phase_ms"] for r in solves], axis=0) /
              max(1, sum(r["iterations"] + 1 for r in solves))).tolist())),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fused_jobs (one launch per step)", "avg_launch_ms": launch_ms,
-                     "algorithmic_bytes_per_launch": nbytes},
-        "occupancy": slot_occupancy(rec[-1][0], rec[-1][1], nslots),
+                     "kernel": "k_fused_jobs (one launch per timed region)", "avg_launch_ms": launch_ms,
+                     "bytes_per_launch": nbytes,
+                     "bytes": "HBM bytes the path must move: stored end-point rows written and read once, "
+                              "per-site arrays once per evaluation, DIIS history per iteration"},
+        "occupancy": slot_occupancy(res, launch_ms, nslots),
         "setup_s": t_setup,
         "autotune": tune,
     }
-    traffic, src = profiled_traffic(J, nslots, K)
+    traffic, src = profiled_traffic(J, nslots, ck)
     if traffic is not None:
-        out["roofline"]["traffic"] = traffic
+        out["roofline"]["traffic"] = traffic * K
         out["roofline"]["traffic_source"] = src + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)"
-        out["roofline"]["hbm_gbs_measured_traffic"] = traffic / (launch_ms * 1e-3) / 1e9
-        # measured HBM bytes over the launch against the peak (frac counts the
-        # algorithmic bytes, most of which stay on chip)
-        out["roofline"]["frac_measured_traffic"] = out["roofline"]["hbm_gbs_measured_traffic"] / HBM_PEAK_GBS
-    # the same jobs with FP32 propagators and FP64 refinement
-    mixed_ok = not args.no_mixed and slots[0][0].layout()["strip_sx"] > 0
-    if not args.no_mixed and not mixed_ok:
-        out["mixed_precision"] = {"skipped": "the tuned layout is not a register-strip layout"}
-    if mixed_ok:
-        for row in slots:
-            for s in row:
-                s.set_precision(True, args.switch)
-        recm, elm = run(args.steps, args.warmup)
-        sm, convm, msm, _ = summary(recm, elm)
-        Fm = [[r["free_energy"] for r in job] for job in recm[-1][0]]
-        both = [(a, b) for ja, jb, jfa, jfb in zip(Fm, F64, recm[-1][0], rec[-1][0])
-                for a, b, ra, rb in zip(ja, jb, jfa, jfb) if ra["status"] == 0 and rb["status"] == 0]
-        out["mixed_precision"] = {
-            "value": convm / elm, "unit": "solves/s", "ms_per_step": 1e3 * elm / args.steps,
-            "converged_fraction": convm / max(1, len(sm)), "switch_residual": args.switch,
-            "fp32_iteration_fraction": sum(r["fp32_iterations"] for r in sm) / max(1, sum(r["iterations"] for r in sm)),
-            "median_rel_free_energy_diff_vs_fp64": float(np.median([abs(a - b) / abs(b) for a, b in both]))
-            if both else None,
-            "latency_ms": {"p50": float(np.percentile([r["solve_ms"] for r in sm], 50)),
-                           "p99": float(np.percentile([r["solve_ms"] for r in sm], 99))}}
-        for row in slots:
-            for s in row:
-                s.set_precision(False)
+    # one batch with FP32 propagators and FP64 refinement
+    if not args.no_mixed:
+        if slots[0][0].layout()["strip_sx"] > 0:
+            for row in slots:
+                for s in row:
+                    s.set_precision(True, args.switch)
+            resm, msm, elm = launch(1)
+            sm = [r for job in resm for r in job]
+            convm = sum(1 for r in sm if r["status"] == 0)
+            cmpm = compare_with_reference(resm, batch_seeds, ref)
+            rel64 = [abs(a["free_energy"] - b["free_energy"]) / abs(b["free_energy"])
+                     for ja, jb in zip(resm, last) for a, b in zip(ja, jb) if a["status"] == 0 and b["status"] == 0]
+            out["mixed_precision"] = {
+                "value": (cmpm["converged_both"] if ref else convm) / elm, "unit": "solves/s",
+                "ms_per_step": 1e3 * elm, "steps": 1, "converged_fraction": convm / max(1, len(sm)),
+                "switch_residual": args.switch,
+                "fp32_iteration_fraction": sum(r["fp32_iterations"] for r in sm) /
+                max(1, sum(r["iterations"] for r in sm)),
+                "rel_free_energy_vs_fp64": {"max": float(np.max(rel64)), "median": float(np.median(rel64))}
+                if rel64 else None,
+                "vs_reference": cmpm if ref else None,
+                "latency_ms": {"p50": float(np.percentile([r["solve_ms"] for r in sm], 50)),
+                               "p99": float(np.percentile([r["solve_ms"] for r in sm], 99))}}
+            for row in slots:
+                for s in row:
+                    s.set_precision(False)
+        else:
+            out["mixed_precision"] = {"skipped": "the tuned layout is not a register-strip layout"}
     # one job alone (the latency of the chained job on an otherwise idle GPU),
     # with the end-point storage tuned for one system in flight
-    K1 = K
+    K1 = ck
     if args.checkpoint == 0 and not args.no_tune:
         K1 = autotune_batch(slots[0][:1])["chosen"]
         for s in slots[0]:
             s.set_checkpoint(K1)
-    one, el1 = None, 0.0
+    synchronize()
     t1 = time.perf_counter()
     one, _ = solve_jobs(slots[:1], seeds_dev, 1)
     synchronize()
     el1 = time.perf_counter() - t1
     out["single_job"] = {"ms": 1e3 * el1, "solves_per_s": sum(1 for r in one[0] if r["status"] == 0) / el1,
-                         "iterations": [r["iterations"] for r in one[0]], "checkpoint": K1}
+                         "iterations": [r["iterations"] for r in one[0]], "checkpoint": K1,
+                         "reference_iterations": [q["iterations"] for q in ref[SEED]] if SEED in ref else None}
     if not args.no_cpu:
         P = host_cores()
-        cb = cpu_baseline_brush(seeds[0], slots[0][0].field_names, slots[0][0].dims, procs=P)
+        cb = cpu_baseline_brush(batch_seeds[:P], slots[0][0].field_names, slots[0][0].dims, nvar0)
         if cb is not None:
-            out["cpu_baseline"] = {"value": cb["converged"] / cb["seconds"], "unit": "solves/s", "cores": P,
-                                   "kind": "reference",
-                                   "sample": "%d copies of one job (3 chained solves, seed %d) by the reference "
-                                             "program built from its sources, one process per core, %.1f s wall; "
-                                             "iterations %s" % (P, SEED, cb["seconds"], cb["iterations"]),
-                                   "free_energy": cb["free_energy"]}
+            jobs_cpu = [cb["runs"][s] for s in batch_seeds[:P]]
+            conv_cpu = sum(1 for rs in jobs_cpu for q in rs if q["residual"] <= TOL)
+            # the same solves in the library (the last timed batch)
+            both_cpu = sum(1 for rs, job in zip(jobs_cpu, last[:P]) for q, r in zip(rs, job)
+                           if q["residual"] <= TOL and r["status"] == 0)
+            out["cpu_baseline"] = {
+                "value": both_cpu / cb["seconds"], "unit": "solves/s", "cores": P, "kind": "reference",
+                "sample": "the reference program built from its sources, one process per core, each on one job "
+                          "of the batch (seeds %d..%d, 3 chained solves each), %.1f s wall; counts solves "
+                          "converged in both paths (%d of %d; %d converged in the reference)"
+                          % (batch_seeds[0], batch_seeds[P - 1], cb["seconds"], both_cpu, 3 * P, conv_cpu)}
     for row in slots:
         for s in row:
             s.close()
@@ -315,9 +379,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="brush")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--jobs", type=int, default=0, help="independent brush jobs per step (0: twice the slots)")
+    ap.add_argument("--jobs", type=int, default=0, help="brush jobs per step (0: 128, the seeds the reference ran)")
     ap.add_argument("--slots", type=int, default=0, help="resident job slots (0: one per CU)")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision rerun")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision batch")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch layout (no autotuning)")
     ap.add_argument("--no-graph", action="store_true", help="box: launch the grid-path kernels one by one")
     ap.add_argument("--checkpoint", type=int, default=0,

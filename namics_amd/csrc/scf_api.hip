@@ -422,11 +422,11 @@ int grid_solve(scf_system *S, scf_solve_info *info) {
       hipError_t ce = grid_iteration(h, S->d, h.partial, S->nb, st);
       hipError_t ee = hipStreamEndCapture(st, &graph);
       if (ce != hipSuccess || ee != hipSuccess) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return ce != hipSuccess ? hip_fail(ce, "grid_iteration capture") : hip_fail(ee, "hipStreamEndCapture");
       }
       hipError_t ie = hipGraphInstantiate(&S->graph_exec, graph, nullptr, nullptr, 0);
-      hipGraphDestroy(graph);
+      (void)hipGraphDestroy(graph);
       if (ie != hipSuccess) {
         S->graph_exec = nullptr;
         return hip_fail(ie, "hipGraphInstantiate");
@@ -803,6 +803,8 @@ int scf_carry_fields(scf_system *dst, const scf_system *src) {
   return SCF_OK;
 }
 
+}  // extern "C"
+
 // carry map of `dst` from `src` by segment name (initial_guess : previous_result)
 static int set_carry_map(scf_system *dst, const scf_system *src) {
   bool changed = false;
@@ -818,12 +820,9 @@ static int set_carry_map(scf_system *dst, const scf_system *src) {
   return SCF_OK;
 }
 
-int scf_solve_jobs(scf_system *const *slots, int nslots, int chain, const double *seeds_dev, int njobs,
-                   scf_job_result *results, double *kernel_ms) {
-  if (!slots || nslots <= 0 || chain <= 0 || !seeds_dev || njobs < 0 || !results)
-    return fail(SCF_E_ARG, "bad argument");
-  size_t lds = 0;
-  int threads = 64;
+namespace nscf {
+
+int prepare_jobs(scf_system *const *slots, int nslots, int chain, int *threads, size_t *lds) {
   for (int s = 0; s < nslots; s++)
     for (int c = 0; c < chain; c++) {
       scf_system *S = slots[(size_t)s * chain + c];
@@ -838,9 +837,40 @@ int scf_solve_jobs(scf_system *const *slots, int nslots, int chain, const double
         if (rc != SCF_OK) return rc;
       }
       HIP_TRY(hipStreamSynchronize(S->stream));
-      threads = std::max(threads, S->h.threads);
-      lds = std::max(lds, S->lds_bytes);
+      *threads = std::max(*threads, S->h.threads);
+      *lds = std::max(*lds, S->lds_bytes);
     }
+  return SCF_OK;
+}
+
+void convert_job_records(const JobRecord *h, size_t n, unsigned long long t0, scf_job_result *results) {
+  const double khz = wall_clock_khz();
+  for (size_t i = 0; i < n; i++) {
+    scf_job_result &r = results[i];
+    r.iterations = h[i].iterations;
+    r.status = h[i].status != DS_OK ? h[i].status : (h[i].converged ? SCF_OK : SCF_NOT_CONVERGED);
+    r.fp32_iterations = h[i].fp32_iterations;
+    r.residual = h[i].residual;
+    r.free_energy = h[i].F;
+    r.grand_potential = h[i].GP;
+    r.solve_ms = (double)(h[i].t_end - h[i].t_begin) / khz;
+    for (int q = 0; q < 4; q++) r.phase_ms[q] = (double)h[i].t_phase[q] / khz;
+    r.start_ms = (double)(h[i].t_begin - t0) / khz;
+  }
+}
+
+}  // namespace nscf
+
+extern "C" {
+
+int scf_solve_jobs(scf_system *const *slots, int nslots, int chain, const double *seeds_dev, int njobs,
+                   scf_job_result *results, double *kernel_ms) {
+  if (!slots || nslots <= 0 || chain <= 0 || !seeds_dev || njobs < 0 || !results)
+    return fail(SCF_E_ARG, "bad argument");
+  int threads = 64;
+  size_t lds = 0;
+  int rc = prepare_jobs(slots, nslots, chain, &threads, &lds);
+  if (rc != SCF_OK) return rc;
   if (njobs == 0) return SCF_OK;
   scf_system *S0 = slots[0];
   hipStream_t st = S0->stream;
@@ -853,7 +883,6 @@ int scf_solve_jobs(scf_system *const *slots, int nslots, int chain, const double
   JobRecord *rec = (JobRecord *)(tmp + 256 + nsys * sizeof(DevSys *));
   std::vector<DevSys *> list(nsys);
   for (size_t i = 0; i < nsys; i++) list[i] = slots[i]->d;
-  int rc = SCF_OK;
   float ms = 0.f;
   hipError_t e = hipMemcpy(d_slots, list.data(), nsys * sizeof(DevSys *), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(counter, 0, sizeof(int));
@@ -868,21 +897,9 @@ int scf_solve_jobs(scf_system *const *slots, int nslots, int chain, const double
   if (e == hipSuccess) e = hipMemcpy(h.data(), rec, nrec * sizeof(JobRecord), hipMemcpyDeviceToHost);
   hipFree(tmp);
   if (e != hipSuccess) return hip_fail(e, "job queue");
-  const double khz = wall_clock_khz();
   unsigned long long t0 = ~0ull;
   for (size_t i = 0; i < nrec; i++) t0 = h[i].t_begin < t0 ? h[i].t_begin : t0;
-  for (size_t i = 0; i < nrec; i++) {
-    scf_job_result &r = results[i];
-    r.iterations = h[i].iterations;
-    r.status = h[i].status != DS_OK ? h[i].status : (h[i].converged ? SCF_OK : SCF_NOT_CONVERGED);
-    r.fp32_iterations = h[i].fp32_iterations;
-    r.residual = h[i].residual;
-    r.free_energy = h[i].F;
-    r.grand_potential = h[i].GP;
-    r.solve_ms = (double)(h[i].t_end - h[i].t_begin) / khz;
-    for (int q = 0; q < 4; q++) r.phase_ms[q] = (double)h[i].t_phase[q] / khz;
-    r.start_ms = (double)(h[i].t_begin - t0) / khz;
-  }
+  convert_job_records(h.data(), nrec, t0, results);
   if (kernel_ms) *kernel_ms = ms;
   for (size_t i = 0; i < nsys; i++) slots[i]->have_results = 1;
   return rc;

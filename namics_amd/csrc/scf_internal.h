@@ -74,5 +74,11 @@ size_t lds_limit();
 void plan_layout(scf_system *S, int want_sx);
 int tuned_strip(scf_system *S);
 std::string shape_key(const scf_system *S);
+// Job queue: check that `slots` (nslots x chain systems) can share one
+// launch, set the carry maps between chain positions, and return the
+// workgroup size and dynamic LDS the launch needs.
+int prepare_jobs(scf_system *const *slots, int nslots, int chain, int *threads, size_t *lds);
+// Device job records -> caller results; t0 = the earliest solve start.
+void convert_job_records(const JobRecord *h, size_t n, unsigned long long t0, scf_job_result *results);
 
 }  // namespace nscf

@@ -1196,7 +1196,7 @@ std::string describe_json(const SystemSpec &s) {
     str(m.name);
     o << ",\"chainlength\":" << m.chainlength << ",\"freedom\":" << m.freedom << ",\"markov\":" << m.markov
       << ",\"pinned\":" << (m.pinned ? "true" : "false") << ",\"save_memory\":" << (m.save_memory ? "true" : "false")
-      << ",\"size\":" << m.size << "}";
+      << ",\"size\":" << m.size << ",\"blocks\":" << m.blocks.size() << ",\"molmon\":" << m.molmon.size() << "}";
   }
   o << "],\"fields\":[";
   for (size_t i = 0; i < s.itmon.size(); i++) {

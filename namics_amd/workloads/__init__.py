@@ -153,6 +153,40 @@ def carry_fields(x_prev, names_prev, names_new, M):
     return x
 
 
+def brush_job_input(guess_file, chis=(0.0, 0.5, 0.8), **kw):
+    """One benchmark brush job as a reference input: one `start` block per chi
+    value, the first from the fields in `guess_file` (guess-file format), each
+    later one from the previous result, with a kal row per solve (iterations,
+    residual, free energy, grand potential) appended to output/<name>.kal."""
+    text = brush_cylinder(chis[0], **kw)
+    text += ("sys : NN : initial_guess : file\nsys : NN : guess_inputfile : %s\n"
+             "newton : isaac : e_info : false\n"
+             "output : kal : append : true\nkal : newton : isaac : iterations\n"
+             "kal : newton : isaac : residual\nkal : sys : NN : free_energy\n"
+             "kal : sys : NN : grand_potential\nstart\n") % guess_file
+    for c in chis[1:]:
+        text += "mon : A : chi_W : %r\nsys : NN : initial_guess : previous_result\nstart\n" % c
+    return text
+
+
+def read_job_kal(path):
+    """Rows of a kal file written by `brush_job_input`, one dict per solve:
+    iterations, residual, free_energy, grand_potential."""
+    with open(path) as f:
+        lines = [l.rstrip("\n").split("\t") for l in f if l.strip()]
+    keys = {"newton_isaac_iterations": "iterations", "newton_isaac_residual": "residual",
+            "sys_NN_free_energy": "free_energy", "sys_NN_grand_potential": "grand_potential"}
+    out = []
+    for row in lines[1:]:
+        r = {}
+        for k, v in zip(lines[0], row):
+            if k in keys:
+                r[keys[k]] = float(v)
+        r["iterations"] = int(r["iterations"])
+        out.append(r)
+    return out
+
+
 def brush_chain_input(chis=(0.0, 0.5, 0.8), **kw):
     """The brush solves as one reference input with one `start` block per
     chi value (each solve starts from the previous result)."""
