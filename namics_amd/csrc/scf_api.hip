@@ -90,11 +90,39 @@ int validate(const SystemSpec &s, std::string &why) {
   if (nb > kMaxBlocks) return why = "more than 16 blocks over all molecules", SCF_E_UNSUPPORTED;
   if (nmm > kMaxMM) return why = "more than 16 molecule-segment pairs", SCF_E_UNSUPPORTED;
   if (s.solver.m > kMaxDIIS || s.solver.m < 1) return why = "DIIS history m must be 1..16", SCF_E_UNSUPPORTED;
+  if (save_memory_interval(s) > 1)
+    for (auto &m : s.mol)
+      if (m.markov == 2)
+        return why = "save_memory with second-order (Markov 2) chains is not supported", SCF_E_UNSUPPORTED;
   if (l.gradients >= 2)
     for (int d = 0; d < 6; d++)
       if (l.bc[d] == BC_SURFACE) return why = "surface boundaries are supported with 1 gradient only", SCF_E_UNSUPPORTED;
   if (l.gradients == 2 && l.geometry == GEOM_PLANAR && !l.stencil_full && l.lattice_type != LAT_SIMPLE_CUBIC)
     return why = "2-gradient planar hexagonal stencil without stencil_full is not supported", SCF_E_UNSUPPORTED;
+  return SCF_OK;
+}
+
+int ckpt_rows(int N, int K) { return K <= 1 ? N : (N + K - 1) / K + std::min(K, N) - 1; }
+
+int save_memory_interval(const SystemSpec &s) {
+  int n = 0;
+  for (auto &m : s.mol)
+    if (m.save_memory) n = std::max(n, m.chainlength);
+  if (n < 3) return 0;
+  const int kept = std::max(1, (int)std::lround(std::sqrt(2.0 * n)));
+  const int K = (n + kept - 1) / kept;
+  return K > 1 ? K : 0;
+}
+
+int check_layout(const scf_system *S, int mode, int strip_sx) {
+  bool markov1 = true;
+  for (auto &m : S->spec.mol) markov1 = markov1 && m.markov == 1;
+  const bool strip = mode == SCF_MODE_FUSED && strip_sx > 0;
+  if (S->h.precision != SCF_PREC_FP64 && !strip)
+    return fail(SCF_E_UNSUPPORTED, "FP32 propagators need the 2-gradient register-strip path");
+  if (S->h.ckpt > 1 && !strip && !(mode == SCF_MODE_GRID && markov1))
+    return fail(SCF_E_UNSUPPORTED, "checkpointed propagators (save_memory) need first-order chains on the grid "
+                                   "path or the 2-gradient register-strip path");
   return SCF_OK;
 }
 
@@ -254,9 +282,12 @@ int build(scf_system *S, const Placement *pl) {
       for (int b : m.molmon) cf -= fraction(m, a) * fraction(m, b) * (s.seg[a].chi[b] / 2);
     h.mol_constF[i] = cf;
     for (int k = 0; k < nseg; k++) h.frac[i][k] = fraction(m, k);
-    maxNR = std::max(maxNR, m.chainlength * m.size);
     maxR = std::max(maxR, m.size);
   }
+  // end-point storage: every row, or checkpoints with the input's save_memory
+  h.ckpt = save_memory_interval(s);
+  for (auto &m : s.mol) maxNR = std::max(maxNR, ckpt_rows(m.chainlength, h.ckpt) * m.size);
+  S->gf_rows = maxNR;
   h.mol_block_begin[nmol] = nb;
   h.mol_mm_begin[nmol] = nmm;
   h.nblocks = nb;
@@ -531,8 +562,11 @@ int scf_system_set_mode(scf_system *S, int mode) {
   if (mode == SCF_MODE_GRID)
     for (auto &m : S->spec.mol)
       if (m.markov == 2) return fail(SCF_E_UNSUPPORTED, "second-order chains run on the fused path only");
+  const int saved = S->mode;
   S->mode = mode;
-  return SCF_OK;
+  int rc = check_layout(S, S->effective_mode(), S->h.strip_sx);
+  if (rc != SCF_OK) S->mode = saved;
+  return rc;
 }
 
 int scf_system_set_precision(scf_system *S, int mode, double switch_residual) {
@@ -540,9 +574,13 @@ int scf_system_set_precision(scf_system *S, int mode, double switch_residual) {
   if (mode != SCF_PREC_FP64 && mode != SCF_PREC_MIXED) return fail(SCF_E_ARG, "unknown precision mode");
   if (mode == SCF_PREC_MIXED && !(switch_residual > S->h.tol))
     return fail(SCF_E_ARG, "the FP64 switch level must lie above the tolerance");
-  if (mode == SCF_PREC_MIXED && (S->h.strip_sx == 0 || S->effective_mode() != SCF_MODE_FUSED))
-    return fail(SCF_E_UNSUPPORTED, "FP32 propagators need the 2-gradient register-strip path");
+  const int saved = S->h.precision;
   S->h.precision = mode;
+  int rc = check_layout(S, S->effective_mode(), S->h.strip_sx);
+  if (rc != SCF_OK) {
+    S->h.precision = saved;
+    return rc;
+  }
   if (mode == SCF_PREC_MIXED) S->h.mixed_switch = switch_residual;
   return upload_solver(S);
 }
@@ -561,13 +599,19 @@ int scf_system_set_graph(scf_system *S, int on) {
 int scf_system_set_checkpoint(scf_system *S, int interval) {
   if (!S) return fail(SCF_E_ARG, "null system");
   if (interval < 0) return fail(SCF_E_ARG, "negative checkpoint interval");
-  if (interval <= 1) {
-    S->h.ckpt = 0;
-    return upload_solver(S);
+  const int K = interval <= 1 ? 0 : interval;
+  for (auto &m : S->spec.mol)
+    if (ckpt_rows(m.chainlength, K) * m.size > S->gf_rows)
+      return fail(SCF_E_ARG, "the arena holds " + std::to_string(S->gf_rows) +
+                                 " end-point rows (the input's save_memory); interval " + std::to_string(interval) +
+                                 " needs " + std::to_string(ckpt_rows(m.chainlength, K) * m.size));
+  const int saved = S->h.ckpt;
+  S->h.ckpt = K;
+  int rc = check_layout(S, S->effective_mode(), S->h.strip_sx);
+  if (rc != SCF_OK) {
+    S->h.ckpt = saved;
+    return rc;
   }
-  if (S->h.strip_sx == 0 || S->effective_mode() != SCF_MODE_FUSED)
-    return fail(SCF_E_UNSUPPORTED, "checkpointed propagators need the 2-gradient register-strip path");
-  S->h.ckpt = interval;
   return upload_solver(S);
 }
 
@@ -591,24 +635,30 @@ int scf_system_autotune(scf_system *S, int evals, scf_tune_report *rep) {
   if (!rep) rep = &tmp;
   std::memset(rep, 0, sizeof(*rep));
   const LatticeSpec &l = S->spec.lat;
-  const bool need_strip = S->h.precision != SCF_PREC_FP64 || S->h.ckpt > 1;
-  // candidate layouts
+  // candidate layouts: those the system's options (precision, checkpointed
+  // storage, second-order chains) run on
   bool markov2 = false;
   for (auto &m : S->spec.mol) markov2 = markov2 || m.markov == 2;
   std::vector<TunedLayout> cand;
-  if (S->mode == SCF_MODE_AUTO && !markov2 && !need_strip) cand.push_back({SCF_MODE_GRID, 0});
+  auto consider = [&](int mode, int sx) {
+    if (check_layout(S, mode, sx) == SCF_OK) cand.push_back({mode, sx});
+  };
+  if (S->mode == SCF_MODE_AUTO && !markov2) consider(SCF_MODE_GRID, 0);
   if (S->mode != SCF_MODE_GRID) {
     if (l.gradients == 2) {
       for (int sx : {8, 4}) {
         plan_layout(S, sx);
-        if (S->h.strip_sx == sx) cand.push_back({SCF_MODE_FUSED, sx});
+        if (S->h.strip_sx == sx) consider(SCF_MODE_FUSED, sx);
       }
-      if (!need_strip) cand.push_back({SCF_MODE_FUSED, 0});
+      consider(SCF_MODE_FUSED, 0);
     } else {
-      cand.push_back({SCF_MODE_FUSED, 0});
+      consider(SCF_MODE_FUSED, 0);
     }
   }
-  if (cand.empty()) cand.push_back({SCF_MODE_GRID, 0});
+  if (cand.empty()) {
+    plan_layout(S, -1);
+    return check_layout(S, S->effective_mode(), S->h.strip_sx);
+  }
   const int saved_mode = S->mode, saved_tuned = S->tuned_mode;
   double best = 1e300;
   int besti = 0;
@@ -680,8 +730,16 @@ int scf_autotune_batch(scf_system *const *systems, int n, int evals, scf_batch_t
     lds = std::max(lds, S->lds_bytes);
     HIP_TRY(hipStreamSynchronize(S->stream));
   }
-  std::vector<int> cand{1};
-  if (strip) cand.push_back(2);
+  // end-point storages every system can hold (the input's save_memory sizes
+  // the arena for checkpoints only)
+  std::vector<int> cand;
+  for (int K : {1, 2}) {
+    bool fits = K == 1 || strip;
+    for (int i = 0; i < n && fits; i++)
+      for (auto &m : systems[i]->spec.mol) fits = fits && ckpt_rows(m.chainlength, K) * m.size <= systems[i]->gf_rows;
+    if (fits) cand.push_back(K);
+  }
+  if (cand.empty()) return fail(SCF_E_UNSUPPORTED, "no end-point storage fits every system");
   scf_system *S0 = systems[0];
   hipStream_t st = S0->stream;
   std::vector<DevSys *> list;

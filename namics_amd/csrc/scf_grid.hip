@@ -7,6 +7,8 @@
 // partial sums in HBM, summed in workgroup order by one thread.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "scf_device.h"
 #include "scf_kernels.h"
 
@@ -119,12 +121,14 @@ __global__ void __launch_bounds__(kGridThreads) k_row_init(const DevSys *__restr
   }
 }
 
-// one forward step: Gf[s] = W(Gf[s-1]) * G1
-__global__ void __launch_bounds__(kGridThreads) k_step_fwd(const DevSys *__restrict__ d, int s, int seg) {
+// one forward step: Gf[dst] = W(Gf[src]) * G1 (src, dst: end-point storage
+// slots, see grid_residual_body)
+__global__ void __launch_bounds__(kGridThreads) k_step_fwd(const DevSys *__restrict__ d, int src, int dst,
+                                                           int seg) {
   CSys &S = uniform_sys(*d);
   const int M = S.M;
-  const double *__restrict__ prev = S.Gf + (size_t)(s - 1) * M;
-  double *__restrict__ gs = S.Gf + (size_t)s * M;
+  const double *__restrict__ prev = S.Gf + (size_t)src * M;
+  double *__restrict__ gs = S.Gf + (size_t)dst * M;
   const double *__restrict__ g1 = S.G1 + seg * M;
   GRID_LOOP(S) {
     int x, y, z;
@@ -134,11 +138,12 @@ __global__ void __launch_bounds__(kGridThreads) k_step_fwd(const DevSys *__restr
 }
 
 // first backward row and its composition
-__global__ void __launch_bounds__(kGridThreads) k_bwd_first(const DevSys *__restrict__ d, int s, int seg, int mm) {
+__global__ void __launch_bounds__(kGridThreads) k_bwd_first(const DevSys *__restrict__ d, int slot, int seg,
+                                                            int mm) {
   CSys &S = uniform_sys(*d);
   const int M = S.M;
   const double *g1 = S.G1 + seg * M;
-  const double *gf = S.Gf + (size_t)s * M;
+  const double *gf = S.Gf + (size_t)slot * M;
   double *rho = S.rho + mm * M;
   GRID_LOOP(S) {
     int x, y, z;
@@ -149,15 +154,16 @@ __global__ void __launch_bounds__(kGridThreads) k_bwd_first(const DevSys *__rest
   }
 }
 
-// backward step with fused composition: Gb[dst] = W(Gb[src]) * G1, rho += Gf[s] Gb[dst]
-__global__ void __launch_bounds__(kGridThreads) k_step_bwd(const DevSys *__restrict__ d, int s, int seg, int mm,
+// backward step with fused composition: Gb[dst] = W(Gb[src]) * G1,
+// rho += Gf[slot] Gb[dst]
+__global__ void __launch_bounds__(kGridThreads) k_step_bwd(const DevSys *__restrict__ d, int slot, int seg, int mm,
                                                            int src) {
   CSys &S = uniform_sys(*d);
   const int M = S.M;
   const double *__restrict__ gb = S.Gb + (size_t)src * M;
   double *__restrict__ gbn = S.Gb + (size_t)(1 - src) * M;
   const double *__restrict__ g1 = S.G1 + seg * M;
-  const double *__restrict__ gf = S.Gf + (size_t)s * M;
+  const double *__restrict__ gf = S.Gf + (size_t)slot * M;
   double *__restrict__ rho = S.rho + mm * M;
   GRID_LOOP(S) {
     int x, y, z;
@@ -183,7 +189,8 @@ __device__ __forceinline__ double stencil3(CSys &S, const gdbl *__restrict__ g, 
   return v * (1.0 / 6.0);
 }
 
-__global__ void __launch_bounds__(kGridThreads) k_step3_fwd(const DevSys *__restrict__ d, int s, int seg) {
+__global__ void __launch_bounds__(kGridThreads) k_step3_fwd(const DevSys *__restrict__ d, int src, int dst,
+                                                            int seg) {
   CSys &S = uniform_sys(*d);
   const int MZ = S.MZ, M = S.M;
   const int p = blockIdx.x * kGridThreads + threadIdx.x;
@@ -191,11 +198,11 @@ __global__ void __launch_bounds__(kGridThreads) k_step3_fwd(const DevSys *__rest
   const int yy = p / MZ;
   const int x = blockIdx.y + 1, y = yy + 1, z = p - yy * MZ + 1;
   const int i = x * S.JX + y * S.JY + z;
-  const gdbl *prev = hbm(S.Gf) + (size_t)(s - 1) * M;
-  hbm(S.Gf)[(size_t)s * M + i] = stencil3(S, prev, x, y, z) * hbm(S.G1)[(size_t)seg * M + i];
+  const gdbl *prev = hbm(S.Gf) + (size_t)src * M;
+  hbm(S.Gf)[(size_t)dst * M + i] = stencil3(S, prev, x, y, z) * hbm(S.G1)[(size_t)seg * M + i];
 }
 
-__global__ void __launch_bounds__(kGridThreads) k_step3_bwd(const DevSys *__restrict__ d, int s, int seg, int mm,
+__global__ void __launch_bounds__(kGridThreads) k_step3_bwd(const DevSys *__restrict__ d, int slot, int seg, int mm,
                                                             int src) {
   CSys &S = uniform_sys(*d);
   const int MZ = S.MZ, M = S.M;
@@ -208,13 +215,14 @@ __global__ void __launch_bounds__(kGridThreads) k_step3_bwd(const DevSys *__rest
   const double v = stencil3(S, gb, x, y, z) * hbm(S.G1)[(size_t)seg * M + i];
   hbm(S.Gb)[(size_t)(1 - src) * M + i] = v;
   gdbl *rho = hbm(S.rho) + (size_t)mm * M;
-  rho[i] += hbm(S.Gf)[(size_t)s * M + i] * v;
+  rho[i] += hbm(S.Gf)[(size_t)slot * M + i] * v;
 }
 
-__global__ void __launch_bounds__(kGridThreads) k_gn_partial(const DevSys *__restrict__ d, int s, double *partial) {
+__global__ void __launch_bounds__(kGridThreads) k_gn_partial(const DevSys *__restrict__ d, int slot,
+                                                             double *partial) {
   CSys &S = uniform_sys(*d);
   __shared__ double red[kGridThreads / 64];
-  const double *gl = S.Gf + (size_t)s * S.M;
+  const double *gl = S.Gf + (size_t)slot * S.M;
   double acc = 0;
   GRID_LOOP(S) {
     int x, y, z;
@@ -549,6 +557,21 @@ hipError_t grid_residual(const DevSys &h, DevSys *d, double *partial, int nb, hi
   return grid_residual_body(h, d, partial, nb, st);
 }
 
+// End-point storage of the grid path.  With every row kept (ckpt <= 1) row
+// s of the forward recursion is Gf slot s.  With checkpoints every K rows
+// (the reference's save_memory, src/molecule.cpp:2300-2446) the slots hold
+// rows 0, K, 2K, ... (slot s/K) and one buffer of K-1 rows (slots nck ..
+// nck+K-2) for the rows between two checkpoints: the forward pass writes
+// each such row into the buffer, so after it the buffer holds the last
+// segment; the backward pass recomputes the rows of every earlier segment
+// from its checkpoint, with the forward arithmetic, just before it composes
+// them.  Densities and partition functions are bit-identical to full
+// storage.
+struct RowSlots {
+  int K, nck;
+  int operator()(int r) const { return K <= 1 ? r : (r % K == 0 ? r / K : nck + r % K - 1); }
+};
+
 hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
   const dim3 B(kGridThreads), G(nb);
   const dim3 G3((h.MY * h.MZ + kGridThreads - 1) / kGridThreads, h.MX);
@@ -559,27 +582,38 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
   for (int mo = 0; mo < h.nmol; mo++) {
     if (h.mol_markov[mo] != 1) return hipErrorInvalidValue;
     const int b0 = h.mol_block_begin[mo], b1 = h.mol_block_begin[mo + 1], N = h.mol_N[mo];
+    // block (segment, molecule-segment pair) of every row of the chain
+    std::vector<int> rb(N);
+    for (int b = b0, r = 0; b < b1; b++)
+      for (int j = 0; j < h.block_n[b]; j++) rb[r++] = b;
+    const RowSlots loc{h.ckpt, h.ckpt > 1 ? (N + h.ckpt - 1) / h.ckpt : N};
+    auto fwd = [&](int r) {
+      const int seg = h.block_seg[rb[r]];
+      if (six)
+        hipLaunchKernelGGL(k_step3_fwd, G3, B, 0, st, d, loc(r - 1), loc(r), seg);
+      else
+        hipLaunchKernelGGL(k_step_fwd, G, B, 0, st, d, loc(r - 1), loc(r), seg);
+    };
     hipLaunchKernelGGL(k_row_init, G, B, 0, st, d, h.block_seg[b0]);
-    int s = 1;
-    for (int b = b0; b < b1; b++)
-      for (int j = (b == b0 ? 1 : 0); j < h.block_n[b]; j++, s++)
-        if (six)
-          hipLaunchKernelGGL(k_step3_fwd, G3, B, 0, st, d, s, h.block_seg[b]);
-        else
-          hipLaunchKernelGGL(k_step_fwd, G, B, 0, st, d, s, h.block_seg[b]);
-    hipLaunchKernelGGL(k_gn_partial, G, B, 0, st, d, N - 1, partial);
+    for (int r = 1; r < N; r++) fwd(r);
+    hipLaunchKernelGGL(k_gn_partial, G, B, 0, st, d, loc(N - 1), partial);
     hipLaunchKernelGGL(k_finalize_gn, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb, mo);
-    hipLaunchKernelGGL(k_bwd_first, G, B, 0, st, d, N - 1, h.block_seg[b1 - 1], h.block_mm[b1 - 1]);
-    s = N - 2;
-    int src = 0;
-    for (int b = b1 - 1; b >= b0; b--)
-      for (int j = (b == b1 - 1 ? h.block_n[b] - 2 : h.block_n[b] - 1); j >= 0; j--, s--) {
-        if (six)
-          hipLaunchKernelGGL(k_step3_bwd, G3, B, 0, st, d, s, h.block_seg[b], h.block_mm[b], src);
-        else
-          hipLaunchKernelGGL(k_step_bwd, G, B, 0, st, d, s, h.block_seg[b], h.block_mm[b], src);
-        src = 1 - src;
+    hipLaunchKernelGGL(k_bwd_first, G, B, 0, st, d, loc(N - 1), h.block_seg[rb[N - 1]], h.block_mm[rb[N - 1]]);
+    int src = 0, seg_in_buffer = h.ckpt > 1 ? (N - 1) / h.ckpt : -1;
+    for (int s = N - 2; s >= 0; s--) {
+      if (h.ckpt > 1 && s / h.ckpt != seg_in_buffer) {
+        // rows of segment q (qK+1 .. qK+K-1) back into the buffer
+        seg_in_buffer = s / h.ckpt;
+        const int a0 = seg_in_buffer * h.ckpt;
+        for (int r = a0 + 1; r < a0 + h.ckpt && r < N; r++) fwd(r);
       }
+      const int b = rb[s];
+      if (six)
+        hipLaunchKernelGGL(k_step3_bwd, G3, B, 0, st, d, loc(s), h.block_seg[b], h.block_mm[b], src);
+      else
+        hipLaunchKernelGGL(k_step_bwd, G, B, 0, st, d, loc(s), h.block_seg[b], h.block_mm[b], src);
+      src = 1 - src;
+    }
   }
   hipLaunchKernelGGL(k_norms, dim3(1), dim3(64), 0, st, d);
   hipLaunchKernelGGL(k_densities, G, B, 0, st, d);

@@ -25,6 +25,8 @@ struct scf_system {
   int mode = SCF_MODE_AUTO;
   int tuned_mode = SCF_MODE_AUTO;  // the autotuner's choice for this shape (used under AUTO)
   int nb = 0;            // workgroups of the grid path
+  int gf_rows = 0;       // end-point rows (of M sites) the arena holds: chain length x components, or
+                         // fewer with the input's save_memory
   int m_alloc = 0;       // DIIS history the arena was sized for
   size_t lds_bytes = 0;  // dynamic LDS the fused path asks for (0: rows in HBM)
   bool have_results = false;
@@ -35,6 +37,8 @@ struct scf_system {
 
   int effective_mode() const {
     if (mode != SCF_MODE_AUTO) return mode;
+    // checkpointed end-point storage runs on the strip chains or the grid path
+    if (h.ckpt > 1 && h.strip_sx == 0) return SCF_MODE_GRID;
     if (tuned_mode != SCF_MODE_AUTO) return tuned_mode;
     bool markov2 = false;  // second-order chains run on the fused path only
     for (auto &m : spec.mol) markov2 = markov2 || m.markov == 2;
@@ -74,6 +78,16 @@ size_t lds_limit();
 void plan_layout(scf_system *S, int want_sx);
 int tuned_strip(scf_system *S);
 std::string shape_key(const scf_system *S);
+// End-point rows kept with checkpoint interval K (K <= 1: all N rows): the
+// checkpoints 0, K, 2K, ... plus a buffer for the rows of one segment.
+int ckpt_rows(int N, int K);
+// Checkpoint interval for the input's save_memory (0: every row): about
+// sqrt(2 N) rows kept for the longest chain that asks for it, as the
+// reference plans them (src/molecule.cpp:174-191), on a uniform grid.
+int save_memory_interval(const SystemSpec &s);
+// SCF_OK when the system's options (precision, checkpoints) run on the
+// given layout, else SCF_E_UNSUPPORTED with the reason.
+int check_layout(const scf_system *S, int mode, int strip_sx);
 // Job queue: check that `slots` (nslots x chain systems) can share one
 // launch, set the carry maps between chain positions, and return the
 // workgroup size and dynamic LDS the launch needs.

@@ -331,8 +331,9 @@ def test_checkpointed_propagators_equal_full_storage(golden, case):
     K = 2 recomputes each odd row on chip), also for chains of four blocks,
     of even and odd length, whose boundaries fall inside the recomputed
     segments; and FP64 and mixed-precision solves follow the same
-    trajectory to the last bit.  Lattices off the strip path refuse the
-    option."""
+    trajectory to the last bit.  Off the strip path first-order chains
+    keep their checkpoints on the grid path; second-order chains refuse
+    them."""
     from namics_amd import ScfSystem, ScfError, solve_resident_info
     fx = golden[case]
     o0 = Oracle(fx["text"])
@@ -384,11 +385,25 @@ def test_checkpointed_propagators_equal_full_storage(golden, case):
         assert (i1["iterations"], i1["status"], i1["fp32_iterations"], i1["residual"]) == \
             (i0["iterations"], i0["status"], i0["fp32_iterations"], i0["residual"])
         assert np.array_equal(x_1, x_0) and F1 == F0
+    # off the strip path a first-order chain keeps its checkpoints on the grid
+    # path (bit-identical to the grid path's full storage); second-order
+    # chains, which run on the fused path only, refuse them
+    from namics_amd import scf
     off = ScfSystem(golden["brush2d_cyl"]["text"])
-    with pytest.raises(ScfError):
-        off.set_checkpoint(4)
+    x = np.random.default_rng(3).uniform(-0.3, 0.3, off.nvar)
+    off.set_mode(scf.MODE_GRID)
+    g0, r0 = off.residual(x)
+    off.set_mode(scf.MODE_AUTO)
+    off.set_checkpoint(4)
+    assert off.layout()["mode"] == "grid"
+    g1, r1 = off.residual(x)
+    assert np.array_equal(g1, g0) and r1 == r0
     off.set_checkpoint(0)
     off.close()
+    m2 = ScfSystem(golden["planar_markov2"]["text"])
+    with pytest.raises(ScfError):
+        m2.set_checkpoint(4)
+    m2.close()
 
 
 @pytest.mark.parametrize("case", ["brush2d_rod_strip", "planar2d_particle", "diblock3d_periodic",
