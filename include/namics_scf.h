@@ -298,6 +298,21 @@ typedef struct {
 int scf_solve_jobs(scf_system *const *slots, int nslots, int chain, const double *seeds_dev, int njobs,
                    scf_job_result *results, double *kernel_ms);
 
+/* The job queue without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops
+ * and harnesses that forbid hipMalloc).  Same arguments, kernels and results
+ * as scf_solve_jobs, but the queue's scratch (counter, slot list, one record
+ * per solve: 256 B + nslots*chain*8 B rounded up to 256 B + njobs*chain*88 B)
+ * lives in a fixed 4 MiB array in device memory that the library holds on
+ * every GPU it runs on until the process ends; larger calls are refused with
+ * SCF_E_ARG before anything runs.  The call allocates nothing and returns
+ * when the launch has finished.  All calls on one GPU share that GPU's
+ * array: this library serialises its own calls per device, and calls from
+ * several threads that bypass it (other copies of the library in the
+ * process) would race on the array. */
+int scf_solve_jobs_noalloc(scf_system *const *slots, int nslots, int chain, const double *seeds_dev, int njobs,
+                           scf_job_result *results, double *kernel_ms);
+
 /* Scalars of the last residual / solve. */
 int scf_get_results(const scf_system *sys, scf_results *out);
 

@@ -137,6 +137,7 @@ def lib():
         L.scf_device_copy.argtypes = [vp, vp, ctypes.c_size_t]
         L.scf_solve_jobs.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, vp, ctypes.c_int,
                                      ctypes.POINTER(JobResult), ctypes.POINTER(ctypes.c_double)]
+        L.scf_solve_jobs_noalloc.argtypes = L.scf_solve_jobs.argtypes
         L.scf_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -426,17 +427,19 @@ def synchronize():
     _check(lib().scf_device_synchronize())
 
 
-def solve_jobs(slots, seeds, njobs):
+def solve_jobs(slots, seeds, njobs, noalloc=False):
     """Persistent job queue: `slots` is a list of equal chains (lists of
     systems), `seeds` a DeviceBuffer holding njobs x nvar seed fields of chain
     position 0.  Returns (results as a list of per-job lists of dicts,
-    kernel_ms)."""
+    kernel_ms).  noalloc: the entry point that takes its scratch from the
+    library's fixed device array instead of allocating it."""
     nslots, chain = len(slots), len(slots[0])
     flat = [s for row in slots for s in row]
     hs = (ctypes.c_void_p * len(flat))(*[s.h.value for s in flat])
     res = (JobResult * (njobs * chain))()
     ms = ctypes.c_double(0.0)
-    _check(lib().scf_solve_jobs(hs, nslots, chain, seeds.ptr, njobs, res, ctypes.byref(ms)))
+    fn = lib().scf_solve_jobs_noalloc if noalloc else lib().scf_solve_jobs
+    _check(fn(hs, nslots, chain, seeds.ptr, njobs, res, ctypes.byref(ms)))
     def conv(r):
         d = {f: getattr(r, f) for f, _ in JobResult._fields_}
         d["phase_ms"] = list(d["phase_ms"])
