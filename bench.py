@@ -88,6 +88,32 @@ def hbm_bytes_per_evaluation(desc, ckpt=1, w=8, kd=10):
     return gf + 8 * M * rows + 8 * diis
 
 
+def recomputed_rows(N, ckpt):
+    """Rows the grid path's backward pass recomputes per evaluation with
+    checkpoint interval K: every row between two checkpoints except those of
+    the last segment, which the forward pass left in the buffer."""
+    if ckpt <= 1:
+        return 0
+    last = (N - 1) // ckpt * ckpt
+    return (N - -(-N // ckpt)) - (N - 1 - last)
+
+
+def grid_bytes_per_evaluation(desc, ckpt=1, w=8, kd=10):
+    """Bytes one residual evaluation plus one DIIS update move on the grid
+    path, where every propagation step is a launch that streams its rows
+    through memory: forward 3w per site (read G(s-1) and G1, write G(s)),
+    backward with composition 6w (read Gb, G1 and Gf, write Gb, read and
+    write rho), 3w per recomputed row with checkpoints, over the interior
+    sites; the per-site arrays of the other phases and the DIIS history as
+    in hbm_bytes_per_evaluation."""
+    W = desc["MX"] * max(1, desc["MY"]) * max(1, desc["MZ"])
+    chains = sum((9 * m["chainlength"] + 3 * recomputed_rows(m["chainlength"], ckpt)) * w * W
+                 for m in desc["molecules"])
+    rest = hbm_bytes_per_evaluation(desc, ckpt, w, kd) - \
+        sum(2 * w * m.get("size", 1) * W * stored_rows(m["chainlength"], ckpt) for m in desc["molecules"])
+    return chains + rest
+
+
 def host_cores():
     """Cores this process may run on (at most 16, the host share of one GPU)."""
     try:

@@ -84,7 +84,9 @@ int scf_system_create(const char *input_text, int start, scf_system **out, char 
 void scf_system_destroy(scf_system *sys);
 
 /* JSON description: names of segments, molecules and field blocks, lattice
- * sizes (M, MX, MY, MZ), nvar, solver settings.  Returns bytes needed. */
+ * sizes (M, MX, MY, MZ), nvar, solver settings, and the current end-point
+ * checkpoint interval ("checkpoint", 0 = every row kept).  Returns bytes
+ * needed. */
 int scf_system_describe(const scf_system *sys, char *buf, size_t len);
 
 /* The same description for input text without creating a system (no
@@ -124,10 +126,14 @@ int scf_system_set_precision(scf_system *sys, int mode, double switch_residual);
  * checkpoints from the earlier one just before it composes them.  Densities,
  * partition functions and every iterate are bit-identical to full storage;
  * the end-point traffic falls about interval-fold for up to 50% more
- * propagation steps.  interval 0 (or 1) stores every row.  Applies to the
- * 2-gradient strip path; SCF_E_UNSUPPORTED elsewhere.  The schedule is
- * uniform rather than the reference's per-block round(sqrt(2 N_b)) plan,
- * which changes no output. */
+ * propagation steps.  interval 0 (or 1) stores every row.  Runs on the
+ * 2-gradient strip path and, for first-order chains, on the grid path (a
+ * system off the strip path moves to the grid path); second-order chains
+ * get SCF_E_UNSUPPORTED.  An input with `mol : <name> : save_memory : true`
+ * starts with the interval that keeps about sqrt(2N) rows of the longest
+ * such chain and an arena sized for them; a larger storage than that arena
+ * holds is refused with SCF_E_ARG.  The schedule is uniform rather than the
+ * reference's per-block round(sqrt(2 N_b)) plan, which changes no output. */
 int scf_system_set_checkpoint(scf_system *sys, int interval);
 
 /* Grid-path iteration replay: on (default), the first grid-path DIIS
@@ -144,8 +150,8 @@ int scf_system_set_graph(scf_system *sys, int on);
  * system, and remembers it for the lattice shape, so systems built later in
  * the process with the same shape start from it.  Every layout computes the
  * same residual (to rounding).  A layout the system's options exclude (FP32
- * propagators and checkpointing need a strip layout; an explicit
- * scf_system_set_mode) is not tried.  No counterpart in the reference, which
+ * propagators need a strip layout, checkpointing a strip layout or the grid
+ * path; an explicit scf_system_set_mode) is not tried.  No counterpart in the reference, which
  * has one code path per lattice. */
 #define SCF_TUNE_MAX 4
 typedef struct {

@@ -15,7 +15,7 @@ import numpy as np
 
 
 def run_box(args, n=96, seed=41):
-    from bench import HBM_PEAK_GBS, propagator_bytes
+    from bench import HBM_PEAK_GBS, grid_bytes_per_evaluation
     from namics_amd import ScfSystem, solve_resident_info, synchronize
     from namics_amd.workloads import diblock_box, seeded_box_fields
     s = ScfSystem(diblock_box(n))
@@ -39,7 +39,8 @@ def run_box(args, n=96, seed=41):
     conv = sum(1 for i in infos if i["status"] == 0)
     it = infos[-1]["iterations"]
     solve_s = float(np.mean([r[1] for r in runs]))
-    nbytes = propagator_bytes(s.desc, it)
+    ck = s.checkpoint_interval()
+    nbytes = sum(grid_bytes_per_evaluation(s.desc, ck, kd=min(k, s.desc["solver"]["m"])) for k in range(it + 1))
     achieved = nbytes / solve_s / 1e9
     res = s.results()
     s.close()
@@ -57,7 +58,8 @@ def run_box(args, n=96, seed=41):
         "dtype": "f64",
         "data": "synthetic (seeded uniform fields plus a Gaussian well)",
         "config": {"workload": "3-gradient periodic box %d^3, (A)10(B)10 at 0.05, chi_BW = chi_AB = 1.6, "
-                               "save_memory, DIIS 1e-7, FP64, grid path" % n, "seed": seed,
+                               "save_memory (end-point checkpoint interval %d), DIIS 1e-7, FP64, grid path"
+                               % (n, ck), "seed": seed,
                    "graph": not args.no_graph},
         "iterations": it,
         "ms_per_iteration": 1e3 * solve_s / (it + 1),
@@ -67,5 +69,7 @@ def run_box(args, n=96, seed=41):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "whole solve (per-kernel durations in profiles/k_step3_box96.md)",
-                     "algorithmic_bytes_per_solve": nbytes},
+                     "bytes_per_solve": nbytes,
+                     "bytes": "every grid-path launch's streams: 9 x 8 B per site and contour step, 3 x 8 B per "
+                              "recomputed row, per-site arrays once per evaluation, DIIS history per iteration"},
     }

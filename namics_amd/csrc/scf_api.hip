@@ -538,7 +538,8 @@ void scf_system_destroy(scf_system *S) {
 
 int scf_system_describe(const scf_system *S, char *buf, size_t len) {
   if (!S) return fail(SCF_E_ARG, "null system");
-  std::string d = describe_json(S->spec);
+  // the system's current end-point checkpoint interval first (0: every row)
+  std::string d = "{\"checkpoint\":" + std::to_string(S->h.ckpt) + "," + describe_json(S->spec).substr(1);
   if (buf && len > 0) snprintf(buf, len, "%s", d.c_str());
   return (int)d.size() + 1;
 }

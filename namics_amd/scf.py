@@ -266,6 +266,14 @@ class ScfSystem:
         return dict(mode={MODE_FUSED: "fused", MODE_GRID: "grid"}.get(m.value, m.value), strip_sx=sx.value,
                     threads=th.value)
 
+    def checkpoint_interval(self):
+        """Current end-point checkpoint interval (0: every row kept)."""
+        L = lib()
+        n = L.scf_system_describe(self.h, None, 0)
+        buf = ctypes.create_string_buffer(n)
+        L.scf_system_describe(self.h, buf, n)
+        return json.loads(buf.value.decode())["checkpoint"]
+
     def set_checkpoint(self, interval):
         """Keep every `interval`-th end-point distribution and recompute the
         rest in the backward pass (save_memory); 0 stores every row."""

@@ -12,14 +12,18 @@ it reports, compactly enough to commit:
   same quantities;
 * batch1d_sample.json -- the first draws of the batched 1-gradient sweep
   (seed 12345) solved by the reference's DIIS to 1e-7 from zero fields
-  (iteration limit 1000): iterations, residual, F, Omega.
+  (iteration limit 1000): iterations, residual, F, Omega;
+* serve_cfg5.json -- a 56 x 182 cylindrical brush (N = 211) and a 54^3
+  diblock box drawn by the request-stream generator (seed 11): evaluated at
+  the request's starting fields (as above) and solved by the reference's
+  DIIS from them (tolerance 1e-7, iteration limit 1000).
 
 Profile moments (per profile p over the interior sites in x-major order,
 i = 0, 1, ...): sum p, sum p^2, sum i p / sum p, min, max, and p at the first,
 middle and last site.  The starting fields are regenerated from the seeds by
 the tests (numpy's PCG64 is reproducible).
 
-Usage: python tests/golden/make_fullsize.py [brush|box|batch ...]
+Usage: python tests/golden/make_fullsize.py [brush|box|batch|serve ...]
 """
 import json
 import os
@@ -117,13 +121,61 @@ def batch(count=12):
     return {"draws": res, "seed": 12345, "tolerance": 1e-7, "iterationlimit": 1000}
 
 
+SERVE_SEED, SERVE_COUNT = 11, 15   # the request-stream test's mix: 12 1-D systems, 2 brushes, 1 box
+SERVE_PINNED = (12, 14)           # the 56 x 182 brush (N = 211) and the 54^3 diblock box
+
+
+def solve_from(text, x):
+    """The reference's DIIS solve from fields x with the input's own
+    tolerance and iteration limit: iterations, residual, F, Omega and the
+    per-molecule results."""
+    o = oracle_layout(text)
+    with tempfile.TemporaryDirectory() as d:
+        write_guess(os.path.join(d, "x.ig"), o.field_names, o.dims, x)
+        full = text + "\n" + output_lines(o) + (
+            "sys : %s : initial_guess : file\n"
+            "sys : %s : guess_inputfile : x.ig\n"
+            "newton : %s : e_info : false\n"
+            "start\n") % (o.sys_name, o.sys_name, o.newton_name)
+        r = run_ref(d, "cv", full)
+        if r.returncode != 0:
+            raise RuntimeError("reference failed: " + r.stdout[-2000:] + r.stderr[-2000:])
+        kal = read_kal(os.path.join(d, "output", "cv.kal"))
+    row = kal[-1]
+    return {"iterations": int(row["newton_%s_iterations" % o.newton_name]),
+            "residual": row["newton_%s_residual" % o.newton_name],
+            "free_energy": row["sys_%s_free_energy" % o.sys_name],
+            "grand_potential": row["sys_%s_grand_potential" % o.sys_name],
+            "mol": {m: {p: row["mol_%s_%s" % (m, p)] for p in ("GN", "n", "theta", "phibulk", "Mu")}
+                    for m in o.mol_names}}
+
+
+def serve():
+    """Two requests of the request-stream mix at their full sizes (a
+    2-gradient brush and a 3-gradient box): the reference's evaluation at the
+    request's starting fields and its DIIS solve from them."""
+    from namics_amd.bench_serve import fields_for, make_requests
+    reqs = make_requests(SERVE_COUNT, SERVE_SEED)
+    out = {"seed": SERVE_SEED, "count": SERVE_COUNT, "requests": {}}
+    for k in SERVE_PINNED:
+        cls, text, spec = reqs[k]
+        o = oracle_layout(text)
+        x = fields_for(spec, {"nvar": o.nvar, "fields": o.field_names})
+        ev = evaluate_at(text, x)
+        cv = solve_from(text, x)
+        print("request", k, cls, "iterations", cv["iterations"], "residual", cv["residual"], flush=True)
+        out["requests"][str(k)] = {"class": cls, "evaluation": ev, "converged": cv}
+    return out
+
+
 def main():
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
-    which = sys.argv[1:] or ["brush", "box", "batch"]
+    which = sys.argv[1:] or ["brush", "box", "batch", "serve"]
     for w in which:
-        out = {"brush": brush, "box": box, "batch": batch}[w]()
-        name = {"brush": "brush64x128_eval", "box": "box96_eval", "batch": "batch1d_sample"}[w]
+        out = {"brush": brush, "box": box, "batch": batch, "serve": serve}[w]()
+        name = {"brush": "brush64x128_eval", "box": "box96_eval", "batch": "batch1d_sample",
+                "serve": "serve_cfg5"}[w]
         with open(os.path.join(HERE, name + ".json"), "w") as f:
             json.dump(out, f)
         print("wrote", name, flush=True)
