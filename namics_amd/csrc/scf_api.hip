@@ -613,6 +613,12 @@ int scf_system_set_checkpoint(scf_system *S, int interval) {
     S->h.ckpt = saved;
     return rc;
   }
+  if (K != saved && S->graph_exec) {
+    // the captured grid-path iteration holds the old step schedule
+    HIP_TRY(hipStreamSynchronize(S->stream));
+    (void)hipGraphExecDestroy(S->graph_exec);
+    S->graph_exec = nullptr;
+  }
   return upload_solver(S);
 }
 

@@ -105,3 +105,26 @@ def test_box96_save_memory_stores_a_subset_and_matches_full_storage():
     _profiles_equal(sm, full)
     sm.close()
     full.close()
+
+
+def test_checkpoint_change_after_a_graph_solve_replays_the_new_schedule(golden):
+    """The grid path captures one DIIS iteration into a hipGraph on its first
+    solve, with the end-point storage schedule of that moment.  Changing the
+    checkpoint interval afterwards drops the captured graph: the next solve
+    gives exactly what a fresh system with that interval gives."""
+    from namics_amd import ScfSystem
+    text = golden["diblock3d_periodic"]["text"] + "\nnewton : isaac : method : DIIS\n" \
+        "newton : isaac : tolerance : 1e-09\nnewton : isaac : iterationlimit : 40\n"
+    x0 = np.random.default_rng(12).uniform(-0.1, 0.1, ScfSystem(text).nvar)
+    a = ScfSystem(text)
+    assert a.layout()["mode"] == "grid"
+    a.solve(x0)                      # captures the iteration with every row stored
+    a.set_checkpoint(3)
+    xa, ita, ra, sta = a.solve(x0)   # replays a new capture
+    b = ScfSystem(text)
+    b.set_checkpoint(3)
+    b.set_graph(False)
+    xb, itb, rb, stb = b.solve(x0)
+    assert (ita, sta) == (itb, stb) and ra == rb and np.array_equal(xa, xb)
+    a.close()
+    b.close()
