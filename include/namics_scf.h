@@ -30,7 +30,7 @@ typedef struct scf_system scf_system;
 enum {
   SCF_OK = 0,
   SCF_NOT_CONVERGED = 1,     /* iteration limit reached (src/sfnewton.cpp:614-617) */
-  SCF_E_GN = -1,             /* partition function GN <= 0 */
+  SCF_E_GN = -1,             /* partition function GN <= 0, or not finite (FP64 overflow) */
   SCF_E_NAN = -2,            /* NaN in the DIIS Gram matrix */
   SCF_E_SVD = -3,            /* NaN in the SVD */
   SCF_E_PHIB = -4,           /* solvent bulk volume fraction < 0 */

@@ -84,6 +84,8 @@ def run_batch(args, n=512, seed=12345):
     conv = [r for r in flat if r[2] == 0]
     lat = [r[4] for r in conv] or [0.0]
     kernel_ms = [rec[0][3] for rec in recs]
+    # the draw whose solve sets the launch time
+    slow = max(range(n), key=lambda k: recs[-1][k][4])
     from bench import hbm_bytes_per_evaluation
     nbytes = sum(hbm_bytes_per_evaluation(s.desc, kd=min(r[0], s.desc["solver"]["m"])) * (r[0] + 1)
                  for rec in recs for s, r in zip(systems, rec))
@@ -97,6 +99,8 @@ def run_batch(args, n=512, seed=12345):
                                "N 20-2000, half Markov 2), DIIS 1e-7, FP64", "seed": seed},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "converged_fraction": len(conv) / len(flat),
+        "slowest": dict(draws[slow], solve_ms=recs[-1][slow][4], iterations=recs[-1][slow][0],
+                        status=recs[-1][slow][2]),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": None, "kernel": "k_fused",
                      "avg_launch_ms": float(np.mean(kernel_ms)),

@@ -28,7 +28,7 @@ constexpr int kMaxComp = 2 * kMaxFjc + 1;  // Markov-2 direction components (FJC
 
 enum DevStatus : int {
   DS_OK = 0,
-  DS_GN = -1,        // GN <= 0 (reference throws -1, src/system.cpp:2398-2402)
+  DS_GN = -1,        // GN <= 0 (reference throws -1, src/system.cpp:2398-2402) or not finite
   DS_NAN = -2,       // NaN in the DIIS matrix (-2, src/sfnewton.cpp:808-809)
   DS_SVD = -3,       // NaN in the SVD (-3, src/tools.cpp:1225-1228)
   DS_PHIB = -4,      // negative solvent phibulk (-4, src/system.cpp:2521-2525)
@@ -978,6 +978,15 @@ __device__ inline void scalars_norms(const DevSys &S) {
   DevScalars &sc = *S.sc;
   double B = 0;
   for (int i = 0; i < S.nmol; i++) {
+    // A partition function that overflowed (or came out NaN) ends the
+    // evaluation as GN <= 0 does.  The reference carries on: its densities
+    // turn NaN, Classical_residual sets g = 0 wherever phi_tot > 0 fails
+    // (src/system.cpp:2114-2142), and ||g|| = 0 passes the stop test, so a
+    // blown-up iterate would be reported as converged with NaN results.
+    if (!isfinite(sc.GN[i])) {
+      sc.status = DS_GN;
+      return;
+    }
     double nrm = 0;
     const int fr = S.mol_freedom[i];
     if (fr == 0) {  // free

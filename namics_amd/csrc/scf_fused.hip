@@ -223,6 +223,161 @@ __device__ __noinline__ void chain1d_m1(const DevSys &S0, int mo) {
   __syncthreads();
 }
 
+// Second-order (Markov 2) chain on a 1-gradient simple-cubic lattice, one
+// site per thread (M <= threads), rows in LDS: three direction components per
+// site (+x, lateral, -x), the stencil of m2_step, composition weights
+// (1/6, 4/6, 1/6) (LGrad1::AddPhiS src/LGrad1.cpp:876-910),
+// GN = (G0 + 4 G1 + G2) / 6 (LGrad1::ComputeGN src/LGrad1.cpp:943-955).
+// The arithmetic is chain_markov2's, in the same order; what differs is that
+// no global access sits on a step's critical path: the Boltzmann weight of a
+// block and its density stay in registers (the density is read once at the
+// block's start and written once at its end), and Gf of the backward pass is
+// fetched one step ahead.
+template <int kSPT>
+__device__ __noinline__ void chain1d_m2(const DevSys &S0, int mo) {
+  CSys &S = uniform_sys(S0);
+  const int M = S.M;
+  const int tid = work_tid(S), nt = S.threads;
+  const int R = 3 * M;
+  double P[kMaxComp];
+#pragma unroll
+  for (int k = 0; k < kMaxComp; k++) P[k] = S.mol_P[mo][k];
+  constexpr double wc[3] = {1.0 / 6.0, 4.0 / 6.0, 1.0 / 6.0};
+  int ii[kSPT];
+  bool act[kSPT];
+#pragma unroll
+  for (int k = 0; k < kSPT; k++) {
+    const int w = tid + k * nt;
+    act[k] = w < M;
+    ii[k] = act[k] ? w : 0;
+  }
+  const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
+  const int N = S.mol_N[mo];
+  gdbl *Gf = hbm(S.Gf);
+  const gdbl *G1 = hbm(S.G1);
+  ldbl *ra = lds_row(0), *rb = lds_row(S.row_stride);
+  double g1[kSPT], o[kSPT][3];
+#pragma unroll
+  for (int k = 0; k < kSPT; k++) {
+    g1[k] = act[k] ? G1[(size_t)S.block_seg[b0] * M + ii[k]] : 0.0;
+    if (act[k])
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        Gf[c * M + ii[k]] = g1[k];
+        ra[c * M + ii[k]] = g1[k];
+      }
+  }
+  lds_barrier();
+  ldbl *prev = ra, *next = rb;
+  int s = 1;
+  for (int b = b0; b < b1; b++) {
+    if (b != b0) {
+#pragma unroll
+      for (int k = 0; k < kSPT; k++)
+        if (act[k]) g1[k] = G1[(size_t)S.block_seg[b] * M + ii[k]];
+    }
+    for (int j = (b == b0 ? 1 : 0); j < S.block_n[b]; j++, s++) {
+#pragma unroll
+      for (int k = 0; k < kSPT; k++) {
+        m2_step<double>(S, (const ldbl *)prev, P, true, ii[k], o[k]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) o[k][c] *= g1[k];
+      }
+      gdbl *gs = Gf + (size_t)s * R;
+#pragma unroll
+      for (int k = 0; k < kSPT; k++)
+        if (act[k])
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            gs[c * M + ii[k]] = o[k][c];
+            next[c * M + ii[k]] = o[k][c];
+          }
+      lds_barrier();
+      ldbl *t = prev;
+      prev = next;
+      next = t;
+    }
+  }
+  {
+    double a[3] = {0.0, 0.0, 0.0};
+    const gdbl *gl = Gf + (size_t)(N - 1) * R;
+#pragma unroll
+    for (int k = 0; k < kSPT; k++)
+      if (act[k] && ii[k] >= 1 && ii[k] <= S.MX) {
+        const double wt = site_weight(S0, ii[k]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) a[c] += wt * gl[c * M + ii[k]];
+      }
+    double gn = wg_sum(a[0]);
+    gn += 4.0 * wg_sum(a[1]);
+    gn += wg_sum(a[2]);
+    if (threadIdx.x == 0) S0.sc->GN[mo] = gn / 6.0;
+  }
+  // backward with composition; Gf(s) of every component fetched one step
+  // ahead, the block density in registers
+  ldbl *gbA = ra, *gbB = rb;
+  const int bl = b1 - 1;
+  double racc[kSPT], gf[kSPT][3];
+  {
+    const gdbl *gfl = Gf + (size_t)(N - 1) * R;
+#pragma unroll
+    for (int k = 0; k < kSPT; k++) {
+      g1[k] = act[k] ? G1[(size_t)S.block_seg[bl] * M + ii[k]] : 0.0;
+      racc[k] = act[k] ? hbm(S.rho)[(size_t)S.block_mm[bl] * M + ii[k]] : 0.0;
+      if (act[k]) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          gbA[c * M + ii[k]] = g1[k];
+          racc[k] += wc[c] * gfl[c * M + ii[k]] * g1[k];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) gf[k][c] = (act[k] && N >= 2) ? Gf[(size_t)(N - 2) * R + c * M + ii[k]] : 0.0;
+    }
+  }
+  lds_barrier();
+  s = N - 2;
+  for (int b = bl; b >= b0; b--) {
+    if (b != bl) {
+#pragma unroll
+      for (int k = 0; k < kSPT; k++) {
+        if (act[k]) {
+          hbm(S.rho)[(size_t)S.block_mm[b + 1] * M + ii[k]] = racc[k];
+          g1[k] = G1[(size_t)S.block_seg[b] * M + ii[k]];
+          racc[k] = hbm(S.rho)[(size_t)S.block_mm[b] * M + ii[k]];
+        }
+      }
+    }
+    for (int j = (b == bl ? S.block_n[b] - 2 : S.block_n[b] - 1); j >= 0; j--, s--) {
+#pragma unroll
+      for (int k = 0; k < kSPT; k++) {
+        m2_step<double>(S, (const ldbl *)gbA, P, false, ii[k], o[k]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          o[k][c] *= g1[k];
+          racc[k] += wc[c] * gf[k][c] * o[k][c];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kSPT; k++)
+        if (act[k])
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            gbB[c * M + ii[k]] = o[k][c];
+            if (s >= 1) gf[k][c] = Gf[(size_t)(s - 1) * R + c * M + ii[k]];
+          }
+      lds_barrier();
+      ldbl *t = gbA;
+      gbA = gbB;
+      gbB = t;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kSPT; k++)
+    if (act[k]) hbm(S.rho)[(size_t)S.block_mm[b0] * M + ii[k]] = racc[k];
+  __syncthreads();
+}
+
 // Markov-1 chain on any lattice, sites strided over the threads, rows in LDS
 // (kLds) or HBM.
 template <bool kLds>
@@ -1071,7 +1226,9 @@ __device__ double fused_residual(const DevSys &S0) {
     if (S.mol_markov[mo] == 2) {
       const int kind = S.gradients == 3 ? M2_D3 : (S.gradients == 2 ? M2_D2 : (S.lattice_type != 0 ? M2_HEX1 : M2_SC1));
       if (S.use_lds) {
-        if (kind == M2_SC1)
+        if (kind == M2_SC1 && S.M <= S.threads)
+          chain1d_m2<1>(S0, mo);
+        else if (kind == M2_SC1)
           chain_markov2<true, M2_SC1>(S0, mo);
         else if (kind == M2_HEX1)
           chain_markov2<true, M2_HEX1>(S0, mo);

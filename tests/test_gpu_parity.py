@@ -604,8 +604,12 @@ def test_batch1d_sample_matches_reference():
     1000) converges on the GPU to the same state (F within 10x the tolerance
     on the scale max(1, |F|), Omega likewise), in the reference's number of iterations
     where that is short (<= 300; over longer runs convergence itself is
-    decided at the rounding level, see below);
-    the draws the reference cannot converge do not converge here either."""
+    decided at the rounding level, see below; "the reference's number" is
+    the range its own algorithm spans from starting fields perturbed by
+    +-1e-15..1e-13, computed by the bit-exact oracle);
+    the draws the reference cannot converge do not converge here either;
+    a solve whose propagators overflow ends with SCF_E_GN instead of
+    reporting a NaN state as converged."""
     from namics_amd import ScfSystem, solve_resident_info
     from namics_amd.workloads import homopolymer_adsorption
     fx = _load_fixture("batch1d_sample")
@@ -634,6 +638,7 @@ def test_batch1d_sample_matches_reference():
             continue
         assert info["status"] == 0, d["draw"]
         r = s.results()
+        assert np.isfinite(r["free_energy"]), d["draw"]
         # ||g|| <= 1e-7 fixes F only to O(1e-7): two converged states agree
         # to 10x the tolerance, on the scale max(1, |F|) (F near zero in a
         # dilute solution)
@@ -642,6 +647,10 @@ def test_batch1d_sample_matches_reference():
         assert abs(r["grand_potential"] - d["grand_potential"]) < tol * max(
             1.0, abs(d["free_energy"]) + abs(d["grand_potential"]))
         if d["iterations"] <= 300:
-            assert abs(info["iterations"] - d["iterations"]) <= max(2, int(0.1 * d["iterations"])), d["draw"]
+            # within the range the reference's own DIIS spans from starting
+            # fields perturbed in the last bits (make_batch1d_spread.py: draw
+            # 10 takes 73-149 iterations, the short solves always the same)
+            lo, hi = d["iterations_spread"]
+            assert lo - max(2, int(0.1 * lo)) <= info["iterations"] <= hi + max(2, int(0.1 * hi)), d["draw"]
     for s in systems:
         s.close()
