@@ -248,10 +248,14 @@ int scf_residual(scf_system *sys, const double *x, double *g, double *residual);
  *     (src/sfnewton.cpp:636-747): solved by a Jacobian-free Newton-Krylov
  *     method of this library instead (GMRES on forward-difference J v,
  *     backtracking line search) that reaches the same fixed point by a
- *     different path; `iterations` counts its Newton steps.
- * The same rule holds for scf_solve_batch and scf_solve_resident (Newton
- * systems are solved one after another, DIIS systems in one launch); the job
- * queue and the request server solve every input by DIIS. */
+ *     different path; `iterations` counts its Newton steps;
+ *   - LBFGS -> the reference's LBFGSpp-based solver (src/LBFGS.h,
+ *     src/solve_scf.cpp:526-556,630-644), its steps restated over device
+ *     residuals (two-loop recursion, trust-radius step, m = 6 pairs by
+ *     default).
+ * The same rule holds for scf_solve_batch and scf_solve_resident (Newton and
+ * LBFGS systems are solved one after another, DIIS systems in one launch);
+ * the job queue and the request server solve every input by DIIS. */
 int scf_solve(scf_system *sys, double *x, scf_solve_info *info);
 
 /* Solve n independent systems in one launch (one workgroup per system);

@@ -89,7 +89,11 @@ int validate(const SystemSpec &s, std::string &why) {
   }
   if (nb > kMaxBlocks) return why = "more than 16 blocks over all molecules", SCF_E_UNSUPPORTED;
   if (nmm > kMaxMM) return why = "more than 16 molecule-segment pairs", SCF_E_UNSUPPORTED;
-  if (s.solver.m > kMaxDIIS || s.solver.m < 1) return why = "DIIS history m must be 1..16", SCF_E_UNSUPPORTED;
+  if (s.solver.method == SOLVER_LBFGS) {
+    if (s.solver.m < 1) return why = "LBFGS needs at least one correction pair (m >= 1)", SCF_E_UNSUPPORTED;
+  } else if (s.solver.m > kMaxDIIS || s.solver.m < 1) {
+    return why = "DIIS history m must be 1..16", SCF_E_UNSUPPORTED;
+  }
   if (save_memory_interval(s) > 1)
     for (auto &m : s.mol)
       if (m.markov == 2)
@@ -292,7 +296,9 @@ int build(scf_system *S, const Placement *pl) {
   h.mol_mm_begin[nmol] = nmm;
   h.nblocks = nb;
   h.nmm = nmm;
-  h.m = s.solver.m;
+  // DIIS history of the device solvers (the job queue and the server solve
+  // every input by DIIS); an LBFGS input's m counts its correction pairs
+  h.m = s.solver.method == SOLVER_LBFGS ? std::min(s.solver.m, kMaxDIIS) : s.solver.m;
   h.itlim = s.solver.iterationlimit;
   h.max_g = s.solver.max_g;
   h.deltamax = s.solver.deltamax;
@@ -789,10 +795,14 @@ int scf_autotune_batch(scf_system *const *systems, int n, int evals, scf_batch_t
 
 int scf_system_set_solver(scf_system *S, double tolerance, double deltamax, int m, int iterationlimit) {
   if (!S) return fail(SCF_E_ARG, "null system");
-  if (m > S->m_alloc) return fail(SCF_E_UNSUPPORTED, "DIIS history m above the input's setting");
+  const bool lbfgs = S->spec.solver.method == SOLVER_LBFGS;
+  if (m > S->m_alloc && !lbfgs) return fail(SCF_E_UNSUPPORTED, "DIIS history m above the input's setting");
   if (tolerance > 0) S->h.tol = S->spec.solver.tolerance = tolerance;
   if (deltamax > 0) S->h.deltamax = S->spec.solver.deltamax = deltamax;
-  if (m > 0) S->h.m = S->spec.solver.m = m;
+  if (m > 0) {
+    S->spec.solver.m = m;  // LBFGS: correction pairs; the device DIIS keeps what its arena holds
+    S->h.m = lbfgs ? std::min(m, S->m_alloc) : m;
+  }
   if (iterationlimit >= 0) S->h.itlim = S->spec.solver.iterationlimit = iterationlimit;
   return upload_solver(S);
 }
@@ -1011,7 +1021,7 @@ int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info) 
   for (int i = 0; i < n; i++) {
     scf_system *S = systems[i];
     if (S->newton()) {
-      int rc = newton_solve(S, &info[i]);
+      int rc = S->spec.solver.method == SOLVER_LBFGS ? lbfgs_solve(S, &info[i]) : newton_solve(S, &info[i]);
       if (rc != SCF_OK) return rc;
     } else if (S->effective_mode() == SCF_MODE_GRID) {
       HIP_TRY(hipEventRecord(S->ev0, S->stream));

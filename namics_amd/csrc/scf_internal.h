@@ -73,6 +73,8 @@ int read_scalars(const scf_system *S, DevScalars &sc);
 int grid_solve(scf_system *S, scf_solve_info *info);
 // Newton-Krylov solve of a pseudohessian / hessian input (scf_newton.hip)
 int newton_solve(scf_system *S, scf_solve_info *info);
+// LBFGS solve of an input with `method : LBFGS` (scf_newton.hip)
+int lbfgs_solve(scf_system *S, scf_solve_info *info);
 double wall_clock_khz();
 size_t lds_limit();
 void plan_layout(scf_system *S, int want_sx);

@@ -250,4 +250,126 @@ int newton_solve(scf_system *S, scf_solve_info *info) {
   return SCF_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// LBFGS solver for `newton : method : LBFGS` inputs.
+//
+// The reference runs an adaptation of the LBFGSpp solver (Y. Qiu, MIT
+// licence; src/LBFGS.h, src/LBFGSpp/BFGSMat.h, driven from
+// src/solve_scf.cpp:526-556,630-644) with the SCF residual g(x) in the role of
+// the gradient and ||g||_2 as the value; this restates its steps:
+//
+//   * the direction is -H g, H the limited-memory inverse-Hessian estimate of
+//     the last m corrections (s = x_k - x_{k-1}, y = g_k - g_{k-1}), applied
+//     by the two-loop recursion (Nocedal & Wright, Numerical Optimization,
+//     2nd ed., Algorithm 7.4) from H0 = (s.y / y.y) I of the newest pair;
+//   * the step x_new = x + alpha d has alpha = min(1, r / ||d||) for a trust
+//     radius r: before each step r shrinks to 2 alpha_prev ||d|| when it
+//     exceeds that, is multiplied by the factor the previous step earned,
+//     and is kept in [deltamin, deltamax]; a step's factor grows when the
+//     new residual is small or turned away from the old one and shrinks when
+//     it grew (`step_factor`), times alpha / (r / ||d||) of the step taken;
+//   * no line search: every step is taken; the run stops at ||g||_2 <=
+//     tolerance or after iterationlimit steps.
+//
+// Vectors live on the host (m correction pairs, 2 m + 4 field-sized
+// vectors), as for the Newton solver; every residual is the device's.
+namespace {
+
+// factor of the trust radius earned by a step from residual g0 to g
+double step_factor(const std::vector<double> &g, const std::vector<double> &g0) {
+  const double n0 = norm2(g0);
+  double ratio = dot(g, g0) / n0 / n0;
+  double q = norm2(g) / n0;
+  q = q * q;
+  if ((ratio > 1 || q > 1) && q - ratio * std::fabs(ratio) < 0.2) q = 1.5 * q;
+  if (ratio < 0 && q < 2) return 1;
+  if (q > 10) return 0.01;
+  return 0.4 * (1 + 0.75 * q) / (q - ratio * std::fabs(ratio) + 0.1);
+}
+
+}  // namespace
+
+int lbfgs_solve(scf_system *S, scf_solve_info *info) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const DevSys &h = S->h;
+  const size_t n = (size_t)h.nit * h.M;
+  const int m = S->spec.solver.m;
+  const double tol = h.tol, dmax = h.deltamax, dmin = S->spec.solver.deltamin;
+  const int itlim = h.itlim;
+  std::vector<double> x(n), g(n), xp(n), gp(n), d(n);
+  std::vector<std::vector<double>> sv(m, std::vector<double>(n)), yv(m, std::vector<double>(n));
+  std::vector<double> ys(m), al(m);
+  HIP_TRY(hipMemcpyAsync(x.data(), h.x, n * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  HIP_TRY(hipStreamSynchronize(S->stream));
+  double res;
+  int status;
+  int rc = evaluate(S, x.data(), g.data(), &res, &status, false);
+  if (rc != SCF_OK) return rc;
+  double gnorm = norm2(g);
+  int it = 1;
+  if (status == DS_OK && gnorm > tol) {
+    double radius = dmax, factor = 1, alpha = 1, theta = 1;
+    int npairs = 0, next = 0;
+    for (size_t i = 0; i < n; i++) d[i] = -g[i];
+    for (;;) {
+      xp = x;
+      gp = g;
+      const double dn = norm2(d);
+      if (dn > 0 && radius > 2 * alpha * dn) radius = 2 * alpha * dn;
+      radius *= factor;
+      factor = 1;
+      radius = std::min(dmax, std::max(dmin, radius));
+      const double bound = radius / dn;
+      alpha = bound > 1 ? 1 : bound;
+      for (size_t i = 0; i < n; i++) x[i] = xp[i] + alpha * d[i];
+      rc = evaluate(S, x.data(), g.data(), &res, &status, false);
+      if (rc != SCF_OK) return rc;
+      if (status != DS_OK) break;
+      gnorm = norm2(g);
+      factor *= step_factor(g, gp);
+      factor *= alpha / bound;
+      if (alpha < tol) alpha *= 2;
+      if (gnorm <= tol || it >= itlim) break;
+      // newest correction pair
+      std::vector<double> &s = sv[next], &y = yv[next];
+      for (size_t i = 0; i < n; i++) {
+        s[i] = x[i] - xp[i];
+        y[i] = g[i] - gp[i];
+      }
+      ys[next] = dot(s, y);
+      theta = dot(y, y) / ys[next];
+      next = (next + 1) % m;
+      if (npairs < m) npairs++;
+      // d = -H g by the two-loop recursion, newest pair first
+      for (size_t i = 0; i < n; i++) d[i] = -g[i];
+      int j = next;
+      for (int q = 0; q < npairs; q++) {
+        j = (j + m - 1) % m;
+        al[j] = dot(sv[j], d) / ys[j];
+        for (size_t i = 0; i < n; i++) d[i] -= al[j] * yv[j][i];
+      }
+      for (size_t i = 0; i < n; i++) d[i] /= theta;
+      for (int q = 0; q < npairs; q++) {
+        const double beta = dot(yv[j], d) / ys[j];
+        for (size_t i = 0; i < n; i++) d[i] += (al[j] - beta) * sv[j][i];
+        j = (j + 1) % m;
+      }
+      it++;
+    }
+  }
+  // leave the device state (fields, densities, F, Omega, mu) at the final x
+  rc = evaluate(S, x.data(), nullptr, &res, &status, true);
+  if (rc != SCF_OK) return rc;
+  info->iterations = it;
+  info->residual = res;
+  info->status = status != DS_OK ? status : (gnorm <= tol ? SCF_OK : SCF_NOT_CONVERGED);
+  info->fp32_iterations = 0;
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  info->kernel_ms = ms;
+  info->solve_ms = ms;
+  S->have_results = status == DS_OK;
+  return SCF_OK;
+}
+
 }  // namespace nscf

@@ -1045,13 +1045,26 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
         so.method = SOLVER_PSEUDOHESSIAN;
       else if (v == "hessian")
         so.method = SOLVER_HESSIAN;
+      else if (v == "LBFGS")
+        so.method = SOLVER_LBFGS;
       else {
-        err = "newton method '" + v + "' is not supported (DIIS, pseudohessian, hessian)";
+        err = "newton method '" + v + "' is not supported (DIIS, pseudohessian, hessian, LBFGS)";
         return false;
       }
     }
-    if (get(p, "newton", nn, "m", v)) {
+    // Solve_scf::CheckInput (src/solve_scf.cpp:170-175,246-250): LBFGS keeps
+    // m = 6 correction pairs unless given (0..1000), the other methods a DIIS
+    // history of 10 (0..100); deltamin defaults to 0, out of 0..100 it is
+    // deltamax / 1e5
+    if (so.method == SOLVER_LBFGS) {
+      so.m = 6;
+      if (get(p, "newton", nn, "m", v))
+        if (!to_int(v, so.m) || so.m < 0 || so.m > 1000) so.m = 6;
+    } else if (get(p, "newton", nn, "m", v)) {
       if (!to_int(v, so.m) || so.m < 0 || so.m > 100) so.m = 10;
+    }
+    if (get(p, "newton", nn, "deltamin", v)) {
+      if (!to_real(v, so.deltamin) || so.deltamin < 0 || so.deltamin > 100) so.deltamin = so.deltamax / 100000;
     }
     if (get(p, "newton", nn, "stop_criterion", v)) {
       so.max_g = (v == "max_of_element_of_|g|");

@@ -31,7 +31,7 @@ enum BoundaryKind : int { BC_MIRROR = 0, BC_PERIODIC = 1, BC_SURFACE = 2 };
 enum LatticeType : int { LAT_SIMPLE_CUBIC = 0, LAT_HEXAGONAL = 1 };
 enum SegFreedom : int { SEG_FREE = 0, SEG_FROZEN = 1, SEG_PINNED = 2 };
 enum MolFreedom : int { MOL_FREE = 0, MOL_RESTRICTED = 1, MOL_SOLVENT = 2 };
-enum SolverMethod : int { SOLVER_DIIS = 0, SOLVER_PSEUDOHESSIAN = 1, SOLVER_HESSIAN = 2 };
+enum SolverMethod : int { SOLVER_DIIS = 0, SOLVER_PSEUDOHESSIAN = 1, SOLVER_HESSIAN = 2, SOLVER_LBFGS = 3 };
 
 struct LatticeSpec {
   int gradients = 1;
@@ -103,7 +103,8 @@ struct SolverSpec {
   int method = SOLVER_PSEUDOHESSIAN;
   double tolerance = 1e-7;
   double deltamax = 0.1;
-  int m = 10;
+  double deltamin = 0.0;       // smallest step bound of the LBFGS trust region
+  int m = 10;                  // DIIS history, or the LBFGS correction pairs (default 6)
   int iterationlimit = 1000;
   bool max_g = false;          // stop on max|g| instead of ||g||_2
 };

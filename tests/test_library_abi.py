@@ -78,3 +78,19 @@ def test_ctypes_structs_match_header(tmp_path):
         assert int(got[cname]) == ctypes.sizeof(py), cname
         for f, _ in py._fields_:
             assert int(got[cname + "." + f]) == getattr(py, f).offset, (cname, f)
+
+
+def test_lbfgs_input_keys():
+    """`newton : method : LBFGS` is accepted with the reference's defaults
+    (Solve_scf::CheckInput src/solve_scf.cpp:170-175,246-250): m = 6
+    correction pairs unless given, out-of-range m falls back to 6."""
+    from namics_amd.scf import describe_input
+    from tests.golden.cases import CFG1
+    d = describe_input(CFG1 + "\nnewton : isaac : method : LBFGS\n")
+    assert d["solver"]["method"] == 3 and d["solver"]["m"] == 6
+    d = describe_input(CFG1 + "\nnewton : isaac : method : LBFGS\nnewton : isaac : m : 40\n")
+    assert d["solver"]["m"] == 40
+    d = describe_input(CFG1 + "\nnewton : isaac : method : LBFGS\nnewton : isaac : m : 5000\n")
+    assert d["solver"]["m"] == 6
+    d = describe_input(CFG1 + "\nnewton : isaac : method : DIIS\n")
+    assert d["solver"]["method"] == 0 and d["solver"]["m"] == 10
