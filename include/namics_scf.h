@@ -223,9 +223,16 @@ int scf_read_guess(scf_system *sys, const char *path);
  * `out_info : folder : path` is given, relative to the working directory).
  * A `var : <kind>-<name> : scan : <property>` block (mol theta, n, phibulk;
  * mon chi_X) solves every scan point in turn, each from the previous one.
+ * A `var : mol-<name> : search : theta | n | phibulk` block with a target
+ * (`var : sys-<name> : grand_potential | free_energy : value`, or
+ * `var : mol-<other> : theta | n | phibulk | mu : value`) varies the searched
+ * property, each solve from the previous one, until the target's error is
+ * within newton's super_tolerance (10 x tolerance), by this library's own
+ * secant / Illinois regula-falsi root finder, and writes the final state.
  * *n_solves counts the solves (start blocks times scan points), *n_converged
  * those that converged.  Replaces: the start and scan loops of main
- * (src/namics.cpp:163-737, Variate::PutVarScan src/variate.cpp:361-400). */
+ * (src/namics.cpp:163-737, Variate::PutVarScan src/variate.cpp:361-400,
+ * Solve_scf::SuperIterate src/solve_scf.cpp:717-750). */
 int scf_run_input_file(const char *path, int *n_solves, int *n_converged);
 
 /* Override solver settings: tolerance, deltamax, DIIS history m, iteration

@@ -24,6 +24,7 @@
 // output folder (./output/ unless `out_info : folder : path` says otherwise).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -224,6 +225,109 @@ std::string output_stem(const std::string &input_path, bool use_output_folder) {
     if (k != std::string::npos) stem = stem.substr(k + 1);
   }
   return stem;
+}
+
+// ---- var search (see VarSearch in scf_spec.h)
+
+// Set the searched property of the system to X (Molecule::PutValue,
+// src/molecule.cpp:995-1015): theta fixes n = theta / N, n fixes theta = n N,
+// phibulk the bulk volume fraction of a free molecule.
+static int put_search_value(scf_system *S, double X) {
+  const VarSearch &vs = S->spec.search;
+  MoleculeSpec &m = S->spec.mol[vs.mol];
+  if (vs.prop == "theta") {
+    m.theta = X;
+    m.n = X / m.chainlength;
+  } else if (vs.prop == "n") {
+    m.n = X;
+    m.theta = X * m.chainlength;
+  } else {
+    m.phibulk = X;
+  }
+  S->h.mol_n_in[vs.mol] = m.n;
+  S->h.mol_phibulk_in[vs.mol] = m.phibulk;
+  return upload_solver(S);
+}
+
+// Error of the target after a solve (System::GetError src/system.cpp:
+// 1405-1427, Molecule::GetError src/molecule.cpp:942-969).
+static double search_error(const VarSearch &vs, const scf_results &r) {
+  const double v = vs.target_value;
+  if (vs.target_kind == "sys") return vs.target_prop == "free_energy" ? r.free_energy - v : -(r.grand_potential - v);
+  const int i = vs.target_mol;
+  if (vs.target_prop == "theta") return v != 0 ? r.theta[i] / v - 1.0 : r.theta[i];
+  if (vs.target_prop == "n") return v != 0 ? r.n[i] / v - 1.0 : r.n[i];
+  if (vs.target_prop == "phibulk") return -(v != 0 ? r.phibulk[i] / v - 1.0 : r.phibulk[i]);
+  return v != 0 ? r.mu[i] / v - 1.0 : r.mu[i];
+}
+
+// Root of the target error over the searched value; every evaluation is a
+// solve that starts from the previous one's fields.  Secant steps, each at
+// most super_deltamax times the current value (and halving it rather than
+// crossing zero), until the error changes sign; from then on regula falsi
+// with the Illinois modification (Dowell & Jarratt, BIT 11 (1971) 168), which
+// keeps the root bracketed.  Stops when |error| <= super_tolerance (*met) or
+// after super_iterationlimit evaluations; `info` describes the last solve.
+static int run_search(scf_system *S, scf_solve_info *info, bool *met, int *evaluations) {
+  const VarSearch &vs = S->spec.search;
+  *met = false;
+  *evaluations = 0;
+  bool solved = true;
+  auto eval = [&](double X, double &e) -> int {
+    int rc = put_search_value(S, X);
+    if (rc == SCF_OK) rc = scf_solve_resident(&S, 1, info);
+    (*evaluations)++;
+    if (rc != SCF_OK) return rc;
+    solved = info->status == SCF_OK;
+    scf_results r;
+    rc = scf_get_results(S, &r);
+    e = search_error(vs, r);
+    return rc;
+  };
+  double a = vs.start_value, fa = 0, b = 0, fb = 0;
+  int rc = eval(a, fa);
+  if (rc != SCF_OK || !solved) return rc;
+  if (std::fabs(fa) <= vs.super_tolerance) {
+    *met = true;
+    return SCF_OK;
+  }
+  b = a != 0 ? a * 1.01 : 1e-3;
+  rc = eval(b, fb);
+  if (rc != SCF_OK || !solved) return rc;
+  bool bracket = fa * fb < 0;
+  for (int it = 0; it < vs.super_iterationlimit; it++) {
+    if (std::fabs(fb) <= vs.super_tolerance) {
+      *met = true;
+      return SCF_OK;
+    }
+    if (fb == fa) return SCF_OK;  // no slope: the search stops unmet
+    double c = b - fb * (b - a) / (fb - fa);
+    if (!bracket) {
+      const double lim = vs.super_deltamax * std::fabs(b);
+      if (c > b + lim) c = b + lim;
+      if (c < b - lim) c = b - lim;
+      if (c <= 0) c = 0.5 * b;  // theta, n and phibulk stay positive
+    }
+    double fc = 0;
+    rc = eval(c, fc);
+    if (rc != SCF_OK || !solved) return rc;
+    if (bracket) {
+      if (fc * fb < 0) {
+        a = b;
+        fa = fb;
+      } else {
+        fa *= 0.5;
+      }
+    } else {
+      a = b;
+      fa = fb;
+      bracket = fc * fb < 0;
+    }
+    b = c;
+    fb = fc;
+  }
+  *met = std::fabs(fb) <= vs.super_tolerance;
+  return SCF_OK;
 }
 
 }  // namespace nscf
@@ -456,7 +560,15 @@ int scf_run_input_file(const char *path, int *n_starts_out, int *n_converged_out
         rc = scf_carry_fields(S, prev);
       }
       scf_solve_info info{};
-      if (rc == SCF_OK) rc = scf_solve_resident(&S, 1, &info);
+      if (rc == SCF_OK && s.search.present) {
+        // var search: solves until the target is met; counted as one solve
+        bool met = false;
+        int evals = 0;
+        rc = run_search(S, &info, &met, &evals);
+        if (rc == SCF_OK && info.status == SCF_OK && !met) info.status = SCF_NOT_CONVERGED;
+      } else if (rc == SCF_OK) {
+        rc = scf_solve_resident(&S, 1, &info);
+      }
       solves++;
       if (rc == SCF_OK && info.status == SCF_OK) converged++;
       const std::string stem = output_stem(path, s.out.use_output_folder);

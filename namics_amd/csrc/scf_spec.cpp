@@ -1143,6 +1143,79 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     }
     vs.present = true;
   }
+  // var search with its target (see VarSearch)
+  {
+    VarSearch &vs = s.search;
+    int n_search = 0, n_target = 0;
+    for (auto &ln : p.lines) {
+      if (ln.key != "var") continue;
+      const size_t dash = ln.name.find('-');
+      if (dash == std::string::npos) continue;
+      const std::string kind = ln.name.substr(0, dash), name = ln.name.substr(dash + 1);
+      if (ln.prop == "search") {
+        n_search++;
+        if (kind != "mol" || (ln.value != "theta" && ln.value != "n" && ln.value != "phibulk")) {
+          err = "var search of " + ln.name + " : " + ln.value + " is not supported (mol theta, n, phibulk)";
+          return false;
+        }
+        for (size_t i = 0; i < s.mol.size(); i++)
+          if (s.mol[i].name == name) {
+            const MoleculeSpec &m = s.mol[i];
+            vs.mol = (int)i;
+            vs.start_value = ln.value == "theta" ? m.theta : (ln.value == "n" ? m.n : m.phibulk);
+          }
+        if (vs.mol < 0) {
+          err = "var search: no molecule " + name;
+          return false;
+        }
+        vs.name = name;
+        vs.prop = ln.value;
+      }
+      const bool sys_target = kind == "sys" && (ln.prop == "free_energy" || ln.prop == "grand_potential");
+      const bool mol_target =
+          kind == "mol" && (ln.prop == "theta" || ln.prop == "n" || ln.prop == "phibulk" || ln.prop == "mu");
+      if (sys_target || mol_target) {
+        n_target++;
+        vs.target_kind = kind;
+        vs.target_prop = ln.prop;
+        if (!to_real(ln.value, vs.target_value)) {
+          err = "var target " + ln.name + " : " + ln.prop + " needs a number";
+          return false;
+        }
+        if (mol_target) {
+          for (size_t i = 0; i < s.mol.size(); i++)
+            if (s.mol[i].name == name) vs.target_mol = (int)i;
+          if (vs.target_mol < 0) {
+            err = "var target: no molecule " + name;
+            return false;
+          }
+        }
+      } else if (kind == "sys" && ln.prop == "Laplace_pressure") {
+        err = "var target Laplace_pressure is not supported";
+        return false;
+      }
+    }
+    if (n_search > 1 || n_target > 1) {
+      err = "at most one var search and one target (Variate, as in the reference)";
+      return false;
+    }
+    if (n_search != n_target) {
+      err = n_search ? "lonely search: a var search needs a target" : "lonely target: a var target needs a search";
+      return false;
+    }
+    if (n_search) {
+      vs.present = true;
+      vs.super_tolerance = 10 * so.tolerance;
+      vs.super_iterationlimit = so.iterationlimit / 10;
+      if (get(p, "newton", nn, "super_tolerance", v)) to_real(v, vs.super_tolerance);
+      if (get(p, "newton", nn, "super_iterationlimit", v)) to_int(v, vs.super_iterationlimit);
+      if (get(p, "newton", nn, "super_deltamax", v)) to_real(v, vs.super_deltamax);
+      if (s.scan.present) {
+        err = "a var search and a var scan in one start block are not supported";
+        return false;
+      }
+    }
+  }
   // output requests (Output::CheckInput / Load, src/output.cpp:32-178):
   // `output : kal | pro : option : value` and `kal | pro : key : name : prop`
   // item lines in input order; name `*` expands to every molecule / segment

@@ -146,6 +146,27 @@ struct VarScan {
   int num_of_steps = 0;          // scan points after the first
 };
 
+// A `var : mol-<name> : search : theta | n | phibulk` request paired with a
+// target, `var : sys-<name> : grand_potential | free_energy : value` or
+// `var : mol-<name> : theta | n | phibulk | mu : value` (Variate::CheckInput,
+// Molecule::PutVarInfo src/molecule.cpp:717-776, System::PutVarInfo
+// src/system.cpp:1365-1400): the searched property is varied until the
+// target's error (Molecule::GetError src/molecule.cpp:942-969,
+// System::GetError src/system.cpp:1405-1427) is within super_tolerance.
+struct VarSearch {
+  bool present = false;
+  std::string name, prop;           // searched molecule and property
+  int mol = -1;
+  double start_value = 0;
+  std::string target_kind;          // "sys" or "mol"
+  std::string target_prop;          // free_energy, grand_potential; theta, n, phibulk, mu
+  int target_mol = -1;
+  double target_value = 0;
+  double super_tolerance = 0;       // newton super_tolerance (default 10 x tolerance)
+  double super_deltamax = 0.5;      // newton super_deltamax
+  int super_iterationlimit = 0;     // newton super_iterationlimit (default iterationlimit / 10)
+};
+
 struct SystemSpec {
   std::string sys_name = "NN";
   std::string newton_name = "NN";
@@ -163,6 +184,7 @@ struct SystemSpec {
   int start = 1;               // the start block this system was built for
   OutputSpec out;
   VarScan scan;
+  VarSearch search;
   std::string lattice_type_name, method_name = "pseudohessian", stop_criterion_name = "norm_of_g";
   std::vector<std::string> mol_composition, seg_freedom_name, mol_freedom_name;
   std::vector<double> seg_valence;

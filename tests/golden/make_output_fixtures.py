@@ -54,6 +54,15 @@ kal : lat : flat : lattice_type
 kal : lat : flat : n_layers
 """
 
+SEARCH_KAL = """output : kal : append : false
+kal : sys : NN : grand_potential
+kal : sys : NN : free_energy
+kal : mol : pol : phibulk
+kal : mol : pol : theta
+kal : mol : pol : Mu
+"""
+DIIS10 = "newton : isaac : method : DIIS\nnewton : isaac : tolerance : 1e-10\nnewton : isaac : iterationlimit : 3000\n"
+
 INPUTS = {
     # two start blocks: kal rows appended, one pro file per start, the
     # second start from the first one's fields, a guess file per start
@@ -132,6 +141,22 @@ pro : mol : water : phi
 output : vtk : append : false
 vtk : mol : surf : phi
 """ + DIIS + "start\n",
+    # var searches: the bulk volume fraction of the adsorbing polymer varied
+    # until the grand potential reaches a set value, and (at another chi)
+    # until the solvent's amount theta does (a search and a target on one
+    # molecule give NaN results in the reference); each from the first
+    # start block's solution
+    "search_omega": CASES["cfg1_planar_adsorption"]["text"] + SEARCH_KAL + DIIS10 + """start
+var : mol-pol : search : phibulk
+var : sys-NN : grand_potential : -0.0025
+start
+""",
+    "search_theta": CASES["cfg1_planar_adsorption"]["text"] + SEARCH_KAL + DIIS10 + """mon : A : chi_W : 0.6
+start
+var : mol-pol : search : phibulk
+var : mol-water : theta : 98.4
+start
+""",
 }
 
 
