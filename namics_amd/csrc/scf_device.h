@@ -1058,14 +1058,39 @@ __device__ inline void scalars_mu(const DevSys &S) {
   }
 }
 
+// Lane exchanges for the DIIS solve.  dpp_f64/dpp_i32 move a value across
+// lanes within rows of 16 by DPP (a VALU operand modifier, no LDS round
+// trip): 0xB1 = quad_perm [1,0,3,2] (lane ^ 1), 0x4E = quad_perm [2,3,0,1]
+// (lane ^ 2), 0x124 / 0x128 = row_ror 4 / 8.  readlane_f64 reads one lane's
+// value (the lane index must be the same on every lane).
+template <int kCtrl>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, false);
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i32<kCtrl>((int)b), hi = dpp_i32<kCtrl>((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // DIIS coefficients on the device by one wavefront: c = A^-1 (1..1) / sum,
-// A is the kd x kd Gram matrix (leading dimension kd).  Lane r holds row r;
-// Gauss-Jordan elimination with partial pivoting (pivot = largest |a_rk|
-// among rows not yet used).  Mathematically the reference's SVD solve
-// (SFNewton::Ax src/sfnewton.cpp:793-835) computes the same vector; a
-// parallel pivoted elimination takes microseconds where a single-lane SVD
-// would take a large part of an iteration.  Returns false on NaN / zero pivot.
-__device__ inline bool wave_diis_coefficients(const double *A, int n, double *C) {
+// A is the kd x kd Gram matrix (leading dimension kd, kd <= kMaxDIIS = 16).
+// Lane r holds row r; Gauss-Jordan elimination with partial pivoting (pivot
+// = largest |a_rk| among rows not yet used, lowest lane on ties).
+// Mathematically the reference's SVD solve (SFNewton::Ax
+// src/sfnewton.cpp:793-835) computes the same vector.  The pivot search is a
+// max-reduction over lanes 0..15 by DPP (the winner is the unique largest
+// (|a|, -lane) pair, whatever order the lanes are combined in), and the pivot
+// row is read by readlane, so an elimination step has no LDS round trip.
+// Returns false on NaN / zero pivot.
+static_assert(kMaxDIIS <= 16, "wave_diis_coefficients: rows live in lanes 0..15");
+__device__ __forceinline__ bool wave_diis_coefficients(const double *A, int n, double *C) {
   const int lane = threadIdx.x & 63;
   double row[kMaxDIIS];
   double rhs = 1.0;
@@ -1082,26 +1107,30 @@ __device__ inline bool wave_diis_coefficients(const double *A, int n, double *C)
     double cand = (lane < n && !used) ? fabs(ak) : -1.0;
     if (cand != cand) cand = 1e308;  // a NaN is picked and reported
     int who = lane;
-    for (int off = 32; off > 0; off >>= 1) {
-      const double oc = __shfl_xor(cand, off, 64);
-      const int ow = __shfl_xor(who, off, 64);
+    auto take = [&](double oc, int ow) {
       if (oc > cand || (oc == cand && ow < who)) {
         cand = oc;
         who = ow;
       }
-    }
+    };
+    take(dpp_f64<0xB1>(cand), dpp_i32<0xB1>(who));
+    take(dpp_f64<0x4E>(cand), dpp_i32<0x4E>(who));
+    take(dpp_f64<0x124>(cand), dpp_i32<0x124>(who));
+    take(dpp_f64<0x128>(cand), dpp_i32<0x128>(who));
+    who = __builtin_amdgcn_readfirstlane(who);  // lane 0's row of 16 holds every candidate
     if (lane == who) {
       used = true;
       pc = k;
     }
-    const double piv = __shfl(ak, who, 64);
-    const double prhs = __shfl(rhs, who, 64);
+    const double piv = readlane_f64(ak, who);
+    const double prhs = readlane_f64(rhs, who);
     if (!(piv != 0.0) || piv != piv) bad = true;
     const bool upd = lane < n && lane != who && piv != 0.0;
     const double f = upd ? ak / piv : 0.0;
 #pragma unroll
     for (int c = 0; c < kMaxDIIS; c++) {
-      const double pcv = __shfl(row[c], who, 64);
+      if (c >= n) break;  // n is the same on every lane
+      const double pcv = readlane_f64(row[c], who);
       if (upd) row[c] = (c == k) ? 0.0 : row[c] - f * pcv;
     }
     if (upd) rhs -= f * prhs;

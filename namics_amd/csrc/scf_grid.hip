@@ -28,6 +28,28 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
   return t;
 }
 
+// block_sum of acc[0..ncols-1] at once, in the same order (lane tree, then
+// waves 0, 1, ... in turn) behind one barrier; the workgroup's sum of column
+// i goes to partial[i * gridDim.x + blockIdx.x].  red needs kMaxDIIS * waves
+// doubles.
+__device__ __forceinline__ void block_sums_store(const double *acc, int ncols, double *red, double *partial) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int i = 0; i < kMaxDIIS; i++) {
+    if (i < ncols) {
+      double v = acc[i];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) red[i * nw + wave] = v;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < ncols) {
+    double t = 0;
+    for (int w = 0; w < nw; w++) t += red[threadIdx.x * nw + w];
+    partial[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
+}
+
 __device__ __forceinline__ double block_max(double v, double *red) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -57,24 +79,29 @@ __device__ __forceinline__ double ordered_max(const double *partial, int nparts,
   return block_max(t, red);
 }
 
-// Column sums of partial[nparts][kMaxDIIS], columns 0..ncols-1, in one pass:
-// thread t adds rows t, t + blockDim.x, ... in order (loads of two rows in
-// flight), the waves reduce by lane exchange, and the per-wave sums of all
-// columns meet in LDS behind one barrier (red needs kMaxDIIS * waves
-// doubles).  The sums land in out[] and are visible to the workgroup on
+// Column sums of the column-major partial[kMaxDIIS][nparts] (entry (b, i) at
+// i * nparts + b), columns 0..ncols-1, in a fixed order: thread t adds rows
+// t, t + blockDim.x, ... (loads of two rows in flight, consecutive threads on
+// consecutive addresses); the per-thread sums go to LDS, 16 threads per
+// column add 16 of them each, and one thread per column adds those 16.  No
+// lane-exchange chains: the whole reduction is two rounds of LDS reads.
+// Needs blockDim.x == 16 * kMaxDIIS and red of kMaxDIIS * blockDim.x
+// doubles.  The sums land in out[] and are visible to the workgroup on
 // return.
+static_assert(kGridThreads == 16 * kMaxDIIS, "ordered_sums: 16 threads per column");
 __device__ __forceinline__ void ordered_sums(const double *partial, int nparts, int ncols, double *out,
                                              double *red) {
   double acc[kMaxDIIS];
 #pragma unroll
   for (int i = 0; i < kMaxDIIS; i++) acc[i] = 0;
-  int b = threadIdx.x;
-  for (; b + (int)blockDim.x < nparts; b += 2 * blockDim.x) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  int b = t;
+  for (; b + nt < nparts; b += 2 * nt) {
     double a0[kMaxDIIS], a1[kMaxDIIS];
 #pragma unroll
     for (int i = 0; i < kMaxDIIS; i++) {
-      a0[i] = i < ncols ? partial[(size_t)b * kMaxDIIS + i] : 0.0;
-      a1[i] = i < ncols ? partial[(size_t)(b + blockDim.x) * kMaxDIIS + i] : 0.0;
+      a0[i] = i < ncols ? partial[(size_t)i * nparts + b] : 0.0;
+      a1[i] = i < ncols ? partial[(size_t)i * nparts + b + nt] : 0.0;
     }
 #pragma unroll
     for (int i = 0; i < kMaxDIIS; i++) acc[i] = (acc[i] + a0[i]) + a1[i];
@@ -82,19 +109,22 @@ __device__ __forceinline__ void ordered_sums(const double *partial, int nparts, 
   if (b < nparts)
 #pragma unroll
     for (int i = 0; i < kMaxDIIS; i++)
-      if (i < ncols) acc[i] += partial[(size_t)b * kMaxDIIS + i];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+      if (i < ncols) acc[i] += partial[(size_t)i * nparts + b];
 #pragma unroll
-  for (int i = 0; i < kMaxDIIS; i++) {
-    double v = acc[i];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) red[i * nw + wave] = v;
-  }
+  for (int i = 0; i < kMaxDIIS; i++)
+    if (i < ncols) red[i * nt + t] = acc[i];
   __syncthreads();
-  if ((int)threadIdx.x < ncols) {
-    double t = 0;
-    for (int w = 0; w < nw; w++) t += red[threadIdx.x * nw + w];
-    out[threadIdx.x] = t;
+  const int c = t >> 4, q = t & 15, per = nt >> 4;
+  double v = 0;
+  if (c < ncols)
+    for (int j = 0; j < per; j++) v += red[c * nt + j * 16 + q];
+  __syncthreads();
+  if (c < ncols) red[c * 16 + q] = v;
+  __syncthreads();
+  if (t < ncols) {
+    double u = 0;
+    for (int j = 0; j < 16; j++) u += red[t * 16 + j];
+    out[t] = u;
   }
   __syncthreads();
 }
@@ -175,8 +205,8 @@ __global__ void __launch_bounds__(kGridThreads) k_step_bwd(const DevSys *__restr
 }
 
 // 3-gradient steps (LGrad3::propagate src/LGrad3.cpp:731-743): one thread
-// per site of an x-plane, blockIdx.y = x - 1, the (y, z) plane flattened
-// with z fastest, so no division per site and every load is coalesced.
+// per site of an x-plane, (y, z) flattened with z fastest, so every load is
+// coalesced; a workgroup covers part of one plane.
 __device__ __forceinline__ double stencil3(CSys &S, const gdbl *__restrict__ g, int x, int y, int z) {
   const int JX = S.JX, JY = S.JY;
   const int bx = x * JX, by = y * JY;
@@ -189,14 +219,33 @@ __device__ __forceinline__ double stencil3(CSys &S, const gdbl *__restrict__ g, 
   return v * (1.0 / 6.0);
 }
 
+// Workgroups are handed to the 8 XCDs round-robin by dispatch order.  The
+// 3-gradient steps renumber them so that each XCD takes a contiguous run of
+// x-planes: the x-1 / x+1 planes a workgroup reads are then mostly read by
+// workgroups of the same XCD and hit its L2 instead of the MALL.
+__device__ __forceinline__ int xcd_block_id() {
+  const int nb = gridDim.x * gridDim.y;
+  const int hw = blockIdx.y * gridDim.x + blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, c = hw & 7, k = hw >> 3;
+  return (c < r ? c * (q + 1) : r * (q + 1) + (c - r) * q) + k;
+}
+
+__device__ __forceinline__ void xcd_plane_block(int &xplane, int &bx) {
+  const int L = xcd_block_id();
+  xplane = L / gridDim.x;
+  bx = L - xplane * gridDim.x;
+}
+
 __global__ void __launch_bounds__(kGridThreads) k_step3_fwd(const DevSys *__restrict__ d, int src, int dst,
                                                             int seg) {
   CSys &S = uniform_sys(*d);
   const int MZ = S.MZ, M = S.M;
-  const int p = blockIdx.x * kGridThreads + threadIdx.x;
+  int xp, bx;
+  xcd_plane_block(xp, bx);
+  const int p = bx * kGridThreads + threadIdx.x;
   if (p >= S.MY * MZ) return;
   const int yy = p / MZ;
-  const int x = blockIdx.y + 1, y = yy + 1, z = p - yy * MZ + 1;
+  const int x = xp + 1, y = yy + 1, z = p - yy * MZ + 1;
   const int i = x * S.JX + y * S.JY + z;
   const gdbl *prev = hbm(S.Gf) + (size_t)src * M;
   hbm(S.Gf)[(size_t)dst * M + i] = stencil3(S, prev, x, y, z) * hbm(S.G1)[(size_t)seg * M + i];
@@ -206,10 +255,12 @@ __global__ void __launch_bounds__(kGridThreads) k_step3_bwd(const DevSys *__rest
                                                             int src) {
   CSys &S = uniform_sys(*d);
   const int MZ = S.MZ, M = S.M;
-  const int p = blockIdx.x * kGridThreads + threadIdx.x;
+  int xp, bx;
+  xcd_plane_block(xp, bx);
+  const int p = bx * kGridThreads + threadIdx.x;
   if (p >= S.MY * MZ) return;
   const int yy = p / MZ;
-  const int x = blockIdx.y + 1, y = yy + 1, z = p - yy * MZ + 1;
+  const int x = xp + 1, y = yy + 1, z = p - yy * MZ + 1;
   const int i = x * S.JX + y * S.JY + z;
   const gdbl *gb = hbm(S.Gb) + (size_t)src * M;
   const double v = stencil3(S, gb, x, y, z) * hbm(S.G1)[(size_t)seg * M + i];
@@ -380,7 +431,7 @@ __global__ void __launch_bounds__(kGridThreads) k_axpy_first(const DevSys *__res
 __global__ void __launch_bounds__(kGridThreads) k_diis_step(const DevSys *__restrict__ d, int k, int kd,
                                                              double *partial) {
   CSys &S = uniform_sys(*d);
-  __shared__ double red[kGridThreads / 64];
+  __shared__ double red[kMaxDIIS * (kGridThreads / 64)];
   if (S.sc->status != DS_OK) return;
   const int nvar = S.nit * S.M, m = S.m;
   const double dmax = S.deltamax;
@@ -405,20 +456,14 @@ __global__ void __launch_bounds__(kGridThreads) k_diis_step(const DevSys *__rest
       }
     }
   }
-#pragma unroll
-  for (int i = 0; i < kMaxDIIS; i++) {
-    if (i < kd) {
-      const double t = block_sum(acc[i], red);
-      if (threadIdx.x == 0) partial[blockIdx.x * kMaxDIIS + i] = t;
-    }
-  }
+  block_sums_store(acc, kd, red, partial);
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_diis_solve(const DevSys *__restrict__ d, const double *partial,
                                                               int nparts, int k_diis, int kd) {
   __shared__ double dots[kMaxDIIS];
   __shared__ double Ap[kMaxDIIS * kMaxDIIS];
-  __shared__ double red[kMaxDIIS * (kGridThreads / 64)];
+  __shared__ double red[kMaxDIIS * kGridThreads];
   const DevSys &S = *d;
   DevScalars &sc = *S.sc;
   const int st = sc.status;
@@ -477,7 +522,7 @@ __device__ __forceinline__ void diis_indices(const Sys &S, int &k, int &kd, int 
 
 __global__ void __launch_bounds__(kGridThreads) k_diis_step_dev(const DevSys *__restrict__ d, double *partial) {
   CSys &S = uniform_sys(*d);
-  __shared__ double red[kGridThreads / 64];
+  __shared__ double red[kMaxDIIS * (kGridThreads / 64)];
   if (S.sc->status != DS_OK) return;
   int k, kd, k_diis;
   diis_indices(S, k, kd, k_diis);
@@ -504,20 +549,14 @@ __global__ void __launch_bounds__(kGridThreads) k_diis_step_dev(const DevSys *__
       }
     }
   }
-#pragma unroll
-  for (int i = 0; i < kMaxDIIS; i++) {
-    if (i < kd) {
-      const double t = block_sum(acc[i], red);
-      if (threadIdx.x == 0) partial[blockIdx.x * kMaxDIIS + i] = t;
-    }
-  }
+  block_sums_store(acc, kd, red, partial);
 }
 
 __global__ void __launch_bounds__(kGridThreads) k_diis_solve_dev(const DevSys *__restrict__ d,
                                                                   const double *partial, int nparts) {
   __shared__ double dots[kMaxDIIS];
   __shared__ double Ap[kMaxDIIS * kMaxDIIS];
-  __shared__ double red[kMaxDIIS * (kGridThreads / 64)];
+  __shared__ double red[kMaxDIIS * kGridThreads];
   const DevSys &S = *d;
   DevScalars &sc = *S.sc;
   if (sc.status != DS_OK) return;
