@@ -77,16 +77,24 @@ typedef struct {
 
 /* Parse `.in` text (the reference's input format) at start block `start`
  * (1-based) and allocate the system on the current HIP device.
+ * Compositions may be linear, `(A)10(B)10`, or branched with square
+ * brackets, `(C)16(N)1[(N)1][(N)1](N)1` (first-order chains without
+ * save_memory; such a system runs on the grid path); aliases, repeat groups
+ * and the @comb / @dend / @water keywords are refused with SCF_E_PARSE and a
+ * message naming them.
  * Replaces: Input + Lattice/Segment/Molecule/System/Solve_scf::CheckInput and
- * AllocateMemory (src/namics.cpp:153-417, src/lattice.cpp:267-601). */
+ * AllocateMemory (src/namics.cpp:153-417, src/lattice.cpp:267-601), the
+ * composition parser Molecule::Decomposition / GenerateTree / Interpret
+ * (src/molecule.cpp:1155-1280,1383-1767). */
 int scf_system_create(const char *input_text, int start, scf_system **out, char *err, size_t errlen);
 
 void scf_system_destroy(scf_system *sys);
 
 /* JSON description: names of segments, molecules and field blocks, lattice
  * sizes (M, MX, MY, MZ), nvar, solver settings, and the current end-point
- * checkpoint interval ("checkpoint", 0 = every row kept).  Returns bytes
- * needed. */
+ * checkpoint interval ("checkpoint", 0 = every row kept); a branched
+ * molecule also lists its blocks, generations and row schedule ("branch").
+ * Returns bytes needed. */
 int scf_system_describe(const scf_system *sys, char *buf, size_t len);
 
 /* The same description for input text without creating a system (no

@@ -84,8 +84,16 @@ int validate(const SystemSpec &s, std::string &why) {
   if ((int)s.mol.size() > kMaxMol) return why = "more than 8 molecule types", SCF_E_UNSUPPORTED;
   int nb = 0, nmm = 0;
   for (auto &m : s.mol) {
-    nb += (int)m.blocks.size();
+    if (!m.branched) nb += (int)m.blocks.size();  // a branched molecule runs from its schedule
     nmm += (int)m.molmon.size();
+  }
+  for (auto &m : s.mol) {
+    if (!m.branched) continue;
+    if (m.markov != 1) return why = "branched molecules need first-order (Markov 1) chains", SCF_E_UNSUPPORTED;
+    if (m.save_memory) return why = "save_memory with branched molecules is not supported", SCF_E_UNSUPPORTED;
+    BranchPlan plan;
+    std::string e;
+    if (!build_branch_plan(m, plan, e)) return why = "mol '" + m.name + "': " + e, SCF_E_UNSUPPORTED;
   }
   if (nb > kMaxBlocks) return why = "more than 16 blocks over all molecules", SCF_E_UNSUPPORTED;
   if (nmm > kMaxMM) return why = "more than 16 molecule-segment pairs", SCF_E_UNSUPPORTED;
@@ -119,6 +127,8 @@ int save_memory_interval(const SystemSpec &s) {
 }
 
 int check_layout(const scf_system *S, int mode, int strip_sx) {
+  if (S->branched() && mode == SCF_MODE_FUSED)
+    return fail(SCF_E_UNSUPPORTED, "branched molecules run on the grid path");
   bool markov1 = true;
   for (auto &m : S->spec.mol) markov1 = markov1 && m.markov == 1;
   const bool strip = mode == SCF_MODE_FUSED && strip_sx > 0;
@@ -257,6 +267,7 @@ int build(scf_system *S, const Placement *pl) {
   }
   for (int i = 0; i < nit; i++) h.itmon[i] = s.itmon[i];
   h.nmol = nmol;
+  S->plans.assign(nmol, BranchPlan());
   h.solvent = s.solvent;
   int nb = 0, nmm = 0, maxNR = 0, maxR = 0;
   for (int i = 0; i < nmol; i++) {
@@ -268,7 +279,13 @@ int build(scf_system *S, const Placement *pl) {
     h.mol_size[i] = m.size;
     h.mol_block_begin[i] = nb;
     h.mol_mm_begin[i] = nmm;
-    for (size_t b = 0; b < m.blocks.size(); b++) {
+    h.branch[i] = nullptr;
+    if (m.branched) {
+      std::string e;
+      if (!build_branch_plan(m, S->plans[i], e)) return fail(SCF_E_UNSUPPORTED, e);
+      h.branch[i] = &S->plans[i];
+    }
+    for (size_t b = 0; b < m.blocks.size() && !m.branched; b++) {
       h.block_seg[nb] = m.blocks[b].seg;
       h.block_n[nb] = m.blocks[b].n;
       h.block_mm[nb] = nmm + m.block_molmon[b];
@@ -290,7 +307,10 @@ int build(scf_system *S, const Placement *pl) {
   }
   // end-point storage: every row, or checkpoints with the input's save_memory
   h.ckpt = save_memory_interval(s);
-  for (auto &m : s.mol) maxNR = std::max(maxNR, ckpt_rows(m.chainlength, h.ckpt) * m.size);
+  for (int i = 0; i < nmol; i++) {
+    const MoleculeSpec &m = s.mol[i];
+    maxNR = std::max(maxNR, m.branched ? S->plans[i].rows : ckpt_rows(m.chainlength, h.ckpt) * m.size);
+  }
   S->gf_rows = maxNR;
   h.mol_block_begin[nmol] = nb;
   h.mol_mm_begin[nmol] = nmm;

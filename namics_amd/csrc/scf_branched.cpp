@@ -1,0 +1,206 @@
+// Propagator schedule of a branched molecule.
+//
+// The reference computes a branched chain (square brackets in the
+// composition) by a recursion over its generations:
+// mol_branched::ForwardBra / BackwardBra (src/mol_branched.cpp:166-300) on
+// top of Molecule::propagate_forward / propagate_backward
+// (src/molecule.cpp:2300-2446), driven by mol_branched::ComputePhi
+// (:303-420).  The recursion depends only on the composition, so it is
+// unrolled here, once, on the host, into a flat list of row operations that
+// the device executes in order (scf_grid.hip).  The molecule is the one the
+// parser leaves (turned around as the reference turns it: blocks from the
+// main chain's far end, first_s / last_s the first and last segment of each
+// generation).  The schedule follows the reference step by step, including
+// one corner of it: a generation whose last block is a branch point returns
+// the end row of the segment in front of its side groups, so a composition
+// that starts with a one-segment branch point sums GN over that row.  The
+// Markov-1 Initiate and Terminate are plain copies (LGrad1::Initiate /
+// Terminate src/LGrad1.cpp:943-969), and every scratch array the reference
+// allocates per call (GS, GX) is a row of its own per recursion depth, so
+// nested calls keep their caller's rows.
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "scf_spec.h"
+
+namespace nscf {
+
+namespace {
+
+struct PlanBuilder {
+  const MoleculeSpec &m;
+  BranchPlan &plan;
+  std::string &err;
+  int cl = 0;                     // chain length: rows 0 .. cl-1 are the forward rows
+  int gb0 = 0;                    // backward rows gb0, gb0 + 1 (Gg_b, slots s % 2)
+  std::vector<std::vector<int>> gs;  // scratch rows per recursion depth
+  int next_row = 0;
+  bool ok = true;
+
+  int scratch(int depth, int k) {
+    while ((int)gs.size() <= depth) gs.emplace_back();
+    while ((int)gs[depth].size() <= k) gs[depth].push_back(next_row++);
+    return gs[depth][k];
+  }
+  void emit(std::vector<BranchOp> &ops, int kind, int a, int b, int c, int seg) { ops.push_back({kind, a, b, c, seg}); }
+  int seg_of(int block) const { return m.blocks[block].seg; }
+  int mm_of(int block) const { return m.block_molmon[block]; }
+
+  // Molecule::propagate_forward without save_memory: rows s, s+1, ... of
+  // one block; the first row of a generation is its segment weight.
+  int forward_block(int block, int gen, int &s) {
+    for (int k = 0; k < m.blocks[block].n; k++) {
+      if (s > m.first_s[gen])
+        emit(plan.forward, BR_PROP, s, s - 1, 0, seg_of(block));
+      else
+        emit(plan.forward, BR_INIT, m.first_s[gen], 0, 0, seg_of(block));
+      s++;
+    }
+    return s - 1;
+  }
+
+  // mol_branched::ForwardBra: returns the row the reference returns (-1 for
+  // its null pointer).
+  int forward(int gen, int &s, int depth) {
+    const int b0 = m.first_b[gen], bN = m.last_b[gen];
+    int glast = -1;
+    for (int k = b0; k <= bN && ok; ++k) {
+      if (b0 < k && k < bN && m.gnr[k] != gen) {
+        // a junction: side groups start at block k
+        const int g0 = scratch(depth, 0), g1 = scratch(depth, 1), g2 = scratch(depth, 2);
+        if (glast < 0) {
+          err = "branched composition: a side group with no segment in front of it";
+          ok = false;
+          return -1;
+        }
+        emit(plan.forward, BR_COPY, g0, glast, 0, 0);
+        std::vector<int> ends;
+        while (k <= bN && m.gnr[k] != gen && ok) {
+          const int g = m.gnr[k];
+          ends.push_back(forward(g, s, depth + 1));
+          k += m.last_b[g] - m.first_b[g] + 1;
+        }
+        if (!ok) return -1;
+        if (k > bN) {
+          err = "branched composition: a side group at the end of a generation";
+          ok = false;
+          return -1;
+        }
+        // the segment after the group: backbone step times every side chain
+        // end stepped once without a segment weight
+        emit(plan.forward, BR_PROP, g2, g0, 0, seg_of(k));
+        for (int e : ends) {
+          if (e < 0) {
+            err = "branched composition: a side group without an end row";
+            ok = false;
+            return -1;
+          }
+          emit(plan.forward, BR_COPY, g0, e, 0, 0);
+          emit(plan.forward, BR_PROP, g1, g0, 0, -1);
+          emit(plan.forward, BR_TIMES, g2, g2, g1, 0);
+        }
+        emit(plan.forward, BR_COPY, s, g2, 0, 0);
+        s++;
+      } else {
+        glast = forward_block(k, gen, s);
+      }
+    }
+    return glast;
+  }
+
+  // Molecule::propagate_backward without save_memory.
+  void backward_block(int block, int &s) {
+    for (int k = 0; k < m.blocks[block].n; k++) {
+      const int cur = gb0 + ((s % 2) + 2) % 2, prev = gb0 + (((s + 1) % 2) + 2) % 2;
+      if (s < cl - 1)
+        emit(plan.backward, BR_PROP, cur, prev, 0, seg_of(block));
+      else
+        emit(plan.backward, BR_INIT, cur, 0, 0, seg_of(block));
+      if (s < 0 || s >= cl) {
+        err = "branched composition: the backward pass leaves the chain";
+        ok = false;
+        return;
+      }
+      emit(plan.backward, BR_ADDPHI, s, cur, mm_of(block), 0);
+      s--;
+    }
+  }
+
+  // mol_branched::BackwardBra
+  void backward(int gen, int &s, int depth) {
+    const int b0 = m.first_b[gen], bN = m.last_b[gen];
+    for (int k = bN; k >= b0 && ok; k--) {
+      if (m.gnr[k] != gen) {
+        std::vector<int> br, rows;
+        int ss = 0;
+        while (k >= 0 && m.gnr[k] != gen) {
+          const int g = m.gnr[k];
+          br.push_back(g);
+          rows.push_back(m.last_s[g]);
+          ss = m.first_s[g];
+          k -= m.last_b[g] - m.first_b[g] + 1;
+        }
+        br.push_back(gen);
+        ss--;
+        rows.push_back(ss);
+        const int L = (int)br.size();
+        for (int r : rows)
+          if (r < 0 || r >= cl) {
+            err = "branched composition: a junction reads a row outside the chain";
+            ok = false;
+            return;
+          }
+        const int g0 = scratch(depth, 0), g1 = scratch(depth, 1), g2 = scratch(depth, 2), g3 = scratch(depth, 3);
+        emit(plan.backward, BR_COPY, g3, gb0 + (((s + 1) % 2) + 2) % 2, 0, 0);
+        for (int i = 0; i < L && ok; i++) {
+          emit(plan.backward, BR_COPY, g2, g3, 0, 0);
+          for (int j = 0; j < L; j++) {
+            if (j == i) continue;
+            emit(plan.backward, BR_COPY, g0, rows[j], 0, 0);
+            emit(plan.backward, BR_PROP, g1, g0, 0, -1);
+            emit(plan.backward, BR_TIMES, g2, g2, g1, 0);
+          }
+          emit(plan.backward, BR_COPY, gb0, g2, 0, 0);
+          emit(plan.backward, BR_COPY, gb0 + 1, g2, 0, 0);
+          if (i < L - 1) backward(br[i], s, depth + 1);
+        }
+        k++;
+      } else {
+        backward_block(k, s);
+      }
+    }
+  }
+};
+
+}  // namespace
+
+bool build_branch_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &err) {
+  plan = BranchPlan();
+  if (!m.branched) {
+    err = "not a branched molecule";
+    return false;
+  }
+  PlanBuilder b{m, plan, err};
+  b.cl = m.chainlength;
+  b.gb0 = m.chainlength;
+  b.next_row = m.chainlength + 2;
+  int s = 0;
+  const int g = b.forward(0, s, 0);
+  if (!b.ok) return false;
+  // every contour row must have been written before the backward pass reads
+  // it: the block after each side group is one segment long
+  if (s != m.chainlength || g < 0) {
+    err = "branched composition: the segment after each side group must be a block of its own, (X)1 "
+          "(the reference's propagation writes one row there)";
+    return false;
+  }
+  plan.gn_row = g;
+  s--;
+  b.backward(0, s, 0);
+  if (!b.ok) return false;
+  plan.rows = b.next_row;
+  return true;
+}
+
+}  // namespace nscf

@@ -99,7 +99,10 @@ struct DevSys {
   // ---- work arrays (device)
   double *x, *g;
   double *G1;         // nseg * M
-  double *Gf;         // max over molecules of N * size * M
+  // host only: the row schedule of a branched molecule (a BranchPlan, see
+  // scf_spec.h), null for a linear chain; never read on the device
+  const void *branch[kMaxMol];
+  double *Gf;         // max over molecules of N * size * M (a branched molecule: its schedule's rows)
   double *Gb;         // 2 * size * M
   double *rho;        // nmm * M (molecule-segment densities)
   double *phi;        // nseg * M

@@ -11,6 +11,7 @@
 
 #include "scf_device.h"
 #include "scf_kernels.h"
+#include "scf_spec.h"
 
 namespace nscf {
 namespace {
@@ -164,6 +165,34 @@ __global__ void __launch_bounds__(kGridThreads) k_step_fwd(const DevSys *__restr
     int x, y, z;
     const int i = work_site(S, w, x, y, z);
     gs[i] = stencil_prop<double>(S, prev, i, x, y, z) * g1[i];
+  }
+}
+
+// One operation of a branched molecule's row schedule (BranchOpKind in
+// scf_spec.h) over the sites a phase visits: rows are Gf rows of M sites.
+__global__ void __launch_bounds__(kGridThreads) k_branch_op(const DevSys *__restrict__ d, int kind, int a, int b,
+                                                            int c, int seg) {
+  CSys &S = uniform_sys(*d);
+  const int M = S.M;
+  gdbl *ra = hbm(S.Gf) + (size_t)a * M;
+  const gdbl *rb = hbm(S.Gf) + (size_t)b * M;
+  const gdbl *rc = hbm(S.Gf) + (size_t)c * M;
+  const gdbl *g1 = hbm(S.G1) + (size_t)(seg < 0 ? 0 : seg) * M;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    if (kind == BR_INIT) {
+      ra[i] = g1[i];
+    } else if (kind == BR_PROP) {
+      const double v = stencil_prop<double>(S, S.Gf + (size_t)b * M, i, x, y, z);
+      ra[i] = seg < 0 ? v : v * g1[i];
+    } else if (kind == BR_COPY) {
+      ra[i] = rb[i];
+    } else if (kind == BR_TIMES) {
+      ra[i] = rb[i] * rc[i];
+    } else {  // BR_ADDPHI
+      hbm(S.rho)[(size_t)c * M + i] += ra[i] * rb[i];
+    }
   }
 }
 
@@ -619,6 +648,17 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
   const bool six = h.gradients == 3 && h.stencil_full && h.lattice_type == 0;
   hipLaunchKernelGGL(k_prepare, G, B, 0, st, d);
   for (int mo = 0; mo < h.nmol; mo++) {
+    if (h.branch[mo]) {
+      // a branched molecule: its schedule, GN over the row it names
+      const BranchPlan &plan = *(const BranchPlan *)h.branch[mo];
+      for (const BranchOp &o : plan.forward)
+        hipLaunchKernelGGL(k_branch_op, G, B, 0, st, d, o.kind, o.a, o.b, o.c, o.seg);
+      hipLaunchKernelGGL(k_gn_partial, G, B, 0, st, d, plan.gn_row, partial);
+      hipLaunchKernelGGL(k_finalize_gn, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb, mo);
+      for (const BranchOp &o : plan.backward)
+        hipLaunchKernelGGL(k_branch_op, G, B, 0, st, d, o.kind, o.a, o.b, o.c, o.seg);
+      continue;
+    }
     if (h.mol_markov[mo] != 1) return hipErrorInvalidValue;
     const int b0 = h.mol_block_begin[mo], b1 = h.mol_block_begin[mo + 1], N = h.mol_N[mo];
     // block (segment, molecule-segment pair) of every row of the chain

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/namics_scf.h"
 #include "scf_device.h"
@@ -32,11 +33,19 @@ struct scf_system {
   bool have_results = false;
   bool owns_stream = true;  // false when a server lends its stream
   char *region = nullptr;   // arena placed in a caller's device region (not owned)
+  std::vector<nscf::BranchPlan> plans;  // row schedules of the branched molecules (h.branch points here)
 
   bool newton() const { return spec.solver.method != nscf::SOLVER_DIIS; }
+  bool branched() const {
+    for (auto &m : spec.mol)
+      if (m.branched) return true;
+    return false;
+  }
 
   int effective_mode() const {
     if (mode != SCF_MODE_AUTO) return mode;
+    // branched molecules run their row schedules on the grid path
+    if (branched()) return SCF_MODE_GRID;
     // checkpointed end-point storage runs on the strip chains or the grid path
     if (h.ckpt > 1 && h.strip_sx == 0) return SCF_MODE_GRID;
     if (tuned_mode != SCF_MODE_AUTO) return tuned_mode;

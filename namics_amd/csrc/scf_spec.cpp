@@ -474,7 +474,8 @@ bool parse_composition(const std::string &c, const std::vector<std::string> &mon
   size_t i = 0;
   while (i < c.size()) {
     if (c[i] != '(') {
-      err = "composition '" + c + "': only linear block sequences like (A)10(B)10 are supported";
+      err = "composition '" + c + "': only block sequences like (A)10(B)10, with side groups in square brackets, "
+            "are supported (no aliases, repeat groups or @ keywords)";
       return false;
     }
     size_t close = c.find(')', i);
@@ -506,6 +507,207 @@ bool parse_composition(const std::string &c, const std::vector<std::string> &mon
     err = "empty composition";
     return false;
   }
+  return true;
+}
+
+// ---- branched compositions
+// Positions of '[' and of ']' in order of appearance (two separate lists,
+// not matched pairs), as Input::EvenSquareBrackets (src/input.cpp:89-100)
+// returns them; false when the brackets do not balance.
+bool square_positions(const std::string &c, std::vector<int> &open, std::vector<int> &close) {
+  int depth = 0;
+  for (int i = 0; i < (int)c.size(); i++) {
+    if (c[i] == '[') {
+      open.push_back(i);
+      depth++;
+    } else if (c[i] == ']') {
+      close.push_back(i);
+      if (depth-- == 0) return false;
+    }
+  }
+  return depth == 0;
+}
+
+// The integer a repeat count starts with (Input::Get_int src/input.cpp:162-174:
+// a leading digit, or '-' and a digit; anything else gives `dflt`).
+int leading_int(const std::string &t, int dflt) {
+  size_t i = 0;
+  bool neg = false;
+  if (t.size() > 1 && t[0] == '-' && isdigit((unsigned char)t[1])) {
+    neg = true;
+    i = 1;
+  } else if (t.empty() || !isdigit((unsigned char)t[0])) {
+    return dflt;
+  }
+  long v = 0;
+  while (i < t.size() && isdigit((unsigned char)t[i]) && v < 100000000) v = v * 10 + (t[i++] - '0');
+  return (int)(neg ? -v : v);
+}
+
+// Splits a branched composition into blocks with their generations, as the
+// reference's Molecule::GenerateTree / Interpret (src/molecule.cpp:1155-1280)
+// do: generation numbers come from the order of the brackets, and the block
+// in front of a deeper generation gives its last segment to a block of its
+// own (the branch point is one segment long).
+struct BranchParser {
+  const std::string &c;
+  const std::vector<std::string> &mons;
+  MoleculeSpec &m;
+  std::string &err;
+  std::vector<int> open, close;
+  int chainlength = 0;
+
+  void add_generation() {
+    m.first_s.push_back(-1);
+    m.last_s.push_back(-1);
+    m.first_b.push_back(-1);
+    m.last_b.push_back(-1);
+  }
+
+  bool piece(const std::string &t, int gen) {
+    if (t == "[") return true;
+    if (t.find(':') != std::string::npos) {
+      err = "composition '" + c + "': aliases are not supported";
+      return false;
+    }
+    std::vector<int> po, pc;
+    int depth = 0;
+    for (int i = 0; i < (int)t.size(); i++) {
+      if (t[i] == '(') {
+        po.push_back(i);
+        depth++;
+      } else if (t[i] == ')') {
+        pc.push_back(i);
+        if (depth-- == 0) break;
+      }
+    }
+    if (depth != 0 || po.empty() || po.size() != pc.size()) {
+      err = "composition '" + c + "': piece '" + t + "' is not a sequence of (segment)n";
+      return false;
+    }
+    for (size_t k = 0; k < po.size(); k++) {
+      const std::string name = t.substr(po[k] + 1, pc[k] - po[k] - 1);
+      auto it = std::find(mons.begin(), mons.end(), name);
+      if (it == mons.end()) {
+        err = "composition '" + c + "': segment '" + name + "' is not defined";
+        return false;
+      }
+      const size_t L = m.blocks.size();
+      if (L > 0 && m.gnr[L - 1] < gen && m.blocks[L - 1].n > 1) {
+        m.blocks[L - 1].n--;
+        m.blocks.push_back({m.blocks[L - 1].seg, 1});
+        m.gnr.push_back(m.gnr[L - 1]);
+        m.last_b[m.gnr[L - 1]]++;
+      }
+      m.blocks.push_back({(int)(it - mons.begin()), 0});
+      m.gnr.push_back(gen);
+      const int nb = (int)m.blocks.size() - 1;
+      if (m.first_s[gen] < 0) m.first_s[gen] = chainlength;
+      if (m.first_b[gen] < 0) m.first_b[gen] = nb;
+      m.last_b[gen] = nb;
+      const int n = leading_int(t.substr(pc[k] + 1), 0);
+      if (n < 1) {
+        err = "composition '" + c + "': the number of repeats must be a positive integer";
+        return false;
+      }
+      m.blocks.back().n = n;
+      chainlength += n;
+      m.last_s[gen] = chainlength;
+    }
+    return true;
+  }
+
+  bool tree(int gen, int &pos) {
+    const int len = (int)c.size(), L = (int)open.size();
+    int pos_open = 0, pos_close = len;
+    while (pos_open < pos_close) {
+      pos_open = len + 1;
+      pos_close = len;
+      bool of = false, cf = false;
+      int gen_open = 0, gen_close = 0;
+      for (int i = 0; i < L && !(of && cf); i++) {
+        if (close[i] > pos && !cf) {
+          cf = true;
+          pos_close = close[i] + 1;
+          gen_close = i + 1;
+        }
+        if (open[i] >= pos && !of) {
+          of = true;
+          pos_open = open[i] + 1;
+          gen_open = i + 1;
+        }
+      }
+      if (pos_close < pos_open) {
+        const std::string t = c.substr(pos, pos_close - pos);
+        if (!t.empty() && t[0] == '[') {
+          pos = pos + 1;
+          add_generation();
+          if (!tree(gen_close, pos)) return false;
+          pos_close = pos_open + 1;
+        } else {
+          pos = pos_close;
+          if (!piece(t, gen)) return false;
+        }
+      } else {
+        const std::string t = c.substr(pos, pos_open - pos);
+        pos = pos_open;
+        if (!piece(t, gen)) return false;
+        add_generation();
+        if (!tree(gen_open, pos)) return false;
+      }
+    }
+    return true;
+  }
+};
+
+bool parse_branched(const std::string &c, const std::vector<std::string> &mons, MoleculeSpec &m,
+                    std::string &err) {
+  if (c.empty() || c[0] != '(') {
+    err = "composition '" + c + "' must start with '('";
+    return false;
+  }
+  if (c.back() == ']') {
+    err = "composition '" + c + "' can not end with ']'";
+    return false;
+  }
+  BranchParser p{c, mons, m, err};
+  if (!square_positions(c, p.open, p.close)) {
+    err = "composition '" + c + "': the square brackets are not balanced";
+    return false;
+  }
+  m.blocks.clear();
+  m.gnr.clear();
+  m.first_s.clear();
+  m.last_s.clear();
+  m.first_b.clear();
+  m.last_b.clear();
+  p.add_generation();
+  int pos = 0;
+  if (!p.tree(0, pos)) return false;
+  for (size_t g = 0; g < m.first_b.size(); g++)
+    if (m.first_b[g] < 0) {
+      err = "composition '" + c + "': an empty side group";
+      return false;
+    }
+  // The reference then turns the molecule around (Molecule::Decomposition
+  // src/molecule.cpp:1742-1767): blocks in reverse order, contour indices
+  // mirrored, so first_s / last_s become the first and the last segment
+  // (inclusive) of each generation, counted from the far end of the main
+  // chain, and first_b / last_b its first and last block.  The propagation
+  // then runs from the main chain's last segment, and the block after each
+  // side group is the branch point (one segment long).
+  const int CL = m.last_s[0], nb = (int)m.blocks.size();
+  for (size_t g = 0; g < m.first_s.size(); g++) {
+    const int fs = CL - m.first_s[g] - 1, ls = CL - m.last_s[g] - 1;
+    m.first_s[g] = ls + 1;
+    m.last_s[g] = fs;
+    const int fb = nb - m.first_b[g] - 1, lb = nb - m.last_b[g] - 1;
+    m.first_b[g] = lb;
+    m.last_b[g] = fb;
+  }
+  std::reverse(m.blocks.begin(), m.blocks.end());
+  std::reverse(m.gnr.begin(), m.gnr.end());
+  m.branched = true;
   return true;
 }
 
@@ -862,7 +1064,11 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       return false;
     }
     s.mol_composition.push_back(v);
-    if (!parse_composition(v, p.mons, m.blocks, err)) return false;
+    if (v.find('[') != std::string::npos) {
+      if (!parse_branched(v, p.mons, m, err)) return false;
+    } else if (!parse_composition(v, p.mons, m.blocks, err)) {
+      return false;
+    }
     m.chainlength = 0;
     for (auto &b : m.blocks) {
       m.chainlength += b.n;
@@ -1282,7 +1488,51 @@ std::string describe_json(const SystemSpec &s) {
     str(m.name);
     o << ",\"chainlength\":" << m.chainlength << ",\"freedom\":" << m.freedom << ",\"markov\":" << m.markov
       << ",\"pinned\":" << (m.pinned ? "true" : "false") << ",\"save_memory\":" << (m.save_memory ? "true" : "false")
-      << ",\"size\":" << m.size << ",\"blocks\":" << m.blocks.size() << ",\"molmon\":" << m.molmon.size() << "}";
+      << ",\"size\":" << m.size << ",\"blocks\":" << m.blocks.size() << ",\"molmon\":" << m.molmon.size();
+    if (m.branched) {
+      // the branched molecule's blocks, generations and row schedule
+      auto ints = [&](const std::vector<int> &v) {
+        o << "[";
+        for (size_t k = 0; k < v.size(); k++) o << (k ? "," : "") << v[k];
+        o << "]";
+      };
+      o << ",\"branch\":{\"block_seg\":[";
+      for (size_t k = 0; k < m.blocks.size(); k++) o << (k ? "," : "") << m.blocks[k].seg;
+      o << "],\"block_n\":[";
+      for (size_t k = 0; k < m.blocks.size(); k++) o << (k ? "," : "") << m.blocks[k].n;
+      o << "],\"block_molmon\":";
+      ints(m.block_molmon);
+      o << ",\"gnr\":";
+      ints(m.gnr);
+      o << ",\"first_b\":";
+      ints(m.first_b);
+      o << ",\"last_b\":";
+      ints(m.last_b);
+      o << ",\"first_s\":";
+      ints(m.first_s);
+      o << ",\"last_s\":";
+      ints(m.last_s);
+      BranchPlan plan;
+      std::string e;
+      if (build_branch_plan(m, plan, e)) {
+        auto ops = [&](const std::vector<BranchOp> &v) {
+          o << "[";
+          for (size_t k = 0; k < v.size(); k++)
+            o << (k ? "," : "") << "[" << v[k].kind << "," << v[k].a << "," << v[k].b << "," << v[k].c << ","
+              << v[k].seg << "]";
+          o << "]";
+        };
+        o << ",\"gn_row\":" << plan.gn_row << ",\"rows\":" << plan.rows << ",\"forward\":";
+        ops(plan.forward);
+        o << ",\"backward\":";
+        ops(plan.backward);
+      } else {
+        o << ",\"error\":";
+        str(e);
+      }
+      o << "}";
+    }
+    o << "}";
   }
   o << "],\"fields\":[";
   for (size_t i = 0; i < s.itmon.size(); i++) {

@@ -97,7 +97,42 @@ struct MoleculeSpec {
   std::vector<double> P;       // Markov-2 bending weights, FJC entries
   // index of the molmon entry for every block
   std::vector<int> block_molmon;
+  // Branched architecture (square brackets in the composition,
+  // Molecule::GenerateTree src/molecule.cpp:1222-1280): the generation of
+  // every block, and per generation its first / last block and the contour
+  // index of its first segment / one past its last (first_s, last_s).
+  bool branched = false;
+  std::vector<int> gnr, first_b, last_b, first_s, last_s;
 };
+
+// The propagator schedule of one branched molecule: a list of row
+// operations over M-site rows.  Rows 0 .. chainlength-1 are the end-point
+// distributions of the forward pass (one per contour index), rows
+// chainlength and chainlength + 1 the two backward rows, the rest scratch
+// rows of the junctions.  `seg` is a segment type; -1 in BR_PROP means no
+// segment weight (the reference's UNITY).
+enum BranchOpKind : int {
+  BR_INIT = 0,    // R[a] = G1[seg]
+  BR_PROP = 1,    // R[a] = W(R[b]) * G1[seg]
+  BR_COPY = 2,    // R[a] = R[b]
+  BR_TIMES = 3,   // R[a] = R[b] * R[c]
+  BR_ADDPHI = 4,  // rho[c] += R[a] * R[b]   (c: molecule-segment index)
+};
+struct BranchOp {
+  int kind, a, b, c, seg;
+};
+struct BranchPlan {
+  std::vector<BranchOp> forward, backward;
+  int gn_row = 0;  // the row whose weighted sum is GN
+  int rows = 0;    // rows the schedule uses
+};
+// Builds the schedule of molecule m (branched, Markov 1, no save_memory) by
+// the reference's recursion (mol_branched::ForwardBra / BackwardBra
+// src/mol_branched.cpp:166-300, Molecule::propagate_forward / _backward
+// src/molecule.cpp:2300-2446, mol_branched::ComputePhi :303-420).  Returns
+// false with a message when the composition leaves rows the reference would
+// read before writing them.
+bool build_branch_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &err);
 
 struct SolverSpec {
   int method = SOLVER_PSEUDOHESSIAN;

@@ -1298,9 +1298,46 @@ struct Oracle {
     return 1.0 * Nseg / s.mol[mol].chainlength;
   }
 
+  // mol_branched::ComputePhi (src/mol_branched.cpp:303-420) by the row
+  // schedule of the molecule (build_branch_plan: the reference's recursion
+  // unrolled into row operations); Markov 1.  UNITY is a row of ones
+  // (the reference's UNITY array), Initiate / Terminate are copies.
+  void compute_phi_branched(int mi) {
+    const MoleculeSpec &m = s.mol[mi];
+    Lat lat(s.lat);
+    vec &rho = mol_phi[mi];
+    std::fill(rho.begin(), rho.end(), 0.0);
+    BranchPlan plan;
+    std::string e;
+    if (!build_branch_plan(m, plan, e)) {
+      status = 3;
+      return;
+    }
+    std::vector<vec> R(plan.rows, vec(M, 0.0));
+    const vec unity(M, 1.0);
+    auto run = [&](const std::vector<BranchOp> &ops) {
+      for (const BranchOp &o : ops) {
+        switch (o.kind) {
+          case BR_INIT: Cp(R[o.a].data(), G1[o.seg].data(), M); break;
+          case BR_PROP: lat.propagate(R[o.a].data(), R[o.b].data(), o.seg < 0 ? unity.data() : G1[o.seg].data()); break;
+          case BR_COPY: Cp(R[o.a].data(), R[o.b].data(), M); break;
+          case BR_TIMES: Times(R[o.a].data(), R[o.b].data(), R[o.c].data(), M); break;
+          default: AddTimes(rho.data() + o.c * M, R[o.a].data(), R[o.b].data(), M); break;
+        }
+      }
+    };
+    run(plan.forward);
+    GN[mi] = lat.weighted_sum(R[plan.gn_row].data());
+    run(plan.backward);
+  }
+
   // Molecule::ComputePhi for a linear chain; accumulates Gf*Gb into mol_phi
   void compute_phi_mol(int mi) {
     const MoleculeSpec &m = s.mol[mi];
+    if (m.branched) {
+      compute_phi_branched(mi);
+      return;
+    }
     Lat lat(s.lat);
     const int N = m.chainlength, size = m.size;
     const int W = size * M;

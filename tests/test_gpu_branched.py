@@ -1,0 +1,86 @@
+"""Branched molecules on the GPU (grid path) against the reference's own
+outputs and the oracle.
+
+The fixtures (tests/golden/branched/*.json) come from the reference program
+(tests/golden/make_branched_fixtures.py): at seeded fields the library's
+residual, F, Omega, per-molecule GN, n, theta, phibulk, mu and density
+profiles agree with the reference to 1e-10 and with the oracle to 1e-12;
+solved by DIIS from the fixture's starting fields the library converges in the reference's
+number of iterations (these runs are short, so rounding does not move the
+path) to the reference's converged state.  A branched molecule runs on the
+grid path only: asking for the fused path is refused.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests.golden.branched_cases import BRANCHED
+from tests.helpers import rel_err, scaled_err
+from tests.oracle_lib import Oracle
+from tests.test_gpu_parity import TOL_ORC, TOL_REF, check_results
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _fixture(name):
+    with open(os.path.join(HERE, "golden", "branched", name + ".json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", list(BRANCHED))
+def test_branched_residual_matches_reference_and_oracle(case):
+    from namics_amd import ScfSystem
+    fx = _fixture(case)
+    o = Oracle(fx["text"])
+    print("branched case", case, flush=True)
+    s = ScfSystem(fx["text"])
+    assert s.layout()["mode"] == "grid"
+    for ev in fx["evaluation"]:
+        x = np.array(ev["x"])
+        g, res = s.residual(x)
+        r = s.results()
+        check_results(s, r, ev, TOL_REF)
+        assert rel_err(res, ev["residual"]) < TOL_REF
+        for m, p in ev["mol_phi"].items():
+            assert scaled_err(s.interior(r["mol_phi"][m]), p) < TOL_REF, m
+        go, reso = o.residual(x)
+        ro = o.results()
+        assert scaled_err(g, go) < TOL_ORC
+        for m in o.mol_names:
+            assert rel_err(r["mol"][m]["GN"], ro["mol"][m]["GN"]) < TOL_ORC
+    s.close()
+
+
+@pytest.mark.parametrize("case", list(BRANCHED))
+def test_branched_diis_matches_reference(case):
+    from namics_amd import ScfSystem
+    fx = _fixture(case)
+    o0 = Oracle(fx["text"])
+    for cv in fx["converged"]:
+        text = fx["text"] + "\nnewton : %s : method : DIIS\nnewton : %s : tolerance : %r\n" \
+            "newton : %s : iterationlimit : 5000\n" % (o0.newton_name, o0.newton_name, cv["tolerance"],
+                                                      o0.newton_name)
+        print("branched case", case, "solve", flush=True)
+        s = ScfSystem(text)
+        x0 = np.array(cv["x0"]) if "x0" in cv else np.zeros(s.nvar)
+        x, it, res, st = s.solve(x0)
+        assert st == 0 and res <= cv["tolerance"]
+        assert it == int(cv["iterations"]), (it, cv["iterations"])
+        r = s.results()
+        tol = max(1e-10, 100 * cv["tolerance"])
+        assert rel_err(r["free_energy"], cv["free_energy"]) < tol
+        for sn, p in cv["seg_phi"].items():
+            assert scaled_err(s.interior(r["seg_phi"][sn]), p) < tol
+        s.close()
+
+
+def test_branched_refuses_the_fused_path():
+    from namics_amd import ScfSystem, scf
+    s = ScfSystem(BRANCHED["branch1d_comb"]["text"])
+    with pytest.raises(scf.ScfError):
+        s.set_mode(scf.MODE_FUSED)
+    s.close()

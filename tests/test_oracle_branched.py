@@ -1,0 +1,107 @@
+"""Branched molecules: the CPU oracle against the reference's own outputs.
+
+The fixtures (tests/golden/branched/*.json) were produced by the reference
+program compiled from its sources (tests/golden/make_branched_fixtures.py).
+The oracle executes the molecule's row schedule (the reference's
+ForwardBra / BackwardBra recursion unrolled by build_branch_plan, shared
+with the library) with its own restated lattice steps, so at given fields it
+must reproduce the reference to a few ulps, and its DIIS must take the
+reference's number of iterations.  The schedule itself is also checked by
+hand on one composition.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests.helpers import rel_err
+from tests.oracle_lib import Oracle
+from tests.golden.branched_cases import BRANCHED
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EVAL_TOL = 1e-12
+
+
+def _fixture(name):
+    with open(os.path.join(HERE, "golden", "branched", name + ".json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", list(BRANCHED))
+def test_branched_evaluation_matches_reference(case):
+    fx = _fixture(case)
+    o = Oracle(fx["text"])
+    for ev in fx["evaluation"]:
+        g, res = o.residual(np.array(ev["x"]))
+        r = o.results()
+        assert rel_err(res, ev["residual"]) < EVAL_TOL
+        assert rel_err(r["free_energy"], ev["free_energy"]) < EVAL_TOL
+        assert abs(r["grand_potential"] - ev["grand_potential"]) < EVAL_TOL * (abs(ev["free_energy"]) +
+                                                                               abs(ev["grand_potential"]))
+        for m, d in ev["mol"].items():
+            for k in ("GN", "n", "theta", "phibulk"):
+                assert rel_err(r["mol"][m][k], d[k]) < EVAL_TOL, (m, k, r["mol"][m][k], d[k])
+            assert abs(r["mol"][m]["Mu"] - d["Mu"]) < EVAL_TOL * max(1.0, abs(d["Mu"])), (m, "Mu")
+        for s, p in ev["seg_phi"].items():
+            assert rel_err(o.interior(r["seg_phi"][s]), p, 1e-200) < EVAL_TOL, s
+        for m, p in ev["mol_phi"].items():
+            assert rel_err(o.interior(r["mol_phi"][m]), p, 1e-200) < EVAL_TOL, m
+
+
+@pytest.mark.parametrize("case", list(BRANCHED))
+def test_branched_diis_matches_reference(case):
+    fx = _fixture(case)
+    for cv in fx["converged"]:
+        o0 = Oracle(fx["text"])
+        o = Oracle(fx["text"] + "\nnewton : %s : method : DIIS\nnewton : %s : tolerance : %r\n"
+                   "newton : %s : iterationlimit : 5000\n" % (o0.newton_name, o0.newton_name, cv["tolerance"],
+                                                              o0.newton_name))
+        x0 = np.array(cv["x0"]) if "x0" in cv else np.zeros(o.nvar)
+        x, it, res, st = o.solve(x0)
+        assert st == 0
+        assert it == int(cv["iterations"])
+        r = o.results()
+        assert rel_err(r["free_energy"], cv["free_energy"]) < 1e-10
+        for s, p in cv["seg_phi"].items():
+            assert rel_err(o.interior(r["seg_phi"][s]), p, 1e-200) < 1e-10
+
+
+def test_branched_parse_by_hand():
+    """(A)3[(B)2](A)1(A)3: the A block in front of the side group gives its
+    last segment to a one-segment branch point, and the molecule is turned
+    around (blocks from the far end of the main chain); generation 1 (the
+    B side chain) covers segments 4..5 of the turned chain.  The schedule
+    puts the branch point on row 6: one step of the junction row 3 with the
+    A weight, times one unweighted step of the side chain's end row 5."""
+    from namics_amd.scf import describe_input
+    text = BRANCHED["branch3d_box"]["text"]
+    b = [m for m in describe_input(text)["molecules"] if m["name"] == "br"][0]["branch"]
+    A, B = 0, 1
+    assert b["block_seg"] == [A, A, B, A, A]
+    assert b["block_n"] == [3, 1, 2, 1, 2]
+    assert b["gnr"] == [0, 0, 1, 0, 0]
+    assert b["first_b"] == [0, 2] and b["last_b"] == [4, 2]
+    assert b["first_s"] == [0, 4] and b["last_s"] == [8, 5]
+    assert b["gn_row"] == 8
+    fwd = b["forward"]
+    INIT, PROP, COPY, TIMES = 0, 1, 2, 3
+    # main chain rows 0..3, then the side chain rows 4, 5
+    assert fwd[:4] == [[INIT, 0, 0, 0, A], [PROP, 1, 0, 0, A], [PROP, 2, 1, 0, A], [PROP, 3, 2, 0, A]]
+    g0 = fwd[4][1]
+    assert fwd[4] == [COPY, g0, 3, 0, 0]
+    assert fwd[5:7] == [[INIT, 4, 0, 0, B], [PROP, 5, 4, 0, B]]
+    g2 = fwd[7][1]
+    assert fwd[7] == [PROP, g2, g0, 0, A]
+    g1 = fwd[9][1]
+    assert fwd[8:12] == [[COPY, g0, 5, 0, 0], [PROP, g1, g0, 0, -1], [TIMES, g2, g2, g1, 0], [COPY, 6, g2, 0, 0]]
+    assert fwd[12:] == [[PROP, 7, 6, 0, A], [PROP, 8, 7, 0, A]]
+
+
+def test_branched_refusals():
+    from namics_amd.scf import ScfError, describe_input
+    base = BRANCHED["branch1d_comb"]["text"]
+    comp = "(A)4[(B)3](A)1(A)4[(B)3](A)1(A)3"
+    for bad in ("(A)4[(B)3]", "(A)4[(B)3](A)1]", "(A)4[[(B)3](A)1", "(A)4[(Q)3](A)1", "(A)4[(B)0](A)1"):
+        with pytest.raises(ScfError):
+            describe_input(base.replace(comp, bad))
