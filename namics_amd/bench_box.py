@@ -7,26 +7,60 @@ the grid path (one launch per phase and per contour step, the iteration
 captured in a hipGraph, or launched kernel by kernel with --no-graph).  One
 step = one solve from the seeded fields, which stay resident on the device.
 
-    python bench.py --workload box [--steps K] [--warmup W] [--no-graph]
+    python bench.py --workload box [--steps K] [--warmup W] [--no-graph] [--no-mixed] [--switch S]
+
+The bench line's `mixed_precision` object times the same solves with FP32
+propagators while ||g|| is above the switch level (default 1e-4).
 """
 import time
 
 import numpy as np
 
 
+def _solve_timed(s, x0):
+    from namics_amd import solve_resident_info
+    s.upload(x0)
+    t = time.perf_counter()
+    info = solve_resident_info([s])[0]
+    return info, time.perf_counter() - t
+
+
+def run_mixed(args, x0, n, f64):
+    """The same solves with FP32 propagators while ||g|| > args.switch, then
+    FP64 to the tolerance: solves/s, ms per iteration, the FP32 iterations
+    and |dF|/F against the FP64 solve."""
+    from namics_amd import ScfSystem, synchronize
+    from namics_amd.workloads import diblock_box
+    s = ScfSystem(diblock_box(n))
+    s.set_graph(not args.no_graph)
+    s.set_precision(True, args.switch)
+    for _ in range(args.warmup):
+        _solve_timed(s, x0)
+    synchronize()
+    t0 = time.perf_counter()
+    runs = [_solve_timed(s, x0) for _ in range(args.steps)]
+    synchronize()
+    elapsed = time.perf_counter() - t0
+    info = runs[-1][0]
+    res = s.results()
+    s.close()
+    solve_s = float(np.mean([r[1] for r in runs]))
+    return {"value": sum(1 for r in runs if r[0]["status"] == 0) / elapsed, "unit": "solves/s",
+            "switch": args.switch, "iterations": info["iterations"], "fp32_iterations": info["fp32_iterations"],
+            "ms_per_iteration": 1e3 * solve_s / (info["iterations"] + 1),
+            "rel_dF_vs_fp64": abs(res["free_energy"] - f64) / abs(f64)}
+
+
 def run_box(args, n=96, seed=41):
     from bench import HBM_PEAK_GBS, grid_bytes_per_evaluation
-    from namics_amd import ScfSystem, solve_resident_info, synchronize
+    from namics_amd import ScfSystem, synchronize
     from namics_amd.workloads import diblock_box, seeded_box_fields
     s = ScfSystem(diblock_box(n))
     s.set_graph(not args.no_graph)
     x0 = seeded_box_fields(s.desc["fields"], n, seed)
 
     def solve():
-        s.upload(x0)
-        t = time.perf_counter()
-        info = solve_resident_info([s])[0]
-        return info, time.perf_counter() - t
+        return _solve_timed(s, x0)
 
     for _ in range(args.warmup):
         solve()
@@ -44,6 +78,7 @@ def run_box(args, n=96, seed=41):
     achieved = nbytes / solve_s / 1e9
     res = s.results()
     s.close()
+    mixed = None if args.no_mixed else run_mixed(args, x0, n, res["free_energy"])
     return {
         "metric": "converged SCF solves/sec + propagator HBM GB/s at 1e-7 residual; p50/p99 latency",
         "value": conv / elapsed,
@@ -66,6 +101,7 @@ def run_box(args, n=96, seed=41):
         "latency_ms": {"p50": 1e3 * float(np.percentile([r[1] for r in runs], 50)),
                        "p99": 1e3 * float(np.percentile([r[1] for r in runs], 99))},
         "free_energy": res["free_energy"],
+        "mixed_precision": mixed,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "whole solve (per-kernel durations in profiles/k_step3_box96.md)",

@@ -132,8 +132,13 @@ int check_layout(const scf_system *S, int mode, int strip_sx) {
   bool markov1 = true;
   for (auto &m : S->spec.mol) markov1 = markov1 && m.markov == 1;
   const bool strip = mode == SCF_MODE_FUSED && strip_sx > 0;
-  if (S->h.precision != SCF_PREC_FP64 && !strip)
-    return fail(SCF_E_UNSUPPORTED, "FP32 propagators need the 2-gradient register-strip path");
+  // FP32 chains on the grid path: the 6-point simple-cubic 3-gradient steps
+  const SystemSpec &sp = S->spec;
+  const bool grid32 = mode == SCF_MODE_GRID && sp.lat.gradients == 3 && sp.lat.stencil_full &&
+                      sp.lat.lattice_type == LAT_SIMPLE_CUBIC && sp.lat.fjc == 1 && markov1 && !S->branched();
+  if (S->h.precision != SCF_PREC_FP64 && !strip && !grid32)
+    return fail(SCF_E_UNSUPPORTED, "FP32 propagators need the 2-gradient register-strip path or the 6-point "
+                                   "3-gradient grid path (first-order linear chains)");
   if (S->h.ckpt > 1 && !strip && !(mode == SCF_MODE_GRID && markov1))
     return fail(SCF_E_UNSUPPORTED, "checkpointed propagators (save_memory) need first-order chains on the grid "
                                    "path or the 2-gradient register-strip path");
@@ -338,6 +343,8 @@ int build(scf_system *S, const Placement *pl) {
   size_t o_x = a.take(nvar * D);
   size_t o_g = a.take(nvar * D);
   size_t o_G1 = a.take((size_t)nseg * M * D);
+  // FP32 segment weights of the grid path's mixed-precision chains (3 gradients)
+  size_t o_G1f = l.gradients == 3 ? a.take((size_t)nseg * M * sizeof(float)) : 0;
   size_t o_Gf = a.take((size_t)maxNR * M * D);
   size_t o_Gb = a.take((size_t)2 * maxR * M * D);
   size_t o_rho = a.take((size_t)nmm * M * D);
@@ -409,6 +416,7 @@ int build(scf_system *S, const Placement *pl) {
   h.x = P(o_x);
   h.g = P(o_g);
   h.G1 = P(o_G1);
+  h.G1f = l.gradients == 3 ? (float *)P(o_G1f) : nullptr;
   h.Gf = P(o_Gf);
   h.Gb = P(o_Gb);
   h.rho = P(o_rho);
@@ -469,36 +477,70 @@ int grid_solve(scf_system *S, scf_solve_info *info) {
   HIP_TRY(grid_residual(h, S->d, h.partial, S->nb, st));
   HIP_TRY(hipMemcpyAsync(&sc, h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  int it = 0;
+  int it = 0, fp32_its = 0;
   double res = sc.residual;
+  // Mixed precision: FP32 chains from the first iteration on while the
+  // residual is above the switch level and still falling (a new best within
+  // kMixedPatience evaluations); then FP64 for good, with the DIIS history
+  // restarted, so the converged state is an FP64 one.
+  bool fp32 = h.precision != SCF_PREC_FP64 && sc.status == DS_OK && res > h.mixed_switch;
+  double best = res;
+  int since = 0;
+  auto leave_fp32 = [&]() -> int {
+    fp32 = false;
+    HIP_TRY(grid_diis_restart(S->d, st));
+    if (res <= h.tol) res = 2 * h.tol;  // the last residual was an FP32 one: evaluate again in FP64
+    return SCF_OK;
+  };
   if (sc.status == DS_OK) {
     HIP_TRY(grid_axpy_first(h, S->d, S->nb, st));
-    if (!S->graph_exec && S->use_graph) {
-      hipGraph_t graph = nullptr;
-      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      hipError_t ce = grid_iteration(h, S->d, h.partial, S->nb, st);
-      hipError_t ee = hipStreamEndCapture(st, &graph);
-      if (ce != hipSuccess || ee != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return ce != hipSuccess ? hip_fail(ce, "grid_iteration capture") : hip_fail(ee, "hipStreamEndCapture");
-      }
-      hipError_t ie = hipGraphInstantiate(&S->graph_exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (ie != hipSuccess) {
-        S->graph_exec = nullptr;
-        return hip_fail(ie, "hipGraphInstantiate");
-      }
-    }
     while (res > h.tol && it < h.itlim) {
-      if (S->graph_exec)
-        HIP_TRY(hipGraphLaunch(S->graph_exec, st));
+      hipGraphExec_t &ge = fp32 ? S->graph_exec32 : S->graph_exec;
+      if (!ge && S->use_graph) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = grid_iteration(h, S->d, h.partial, S->nb, st, fp32);
+        hipError_t ee = hipStreamEndCapture(st, &graph);
+        if (ce != hipSuccess || ee != hipSuccess) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return ce != hipSuccess ? hip_fail(ce, "grid_iteration capture") : hip_fail(ee, "hipStreamEndCapture");
+        }
+        hipError_t ie = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+          ge = nullptr;
+          return hip_fail(ie, "hipGraphInstantiate");
+        }
+      }
+      if (ge)
+        HIP_TRY(hipGraphLaunch(ge, st));
       else
-        HIP_TRY(grid_iteration(h, S->d, h.partial, S->nb, st));
+        HIP_TRY(grid_iteration(h, S->d, h.partial, S->nb, st, fp32));
       HIP_TRY(hipMemcpyAsync(&sc, h.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
       it = sc.iterations;
-      if (sc.status != DS_OK) break;
+      if (fp32) fp32_its++;
+      if (sc.status != DS_OK) {
+        if (!fp32) break;
+        // an FP32 evaluation that failed (GN out of range): the iteration
+        // changed nothing; FP64 from here
+        int rc = leave_fp32();
+        if (rc != SCF_OK) return rc;
+        continue;
+      }
       res = sc.residual;
+      if (fp32) {
+        if (res < best) {
+          best = res;
+          since = 0;
+        } else {
+          since++;
+        }
+        if (!(res > h.mixed_switch) || since >= kMixedPatience) {
+          int rc = leave_fp32();
+          if (rc != SCF_OK) return rc;
+        }
+      }
     }
   }
   if (sc.status == DS_OK) {
@@ -507,6 +549,7 @@ int grid_solve(scf_system *S, scf_solve_info *info) {
   }
   info->iterations = it;
   info->residual = res;
+  info->fp32_iterations = fp32_its;
   info->status = sc.status != DS_OK ? sc.status : (res <= h.tol ? SCF_OK : SCF_NOT_CONVERGED);
   return SCF_OK;
 }
@@ -554,7 +597,7 @@ int scf_system_create(const char *input_text, int start, scf_system **out, char 
 void scf_system_destroy(scf_system *S) {
   if (!S) return;
   if (S->stream) hipStreamSynchronize(S->stream);
-  if (S->graph_exec) hipGraphExecDestroy(S->graph_exec);
+  S->drop_graphs();
   if (S->arena) hipFree(S->arena);
   if (S->ev0) hipEventDestroy(S->ev0);
   if (S->ev1) hipEventDestroy(S->ev1);
@@ -609,16 +652,20 @@ int scf_system_set_precision(scf_system *S, int mode, double switch_residual) {
     return rc;
   }
   if (mode == SCF_PREC_MIXED) S->h.mixed_switch = switch_residual;
+  if (mode != saved && (S->graph_exec || S->graph_exec32)) {
+    // a mixed solve's captured iterations also refresh the gauge shifts
+    HIP_TRY(hipStreamSynchronize(S->stream));
+    S->drop_graphs();
+  }
   return upload_solver(S);
 }
 
 int scf_system_set_graph(scf_system *S, int on) {
   if (!S) return fail(SCF_E_ARG, "null system");
   S->use_graph = on != 0;
-  if (!S->use_graph && S->graph_exec) {
+  if (!S->use_graph && (S->graph_exec || S->graph_exec32)) {
     HIP_TRY(hipStreamSynchronize(S->stream));
-    hipGraphExecDestroy(S->graph_exec);
-    S->graph_exec = nullptr;
+    S->drop_graphs();
   }
   return SCF_OK;
 }
@@ -639,11 +686,10 @@ int scf_system_set_checkpoint(scf_system *S, int interval) {
     S->h.ckpt = saved;
     return rc;
   }
-  if (K != saved && S->graph_exec) {
-    // the captured grid-path iteration holds the old step schedule
+  if (K != saved && (S->graph_exec || S->graph_exec32)) {
+    // the captured grid-path iterations hold the old step schedule
     HIP_TRY(hipStreamSynchronize(S->stream));
-    (void)hipGraphExecDestroy(S->graph_exec);
-    S->graph_exec = nullptr;
+    S->drop_graphs();
   }
   return upload_solver(S);
 }
@@ -1053,7 +1099,6 @@ int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info) 
       HIP_TRY(hipEventElapsedTime(&ms, S->ev0, S->ev1));
       info[i].kernel_ms = ms;
       info[i].solve_ms = ms;
-      info[i].fp32_iterations = 0;
     } else {
       DevScalars sc;
       int rc = read_scalars(S, sc);

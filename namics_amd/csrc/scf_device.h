@@ -21,6 +21,7 @@ namespace nscf {
 
 constexpr int kMaxSeg = 8;
 constexpr int kMaxMol = 8;
+constexpr int kMixedPatience = 25;  // mixed precision: FP32 evaluations without a new best residual before FP64
 constexpr int kMaxBlocks = 16;
 constexpr int kMaxMM = 16;
 constexpr int kMaxFjc = 4;   // lattice refinement up to FJC_choices 9
@@ -99,6 +100,7 @@ struct DevSys {
   // ---- work arrays (device)
   double *x, *g;
   double *G1;         // nseg * M
+  float *G1f;         // nseg * M (3-gradient grid path): FP32 segment weights times the gauge factor e^c
   // host only: the row schedule of a branched molecule (a BranchPlan, see
   // scf_spec.h), null for a linear chain; never read on the device
   const void *branch[kMaxMol];

@@ -1348,7 +1348,6 @@ __device__ __noinline__ void fused_thermo(const DevSys &S0) {
 }
 
 // ------------------------------------------------------------- DIIS solve
-constexpr int kMixedPatience = 25;  // FP32 evaluations without a new best residual before FP64
 
 // SFNewton::iterate_DIIS (src/sfnewton.cpp:1037-1130).  The residual never
 // changes x, so the iteration's x0 is x itself at the update: one pass per

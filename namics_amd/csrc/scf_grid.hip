@@ -298,6 +298,136 @@ __global__ void __launch_bounds__(kGridThreads) k_step3_bwd(const DevSys *__rest
   rho[i] += hbm(S.Gf)[(size_t)slot * M + i] * v;
 }
 
+// ---- FP32 end points on the 6-point 3-gradient steps (mixed precision).
+// Rows hold G(s) e^{c (s+1)} in FP32 (forward) and G(s) e^{c (N-s)}
+// (backward), where c = -ln(GN / V) / N is the molecule's gauge shift from
+// the latest GN: the segment weights are G1' = G1 e^c (k_g1_shift32), so
+// the shifted end points average about one per site whatever N and the
+// fields.  GN and the composition are taken back to FP64 and unshifted
+// (GN = GN' e^{-cN}; rho += Gf' Gb' e^{-c(N+1)}).  The steps move 12 B per
+// site (forward) and 28 B (backward, rho in FP64) instead of 24 and 48.
+typedef __attribute__((address_space(1))) float gflt;
+__device__ __forceinline__ gflt *hbmf(float *p) { return (gflt *)p; }
+__device__ __forceinline__ const gflt *hbmf(const float *p) { return (const gflt *)p; }
+
+__device__ __forceinline__ float stencil3f(CSys &S, const gflt *__restrict__ g, int x, int y, int z) {
+  const int JX = S.JX, JY = S.JY;
+  const int bx = x * JX, by = y * JY;
+  float v = g[map_x(S, x - 1) * JX + by + z];
+  v += g[map_x(S, x + 1) * JX + by + z];
+  v += g[bx + map_y(S, y - 1) * JY + z];
+  v += g[bx + map_y(S, y + 1) * JY + z];
+  v += g[bx + by + map_z(S, z - 1)];
+  v += g[bx + by + map_z(S, z + 1)];
+  return v * (1.0f / 6.0f);
+}
+
+// G1' = G1 e^c of every segment, for molecule mo's chain
+__global__ void __launch_bounds__(kGridThreads) k_g1_shift32(const DevSys *__restrict__ d, int mo) {
+  CSys &S = uniform_sys(*d);
+  const double es = exp(S.sc->lnshift[mo]);
+  const int n = S.nseg * S.M;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    hbmf(S.G1f)[i] = (float)(hbm(S.G1)[i] * es);
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_row_init32(const DevSys *__restrict__ d, int seg) {
+  CSys &S = uniform_sys(*d);
+  const gflt *g1 = hbmf(S.G1f) + (size_t)seg * S.M;
+  gflt *gf = hbmf((float *)S.Gf);
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    gf[i] = g1[i];
+  }
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_step3_fwd32(const DevSys *__restrict__ d, int src, int dst,
+                                                              int seg) {
+  CSys &S = uniform_sys(*d);
+  const int MZ = S.MZ, M = S.M;
+  int xp, bx;
+  xcd_plane_block(xp, bx);
+  const int p = bx * kGridThreads + threadIdx.x;
+  if (p >= S.MY * MZ) return;
+  const int yy = p / MZ;
+  const int x = xp + 1, y = yy + 1, z = p - yy * MZ + 1;
+  const int i = x * S.JX + y * S.JY + z;
+  const gflt *prev = hbmf((const float *)S.Gf) + (size_t)src * M;
+  hbmf((float *)S.Gf)[(size_t)dst * M + i] = stencil3f(S, prev, x, y, z) * hbmf(S.G1f)[(size_t)seg * M + i];
+}
+
+// first backward row (G1') and its composition, unshifted
+__global__ void __launch_bounds__(kGridThreads) k_bwd_first32(const DevSys *__restrict__ d, int slot, int seg,
+                                                              int mm, int mo) {
+  CSys &S = uniform_sys(*d);
+  const int M = S.M;
+  const double unshift = exp(-S.sc->lnshift[mo] * (S.mol_N[mo] + 1));
+  const gflt *g1 = hbmf(S.G1f) + (size_t)seg * M;
+  const gflt *gf = hbmf((const float *)S.Gf) + (size_t)slot * M;
+  gdbl *rho = hbm(S.rho) + (size_t)mm * M;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    const float v = g1[i];
+    hbmf((float *)S.Gb)[i] = v;
+    rho[i] += (double)gf[i] * (double)v * unshift;
+  }
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_step3_bwd32(const DevSys *__restrict__ d, int slot, int seg,
+                                                              int mm, int src, int mo) {
+  CSys &S = uniform_sys(*d);
+  const int MZ = S.MZ, M = S.M;
+  int xp, bx;
+  xcd_plane_block(xp, bx);
+  const int p = bx * kGridThreads + threadIdx.x;
+  if (p >= S.MY * MZ) return;
+  const int yy = p / MZ;
+  const int x = xp + 1, y = yy + 1, z = p - yy * MZ + 1;
+  const int i = x * S.JX + y * S.JY + z;
+  const double unshift = exp(-S.sc->lnshift[mo] * (S.mol_N[mo] + 1));
+  const gflt *gb = hbmf((const float *)S.Gb) + (size_t)src * M;
+  const float v = stencil3f(S, gb, x, y, z) * hbmf(S.G1f)[(size_t)seg * M + i];
+  hbmf((float *)S.Gb)[(size_t)(1 - src) * M + i] = v;
+  gdbl *rho = hbm(S.rho) + (size_t)mm * M;
+  rho[i] += (double)hbmf((const float *)S.Gf)[(size_t)slot * M + i] * (double)v * unshift;
+}
+
+__global__ void __launch_bounds__(kGridThreads) k_gn_partial32(const DevSys *__restrict__ d, int slot,
+                                                               double *partial) {
+  CSys &S = uniform_sys(*d);
+  __shared__ double red[kGridThreads / 64];
+  const gflt *gl = hbmf((const float *)S.Gf) + (size_t)slot * S.M;
+  double acc = 0;
+  GRID_LOOP(S) {
+    int x, y, z;
+    const int i = work_site(S, w, x, y, z);
+    if (is_interior(S, x, y, z)) acc += site_weight(S, i) * (double)gl[i];
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// gauge shifts for the next evaluation, c = -ln(GN / V) / N from this GN
+// (0 where GN is not positive and finite; such an FP32 evaluation fails)
+__global__ void k_gauge_shifts(const DevSys *__restrict__ d) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const DevSys &S = *d;
+  DevScalars &sc = *S.sc;
+  for (int mo = 0; mo < S.nmol; mo++) {
+    const double gn = sc.GN[mo];
+    sc.lnshift[mo] = (gn > 0 && isfinite(gn)) ? -log(gn / S.volume) / S.mol_N[mo] : 0.0;
+  }
+}
+
+__global__ void k_diis_restart(const DevSys *__restrict__ d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    d->sc->k_diis = 0;
+    d->sc->status = DS_OK;
+  }
+}
+
 __global__ void __launch_bounds__(kGridThreads) k_gn_partial(const DevSys *__restrict__ d, int slot,
                                                              double *partial) {
   CSys &S = uniform_sys(*d);
@@ -319,6 +449,14 @@ __global__ void __launch_bounds__(kGridThreads) k_finalize_gn(const DevSys *__re
   __shared__ double red[kGridThreads / 64];
   const double t = ordered_sum(partial, nparts, 1, 0, red);
   if (threadIdx.x == 0) d->sc->GN[mo] = t;
+}
+
+// the same from FP32 end points: GN = GN' e^{-cN}
+__global__ void __launch_bounds__(kGridThreads) k_finalize_gn32(const DevSys *__restrict__ d,
+                                                                 const double *partial, int nparts, int mo) {
+  __shared__ double red[kGridThreads / 64];
+  const double t = ordered_sum(partial, nparts, 1, 0, red);
+  if (threadIdx.x == 0) d->sc->GN[mo] = t * exp(-d->sc->lnshift[mo] * d->mol_N[mo]);
 }
 
 __global__ void k_norms(const DevSys *__restrict__ d) {
@@ -640,7 +778,7 @@ struct RowSlots {
   int operator()(int r) const { return K <= 1 ? r : (r % K == 0 ? r / K : nck + r % K - 1); }
 };
 
-hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
+hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st, bool fp32) {
   const dim3 B(kGridThreads), G(nb);
   const dim3 G3((h.MY * h.MZ + kGridThreads - 1) / kGridThreads, h.MX);
   // the plane-mapped 6-point kernels serve the default simple-cubic 3-D
@@ -660,6 +798,7 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
       continue;
     }
     if (h.mol_markov[mo] != 1) return hipErrorInvalidValue;
+    if (fp32 && !(six && h.G1f)) return hipErrorInvalidValue;
     const int b0 = h.mol_block_begin[mo], b1 = h.mol_block_begin[mo + 1], N = h.mol_N[mo];
     // block (segment, molecule-segment pair) of every row of the chain
     std::vector<int> rb(N);
@@ -668,16 +807,30 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
     const RowSlots loc{h.ckpt, h.ckpt > 1 ? (N + h.ckpt - 1) / h.ckpt : N};
     auto fwd = [&](int r) {
       const int seg = h.block_seg[rb[r]];
-      if (six)
+      if (fp32)
+        hipLaunchKernelGGL(k_step3_fwd32, G3, B, 0, st, d, loc(r - 1), loc(r), seg);
+      else if (six)
         hipLaunchKernelGGL(k_step3_fwd, G3, B, 0, st, d, loc(r - 1), loc(r), seg);
       else
         hipLaunchKernelGGL(k_step_fwd, G, B, 0, st, d, loc(r - 1), loc(r), seg);
     };
-    hipLaunchKernelGGL(k_row_init, G, B, 0, st, d, h.block_seg[b0]);
+    if (fp32) {
+      hipLaunchKernelGGL(k_g1_shift32, G, B, 0, st, d, mo);
+      hipLaunchKernelGGL(k_row_init32, G, B, 0, st, d, h.block_seg[b0]);
+    } else {
+      hipLaunchKernelGGL(k_row_init, G, B, 0, st, d, h.block_seg[b0]);
+    }
     for (int r = 1; r < N; r++) fwd(r);
-    hipLaunchKernelGGL(k_gn_partial, G, B, 0, st, d, loc(N - 1), partial);
-    hipLaunchKernelGGL(k_finalize_gn, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb, mo);
-    hipLaunchKernelGGL(k_bwd_first, G, B, 0, st, d, loc(N - 1), h.block_seg[rb[N - 1]], h.block_mm[rb[N - 1]]);
+    if (fp32) {
+      hipLaunchKernelGGL(k_gn_partial32, G, B, 0, st, d, loc(N - 1), partial);
+      hipLaunchKernelGGL(k_finalize_gn32, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb, mo);
+      hipLaunchKernelGGL(k_bwd_first32, G, B, 0, st, d, loc(N - 1), h.block_seg[rb[N - 1]], h.block_mm[rb[N - 1]],
+                         mo);
+    } else {
+      hipLaunchKernelGGL(k_gn_partial, G, B, 0, st, d, loc(N - 1), partial);
+      hipLaunchKernelGGL(k_finalize_gn, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb, mo);
+      hipLaunchKernelGGL(k_bwd_first, G, B, 0, st, d, loc(N - 1), h.block_seg[rb[N - 1]], h.block_mm[rb[N - 1]]);
+    }
     int src = 0, seg_in_buffer = h.ckpt > 1 ? (N - 1) / h.ckpt : -1;
     for (int s = N - 2; s >= 0; s--) {
       if (h.ckpt > 1 && s / h.ckpt != seg_in_buffer) {
@@ -687,13 +840,17 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
         for (int r = a0 + 1; r < a0 + h.ckpt && r < N; r++) fwd(r);
       }
       const int b = rb[s];
-      if (six)
+      if (fp32)
+        hipLaunchKernelGGL(k_step3_bwd32, G3, B, 0, st, d, loc(s), h.block_seg[b], h.block_mm[b], src, mo);
+      else if (six)
         hipLaunchKernelGGL(k_step3_bwd, G3, B, 0, st, d, loc(s), h.block_seg[b], h.block_mm[b], src);
       else
         hipLaunchKernelGGL(k_step_bwd, G, B, 0, st, d, loc(s), h.block_seg[b], h.block_mm[b], src);
       src = 1 - src;
     }
   }
+  // a mixed-precision solve keeps its gauge shifts current in both precisions
+  if (h.precision != 0) hipLaunchKernelGGL(k_gauge_shifts, dim3(1), dim3(1), 0, st, d);
   hipLaunchKernelGGL(k_norms, dim3(1), dim3(64), 0, st, d);
   hipLaunchKernelGGL(k_densities, G, B, 0, st, d);
   hipLaunchKernelGGL(k_residual_partial, G, B, 0, st, d, partial);
@@ -716,14 +873,19 @@ hipError_t grid_init(DevSys *d, hipStream_t st) {
 // One DIIS iteration of the grid path with every index on the device, so the
 // launch sequence is identical from one iteration to the next and can be
 // captured once into a hipGraph.
-hipError_t grid_iteration(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st) {
+hipError_t grid_iteration(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st, bool fp32) {
   hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(1), 0, st, d);
   hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, st, d);
-  hipError_t e = grid_residual_body(h, d, partial, nb, st);
+  hipError_t e = grid_residual_body(h, d, partial, nb, st, fp32);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_diis_step_dev, dim3(nb), dim3(kGridThreads), 0, st, d, partial);
   hipLaunchKernelGGL(k_diis_solve_dev, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb);
   hipLaunchKernelGGL(k_diis_combine_dev, dim3(nb), dim3(kGridThreads), 0, st, d);
+  return hipGetLastError();
+}
+
+hipError_t grid_diis_restart(DevSys *d, hipStream_t st) {
+  hipLaunchKernelGGL(k_diis_restart, dim3(1), dim3(1), 0, st, d);
   return hipGetLastError();
 }
 

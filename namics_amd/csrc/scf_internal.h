@@ -21,7 +21,8 @@ struct scf_system {
   size_t arena_bytes = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipGraphExec_t graph_exec = nullptr;  // captured grid-path iteration
+  hipGraphExec_t graph_exec = nullptr;    // captured grid-path iteration
+  hipGraphExec_t graph_exec32 = nullptr;  // the same with FP32 propagators (mixed precision)
   bool use_graph = true;
   int mode = SCF_MODE_AUTO;
   int tuned_mode = SCF_MODE_AUTO;  // the autotuner's choice for this shape (used under AUTO)
@@ -36,6 +37,12 @@ struct scf_system {
   std::vector<nscf::BranchPlan> plans;  // row schedules of the branched molecules (h.branch points here)
 
   bool newton() const { return spec.solver.method != nscf::SOLVER_DIIS; }
+  // destroys the captured grid-path iterations (both precisions)
+  void drop_graphs() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph_exec32) (void)hipGraphExecDestroy(graph_exec32);
+    graph_exec = graph_exec32 = nullptr;
+  }
   bool branched() const {
     for (auto &m : spec.mol)
       if (m.branched) return true;

@@ -31,10 +31,17 @@ hipError_t grid_diis_step(const DevSys &h, DevSys *d, double *d_partial, int nbl
 hipError_t grid_diis_combine(const DevSys &h, DevSys *d, int nblocks, int k, int kd, hipStream_t stream);
 hipError_t grid_axpy_first(const DevSys &h, DevSys *d, int nblocks, hipStream_t stream);
 hipError_t grid_copy_x0(const DevSys &h, DevSys *d, int nblocks, hipStream_t stream);
-hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *d_partial, int nblocks, hipStream_t stream);
+// fp32: chains with FP32 end points and the molecules' gauge shifts (the
+// mixed-precision phase of a solve; 6-point 3-gradient steps only).  With
+// precision MIXED every evaluation also refreshes the gauge shifts from GN.
+hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *d_partial, int nblocks, hipStream_t stream,
+                              bool fp32 = false);
 hipError_t grid_init(DevSys *d, hipStream_t stream);
 // One DIIS iteration with all indices on the device (capturable in a hipGraph).
-hipError_t grid_iteration(const DevSys &h, DevSys *d, double *d_partial, int nblocks, hipStream_t stream);
+hipError_t grid_iteration(const DevSys &h, DevSys *d, double *d_partial, int nblocks, hipStream_t stream,
+                          bool fp32 = false);
+// Restarts the DIIS history (k_diis = 0) and clears the status, on the device.
+hipError_t grid_diis_restart(DevSys *d, hipStream_t stream);
 
 // Plain chain propagation for unit tests: out[s*M+i] = G_s(i) for a field
 // whose Boltzmann weight is G1 (Lattice::propagate repeated N-1 times).

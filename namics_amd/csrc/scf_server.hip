@@ -178,10 +178,7 @@ void grid_worker(scf_server *srv) {
       sc.converged = info.status == SCF_OK;
       if (info.status < 0) sc.status = info.status;
     }
-    if (r->S->graph_exec) {
-      hipGraphExecDestroy(r->S->graph_exec);
-      r->S->graph_exec = nullptr;
-    }
+    r->S->drop_graphs();
     std::lock_guard<std::mutex> lk(srv->mu);
     finish(srv, r, sc, t1, 1e3 * (t1 - t0));
     srv->grid_busy--;
