@@ -122,10 +122,12 @@ enum {
 /* Precision of the solves of this system.  SCF_PREC_MIXED evaluates the
  * propagator recursion in FP32 (per-molecule gauge shift, FP64 composition
  * sums, partition functions, DIIS and thermodynamics) until the residual
- * drops below `switch_residual` (> the tolerance), then continues in FP64 to
- * the tolerance, so the converged state is an FP64 one.  Applies to the
- * 2-gradient strip path; other lattices stay FP64.  No counterpart in the
- * reference (its arithmetic is FP64 throughout). */
+ * drops below `switch_residual` (> the tolerance) or stops falling for 25
+ * evaluations, then continues in FP64 to the tolerance with a fresh DIIS
+ * history, so the converged state is an FP64 one.  Applies to the
+ * 2-gradient strip path and to the 6-point simple-cubic 3-gradient grid path
+ * (first-order linear chains); elsewhere it returns SCF_E_UNSUPPORTED.  No
+ * counterpart in the reference (its arithmetic is FP64 throughout). */
 int scf_system_set_precision(scf_system *sys, int mode, double switch_residual);
 
 /* Checkpointed propagators (the reference's `save_memory`, src/molecule.cpp
