@@ -298,6 +298,34 @@ __global__ void __launch_bounds__(kGridThreads) k_step3_bwd(const DevSys *__rest
   rho[i] += hbm(S.Gf)[(size_t)slot * M + i] * v;
 }
 
+// The forward pass's first and last 6-point steps with the neighbouring
+// one-row phases folded in: the first step reads row 0 from the segment
+// weights and stores it (k_row_init), the last one also starts the
+// backward pass (k_bwd_first: Gb = G1, rho += G(N-1) G1).  Same arithmetic
+// as the separate launches, one launch less each.
+template <bool kFirst, bool kLast>
+__global__ void __launch_bounds__(kGridThreads) k_step3_fwd_x(const DevSys *__restrict__ d, int src, int dst,
+                                                              int seg, int seg0, int slot0, int mm) {
+  CSys &S = uniform_sys(*d);
+  const int MZ = S.MZ, M = S.M;
+  int xp, bx;
+  xcd_plane_block(xp, bx);
+  const int p = bx * kGridThreads + threadIdx.x;
+  if (p >= S.MY * MZ) return;
+  const int yy = p / MZ;
+  const int x = xp + 1, y = yy + 1, z = p - yy * MZ + 1;
+  const int i = x * S.JX + y * S.JY + z;
+  const gdbl *prev = kFirst ? hbm(S.G1) + (size_t)seg0 * M : hbm(S.Gf) + (size_t)src * M;
+  const double g = hbm(S.G1)[(size_t)seg * M + i];
+  const double v = stencil3(S, prev, x, y, z) * g;
+  hbm(S.Gf)[(size_t)dst * M + i] = v;
+  if (kFirst) hbm(S.Gf)[(size_t)slot0 * M + i] = prev[i];
+  if (kLast) {
+    hbm(S.Gb)[i] = g;
+    hbm(S.rho)[(size_t)mm * M + i] += v * g;
+  }
+}
+
 // ---- FP32 end points on the 6-point 3-gradient steps (mixed precision).
 // Rows hold G(s) e^{c (s+1)} in FP32 (forward) and G(s) e^{c (N-s)}
 // (backward), where c = -ln(GN / V) / N is the molecule's gauge shift from
@@ -355,6 +383,31 @@ __global__ void __launch_bounds__(kGridThreads) k_step3_fwd32(const DevSys *__re
   const int i = x * S.JX + y * S.JY + z;
   const gflt *prev = hbmf((const float *)S.Gf) + (size_t)src * M;
   hbmf((float *)S.Gf)[(size_t)dst * M + i] = stencil3f(S, prev, x, y, z) * hbmf(S.G1f)[(size_t)seg * M + i];
+}
+
+// k_step3_fwd_x with FP32 end points
+template <bool kFirst, bool kLast>
+__global__ void __launch_bounds__(kGridThreads) k_step3_fwd32_x(const DevSys *__restrict__ d, int src, int dst,
+                                                                int seg, int seg0, int slot0, int mm, int mo) {
+  CSys &S = uniform_sys(*d);
+  const int MZ = S.MZ, M = S.M;
+  int xp, bx;
+  xcd_plane_block(xp, bx);
+  const int p = bx * kGridThreads + threadIdx.x;
+  if (p >= S.MY * MZ) return;
+  const int yy = p / MZ;
+  const int x = xp + 1, y = yy + 1, z = p - yy * MZ + 1;
+  const int i = x * S.JX + y * S.JY + z;
+  const gflt *prev = kFirst ? hbmf(S.G1f) + (size_t)seg0 * M : hbmf((const float *)S.Gf) + (size_t)src * M;
+  const float g = hbmf(S.G1f)[(size_t)seg * M + i];
+  const float v = stencil3f(S, prev, x, y, z) * g;
+  hbmf((float *)S.Gf)[(size_t)dst * M + i] = v;
+  if (kFirst) hbmf((float *)S.Gf)[(size_t)slot0 * M + i] = prev[i];
+  if (kLast) {
+    const double unshift = exp(-S.sc->lnshift[mo] * (S.mol_N[mo] + 1));
+    hbmf((float *)S.Gb)[i] = g;
+    hbm(S.rho)[(size_t)mm * M + i] += (double)v * (double)g * unshift;
+  }
 }
 
 // first backward row (G1') and its composition, unshifted
@@ -672,11 +725,15 @@ __global__ void k_grid_init(const DevSys *__restrict__ d) {
   }
 }
 
-// start of one DIIS iteration: iterations++, k_diis++ (SFNewton::iterate_DIIS)
+// start of one DIIS iteration: iterations++, k_diis++ (SFNewton::iterate_DIIS),
+// and a clear status for the evaluation that follows
 __global__ void k_iter_begin(const DevSys *__restrict__ d) {
-  if (threadIdx.x == 0 && blockIdx.x == 0 && d->sc->status == DS_OK) {
-    d->sc->iterations++;
-    d->sc->k_diis++;
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (d->sc->status == DS_OK) {
+      d->sc->iterations++;
+      d->sc->k_diis++;
+    }
+    d->sc->status = DS_OK;
   }
 }
 
@@ -814,22 +871,60 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
       else
         hipLaunchKernelGGL(k_step_fwd, G, B, 0, st, d, loc(r - 1), loc(r), seg);
     };
-    if (fp32) {
-      hipLaunchKernelGGL(k_g1_shift32, G, B, 0, st, d, mo);
-      hipLaunchKernelGGL(k_row_init32, G, B, 0, st, d, h.block_seg[b0]);
-    } else {
-      hipLaunchKernelGGL(k_row_init, G, B, 0, st, d, h.block_seg[b0]);
+    // 6-point chains fold the row init into the first step and the first
+    // backward row into the last one
+    const bool fold = six && N >= 2;
+    const int seg0 = h.block_seg[b0], segN = h.block_seg[rb[N - 1]], mmN = h.block_mm[rb[N - 1]];
+    auto fwd_folded = [&](int r) {
+      const bool first = r == 1, last = r == N - 1;
+      const int seg = h.block_seg[rb[r]];
+      if (fp32) {
+        if (first && last)
+          hipLaunchKernelGGL((k_step3_fwd32_x<true, true>), G3, B, 0, st, d, loc(r - 1), loc(r), seg, seg0, loc(0),
+                             mmN, mo);
+        else if (first)
+          hipLaunchKernelGGL((k_step3_fwd32_x<true, false>), G3, B, 0, st, d, loc(r - 1), loc(r), seg, seg0, loc(0),
+                             mmN, mo);
+        else if (last)
+          hipLaunchKernelGGL((k_step3_fwd32_x<false, true>), G3, B, 0, st, d, loc(r - 1), loc(r), seg, seg0,
+                             loc(0), mmN, mo);
+        else
+          fwd(r);
+      } else {
+        if (first && last)
+          hipLaunchKernelGGL((k_step3_fwd_x<true, true>), G3, B, 0, st, d, loc(r - 1), loc(r), seg, seg0, loc(0),
+                             mmN);
+        else if (first)
+          hipLaunchKernelGGL((k_step3_fwd_x<true, false>), G3, B, 0, st, d, loc(r - 1), loc(r), seg, seg0, loc(0),
+                             mmN);
+        else if (last)
+          hipLaunchKernelGGL((k_step3_fwd_x<false, true>), G3, B, 0, st, d, loc(r - 1), loc(r), seg, seg0, loc(0),
+                             mmN);
+        else
+          fwd(r);
+      }
+    };
+    if (fp32) hipLaunchKernelGGL(k_g1_shift32, G, B, 0, st, d, mo);
+    if (!fold) {
+      if (fp32)
+        hipLaunchKernelGGL(k_row_init32, G, B, 0, st, d, seg0);
+      else
+        hipLaunchKernelGGL(k_row_init, G, B, 0, st, d, seg0);
     }
-    for (int r = 1; r < N; r++) fwd(r);
+    for (int r = 1; r < N; r++) {
+      if (fold)
+        fwd_folded(r);
+      else
+        fwd(r);
+    }
     if (fp32) {
       hipLaunchKernelGGL(k_gn_partial32, G, B, 0, st, d, loc(N - 1), partial);
       hipLaunchKernelGGL(k_finalize_gn32, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb, mo);
-      hipLaunchKernelGGL(k_bwd_first32, G, B, 0, st, d, loc(N - 1), h.block_seg[rb[N - 1]], h.block_mm[rb[N - 1]],
-                         mo);
+      if (!fold) hipLaunchKernelGGL(k_bwd_first32, G, B, 0, st, d, loc(N - 1), segN, mmN, mo);
     } else {
       hipLaunchKernelGGL(k_gn_partial, G, B, 0, st, d, loc(N - 1), partial);
       hipLaunchKernelGGL(k_finalize_gn, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb, mo);
-      hipLaunchKernelGGL(k_bwd_first, G, B, 0, st, d, loc(N - 1), h.block_seg[rb[N - 1]], h.block_mm[rb[N - 1]]);
+      if (!fold) hipLaunchKernelGGL(k_bwd_first, G, B, 0, st, d, loc(N - 1), segN, mmN);
     }
     int src = 0, seg_in_buffer = h.ckpt > 1 ? (N - 1) / h.ckpt : -1;
     for (int s = N - 2; s >= 0; s--) {
@@ -875,7 +970,6 @@ hipError_t grid_init(DevSys *d, hipStream_t st) {
 // captured once into a hipGraph.
 hipError_t grid_iteration(const DevSys &h, DevSys *d, double *partial, int nb, hipStream_t st, bool fp32) {
   hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(1), 0, st, d);
-  hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, st, d);
   hipError_t e = grid_residual_body(h, d, partial, nb, st, fp32);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_diis_step_dev, dim3(nb), dim3(kGridThreads), 0, st, d, partial);
