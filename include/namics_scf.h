@@ -269,9 +269,12 @@ int scf_residual(scf_system *sys, const double *x, double *g, double *residual);
  *   - LBFGS -> the reference's LBFGSpp-based solver (src/LBFGS.h,
  *     src/solve_scf.cpp:526-556,630-644), its steps restated over device
  *     residuals (two-loop recursion, trust-radius step, m = 6 pairs by
- *     default).
- * The same rule holds for scf_solve_batch and scf_solve_resident (Newton and
- * LBFGS systems are solved one after another, DIIS systems in one launch);
+ *     default);
+ *   - BRR -> SFNewton::iterate_BRR (src/sfnewton.cpp:929-1035), Broyden's
+ *     inverse update as m = 10 rank-one terms with the reference's rank
+ *     reduction (QR, SVD), restated over device residuals.
+ * The same rule holds for scf_solve_batch and scf_solve_resident (Newton,
+ * LBFGS and BRR systems are solved one after another, DIIS systems in one launch);
  * the job queue and the request server solve every input by DIIS. */
 int scf_solve(scf_system *sys, double *x, scf_solve_info *info);
 

@@ -99,6 +99,8 @@ int validate(const SystemSpec &s, std::string &why) {
   if (nmm > kMaxMM) return why = "more than 16 molecule-segment pairs", SCF_E_UNSUPPORTED;
   if (s.solver.method == SOLVER_LBFGS) {
     if (s.solver.m < 1) return why = "LBFGS needs at least one correction pair (m >= 1)", SCF_E_UNSUPPORTED;
+  } else if (s.solver.method == SOLVER_BRR) {
+    if (s.solver.m < 2) return why = "BRR needs at least two rank-one terms (m >= 2)", SCF_E_UNSUPPORTED;
   } else if (s.solver.m > kMaxDIIS || s.solver.m < 1) {
     return why = "DIIS history m must be 1..16", SCF_E_UNSUPPORTED;
   }
@@ -323,7 +325,8 @@ int build(scf_system *S, const Placement *pl) {
   h.nmm = nmm;
   // DIIS history of the device solvers (the job queue and the server solve
   // every input by DIIS); an LBFGS input's m counts its correction pairs
-  h.m = s.solver.method == SOLVER_LBFGS ? std::min(s.solver.m, kMaxDIIS) : s.solver.m;
+  h.m = s.solver.method == SOLVER_LBFGS || s.solver.method == SOLVER_BRR ? std::min(s.solver.m, kMaxDIIS)
+                                                                           : s.solver.m;
   h.itlim = s.solver.iterationlimit;
   h.max_g = s.solver.max_g;
   h.deltamax = s.solver.deltamax;
@@ -861,7 +864,7 @@ int scf_autotune_batch(scf_system *const *systems, int n, int evals, scf_batch_t
 
 int scf_system_set_solver(scf_system *S, double tolerance, double deltamax, int m, int iterationlimit) {
   if (!S) return fail(SCF_E_ARG, "null system");
-  const bool lbfgs = S->spec.solver.method == SOLVER_LBFGS;
+  const bool lbfgs = S->spec.solver.method == SOLVER_LBFGS || S->spec.solver.method == SOLVER_BRR;
   if (m > S->m_alloc && !lbfgs) return fail(SCF_E_UNSUPPORTED, "DIIS history m above the input's setting");
   if (tolerance > 0) S->h.tol = S->spec.solver.tolerance = tolerance;
   if (deltamax > 0) S->h.deltamax = S->spec.solver.deltamax = deltamax;
@@ -1087,7 +1090,9 @@ int scf_solve_resident(scf_system *const *systems, int n, scf_solve_info *info) 
   for (int i = 0; i < n; i++) {
     scf_system *S = systems[i];
     if (S->newton()) {
-      int rc = S->spec.solver.method == SOLVER_LBFGS ? lbfgs_solve(S, &info[i]) : newton_solve(S, &info[i]);
+      const int method = S->spec.solver.method;
+      int rc = method == SOLVER_LBFGS ? lbfgs_solve(S, &info[i])
+                                      : (method == SOLVER_BRR ? brr_solve(S, &info[i]) : newton_solve(S, &info[i]));
       if (rc != SCF_OK) return rc;
     } else if (S->effective_mode() == SCF_MODE_GRID) {
       HIP_TRY(hipEventRecord(S->ev0, S->stream));

@@ -91,6 +91,8 @@ int grid_solve(scf_system *S, scf_solve_info *info);
 int newton_solve(scf_system *S, scf_solve_info *info);
 // LBFGS solve of an input with `method : LBFGS` (scf_newton.hip)
 int lbfgs_solve(scf_system *S, scf_solve_info *info);
+// Broyden rank-reduction solve of an input with `method : BRR` (scf_newton.hip)
+int brr_solve(scf_system *S, scf_solve_info *info);
 double wall_clock_khz();
 size_t lds_limit();
 void plan_layout(scf_system *S, int want_sx);

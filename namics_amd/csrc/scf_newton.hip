@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 #include "scf_internal.h"
@@ -364,6 +365,242 @@ int lbfgs_solve(scf_system *S, scf_solve_info *info) {
   info->iterations = it;
   info->residual = res;
   info->status = status != DS_OK ? status : (gnorm <= tol ? SCF_OK : SCF_NOT_CONVERGED);
+  info->fp32_iterations = 0;
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  info->kernel_ms = ms;
+  info->solve_ms = ms;
+  S->have_results = status == DS_OK;
+  return SCF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Broyden rank-reduction solver for `newton : method : BRR`
+// (SFNewton::iterate_BRR src/sfnewton.cpp:929-1035; Broyden's inverse update
+// kept as a sum of at most m rank-one terms C_j D_j^T, reduced to its
+// dominant terms when full).  Per iteration:
+//
+//   * s = -g + sum_{j=1..k} C_j (D_j . g), and the step x = x0 + alpha s with
+//     alpha = deltamax while ||g|| > 1, else deltamax - it (1 - deltamax)
+//     ln(||g||) / 1e4, at most 1 (a negative alpha becomes the tolerance);
+//   * with y = g0 - g, a new term: C_k = (s - c) / (s . c) ||d||, D_k = d /
+//     ||d||, where c = sum_{j<k} C_j (D_j . y) - y and d = sum_{j<k} D_j
+//     (C_j . s) - s (the unscaled direction s, as the reference has it);
+//   * when the m terms are in use, D = Q R (thin QR), C <- C R^T, D <- Q,
+//     then C = U S V^T (thin SVD), C <- U S, D <- D V: the same operator,
+//     its terms ordered by singular value; the last term is dropped, and one
+//     more for every singular value (of the 2nd..(m-1)th) below tolerance
+//     times the largest.  Term 0 takes part in the reduction but not in the
+//     updates (the reference's blocks start at column 1).
+//
+// The reference takes the QR and SVD from Eigen (HouseholderQR, JacobiSVD);
+// here a Householder QR that leaves a zero column unreflected (as Eigen's
+// makeHouseholder does) and a one-sided Jacobi SVD.  Both give the same
+// terms up to signs and rounding, and the operator C D^T does not depend on
+// the signs.  Vectors live on the host (2 (m + 1) field-sized columns), as
+// for the Newton and LBFGS solvers; every residual is the device's.
+namespace {
+
+using Cols = std::vector<std::vector<double>>;
+
+// thin Householder QR of the n x p matrix A (columns): Q (n x p, columns)
+// and R (p x p, R[i][j] for i <= j)
+void thin_qr(const Cols &A, Cols &Q, std::vector<std::vector<double>> &R) {
+  const int p = (int)A.size();
+  const size_t n = A[0].size();
+  Cols W = A;  // reflected in place
+  std::vector<double> tau(p, 0.0);
+  std::vector<std::vector<double>> ess(p);
+  R.assign(p, std::vector<double>(p, 0.0));
+  for (int j = 0; j < p; j++) {
+    std::vector<double> &c = W[j];
+    double tail = 0;
+    for (size_t i = j + 1; i < n; i++) tail += c[i] * c[i];
+    const double c0 = c[j];
+    double beta;
+    if (tail <= std::numeric_limits<double>::min()) {
+      tau[j] = 0;
+      beta = c0;
+    } else {
+      beta = std::sqrt(c0 * c0 + tail);
+      if (c0 >= 0) beta = -beta;
+      ess[j].assign(n, 0.0);
+      for (size_t i = j + 1; i < n; i++) ess[j][i] = c[i] / (c0 - beta);
+      tau[j] = (beta - c0) / beta;
+    }
+    R[j][j] = beta;
+    // apply H_j = I - tau w w^T (w = e_j + essential) to the later columns
+    if (tau[j] != 0)
+      for (int l = j + 1; l < p; l++) {
+        std::vector<double> &a = W[l];
+        double t = a[j];
+        for (size_t i = j + 1; i < n; i++) t += ess[j][i] * a[i];
+        t *= tau[j];
+        a[j] -= t;
+        for (size_t i = j + 1; i < n; i++) a[i] -= t * ess[j][i];
+      }
+    for (int l = j + 1; l < p; l++) R[j][l] = W[l][j];
+  }
+  // Q = H_0 H_1 ... H_{p-1} applied to the first p unit vectors
+  Q.assign(p, std::vector<double>(n, 0.0));
+  for (int c = 0; c < p; c++) Q[c][c] = 1.0;
+  for (int j = p - 1; j >= 0; j--) {
+    if (tau[j] == 0) continue;
+    for (int c = 0; c < p; c++) {
+      std::vector<double> &q = Q[c];
+      double t = q[j];
+      for (size_t i = j + 1; i < n; i++) t += ess[j][i] * q[i];
+      t *= tau[j];
+      q[j] -= t;
+      for (size_t i = j + 1; i < n; i++) q[i] -= t * ess[j][i];
+    }
+  }
+}
+
+// one-sided Jacobi SVD of the n x p matrix A (columns): A V = U S; returns
+// U S (columns, by decreasing singular value), V (p x p, V[row][col]) and S
+void thin_svd(const Cols &A, Cols &US, std::vector<std::vector<double>> &V, std::vector<double> &S) {
+  const int p = (int)A.size();
+  Cols W = A;
+  std::vector<std::vector<double>> Vr(p, std::vector<double>(p, 0.0));
+  for (int c = 0; c < p; c++) Vr[c][c] = 1.0;
+  for (int sweep = 0; sweep < 60; sweep++) {
+    bool rotated = false;
+    for (int a = 0; a < p - 1; a++)
+      for (int b = a + 1; b < p; b++) {
+        const double al = dot(W[a], W[a]), be = dot(W[b], W[b]), ga = dot(W[a], W[b]);
+        if (ga == 0 || std::fabs(ga) <= 1e-15 * std::sqrt(al * be)) continue;
+        rotated = true;
+        const double zeta = (be - al) / (2 * ga);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
+        const double c = 1 / std::sqrt(1 + t * t), s = c * t;
+        for (size_t i = 0; i < W[a].size(); i++) {
+          const double wa = W[a][i], wb = W[b][i];
+          W[a][i] = c * wa - s * wb;
+          W[b][i] = s * wa + c * wb;
+        }
+        for (int r = 0; r < p; r++) {
+          const double va = Vr[r][a], vb = Vr[r][b];
+          Vr[r][a] = c * va - s * vb;
+          Vr[r][b] = s * va + c * vb;
+        }
+      }
+    if (!rotated) break;
+  }
+  std::vector<double> sig(p);
+  for (int c = 0; c < p; c++) sig[c] = norm2(W[c]);
+  std::vector<int> order(p);
+  for (int c = 0; c < p; c++) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return sig[u] > sig[v]; });
+  US.assign(p, {});
+  V.assign(p, std::vector<double>(p, 0.0));
+  S.assign(p, 0.0);
+  for (int j = 0; j < p; j++) {
+    US[j] = W[order[j]];
+    S[j] = sig[order[j]];
+    for (int r = 0; r < p; r++) V[r][j] = Vr[r][order[j]];
+  }
+}
+
+}  // namespace
+
+int brr_solve(scf_system *S, scf_solve_info *info) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const DevSys &h = S->h;
+  const size_t n = (size_t)h.nit * h.M;
+  const int m = S->spec.solver.m, p = m + 1;
+  const double tol = h.tol, dmax = h.deltamax;
+  const int itlim = h.itlim;
+  std::vector<double> x(n), g(n), x0(n), g0(n), sdir(n), y(n), w(n);
+  Cols C(p, std::vector<double>(n, 0.0)), D(p, std::vector<double>(n, 0.0));
+  HIP_TRY(hipMemcpyAsync(x.data(), h.x, n * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  HIP_TRY(hipStreamSynchronize(S->stream));
+  double res;
+  int status;
+  int rc = evaluate(S, x.data(), g.data(), &res, &status, false);
+  if (rc != SCF_OK) return rc;
+  double error = norm2(g);
+  int k = 0, it = 0;
+  bool broke = false;
+  // sum_{j=1..kk} A_j (B_j . v) into out (out starts at zero)
+  auto apply = [&](const Cols &A, const Cols &B, int kk, const std::vector<double> &v, std::vector<double> &out) {
+    std::fill(out.begin(), out.end(), 0.0);
+    for (int j = 1; j <= kk; j++) {
+      const double c = dot(B[j], v);
+      for (size_t i = 0; i < n; i++) out[i] += A[j][i] * c;
+    }
+  };
+  while (status == DS_OK && it < itlim && error > tol) {
+    x0 = x;
+    apply(C, D, k, g, w);
+    for (size_t i = 0; i < n; i++) sdir[i] = -g[i] + w[i];
+    g0 = g;
+    double alpha = error > 1 ? dmax : dmax - it * (1.0 - dmax) / 1e4 * std::log(error);
+    if (alpha > 1) alpha = 1;
+    if (alpha < 0) alpha = tol;
+    for (size_t i = 0; i < n; i++) x[i] = x0[i] + alpha * sdir[i];
+    rc = evaluate(S, x.data(), g.data(), &res, &status, false);
+    if (rc != SCF_OK) return rc;
+    if (status != DS_OK) break;
+    error = norm2(g);
+    for (size_t i = 0; i < n; i++) y[i] = g0[i] - g[i];
+    it++;
+    if (k == m) {
+      // rank reduction: D = Q R, C <- C R^T, D <- Q; C = U S V^T, C <- U S, D <- D V
+      Cols Q;
+      std::vector<std::vector<double>> R, V;
+      thin_qr(D, Q, R);
+      Cols CR(p, std::vector<double>(n, 0.0));
+      for (int j = 0; j < p; j++)
+        for (int l = j; l < p; l++) {
+          const double r = R[j][l];
+          if (r != 0)
+            for (size_t i = 0; i < n; i++) CR[j][i] += C[l][i] * r;
+        }
+      std::vector<double> Sv;
+      Cols US;
+      thin_svd(CR, US, V, Sv);
+      Cols DV(p, std::vector<double>(n, 0.0));
+      for (int j = 0; j < p; j++)
+        for (int l = 0; l < p; l++) {
+          const double v = V[l][j];
+          if (v != 0)
+            for (size_t i = 0; i < n; i++) DV[j][i] += Q[l][i] * v;
+        }
+      C = US;
+      D = DV;
+      std::fill(C[k].begin(), C[k].end(), 0.0);
+      std::fill(D[k].begin(), D[k].end(), 0.0);
+      k--;
+      for (int i = m - 1; i > 1; i--)
+        if (Sv[i] / Sv[0] < tol) {
+          std::fill(C[k].begin(), C[k].end(), 0.0);
+          std::fill(D[k].begin(), D[k].end(), 0.0);
+          k--;
+        }
+    }
+    k++;
+    // new rank-one term from s and y
+    apply(C, D, k - 1, y, C[k]);
+    for (size_t i = 0; i < n; i++) C[k][i] -= y[i];
+    const double sthy = dot(sdir, C[k]);
+    apply(D, C, k - 1, sdir, D[k]);
+    for (size_t i = 0; i < n; i++) D[k][i] -= sdir[i];
+    const double nhts = norm2(D[k]);
+    if (sthy == 0 || nhts == 0) {
+      broke = true;  // the reference gives up here
+      break;
+    }
+    for (size_t i = 0; i < n; i++) {
+      C[k][i] = (sdir[i] - C[k][i]) / sthy * nhts;
+      D[k][i] = D[k][i] / nhts;
+    }
+  }
+  // leave the device state (fields, densities, F, Omega, mu) at the final x
+  rc = evaluate(S, x.data(), nullptr, &res, &status, true);
+  if (rc != SCF_OK) return rc;
+  info->iterations = it;
+  info->residual = res;
+  info->status = status != DS_OK ? status : (!broke && error <= tol ? SCF_OK : SCF_NOT_CONVERGED);
   info->fp32_iterations = 0;
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   info->kernel_ms = ms;

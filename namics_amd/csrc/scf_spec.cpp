@@ -1253,8 +1253,10 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
         so.method = SOLVER_HESSIAN;
       else if (v == "LBFGS")
         so.method = SOLVER_LBFGS;
+      else if (v == "BRR")
+        so.method = SOLVER_BRR;
       else {
-        err = "newton method '" + v + "' is not supported (DIIS, pseudohessian, hessian, LBFGS)";
+        err = "newton method '" + v + "' is not supported (DIIS, pseudohessian, hessian, LBFGS, BRR)";
         return false;
       }
     }
@@ -1266,6 +1268,11 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       so.m = 6;
       if (get(p, "newton", nn, "m", v))
         if (!to_int(v, so.m) || so.m < 0 || so.m > 1000) so.m = 6;
+    } else if (so.method == SOLVER_BRR) {
+      // BRR keeps m = 10 rank-one terms, 0..1000 (src/solve_scf.cpp:251-255)
+      so.m = 10;
+      if (get(p, "newton", nn, "m", v))
+        if (!to_int(v, so.m) || so.m < 0 || so.m > 1000) so.m = 10;
     } else if (get(p, "newton", nn, "m", v)) {
       if (!to_int(v, so.m) || so.m < 0 || so.m > 100) so.m = 10;
     }

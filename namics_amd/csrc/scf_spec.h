@@ -31,7 +31,13 @@ enum BoundaryKind : int { BC_MIRROR = 0, BC_PERIODIC = 1, BC_SURFACE = 2 };
 enum LatticeType : int { LAT_SIMPLE_CUBIC = 0, LAT_HEXAGONAL = 1 };
 enum SegFreedom : int { SEG_FREE = 0, SEG_FROZEN = 1, SEG_PINNED = 2 };
 enum MolFreedom : int { MOL_FREE = 0, MOL_RESTRICTED = 1, MOL_SOLVENT = 2 };
-enum SolverMethod : int { SOLVER_DIIS = 0, SOLVER_PSEUDOHESSIAN = 1, SOLVER_HESSIAN = 2, SOLVER_LBFGS = 3 };
+enum SolverMethod : int {
+  SOLVER_DIIS = 0,
+  SOLVER_PSEUDOHESSIAN = 1,
+  SOLVER_HESSIAN = 2,
+  SOLVER_LBFGS = 3,
+  SOLVER_BRR = 4
+};
 
 struct LatticeSpec {
   int gradients = 1;

@@ -94,3 +94,17 @@ def test_lbfgs_input_keys():
     assert d["solver"]["m"] == 6
     d = describe_input(CFG1 + "\nnewton : isaac : method : DIIS\n")
     assert d["solver"]["method"] == 0 and d["solver"]["m"] == 10
+
+
+def test_brr_input_keys():
+    """`newton : method : BRR` is accepted with the reference's defaults
+    (Solve_scf::CheckInput src/solve_scf.cpp:251-255): m = 10 rank-one terms
+    unless given, out of 0..1000 it falls back to 10."""
+    from namics_amd.scf import describe_input
+    from tests.golden.cases import CFG1
+    d = describe_input(CFG1 + "\nnewton : isaac : method : BRR\n")
+    assert d["solver"]["method"] == 4 and d["solver"]["m"] == 10
+    d = describe_input(CFG1 + "\nnewton : isaac : method : BRR\nnewton : isaac : m : 300\n")
+    assert d["solver"]["m"] == 300
+    d = describe_input(CFG1 + "\nnewton : isaac : method : BRR\nnewton : isaac : m : 5000\n")
+    assert d["solver"]["m"] == 10
