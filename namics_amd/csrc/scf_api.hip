@@ -84,13 +84,15 @@ int validate(const SystemSpec &s, std::string &why) {
   if ((int)s.mol.size() > kMaxMol) return why = "more than 8 molecule types", SCF_E_UNSUPPORTED;
   int nb = 0, nmm = 0;
   for (auto &m : s.mol) {
-    if (!m.branched) nb += (int)m.blocks.size();  // a branched molecule runs from its schedule
+    if (!m.scheduled()) nb += (int)m.blocks.size();  // branched molecules and dendrimers run from a schedule
     nmm += (int)m.molmon.size();
   }
   for (auto &m : s.mol) {
-    if (!m.branched) continue;
-    if (m.markov != 1) return why = "branched molecules need first-order (Markov 1) chains", SCF_E_UNSUPPORTED;
-    if (m.save_memory) return why = "save_memory with branched molecules is not supported", SCF_E_UNSUPPORTED;
+    if (!m.scheduled()) continue;
+    if (m.markov != 1)
+      return why = "branched molecules and dendrimers need first-order (Markov 1) chains", SCF_E_UNSUPPORTED;
+    if (m.save_memory)
+      return why = "save_memory with branched molecules or dendrimers is not supported", SCF_E_UNSUPPORTED;
     BranchPlan plan;
     std::string e;
     if (!build_branch_plan(m, plan, e)) return why = "mol '" + m.name + "': " + e, SCF_E_UNSUPPORTED;
@@ -130,7 +132,7 @@ int save_memory_interval(const SystemSpec &s) {
 
 int check_layout(const scf_system *S, int mode, int strip_sx) {
   if (S->branched() && mode == SCF_MODE_FUSED)
-    return fail(SCF_E_UNSUPPORTED, "branched molecules run on the grid path");
+    return fail(SCF_E_UNSUPPORTED, "branched molecules and dendrimers run on the grid path");
   bool markov1 = true;
   for (auto &m : S->spec.mol) markov1 = markov1 && m.markov == 1;
   const bool strip = mode == SCF_MODE_FUSED && strip_sx > 0;
@@ -147,10 +149,12 @@ int check_layout(const scf_system *S, int mode, int strip_sx) {
   return SCF_OK;
 }
 
+// share of segment seg in molecule m (Molecule::fraction src/molecule.cpp:2686-2697,
+// with the blocks' degeneracy for dendrimers, mol_dend::fraction src/mol_dendrimer.cpp:12-22)
 double fraction(const MoleculeSpec &m, int seg) {
   int n = 0;
-  for (auto &b : m.blocks)
-    if (b.seg == seg) n += b.n;
+  for (size_t b = 0; b < m.blocks.size(); b++)
+    if (m.blocks[b].seg == seg) n += m.blocks[b].n * (b < m.block_deg.size() ? m.block_deg[b] : 1);
   return 1.0 * n / m.chainlength;
 }
 
@@ -287,12 +291,12 @@ int build(scf_system *S, const Placement *pl) {
     h.mol_block_begin[i] = nb;
     h.mol_mm_begin[i] = nmm;
     h.branch[i] = nullptr;
-    if (m.branched) {
+    if (m.scheduled()) {
       std::string e;
       if (!build_branch_plan(m, S->plans[i], e)) return fail(SCF_E_UNSUPPORTED, e);
       h.branch[i] = &S->plans[i];
     }
-    for (size_t b = 0; b < m.blocks.size() && !m.branched; b++) {
+    for (size_t b = 0; b < m.blocks.size() && !m.scheduled(); b++) {
       h.block_seg[nb] = m.blocks[b].seg;
       h.block_n[nb] = m.blocks[b].n;
       h.block_mm[nb] = nmm + m.block_molmon[b];
@@ -316,7 +320,7 @@ int build(scf_system *S, const Placement *pl) {
   h.ckpt = save_memory_interval(s);
   for (int i = 0; i < nmol; i++) {
     const MoleculeSpec &m = s.mol[i];
-    maxNR = std::max(maxNR, m.branched ? S->plans[i].rows : ckpt_rows(m.chainlength, h.ckpt) * m.size);
+    maxNR = std::max(maxNR, m.scheduled() ? S->plans[i].rows : ckpt_rows(m.chainlength, h.ckpt) * m.size);
   }
   S->gf_rows = maxNR;
   h.mol_block_begin[nmol] = nb;

@@ -175,8 +175,96 @@ struct PlanBuilder {
 
 }  // namespace
 
+// mol_dend::BackAndForth (src/mol_dendrimer.cpp:122-188): one path from the
+// core through one arm of every generation, rows 0 .. slast (row 0 the core's
+// branch point).  Forward from the outer end: each generation's spacer, then
+// its branch point as its weight times the spacer's first row stepped once,
+// to the power of the functionality.  Backward from the core: the core's
+// weight times all but one of its arms, then along the path, every
+// composition weighted by the block's degeneracy; at each further branch
+// point the backward row takes the other arms of that generation.
+bool build_dendrimer_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &err) {
+  const int G = (int)m.n_arm.size();
+  if (G == 0 || (int)m.first_b.size() != G || (int)m.last_s.size() != G) {
+    err = "dendrimer without generations";
+    return false;
+  }
+  const int slast = m.last_s[G - 1];
+  const int gb0 = slast + 1, g0 = slast + 3, g1 = slast + 4, g2 = slast + 5;
+  std::vector<BranchOp> &fw = plan.forward, &bw = plan.backward;
+  auto op = [](std::vector<BranchOp> &v, int kind, int a, int b, int c, int seg, double f = 1.0) {
+    v.push_back({kind, a, b, c, seg, f});
+  };
+  auto seg_of = [&](int b) { return m.blocks[b].seg; };
+  auto mm_of = [&](int b) { return m.block_molmon[b]; };
+  auto gb = [&](int s) { return gb0 + (((s % 2) + 2) % 2); };
+  int s = slast;
+  for (int g = G - 1; g >= 0; g--) {
+    op(fw, BR_ONE, g2, 0, 0, 0);
+    const int b0 = m.first_b[g], bN = m.last_b[g];
+    for (int b = bN; b >= b0; b--)
+      for (int k = 0; k < m.blocks[b].n; k++) {
+        if (s < slast)
+          op(fw, BR_PROP, s, s + 1, 0, seg_of(b));
+        else
+          op(fw, BR_INIT, slast, 0, 0, seg_of(b));
+        s--;
+      }
+    if (s < 0 || b0 < 1) {
+      err = "dendrimer path out of range";
+      return false;
+    }
+    op(fw, BR_COPY, g0, s + 1, 0, 0);
+    op(fw, BR_PROP, g1, g0, 0, -1);
+    op(fw, BR_PROP, g2, g0, 0, seg_of(b0 - 1));
+    for (int k = 0; k < m.n_arm[g] - 1; k++) op(fw, BR_TIMES, g2, g2, g1, 0);
+    op(fw, BR_COPY, s, g2, 0, 0);
+    s--;
+  }
+  if (s != -1) {
+    err = "dendrimer path does not end at the core";
+    return false;
+  }
+  plan.gn_row = 0;
+  op(bw, BR_INIT, gb0, 0, 0, seg_of(0));
+  op(bw, BR_ADDPHI, 0, gb0, mm_of(0), 0);
+  s = 0;
+  op(bw, BR_ONE, g2, 0, 0, 0);
+  op(bw, BR_COPY, g0, 1, 0, 0);
+  op(bw, BR_PROP, g1, g0, 0, -1);
+  for (int k = 0; k < m.n_arm[0] - 1; k++) op(bw, BR_TIMES, g2, g2, g1, 0);
+  op(bw, BR_TIMES_G1, g2, g2, 0, seg_of(0));
+  op(bw, BR_COPY, gb0, g2, 0, 0);
+  for (int g = 0; g < G; g++) {
+    op(bw, BR_ONE, g2, 0, 0, 0);
+    const int b0 = m.first_b[g], bN = m.last_b[g];
+    for (int b = b0; b <= bN; b++)
+      for (int k = 0; k < m.blocks[b].n; k++) {
+        op(bw, BR_PROP, gb(s + 1), gb(s), 0, seg_of(b));
+        s++;
+        op(bw, BR_ADDPHI, s, gb(s), mm_of(b), 0, (double)m.block_deg[b]);
+      }
+    if (s < slast) {
+      const int bp = bN + 1;  // the next generation's branch point
+      op(bw, BR_ONE, g2, 0, 0, 0);
+      op(bw, BR_COPY, g0, s + 2, 0, 0);
+      op(bw, BR_PROP, g1, g0, 0, -1);
+      for (int k = 0; k < m.n_arm[g + 1] - 1; k++) op(bw, BR_TIMES, g2, g2, g1, 0);
+      op(bw, BR_PROP, gb(s + 1), gb(s), 0, seg_of(bp));
+      s++;
+      op(bw, BR_ADDPHI, s, gb(s), mm_of(bp), 0, (double)m.block_deg[bp]);
+      op(bw, BR_COPY, g0, gb(s), 0, 0);
+      op(bw, BR_TIMES, g0, g0, g2, 0);
+      op(bw, BR_COPY, gb(s), g0, 0, 0);
+    }
+  }
+  plan.rows = g2 + 1;
+  return true;
+}
+
 bool build_branch_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &err) {
   plan = BranchPlan();
+  if (m.dendrimer) return build_dendrimer_plan(m, plan, err);
   if (!m.branched) {
     err = "not a branched molecule";
     return false;

@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(kGridThreads) k_step_fwd(const DevSys *__restr
 // One operation of a branched molecule's row schedule (BranchOpKind in
 // scf_spec.h) over the sites a phase visits: rows are Gf rows of M sites.
 __global__ void __launch_bounds__(kGridThreads) k_branch_op(const DevSys *__restrict__ d, int kind, int a, int b,
-                                                            int c, int seg) {
+                                                            int c, int seg, double f) {
   CSys &S = uniform_sys(*d);
   const int M = S.M;
   gdbl *ra = hbm(S.Gf) + (size_t)a * M;
@@ -190,8 +190,12 @@ __global__ void __launch_bounds__(kGridThreads) k_branch_op(const DevSys *__rest
       ra[i] = rb[i];
     } else if (kind == BR_TIMES) {
       ra[i] = rb[i] * rc[i];
-    } else {  // BR_ADDPHI
-      hbm(S.rho)[(size_t)c * M + i] += ra[i] * rb[i];
+    } else if (kind == BR_ONE) {
+      ra[i] = 1.0;
+    } else if (kind == BR_TIMES_G1) {
+      ra[i] = rb[i] * g1[i];
+    } else {  // BR_ADDPHI: (f Gf) Gb, f = 1 or a dendrimer block's degeneracy
+      hbm(S.rho)[(size_t)c * M + i] += f * ra[i] * rb[i];
     }
   }
 }
@@ -847,11 +851,11 @@ hipError_t grid_residual_body(const DevSys &h, DevSys *d, double *partial, int n
       // a branched molecule: its schedule, GN over the row it names
       const BranchPlan &plan = *(const BranchPlan *)h.branch[mo];
       for (const BranchOp &o : plan.forward)
-        hipLaunchKernelGGL(k_branch_op, G, B, 0, st, d, o.kind, o.a, o.b, o.c, o.seg);
+        hipLaunchKernelGGL(k_branch_op, G, B, 0, st, d, o.kind, o.a, o.b, o.c, o.seg, o.f);
       hipLaunchKernelGGL(k_gn_partial, G, B, 0, st, d, plan.gn_row, partial);
       hipLaunchKernelGGL(k_finalize_gn, dim3(1), dim3(kGridThreads), 0, st, d, partial, nb, mo);
       for (const BranchOp &o : plan.backward)
-        hipLaunchKernelGGL(k_branch_op, G, B, 0, st, d, o.kind, o.a, o.b, o.c, o.seg);
+        hipLaunchKernelGGL(k_branch_op, G, B, 0, st, d, o.kind, o.a, o.b, o.c, o.seg, o.f);
       continue;
     }
     if (h.mol_markov[mo] != 1) return hipErrorInvalidValue;

@@ -45,13 +45,13 @@ struct scf_system {
   }
   bool branched() const {
     for (auto &m : spec.mol)
-      if (m.branched) return true;
+      if (m.scheduled()) return true;
     return false;
   }
 
   int effective_mode() const {
     if (mode != SCF_MODE_AUTO) return mode;
-    // branched molecules run their row schedules on the grid path
+    // branched molecules and dendrimers run their row schedules on the grid path
     if (branched()) return SCF_MODE_GRID;
     // checkpointed end-point storage runs on the strip chains or the grid path
     if (h.ckpt > 1 && h.strip_sx == 0) return SCF_MODE_GRID;

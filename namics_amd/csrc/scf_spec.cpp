@@ -660,6 +660,74 @@ struct BranchParser {
   }
 };
 
+// A symmetric dendrimer, `@dend(A,(B)2,3; A,(B)2,3)`: per generation a
+// branch-point segment, a spacer and a functionality (Molecule::Decomposition
+// src/molecule.cpp:1437-1555).  The blocks run along one path from the core
+// (branch point, spacer, next branch point, ...); a block's degeneracy is
+// the number of copies of it in the molecule.
+bool parse_dendrimer(const std::string &c, const std::vector<std::string> &mons, MoleculeSpec &m,
+                     std::string &err) {
+  if (c.size() < 7 || c.compare(0, 6, "@dend(") != 0 || c.back() != ')') {
+    err = "composition '" + c + "': a dendrimer is written @dend(A,(B)n,f; ...)";
+    return false;
+  }
+  const std::string s = c.substr(6, c.size() - 7);
+  if (s.find(':') != std::string::npos || s.find('#') != std::string::npos) {
+    err = "composition '" + c + "': aliases are not supported";
+    return false;
+  }
+  m.blocks.clear();
+  m.block_deg.clear();
+  m.n_arm.clear();
+  m.first_s.clear();
+  m.last_s.clear();
+  m.first_b.clear();
+  m.last_b.clear();
+  int degeneracy = 1, N = -1, chainlength = 0;
+  for (const std::string &gen : split(s, ';')) {
+    auto sub = split(gen, ',');
+    if (sub.size() < 2 || (sub.size() - 1) % 2 != 0) {
+      err = "composition '" + c + "': each generation is branch point, spacer, functionality";
+      return false;
+    }
+    if (sub.size() > 3) {
+      err = "composition '" + c + "': asymmetric dendrimers are not supported";
+      return false;
+    }
+    auto it = std::find(mons.begin(), mons.end(), sub[0]);
+    if (it == mons.end()) {
+      err = "composition '" + c + "': branch point '" + sub[0] + "' is not a defined segment";
+      return false;
+    }
+    m.blocks.push_back({(int)(it - mons.begin()), 1});
+    m.block_deg.push_back(degeneracy);
+    chainlength += degeneracy;
+    N++;
+    const int f = leading_int(sub[2], 0);
+    if (f < 1) {
+      err = "composition '" + c + "': the functionality of a generation must be a positive integer";
+      return false;
+    }
+    std::vector<BlockSpec> spacer;
+    if (!parse_composition(sub[1], mons, spacer, err)) return false;
+    m.n_arm.push_back(f);
+    m.first_s.push_back(N + 1);
+    m.first_b.push_back((int)m.blocks.size());
+    for (const BlockSpec &b : spacer) {
+      m.blocks.push_back(b);
+      N += b.n;
+      m.block_deg.push_back(degeneracy * f);
+      chainlength += degeneracy * b.n * f;
+    }
+    m.last_b.push_back((int)m.blocks.size() - 1);
+    m.last_s.push_back(N);
+    degeneracy *= f;
+  }
+  m.dendrimer = true;
+  (void)chainlength;  // recomputed from the blocks and their degeneracies by the caller
+  return true;
+}
+
 bool parse_branched(const std::string &c, const std::vector<std::string> &mons, MoleculeSpec &m,
                     std::string &err) {
   if (c.empty() || c[0] != '(') {
@@ -1064,14 +1132,22 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       return false;
     }
     s.mol_composition.push_back(v);
-    if (v.find('[') != std::string::npos) {
+    if (!v.empty() && v[0] == '@') {
+      if (v.compare(0, 6, "@dend(") != 0) {
+        err = "composition '" + v + "': of the @ architectures only @dend (symmetric dendrimers) is supported";
+        return false;
+      }
+      if (!parse_dendrimer(v, p.mons, m, err)) return false;
+    } else if (v.find('[') != std::string::npos) {
       if (!parse_branched(v, p.mons, m, err)) return false;
     } else if (!parse_composition(v, p.mons, m.blocks, err)) {
       return false;
     }
+    if (m.block_deg.size() != m.blocks.size()) m.block_deg.assign(m.blocks.size(), 1);
     m.chainlength = 0;
-    for (auto &b : m.blocks) {
-      m.chainlength += b.n;
+    for (size_t bi = 0; bi < m.blocks.size(); bi++) {
+      const BlockSpec &b = m.blocks[bi];
+      m.chainlength += b.n * m.block_deg[bi];
       if (s.seg[b.seg].freedom == SEG_FROZEN) {
         err = "mol '" + m.name + "' contains a frozen segment";
         return false;
@@ -1496,8 +1572,8 @@ std::string describe_json(const SystemSpec &s) {
     o << ",\"chainlength\":" << m.chainlength << ",\"freedom\":" << m.freedom << ",\"markov\":" << m.markov
       << ",\"pinned\":" << (m.pinned ? "true" : "false") << ",\"save_memory\":" << (m.save_memory ? "true" : "false")
       << ",\"size\":" << m.size << ",\"blocks\":" << m.blocks.size() << ",\"molmon\":" << m.molmon.size();
-    if (m.branched) {
-      // the branched molecule's blocks, generations and row schedule
+    if (m.scheduled()) {
+      // the branched molecule's or dendrimer's blocks, generations and row schedule
       auto ints = [&](const std::vector<int> &v) {
         o << "[";
         for (size_t k = 0; k < v.size(); k++) o << (k ? "," : "") << v[k];
@@ -1509,6 +1585,10 @@ std::string describe_json(const SystemSpec &s) {
       for (size_t k = 0; k < m.blocks.size(); k++) o << (k ? "," : "") << m.blocks[k].n;
       o << "],\"block_molmon\":";
       ints(m.block_molmon);
+      o << ",\"dendrimer\":" << (m.dendrimer ? "true" : "false") << ",\"block_deg\":";
+      ints(m.block_deg);
+      o << ",\"n_arm\":";
+      ints(m.n_arm);
       o << ",\"gnr\":";
       ints(m.gnr);
       o << ",\"first_b\":";
@@ -1526,7 +1606,7 @@ std::string describe_json(const SystemSpec &s) {
           o << "[";
           for (size_t k = 0; k < v.size(); k++)
             o << (k ? "," : "") << "[" << v[k].kind << "," << v[k].a << "," << v[k].b << "," << v[k].c << ","
-              << v[k].seg << "]";
+              << v[k].seg << "," << v[k].f << "]";
           o << "]";
         };
         o << ",\"gn_row\":" << plan.gn_row << ",\"rows\":" << plan.rows << ",\"forward\":";

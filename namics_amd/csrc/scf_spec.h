@@ -109,6 +109,17 @@ struct MoleculeSpec {
   // index of its first segment / one past its last (first_s, last_s).
   bool branched = false;
   std::vector<int> gnr, first_b, last_b, first_s, last_s;
+  // Symmetric dendrimer (`@dend(A,(B)n,f; ...)`, Molecule::Decomposition
+  // src/molecule.cpp:1437-1555): one branch-point block and the spacer
+  // blocks of one arm per generation, along one path from the core; block_deg
+  // is each block's degeneracy (how many copies the molecule holds), n_arm
+  // the functionality of each generation, and first_b / last_b / first_s /
+  // last_s the spacer blocks and path indices of each generation.  Linear
+  // and branched molecules have block_deg 1.
+  bool dendrimer = false;
+  std::vector<int> block_deg, n_arm;
+  // propagated by a row schedule (build_branch_plan) instead of a linear chain
+  bool scheduled() const { return branched || dendrimer; }
 };
 
 // The propagator schedule of one branched molecule: a list of row
@@ -118,14 +129,17 @@ struct MoleculeSpec {
 // rows of the junctions.  `seg` is a segment type; -1 in BR_PROP means no
 // segment weight (the reference's UNITY).
 enum BranchOpKind : int {
-  BR_INIT = 0,    // R[a] = G1[seg]
-  BR_PROP = 1,    // R[a] = W(R[b]) * G1[seg]
-  BR_COPY = 2,    // R[a] = R[b]
-  BR_TIMES = 3,   // R[a] = R[b] * R[c]
-  BR_ADDPHI = 4,  // rho[c] += R[a] * R[b]   (c: molecule-segment index)
+  BR_INIT = 0,      // R[a] = G1[seg]
+  BR_PROP = 1,      // R[a] = W(R[b]) * G1[seg]
+  BR_COPY = 2,      // R[a] = R[b]
+  BR_TIMES = 3,     // R[a] = R[b] * R[c]
+  BR_ADDPHI = 4,    // rho[c] += f * R[a] * R[b]   (c: molecule-segment index; f = 1 or a degeneracy)
+  BR_ONE = 5,       // R[a] = 1
+  BR_TIMES_G1 = 6,  // R[a] = R[b] * G1[seg]
 };
 struct BranchOp {
   int kind, a, b, c, seg;
+  double f = 1.0;
 };
 struct BranchPlan {
   std::vector<BranchOp> forward, backward;

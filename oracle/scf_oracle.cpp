@@ -1291,14 +1291,18 @@ struct Oracle {
     }
   }
 
+  // Molecule::fraction (src/molecule.cpp:2686-2697); dendrimer blocks count
+  // their degeneracy (mol_dend::fraction src/mol_dendrimer.cpp:12-22)
   double fraction(int mol, int seg) const {
+    const MoleculeSpec &m = s.mol[mol];
     int Nseg = 0;
-    for (auto &b : s.mol[mol].blocks)
-      if (b.seg == seg) Nseg += b.n;
-    return 1.0 * Nseg / s.mol[mol].chainlength;
+    for (size_t b = 0; b < m.blocks.size(); b++)
+      if (m.blocks[b].seg == seg) Nseg += m.blocks[b].n * (b < m.block_deg.size() ? m.block_deg[b] : 1);
+    return 1.0 * Nseg / m.chainlength;
   }
 
-  // mol_branched::ComputePhi (src/mol_branched.cpp:303-420) by the row
+  // mol_branched::ComputePhi (src/mol_branched.cpp:303-420) and
+  // mol_dend::BackAndForth (src/mol_dendrimer.cpp:122-188) by the row
   // schedule of the molecule (build_branch_plan: the reference's recursion
   // unrolled into row operations); Markov 1.  UNITY is a row of ones
   // (the reference's UNITY array), Initiate / Terminate are copies.
@@ -1322,7 +1326,15 @@ struct Oracle {
           case BR_PROP: lat.propagate(R[o.a].data(), R[o.b].data(), o.seg < 0 ? unity.data() : G1[o.seg].data()); break;
           case BR_COPY: Cp(R[o.a].data(), R[o.b].data(), M); break;
           case BR_TIMES: Times(R[o.a].data(), R[o.b].data(), R[o.c].data(), M); break;
-          default: AddTimes(rho.data() + o.c * M, R[o.a].data(), R[o.b].data(), M); break;
+          case BR_ONE: Cp(R[o.a].data(), unity.data(), M); break;
+          case BR_TIMES_G1: Times(R[o.a].data(), R[o.b].data(), G1[o.seg].data(), M); break;
+          default:
+            // AddTimes (f = 1) or YplusisCtimesAtimesB (a degeneracy): rho += (f Gf) Gb
+            if (o.f == 1.0)
+              AddTimes(rho.data() + o.c * M, R[o.a].data(), R[o.b].data(), M);
+            else
+              YplusisCtimesAtimesB(rho.data() + o.c * M, R[o.a].data(), R[o.b].data(), o.f, M);
+            break;
         }
       }
     };
@@ -1334,7 +1346,7 @@ struct Oracle {
   // Molecule::ComputePhi for a linear chain; accumulates Gf*Gb into mol_phi
   void compute_phi_mol(int mi) {
     const MoleculeSpec &m = s.mol[mi];
-    if (m.branched) {
+    if (m.scheduled()) {
       compute_phi_branched(mi);
       return;
     }

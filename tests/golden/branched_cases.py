@@ -9,7 +9,8 @@ cover: side groups on a branch point inside the backbone, adjacent groups
 on one branch point, a group nested in a side chain, a chain that starts
 with a branch point (the reference then sums GN over the row in front of
 the side group), free and restricted molecules, and 1-, 2- and 3-gradient
-lattices (planar and cylindrical).
+lattices (planar and cylindrical); and symmetric dendrimers (@dend): a
+star and two-generation dendrimers, with a multi-block spacer.
 """
 
 WALL_1D = """
@@ -170,5 +171,90 @@ mol : water : freedom : solvent
 """,
         eval=[(76, 0.2)],
         converged=[dict(method="DIIS", tol=1e-7, seed=86, amp=0.2)],
+    ),
+    # symmetric dendrimers (@dend): a four-arm star at a wall, a
+    # two-generation dendrimer in a cylindrical micelle-like confinement,
+    # and both in 2- and 3-gradient boxes
+    "dend1d_star": dict(
+        text=WALL_1D + """
+mon : A : freedom : free
+mon : B : freedom : free
+mon : W : freedom : free
+mon : S : chi_B : -0.6
+mon : B : chi_W : 0.5
+mol : star : composition : @dend(A,(B)8,4)
+mol : star : freedom : restricted
+mol : star : theta : 4.0
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(91, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-7)],
+    ),
+    "dend1d_gen2": dict(
+        text=WALL_1D + """
+mon : A : freedom : free
+mon : B : freedom : free
+mon : C : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -0.4
+mon : C : chi_W : 0.9
+mol : dend : composition : @dend(A,(B)3,3;A,(B)2(C)1,2)
+mol : dend : freedom : free
+mol : dend : phibulk : 0.003
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(92, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-7)],
+    ),
+    "dend2d_star": dict(
+        text="""
+lat : flat : gradients : 2
+lat : flat : n_layers_x : 10
+lat : flat : n_layers_y : 12
+lat : flat : lattice_type : simple_cubic
+lat : flat : lowerbound_x : mirror
+lat : flat : upperbound_x : mirror
+lat : flat : lowerbound_y : periodic
+lat : flat : upperbound_y : periodic
+mon : A : freedom : free
+mon : B : freedom : free
+mon : W : freedom : free
+mon : B : chi_W : 0.8
+mol : star : composition : @dend(A,(B)4,3)
+mol : star : freedom : restricted
+mol : star : theta : 30
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(93, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-7, seed=94, amp=0.1)],
+    ),
+    "dend3d_gen2": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 8
+lat : box : n_layers_y : 8
+lat : box : n_layers_z : 8
+lat : box : lattice_type : simple_cubic
+lat : box : lowerbound_x : periodic
+lat : box : upperbound_x : periodic
+lat : box : lowerbound_y : periodic
+lat : box : upperbound_y : periodic
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : A : freedom : free
+mon : B : freedom : free
+mon : W : freedom : free
+mon : B : chi_W : 1.1
+mol : dend : composition : @dend(A,(B)2,3;A,(B)2,2)
+mol : dend : freedom : restricted
+mol : dend : theta : 60
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(95, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-7, seed=96, amp=0.2)],
     ),
 }

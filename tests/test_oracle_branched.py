@@ -84,7 +84,8 @@ def test_branched_parse_by_hand():
     assert b["first_b"] == [0, 2] and b["last_b"] == [4, 2]
     assert b["first_s"] == [0, 4] and b["last_s"] == [8, 5]
     assert b["gn_row"] == 8
-    fwd = b["forward"]
+    assert all(o[5] == 1 for o in b["forward"] + b["backward"])  # no degeneracy weights
+    fwd = [o[:5] for o in b["forward"]]
     INIT, PROP, COPY, TIMES = 0, 1, 2, 3
     # main chain rows 0..3, then the side chain rows 4, 5
     assert fwd[:4] == [[INIT, 0, 0, 0, A], [PROP, 1, 0, 0, A], [PROP, 2, 1, 0, A], [PROP, 3, 2, 0, A]]
@@ -105,3 +106,33 @@ def test_branched_refusals():
     for bad in ("(A)4[(B)3]", "(A)4[(B)3](A)1]", "(A)4[[(B)3](A)1", "(A)4[(Q)3](A)1", "(A)4[(B)0](A)1"):
         with pytest.raises(ScfError):
             describe_input(base.replace(comp, bad))
+
+
+def test_dendrimer_parse_by_hand():
+    """@dend(A,(B)3,3;A,(B)2(C)1,2): the path from the core is A, (B)3, A,
+    (B)2, (C)1; the core counts once, the first spacer three times, the
+    second generation's branch point three times and its spacer 3 x 2 = 6
+    times: chain length 1 + 9 + 3 + 18 = 31.  Path indices: generation 0's
+    spacer 1..3, generation 1's 5..7."""
+    from namics_amd.scf import describe_input
+    text = BRANCHED["dend1d_gen2"]["text"]
+    m = [m for m in describe_input(text)["molecules"] if m["name"] == "dend"][0]
+    b = m["branch"]
+    A, B, C = 1, 2, 3  # S is segment 0
+    assert m["chainlength"] == 31 and b["dendrimer"]
+    assert b["block_seg"] == [A, B, A, B, C] and b["block_n"] == [1, 3, 1, 2, 1]
+    assert b["block_deg"] == [1, 3, 3, 6, 6] and b["n_arm"] == [3, 2]
+    assert b["first_s"] == [1, 5] and b["last_s"] == [3, 7]
+    assert b["first_b"] == [1, 3] and b["last_b"] == [1, 4]
+    assert b["gn_row"] == 0
+    # the compositions of the spacer rows carry the blocks' degeneracies
+    weights = sorted({o[5] for o in b["backward"] if o[0] == 4})
+    assert weights == [1, 3, 6]
+
+
+def test_dendrimer_refusals():
+    from namics_amd.scf import ScfError, describe_input
+    base = BRANCHED["dend1d_star"]["text"]
+    for bad in ("@dend(A,(B)8,4,(B)3,1)", "@dend(A,(B)8)", "@dend(Q,(B)8,4)", "@dend(A,(B)8,0)", "@comb(A,(B)8,4)"):
+        with pytest.raises(ScfError):
+            describe_input(base.replace("@dend(A,(B)8,4)", bad))
