@@ -351,7 +351,9 @@ def run_brush(args):
                 "switch_residual": args.switch,
                 "fp32_iteration_fraction": sum(r["fp32_iterations"] for r in sm) /
                 max(1, sum(r["iterations"] for r in sm)),
-                "rel_free_energy_vs_fp64": {"max": float(np.max(rel64)), "median": float(np.median(rel64))}
+                "rel_free_energy_vs_fp64": {"max": float(np.max(rel64)), "median": float(np.median(rel64)),
+                                            "bar": 1e-8,
+                                            "within_bar_fraction": float(np.mean(np.array(rel64) <= 1e-8))}
                 if rel64 else None,
                 "vs_reference": cmpm if ref else None,
                 "latency_ms": {"p50": float(np.percentile([r["solve_ms"] for r in sm], 50)),

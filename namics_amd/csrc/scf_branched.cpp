@@ -262,9 +262,143 @@ bool build_dendrimer_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &
   return true;
 }
 
+// mol_comb::GoBackAndForth (src/mol_comb.cpp:206-400), no ring.  Rows are
+// path indices as the reference uses them: the tooth is propagated once,
+// from its free end, into rows first_s[1]+1 .. last_s[1]+1, and its row next
+// to the backbone, stepped once without a weight, is the factor GB every
+// branch point carries; the backbone runs from the tail's end to the head,
+// the first branch point stored at row last_s[0]+1.  The backward pass
+// walks the backbone, turns each branch point's forward row into the
+// tooth's backward start (Gb Gf / GB / G1) and, last, propagates the tooth
+// once per tooth from those rows.
+bool build_comb_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &err) {
+  const int narm = m.n_arm.empty() ? 0 : m.n_arm[0];
+  const int G = narm + 3;
+  if (narm < 1 || (int)m.first_b.size() != G) {
+    err = "comb without teeth";
+    return false;
+  }
+  int maxrow = 0;
+  for (int g = 0; g < G; g++) maxrow = std::max(maxrow, m.last_s[g] + 1);
+  const int gb0 = maxrow + 1, GB0 = maxrow + 3, GB1 = maxrow + 4;
+  std::vector<BranchOp> &fw = plan.forward, &bw = plan.backward;
+  auto op = [](std::vector<BranchOp> &v, int kind, int a, int b, int c, int seg) { v.push_back({kind, a, b, c, seg}); };
+  auto seg_of = [&](int b) { return m.blocks[b].seg; };
+  auto mm_of = [&](int b) { return m.block_molmon[b]; };
+  auto gb = [&](int s) { return gb0 + (((s % 2) + 2) % 2); };
+  // the tooth, from its free end
+  int s = m.last_s[1] + 1, slast = s;
+  for (int b = m.last_b[1]; b >= m.first_b[1]; b--)
+    for (int k = 0; k < m.blocks[b].n; k++) {
+      if (s < slast)
+        op(fw, BR_PROP, s, s + 1, 0, seg_of(b));
+      else
+        op(fw, BR_INIT, slast, 0, 0, seg_of(b));
+      s--;
+    }
+  op(fw, BR_COPY, GB1, s + 1, 0, 0);
+  op(fw, BR_PROP, GB0, GB1, 0, -1);
+  const int sfirst = s + 1;
+  // the tail, from the backbone's end
+  s = m.last_s[narm + 2];
+  slast = s;
+  for (int b = m.last_b[narm + 2]; b >= m.first_b[narm + 2]; b--)
+    for (int k = 0; k < m.blocks[b].n; k++) {
+      if (s < slast)
+        op(fw, BR_PROP, s, s + 1, 0, seg_of(b));
+      else
+        op(fw, BR_INIT, slast, 0, 0, seg_of(b));
+      s--;
+    }
+  // the repeats: spacer, then the branch point with its tooth
+  for (int g = narm + 1; g >= 2; g--) {
+    const int b0 = m.first_b[g];
+    for (int b = m.last_b[g]; b >= b0; b--)
+      for (int k = 0; k < m.blocks[b].n; k++) {
+        op(fw, BR_PROP, s, s + 1, 0, seg_of(b));
+        s--;
+      }
+    const int row = g == 2 ? m.last_s[0] + 1 : s;
+    op(fw, BR_PROP, row, s + 1, 0, seg_of(b0 - 1));
+    op(fw, BR_TIMES, row, row, GB0, 0);
+    s--;
+  }
+  // the head
+  s = m.last_s[0];
+  for (int b = m.last_b[0]; b >= m.first_b[0]; b--)
+    for (int k = 0; k < m.blocks[b].n; k++) {
+      op(fw, BR_PROP, s, s + 1, 0, seg_of(b));
+      s--;
+    }
+  if (s != -1) {
+    err = "comb path does not end at the head";
+    return false;
+  }
+  plan.gn_row = 0;
+  op(bw, BR_INIT, gb0, 0, 0, seg_of(0));
+  op(bw, BR_ADDPHI, 0, gb0, mm_of(0), 0);
+  s = -1;
+  for (int b = m.first_b[0]; b <= m.last_b[0]; b++)
+    for (int k = 0; k < m.blocks[b].n; k++) {
+      if (s >= 0) {
+        op(bw, BR_PROP, gb(s + 1), gb(s), 0, seg_of(b));
+        op(bw, BR_ADDPHI, s + 1, gb(s + 1), mm_of(b), 0);
+      }
+      s++;
+    }
+  for (int g = 2; g <= narm + 1; g++) {
+    const int bp = m.first_b[g] - 1;
+    op(bw, BR_PROP, gb(s + 1), gb(s), 0, seg_of(bp));
+    op(bw, BR_ADDPHI, gb(s + 1), s + 1, mm_of(bp), 0);
+    op(bw, BR_TIMES, s + 1, gb(s + 1), s + 1, 0);
+    op(bw, BR_DIV, s + 1, GB0, 0, 0);
+    op(bw, BR_DIV_G1, s + 1, 0, 0, seg_of(bp));
+    op(bw, BR_TIMES, gb(m.first_s[g] - 1), gb(s + 1), GB0, 0);
+    s = m.first_s[g] - 1;
+    for (int b = m.first_b[g]; b <= m.last_b[g]; b++)
+      for (int k = 0; k < m.blocks[b].n; k++) {
+        op(bw, BR_PROP, gb(s + 1), gb(s), 0, seg_of(b));
+        op(bw, BR_ADDPHI, s + 1, gb(s + 1), mm_of(b), 0);
+        s++;
+      }
+  }
+  for (int b = m.first_b[narm + 2]; b <= m.last_b[narm + 2]; b++)
+    for (int k = 0; k < m.blocks[b].n; k++) {
+      op(bw, BR_PROP, gb(s + 1), gb(s), 0, seg_of(b));
+      op(bw, BR_ADDPHI, s + 1, gb(s + 1), mm_of(b), 0);
+      s++;
+    }
+  // the teeth
+  for (int g = 2; g <= narm + 1; g++) {
+    s = sfirst - 1;
+    op(bw, BR_COPY, GB0, g == 2 ? m.last_s[0] + 1 : m.first_s[g] - 1, 0, 0);
+    const int b0 = m.first_b[1];
+    for (int b = b0; b <= m.last_b[1]; b++)
+      for (int k = 0; k < m.blocks[b].n; k++) {
+        if (k == 0 && b == b0) {
+          op(bw, BR_PROP, GB1, GB0, 0, seg_of(b));
+          op(bw, BR_COPY, gb(s + 1), GB1, 0, 0);
+        } else {
+          op(bw, BR_PROP, gb(s + 1), gb(s), 0, seg_of(b));
+        }
+        op(bw, BR_ADDPHI, s + 1, gb(s + 1), mm_of(b), 0);
+        s++;
+      }
+  }
+  plan.rows = GB1 + 1;
+  for (const auto &v : {plan.forward, plan.backward})
+    for (const BranchOp &o : v)
+      if (o.a < 0 || o.a >= plan.rows || o.b < 0 || o.b >= plan.rows || (o.kind == BR_TIMES && o.c >= plan.rows)) {
+        err = "comb schedule reads a row outside the plan";
+        return false;
+      }
+  return true;
+}
+
 bool build_branch_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &err) {
   plan = BranchPlan();
   if (m.dendrimer) return build_dendrimer_plan(m, plan, err);
+  if (m.comb) return build_comb_plan(m, plan, err);
   if (!m.branched) {
     err = "not a branched molecule";
     return false;

@@ -194,6 +194,10 @@ __global__ void __launch_bounds__(kGridThreads) k_branch_op(const DevSys *__rest
       ra[i] = 1.0;
     } else if (kind == BR_TIMES_G1) {
       ra[i] = rb[i] * g1[i];
+    } else if (kind == BR_DIV) {
+      ra[i] = rb[i] != 0 ? ra[i] / rb[i] : 0.0;
+    } else if (kind == BR_DIV_G1) {
+      ra[i] = g1[i] != 0 ? ra[i] / g1[i] : 0.0;
     } else {  // BR_ADDPHI: (f Gf) Gb, f = 1 or a dendrimer block's degeneracy
       hbm(S.rho)[(size_t)c * M + i] += f * ra[i] * rb[i];
     }

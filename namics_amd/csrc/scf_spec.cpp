@@ -728,6 +728,76 @@ bool parse_dendrimer(const std::string &c, const std::vector<std::string> &mons,
   return true;
 }
 
+// A comb, `@comb((A)5; A,(B)3,(A)6,4; (A)7)`: head of the backbone; branch
+// point, tooth, spacer and number of teeth; tail (Molecule::Decomposition
+// src/molecule.cpp:1556-1735).  Generation 0 is the head, 1 the tooth (one
+// copy, degeneracy = number of teeth), then per tooth its branch point and a
+// spacer generation, and last the tail; first_s / last_s count the path
+// (head, tooth, branch point, spacer, ..., tail) from 0.
+bool parse_comb(const std::string &c, const std::vector<std::string> &mons, MoleculeSpec &m, std::string &err) {
+  if (c.size() < 7 || c.compare(0, 6, "@comb(") != 0 || c.back() != ')') {
+    err = "composition '" + c + "': a comb is written @comb((A)5; A,(B)3,(A)6,4; (A)7)";
+    return false;
+  }
+  const std::string s = c.substr(6, c.size() - 7);
+  if (s.find(':') != std::string::npos || s.find('#') != std::string::npos) {
+    err = "composition '" + c + "': aliases are not supported";
+    return false;
+  }
+  auto gens = split(s, ';');
+  if (gens.size() != 3) {
+    err = "composition '" + c + "': a comb has a head, a repeat and a tail separated by ';'";
+    return false;
+  }
+  auto mid = split(gens[1], ',');
+  if (mid.size() != 4) {
+    err = "composition '" + c + "': the repeat of a comb is branch point, tooth, spacer, number of teeth";
+    return false;
+  }
+  const int narm = leading_int(mid[3], 0);
+  if (narm < 1) {
+    err = "composition '" + c + "': the number of teeth must be a positive integer";
+    return false;
+  }
+  auto bp = std::find(mons.begin(), mons.end(), mid[0]);
+  if (bp == mons.end()) {
+    err = "composition '" + c + "': branch point '" + mid[0] + "' is not a defined segment";
+    return false;
+  }
+  m.blocks.clear();
+  m.block_deg.clear();
+  m.n_arm.assign(1, narm);
+  m.first_s.clear();
+  m.last_s.clear();
+  m.first_b.clear();
+  m.last_b.clear();
+  int N = -1;
+  auto generation = [&](const std::string &comp, int deg) {
+    std::vector<BlockSpec> bl;
+    if (!parse_composition(comp, mons, bl, err)) return false;
+    m.first_s.push_back(N + 1);
+    m.first_b.push_back((int)m.blocks.size());
+    for (const BlockSpec &b : bl) {
+      m.blocks.push_back(b);
+      m.block_deg.push_back(deg);
+      N += b.n;
+    }
+    m.last_b.push_back((int)m.blocks.size() - 1);
+    m.last_s.push_back(N);
+    return true;
+  };
+  if (!generation(gens[0], 1) || !generation(mid[1], narm)) return false;
+  for (int a = 0; a < narm; a++) {
+    m.blocks.push_back({(int)(bp - mons.begin()), 1});
+    m.block_deg.push_back(1);
+    N++;
+    if (!generation(mid[2], 1)) return false;
+  }
+  if (!generation(gens[2], 1)) return false;
+  m.comb = true;
+  return true;
+}
+
 bool parse_branched(const std::string &c, const std::vector<std::string> &mons, MoleculeSpec &m,
                     std::string &err) {
   if (c.empty() || c[0] != '(') {
@@ -1133,11 +1203,15 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     }
     s.mol_composition.push_back(v);
     if (!v.empty() && v[0] == '@') {
-      if (v.compare(0, 6, "@dend(") != 0) {
-        err = "composition '" + v + "': of the @ architectures only @dend (symmetric dendrimers) is supported";
+      if (v.compare(0, 6, "@comb(") == 0) {
+        if (!parse_comb(v, p.mons, m, err)) return false;
+      } else if (v.compare(0, 6, "@dend(") == 0) {
+        if (!parse_dendrimer(v, p.mons, m, err)) return false;
+      } else {
+        err = "composition '" + v + "': of the @ architectures only @dend (symmetric dendrimers) and @comb are "
+              "supported";
         return false;
       }
-      if (!parse_dendrimer(v, p.mons, m, err)) return false;
     } else if (v.find('[') != std::string::npos) {
       if (!parse_branched(v, p.mons, m, err)) return false;
     } else if (!parse_composition(v, p.mons, m.blocks, err)) {
@@ -1585,7 +1659,8 @@ std::string describe_json(const SystemSpec &s) {
       for (size_t k = 0; k < m.blocks.size(); k++) o << (k ? "," : "") << m.blocks[k].n;
       o << "],\"block_molmon\":";
       ints(m.block_molmon);
-      o << ",\"dendrimer\":" << (m.dendrimer ? "true" : "false") << ",\"block_deg\":";
+      o << ",\"dendrimer\":" << (m.dendrimer ? "true" : "false") << ",\"comb\":" << (m.comb ? "true" : "false")
+        << ",\"block_deg\":";
       ints(m.block_deg);
       o << ",\"n_arm\":";
       ints(m.n_arm);

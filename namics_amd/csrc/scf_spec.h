@@ -118,8 +118,13 @@ struct MoleculeSpec {
   // and branched molecules have block_deg 1.
   bool dendrimer = false;
   std::vector<int> block_deg, n_arm;
+  // Comb (`@comb((A)5; A,(B)3,(A)6,4; (A)7)`, Molecule::Decomposition
+  // src/molecule.cpp:1556-1735): generation 0 the head of the backbone, 1
+  // the tooth (degeneracy n_arm[0]), 2 .. n_arm[0]+1 the spacers, each after
+  // a one-segment branch point, and n_arm[0]+2 the tail.
+  bool comb = false;
   // propagated by a row schedule (build_branch_plan) instead of a linear chain
-  bool scheduled() const { return branched || dendrimer; }
+  bool scheduled() const { return branched || dendrimer || comb; }
 };
 
 // The propagator schedule of one branched molecule: a list of row
@@ -136,6 +141,8 @@ enum BranchOpKind : int {
   BR_ADDPHI = 4,    // rho[c] += f * R[a] * R[b]   (c: molecule-segment index; f = 1 or a degeneracy)
   BR_ONE = 5,       // R[a] = 1
   BR_TIMES_G1 = 6,  // R[a] = R[b] * G1[seg]
+  BR_DIV = 7,       // R[a] = R[b] != 0 ? R[a] / R[b] : 0
+  BR_DIV_G1 = 8,    // R[a] = G1[seg] != 0 ? R[a] / G1[seg] : 0
 };
 struct BranchOp {
   int kind, a, b, c, seg;

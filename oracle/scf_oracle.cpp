@@ -1301,8 +1301,9 @@ struct Oracle {
     return 1.0 * Nseg / m.chainlength;
   }
 
-  // mol_branched::ComputePhi (src/mol_branched.cpp:303-420) and
-  // mol_dend::BackAndForth (src/mol_dendrimer.cpp:122-188) by the row
+  // mol_branched::ComputePhi (src/mol_branched.cpp:303-420),
+  // mol_dend::BackAndForth (src/mol_dendrimer.cpp:122-188) and
+  // mol_comb::GoBackAndForth (src/mol_comb.cpp:206-400) by the row
   // schedule of the molecule (build_branch_plan: the reference's recursion
   // unrolled into row operations); Markov 1.  UNITY is a row of ones
   // (the reference's UNITY array), Initiate / Terminate are copies.
@@ -1328,6 +1329,8 @@ struct Oracle {
           case BR_TIMES: Times(R[o.a].data(), R[o.b].data(), R[o.c].data(), M); break;
           case BR_ONE: Cp(R[o.a].data(), unity.data(), M); break;
           case BR_TIMES_G1: Times(R[o.a].data(), R[o.b].data(), G1[o.seg].data(), M); break;
+          case BR_DIV: Div(R[o.a].data(), R[o.b].data(), M); break;
+          case BR_DIV_G1: Div(R[o.a].data(), G1[o.seg].data(), M); break;
           default:
             // AddTimes (f = 1) or YplusisCtimesAtimesB (a degeneracy): rho += (f Gf) Gb
             if (o.f == 1.0)

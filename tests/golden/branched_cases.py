@@ -9,8 +9,9 @@ cover: side groups on a branch point inside the backbone, adjacent groups
 on one branch point, a group nested in a side chain, a chain that starts
 with a branch point (the reference then sums GN over the row in front of
 the side group), free and restricted molecules, and 1-, 2- and 3-gradient
-lattices (planar and cylindrical); and symmetric dendrimers (@dend): a
-star and two-generation dendrimers, with a multi-block spacer.
+lattices (planar and cylindrical); symmetric dendrimers (@dend): a
+star and two-generation dendrimers, with a multi-block spacer; and combs
+(@comb) with one- and two-block teeth.
 """
 
 WALL_1D = """
@@ -256,5 +257,65 @@ mol : water : freedom : solvent
 """,
         eval=[(95, 0.2)],
         converged=[dict(method="DIIS", tol=1e-7, seed=96, amp=0.2)],
+    ),
+    # combs (@comb): head, repeats of branch point + tooth + spacer, tail
+    "comb1d_wall": dict(
+        text=WALL_1D + """
+mon : A : freedom : free
+mon : B : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -0.7
+mon : B : chi_W : 0.6
+mol : comb : composition : @comb((A)3;A,(B)3,(A)2,3;(A)4)
+mol : comb : freedom : restricted
+mol : comb : theta : 3.0
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(97, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-7)],
+    ),
+    "comb1d_free": dict(
+        text=WALL_1D + """
+mon : A : freedom : free
+mon : B : freedom : free
+mon : C : freedom : free
+mon : W : freedom : free
+mon : S : chi_B : -0.5
+mon : C : chi_W : 0.8
+mol : comb : composition : @comb((A)1;A,(B)2(C)1,(A)3,2;(A)2)
+mol : comb : freedom : free
+mol : comb : phibulk : 0.004
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(98, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-7)],
+    ),
+    "comb3d_box": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 8
+lat : box : n_layers_y : 8
+lat : box : n_layers_z : 8
+lat : box : lattice_type : simple_cubic
+lat : box : lowerbound_x : periodic
+lat : box : upperbound_x : periodic
+lat : box : lowerbound_y : periodic
+lat : box : upperbound_y : periodic
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : A : freedom : free
+mon : B : freedom : free
+mon : W : freedom : free
+mon : B : chi_W : 1.0
+mol : comb : composition : @comb((A)2;A,(B)2,(A)2,2;(A)2)
+mol : comb : freedom : restricted
+mol : comb : theta : 50
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(99, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-7, seed=100, amp=0.1)],
     ),
 }

@@ -136,3 +136,33 @@ def test_dendrimer_refusals():
     for bad in ("@dend(A,(B)8,4,(B)3,1)", "@dend(A,(B)8)", "@dend(Q,(B)8,4)", "@dend(A,(B)8,0)", "@comb(A,(B)8,4)"):
         with pytest.raises(ScfError):
             describe_input(base.replace("@dend(A,(B)8,4)", bad))
+
+
+def test_comb_parse_by_hand():
+    """@comb((A)3;A,(B)3,(A)2,3;(A)4): head (A)3 (path 0..2), the tooth
+    (B)3 once with degeneracy 3 (path 3..5), then three times branch point A
+    (path 6, 9, 12) and spacer (A)2 (7..8, 10..11, 13..14), and the tail
+    (A)4 (15..18): 3 + 3 x 3 + 3 x (1 + 2) + 4 = 25 segments."""
+    from namics_amd.scf import describe_input
+    text = BRANCHED["comb1d_wall"]["text"]
+    m = [m for m in describe_input(text)["molecules"] if m["name"] == "comb"][0]
+    b = m["branch"]
+    A, B = 1, 2
+    assert m["chainlength"] == 25 and b["comb"]
+    assert b["block_seg"] == [A, B, A, A, A, A, A, A, A] and b["block_n"] == [3, 3, 1, 2, 1, 2, 1, 2, 4]
+    assert b["block_deg"] == [1, 3, 1, 1, 1, 1, 1, 1, 1] and b["n_arm"] == [3]
+    assert b["first_s"] == [0, 3, 7, 10, 13, 15] and b["last_s"] == [2, 5, 8, 11, 14, 18]
+    assert b["gn_row"] == 0
+    # each branch point's forward row becomes a tooth's backward start:
+    # one division by the tooth factor and one by the segment weight per tooth
+    assert sum(1 for o in b["backward"] if o[0] == 7) == 3 and sum(1 for o in b["backward"] if o[0] == 8) == 3
+
+
+def test_comb_refusals():
+    from namics_amd.scf import ScfError, describe_input
+    base = BRANCHED["comb1d_wall"]["text"]
+    comp = "@comb((A)3;A,(B)3,(A)2,3;(A)4)"
+    for bad in ("@comb((A)3;A,(B)3,(A)2;(A)4)", "@comb((A)3;A,(B)3,(A)2,0;(A)4)", "@comb((A)3;(A)4)",
+                "@comb((A)3;Q,(B)3,(A)2,3;(A)4)"):
+        with pytest.raises(ScfError):
+            describe_input(base.replace(comp, bad))
