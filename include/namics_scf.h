@@ -78,11 +78,11 @@ typedef struct {
 /* Parse `.in` text (the reference's input format) at start block `start`
  * (1-based) and allocate the system on the current HIP device.
  * Compositions may be linear, `(A)10(B)10`, branched with square
- * brackets, `(C)16(N)1[(N)1][(N)1](N)1`, symmetric dendrimers,
- * `@dend(A,(B)2,3; A,(B)2,3)`, or combs, `@comb((A)5; A,(B)3,(A)6,4; (A)7)`
- * (the last three: first-order chains without save_memory, run on the grid
- * path); aliases, repeat groups, asymmetric dendrimers and @water are
- * refused with SCF_E_PARSE and a message naming them.
+ * brackets, `(C)16(N)1[(N)1][(N)1](N)1`, dendrimers,
+ * `@dend(A,(B)2,3; A,(B)2,3)` or `@dend(A,(B)6,3,(B)5(C)1,1)`, or combs,
+ * `@comb((A)5; A,(B)3,(A)6,4; (A)7)` (the last three: first-order chains
+ * without save_memory, run on the grid path); aliases, repeat groups and
+ * @water are refused with SCF_E_PARSE and a message naming them.
  * Replaces: Input + Lattice/Segment/Molecule/System/Solve_scf::CheckInput and
  * AllocateMemory (src/namics.cpp:153-417, src/lattice.cpp:267-601), the
  * composition parser Molecule::Decomposition / GenerateTree / Interpret

@@ -19,6 +19,8 @@
 // allocates per call (GS, GX) is a row of its own per recursion depth, so
 // nested calls keep their caller's rows.
 #include <algorithm>
+#include <array>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -271,6 +273,103 @@ bool build_dendrimer_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &
 // walks the backbone, turns each branch point's forward row into the
 // tooth's backward start (Gb Gf / GB / G1) and, last, propagates the tooth
 // once per tooth from those rows.
+// mol_asym_dend::Forward / Backward (src/mol_asym_dendrimer.cpp:127-229),
+// several arm types per generation.  Forward from the outer generation:
+// each arm type's spacer from the next generation's branch point, and the
+// branch point as its weight times every arm's first row stepped once, to
+// the power of that arm type's count.  Backward by recursion: at a branch
+// point of generation g (reached n_repeats times), GX = Gb Gf / G1; each
+// arm type starts from GX divided by its own stepped first row, and the next
+// generation is walked again for every arm type of this one (compositions
+// weighted by n_arm n_repeats).  GX and the stepped rows are per recursion
+// depth, as the reference allocates them per call.
+bool build_asym_dendrimer_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &err) {
+  const int G = (int)m.first_a.size();
+  int maxrow = 0;
+  for (int l : m.last_s) maxrow = std::max(maxrow, l);
+  const int gb0 = maxrow + 1;
+  const int fg0 = maxrow + 3, fg1 = maxrow + 4, fg2 = maxrow + 5;  // forward GS rows
+  int next_row = maxrow + 6;
+  std::vector<std::array<int, 3>> depth_rows;  // per generation: GX, GS, GS+M
+  for (int g = 0; g < G; g++) {
+    depth_rows.push_back({next_row, next_row + 1, next_row + 2});
+    next_row += 3;
+  }
+  std::vector<BranchOp> &fw = plan.forward, &bw = plan.backward;
+  auto op = [](std::vector<BranchOp> &v, int kind, int a, int b, int c, int seg, double f = 1.0) {
+    v.push_back({kind, a, b, c, seg, f});
+  };
+  auto seg_of = [&](int b) { return m.blocks[b].seg; };
+  auto mm_of = [&](int b) { return m.block_molmon[b]; };
+  auto gb = [&](int s) { return gb0 + (((s % 2) + 2) % 2); };
+  int slast = 0, s = 0;
+  for (int g = G - 1; g >= 0; g--) {
+    s = m.last_s[m.last_a[g]];
+    op(fw, BR_ONE, fg2, 0, 0, 0);
+    for (int a = m.last_a[g]; a >= m.first_a[g]; a--) {
+      const int b0 = m.first_b[a], bN = m.last_b[a];
+      for (int b = bN; b >= b0; b--)
+        for (int k = 0; k < m.blocks[b].n; k++) {
+          if (g == G - 1 && s == m.last_s[a])
+            op(fw, BR_INIT, s, 0, 0, seg_of(b));
+          else if (b == bN && k == 0)
+            op(fw, BR_PROP, s, slast, 0, seg_of(b));
+          else
+            op(fw, BR_PROP, s, s + 1, 0, seg_of(b));
+          s--;
+        }
+      op(fw, BR_COPY, fg0, s + 1, 0, 0);
+      op(fw, BR_PROP, fg1, fg0, 0, -1);
+      for (int k = 0; k < m.n_arm[a]; k++) op(fw, BR_TIMES, fg2, fg2, fg1, 0);
+    }
+    op(fw, BR_TIMES_G1, fg2, fg2, 0, seg_of(m.first_b[m.first_a[g]] - 1));
+    if (s < 0) {
+      err = "dendrimer path out of range";
+      return false;
+    }
+    op(fw, BR_COPY, s, fg2, 0, 0);
+    slast = s;
+  }
+  if (s != 0) {
+    err = "dendrimer path does not end at the core";
+    return false;
+  }
+  plan.gn_row = 0;
+  // the backward recursion
+  std::function<void(int, int)> backward = [&](int g, int nrep) {
+    const int GX = depth_rows[g][0], GS0 = depth_rows[g][1], GS1 = depth_rows[g][2];
+    const int bp = m.first_b[m.first_a[g]] - 1;
+    int s = 0;
+    if (g == 0) {
+      op(bw, BR_INIT, gb(0), 0, 0, seg_of(bp));
+    } else {
+      s = m.first_s[m.first_a[g]] - 1;
+      op(bw, BR_PROP, gb(s), gb(s - 1), 0, seg_of(bp));
+    }
+    op(bw, BR_TIMES, GX, gb(s), s, 0);
+    op(bw, BR_ADDPHI, s, gb(s), mm_of(bp), 0, (double)nrep);
+    op(bw, BR_DIV_G1, GX, 0, 0, seg_of(bp));
+    for (int a = m.first_a[g]; a <= m.last_a[g]; a++) {
+      s = m.first_s[a];
+      op(bw, BR_COPY, GS0, s, 0, 0);
+      op(bw, BR_PROP, GS1, GS0, 0, -1);
+      op(bw, BR_COPY, gb(s - 1), GX, 0, 0);
+      op(bw, BR_DIV, gb(s - 1), GS1, 0, 0);
+      for (int b = m.first_b[a]; b <= m.last_b[a]; b++)
+        for (int k = 0; k < m.blocks[b].n; k++) {
+          op(bw, BR_PROP, gb(s), gb(s - 1), 0, seg_of(b));
+          op(bw, BR_ADDPHI, s, gb(s), mm_of(b), 0, (double)m.n_arm[a] * nrep);
+          s++;
+        }
+      op(bw, BR_COPY, gb(s), gb(s - 1), 0, 0);
+      if (g < G - 1) backward(g + 1, m.n_arm[a] * nrep);
+    }
+  };
+  backward(0, 1);
+  plan.rows = next_row;
+  return true;
+}
+
 bool build_comb_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &err) {
   const int narm = m.n_arm.empty() ? 0 : m.n_arm[0];
   const int G = narm + 3;
@@ -397,7 +496,7 @@ bool build_comb_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &err) 
 
 bool build_branch_plan(const MoleculeSpec &m, BranchPlan &plan, std::string &err) {
   plan = BranchPlan();
-  if (m.dendrimer) return build_dendrimer_plan(m, plan, err);
+  if (m.dendrimer) return m.asymmetric ? build_asym_dendrimer_plan(m, plan, err) : build_dendrimer_plan(m, plan, err);
   if (m.comb) return build_comb_plan(m, plan, err);
   if (!m.branched) {
     err = "not a branched molecule";

@@ -679,21 +679,21 @@ bool parse_dendrimer(const std::string &c, const std::vector<std::string> &mons,
   m.blocks.clear();
   m.block_deg.clear();
   m.n_arm.clear();
+  m.first_a.clear();
+  m.last_a.clear();
   m.first_s.clear();
   m.last_s.clear();
   m.first_b.clear();
   m.last_b.clear();
-  int degeneracy = 1, N = -1, chainlength = 0;
+  m.asymmetric = false;
+  int degeneracy = 1, N = -1, arm = -1;
   for (const std::string &gen : split(s, ';')) {
     auto sub = split(gen, ',');
-    if (sub.size() < 2 || (sub.size() - 1) % 2 != 0) {
-      err = "composition '" + c + "': each generation is branch point, spacer, functionality";
+    if (sub.size() < 3 || (sub.size() - 1) % 2 != 0) {
+      err = "composition '" + c + "': each generation is branch point, then spacer and functionality per arm type";
       return false;
     }
-    if (sub.size() > 3) {
-      err = "composition '" + c + "': asymmetric dendrimers are not supported";
-      return false;
-    }
+    if (sub.size() > 3) m.asymmetric = true;
     auto it = std::find(mons.begin(), mons.end(), sub[0]);
     if (it == mons.end()) {
       err = "composition '" + c + "': branch point '" + sub[0] + "' is not a defined segment";
@@ -701,39 +701,37 @@ bool parse_dendrimer(const std::string &c, const std::vector<std::string> &mons,
     }
     m.blocks.push_back({(int)(it - mons.begin()), 1});
     m.block_deg.push_back(degeneracy);
-    chainlength += degeneracy;
     N++;
-    const int f = leading_int(sub[2], 0);
-    if (f < 1) {
-      err = "composition '" + c + "': the functionality of a generation must be a positive integer";
-      return false;
+    m.first_a.push_back(arm + 1);
+    int arms = 0;
+    for (size_t k = 1; k + 1 < sub.size(); k += 2) {
+      arm++;
+      const int f = leading_int(sub[k + 1], 0);
+      if (f < 1) {
+        err = "composition '" + c + "': the number of arms of an arm type must be a positive integer";
+        return false;
+      }
+      std::vector<BlockSpec> spacer;
+      if (!parse_composition(sub[k], mons, spacer, err)) return false;
+      m.n_arm.push_back(f);
+      arms += f;
+      m.first_s.push_back(N + 1);
+      m.first_b.push_back((int)m.blocks.size());
+      for (const BlockSpec &b : spacer) {
+        m.blocks.push_back(b);
+        N += b.n;
+        m.block_deg.push_back(degeneracy * f);
+      }
+      m.last_b.push_back((int)m.blocks.size() - 1);
+      m.last_s.push_back(N);
     }
-    std::vector<BlockSpec> spacer;
-    if (!parse_composition(sub[1], mons, spacer, err)) return false;
-    m.n_arm.push_back(f);
-    m.first_s.push_back(N + 1);
-    m.first_b.push_back((int)m.blocks.size());
-    for (const BlockSpec &b : spacer) {
-      m.blocks.push_back(b);
-      N += b.n;
-      m.block_deg.push_back(degeneracy * f);
-      chainlength += degeneracy * b.n * f;
-    }
-    m.last_b.push_back((int)m.blocks.size() - 1);
-    m.last_s.push_back(N);
-    degeneracy *= f;
+    m.last_a.push_back(arm);
+    degeneracy *= arms;
   }
   m.dendrimer = true;
-  (void)chainlength;  // recomputed from the blocks and their degeneracies by the caller
   return true;
 }
 
-// A comb, `@comb((A)5; A,(B)3,(A)6,4; (A)7)`: head of the backbone; branch
-// point, tooth, spacer and number of teeth; tail (Molecule::Decomposition
-// src/molecule.cpp:1556-1735).  Generation 0 is the head, 1 the tooth (one
-// copy, degeneracy = number of teeth), then per tooth its branch point and a
-// spacer generation, and last the tail; first_s / last_s count the path
-// (head, tooth, branch point, spacer, ..., tail) from 0.
 bool parse_comb(const std::string &c, const std::vector<std::string> &mons, MoleculeSpec &m, std::string &err) {
   if (c.size() < 7 || c.compare(0, 6, "@comb(") != 0 || c.back() != ')') {
     err = "composition '" + c + "': a comb is written @comb((A)5; A,(B)3,(A)6,4; (A)7)";

@@ -109,15 +109,20 @@ struct MoleculeSpec {
   // index of its first segment / one past its last (first_s, last_s).
   bool branched = false;
   std::vector<int> gnr, first_b, last_b, first_s, last_s;
-  // Symmetric dendrimer (`@dend(A,(B)n,f; ...)`, Molecule::Decomposition
-  // src/molecule.cpp:1437-1555): one branch-point block and the spacer
-  // blocks of one arm per generation, along one path from the core; block_deg
+  // Dendrimer (`@dend(A,(B)n,f; ...)`, Molecule::Decomposition
+  // src/molecule.cpp:1437-1555): per generation a branch-point block and the
+  // spacer blocks of each arm type, along one path from the core; block_deg
   // is each block's degeneracy (how many copies the molecule holds), n_arm
-  // the functionality of each generation, and first_b / last_b / first_s /
-  // last_s the spacer blocks and path indices of each generation.  Linear
-  // and branched molecules have block_deg 1.
+  // the number of arms of each arm type, and first_b / last_b / first_s /
+  // last_s the spacer blocks and path indices of each arm type.  Linear and
+  // branched molecules have block_deg 1.
   bool dendrimer = false;
   std::vector<int> block_deg, n_arm;
+  // dendrimers: the arms of each generation (first_a / last_a index the
+  // per-arm n_arm, first_b / last_b, first_s / last_s); a symmetric
+  // dendrimer has one arm type per generation, an asymmetric one several
+  std::vector<int> first_a, last_a;
+  bool asymmetric = false;
   // Comb (`@comb((A)5; A,(B)3,(A)6,4; (A)7)`, Molecule::Decomposition
   // src/molecule.cpp:1556-1735): generation 0 the head of the backbone, 1
   // the tooth (degeneracy n_arm[0]), 2 .. n_arm[0]+1 the spacers, each after

@@ -9,9 +9,9 @@ cover: side groups on a branch point inside the backbone, adjacent groups
 on one branch point, a group nested in a side chain, a chain that starts
 with a branch point (the reference then sums GN over the row in front of
 the side group), free and restricted molecules, and 1-, 2- and 3-gradient
-lattices (planar and cylindrical); symmetric dendrimers (@dend): a
-star and two-generation dendrimers, with a multi-block spacer; and combs
-(@comb) with one- and two-block teeth.
+lattices (planar and cylindrical); symmetric and asymmetric
+dendrimers (@dend): stars and two-generation dendrimers, with multi-block
+spacers; and combs (@comb) with one- and two-block teeth.
 """
 
 WALL_1D = """
@@ -317,5 +317,68 @@ mol : water : freedom : solvent
 """,
         eval=[(99, 0.2)],
         converged=[dict(method="DIIS", tol=1e-7, seed=100, amp=0.1)],
+    ),
+    # asymmetric dendrimers: arm types of different length and count per
+    # generation (a star with one different arm; two generations)
+    "adend1d_star": dict(
+        text=WALL_1D + """
+mon : A : freedom : free
+mon : B : freedom : free
+mon : C : freedom : free
+mon : W : freedom : free
+mon : S : chi_C : -1.0
+mon : B : chi_W : 0.4
+mol : star : composition : @dend(A,(B)6,3,(B)5(C)1,1)
+mol : star : freedom : restricted
+mol : star : theta : 3.0
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(101, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-7)],
+    ),
+    "adend1d_gen2": dict(
+        text=WALL_1D + """
+mon : A : freedom : free
+mon : B : freedom : free
+mon : C : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -0.5
+mon : C : chi_W : 0.7
+mol : dend : composition : @dend(A,(B)2,2,(C)3,1;A,(B)2,2)
+mol : dend : freedom : free
+mol : dend : phibulk : 0.002
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(102, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-7)],
+    ),
+    "adend3d_box": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 8
+lat : box : n_layers_y : 8
+lat : box : n_layers_z : 8
+lat : box : lattice_type : simple_cubic
+lat : box : lowerbound_x : periodic
+lat : box : upperbound_x : periodic
+lat : box : lowerbound_y : periodic
+lat : box : upperbound_y : periodic
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : A : freedom : free
+mon : B : freedom : free
+mon : C : freedom : free
+mon : W : freedom : free
+mon : C : chi_W : 1.0
+mol : dend : composition : @dend(A,(B)2,2,(C)2,2;A,(B)1,2)
+mol : dend : freedom : restricted
+mol : dend : theta : 50
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(103, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-7, seed=104, amp=0.1)],
     ),
 }

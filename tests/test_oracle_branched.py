@@ -133,7 +133,8 @@ def test_dendrimer_parse_by_hand():
 def test_dendrimer_refusals():
     from namics_amd.scf import ScfError, describe_input
     base = BRANCHED["dend1d_star"]["text"]
-    for bad in ("@dend(A,(B)8,4,(B)3,1)", "@dend(A,(B)8)", "@dend(Q,(B)8,4)", "@dend(A,(B)8,0)", "@comb(A,(B)8,4)"):
+    for bad in ("@dend(A,(B)8,4,(B)3)", "@dend(A,(B)8)", "@dend(Q,(B)8,4)", "@dend(A,(B)8,0)", "@comb(A,(B)8,4)",
+                "@water(W)"):
         with pytest.raises(ScfError):
             describe_input(base.replace("@dend(A,(B)8,4)", bad))
 
@@ -166,3 +167,20 @@ def test_comb_refusals():
                 "@comb((A)3;Q,(B)3,(A)2,3;(A)4)"):
         with pytest.raises(ScfError):
             describe_input(base.replace(comp, bad))
+
+
+def test_asymmetric_dendrimer_parse_by_hand():
+    """@dend(A,(B)2,2,(C)3,1;A,(B)2,2): generation 0 has two arm types (two
+    (B)2 arms, one (C)3 arm), generation 1 one ((B)2, twice per branch point).
+    The core counts once; its arms 2 x 2 + 1 x 3; the three branch points of
+    generation 1 (one per arm of generation 0) three times; their arms
+    3 x 2 x 2 = 12: chain length 1 + 7 + 3 + 12 = 23."""
+    from namics_amd.scf import describe_input
+    text = BRANCHED["adend1d_gen2"]["text"]
+    m = [m for m in describe_input(text)["molecules"] if m["name"] == "dend"][0]
+    b = m["branch"]
+    A, B, C = 1, 2, 3
+    assert m["chainlength"] == 23 and b["dendrimer"]
+    assert b["block_seg"] == [A, B, C, A, B] and b["block_n"] == [1, 2, 3, 1, 2]
+    assert b["block_deg"] == [1, 2, 1, 3, 6] and b["n_arm"] == [2, 1, 2]
+    assert b["first_s"] == [1, 3, 7] and b["last_s"] == [2, 5, 8]
