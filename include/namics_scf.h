@@ -82,7 +82,12 @@ typedef struct {
  * `@dend(A,(B)2,3; A,(B)2,3)` or `@dend(A,(B)6,3,(B)5(C)1,1)`, or combs,
  * `@comb((A)5; A,(B)3,(A)6,4; (A)7)` (the last three: first-order chains
  * without save_memory, run on the grid path); aliases, repeat groups and
- * @water are refused with SCF_E_PARSE and a message naming them.
+ * @water are refused with SCF_E_PARSE and a message naming them.  Charged
+ * systems (`mon : <name> : valence`, `epsilon`, `lat : <name> : bondlength`,
+ * a molecule with freedom `neutralizer`) solve for the electrostatic
+ * potential as one more field block on 1-gradient lattices (fused path);
+ * other lattices, charged branched molecules and fixed surface potentials
+ * (`e.psi0/kT`) are refused.
  * Replaces: Input + Lattice/Segment/Molecule/System/Solve_scf::CheckInput and
  * AllocateMemory (src/namics.cpp:153-417, src/lattice.cpp:267-601), the
  * composition parser Molecule::Decomposition / GenerateTree / Interpret
@@ -354,6 +359,15 @@ int scf_get_results(const scf_system *sys, scf_results *out);
  * molecule total densities (n_mol x M), x-major with ghost layers.
  * Replaces: the `pro` output of mon/mol phi (src/output.cpp:385-708). */
 int scf_get_profiles(const scf_system *sys, double *seg_phi, double *mol_phi);
+
+/* Electrostatic profiles of the last residual / solve of a charged system
+ * (M each, x-major with ghost layers; any pointer may be NULL): the
+ * dimensionless potential e psi / kT, the charge density q (valence-weighted
+ * volume fractions over phi_tot) and the relative dielectric constant eps.
+ * SCF_E_ARG when the system carries no charge.
+ * Replaces: the `pro : sys : <name> : psi | q | eps` profiles
+ * (System::PushOutput / GetPointer src/system.cpp:1610-1650). */
+int scf_get_electrostatics(const scf_system *sys, double *psi, double *q, double *eps);
 
 /* Device time of the last fused residual / solve by phase, in ms:
  * [0] Boltzmann weights (Segment::PrepareForCalculations), [1] propagator

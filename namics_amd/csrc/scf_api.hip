@@ -115,6 +115,16 @@ int validate(const SystemSpec &s, std::string &why) {
       if (l.bc[d] == BC_SURFACE) return why = "surface boundaries are supported with 1 gradient only", SCF_E_UNSUPPORTED;
   if (l.gradients == 2 && l.geometry == GEOM_PLANAR && !l.stencil_full && l.lattice_type != LAT_SIMPLE_CUBIC)
     return why = "2-gradient planar hexagonal stencil without stencil_full is not supported", SCF_E_UNSUPPORTED;
+  if (s.charged) {
+    // the potential runs in the one-workgroup 1-gradient kernels
+    if (l.gradients != 1) return why = "charged systems are supported on 1-gradient lattices", SCF_E_UNSUPPORTED;
+    for (auto &m : s.mol)
+      if (m.scheduled())
+        return why = "charged systems with branched molecules are not supported", SCF_E_UNSUPPORTED;
+    if (s.fixed_psi0)
+      return why = "fixed surface potentials (e.psi0/kT) are not supported", SCF_E_UNSUPPORTED;
+    if ((int)s.itmon.size() + 1 > kMaxSeg + 1) return why = "too many field blocks", SCF_E_UNSUPPORTED;
+  }
   return SCF_OK;
 }
 
@@ -133,6 +143,8 @@ int save_memory_interval(const SystemSpec &s) {
 int check_layout(const scf_system *S, int mode, int strip_sx) {
   if (S->branched() && mode == SCF_MODE_FUSED)
     return fail(SCF_E_UNSUPPORTED, "branched molecules and dendrimers run on the grid path");
+  if (S->spec.charged && mode != SCF_MODE_FUSED)
+    return fail(SCF_E_UNSUPPORTED, "charged systems run on the fused (one workgroup per system) path");
   bool markov1 = true;
   for (auto &m : S->spec.mol) markov1 = markov1 && m.markov == 1;
   const bool strip = mode == SCF_MODE_FUSED && strip_sx > 0;
@@ -235,7 +247,7 @@ int build(scf_system *S, const Placement *pl) {
   std::memset(&h, 0, sizeof(DevSys));
   const int M = l.M;
   const int nseg = (int)s.seg.size(), nmol = (int)s.mol.size(), nit = (int)s.itmon.size();
-  const size_t nvar = (size_t)nit * M;
+  const size_t nvar = (size_t)s.nvar();
   h.gradients = l.gradients;
   h.geometry = l.geometry;
   h.lattice_type = l.lattice_type;
@@ -269,8 +281,39 @@ int build(scf_system *S, const Placement *pl) {
   h.wplanar = l.gradients == 1 ? 1.0 / l.fjc : (l.gradients == 2 ? 1.0 / (l.fjc * l.fjc) : 1.0);
   h.lambda = l.lambda;
   h.nseg = nseg;
-  h.nit = nit;
+  h.nit = s.nblocks_x();  // field blocks, plus the potential of a charged system
+  h.nitm = nit;
+  h.charged = s.charged ? 1 : 0;
+  h.neutralizer = s.charged ? s.neutralizer : -1;
+  if (s.charged) {
+    // constants of the reference (src/namics.cpp:43-49); the bond length is
+    // `bondlength` over fjc (src/lattice.cpp:332-337)
+    const double e = 1.60217e-19, T = 298.15, k_B = 1.38065e-23, eps0 = 8.85418e-12;
+    const double k_BT = k_B * T, b = l.bond_length / l.fjc, f = l.fjc;
+    // UpdateEE (src/LG1Planar.cpp:163-168, src/LGrad1.cpp:593-652)
+    double pf = 0.5 * eps0 * b / k_BT * (k_BT / e) * (k_BT / e);
+    // UpdatePsi (src/LG1Planar.cpp:185-188, src/LGrad1.cpp:659-680)
+    double C = e * e / (eps0 * k_BT * b);
+    if (l.geometry == GEOM_PLANAR) {
+      pf = pf / 2.0 * f * f;
+      C = C * 2.0 / f / f;
+      h.el_ri = 0;
+    } else if (l.geometry == GEOM_CYLINDRICAL) {
+      pf = pf * kPIE;
+      C = C / kPIE;
+      h.el_ri = (int)l.offset_first_layer;
+    } else {
+      pf = pf * kPIE * 2 / f;
+      C = C / (2.0 * kPIE) * f;
+      h.el_ri = (int)(l.offset_first_layer + 1.0);
+    }
+    h.el_pf = pf;
+    h.el_C = C;
+    h.el_r0 = l.offset_first_layer;
+  }
   for (int k = 0; k < nseg; k++) {
+    h.seg_val[k] = s.seg[k].valence;
+    h.seg_eps[k] = s.seg[k].epsilon;
     h.seg_block[k] = s.seg[k].field_block;
     h.seg_frozen[k] = s.seg[k].freedom == SEG_FROZEN;
     h.seg_pinned[k] = s.seg[k].freedom == SEG_PINNED;
@@ -309,6 +352,7 @@ int build(scf_system *S, const Placement *pl) {
     for (int k = 0; k < kMaxComp && k < (int)m.P.size(); k++) h.mol_P[i][k] = m.P[k];
     h.mol_phibulk_in[i] = m.phibulk;
     h.mol_n_in[i] = m.n;
+    h.mol_charge[i] = s.mol_charge[i];
     double cf = 0;
     for (int a : m.molmon)
       for (int b : m.molmon) cf -= fraction(m, a) * fraction(m, b) * (s.seg[a].chi[b] / 2);
@@ -360,6 +404,7 @@ int build(scf_system *S, const Placement *pl) {
   size_t o_pt = a.take(M * D);
   size_t o_alpha = a.take(M * D);
   size_t o_mpt = a.take((size_t)nmol * M * D);
+  size_t o_el = s.charged ? a.take((size_t)6 * M * D) : 0;
   S->m_alloc = h.m;
   size_t o_xR = a.take((size_t)h.m * nvar * D);
   size_t o_xx0 = a.take((size_t)h.m * nvar * D);
@@ -432,6 +477,14 @@ int build(scf_system *S, const Placement *pl) {
   h.phitot = P(o_pt);
   h.alpha = P(o_alpha);
   h.mol_phitot = P(o_mpt);
+  if (s.charged) {
+    h.psi = P(o_el);
+    h.EE = P(o_el) + M;
+    h.q = P(o_el) + 2 * M;
+    h.eps = P(o_el) + 3 * M;
+    h.elA = P(o_el) + 4 * M;
+    h.elB = P(o_el) + 5 * M;
+  }
   h.xR = P(o_xR);
   h.xx0 = P(o_xx0);
   h.x0 = P(o_x0);
@@ -947,6 +1000,14 @@ int scf_carry_fields(scf_system *dst, const scf_system *src) {
     else
       HIP_TRY(hipMemsetAsync(d, 0, M * sizeof(double), dst->stream));
   }
+  if (dst->spec.charged) {  // the potential block (Solve_scf::Guess src/solve_scf.cpp:519-521)
+    double *d = dst->h.x + dst->spec.itmon.size() * (size_t)M;
+    if (src->spec.charged)
+      HIP_TRY(hipMemcpyAsync(d, src->h.x + src->spec.itmon.size() * (size_t)M, M * sizeof(double),
+                             hipMemcpyDeviceToDevice, dst->stream));
+    else
+      HIP_TRY(hipMemsetAsync(d, 0, M * sizeof(double), dst->stream));
+  }
   HIP_TRY(hipStreamSynchronize(dst->stream));
   return SCF_OK;
 }
@@ -961,6 +1022,12 @@ static int set_carry_map(scf_system *dst, const scf_system *src) {
     int found = -1;
     for (size_t i = 0; i < src->spec.itmon.size(); i++)
       if (src->spec.seg[src->spec.itmon[i]].name == name) found = (int)i;
+    if (dst->h.carry_src[j] != found) changed = true;
+    dst->h.carry_src[j] = found;
+  }
+  if (dst->spec.charged) {
+    const int j = (int)dst->spec.itmon.size();
+    const int found = src->spec.charged ? (int)src->spec.itmon.size() : -1;
     if (dst->h.carry_src[j] != found) changed = true;
     dst->h.carry_src[j] = found;
   }
@@ -1167,6 +1234,16 @@ int scf_get_profiles(const scf_system *S, double *seg_phi, double *mol_phi) {
   const size_t M = S->h.M;
   if (seg_phi) HIP_TRY(hipMemcpy(seg_phi, S->h.phi, S->h.nseg * M * sizeof(double), hipMemcpyDeviceToHost));
   if (mol_phi) HIP_TRY(hipMemcpy(mol_phi, S->h.mol_phitot, S->h.nmol * M * sizeof(double), hipMemcpyDeviceToHost));
+  return SCF_OK;
+}
+
+int scf_get_electrostatics(const scf_system *S, double *psi, double *q, double *eps) {
+  if (!S) return fail(SCF_E_ARG, "null system");
+  if (!S->spec.charged) return fail(SCF_E_ARG, "the system carries no charge");
+  const size_t bytes = S->h.M * sizeof(double);
+  if (psi) HIP_TRY(hipMemcpy(psi, S->h.psi, bytes, hipMemcpyDeviceToHost));
+  if (q) HIP_TRY(hipMemcpy(q, S->h.q, bytes, hipMemcpyDeviceToHost));
+  if (eps) HIP_TRY(hipMemcpy(eps, S->h.eps, bytes, hipMemcpyDeviceToHost));
   return SCF_OK;
 }
 

@@ -95,6 +95,8 @@ struct DevSys {
   int mm_seg[kMaxMM];
   double mol_P[kMaxMol][kMaxComp];  // Markov-2 bending weights, FJC entries
   double mol_phibulk_in[kMaxMol], mol_n_in[kMaxMol];
+  double mol_charge[kMaxMol];  // net valence per segment (Molecule::Charge)
+  int neutralizer;             // molecule whose bulk fraction neutralizes the bulk, or -1
   double frac[kMaxMol][kMaxSeg];
   double mol_constF[kMaxMol];
   // ---- work arrays (device)
@@ -111,6 +113,14 @@ struct DevSys {
   double *phi_side;   // nseg * M
   double *phitot, *alpha;  // M
   double *mol_phitot; // nmol * M
+  // ---- electrostatics (charged systems, 1 gradient): x and g end with the
+  // potential block; nitm counts the segment field blocks in front of it
+  int charged, nitm, el_ri;  // el_ri: integer radius offset of the field energy (curved)
+  double seg_val[kMaxSeg], seg_eps[kMaxSeg];
+  double el_pf, el_C;        // field-energy prefactor and Poisson constant (geometry factors included)
+  double el_r0;              // offset_first_layer * fjc
+  double *psi, *EE, *q, *eps;  // M each: potential (mirrored ghosts), field energy, charge, dielectric constant
+  double *elA, *elB;           // M each: the potential sweep as X[x] = A[x] X[x-1] + B[x]
   // ---- DIIS state
   double *xR, *xx0, *x0;  // m * nvar, m * nvar, nvar
   int m, itlim, max_g;
@@ -122,7 +132,7 @@ struct DevSys {
   int strip_stride;       // 2-gradient strip path: LDS slots per strip column
   int threads;            // preferred workgroup size for the fused kernel
   int precision;          // 0: FP64 throughout; 1: FP32 propagators, FP64 refinement
-  int carry_src[kMaxSeg]; // job queue: field block of the preceding system for each block (-1: zero)
+  int carry_src[kMaxSeg + 1];  // job queue: field block of the preceding system for each block (-1: zero)
   double mixed_switch;    // residual below which a mixed solve continues in FP64
   double volume;          // weighted interior volume (sum of site weights)
   int ckpt;               // 2-gradient strip path: keep every ckpt-th end-point row (0: all rows)
@@ -802,6 +812,111 @@ __device__ __forceinline__ void m3_step(const Sys &S, Ptr g, const double *P, bo
   }
 }
 
+// ----------------------------------------------------- electrostatics
+// Charged systems on 1-gradient lattices.  The potential psi is the last
+// block of x; its ghosts are mirrored whatever the boundary conditions
+// (LGrad1::set_M_bounds src/LGrad1.cpp:820-832).
+template <typename Sys>
+__device__ __forceinline__ int el_mirror(const Sys &S, int x) {
+  const int f = S.fjc;
+  if (x < f) return 2 * f - x - 1;
+  if (x >= S.MX + f) return 2 * (S.MX + f) - x - 1;
+  return x;
+}
+
+template <typename Sys>
+__device__ __forceinline__ double el_psi(const Sys &S, int x) {
+  return hbm(S.x)[S.nitm * S.M + el_mirror(S, x)];
+}
+
+// Field energy density at site x, zero on ghosts (LG1Planar::UpdateEE
+// src/LG1Planar.cpp:162-178; LGrad1::UpdateEE src/LGrad1.cpp:592-651, where
+// the cylindrical and spherical radii are integers).
+template <typename Sys>
+__device__ __forceinline__ double el_EE(const Sys &S, int x) {
+  const int f = S.fjc;
+  if (x < f || x >= S.MX + f) return 0.0;
+  const double pm = el_psi(S, x - 1), p0 = el_psi(S, x), pp = el_psi(S, x + 1);
+  if (S.geometry == 0) {
+    double em = pm - p0;
+    em *= em;
+    double ep = p0 - pp;
+    ep *= ep;
+    return S.el_pf * (em + ep);
+  }
+  if (S.geometry == 1) {
+    const int r = S.el_ri + (x - f + 1);
+    double em = p0 - pm;
+    em *= (r - 1) * em;
+    double ep = p0 - pp;
+    ep *= r * ep;
+    return S.el_pf * (em + ep) / hbm(S.L)[x];
+  }
+  const int r = S.el_ri + (x - f);
+  double ep = r * (p0 - pp);
+  ep *= ep;
+  if (x == f) return S.el_pf * ep / hbm(S.L)[x];
+  double em = (r - 1) * (pm - p0);
+  em *= em;
+  return S.el_pf * (em + ep) / hbm(S.L)[x];
+}
+
+// Charge density and dielectric constant at site i (all sites, ghosts
+// included) from the segment densities with their ghosts bounded, divided
+// by phitot (zero where phitot is zero); also stores psi and EE for the
+// thermodynamics (System::DoElectrostatics src/system.cpp:1920-1953; the
+// residual sets psi[0] = psi[1], src/system.cpp:2164).
+template <typename Sys>
+__device__ __forceinline__ void site_electro(const Sys &S, int i) {
+  const int M = S.M;
+  const int b = map_x1(S, i);
+  double q = 0, e = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxSeg; k++) {
+    if (k >= S.nseg) break;
+    const double p = hbm(S.phi)[k * M + b];
+    q = q + S.seg_val[k] * p;
+    e = e + S.seg_eps[k] * p;
+  }
+  const double pt = hbm(S.phitot)[i];
+  hbm(S.q)[i] = pt != 0 ? q / pt : 0.0;
+  hbm(S.eps)[i] = pt != 0 ? e / pt : 0.0;
+  hbm(S.psi)[i] = el_psi(S, i == 0 ? 1 : i);
+  hbm(S.EE)[i] = el_EE(S, i);
+}
+
+// One site of the potential sweep (LG1Planar::UpdatePsi src/LG1Planar.cpp:181-199,
+// LGrad1::UpdatePsi src/LGrad1.cpp:654-686) at interior x.  Planar and
+// cylindrical sweeps are upwind, X[x] = (epsXmin X[x-1] + C q + epsXplus
+// psi[x+1]) / (epsXmin + epsXplus) starting from psi at the lower ghost:
+// stored as the affine map X[x] = A X[x-1] + B.  The spherical sweep reads
+// psi on both sides and returns X directly.
+template <typename Sys>
+__device__ __forceinline__ double site_psi_sweep(const Sys &S, int x) {
+  const int f = S.fjc;
+  const double Em = hbm(S.eps)[x - 1], E0 = hbm(S.eps)[x], Ep = hbm(S.eps)[x + 1];
+  const double q = hbm(S.q)[x];
+  const double pp = el_psi(S, x + 1);
+  double wm = 1.0, wp = 1.0;  // radial weights of the two bonds
+  if (S.geometry == 1) {
+    wm = S.el_r0 + (x - f);
+    wp = S.el_r0 + (x - f + 1);
+  } else if (S.geometry == 2) {
+    const double rm = S.el_r0 + (x - f), rp = S.el_r0 + (x - f + 1);
+    wm = rm * rm;
+    wp = rp * rp;
+  }
+  const double epsXmin = wm * (Em + E0), epsXplus = wp * (E0 + Ep);
+  const double den = epsXmin + epsXplus;
+  if (S.geometry == 2) {
+    const double pm = el_psi(S, x - 1);
+    return (epsXmin * pm + S.el_C * q * hbm(S.L)[x] + epsXplus * pp) / den;
+  }
+  hbm(S.elA)[x] = epsXmin / den;
+  hbm(S.elB)[x] = (S.el_C * q + epsXplus * pp) / den;
+  return 0.0;
+}
+
 // ----------------------------------------------------- per-site phases
 // Boltzmann weights of every segment at site i (Segment::PrepareForCalculations
 // src/segment.cpp:835-879): u is the segment's field block with its ghosts
@@ -817,9 +932,20 @@ __device__ __forceinline__ void site_prepare(const Sys &S, int i, int x, int y, 
     b = map_x(S, x) * S.JX + map_y(S, y) * S.JY + map_z(S, z);
   }
   const int M = S.M;
+  // charged systems: u also carries -epsilon EE + valence psi at the source
+  // site (System::PutU src/system.cpp:2001-2040)
+  double EEb = 0.0, psib = 0.0;
+  if (S.charged) {
+    EEb = el_EE(S, b);
+    psib = el_psi(S, b);
+  }
   for (int k = 0; k < S.nseg; k++) {
     const int blk = S.seg_block[k];
-    const double u = blk >= 0 ? hbm(S.x)[blk * M + b] : 0.0;
+    double u = blk >= 0 ? hbm(S.x)[blk * M + b] : 0.0;
+    if (S.charged && blk >= 0) {
+      u = u + (-1.0 * S.seg_eps[k]) * EEb;
+      if (S.seg_val[k] != 0) u = u + S.seg_val[k] * psib;
+    }
     double w = exp(-u);
     if (S.seg_pinned[k]) w = w * hbm(S.seg_mask)[k * M + i];
     w = w * hbm(S.ksam)[i];
@@ -881,7 +1007,7 @@ __device__ __forceinline__ double site_residual(const Sys &S, int i, int x, int 
     if (k < S.nseg) side[k] = stencil_side(S, hbm(S.phi) + k * M, i, x, y, z);
   }
 #pragma unroll
-  for (int it = 0; it < kMaxSeg; it++) xin[it] = it < S.nit ? hbm(S.x)[it * M + i] : 0.0;
+  for (int it = 0; it < kMaxSeg; it++) xin[it] = it < S.nitm ? hbm(S.x)[it * M + i] : 0.0;
   const double pt = hbm(S.phitot)[i];
   const double ks = hbm(S.ksam)[i];
 #pragma unroll
@@ -892,7 +1018,7 @@ __device__ __forceinline__ double site_residual(const Sys &S, int i, int x, int 
 #pragma unroll
   for (int it = 0; it < kMaxSeg; it++) {
     gv[it] = 0.0;
-    if (it < S.nit) {
+    if (it < S.nitm) {
       const int si = S.itmon[it];
       double gi = xin[it];
 #pragma unroll
@@ -906,12 +1032,12 @@ __device__ __forceinline__ double site_residual(const Sys &S, int i, int x, int 
       alpha += gi;
     }
   }
-  alpha = alpha * (1.0 / S.nit);
+  alpha = alpha * (1.0 / S.nitm);
   hbm(S.alpha)[i] = alpha;
   double g2 = 0;
 #pragma unroll
   for (int it = 0; it < kMaxSeg; it++) {
-    if (it < S.nit) {
+    if (it < S.nitm) {
       double gi = pt > 0 ? gv[it] - alpha + 1 / pt - 1.0 : 0.0;
       gi = gi * ks;
       hbm(S.g)[it * M + i] = gi;
@@ -952,6 +1078,12 @@ __device__ __forceinline__ void site_thermo(const Sys &S, int i, double logc[kMa
     if (mo < S.nmol) f += S.mol_constF[mo] * hbm(S.mol_phitot)[mo * M + i];
   const double ks = hbm(S.ksam)[i];
   F = f * ks;
+  double qpsi = 0.0;
+  if (S.charged) {
+    // q psi / 2, frozen sites included (System::GetFreeEnergy src/system.cpp:3019-3027)
+    qpsi = hbm(S.q)[i] * hbm(S.psi)[i];
+    F = F + qpsi * 0.5;
+  }
   double gp = 0;
   for (int mo = 0; mo < S.nmol; mo++)
     gp += (hbm(S.mol_phitot)[mo * M + i] - sc.phibulk[mo]) * (1.0 / S.mol_N[mo]);
@@ -965,7 +1097,19 @@ __device__ __forceinline__ void site_thermo(const Sys &S, int i, double logc[kMa
       if (chi != 0) gp += (pj * hbm(S.phi_side)[k * M + i] - pbA * sc.seg_phibulk[k]) * chi;
     }
   }
-  GP = -gp * ks;
+  if (S.charged) {
+    // eps EE - q psi / 2 on accessible sites, q psi / 2 on the others
+    // (System::GetGrandPotential src/system.cpp:3223-3238)
+    const double q = hbm(S.q)[i];
+    double t = hbm(S.EE)[i] * hbm(S.eps)[i];
+    t = t * -2.0;
+    t = t + qpsi;
+    t = t * (-1.0 / 2.0);
+    GP = (-gp + t) * ks;
+    GP = GP + (q - q * ks) * hbm(S.psi)[i] * 0.5;
+  } else {
+    GP = -gp * ks;
+  }
 }
 
 // site weight of sums over the lattice (Lattice::WeightedSum): 1 / fjc on
@@ -981,7 +1125,7 @@ __device__ __forceinline__ double site_weight(const Sys &S, int i) {
 // src/system.cpp:2366-2558).  Single thread.
 __device__ inline void scalars_norms(const DevSys &S) {
   DevScalars &sc = *S.sc;
-  double B = 0;
+  double A = 0, B = 0;  // bulk charge and volume fraction of free and restricted molecules
   for (int i = 0; i < S.nmol; i++) {
     // A partition function that overflowed (or came out NaN) ends the
     // evaluation as GN <= 0 does.  The reference carries on: its densities
@@ -997,6 +1141,7 @@ __device__ inline void scalars_norms(const DevSys &S) {
     if (fr == 0) {  // free
       nrm = sc.phibulk[i] / S.mol_N[i];
       sc.n[i] = nrm * sc.GN[i];
+      A += sc.phibulk[i] * S.mol_charge[i];
       B += sc.phibulk[i];
     }
     if (fr == 1) {  // restricted
@@ -1006,6 +1151,7 @@ __device__ inline void scalars_norms(const DevSys &S) {
           sc.phibulk[i] = 0;
         } else {
           sc.phibulk[i] = S.mol_N[i] * nrm;
+          A += sc.phibulk[i] * S.mol_charge[i];
           B += sc.phibulk[i];
         }
       } else {
@@ -1023,6 +1169,16 @@ __device__ inline void scalars_norms(const DevSys &S) {
       sc.phibulk[i] = 0;
     }
     sc.norm[i] = nrm;
+  }
+  // the neutralizer's bulk fraction makes the bulk neutral
+  // (System::ComputePhis src/system.cpp:2487-2516)
+  if (S.charged && S.neutralizer >= 0) {
+    const int nz = S.neutralizer, sv = S.solvent;
+    sc.phibulk[nz] = ((B - 1.0) * S.mol_charge[sv] - A) / (S.mol_charge[nz] - S.mol_charge[sv]);
+    B += sc.phibulk[nz];
+    const double nrm = sc.phibulk[nz] / S.mol_N[nz];
+    sc.n[nz] = nrm * sc.GN[nz];
+    sc.norm[nz] = nrm;
   }
   if (S.solvent >= 0) {
     const int sv = S.solvent;

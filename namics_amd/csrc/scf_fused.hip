@@ -1202,6 +1202,71 @@ __device__ __noinline__ void chain_strip_dispatch(const DevSys &S, int mo, bool 
     chain_strip_select<double>(S, mo, 0.0);
 }
 
+// ------------------------------------------------------ electrostatics
+// The potential block of the residual of a charged 1-gradient system
+// (System::Classical_residual src/system.cpp:2161-2169): charge density,
+// dielectric constant, psi and EE at every site, then one sweep of the
+// Poisson equation X and g = psi - X on the interior (zero on ghosts).
+// The upwind sweep of planar and cylindrical lattices is a first-order
+// linear recurrence; the first wave runs it as a scan of affine maps (each
+// lane composes a run of consecutive sites, the lanes combine in six
+// shuffle steps, then each lane replays its run from its entry value).
+// acc is the thread's running sum of g^2 (or max |g|); returns it updated.
+__device__ __noinline__ double electro_residual(const DevSys &S0, double acc) {
+  CSys &S = uniform_sys(S0);
+  const int M = S.M, f = S.fjc, MX = S.MX;
+  const int tid = work_tid(S), nt = S.threads;
+  const bool max_g = S.max_g;
+  gdbl *gp = hbm(S.g) + S.nitm * M;
+  const gdbl *xp = hbm(S.x) + S.nitm * M;
+  for (int i = tid; i < M; i += nt) {
+    site_electro(S, i);
+    if (i < f || i >= MX + f) gp[i] = 0.0;
+  }
+  __syncthreads();
+  const bool upwind = S.geometry != 2;
+  for (int x = f + tid; x < MX + f; x += nt) {
+    const double X = site_psi_sweep(S, x);
+    if (!upwind) {
+      const double gv = xp[x] - X;
+      gp[x] = gv;
+      acc = max_g ? fmax(acc, fabs(gv)) : acc + gv * gv;
+    }
+  }
+  if (!upwind) return acc;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    const int chunk = (MX + 63) / 64;
+    const int lo = f + lane * chunk, hi = min(MX + f, lo + chunk);
+    const gdbl *A = hbm(S.elA), *B = hbm(S.elB);
+    double a = 1.0, b = 0.0;  // this lane's run as X_out = a X_in + b
+    for (int x = lo; x < hi; x++) {
+      const double Ax = A[x];
+      b = Ax * b + B[x];
+      a = Ax * a;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double ap = __shfl_up(a, off), bp = __shfl_up(b, off);
+      if (lane >= off) {
+        b = a * bp + b;
+        a = a * ap;
+      }
+    }
+    const double ain = __shfl_up(a, 1), bin = __shfl_up(b, 1);
+    const double X0 = el_psi(S, f - 1);
+    double X = lane == 0 ? X0 : ain * X0 + bin;
+    for (int x = lo; x < hi; x++) {
+      X = A[x] * X + B[x];
+      const double gv = xp[x] - X;
+      gp[x] = gv;
+      acc = max_g ? fmax(acc, fabs(gv)) : acc + gv * gv;
+    }
+  }
+  return acc;
+}
+
 // ------------------------------------------------------------- residual
 // One Classical_residual: fields x -> g, densities, norms.  Returns ||g||
 // (or max|g|) to every thread, or a negative value on a status error.
@@ -1282,11 +1347,12 @@ __device__ double fused_residual(const DevSys &S0) {
     if (max_g) {
       const double g2 = site_residual(S, i, x, y, z);
       (void)g2;
-      for (int it = 0; it < S.nit; it++) acc = fmax(acc, fabs(g[it * M + i]));
+      for (int it = 0; it < S.nitm; it++) acc = fmax(acc, fabs(g[it * M + i]));
     } else {
       acc += site_residual(S, i, x, y, z);
     }
   }
+  if (S.charged) acc = electro_residual(S0, acc);
   const double r = max_g ? wg_max(acc) : sqrt(wg_sum(acc));
   if (threadIdx.x == 0) {
     sc.residual = r;

@@ -128,6 +128,7 @@ Value vstr(const std::string &v) {
 struct State {
   scf_results r;
   std::vector<double> seg_phi, mol_phi;  // [n][M]
+  std::vector<double> psi, q, eps;       // [M] each, charged systems
 };
 
 int load_state(const scf_system *S, State &st) {
@@ -136,7 +137,21 @@ int load_state(const scf_system *S, State &st) {
   const size_t M = S->h.M;
   st.seg_phi.assign(S->h.nseg * M, 0.0);
   st.mol_phi.assign(S->h.nmol * M, 0.0);
-  return scf_get_profiles(S, st.seg_phi.data(), st.mol_phi.data());
+  rc = scf_get_profiles(S, st.seg_phi.data(), st.mol_phi.data());
+  if (rc != SCF_OK || !S->spec.charged) return rc;
+  st.psi.assign(M, 0.0);
+  st.q.assign(M, 0.0);
+  st.eps.assign(M, 0.0);
+  return scf_get_electrostatics(S, st.psi.data(), st.q.data(), st.eps.data());
+}
+
+// a system profile of a charged system (System::GetPointer src/system.cpp:1632-1650)
+const double *sys_profile(const SystemSpec &s, const State &st, const OutputItem &it) {
+  if (it.key != "sys" || it.name != s.sys_name || !s.charged) return nullptr;
+  if (it.prop == "psi") return st.psi.data();
+  if (it.prop == "q") return st.q.data();
+  if (it.prop == "eps") return st.eps.data();
+  return nullptr;
 }
 
 // One kal quantity (the scalars the reference's objects push for output,
@@ -150,6 +165,7 @@ Value kal_value(const scf_system *S, const State &st, const scf_solve_info *info
     if (p == "free_energy") return vreal(st.r.free_energy);
     if (p == "grand_potential") return vreal(st.r.grand_potential);
     if (p == "start") return vint(s.start);
+    if (p == "Dpsi" && s.charged) return vreal(st.psi[l.M - 1] - st.psi[0]);
   } else if (it.key == "newton" && it.name == s.newton_name) {
     if (p == "iterations" && info) return vint(info->iterations);
     if (p == "residual") return vreal(info ? info->residual : st.r.residual);
@@ -383,6 +399,7 @@ int scf_write_pro(scf_system *S, const char *path) {
     if (it.prop == "phi" && it.key == "mol")
       for (size_t i = 0; i < s.mol.size(); i++)
         if (s.mol[i].name == it.name) src = st.mol_phi.data() + i * (size_t)l.M;
+    if (!src) src = sys_profile(s, st, it);
     if (!src) continue;  // not a profile this system knows: left out, as the reference does
     std::vector<double> X(src, src + l.M);
     if (req.write_bounds) set_ghosts(l, X);
@@ -489,7 +506,7 @@ int scf_store_guess(scf_system *S, const char *path) {
   if (!fp) return fail(SCF_E_ARG, std::string("cannot open ") + path);
   std::fprintf(fp, "%s\n", s.method_name.c_str());
   std::fprintf(fp, " %i\t%i\t%i\t%i\n", l.MX, l.gradients >= 2 ? l.MY : 0, l.gradients == 3 ? l.MZ : 0, l.fjc);
-  std::fprintf(fp, "%s\n", "false");
+  std::fprintf(fp, "%s\n", s.charged ? "true" : "false");
   std::fprintf(fp, "%i\n", (int)s.itmon.size());
   for (int k : s.itmon) std::fprintf(fp, "%s\n", s.seg[k].name.c_str());
   std::fprintf(fp, "%i\n", 0);

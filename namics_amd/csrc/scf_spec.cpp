@@ -1107,9 +1107,37 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     }
     s.seg_freedom_name.push_back(v);
     {
+      // valence and epsilon keep an out-of-range value, as the reference
+      // does after its warning (Segment::CheckInput src/segment.cpp:1056-1067)
       std::string vv;
       double val = 0;
-      s.seg_valence.push_back(get(p, "mon", sg.name, "valence", vv) && to_real(vv, val) ? val : 0.0);
+      sg.valence = get(p, "mon", sg.name, "valence", vv) && to_real(vv, val) ? val : 0.0;
+      s.seg_valence.push_back(sg.valence);
+      sg.epsilon = 80.0;
+      if (get(p, "mon", sg.name, "epsilon", vv) && !to_real(vv, sg.epsilon)) sg.epsilon = 80.0;
+      if (sg.valence != 0) {
+        const double b = l.bond_length / l.fjc;
+        if (b < 1e-12 || b > 1e-8) {
+          err = "mon '" + sg.name + "' is charged: set lat bondlength in 1e-12..1e-8 m";
+          return false;
+        }
+      }
+      if (get(p, "mon", sg.name, "e.psi0/kT", vv)) {
+        sg.fixed_psi0 = true;
+        if (!to_real(vv, sg.psi0)) sg.psi0 = 0.0;
+        if (sg.psi0 != 0 && sg.valence != 0) {
+          err = "mon '" + sg.name + "': set only one of 'valence' and 'e.psi0/kT'";
+          return false;
+        }
+        if (sg.psi0 != 0 && v != "frozen") {
+          err = "mon '" + sg.name + "': a potential 'e.psi0/kT' needs freedom 'frozen'";
+          return false;
+        }
+        if (sg.psi0 < -25 || sg.psi0 > 25) {
+          err = "mon '" + sg.name + "': 'e.psi0/kT' out of range -25..25";
+          return false;
+        }
+      }
     }
     if (v == "free")
       sg.freedom = SEG_FREE;
@@ -1133,6 +1161,11 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       for (int a = 0; a < 4; a++) {
         if (v == std::string("lowerbound") + axes[a]) frozen_at_bound[a == 0 ? 0 : a - 1] = true;
         if (v == std::string("upperbound") + axes[a]) frozen_at_bound[a == 0 ? 3 : a + 2] = true;
+        if (sg.valence != 0 && (v == std::string("lowerbound") + axes[a] || v == std::string("upperbound") + axes[a])) {
+          // System::CheckInput src/system.cpp:1193-1200
+          err = "mon '" + sg.name + "': a charged frozen segment may not sit in a boundary";
+          return false;
+        }
       }
     }
     if (sg.freedom == SEG_PINNED) {
@@ -1244,8 +1277,10 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       m.freedom = MOL_RESTRICTED;
     else if (v == "solvent")
       m.freedom = MOL_SOLVENT;
+    else if (v == "neutralizer")
+      m.freedom = MOL_NEUTRALIZER;
     else {
-      err = "mol '" + m.name + "': freedom '" + v + "' is not supported (free, restricted, solvent)";
+      err = "mol '" + m.name + "': freedom '" + v + "' is not supported (free, restricted, solvent, neutralizer)";
       return false;
     }
     if (m.pinned && m.freedom != MOL_RESTRICTED) {
@@ -1347,6 +1382,64 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     if (!(nmol == 1 && s.mol[0].pinned)) {
       err = "no solvent molecule (freedom : solvent) found";
       return false;
+    }
+  }
+  // Electrostatics (System::CheckInput src/system.cpp:701-729,
+  // Molecule::Charge / IsCharged src/molecule.cpp:1866-1898): a molecule is
+  // charged when its valences do not sum to zero; one charged molecule turns
+  // the potential on for the whole system.
+  s.mol_charge.assign(nmol, 0.0);
+  s.charged = false;
+  s.neutralizer = -1;
+  for (int i = 0; i < nmol; i++) {
+    const MoleculeSpec &m = s.mol[i];
+    double q = 0;
+    for (const BlockSpec &b : m.blocks) q += s.seg[b.seg].valence * b.n;
+    s.mol_charge[i] = q / m.chainlength;
+    if (q != 0) {
+      if (m.dendrimer || m.comb) {
+        err = "mol '" + m.name + "': charged dendrimers and combs are not supported";
+        return false;
+      }
+      s.charged = true;
+    }
+    if (m.freedom == MOL_NEUTRALIZER) {
+      if (s.neutralizer >= 0) {
+        err = "more than one neutralizer molecule";
+        return false;
+      }
+      s.neutralizer = i;
+    }
+  }
+  if (s.neutralizer >= 0 && !s.charged) {
+    err = "mol '" + s.mol[s.neutralizer].name + "': a neutralizer needs charged molecules";
+    return false;
+  }
+  if (s.charged) {
+    if (l.gradients != 1) {
+      err = "charged systems are supported on 1-gradient lattices";
+      return false;
+    }
+    if (s.neutralizer < 0) {
+      double qb = 0;
+      bool needed = false;
+      for (int i = 0; i < nmol; i++) {
+        const MoleculeSpec &m = s.mol[i];
+        if (m.freedom == MOL_FREE) qb += s.mol_charge[i] * m.phibulk;
+        if (m.freedom == MOL_RESTRICTED && s.mol_charge[i] != 0 && !m.pinned) needed = true;
+      }
+      if (needed || std::fabs(qb) > 1e-4) {
+        err = "a neutralizer is needed: un-pinned restricted molecules are charged or the free molecules' bulk "
+              "is not neutral";
+        return false;
+      }
+    } else if (s.solvent >= 0 && s.mol_charge[s.neutralizer] == s.mol_charge[s.solvent]) {
+      err = "the neutralizer carries the solvent's charge";
+      return false;
+    }
+    for (int i = 0; i < nseg; i++) {
+      if (s.seg[i].epsilon != s.seg[0].epsilon) s.grad_epsilon = true;
+      if (s.seg[i].freedom == SEG_FROZEN && s.seg[i].fixed_psi0) s.fixed_psi0 = true;
     }
   }
   compute_lambdas(l, any_markov2);
@@ -1628,12 +1721,17 @@ std::string describe_json(const SystemSpec &s) {
   o << ",\"gradients\":" << l.gradients << ",\"geometry\":" << l.geometry << ",\"fjc\":" << l.fjc << ",\"MX\":" << l.MX
     << ",\"MY\":" << l.MY << ",\"MZ\":" << l.MZ << ",\"M\":" << l.M << ",\"JX\":" << l.JX
     << ",\"JY\":" << l.JY << ",\"nvar\":" << s.nvar() << ",\"solvent\":" << s.solvent;
+  o << ",\"charged\":" << (s.charged ? "true" : "false") << ",\"neutralizer\":" << s.neutralizer
+    << ",\"grad_epsilon\":" << (s.grad_epsilon ? "true" : "false")
+    << ",\"fixed_psi0\":" << (s.fixed_psi0 ? "true" : "false") << ",\"psi_block\":"
+    << (s.charged ? (int)s.itmon.size() : -1);
   o << ",\"segments\":[";
   for (size_t i = 0; i < s.seg.size(); i++) {
     if (i) o << ",";
     o << "{\"name\":";
     str(s.seg[i].name);
-    o << ",\"freedom\":" << s.seg[i].freedom << ",\"field_block\":" << s.seg[i].field_block << "}";
+    o << ",\"freedom\":" << s.seg[i].freedom << ",\"field_block\":" << s.seg[i].field_block
+      << ",\"valence\":" << s.seg[i].valence << ",\"epsilon\":" << s.seg[i].epsilon << "}";
   }
   o << "],\"molecules\":[";
   for (size_t i = 0; i < s.mol.size(); i++) {
@@ -1748,6 +1846,9 @@ bool read_guess_text(const std::string &text, const SystemSpec &s, std::vector<d
     for (size_t j = 0; j < s.itmon.size(); j++)
       if (names[i] == s.seg[s.itmon[j]].name)
         std::copy(vals.begin() + i * m, vals.begin() + (i + 1) * m, x.begin() + j * l.M);
+  // the potential block follows the field blocks when both are charged
+  if (charged == "true" && s.charged)
+    std::copy(vals.begin() + (n1 + n2) * m, vals.begin() + (n1 + n2 + 1) * m, x.begin() + s.itmon.size() * l.M);
   return true;
 }
 

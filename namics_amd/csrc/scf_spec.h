@@ -30,7 +30,9 @@ enum Geometry : int { GEOM_PLANAR = 0, GEOM_CYLINDRICAL = 1, GEOM_SPHERICAL = 2 
 enum BoundaryKind : int { BC_MIRROR = 0, BC_PERIODIC = 1, BC_SURFACE = 2 };
 enum LatticeType : int { LAT_SIMPLE_CUBIC = 0, LAT_HEXAGONAL = 1 };
 enum SegFreedom : int { SEG_FREE = 0, SEG_FROZEN = 1, SEG_PINNED = 2 };
-enum MolFreedom : int { MOL_FREE = 0, MOL_RESTRICTED = 1, MOL_SOLVENT = 2 };
+// MOL_NEUTRALIZER: its bulk volume fraction follows from electroneutrality
+// of the bulk at every evaluation (System::ComputePhis src/system.cpp:2487-2516).
+enum MolFreedom : int { MOL_FREE = 0, MOL_RESTRICTED = 1, MOL_SOLVENT = 2, MOL_NEUTRALIZER = 3 };
 enum SolverMethod : int {
   SOLVER_DIIS = 0,
   SOLVER_PSEUDOHESSIAN = 1,
@@ -80,6 +82,12 @@ struct SegmentSpec {
   std::vector<int> mask;       // M entries, 1 on frozen / pinned sites
   std::vector<double> chi;     // n_seg entries, symmetric chi row
   int field_block = -1;        // which block of x holds this segment's field
+  // Electrostatics (Segment::CheckInput src/segment.cpp:1056-1091): charge
+  // per segment in units of e, relative dielectric constant, and a fixed
+  // dimensionless surface potential e psi0 / kT on a frozen segment.
+  double valence = 0.0, epsilon = 80.0;
+  bool fixed_psi0 = false;
+  double psi0 = 0.0;
 };
 
 struct BlockSpec {
@@ -255,7 +263,16 @@ struct SystemSpec {
   std::string lattice_type_name, method_name = "pseudohessian", stop_criterion_name = "norm_of_g";
   std::vector<std::string> mol_composition, seg_freedom_name, mol_freedom_name;
   std::vector<double> seg_valence;
-  int nvar() const { return (int)itmon.size() * lat.M; }
+  // Electrostatics (System::CheckInput src/system.cpp:701-729): on when a
+  // molecule carries a net charge; then x ends with one more block of M, the
+  // dimensionless potential psi, and g with its Poisson residual.
+  bool charged = false;
+  int neutralizer = -1;        // the molecule with freedom neutralizer, or -1
+  bool grad_epsilon = false;   // segments differ in epsilon
+  bool fixed_psi0 = false;     // a frozen segment fixes the potential
+  std::vector<double> mol_charge;  // Molecule::Charge: net valence per segment
+  int nblocks_x() const { return (int)itmon.size() + (charged ? 1 : 0); }
+  int nvar() const { return nblocks_x() * lat.M; }
   int max_chain() const {
     int n = 0;
     for (auto &m : mol) n = m.chainlength > n ? m.chainlength : n;

@@ -104,6 +104,7 @@ def lib():
         L.scf_solve_resident.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(SolveInfo)]
         L.scf_get_results.argtypes = [vp, ctypes.POINTER(Results)]
         L.scf_get_profiles.argtypes = [vp, _D, _D]
+        L.scf_get_electrostatics.argtypes = [vp, _D, _D, _D]
         L.scf_propagate_chain.argtypes = [vp, _D, ctypes.c_int, _D]
         L.scf_get_phase_ms.argtypes = [vp, _D]
         L.scf_system_set_precision.argtypes = [vp, ctypes.c_int, ctypes.c_double]
@@ -243,6 +244,10 @@ class ScfSystem:
         _check(lib().scf_get_profiles(self.h, _p(phi), _p(mphi)))
         out["seg_phi"] = {s: phi[i] for i, s in enumerate(self.seg_names)}
         out["mol_phi"] = {m: mphi[i] for i, m in enumerate(self.mol_names)}
+        if self.desc.get("charged"):
+            el = np.zeros((3, self.M))
+            _check(lib().scf_get_electrostatics(self.h, _p(el[0]), _p(el[1]), _p(el[2])))
+            out["psi"], out["q"], out["eps"] = el[0], el[1], el[2]
         return out
 
     def set_precision(self, mixed, switch_residual=1e-4):

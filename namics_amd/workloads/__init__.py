@@ -196,16 +196,17 @@ def brush_chain_input(chis=(0.0, 0.5, 0.8), **kw):
     return text
 
 
-def write_guess(path, names, dims, x, method="DIIS", fjc=1):
+def write_guess(path, names, dims, x, method="DIIS", fjc=1, charged=False):
     """Write fields in the reference's guess-file format (Lattice::StoreGuess,
     src/lattice.cpp:971-1019; read back by Lattice::ReadGuess) at full
     precision: method line, sizes (grid layers) and refinement, charge flag,
-    field-block names, values."""
+    field-block names, values (with charged=True, x ends with the potential
+    block)."""
     mx, my, mz = dims
     with open(path, "w") as f:
         f.write("%s\n" % method)
         f.write(" %d\t%d\t%d\t%d\n" % (mx, my, mz, fjc))
-        f.write("false\n")
+        f.write("true\n" if charged else "false\n")
         f.write("%d\n" % len(names))
         for n in names:
             f.write("%s\n" % n)

@@ -1254,6 +1254,7 @@ struct Oracle {
   std::vector<vec> mol_phi;               // per mol: molmon * M
   std::vector<vec> mol_phitot;
   vec phitot, alpha, F_dens, GP_dens;
+  vec psi, EE, q, eps, Xel;               // electrostatics (charged systems)
   vec GN, norm, n, theta, phibulk, Mu, theta_out;
   double F = 0, GP = 0, residual = 0;
   int status = 0;  // 0 ok, -1 GN<=0, -2/-3 NaN in DIIS, -4 negative solvent phibulk
@@ -1274,6 +1275,11 @@ struct Oracle {
     mol_phitot.assign(nmol, vec(M, 0.0));
     for (int i = 0; i < nmol; i++) mol_phi[i].assign(s.mol[i].molmon.size() * M, 0.0);
     phitot.assign(M, 0.0);
+    psi.assign(M, 0.0);
+    EE.assign(M, 0.0);
+    q.assign(M, 0.0);
+    eps.assign(M, 0.0);
+    Xel.assign(M, 0.0);
     alpha.assign(M, 0.0);
     F_dens.assign(M, 0.0);
     GP_dens.assign(M, 0.0);
@@ -1447,15 +1453,156 @@ struct Oracle {
     }
   }
 
+  // ------------------------------------------------------ electrostatics
+  // Physical constants of the reference (src/namics.cpp:43-49); the bond
+  // length is the lattice's `bondlength` over fjc (src/lattice.cpp:332-337).
+  static constexpr double kE = 1.60217e-19, kT = 298.15, kB = 1.38065e-23, kEps0 = 8.85418e-12;
+  double bond() const { return s.lat.bond_length / s.lat.fjc; }
+
+  // LGrad1::set_M_bounds (src/LGrad1.cpp:820-832): mirrored ghosts whatever
+  // the boundary conditions
+  void set_M_bounds(double *X) const {
+    const int f = s.lat.fjc, MX = s.lat.MX;
+    for (int k = 0; k < f; k++) {
+      X[k] = X[2 * f - k - 1];
+      X[MX + f + k] = X[MX + f - k - 1];
+    }
+  }
+
+  // LG1Planar::UpdateEE (src/LG1Planar.cpp:162-178) and LGrad1::UpdateEE
+  // (src/LGrad1.cpp:592-651): the field energy density of psi at each site.
+  void update_EE() {
+    const int f = s.lat.fjc, MX = s.lat.MX;
+    const double k_BT = kB * kT;
+    double pf = 0.5 * kEps0 * bond() / k_BT * (k_BT / kE) * (k_BT / kE);
+    double *P = psi.data();
+    set_M_bounds(P);
+    Zero(EE.data(), M);
+    const int geo = s.lat.geometry;
+    const double off = s.lat.offset_first_layer;  // offset_first_layer * fjc
+    if (geo == GEOM_PLANAR) {
+      pf = pf / 2.0 * f * f;
+      double Explus = P[f - 1] - P[f];
+      Explus *= Explus;
+      for (int x = f; x < MX + f; x++) {
+        double Exmin = Explus;
+        Explus = P[x] - P[x + 1];
+        Explus *= Explus;
+        EE[x] = pf * (Exmin + Explus);
+      }
+    } else if (geo == GEOM_CYLINDRICAL) {
+      int r = (int)off;  // an int in the reference
+      pf = pf * kPIE;
+      for (int x = f; x < MX + f; x++) {
+        r++;
+        double Exmin = P[x] - P[x - 1];
+        Exmin *= (r - 1) * Exmin;
+        double Explus = P[x] - P[x + 1];
+        Explus *= r * Explus;
+        EE[x] = pf * (Exmin + Explus) / s.lat.L[x];
+      }
+    } else {
+      pf = pf * kPIE * 2 / f;
+      int r = (int)(off + 1.0);  // an int in the reference
+      double Explus = r * (P[f] - P[f + 1]);
+      Explus *= Explus;
+      EE[f] = pf * Explus / s.lat.L[f];
+      for (int x = f + 1; x < MX + f; x++) {
+        r += 1;
+        double Exmin = Explus;
+        Explus = r * (P[x] - P[x + 1]);
+        Explus *= Explus;
+        EE[x] = pf * (Exmin + Explus) / s.lat.L[x];
+      }
+    }
+  }
+
+  // System::DoElectrostatics (src/system.cpp:1920-1953) without fixed
+  // potentials: charge density and dielectric constant per site, psi from x.
+  void do_electrostatics(double *gp, const double *xp) {
+    Lat lat(s.lat);
+    Zero(q.data(), M);
+    Zero(eps.data(), M);
+    for (int i = 0; i < nseg; i++) {
+      YplusisCtimesX(q.data(), phi[i].data(), s.seg[i].valence, M);
+      lat.set_bounds(phi[i].data());
+      YplusisCtimesX(eps.data(), phi[i].data(), s.seg[i].epsilon, M);
+    }
+    Div(q.data(), phitot.data(), M);
+    Div(eps.data(), phitot.data(), M);
+    Cp(psi.data(), xp, M);
+    Cp(gp, psi.data(), M);
+    set_M_bounds(psi.data());
+  }
+
+  // LG1Planar::UpdatePsi (src/LG1Planar.cpp:181-199) and LGrad1::UpdatePsi
+  // (src/LGrad1.cpp:654-686) without fixed potentials: one sweep of the
+  // discretised Poisson equation (upwind in planar and cylindrical
+  // geometry, Jacobi in spherical), g -= the swept psi.
+  void update_psi(double *gp) {
+    const int f = s.lat.fjc, MX = s.lat.MX;
+    const double k_BT = kB * kT;
+    double C = kE * kE / (kEps0 * k_BT * bond());
+    const double *P = psi.data(), *E = eps.data();
+    double *X = Xel.data();
+    const int geo = s.lat.geometry;
+    const double off = s.lat.offset_first_layer;
+    if (geo == GEOM_PLANAR) {
+      C = C * 2.0 / f / f;
+      double epsXplus = E[f - 1] + E[f];
+      for (int x = f; x < MX + f; x++) {
+        double epsXmin = epsXplus;
+        epsXplus = E[x] + E[x + 1];
+        double a = x == f ? P[f - 1] : X[x - 1];
+        X[x] = (epsXmin * a + C * q[x] + epsXplus * P[x + 1]) / (epsXmin + epsXplus);
+      }
+    } else if (geo == GEOM_CYLINDRICAL) {
+      C = C / kPIE;
+      double r = off;
+      double epsXplus = r * (E[f - 1] + E[f]);
+      for (int x = f; x < MX + f; x++) {
+        r++;
+        double epsXmin = epsXplus;
+        epsXplus = r * (E[x] + E[x + 1]);
+        double a = x == f ? P[f - 1] : X[x - 1];
+        X[x] = (epsXmin * a + C * q[x] + epsXplus * P[x + 1]) / (epsXmin + epsXplus);
+      }
+    } else {
+      C = C / (2.0 * kPIE) * f;
+      double r = off;
+      double epsXplus = r * r * (E[f - 1] + E[f]);
+      for (int x = f; x < MX + f; x++) {
+        double epsXmin = epsXplus;
+        r++;
+        epsXplus = r * r * (E[x] + E[x + 1]);
+        X[x] = (epsXmin * P[x - 1] + C * q[x] * s.lat.L[x] + epsXplus * P[x + 1]) / (epsXmin + epsXplus);
+      }
+    }
+    // YisAminB over all M: the ghosts of X are never written (zero); the
+    // caller removes the ghosts of g afterwards
+    for (int z = 0; z < M; z++) gp[z] = gp[z] - X[z];
+  }
+
   // Classical_residual: g from x
   void residual_eval(const double *x, double *g) {
     Lat lat(s.lat);
-    const int nvar = nit * M;
+    const int nvar = s.nvar();
     status = 0;
     Cp(g, x, nvar);
-    // PutU (src/system.cpp:1987-2085)
+    // PutU (src/system.cpp:1987-2085): with charges, u also carries
+    // -epsilon EE + valence psi
+    if (s.charged) {
+      Cp(psi.data(), x + nit * M, M);
+      update_EE();
+    }
     for (int i = 0; i < nseg; i++)
-      if (s.seg[i].field_block >= 0) Cp(u[i].data(), x + s.seg[i].field_block * M, M);
+      if (s.seg[i].field_block >= 0) {
+        Cp(u[i].data(), x + s.seg[i].field_block * M, M);
+        if (s.charged) {
+          YplusisCtimesX(u[i].data(), EE.data(), -1.0 * s.seg[i].epsilon, M);
+          if (s.seg[i].valence != 0) YplusisCtimesX(u[i].data(), psi.data(), s.seg[i].valence, M);
+        }
+      }
     // Segment::PrepareForCalculations (src/segment.cpp:835-879)
     for (int i = 0; i < nseg; i++) {
       const SegmentSpec &sg = s.seg[i];
@@ -1477,13 +1624,14 @@ struct Oracle {
     for (int i = 0; i < nseg; i++)
       if (s.seg[i].freedom == SEG_FROZEN) Add(phitot.data(), phi[i].data(), M);
     for (int i = 0; i < nmol; i++) compute_phi_mol(i);
-    double B = 0;
+    double A = 0, B = 0;  // bulk charge and volume fraction of free and restricted molecules
     for (int i = 0; i < nmol; i++) {
       const MoleculeSpec &m = s.mol[i];
       double nrm = 0;
       if (m.freedom == MOL_FREE) {
         nrm = phibulk[i] / m.chainlength;
         n[i] = nrm * GN[i];
+        A += phibulk[i] * s.mol_charge[i];
         B += phibulk[i];
       }
       if (m.freedom == MOL_RESTRICTED) {
@@ -1493,6 +1641,7 @@ struct Oracle {
             phibulk[i] = 0;
           } else {
             phibulk[i] = m.chainlength * nrm;
+            A += phibulk[i] * s.mol_charge[i];
             B += phibulk[i];
           }
         } else {
@@ -1516,6 +1665,18 @@ struct Oracle {
         if (nrm > 0) Norm(ph, nrm, M);
       }
     }
+    // the neutralizer's bulk fraction makes the bulk neutral
+    // (System::ComputePhis src/system.cpp:2487-2516)
+    const int nz = s.charged ? s.neutralizer : -1;
+    if (nz > -1) {
+      const int sv = s.solvent;
+      phibulk[nz] = ((B - 1.0) * s.mol_charge[sv] - A) / (s.mol_charge[nz] - s.mol_charge[sv]);
+      B += phibulk[nz];
+      const double nrm = phibulk[nz] / s.mol[nz].chainlength;
+      n[nz] = nrm * GN[nz];
+      theta[nz] = n[nz] * s.mol[nz].chainlength;
+      norm[nz] = nrm;
+    }
     if (s.solvent > -1) {
       const int sv = s.solvent;
       const MoleculeSpec &m = s.mol[sv];
@@ -1531,6 +1692,9 @@ struct Oracle {
       for (size_t k = 0; k < m.molmon.size(); k++)
         if (nrm > 0) Norm(mol_phi[sv].data() + k * M, nrm, M);
     }
+    if (nz > -1)
+      for (size_t k = 0; k < s.mol[nz].molmon.size(); k++)
+        if (norm[nz] > 0) Norm(mol_phi[nz].data() + k * M, norm[nz], M);
     for (int i = 0; i < nmol; i++) {
       const MoleculeSpec &m = s.mol[i];
       for (size_t k = 0; k < m.molmon.size(); k++) {
@@ -1571,10 +1735,20 @@ struct Oracle {
       lat.remove_bounds(gi);
       for (int z = 0; z < M; z++) gi[z] = gi[z] * s.ksam[z];
     }
+    // the potential block (src/system.cpp:2161-2169)
+    if (s.charged) {
+      double *gp = g + nit * M;
+      Cp(gp, x + nit * M, M);
+      do_electrostatics(gp, x + nit * M);
+      set_M_bounds(psi.data());
+      psi[0] = psi[1];
+      update_psi(gp);
+      lat.remove_bounds(gp);
+    }
   }
 
   double residual_norm(const double *g) const {
-    const int nvar = nit * M;
+    const int nvar = s.nvar();
     if (s.solver.max_g) {
       double mx = 0;
       for (int i = 0; i < nvar; i++) mx = std::max(mx, std::fabs(g[i]));
@@ -1637,6 +1811,8 @@ struct Oracle {
       Add(Fd, TEMP.data(), M);
     }
     for (int z = 0; z < M; z++) Fd[z] = Fd[z] * s.ksam[z];
+    if (s.charged)  // src/system.cpp:3019-3027: q psi / 2, frozen sites included
+      for (int z = 0; z < M; z++) Fd[z] = Fd[z] + 0.5 * (q[z] * psi[z]);
     F = lat.weighted_sum(Fd);
 
     double *GPd = GP_dens.data();
@@ -1667,7 +1843,20 @@ struct Oracle {
     }
     // u_ext is zero for every segment on this path (no external potentials)
     Norm(GPd, -1.0, M);
-    for (int z = 0; z < M; z++) GPd[z] = GPd[z] * s.ksam[z];
+    if (s.charged) {
+      // src/system.cpp:3223-3238: eps EE - q psi / 2 on accessible sites,
+      // q psi / 2 on the others
+      for (int z = 0; z < M; z++) {
+        double t = EE[z] * eps[z];
+        t = t * -2.0;
+        t = t + q[z] * psi[z];
+        t = t * (-1.0 / 2.0);
+        GPd[z] = (GPd[z] + t) * s.ksam[z];
+        GPd[z] = GPd[z] + (q[z] - q[z] * s.ksam[z]) * psi[z] * 0.5;
+      }
+    } else {
+      for (int z = 0; z < M; z++) GPd[z] = GPd[z] * s.ksam[z];
+    }
     GP = lat.weighted_sum(GPd);
 
     for (int i = 0; i < nmol; i++) {
@@ -1692,7 +1881,7 @@ struct Oracle {
 
   // SFNewton::iterate_DIIS (src/sfnewton.cpp:1037-1130)
   int solve_diis(double *x, int &iterations, double &res) {
-    const int nvar = nit * M;
+    const int nvar = s.nvar();
     const int m = s.solver.m;
     const double dmax = s.solver.deltamax, tol = s.solver.tolerance;
     const int itlim = s.solver.iterationlimit;
@@ -1787,7 +1976,7 @@ int oracle_describe(void *h, char *buf, int len) {
 void oracle_sizes(void *h, int *out) {
   Oracle *o = (Oracle *)h;
   const LatticeSpec &l = o->s.lat;
-  int v[9] = {o->M, o->nit * o->M, o->nseg, o->nmol, l.MX, l.MY, l.MZ, l.gradients, o->nit};
+  int v[9] = {o->M, o->s.nvar(), o->nseg, o->nmol, l.MX, l.MY, l.MZ, l.gradients, o->nit};
   std::memcpy(out, v, sizeof(v));
 }
 
@@ -1801,6 +1990,16 @@ int oracle_read_guess(void *h, const char *guess_text, double *x, char *err, int
   }
   std::copy(xv.begin(), xv.end(), x);
   return 0;
+}
+
+// Electrostatic profiles of the last residual: psi, q, eps, EE (4 rows of M).
+void oracle_electro(void *h, double *out) {
+  Oracle *o = (Oracle *)h;
+  const int M = o->M;
+  std::copy(o->psi.begin(), o->psi.end(), out);
+  std::copy(o->q.begin(), o->q.end(), out + M);
+  std::copy(o->eps.begin(), o->eps.end(), out + 2 * M);
+  std::copy(o->EE.begin(), o->EE.end(), out + 3 * M);
 }
 
 // Residual at x; fills g and keeps the state for the getters below.

@@ -1,0 +1,195 @@
+"""Charged systems (electrostatics on 1-gradient lattices) for reference
+fixtures and parity tests.
+
+Each case is `.in` text in the reference's input format; `eval` lists
+(seed, amplitude) pairs of uniform random fields (the potential block
+included) at which the residual is evaluated, `converged` the DIIS runs.
+The cases cover: a charged polymer adsorbing on a wall with a neutralizer
+counterion and free salt; a charged brush (pinned first segment) in salt
+with segments of different dielectric constant; cylindrical and spherical
+lattices with an offset; a stiff (Markov 2) charged chain; a refined
+planar hexagonal lattice (fjc 2).
+"""
+
+WALL = """
+lat : flat : gradients : 1
+lat : flat : n_layers : {n}
+lat : flat : lattice_type : simple_cubic
+lat : flat : lowerbound : surface
+lat : flat : bondlength : 3e-10
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+"""
+
+SALT = """
+mon : Na : freedom : free
+mon : Cl : freedom : free
+mon : Na : valence : 1
+mon : Cl : valence : -1
+mon : W : freedom : free
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+"""
+
+CHARGED = {
+    # a weakly charged polymer adsorbing on an uncharged wall; Na
+    # neutralizes the polymer, Cl is free salt
+    "el1d_poly": dict(
+        text=WALL.format(n=40) + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.2
+mon : S : chi_A : -1.2
+mon : A : chi_W : 0.5
+mol : pol : composition : (A)50
+mol : pol : freedom : restricted
+mol : pol : theta : 3.0
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.01
+""",
+        eval=[(101, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # a charged brush: pinned first segment, no neutralizer (the free salt's
+    # bulk is neutral), a low-epsilon polymer (grad_epsilon on)
+    "el1d_brush": dict(
+        text=WALL.format(n=30) + SALT + """
+mon : X : freedom : pinned
+mon : X : pinned_range : firstlayer
+mon : A : freedom : free
+mon : A : valence : -0.5
+mon : X : valence : -0.5
+mon : A : epsilon : 5
+mon : X : epsilon : 5
+mon : A : chi_W : 0.3
+mol : brush : composition : (X)1(A)24
+mol : brush : freedom : restricted
+mol : brush : n : 0.05
+mol : Na : composition : (Na)1
+mol : Na : freedom : free
+mol : Na : phibulk : 0.005
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.005
+""",
+        eval=[(102, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # cylindrical lattice with an offset: a charged polymer around a rod
+    "el1d_cyl": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : geometry : cylindrical
+lat : flat : n_layers : 30
+lat : flat : offset_first_layer : 4
+lat : flat : lattice_type : simple_cubic
+lat : flat : lowerbound : surface
+lat : flat : bondlength : 5e-10
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.3
+mon : S : chi_A : -1.0
+mol : pol : composition : (A)40
+mol : pol : freedom : restricted
+mol : pol : theta : 60
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.02
+""",
+        eval=[(103, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # spherical lattice (the Jacobi form of the potential sweep)
+    "el1d_sph": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : geometry : spherical
+lat : flat : n_layers : 25
+lat : flat : offset_first_layer : 3
+lat : flat : lattice_type : simple_cubic
+lat : flat : lowerbound : surface
+lat : flat : bondlength : 5e-10
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.2
+mon : S : chi_A : -1.0
+mol : pol : composition : (A)20
+mol : pol : freedom : restricted
+mol : pol : theta : 200
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.02
+""",
+        eval=[(104, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # a stiff charged chain (Markov 2) in a planar gap between two walls
+    "el1d_stiff": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : n_layers : 30
+lat : flat : lattice_type : simple_cubic
+lat : flat : lowerbound : surface
+lat : flat : upperbound : surface
+lat : flat : bondlength : 3e-10
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : T : freedom : frozen
+mon : T : frozen_range : upperbound
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.4
+mon : S : chi_A : -0.8
+mon : T : chi_A : -0.8
+mol : pol : composition : (A)30
+mol : pol : freedom : restricted
+mol : pol : theta : 2.0
+mol : pol : Markov : 2
+mol : pol : k_stiff : 1
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.01
+""",
+        eval=[(105, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # refined planar hexagonal lattice (two grid layers per bond)
+    "el1d_fjc2": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : n_layers : 20
+lat : flat : FJC_choices : 5
+lat : flat : lattice_type : hexagonal
+lat : flat : lowerbound : surface
+lat : flat : bondlength : 3e-10
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.2
+mon : S : chi_A : -1.0
+mol : pol : composition : (A)20
+mol : pol : freedom : restricted
+mol : pol : theta : 1.5
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.01
+""",
+        eval=[(106, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+}

@@ -94,6 +94,9 @@ def output_lines(o, extra_kal=()):
         pro.append("pro : mon : %s : phi" % s)
     for m in mols:
         pro.append("pro : mol : %s : phi" % m)
+    if o.desc["charged"]:
+        for p in ("psi", "q", "eps"):
+            pro.append("pro : sys : %s : %s" % (o.sys_name, p))
     return "\n".join(kal + pro) + "\n"
 
 
@@ -114,6 +117,9 @@ def collect(o, kal, pro):
         res["seg_phi"][s] = pro["mon_%s_phi" % s]
     for m in o.mol_names:
         res["mol_phi"][m] = pro["mol_%s_phi" % m]
+    if o.desc["charged"]:
+        for p in ("psi", "q", "eps"):
+            res[p] = pro["sys_%s_%s" % (o.sys_name, p)]
     return res
 
 
@@ -128,7 +134,7 @@ def evaluation_fixture(case, seed, amp):
     with tempfile.TemporaryDirectory() as d:
         guess = ""
         if amp > 0:
-            write_guess(os.path.join(d, "x.ig"), o.field_names, o.dims, x, fjc=o.fjc)
+            write_guess(os.path.join(d, "x.ig"), o.field_names, o.dims, x, fjc=o.fjc, charged=o.desc["charged"])
             guess = ("sys : %s : initial_guess : file\n"
                      "sys : %s : guess_inputfile : x.ig\n") % (o.sys_name, o.sys_name)
         text = case["text"] + "\n" + output_lines(o) + guess + (
@@ -156,7 +162,7 @@ def converged_fixture(case, method, tol, itlim=5000, seed=None, amp=0.0):
         if seed is not None:
             rng = np.random.default_rng(seed)
             x = rng.uniform(-amp, amp, o.nvar)
-            write_guess(os.path.join(d, "x.ig"), o.field_names, o.dims, x, fjc=o.fjc)
+            write_guess(os.path.join(d, "x.ig"), o.field_names, o.dims, x, fjc=o.fjc, charged=o.desc["charged"])
             guess = ("sys : %s : initial_guess : file\n"
                      "sys : %s : guess_inputfile : x.ig\n") % (o.sys_name, o.sys_name)
         text = case["text"] + "\n" + output_lines(o) + guess + (

@@ -40,6 +40,7 @@ def lib():
         L.oracle_solve_ones.argtypes = [_D, ctypes.c_int, _D]
         L.oracle_apply.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int]
         L.oracle_apply_markov2.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int, ctypes.c_int]
+        L.oracle_electro.argtypes = [ctypes.c_void_p, _D]
         _lib = L
     return _lib
 
@@ -115,6 +116,12 @@ class Oracle:
         out["seg_phi"] = {s: phi[i] for i, s in enumerate(self.seg_names)}
         out["mol_phi"] = {m: mphi[i] for i, m in enumerate(self.mol_names)}
         return out
+
+    def electro(self):
+        """psi, q, eps, EE of the last residual (charged systems), M each."""
+        a = np.zeros((4, self.M))
+        lib().oracle_electro(self.h, _p(a))
+        return dict(psi=a[0], q=a[1], eps=a[2], EE=a[3])
 
     def g1(self):
         a = np.zeros((len(self.seg_names), self.M))
