@@ -178,16 +178,21 @@ __device__ __noinline__ void chain1d_m1(const DevSys &S0, int mo) {
     const double gn = wg_sum(acc);
     if (threadIdx.x == 0) S0.sc->GN[mo] = gn;
   }
-  // backward with composition, Gf(s-1) fetched one step ahead
+  // backward with composition, Gf(s-1) fetched one step ahead.  On a ghost
+  // the composition reads Gf(s) as the reference leaves it: the value
+  // set_bounds copied there when Gf(s) propagated to s+1 (the site i0), and
+  // zero for the last row, whose ghosts ComputeGN's remove_bounds cleared
+  // (src/LGrad1.cpp:213-221,863-874).  Only phitot on the ghosts sees this.
   ldbl *ga = ra, *gb = rb;
   const int bl = b1 - 1;
+  const bool ghost = ii < 1 || ii > S.MX;
   g1 = active ? G1[(size_t)S.block_seg[bl] * M + ii] : 0.0;
   v = g1;
   double racc = 0.0, gf = 0.0;
   if (active) {
-    racc = Gf[(size_t)(N - 1) * M + ii] * v;
+    racc = (ghost ? 0.0 : Gf[(size_t)(N - 1) * M + ii]) * v;
     ga[ii] = v;
-    if (N >= 2) gf = Gf[(size_t)(N - 2) * M + ii];
+    if (N >= 2) gf = Gf[(size_t)(N - 2) * M + i0];
   }
   lds_barrier();
   s = N - 2;
@@ -211,7 +216,7 @@ __device__ __noinline__ void chain1d_m1(const DevSys &S0, int mo) {
         v = t * g1;
         racc += gf * v;
         gb[ii] = v;
-        if (s >= 1) gf = Gf[(size_t)(s - 1) * M + ii];
+        if (s >= 1) gf = Gf[(size_t)(s - 1) * M + i0];
       }
       lds_barrier();
       ldbl *tmp = ga;
@@ -324,15 +329,19 @@ __device__ __noinline__ void chain1d_m2(const DevSys &S0, int mo) {
     for (int k = 0; k < kSPT; k++) {
       g1[k] = act[k] ? G1[(size_t)S.block_seg[bl] * M + ii[k]] : 0.0;
       racc[k] = act[k] ? hbm(S.rho)[(size_t)S.block_mm[bl] * M + ii[k]] : 0.0;
+      // ghosts: the last row reads zero (ComputeGN's remove_bounds), earlier
+      // rows the components set_bounds(X, Y) paired there (m2_val)
+      const bool ghost = ii[k] < 1 || ii[k] > S.MX;
       if (act[k]) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
           gbA[c * M + ii[k]] = g1[k];
-          racc[k] += wc[c] * gfl[c * M + ii[k]] * g1[k];
+          racc[k] += wc[c] * (ghost ? 0.0 : gfl[c * M + ii[k]]) * g1[k];
         }
       }
 #pragma unroll
-      for (int c = 0; c < 3; c++) gf[k][c] = (act[k] && N >= 2) ? Gf[(size_t)(N - 2) * R + c * M + ii[k]] : 0.0;
+      for (int c = 0; c < 3; c++)
+        gf[k][c] = (act[k] && N >= 2) ? m2_val<double>(S, Gf + (size_t)(N - 2) * R, c, ii[k]) : 0.0;
     }
   }
   lds_barrier();
@@ -364,7 +373,7 @@ __device__ __noinline__ void chain1d_m2(const DevSys &S0, int mo) {
 #pragma unroll
           for (int c = 0; c < 3; c++) {
             gbB[c * M + ii[k]] = o[k][c];
-            if (s >= 1) gf[k][c] = Gf[(size_t)(s - 1) * R + c * M + ii[k]];
+            if (s >= 1) gf[k][c] = m2_val<double>(S, Gf + (size_t)(s - 1) * R, c, ii[k]);
           }
       lds_barrier();
       ldbl *t = gbA;
@@ -463,7 +472,9 @@ __device__ __noinline__ void chain_markov1(const DevSys &S0, int mo) {
       const int i = work_site(S, w, x, y, z);
       const double v = g1[i];
       gbA[i] = v;
-      rho[i] += gf[i] * v;
+      // a 1-gradient ghost (the only ghosts visited) reads the last row as
+      // ComputeGN's remove_bounds left it: zero (src/LGrad1.cpp:213-221)
+      rho[i] += (is_interior(S, x, y, z) ? gf[i] : 0.0) * v;
     }
   }
   Rows<kLds>::barrier();
@@ -478,7 +489,8 @@ __device__ __noinline__ void chain_markov1(const DevSys &S0, int mo) {
         const int i = work_site(S, w, x, y, z);
         const double v = stencil_prop<double>(S, gbA, i, x, y, z) * g1[i];
         gbB[i] = v;
-        rho[i] += gf[i] * v;
+        // on a ghost, Gf(s) as set_bounds left it when it propagated to s+1
+        rho[i] += gf[S.gradients == 1 ? map_x1(S, i) : i] * v;
       }
       Rows<kLds>::barrier();
       RowT *t = gbA;
@@ -1109,11 +1121,12 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
       const int i = work_site(S, w, x, y, z);
       const double v = g1[i];
       double r = rho[i];
+      const bool in = is_interior(S, x, y, z);  // 1-gradient ghosts read zero (ComputeGN's remove_bounds)
 #pragma unroll
       for (int c = 0; c < kC; c++) {
         if (c >= C) break;
         gbA[c * M + i] = v;
-        r += wcomp(c) * gf[c * M + i] * v;
+        r += wcomp(c) * (in ? gf[c * M + i] : 0.0) * v;
       }
       rho[i] = r;
     }
@@ -1144,7 +1157,15 @@ __device__ __noinline__ void chain_markov2(const DevSys &S0, int mo) {
           if (c >= C) break;
           const double v = o[c] * wg;
           gbB[c * M + i] = v;
-          r += wcomp(c) * gf[c * M + i] * v;
+          // 1-gradient ghosts: the components set_bounds(X, Y) paired there
+          double f;
+          if constexpr (kKind == M2_SC1)
+            f = m2_val<double>(S, gf, c, i);
+          else if constexpr (kKind == M2_HEX1)
+            f = m2h_val<double>(S, gf, c, i);
+          else
+            f = gf[c * M + i];
+          r += wcomp(c) * f * v;
         }
         rho[i] = r;
       }
