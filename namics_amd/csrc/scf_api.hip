@@ -121,8 +121,6 @@ int validate(const SystemSpec &s, std::string &why) {
     for (auto &m : s.mol)
       if (m.scheduled())
         return why = "charged systems with branched molecules are not supported", SCF_E_UNSUPPORTED;
-    if (s.fixed_psi0)
-      return why = "fixed surface potentials (e.psi0/kT) are not supported", SCF_E_UNSUPPORTED;
     if ((int)s.itmon.size() + 1 > kMaxSeg + 1) return why = "too many field blocks", SCF_E_UNSUPPORTED;
   }
   return SCF_OK;
@@ -310,6 +308,15 @@ int build(scf_system *S, const Placement *pl) {
     h.el_pf = pf;
     h.el_C = C;
     h.el_r0 = l.offset_first_layer;
+    h.el_C0 = e * e / (eps0 * k_BT * b);
+    h.grad_eps = s.grad_epsilon ? 1 : 0;
+    h.fixed_psi0 = s.fixed_psi0 ? 1 : 0;
+    h.el_last_fixed = -1;
+    for (int k = 0; k < nseg; k++) {
+      h.seg_fixpsi[k] = s.seg[k].freedom == SEG_FROZEN && s.seg[k].fixed_psi0 ? 1 : 0;
+      h.seg_psi0[k] = s.seg[k].psi0;
+      if (h.seg_fixpsi[k]) h.el_last_fixed = k;
+    }
   }
   for (int k = 0; k < nseg; k++) {
     h.seg_val[k] = s.seg[k].valence;

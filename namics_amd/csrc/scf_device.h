@@ -116,7 +116,10 @@ struct DevSys {
   // ---- electrostatics (charged systems, 1 gradient): x and g end with the
   // potential block; nitm counts the segment field blocks in front of it
   int charged, nitm, el_ri;  // el_ri: integer radius offset of the field energy (curved)
-  double seg_val[kMaxSeg], seg_eps[kMaxSeg];
+  int fixed_psi0, grad_eps;  // frozen segments fix the potential; segments differ in epsilon
+  int seg_fixpsi[kMaxSeg], el_last_fixed;  // frozen segments with a fixed potential; the last of them
+  double seg_val[kMaxSeg], seg_eps[kMaxSeg], seg_psi0[kMaxSeg];
+  double el_C0;              // e^2 / (eps0 kT b), the constant of the fixed-potential sweeps
   double el_pf, el_C;        // field-energy prefactor and Poisson constant (geometry factors included)
   double el_r0;              // offset_first_layer * fjc
   double *psi, *EE, *q, *eps;  // M each: potential (mirrored ghosts), field energy, charge, dielectric constant

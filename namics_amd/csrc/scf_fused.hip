@@ -1224,6 +1224,81 @@ __device__ __noinline__ void chain_strip_dispatch(const DevSys &S, int mo, bool 
 }
 
 // ------------------------------------------------------ electrostatics
+// The potential block with fixed surface potentials (frozen segments with
+// e.psi0/kT), as the reference computes it.  Segment::UpdateValence
+// (src/segment.cpp:1972-1979) overwrites psi with psi0 on the segment's
+// mask and ZERO everywhere else (OverwriteC, src/tools_host.h:514-517), and
+// UpdateQ (src/LG1Planar.cpp:226-244, src/LGrad1.cpp:733-767) sets the
+// charge on the mask from that psi, g = -q there; UpdatePsi
+// (src/LG1Planar.cpp:200-222, src/LGrad1.cpp:687-729) then takes Jacobi
+// sweeps of the last segment's overwritten psi off the masks, and g = x -
+// the swept psi there.  So the potential the sweep reads is psi0 on the
+// last segment's mask and zero elsewhere, not the iterate; this is kept
+// for parity.  Each sweep is a separate pass (elA, elB), as every one of
+// the reference's loops reads the values of the loop before it.
+template <typename Sys>
+__device__ __forceinline__ double fixed_psi_at(const Sys &S, int k, int x) {
+  return hbm(S.seg_mask)[k * S.M + el_mirror(S, x)] == 1.0 ? S.seg_psi0[k] : 0.0;
+}
+
+__device__ __noinline__ double electro_fixed(const DevSys &S0, double acc) {
+  CSys &S = uniform_sys(S0);
+  const int M = S.M, f = S.fjc, MX = S.MX;
+  const int tid = work_tid(S), nt = S.threads;
+  const bool max_g = S.max_g, cyl = S.geometry == 1;
+  gdbl *gp = hbm(S.g) + S.nitm * M;
+  const gdbl *xp = hbm(S.x) + S.nitm * M;
+  gdbl *J1 = hbm(S.elA), *J2 = cyl ? hbm(S.elB) : hbm(S.elA);
+  gdbl *q = hbm(S.q), *P = hbm(S.psi);
+  const gdbl *E = hbm(S.eps);
+  const int L = S.el_last_fixed;
+  const double C = S.el_C0, Cq = -S.el_C0, ff = (double)f * f;
+  auto on_mask = [&](int x) {
+    bool m = false;
+    for (int k = 0; k < S.nseg; k++)
+      if (S.seg_fixpsi[k] && hbm(S.seg_mask)[k * M + x] != 0.0) m = true;
+    return m;
+  };
+  for (int x = tid; x < M; x += nt) {
+    if (x < f || x >= MX + f) {  // ghosts keep the last overwrite, mirrored (psi[0] = psi[1])
+      J1[x] = fixed_psi_at(S, L, x == 0 ? 1 : x);
+      continue;
+    }
+    for (int k = 0; k < S.nseg; k++) {
+      if (!S.seg_fixpsi[k] || hbm(S.seg_mask)[k * M + x] != 1.0) continue;
+      const double a = fixed_psi_at(S, k, x - 1), b = fixed_psi_at(S, k, x), c = fixed_psi_at(S, k, x + 1);
+      double qx = -0.5 * (a - 2 * b + c) * ff * E[x] / Cq;
+      if (cyl) qx -= (c - a) / (2.0 * (S.el_r0 + x - f + 0.5)) * f * E[x] / Cq;
+      if (S.grad_eps) qx -= 0.25 * (E[x + 1] - E[x - 1]) * (c - a) * ff / Cq;
+      q[x] = qx;
+      gp[x] = -qx;
+    }
+    const double pL = fixed_psi_at(S, L, x);
+    J1[x] = on_mask(x) ? pL : 0.5 * (fixed_psi_at(S, L, x - 1) + fixed_psi_at(S, L, x + 1)) + q[x] * C / E[x];
+  }
+  __syncthreads();
+  if (cyl) {
+    for (int x = tid; x < M; x += nt) {
+      const bool sweep = x >= f && x < MX + f && !on_mask(x);
+      J2[x] = sweep ? J1[x] + (J1[x + 1] - J1[x - 1]) / (2.0 * (S.el_r0 + x - f + 0.5)) * f : J1[x];
+    }
+    __syncthreads();
+  }
+  for (int x = tid; x < M; x += nt) {
+    const bool in = x >= f && x < MX + f;
+    const bool sweep = in && !on_mask(x);
+    double v = J2[x];
+    if (sweep && S.grad_eps) v = v + 0.25 * (E[x + 1] - E[x - 1]) * (J2[x + 1] - J2[x - 1]) / E[x] * ff;
+    P[x] = v;
+    if (sweep) gp[x] = xp[x] - v;
+    if (in) {
+      const double gv = gp[x];
+      acc = max_g ? fmax(acc, fabs(gv)) : acc + gv * gv;
+    }
+  }
+  return acc;
+}
+
 // The potential block of the residual of a charged 1-gradient system
 // (System::Classical_residual src/system.cpp:2161-2169): charge density,
 // dielectric constant, psi and EE at every site, then one sweep of the
@@ -1245,6 +1320,7 @@ __device__ __noinline__ double electro_residual(const DevSys &S0, double acc) {
     if (i < f || i >= MX + f) gp[i] = 0.0;
   }
   __syncthreads();
+  if (S.fixed_psi0) return electro_fixed(S0, acc);
   const bool upwind = S.geometry != 2;
   for (int x = f + tid; x < MX + f; x += nt) {
     const double X = site_psi_sweep(S, x);

@@ -1533,6 +1533,113 @@ struct Oracle {
     Cp(psi.data(), xp, M);
     Cp(gp, psi.data(), M);
     set_M_bounds(psi.data());
+    // fixed surface potentials: Segment::UpdateValence (src/segment.cpp:1972-1979)
+    // for every frozen segment in order
+    if (s.fixed_psi0)
+      for (int i = 0; i < nseg; i++)
+        if (s.seg[i].freedom == SEG_FROZEN && s.seg[i].fixed_psi0) {
+          // OverwriteC (src/tools_host.h:514-517): psi0 on the mask, zero elsewhere
+          for (int z = 0; z < M; z++) psi[z] = s.seg[i].mask[z] == 1 ? s.seg[i].psi0 : 0.0;
+          set_M_bounds(psi.data());
+          update_q(gp, s.seg[i].mask);
+        }
+  }
+
+  // LG1Planar::UpdateQ (src/LG1Planar.cpp:226-244), LGrad1::UpdateQ
+  // (src/LGrad1.cpp:733-767): the charge on the mask that the fixed
+  // potential needs, and g = -q there.  The spherical correction is never
+  // applied (its geometry test misspells "spherical").
+  void update_q(double *gp, const std::vector<int> &mask) {
+    const int f = s.lat.fjc, MX = s.lat.MX;
+    const double k_BT = kB * kT;
+    const double C = -kE * kE / (kEps0 * k_BT * bond());
+    const double *P = psi.data(), *E = eps.data();
+    double a, b, c;
+    b = P[f - 1];
+    c = P[f];
+    for (int x = f; x < MX + f; x++) {
+      a = b;
+      b = c;
+      c = P[x + 1];
+      if (mask[x] == 1) q[x] = -0.5 * (a - 2 * b + c) * f * f * E[x] / C;
+    }
+    if (s.lat.geometry == GEOM_CYLINDRICAL) {
+      b = P[f - 1];
+      c = P[f];
+      for (int x = f; x < MX + f; x++) {
+        a = b;
+        b = c;
+        c = P[x + 1];
+        if (mask[x] == 1) q[x] -= (c - a) / (2.0 * (s.lat.offset_first_layer + x - f + 0.5)) * f * E[x] / C;
+      }
+    }
+    if (s.grad_epsilon) {
+      double a_, b_, c_;
+      b = P[f - 1];
+      c = P[f];
+      b_ = E[f - 1];
+      c_ = E[f];
+      for (int x = f; x < MX + f; x++) {
+        a = b;
+        b = c;
+        c = P[x + 1];
+        a_ = b_;
+        b_ = c_;
+        c_ = E[x + 1];
+        if (mask[x] == 1) q[x] -= 0.25 * (c_ - a_) * (c - a) * f * f / C;
+      }
+    }
+    for (int x = f; x < MX + f; x++)
+      if (mask[x] == 1) gp[x] = -q[x];
+  }
+
+  // UpdatePsi with fixed potentials (src/LG1Planar.cpp:200-222,
+  // src/LGrad1.cpp:687-729): Jacobi sweeps of psi off the potential masks
+  // (the cylindrical correction only on cylindrical lattices, as above),
+  // g -= the new psi there.
+  void update_psi_fixed(double *gp, const std::vector<int> &pmask) {
+    const int f = s.lat.fjc, MX = s.lat.MX;
+    const double k_BT = kB * kT;
+    const double C = kE * kE / (kEps0 * k_BT * bond());
+    double *P = psi.data();
+    const double *E = eps.data();
+    double a, b, c;
+    b = P[f - 1];
+    c = P[f];
+    for (int x = f; x < MX + f; x++) {
+      a = b;
+      b = c;
+      c = P[x + 1];
+      if (pmask[x] == 0) P[x] = 0.5 * (a + c) + q[x] * C / E[x];
+    }
+    if (s.lat.geometry == GEOM_CYLINDRICAL) {
+      b = P[f - 1];
+      c = P[f];
+      for (int x = f; x < MX + f; x++) {
+        a = b;
+        b = c;
+        c = P[x + 1];
+        if (pmask[x] == 0) P[x] += (c - a) / (2.0 * (s.lat.offset_first_layer + x - f + 0.5)) * f;
+      }
+    }
+    if (s.grad_epsilon) {
+      double a_, b_, c_;
+      b = P[f - 1];
+      c = P[f];
+      b_ = E[f - 1];
+      c_ = E[f];
+      for (int x = f; x < MX + f; x++) {
+        a = b;
+        b = c;
+        c = P[x + 1];
+        a_ = b_;
+        b_ = c_;
+        c_ = E[x + 1];
+        if (pmask[x] == 0) P[x] += 0.25 * (c_ - a_) * (c - a) / E[x] * f * f;
+      }
+    }
+    for (int x = f; x < MX + f; x++)
+      if (pmask[x] == 0) gp[x] -= P[x];
   }
 
   // LG1Planar::UpdatePsi (src/LG1Planar.cpp:181-199) and LGrad1::UpdatePsi
@@ -1742,7 +1849,17 @@ struct Oracle {
       do_electrostatics(gp, x + nit * M);
       set_M_bounds(psi.data());
       psi[0] = psi[1];
-      update_psi(gp);
+      if (s.fixed_psi0) {
+        // psiMask: the masks of the frozen segments with a fixed potential
+        // (System::PrepareForCalculations src/system.cpp:379-388)
+        std::vector<int> pmask(M, 0);
+        for (int i = 0; i < nseg; i++)
+          if (s.seg[i].freedom == SEG_FROZEN && s.seg[i].fixed_psi0)
+            for (int z = 0; z < M; z++) pmask[z] += s.seg[i].mask[z];
+        update_psi_fixed(gp, pmask);
+      } else {
+        update_psi(gp);
+      }
       lat.remove_bounds(gp);
     }
   }

@@ -8,7 +8,9 @@ The cases cover: a charged polymer adsorbing on a wall with a neutralizer
 counterion and free salt; a charged brush (pinned first segment) in salt
 with segments of different dielectric constant; cylindrical and spherical
 lattices with an offset; a stiff (Markov 2) charged chain; a refined
-planar hexagonal lattice (fjc 2).
+planar hexagonal lattice (fjc 2); frozen plates that fix the potential
+(`e.psi0/kT`, planar and cylindrical, the latter with segments of
+different dielectric constant).
 """
 
 WALL = """
@@ -191,5 +193,59 @@ mol : Cl : phibulk : 0.01
 """,
         eval=[(106, 0.2)],
         converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # a frozen plate at the first layer fixing the potential, a charged
+    # polymer and salt in front of it
+    "el1d_fixed": dict(
+        text=WALL.format(n=30) + SALT + """
+mon : P : freedom : frozen
+mon : P : frozen_range : 1;1
+mon : P : e.psi0/kT : 1.5
+mon : A : freedom : free
+mon : A : valence : -0.3
+mon : P : chi_A : -0.5
+mol : pol : composition : (A)20
+mol : pol : freedom : restricted
+mol : pol : theta : 1.5
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.01
+""",
+        eval=[(107, 0.2)],
+        converged=[],
+    ),
+    # the same on a cylindrical lattice, with a low-epsilon polymer
+    "el1d_fixed_cyl": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : geometry : cylindrical
+lat : flat : n_layers : 30
+lat : flat : offset_first_layer : 2
+lat : flat : lattice_type : simple_cubic
+lat : flat : lowerbound : surface
+lat : flat : bondlength : 5e-10
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : P : freedom : frozen
+mon : P : frozen_range : 1;2
+mon : P : e.psi0/kT : -1
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : 0.2
+mon : A : epsilon : 10
+mon : P : chi_A : -0.3
+mol : pol : composition : (A)25
+mol : pol : freedom : restricted
+mol : pol : theta : 30
+mol : Na : composition : (Na)1
+mol : Na : freedom : free
+mol : Na : phibulk : 0.01
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : neutralizer
+""",
+        eval=[(108, 0.2)],
+        converged=[],
     ),
 }

@@ -11,7 +11,11 @@ reproduce the reference to a few ulps, including the potential, charge and
 dielectric profiles, and its DIIS must take the reference's number of
 iterations.  Two properties that do not come from the reference are
 checked too: the neutralizer makes the bulk neutral, and at convergence psi
-solves the discretised Poisson equation the sweep iterates.
+solves the discretised Poisson equation the sweep iterates.  The cases with
+a fixed surface potential (e.psi0/kT) are pinned at given fields only: the
+reference's fixed-potential sweep reads psi0 on the last plate and zero
+elsewhere instead of the iterate, and its DIIS runs on them diverge (the
+reference stops with GN <= 0), as the oracle's do.
 """
 import json
 import os
@@ -166,9 +170,11 @@ def test_charged_input_refusals():
     two = ok.replace("lat : flat : gradients : 1\nlat : flat : n_layers : 20",
                      "lat : flat : gradients : 2\nlat : flat : n_layers_x : 10\nlat : flat : n_layers_y : 10")
     assert "1-gradient" in _parse_error(two)
-    # a fixed surface potential is parsed but refused by the library
+    # a fixed surface potential on a frozen segment; on a free one it is refused
     wall = ok + "mon : S : freedom : frozen\nmon : S : frozen_range : 1;1\nmon : S : e.psi0/kT : 2\n"
     assert describe_input(wall)["fixed_psi0"]
+    assert "frozen" in _parse_error(ok + "mon : W : e.psi0/kT : 2\n")
+    assert "-25..25" in _parse_error(wall.replace("e.psi0/kT : 2", "e.psi0/kT : 30"))
     # valences that sum to zero in every molecule leave the system uncharged
     d = describe_input(BASE.replace("(A)10", "(A)5(Na)5").replace("(Na)1", "(A)1(Na)1") + RESTRICTED +
                        "mol : Na : freedom : free\nmol : Na : phibulk : 0.01\n")
