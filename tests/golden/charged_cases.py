@@ -7,8 +7,9 @@ included) at which the residual is evaluated, `converged` the DIIS runs.
 The cases cover: a charged polymer adsorbing on a wall with a neutralizer
 counterion and free salt; a charged brush (pinned first segment) in salt
 with segments of different dielectric constant; cylindrical and spherical
-lattices with an offset; a stiff (Markov 2) charged chain; a refined
-planar hexagonal lattice (fjc 2); frozen plates that fix the potential
+lattices with an offset; stiff (Markov 2) charged chains between two
+walls, against one wall with a mirrored far boundary, and on a refined
+hexagonal lattice; a refined planar hexagonal lattice (fjc 2); frozen plates that fix the potential
 (`e.psi0/kT`, planar and cylindrical, the latter with segments of
 different dielectric constant).
 """
@@ -192,6 +193,56 @@ mol : Cl : freedom : free
 mol : Cl : phibulk : 0.01
 """,
         eval=[(106, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # stiff charged chains against a wall with the far boundary mirrored:
+    # the ghosts there carry the reference's paired-component end points
+    "el1d_stiff_mirror": dict(
+        text=WALL.format(n=30) + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.3
+mon : S : chi_A : -0.6
+mol : pol : composition : (A)25
+mol : pol : freedom : restricted
+mol : pol : theta : 3.0
+mol : pol : Markov : 2
+mol : pol : k_stiff : 2
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.01
+""",
+        eval=[(109, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # the same on a refined hexagonal lattice (FJC direction components)
+    "el1d_hex_m2": dict(
+        text="""
+lat : flat : gradients : 1
+lat : flat : n_layers : 20
+lat : flat : FJC_choices : 5
+lat : flat : lattice_type : hexagonal
+lat : flat : lowerbound : surface
+lat : flat : bondlength : 3e-10
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.2
+mon : S : chi_A : -0.8
+mol : pol : composition : (A)20
+mol : pol : freedom : restricted
+mol : pol : theta : 1.5
+mol : pol : Markov : 2
+mol : pol : k_stiff : 1
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.01
+""",
+        eval=[(110, 0.2)],
         converged=[dict(method="DIIS", tol=1e-8)],
     ),
     # a frozen plate at the first layer fixing the potential, a charged
