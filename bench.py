@@ -74,7 +74,11 @@ def hbm_bytes_per_evaluation(desc, ckpt=1, w=8, kd=10):
       density (2 rows), the residual g written (nit);
     * the DIIS update with kd history entries: xR_k, e_k and x written, x
       and g read, kd entries of both histories read for the Gram row and kd
-      of xR for the combination: (3 kd + 5) nvar values."""
+      of xR for the combination: (3 kd + 5) nvar values;
+    * a charged system: the potential block read (fields pass and sweep),
+      psi, EE, q, eps written and q, eps read by the sweep and the
+      thermodynamics, the sweep's two coefficient rows written and read, its
+      g row written: 12 rows."""
     g = desc["gradients"]
     M = desc["M"]
     W = M if g == 1 else desc["MX"] * (desc["MY"] if g >= 2 else 1) * (desc["MZ"] if g == 3 else 1)
@@ -84,6 +88,8 @@ def hbm_bytes_per_evaluation(desc, ckpt=1, w=8, kd=10):
     nblocks = sum(m["blocks"] for m in desc["molecules"])
     nmm = sum(m["molmon"] for m in desc["molecules"])
     rows = 3 * nit + nseg + 2 * nblocks + 3 * nmm + 4 * nseg + 2
+    if desc.get("charged"):
+        rows += 12
     diis = (3 * kd + 5) * desc["nvar"]
     return gf + 8 * M * rows + 8 * diis
 

@@ -1,10 +1,16 @@
 """Batched-solve benchmark: many independent 1-gradient systems in one launch.
 
-512 planar adsorption systems drawn from a fixed seed (layers U{50..400},
-homopolymer N log-uniform in [20, 2000], chi(A,W) U[0.3, 1.2], surface
-chi U[-1.5, 0], bulk volume fraction log-uniform in [1e-4, 1e-1], odd draws
-second-order Markov with k_stiff = 1), zero initial fields, DIIS to 1e-7 with
-the reference's defaults (m = 10, deltamax = 0.1, iteration limit 1000), FP64.
+`--workload batch1d`: 512 planar adsorption systems drawn from a fixed seed
+(layers U{50..400}, homopolymer N log-uniform in [20, 2000], chi(A,W)
+U[0.3, 1.2], surface chi U[-1.5, 0], bulk volume fraction log-uniform in
+[1e-4, 1e-1], odd draws second-order Markov with k_stiff = 1).
+`--workload charged`: 512 charged systems (a restricted polyelectrolyte
+adsorbing on a wall with counterions as the neutralizer and free salt;
+layers U{30..200}, N log-uniform in [20, 500], valence U[-1, -0.05],
+surface chi U[-1.5, 0], salt log-uniform in [1e-3, 1e-1], theta
+U[0.5, 5], odd draws second-order), the potential solved with the fields.
+Zero initial fields, DIIS to 1e-7 with the reference's defaults (m = 10,
+deltamax = 0.1, iteration limit 1000), FP64.
 One step = one launch that solves all of them (one workgroup per system).
 Solves/sec counts the systems converged in both the library and the
 reference program (which the CPU baseline runs on every draw, one process
@@ -21,8 +27,12 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def batch_input(d):
-    from namics_amd.workloads import homopolymer_adsorption
-    t = homopolymer_adsorption(d["layers"], d["N"], d["chi"], d["chi_s"], d["phib"], markov=d["markov"])
+    from namics_amd.workloads import homopolymer_adsorption, polyelectrolyte_adsorption
+    if d.get("kind") == "charged":
+        t = polyelectrolyte_adsorption(d["layers"], d["N"], d["valence"], d["chi_s"], d["salt"], d["theta"],
+                                       markov=d["markov"])
+    else:
+        t = homopolymer_adsorption(d["layers"], d["N"], d["chi"], d["chi_s"], d["phib"], markov=d["markov"])
     return t.replace("iterationlimit : 5000", "iterationlimit : 1000")
 
 
@@ -68,11 +78,14 @@ def cpu_all_cores(draws, procs):
                 nan=sum(1 for _, _, n in res if n), slowest_s=max(t for _, t, _ in res))
 
 
-def run_batch(args, n=512, seed=12345):
+def run_batch(args, n=512, seed=None):
     from namics_amd import synchronize
     from namics_amd import ScfSystem, solve_resident
-    from namics_amd.workloads import batch_draws
-    draws = batch_draws(n, seed)
+    from namics_amd.workloads import batch_draws, charged_draws
+    charged = args.workload == "charged"
+    if seed is None:
+        seed = 2024 if charged else 12345
+    draws = charged_draws(n, seed) if charged else batch_draws(n, seed)
     systems = [ScfSystem(batch_input(d)) for d in draws]
     synchronize()
 
@@ -105,8 +118,11 @@ def run_batch(args, n=512, seed=12345):
         "value": len(conv) / elapsed, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded parameter draws)",
-        "config": {"workload": "512 independent 1-gradient adsorption systems in one launch (layers 50-400, "
-                               "N 20-2000, half Markov 2), DIIS 1e-7, FP64", "seed": seed},
+        "config": {"workload": ("512 independent charged 1-gradient systems in one launch (polyelectrolyte "
+                                "adsorption with counterions and salt, layers 30-200, N 20-500, half Markov 2; "
+                                "the potential solved with the fields), DIIS 1e-7, FP64") if charged else
+                               ("512 independent 1-gradient adsorption systems in one launch (layers 50-400, "
+                                "N 20-2000, half Markov 2), DIIS 1e-7, FP64"), "seed": seed},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "converged_fraction": len(conv) / len(flat),
         "slowest": dict(draws[slow], solve_ms=recs[-1][slow][4], iterations=recs[-1][slow][0],

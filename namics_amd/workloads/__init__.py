@@ -124,6 +124,63 @@ newton : isaac : iterationlimit : 5000
 """.format(L=layers, cs=chi_s, c=chi, N=N, pb=phib, extra=extra, tol=tol)
 
 
+def polyelectrolyte_adsorption(layers, N, valence, chi_s, salt, theta, markov=1, k_stiff=1.0, tol=1e-7):
+    """1-gradient planar adsorption of a charged homopolymer: frozen
+    uncharged surface S on the lower ghost layer, (A)N restricted at
+    `theta` with `valence` per segment, counterions Na (+1) as the
+    neutralizer, salt Cl (-1) free at volume fraction `salt`, monomeric
+    solvent W, bond length 3e-10 m."""
+    extra = ""
+    if markov == 2:
+        extra = "mol : pol : Markov : 2\nmol : pol : k_stiff : %r\n" % k_stiff
+    return """
+lat : flat : gradients : 1
+lat : flat : n_layers : {L}
+lat : flat : lattice_type : simple_cubic
+lat : flat : lowerbound : surface
+lat : flat : bondlength : 3e-10
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : A : freedom : free
+mon : W : freedom : free
+mon : Na : freedom : free
+mon : Cl : freedom : free
+mon : A : valence : {v!r}
+mon : Na : valence : 1
+mon : Cl : valence : -1
+mon : S : chi_A : {cs!r}
+mol : pol : composition : (A){N}
+mol : pol : freedom : restricted
+mol : pol : theta : {th!r}
+{extra}mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : {salt!r}
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+newton : isaac : method : DIIS
+newton : isaac : tolerance : {tol!r}
+newton : isaac : iterationlimit : 5000
+""".format(L=layers, v=valence, cs=chi_s, N=N, th=theta, salt=salt, extra=extra, tol=tol)
+
+
+def charged_draws(n, seed=2024):
+    """Charged 1-gradient draws: layers U{30..200}, N log-uniform in
+    [20, 500], valence U[-1, -0.05], chi_S U[-1.5, 0], salt log-uniform in
+    [1e-3, 1e-1], adsorbed amount theta U[0.5, 5]; odd draws second-order
+    Markov with k_stiff = 1."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        out.append(dict(kind="charged", layers=int(rng.integers(30, 201)),
+                        N=int(round(float(np.exp(rng.uniform(np.log(20), np.log(500)))))),
+                        valence=float(rng.uniform(-1.0, -0.05)), chi_s=float(rng.uniform(-1.5, 0.0)),
+                        salt=float(np.exp(rng.uniform(np.log(1e-3), np.log(1e-1)))),
+                        theta=float(rng.uniform(0.5, 5.0)), markov=2 if i % 2 == 1 else 1))
+    return out
+
+
 def batch_draws(n, seed=12345):
     """Configuration-4 style draws: layers U{50..400}, N log-uniform in
     [20,2000], chi U[0.3,1.2], chi_S U[-1.5,0], phib log-uniform in
