@@ -1416,8 +1416,8 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     return false;
   }
   if (s.charged) {
-    if (l.gradients != 1) {
-      err = "charged systems are supported on 1-gradient lattices";
+    if (l.gradients != 1 && l.fjc != 1) {
+      err = "charged systems on 2- and 3-gradient lattices need fjc = 1";
       return false;
     }
     if (s.neutralizer < 0) {
@@ -1440,6 +1440,10 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     for (int i = 0; i < nseg; i++) {
       if (s.seg[i].epsilon != s.seg[0].epsilon) s.grad_epsilon = true;
       if (s.seg[i].freedom == SEG_FROZEN && s.seg[i].fixed_psi0) s.fixed_psi0 = true;
+    }
+    if (s.fixed_psi0 && l.gradients != 1) {
+      err = "fixed surface potentials (e.psi0/kT) are supported on 1-gradient lattices";
+      return false;
     }
   }
   compute_lambdas(l, any_markov2);

@@ -1461,8 +1461,45 @@ struct Oracle {
 
   // LGrad1::set_M_bounds (src/LGrad1.cpp:820-832): mirrored ghosts whatever
   // the boundary conditions
+  // LGrad2::set_M_bounds (src/LGrad2.cpp:1702-1723) and LGrad3::set_M_bounds
+  // (src/LGrad3.cpp:1679-1702) with fjc = 1: faces mirrored; in 2-D the
+  // corners from the x-ghost rows, in 3-D the edges left as they are.
   void set_M_bounds(double *X) const {
-    const int f = s.lat.fjc, MX = s.lat.MX;
+    const LatticeSpec &l = s.lat;
+    const int f = l.fjc, MX = l.MX, MY = l.MY, MZ = l.MZ, JX = l.JX, JY = l.JY;
+    if (l.gradients == 2) {
+      for (int x = 1; x < MX + 1; x++) {
+        X[x * JX + 0] = X[x * JX + 1];
+        X[x * JX + MY + 1] = X[x * JX + MY];
+      }
+      for (int y = 1; y < MY + 1; y++) {
+        X[0 + y] = X[1 * JX + y];
+        X[(MX + 1) * JX + y] = X[MX * JX + y];
+      }
+      X[0] = X[1];
+      X[MY + 1] = X[MY];
+      X[(MX + 1) * JX + 0] = X[(MX + 1) * JX + 1];
+      X[(MX + 1) * JX + MY + 1] = X[(MX + 1) * JX + MY];
+      return;
+    }
+    if (l.gradients == 3) {
+      for (int x = 1; x < MX + 1; x++)
+        for (int y = 1; y < MY + 1; y++) {
+          X[x * JX + y * JY + 0] = X[x * JX + y * JY + 1];
+          X[x * JX + y * JY + MZ + 1] = X[x * JX + y * JY + MZ];
+        }
+      for (int y = 1; y < MY + 1; y++)
+        for (int z = 1; z < MZ + 1; z++) {
+          X[0 + y * JY + z] = X[1 * JX + y * JY + z];
+          X[(MX + 1) * JX + y * JY + z] = X[MX * JX + y * JY + z];
+        }
+      for (int z = 1; z < MZ + 1; z++)
+        for (int x = 1; x < MX + 1; x++) {
+          X[x * JX + 0 + z] = X[x * JX + 1 * JY + z];
+          X[x * JX + (MY + 1) * JY + z] = X[x * JX + MY * JY + z];
+        }
+      return;
+    }
     for (int k = 0; k < f; k++) {
       X[k] = X[2 * f - k - 1];
       X[MX + f + k] = X[MX + f - k - 1];
@@ -1480,6 +1517,57 @@ struct Oracle {
     Zero(EE.data(), M);
     const int geo = s.lat.geometry;
     const double off = s.lat.offset_first_layer;  // offset_first_layer * fjc
+    const int MY = s.lat.MY, JX = s.lat.JX, JY = s.lat.JY;
+    if (s.lat.gradients == 3) {
+      // LGrad3::UpdateEE (src/LGrad3.cpp:1131-1141): AddGradSquare along
+      // each axis over the flattened array, then the prefactor
+      for (int i = 0; i < M - 2; i++) EE[i + 1] += std::pow(P[i] - P[i + 1], 2) + std::pow(P[i + 1] - P[i + 2], 2);
+      for (int i = 0; i < M - 2 * JX; i++)
+        EE[i + JX] += std::pow(P[i] - P[i + JX], 2) + std::pow(P[i + JX] - P[i + 2 * JX], 2);
+      for (int i = 0; i < M - 2 * JY; i++)
+        EE[i + JY] += std::pow(P[i] - P[i + JY], 2) + std::pow(P[i + JY] - P[i + 2 * JY], 2);
+      Norm(EE.data(), pf, M);
+      return;
+    }
+    if (s.lat.gradients == 2 && geo == GEOM_PLANAR) {
+      // LG2Planar::UpdateEE (src/LG2Planar.cpp:1130-1152)
+      pf = pf / 2.0;
+      for (int x = f; x < MX + f; x++)
+        for (int y = f; y < MY + f; y++) {
+          const int z = x * JX + y;
+          double Exmin = P[z] - P[z - JX];
+          Exmin *= Exmin;
+          double Explus = P[z] - P[z + JX];
+          Explus *= Explus;
+          double Eymin = P[z] - P[z - 1];
+          Eymin *= Eymin;
+          double Eyplus = P[z] - P[z + 1];
+          Eyplus *= Eyplus;
+          EE[z] = pf * (Exmin + Explus + Eymin + Eyplus);
+        }
+      return;
+    }
+    if (s.lat.gradients == 2) {
+      // LGrad2::UpdateEE (src/LGrad2.cpp:1392-1417), integer radius
+      pf = pf / 2;
+      int r = (int)off;
+      for (int x = f; x < MX + f; x++) {
+        r++;
+        for (int y = f; y < MY + f; y++) {
+          const int z = x * JX + y;
+          double Exmin = P[z] - P[z - JX];
+          Exmin *= (r - 0.5) * Exmin * 2 * kPIE;
+          double Explus = P[z] - P[z + JX];
+          Explus *= (r + 0.5) * Explus * 2 * kPIE;
+          double Eymin = (P[z] - P[z - 1]) * f;
+          Eymin *= Eymin;
+          double Eyplus = (P[z] - P[z + 1]) * f;
+          Eyplus *= Eyplus;
+          EE[z] = pf * ((Exmin + Explus) / s.lat.L[z] + Eymin + Eyplus);
+        }
+      }
+      return;
+    }
     if (geo == GEOM_PLANAR) {
       pf = pf / 2.0 * f * f;
       double Explus = P[f - 1] - P[f];
@@ -1654,6 +1742,76 @@ struct Oracle {
     double *X = Xel.data();
     const int geo = s.lat.geometry;
     const double off = s.lat.offset_first_layer;
+    const int MY = s.lat.MY, MZ = s.lat.MZ, JX = s.lat.JX, JY = s.lat.JY;
+    if (s.lat.gradients == 3) {
+      // LGrad3::UpdatePsi (src/LGrad3.cpp:1150-1224): symmetric Gauss-Seidel,
+      // a forward lexicographic sweep and a backward one, in place; g -= psi
+      // in the backward sweep
+      double *Q = psi.data();
+      auto site = [&](int x, int y, int z, double epsZmin, double epsZplus) {
+        const int i = x * JX + y * JY + z;
+        const double epsYmin = E[i] + E[i - JY], epsYplus = E[i] + E[i + JY];
+        const double epsXmin = E[i] + E[i - JX], epsXplus = E[i] + E[i + JX];
+        Q[i] = (epsXmin * Q[i - JX] + epsXplus * Q[i + JX] + epsYmin * Q[i - JY] + epsYplus * Q[i + JY] +
+                epsZmin * Q[i - 1] + epsZplus * Q[i + 1] + C * q[i]) /
+               (epsXmin + epsXplus + epsYmin + epsYplus + epsZmin + epsZplus);
+      };
+      for (int x = 1; x < MX + 1; x++)
+        for (int y = 1; y < MY + 1; y++) {
+          double epsZplus = E[x * JX + y * JY] + E[x * JX + y * JY + 1];
+          for (int z = 1; z < MZ + 1; z++) {
+            const double epsZmin = epsZplus;
+            epsZplus = E[x * JX + y * JY + z] + E[x * JX + y * JY + z + 1];
+            site(x, y, z, epsZmin, epsZplus);
+          }
+        }
+      for (int x = MX; x > 0; x--)
+        for (int y = MY; y > 0; y--) {
+          double epsZmin = E[x * JX + y * JY + MZ + 1] + E[x * JX + y * JY + MZ];
+          for (int z = MZ; z > 0; z--) {
+            const double epsZplus = epsZmin;
+            epsZmin = E[x * JX + y * JY + z] + E[x * JX + y * JY + z - 1];
+            site(x, y, z, epsZmin, epsZplus);
+            gp[x * JX + y * JY + z] -= Q[x * JX + y * JY + z];
+          }
+        }
+      return;
+    }
+    if (s.lat.gradients == 2 && geo == GEOM_PLANAR) {
+      // LG2Planar::UpdatePsi (src/LG2Planar.cpp:1163-1181): upwind in x and y
+      C = C * 2.0 / f / f;
+      for (int x = f; x < MX + f; x++)
+        for (int y = f; y < MY + f; y++) {
+          const int i = x * JX + y;
+          const double epsXmin = E[i] + E[i - JX], epsXplus = E[i] + E[i + JX];
+          const double epsYmin = E[i] + E[i - 1], epsYplus = E[i] + E[i + 1];
+          const double ax = x == f ? P[i - JX] : X[i - JX];
+          const double ay = y == f ? P[i - 1] : X[i - 1];
+          X[i] = (C * q[i] + epsXmin * ax + epsXplus * P[i + JX] + epsYmin * ay + epsYplus * P[i + 1]) /
+                 (epsXmin + epsXplus + epsYmin + epsYplus);
+        }
+      for (int z = 0; z < M; z++) gp[z] = gp[z] - X[z];
+      return;
+    }
+    if (s.lat.gradients == 2) {
+      // LGrad2::UpdatePsi (src/LGrad2.cpp:1428-1445): Jacobi
+      C = C / 2 / f / f;
+      double r = off;
+      for (int x = f; x < MX + f; x++) {
+        r++;
+        for (int y = f; y < MY + f; y++) {
+          const int i = x * JX + y;
+          const double Li = s.lat.L[i];
+          const double epsXmin = 2 * kPIE * (r - 1) * (E[i] + E[i - JX]) / Li * f * f;
+          const double epsXplus = 2 * kPIE * r * (E[i] + E[i + JX]) / Li * f * f;
+          const double epsYmin = E[i] + E[i - 1], epsYplus = E[i] + E[i + 1];
+          X[i] = (C * q[i] + epsXmin * P[i - JX] + epsXplus * P[i + JX] + epsYmin * P[i - 1] + epsYplus * P[i + 1]) /
+                 (epsXmin + epsXplus + epsYmin + epsYplus);
+        }
+      }
+      for (int z = 0; z < M; z++) gp[z] = gp[z] - X[z];
+      return;
+    }
     if (geo == GEOM_PLANAR) {
       C = C * 2.0 / f / f;
       double epsXplus = E[f - 1] + E[f];

@@ -11,7 +11,8 @@ lattices with an offset; stiff (Markov 2) charged chains between two
 walls, against one wall with a mirrored far boundary, and on a refined
 hexagonal lattice; a refined planar hexagonal lattice (fjc 2); frozen plates that fix the potential
 (`e.psi0/kT`, planar and cylindrical, the latter with segments of
-different dielectric constant).
+different dielectric constant); 2-gradient cylindrical and planar lattices
+and a 3-gradient box with frozen particles.
 """
 
 WALL = """
@@ -243,6 +244,92 @@ mol : Cl : freedom : free
 mol : Cl : phibulk : 0.01
 """,
         eval=[(110, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # 2-gradient cylindrical lattice: a charged polymer around a frozen rod
+    # (the Jacobi potential sweep of LGrad2)
+    "el2d_cyl": dict(
+        text="""
+lat : cyl : gradients : 2
+lat : cyl : geometry : cylindrical
+lat : cyl : n_layers_x : 12
+lat : cyl : n_layers_y : 10
+lat : cyl : lattice_type : simple_cubic
+lat : cyl : bondlength : 5e-10
+mon : S : freedom : frozen
+mon : S : frozen_range : 1,1;2,10
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.3
+mon : S : chi_A : -1.0
+mol : pol : composition : (A)15
+mol : pol : freedom : restricted
+mol : pol : theta : 40
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.02
+""",
+        eval=[(111, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # 2-gradient planar lattice with a frozen particle, periodic in y, and a
+    # low-epsilon polymer (the upwind sweep of LG2Planar in x and y)
+    "el2d_planar": dict(
+        text="""
+lat : L : gradients : 2
+lat : L : n_layers_x : 10
+lat : L : n_layers_y : 12
+lat : L : lattice_type : simple_cubic
+lat : L : lowerbound_y : periodic
+lat : L : upperbound_y : periodic
+lat : L : bondlength : 3e-10
+mon : S : freedom : frozen
+mon : S : frozen_range : 4,5;6,7
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.4
+mon : A : epsilon : 20
+mon : S : chi_A : -1.2
+mol : pol : composition : (A)12
+mol : pol : freedom : restricted
+mol : pol : theta : 6
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.01
+""",
+        eval=[(112, 0.2)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # 3-gradient box with a frozen particle (the symmetric Gauss-Seidel
+    # sweep of LGrad3)
+    "el3d_box": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 8
+lat : box : n_layers_y : 7
+lat : box : n_layers_z : 9
+lat : box : lattice_type : simple_cubic
+lat : box : bondlength : 3e-10
+mon : S : freedom : frozen
+mon : S : frozen_range : 3,3,4;4,5,5
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.3
+mon : S : chi_A : -1.0
+mol : pol : composition : (A)10
+mol : pol : freedom : restricted
+mol : pol : theta : 12
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.01
+""",
+        eval=[(113, 0.2)],
         converged=[dict(method="DIIS", tol=1e-8)],
     ),
     # a frozen plate at the first layer fixing the potential, a charged

@@ -166,10 +166,11 @@ def test_charged_input_refusals():
                                          "mol : Na : freedom : free\nmol : Na : phibulk : 0.01\n")
     # charges without a bond length
     assert "bondlength" in _parse_error(ok.replace("lat : flat : bondlength : 3e-10\n", ""))
-    # charged 2-gradient lattices are outside this build
+    # charged refined 2-gradient lattices are outside this build
     two = ok.replace("lat : flat : gradients : 1\nlat : flat : n_layers : 20",
-                     "lat : flat : gradients : 2\nlat : flat : n_layers_x : 10\nlat : flat : n_layers_y : 10")
-    assert "1-gradient" in _parse_error(two)
+                     "lat : flat : gradients : 2\nlat : flat : n_layers_x : 10\nlat : flat : n_layers_y : 10\n"
+                     "lat : flat : FJC_choices : 5")
+    assert "fjc = 1" in _parse_error(two.replace("simple_cubic", "hexagonal"))
     # a fixed surface potential on a frozen segment; on a free one it is refused
     wall = ok + "mon : S : freedom : frozen\nmon : S : frozen_range : 1;1\nmon : S : e.psi0/kT : 2\n"
     assert describe_input(wall)["fixed_psi0"]
