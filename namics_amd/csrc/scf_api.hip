@@ -116,11 +116,19 @@ int validate(const SystemSpec &s, std::string &why) {
   if (l.gradients == 2 && l.geometry == GEOM_PLANAR && !l.stencil_full && l.lattice_type != LAT_SIMPLE_CUBIC)
     return why = "2-gradient planar hexagonal stencil without stencil_full is not supported", SCF_E_UNSUPPORTED;
   if (s.charged) {
-    // the potential runs in the one-workgroup 1-gradient kernels
-    if (l.gradients != 1) return why = "charged systems are supported on 1-gradient lattices", SCF_E_UNSUPPORTED;
+    // the potential runs in the one-workgroup kernels; on 2 and 3 gradients
+    // the chains run over every site in the reference's array form
     for (auto &m : s.mol)
       if (m.scheduled())
         return why = "charged systems with branched molecules are not supported", SCF_E_UNSUPPORTED;
+    if (l.gradients >= 2) {
+      for (auto &m : s.mol)
+        if (m.markov != 1 || m.save_memory)
+          return why = "charged 2- and 3-gradient systems need first-order chains without save_memory",
+                 SCF_E_UNSUPPORTED;
+      if (l.lattice_type != LAT_SIMPLE_CUBIC || !l.stencil_full)
+        return why = "charged 2- and 3-gradient systems need the full simple-cubic stencil", SCF_E_UNSUPPORTED;
+    }
     if ((int)s.itmon.size() + 1 > kMaxSeg + 1) return why = "too many field blocks", SCF_E_UNSUPPORTED;
   }
   return SCF_OK;
@@ -191,6 +199,7 @@ void plan_layout(scf_system *S, int want_sx) {
   // Markov-1 chains, the 9-point stencil, ghosts that map onto interior
   // sites, and the rows of one strip layout in LDS
   const bool strip_ok = l.gradients == 2 && l.fjc == 1 && maxR == 1 && (l.geometry != 0 || l.stencil_full) &&
+                        !S->spec.charged &&
                         l.MX + 2 <= 1026 && h.bx1 >= 1 && h.bx1 <= l.MX && h.bxm >= 1 && h.bxm <= l.MX &&
                         h.by1 >= 1 && h.by1 <= l.MY && h.bym >= 1 && h.bym <= l.MY;
   for (int sx : {8, 4}) {
@@ -282,6 +291,7 @@ int build(scf_system *S, const Placement *pl) {
   h.nit = s.nblocks_x();  // field blocks, plus the potential of a charged system
   h.nitm = nit;
   h.charged = s.charged ? 1 : 0;
+  h.all_sites = s.charged && l.gradients >= 2 ? 1 : 0;
   h.neutralizer = s.charged ? s.neutralizer : -1;
   if (s.charged) {
     // constants of the reference (src/namics.cpp:43-49); the bond length is
@@ -292,7 +302,20 @@ int build(scf_system *S, const Placement *pl) {
     double pf = 0.5 * eps0 * b / k_BT * (k_BT / e) * (k_BT / e);
     // UpdatePsi (src/LG1Planar.cpp:185-188, src/LGrad1.cpp:659-680)
     double C = e * e / (eps0 * k_BT * b);
-    if (l.geometry == GEOM_PLANAR) {
+    if (l.gradients == 3) {
+      // LGrad3::UpdateEE / UpdatePsi (src/LGrad3.cpp:1131-1141,1150-1224)
+      h.el_ri = 0;
+    } else if (l.gradients == 2 && l.geometry == GEOM_PLANAR) {
+      // LG2Planar::UpdateEE / UpdatePsi (src/LG2Planar.cpp:1130-1181)
+      pf = pf / 2.0;
+      C = C * 2.0 / f / f;
+      h.el_ri = 0;
+    } else if (l.gradients == 2) {
+      // LGrad2::UpdateEE / UpdatePsi (src/LGrad2.cpp:1392-1445)
+      pf = pf / 2;
+      C = C / 2 / f / f;
+      h.el_ri = (int)l.offset_first_layer;
+    } else if (l.geometry == GEOM_PLANAR) {
       pf = pf / 2.0 * f * f;
       C = C * 2.0 / f / f;
       h.el_ri = 0;

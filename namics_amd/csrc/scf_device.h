@@ -130,6 +130,7 @@ struct DevSys {
   double deltamax, tol;
   // ---- execution
   int use_lds;            // propagator rows fit the LDS budget of the launch
+  int all_sites;          // 2 and 3 gradients: phases visit ghosts too (charged systems, literal chains)
   int row_stride;         // doubles between the two LDS row buffers
   int strip_sx;           // 2-gradient strip path: sites per thread along x (0: not used)
   int strip_stride;       // 2-gradient strip path: LDS slots per strip column
@@ -203,10 +204,11 @@ __device__ __forceinline__ int map_z(const Sys &S, int z) {
 }
 
 // number of sites a phase visits: all sites in 1 gradient (ghosts included,
-// for surface boundaries), interior sites otherwise
+// for surface boundaries) and for charged systems (whose potential reads
+// densities on ghosts), interior sites otherwise
 template <typename Sys>
 __device__ __forceinline__ int n_work_sites(const Sys &S) {
-  return S.gradients == 1 ? S.M : S.MX * S.MY * (S.gradients == 3 ? S.MZ : 1);
+  return S.gradients == 1 || S.all_sites ? S.M : S.MX * S.MY * (S.gradients == 3 ? S.MZ : 1);
 }
 
 // work index -> lattice site (and its coordinates)
@@ -215,6 +217,18 @@ __device__ __forceinline__ int work_site(const Sys &S, int w, int &x, int &y, in
   if (S.gradients == 1) {
     x = w;
     y = z = 0;
+    return w;
+  }
+  if (S.all_sites) {  // every site, ghosts included, in memory order
+    x = w / S.JX;
+    const int r = w - x * S.JX;
+    if (S.gradients == 2) {
+      y = r;
+      z = 0;
+    } else {
+      y = r / S.JY;
+      z = r - y * S.JY;
+    }
     return w;
   }
   if (S.gradients == 2) {
@@ -832,6 +846,71 @@ __device__ __forceinline__ double el_psi(const Sys &S, int x) {
   return hbm(S.x)[S.nitm * S.M + el_mirror(S, x)];
 }
 
+// coordinates of memory index i (x-major, z contiguous)
+template <typename Sys>
+__device__ __forceinline__ void site_coords(const Sys &S, int i, int &x, int &y, int &z) {
+  x = i / S.JX;
+  const int r = i - x * S.JX;
+  if (S.gradients == 2) {
+    y = r;
+    z = 0;
+  } else {
+    y = r / S.JY;
+    z = r - y * S.JY;
+  }
+}
+
+// psi at site (x, y, z) of a 2- or 3-gradient lattice with the ghosts of
+// set_M_bounds (src/LGrad2.cpp:1702-1723, src/LGrad3.cpp:1679-1702, fjc = 1):
+// faces mirrored; 2 gradients also the corners, 3 gradients leave edges and
+// corners as the field has them.
+template <typename Sys>
+__device__ __forceinline__ double el_psi_xyz(const Sys &S, int x, int y, int z) {
+  const gdbl *P = hbm(S.x) + S.nitm * S.M;
+  auto mir = [](int c, int n) { return c < 1 ? 1 : (c > n ? n : c); };
+  if (S.gradients == 2) return P[mir(x, S.MX) * S.JX + mir(y, S.MY)];
+  const int nghost = (x < 1 || x > S.MX) + (y < 1 || y > S.MY) + (z < 1 || z > S.MZ);
+  if (nghost >= 2) return P[x * S.JX + y * S.JY + z];
+  return P[mir(x, S.MX) * S.JX + mir(y, S.MY) * S.JY + mir(z, S.MZ)];
+}
+
+// Field energy density at an interior site of a 2- or 3-gradient lattice
+// (LG2Planar::UpdateEE src/LG2Planar.cpp:1130-1152, LGrad2::UpdateEE
+// src/LGrad2.cpp:1392-1417 with its integer radius, LGrad3::UpdateEE
+// src/LGrad3.cpp:1131-1141: squared differences along each axis).
+template <typename Sys>
+__device__ __forceinline__ double el_EE_xyz(const Sys &S, int i, int x, int y, int z) {
+  const double p0 = el_psi_xyz(S, x, y, z);
+  const double xm = el_psi_xyz(S, x - 1, y, z), xp = el_psi_xyz(S, x + 1, y, z);
+  const double ym = el_psi_xyz(S, x, y - 1, z), yp = el_psi_xyz(S, x, y + 1, z);
+  if (S.gradients == 3) {
+    const double zm = el_psi_xyz(S, x, y, z - 1), zp = el_psi_xyz(S, x, y, z + 1);
+    double e = 0.0;
+    e += (zm - p0) * (zm - p0) + (p0 - zp) * (p0 - zp);
+    e += (xm - p0) * (xm - p0) + (p0 - xp) * (p0 - xp);
+    e += (ym - p0) * (ym - p0) + (p0 - yp) * (p0 - yp);
+    return e * S.el_pf;
+  }
+  if (S.geometry == 0) {
+    double a = p0 - xm, b = p0 - xp, c = p0 - ym, d = p0 - yp;
+    a *= a;
+    b *= b;
+    c *= c;
+    d *= d;
+    return S.el_pf * (a + b + c + d);
+  }
+  const int r = S.el_ri + (x - S.fjc + 1);
+  double a = p0 - xm;
+  a *= (r - 0.5) * a * 2 * 3.14159265;
+  double b = p0 - xp;
+  b *= (r + 0.5) * b * 2 * 3.14159265;
+  double c = (p0 - ym) * S.fjc;
+  c *= c;
+  double d = (p0 - yp) * S.fjc;
+  d *= d;
+  return S.el_pf * ((a + b) / hbm(S.L)[i] + c + d);
+}
+
 // Field energy density at site x, zero on ghosts (LG1Planar::UpdateEE
 // src/LG1Planar.cpp:162-178; LGrad1::UpdateEE src/LGrad1.cpp:592-651, where
 // the cylindrical and spherical radii are integers).
@@ -872,7 +951,12 @@ __device__ __forceinline__ double el_EE(const Sys &S, int x) {
 template <typename Sys>
 __device__ __forceinline__ void site_electro(const Sys &S, int i) {
   const int M = S.M;
-  const int b = map_x1(S, i);
+  int b = map_x1(S, i);
+  if (S.gradients >= 2) {  // the bounded source, coordinate by coordinate
+    int x, y, z;
+    site_coords(S, i, x, y, z);
+    b = S.gradients == 2 ? map_x(S, x) * S.JX + map_y(S, y) : map_x(S, x) * S.JX + map_y(S, y) * S.JY + map_z(S, z);
+  }
   double q = 0, e = 0;
 #pragma unroll
   for (int k = 0; k < kMaxSeg; k++) {
@@ -884,8 +968,16 @@ __device__ __forceinline__ void site_electro(const Sys &S, int i) {
   const double pt = hbm(S.phitot)[i];
   hbm(S.q)[i] = pt != 0 ? q / pt : 0.0;
   hbm(S.eps)[i] = pt != 0 ? e / pt : 0.0;
-  hbm(S.psi)[i] = el_psi(S, i == 0 ? 1 : i);
-  hbm(S.EE)[i] = el_EE(S, i);
+  if (S.gradients == 1) {
+    hbm(S.psi)[i] = el_psi(S, i == 0 ? 1 : i);
+    hbm(S.EE)[i] = el_EE(S, i);
+    return;
+  }
+  int x, y, z;
+  site_coords(S, i == 0 ? 1 : i, x, y, z);
+  hbm(S.psi)[i] = el_psi_xyz(S, x, y, z);
+  if (i == 0) site_coords(S, i, x, y, z);
+  hbm(S.EE)[i] = is_interior(S, x, y, z) ? el_EE_xyz(S, i, x, y, z) : 0.0;
 }
 
 // One site of the potential sweep (LG1Planar::UpdatePsi src/LG1Planar.cpp:181-199,
@@ -938,9 +1030,14 @@ __device__ __forceinline__ void site_prepare(const Sys &S, int i, int x, int y, 
   // charged systems: u also carries -epsilon EE + valence psi at the source
   // site (System::PutU src/system.cpp:2001-2040)
   double EEb = 0.0, psib = 0.0;
-  if (S.charged) {
+  if (S.charged && S.gradients == 1) {
     EEb = el_EE(S, b);
     psib = el_psi(S, b);
+  } else if (S.charged) {  // the source is an interior site (mirror and periodic boundaries)
+    int bx, by, bz;
+    site_coords(S, b, bx, by, bz);
+    EEb = el_EE_xyz(S, b, bx, by, bz);
+    psib = el_psi_xyz(S, bx, by, bz);
   }
   for (int k = 0; k < S.nseg; k++) {
     const int blk = S.seg_block[k];

@@ -501,6 +501,149 @@ __device__ __noinline__ void chain_markov1(const DevSys &S0, int mo) {
   __syncthreads();
 }
 
+// Chains of a charged 2- or 3-gradient system over every site, ghosts
+// included, in the reference's array form (Markov 1, fjc = 1): before each
+// step the previous row's ghosts take their bounded values in place
+// (set_bounds), the step is the reference's passes over the flattened array
+// (each pass a shifted copy over the index range the array allows, so the
+// ghost sites get the partial sums the reference gives them), and the
+// composition adds Gf Gb on every site.  The last forward row's ghosts are
+// cleared before GN, as ComputeGN's remove_bounds clears them.  The charge
+// and dielectric constant read the total density on ghosts, which is why
+// these systems need the ghosts computed the reference's way.  One
+// workgroup, rows in HBM.
+// LGrad3::propagate (src/LGrad3.cpp:731-743), LG2Planar::propagate
+// (src/LG2Planar.cpp:1039-1127, full simple-cubic stencil), LGrad2::propagate
+// (src/LGrad2.cpp:1053-1073), Molecule::propagate_forward / backward
+// (src/molecule.cpp:2300-2446).
+template <typename Sys>
+__device__ __forceinline__ double literal_step(const Sys &S, const gdbl *g, int i) {
+  const int M = S.M, JX = S.JX;
+  if (S.gradients == 3) {
+    const int JY = S.JY;
+    double t = 0.0;
+    if (i >= JX) t += g[i - JX];
+    if (i < M - JX) t += g[i + JX];
+    if (i >= JY) t += g[i - JY];
+    if (i < M - JY) t += g[i + JY];
+    if (i >= 1) t += g[i - 1];
+    if (i < M - 1) t += g[i + 1];
+    return t * (1.0 / 6.0);
+  }
+  if (S.geometry == 0) {
+    const double C1 = 16.0 / 36.0, C2 = 4.0 / 36.0, C3 = 1.0 / 36.0;
+    double t = C1 * g[i];
+    if (i >= 1) t += C2 * g[i - 1];
+    if (i < M - 1) t += C2 * g[i + 1];
+    if (i >= JX) t += C2 * g[i - JX];
+    if (i < M - JX) t += C2 * g[i + JX];
+    if (i >= JX + 1) t += C3 * g[i - JX - 1];
+    if (i >= JX) t += C3 * g[i - JX + 1];
+    if (i >= 1 && i < M - JX + 1) t += C3 * g[i + JX - 1];
+    if (i < M - JX - 1) t += C3 * g[i + JX + 1];
+    return t;
+  }
+  const double lm = hbm(S.lam_m)[i], lp = hbm(S.lam_p)[i];
+  double t = (4.0 / 6.0) * g[i];
+  if (i >= JX) t += g[i - JX] * lm;
+  if (i < M - JX) t += g[i + JX] * lp;
+  if (i >= 1) t += (1.0 / 6.0) * g[i - 1];
+  if (i < M - 1) t += (1.0 / 6.0) * g[i + 1];
+  t = t * 4.0;
+  if (i >= JX + 1) t += g[i - JX - 1] * lm;
+  if (i >= JX && i < M - 1) t += g[i - JX + 1] * lm;
+  if (i >= 1 && i < M - JX) t += g[i + JX - 1] * lp;
+  if (i < M - JX - 1) t += g[i + JX + 1] * lp;
+  return t * (1.0 / 6.0);
+}
+
+// ghosts of a row take their bounded values (set_bounds; mirror and
+// periodic boundaries, whose sources are interior sites)
+template <typename Sys>
+__device__ __forceinline__ void literal_bounds(const Sys &S, gdbl *row, int tid, int nt) {
+  for (int i = tid; i < S.M; i += nt) {
+    int x, y, z;
+    site_coords(S, i, x, y, z);
+    if (is_interior(S, x, y, z)) continue;
+    const int b = S.gradients == 2 ? map_x(S, x) * S.JX + map_y(S, y)
+                                   : map_x(S, x) * S.JX + map_y(S, y) * S.JY + map_z(S, z);
+    row[i] = row[b];
+  }
+}
+
+__device__ __noinline__ void chain_literal(const DevSys &S0, int mo) {
+  CSys &S = uniform_sys(S0);
+  const int M = S.M;
+  const int tid = work_tid(S), nt = S.threads;
+  const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
+  const int N = S.mol_N[mo];
+  gdbl *Gf = hbm(S.Gf);
+  const gdbl *G1 = hbm(S.G1);
+  {
+    const gdbl *g1 = G1 + (size_t)S.block_seg[b0] * M;
+    for (int i = tid; i < M; i += nt) Gf[i] = g1[i];
+  }
+  __syncthreads();
+  int s = 1;
+  for (int b = b0; b < b1; b++) {
+    const gdbl *g1 = G1 + (size_t)S.block_seg[b] * M;
+    for (int j = (b == b0 ? 1 : 0); j < S.block_n[b]; j++, s++) {
+      gdbl *prev = Gf + (size_t)(s - 1) * M, *gs = Gf + (size_t)s * M;
+      literal_bounds(S, prev, tid, nt);
+      __syncthreads();
+      for (int i = tid; i < M; i += nt) gs[i] = literal_step(S, prev, i) * g1[i];
+      __syncthreads();
+    }
+  }
+  {
+    gdbl *gl = Gf + (size_t)(N - 1) * M;
+    double acc = 0;
+    for (int i = tid; i < M; i += nt) {
+      int x, y, z;
+      site_coords(S, i, x, y, z);
+      if (is_interior(S, x, y, z))
+        acc += site_weight(S0, i) * gl[i];
+      else
+        gl[i] = 0.0;  // remove_bounds
+    }
+    const double gn = wg_sum(acc);
+    if (threadIdx.x == 0) S0.sc->GN[mo] = gn;
+  }
+  gdbl *gbA = hbm(S.Gb), *gbB = hbm(S.Gb) + M;
+  {
+    const int bl = b1 - 1;
+    const gdbl *g1 = G1 + (size_t)S.block_seg[bl] * M;
+    gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[bl] * M;
+    const gdbl *gf = Gf + (size_t)(N - 1) * M;
+    for (int i = tid; i < M; i += nt) {
+      const double v = g1[i];
+      gbA[i] = v;
+      rho[i] += gf[i] * v;
+    }
+  }
+  __syncthreads();
+  s = N - 2;
+  for (int b = b1 - 1; b >= b0; b--) {
+    const gdbl *g1 = G1 + (size_t)S.block_seg[b] * M;
+    gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[b] * M;
+    for (int j = (b == b1 - 1 ? S.block_n[b] - 2 : S.block_n[b] - 1); j >= 0; j--, s--) {
+      const gdbl *gf = Gf + (size_t)s * M;
+      literal_bounds(S, gbA, tid, nt);
+      __syncthreads();
+      for (int i = tid; i < M; i += nt) {
+        const double v = literal_step(S, gbA, i) * g1[i];
+        gbB[i] = v;
+        rho[i] += gf[i] * v;
+      }
+      __syncthreads();
+      gdbl *t = gbA;
+      gbA = gbB;
+      gbB = t;
+    }
+  }
+  __syncthreads();
+}
+
 // 2-gradient Markov-1 chain, register-blocked.  Thread t < nact owns the
 // strip of SX sites (x0 .. x0+SX-1, y) with x0 = (t / MY) SX + 1 and
 // y = t % MY + 1, and keeps its own previous values in registers.  The rows
@@ -1299,6 +1442,95 @@ __device__ __noinline__ double electro_fixed(const DevSys &S0, double acc) {
   return acc;
 }
 
+// The potential sweep of a charged 2- or 3-gradient system, after the
+// charge, dielectric constant, psi and EE of every site are stored.
+// * 2-gradient cylindrical (LGrad2::UpdatePsi src/LGrad2.cpp:1428-1445):
+//   Jacobi, one site per thread;
+// * 2-gradient planar (LG2Planar::UpdatePsi src/LG2Planar.cpp:1163-1181):
+//   upwind in x and in y, X[x,y] reads X[x-1,y] and X[x,y-1] of the same
+//   sweep, so it runs as a wavefront over the anti-diagonals (one barrier
+//   each), which gives every site the values the reference's row-by-row loop
+//   gives it;
+// * 3-gradient (LGrad3::UpdatePsi src/LGrad3.cpp:1150-1224): a symmetric
+//   Gauss-Seidel sweep in place, forward in lexicographic order and then
+//   backward, g -= psi in the backward one.  Each sweep runs as a wavefront
+//   over the planes x + y + z = k: a site's lower neighbours lie on plane
+//   k - 1 (already updated, as in the lexicographic loop) and its upper ones
+//   on plane k + 1 (not yet), so the result is the reference's.
+// g = x - X (2 gradients) or x - the swept psi (3 gradients) on the interior.
+__device__ __noinline__ double electro_residual_xyz(const DevSys &S0, double acc) {
+  CSys &S = uniform_sys(S0);
+  const int M = S.M, MX = S.MX, MY = S.MY, MZ = S.MZ, JX = S.JX, JY = S.JY;
+  const int tid = work_tid(S), nt = S.threads;
+  const bool max_g = S.max_g;
+  gdbl *gp = hbm(S.g) + S.nitm * M;
+  const gdbl *xp = hbm(S.x) + S.nitm * M;
+  const gdbl *E = hbm(S.eps), *q = hbm(S.q);
+  gdbl *P = hbm(S.psi);
+  const double C = S.el_C;
+  auto add = [&](int i, double X) {
+    const double gv = xp[i] - X;
+    gp[i] = gv;
+    acc = max_g ? fmax(acc, fabs(gv)) : acc + gv * gv;
+  };
+  if (S.gradients == 2 && S.geometry != 0) {
+    for (int w = tid; w < MX * MY; w += nt) {
+      const int x = 1 + w / MY, y = 1 + w % MY, i = x * JX + y;
+      const double r = S.el_r0 + x;  // offset_first_layer fjc + (x - fjc + 1), fjc = 1
+      const double Li = hbm(S.L)[i];
+      const double epsXmin = 2 * 3.14159265 * (r - 1) * (E[i] + E[i - JX]) / Li;
+      const double epsXplus = 2 * 3.14159265 * r * (E[i] + E[i + JX]) / Li;
+      const double epsYmin = E[i] + E[i - 1], epsYplus = E[i] + E[i + 1];
+      const double X = (C * q[i] + epsXmin * P[i - JX] + epsXplus * P[i + JX] + epsYmin * P[i - 1] +
+                        epsYplus * P[i + 1]) /
+                       (epsXmin + epsXplus + epsYmin + epsYplus);
+      add(i, X);
+    }
+    return acc;
+  }
+  if (S.gradients == 2) {
+    gdbl *X = hbm(S.elA);
+    for (int d = 0; d <= MX + MY - 2; d++) {
+      const int x0 = d - (MY - 1) > 0 ? d - (MY - 1) : 0;  // x - 1 on the diagonal x + y - 2 = d
+      const int x1 = d < MX - 1 ? d : MX - 1;
+      for (int t = tid; t <= x1 - x0; t += nt) {
+        const int x = 1 + x0 + t, y = 1 + d - (x0 + t), i = x * JX + y;
+        const double epsXmin = E[i] + E[i - JX], epsXplus = E[i] + E[i + JX];
+        const double epsYmin = E[i] + E[i - 1], epsYplus = E[i] + E[i + 1];
+        const double ax = x == 1 ? P[i - JX] : X[i - JX];
+        const double ay = y == 1 ? P[i - 1] : X[i - 1];
+        const double v = (C * q[i] + epsXmin * ax + epsXplus * P[i + JX] + epsYmin * ay + epsYplus * P[i + 1]) /
+                         (epsXmin + epsXplus + epsYmin + epsYplus);
+        X[i] = v;
+        add(i, v);
+      }
+      __syncthreads();
+    }
+    return acc;
+  }
+  // 3 gradients: symmetric Gauss-Seidel by planes
+  for (int pass = 0; pass < 2; pass++) {
+    for (int kk = 0; kk <= MX + MY + MZ - 3; kk++) {
+      const int k = pass == 0 ? 3 + kk : MX + MY + MZ - kk;
+      for (int w = tid; w < MX * MY; w += nt) {
+        const int x = 1 + w / MY, y = 1 + w % MY, z = k - x - y;
+        if (z < 1 || z > MZ) continue;
+        const int i = x * JX + y * JY + z;
+        const double epsZmin = E[i] + E[i - 1], epsZplus = E[i] + E[i + 1];
+        const double epsYmin = E[i] + E[i - JY], epsYplus = E[i] + E[i + JY];
+        const double epsXmin = E[i] + E[i - JX], epsXplus = E[i] + E[i + JX];
+        const double v = (epsXmin * P[i - JX] + epsXplus * P[i + JX] + epsYmin * P[i - JY] + epsYplus * P[i + JY] +
+                          epsZmin * P[i - 1] + epsZplus * P[i + 1] + C * q[i]) /
+                         (epsXmin + epsXplus + epsYmin + epsYplus + epsZmin + epsZplus);
+        P[i] = v;
+        if (pass == 1) add(i, v);
+      }
+      __syncthreads();
+    }
+  }
+  return acc;
+}
+
 // The potential block of the residual of a charged 1-gradient system
 // (System::Classical_residual src/system.cpp:2161-2169): charge density,
 // dielectric constant, psi and EE at every site, then one sweep of the
@@ -1317,9 +1549,16 @@ __device__ __noinline__ double electro_residual(const DevSys &S0, double acc) {
   const gdbl *xp = hbm(S.x) + S.nitm * M;
   for (int i = tid; i < M; i += nt) {
     site_electro(S, i);
-    if (i < f || i >= MX + f) gp[i] = 0.0;
+    if (S.gradients == 1) {
+      if (i < f || i >= MX + f) gp[i] = 0.0;
+    } else {
+      int x, y, z;
+      site_coords(S, i, x, y, z);
+      if (!is_interior(S, x, y, z)) gp[i] = 0.0;
+    }
   }
   __syncthreads();
+  if (S.gradients >= 2) return electro_residual_xyz(S0, acc);
   if (S.fixed_psi0) return electro_fixed(S0, acc);
   const bool upwind = S.geometry != 2;
   for (int x = f + tid; x < MX + f; x += nt) {
@@ -1408,6 +1647,8 @@ __device__ double fused_residual(const DevSys &S0) {
         else
           chain_markov2<false, M2_D3>(S0, mo);
       }
+    } else if (S.all_sites) {
+      chain_literal(S0, mo);
     } else if (S.strip_sx > 0) {
       chain_strip_dispatch(S0, mo, fp32, fp32 ? sc.lnshift[mo] : 0.0);
     } else if (S.gradients == 1 && S.fjc == 1 && S.use_lds && S.M <= S.threads) {

@@ -53,6 +53,8 @@ struct scf_system {
     if (mode != SCF_MODE_AUTO) return mode;
     // branched molecules and dendrimers run their row schedules on the grid path
     if (branched()) return SCF_MODE_GRID;
+    // charged systems solve the potential in the one-workgroup kernels
+    if (spec.charged) return SCF_MODE_FUSED;
     // checkpointed end-point storage runs on the strip chains or the grid path
     if (h.ckpt > 1 && h.strip_sx == 0) return SCF_MODE_GRID;
     if (tuned_mode != SCF_MODE_AUTO) return tuned_mode;
