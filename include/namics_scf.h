@@ -85,10 +85,11 @@ typedef struct {
  * @water are refused with SCF_E_PARSE and a message naming them.  Charged
  * systems (`mon : <name> : valence`, `epsilon`, `lat : <name> : bondlength`,
  * a molecule with freedom `neutralizer`) solve for the electrostatic
- * potential as one more field block on 1-gradient lattices (fused path),
- * with fixed surface potentials on frozen segments (`e.psi0/kT`) computed as
- * the reference computes them; other lattices and charged branched
- * molecules are refused.
+ * potential as one more field block on the fused path: on 1-gradient
+ * lattices (with fixed surface potentials on frozen segments, `e.psi0/kT`,
+ * computed as the reference computes them) and on simple-cubic 2- and
+ * 3-gradient lattices with first-order chains (one workgroup per system);
+ * charged branched molecules are refused.
  * Replaces: Input + Lattice/Segment/Molecule/System/Solve_scf::CheckInput and
  * AllocateMemory (src/namics.cpp:153-417, src/lattice.cpp:267-601), the
  * composition parser Molecule::Decomposition / GenerateTree / Interpret
