@@ -110,9 +110,20 @@ int validate(const SystemSpec &s, std::string &why) {
     for (auto &m : s.mol)
       if (m.markov == 2)
         return why = "save_memory with second-order (Markov 2) chains is not supported", SCF_E_UNSUPPORTED;
-  if (l.gradients >= 2)
-    for (int d = 0; d < 6; d++)
-      if (l.bc[d] == BC_SURFACE) return why = "surface boundaries are supported with 1 gradient only", SCF_E_UNSUPPORTED;
+  bool surface = false;
+  for (int d = 0; d < 6; d++) surface = surface || l.bc[d] == BC_SURFACE;
+  if (l.gradients >= 2 && (surface || s.charged)) {
+    // the chains over every site in the reference's array form (chain_literal)
+    for (auto &m : s.mol)
+      if (m.markov != 1 || m.save_memory || m.scheduled())
+        return why = "2- and 3-gradient systems with surface boundaries or charges need first-order linear chains "
+                     "without save_memory",
+               SCF_E_UNSUPPORTED;
+    if (l.lattice_type != LAT_SIMPLE_CUBIC || !l.stencil_full || l.fjc != 1)
+      return why = "2- and 3-gradient systems with surface boundaries or charges need the full simple-cubic "
+                   "stencil without refinement",
+             SCF_E_UNSUPPORTED;
+  }
   if (l.gradients == 2 && l.geometry == GEOM_PLANAR && !l.stencil_full && l.lattice_type != LAT_SIMPLE_CUBIC)
     return why = "2-gradient planar hexagonal stencil without stencil_full is not supported", SCF_E_UNSUPPORTED;
   if (s.charged) {
@@ -121,14 +132,7 @@ int validate(const SystemSpec &s, std::string &why) {
     for (auto &m : s.mol)
       if (m.scheduled())
         return why = "charged systems with branched molecules are not supported", SCF_E_UNSUPPORTED;
-    if (l.gradients >= 2) {
-      for (auto &m : s.mol)
-        if (m.markov != 1 || m.save_memory)
-          return why = "charged 2- and 3-gradient systems need first-order chains without save_memory",
-                 SCF_E_UNSUPPORTED;
-      if (l.lattice_type != LAT_SIMPLE_CUBIC || !l.stencil_full)
-        return why = "charged 2- and 3-gradient systems need the full simple-cubic stencil", SCF_E_UNSUPPORTED;
-    }
+
     if ((int)s.itmon.size() + 1 > kMaxSeg + 1) return why = "too many field blocks", SCF_E_UNSUPPORTED;
   }
   return SCF_OK;
@@ -149,8 +153,9 @@ int save_memory_interval(const SystemSpec &s) {
 int check_layout(const scf_system *S, int mode, int strip_sx) {
   if (S->branched() && mode == SCF_MODE_FUSED)
     return fail(SCF_E_UNSUPPORTED, "branched molecules and dendrimers run on the grid path");
-  if (S->spec.charged && mode != SCF_MODE_FUSED)
-    return fail(SCF_E_UNSUPPORTED, "charged systems run on the fused (one workgroup per system) path");
+  if ((S->spec.charged || S->literal()) && mode != SCF_MODE_FUSED)
+    return fail(SCF_E_UNSUPPORTED, "charged systems and 2- or 3-gradient surface boundaries run on the fused "
+                                   "(one workgroup per system) path");
   bool markov1 = true;
   for (auto &m : S->spec.mol) markov1 = markov1 && m.markov == 1;
   const bool strip = mode == SCF_MODE_FUSED && strip_sx > 0;
@@ -199,7 +204,7 @@ void plan_layout(scf_system *S, int want_sx) {
   // Markov-1 chains, the 9-point stencil, ghosts that map onto interior
   // sites, and the rows of one strip layout in LDS
   const bool strip_ok = l.gradients == 2 && l.fjc == 1 && maxR == 1 && (l.geometry != 0 || l.stencil_full) &&
-                        !S->spec.charged &&
+                        !S->literal() &&
                         l.MX + 2 <= 1026 && h.bx1 >= 1 && h.bx1 <= l.MX && h.bxm >= 1 && h.bxm <= l.MX &&
                         h.by1 >= 1 && h.by1 <= l.MY && h.bym >= 1 && h.bym <= l.MY;
   for (int sx : {8, 4}) {
@@ -291,7 +296,7 @@ int build(scf_system *S, const Placement *pl) {
   h.nit = s.nblocks_x();  // field blocks, plus the potential of a charged system
   h.nitm = nit;
   h.charged = s.charged ? 1 : 0;
-  h.all_sites = s.charged && l.gradients >= 2 ? 1 : 0;
+  h.all_sites = S->literal() ? 1 : 0;
   h.neutralizer = s.charged ? s.neutralizer : -1;
   if (s.charged) {
     // constants of the reference (src/namics.cpp:43-49); the bond length is

@@ -260,6 +260,22 @@ __device__ __forceinline__ bool is_interior(const Sys &S, int x, int y, int z) {
   return true;
 }
 
+// The site set_bounds copies into site (x, y, z) of a 2- or 3-gradient
+// lattice (fjc = 1), or the site itself.  2 gradients follow LGrad2::set_bounds
+// (src/LGrad2.cpp:1652-1673): the y ghosts of the interior rows, then the x
+// ghost rows, then each corner from its own ghost row at the y source, so a
+// corner next to a y surface keeps its value.  3 gradients (mirror and
+// periodic boundaries only) map coordinate by coordinate.
+template <typename Sys>
+__device__ __forceinline__ int bound_src(const Sys &S, int x, int y, int z) {
+  if (S.gradients == 3) return map_x(S, x) * S.JX + map_y(S, y) * S.JY + map_z(S, z);
+  const bool gx = x < 1 || x > S.MX, gy = y < 1 || y > S.MY;
+  const int yb = map_y(S, y);
+  if (gx && gy && (yb < 1 || yb > S.MY)) return x * S.JX + y;
+  return map_x(S, x) * S.JX + yb;
+}
+
+
 // ----------------------------------------------------- Markov-1 step / Side
 // Lattice average of a bounded array at site i=(x,y,z) for the propagator
 // (without the Boltzmann factor).  Arithmetic order follows the reference's
@@ -431,6 +447,15 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
     const int JX = S.JX;
     const int xm = map_x(S, x - 1) * JX, x0 = x * JX, xp = map_x(S, x + 1) * JX;
     const int ym = map_y(S, y - 1), yp = map_y(S, y + 1);
+    // diagonal neighbours; a corner ghost next to a y surface keeps its own
+    // value (bound_src), which the coordinate-wise map does not give
+    int dmm = xm + ym, dmp = xm + yp, dpm = xp + ym, dpp = xp + yp;
+    if (S.all_sites) {
+      dmm = bound_src(S, x - 1, y - 1, 0);
+      dmp = bound_src(S, x - 1, y + 1, 0);
+      dpm = bound_src(S, x + 1, y - 1, 0);
+      dpp = bound_src(S, x + 1, y + 1, 0);
+    }
     if (S.geometry == 0) {
       if (S.stencil_full && S.lattice_type != 0) {
         // hexagonal 9-point, LG2Planar::propagate src/LG2Planar.cpp:1092-1108:
@@ -441,10 +466,10 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
         v += g[x0 + ym];
         v += g[x0 + yp];
         v *= (T)2.0;
-        v += g[xm + ym];
-        v += g[xp + yp];
-        v += g[xm + yp];
-        v += g[xp + ym];
+        v += g[dmm];
+        v += g[dpp];
+        v += g[dmp];
+        v += g[dpm];
         return v * (T)(1.0 / 16.0);
       }
       if (S.stencil_full) {
@@ -454,10 +479,10 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
         v += C2 * g[x0 + yp];
         v += C2 * g[xm + y];
         v += C2 * g[xp + y];
-        v += C3 * g[xm + ym];
-        v += C3 * g[xm + yp];
-        v += C3 * g[xp + ym];
-        v += C3 * g[xp + yp];
+        v += C3 * g[dmm];
+        v += C3 * g[dmp];
+        v += C3 * g[dpm];
+        v += C3 * g[dpp];
         return v;
       }
       T v = g[xm + y];
@@ -477,10 +502,10 @@ __device__ __forceinline__ T stencil_prop(const Sys &S, Ptr g, int i, int x, int
     v += C2 * g[x0 + ym];
     v += C2 * g[x0 + yp];
     v *= C3;
-    v += lm * g[xm + ym];
-    v += lm * g[xm + yp];
-    v += lp * g[xp + ym];
-    v += lp * g[xp + yp];
+    v += lm * g[dmm];
+    v += lm * g[dmp];
+    v += lp * g[dpm];
+    v += lp * g[dpp];
     return v * C2;
   }
   const int JX = S.JX, JY = S.JY;
@@ -952,10 +977,10 @@ template <typename Sys>
 __device__ __forceinline__ void site_electro(const Sys &S, int i) {
   const int M = S.M;
   int b = map_x1(S, i);
-  if (S.gradients >= 2) {  // the bounded source, coordinate by coordinate
+  if (S.gradients >= 2) {  // the site set_bounds copies here
     int x, y, z;
     site_coords(S, i, x, y, z);
-    b = S.gradients == 2 ? map_x(S, x) * S.JX + map_y(S, y) : map_x(S, x) * S.JX + map_y(S, y) * S.JY + map_z(S, z);
+    b = bound_src(S, x, y, z);
   }
   double q = 0, e = 0;
 #pragma unroll
@@ -1021,6 +1046,8 @@ __device__ __forceinline__ void site_prepare(const Sys &S, int i, int x, int y, 
   int b;  // bounded source site for u
   if (S.gradients == 1) {
     b = map_x1(S, x);
+  } else if (S.all_sites) {
+    b = bound_src(S, x, y, z);
   } else if (S.gradients == 2) {
     b = map_x(S, x) * S.JX + map_y(S, y);
   } else {

@@ -557,17 +557,17 @@ __device__ __forceinline__ double literal_step(const Sys &S, const gdbl *g, int 
   return t * (1.0 / 6.0);
 }
 
-// ghosts of a row take their bounded values (set_bounds; mirror and
-// periodic boundaries, whose sources are interior sites)
+// ghosts of a row take their bounded values (set_bounds).  Every source is
+// an interior site or a surface ghost that keeps its value, so the copies do
+// not depend on each other.
 template <typename Sys>
 __device__ __forceinline__ void literal_bounds(const Sys &S, gdbl *row, int tid, int nt) {
   for (int i = tid; i < S.M; i += nt) {
     int x, y, z;
     site_coords(S, i, x, y, z);
     if (is_interior(S, x, y, z)) continue;
-    const int b = S.gradients == 2 ? map_x(S, x) * S.JX + map_y(S, y)
-                                   : map_x(S, x) * S.JX + map_y(S, y) * S.JY + map_z(S, z);
-    row[i] = row[b];
+    const int b = bound_src(S, x, y, z);
+    if (b != i) row[i] = row[b];
   }
 }
 

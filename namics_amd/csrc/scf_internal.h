@@ -49,12 +49,25 @@ struct scf_system {
     return false;
   }
 
+  // 2- and 3-gradient systems whose results depend on ghost values the
+  // reference computes over its flattened arrays: charged systems (the
+  // potential reads densities on ghosts) and surface boundaries (a surface
+  // ghost keeps its own value).  They run one workgroup per system with the
+  // chains over every site (chain_literal).
+  bool literal() const {
+    if (spec.lat.gradients < 2) return false;
+    if (spec.charged) return true;
+    for (int d = 0; d < 6; d++)
+      if (spec.lat.bc[d] == nscf::BC_SURFACE) return true;
+    return false;
+  }
+
   int effective_mode() const {
     if (mode != SCF_MODE_AUTO) return mode;
     // branched molecules and dendrimers run their row schedules on the grid path
     if (branched()) return SCF_MODE_GRID;
     // charged systems solve the potential in the one-workgroup kernels
-    if (spec.charged) return SCF_MODE_FUSED;
+    if (spec.charged || literal()) return SCF_MODE_FUSED;
     // checkpointed end-point storage runs on the strip chains or the grid path
     if (h.ckpt > 1 && h.strip_sx == 0) return SCF_MODE_GRID;
     if (tuned_mode != SCF_MODE_AUTO) return tuned_mode;
