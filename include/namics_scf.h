@@ -85,11 +85,12 @@ typedef struct {
  * @water are refused with SCF_E_PARSE and a message naming them.  Charged
  * systems (`mon : <name> : valence`, `epsilon`, `lat : <name> : bondlength`,
  * a molecule with freedom `neutralizer`) solve for the electrostatic
- * potential as one more field block on the fused path: on 1-gradient
- * lattices (with fixed surface potentials on frozen segments, `e.psi0/kT`,
- * computed as the reference computes them) and on simple-cubic 2- and
- * 3-gradient lattices with first-order chains (one workgroup per system);
- * charged branched molecules are refused.
+ * potential as one more field block on the fused path, on 1-gradient
+ * lattices and on simple-cubic 2- and 3-gradient lattices with first-order
+ * chains (one workgroup per system), fixed surface potentials on frozen
+ * segments (`e.psi0/kT`) included, as the reference computes them; charged
+ * branched molecules are refused.  2-gradient surface boundaries run on the
+ * same one-workgroup kernels.
  * Replaces: Input + Lattice/Segment/Molecule/System/Solve_scf::CheckInput and
  * AllocateMemory (src/namics.cpp:153-417, src/lattice.cpp:267-601), the
  * composition parser Molecule::Decomposition / GenerateTree / Interpret

@@ -1531,6 +1531,163 @@ __device__ __noinline__ double electro_residual_xyz(const DevSys &S0, double acc
   return acc;
 }
 
+// Fixed potentials on 2 and 3 gradients, as the reference computes them
+// (the same overwrite as on 1 gradient: psi0 on a plate's mask, zero
+// elsewhere, then set_M_bounds).
+// * 2 gradients: UpdateQ (src/LGrad2.cpp:1447-1484, src/LG2Planar.cpp:
+//   1211-1236) sets q on each plate's mask from its overwritten psi, g = -q
+//   there (the radial term of LGrad2 reads eps at the flat index x, kept);
+//   UpdatePsi (src/LGrad2.cpp:1448-1482, src/LG2Planar.cpp:1182-1206) takes
+//   one Jacobi sweep of the last plate's overwritten psi off the masks.
+// * 3 gradients: UpdateQ (src/LGrad3.cpp:1314-1347) writes, on the mask and
+//   one layer short of each upper face, the discrete Laplacian over C into
+//   psi in place (lexicographic order: run as a plane wavefront), and g = -q
+//   with the q of DoElectrostatics; UpdatePsi (src/LGrad3.cpp:1239-1282) is
+//   the symmetric Gauss-Seidel sweep off the masks.
+// The working psi is S.psi; elA holds the Jacobi values.
+template <typename Sys>
+__device__ __forceinline__ double fixed_psi_xyz(const Sys &S, int k, int x, int y, int z) {
+  const gdbl *mk = hbm(S.seg_mask) + (size_t)k * S.M;
+  auto mir = [](int c, int n) { return c < 1 ? 1 : (c > n ? n : c); };
+  int i;
+  if (S.gradients == 2) {
+    i = mir(x, S.MX) * S.JX + mir(y, S.MY);
+  } else {
+    const int nghost = (x < 1 || x > S.MX) + (y < 1 || y > S.MY) + (z < 1 || z > S.MZ);
+    i = nghost >= 2 ? x * S.JX + y * S.JY + z : mir(x, S.MX) * S.JX + mir(y, S.MY) * S.JY + mir(z, S.MZ);
+  }
+  return mk[i] == 1.0 ? S.seg_psi0[k] : 0.0;
+}
+
+__device__ __noinline__ double electro_fixed_xyz(const DevSys &S0, double acc) {
+  CSys &S = uniform_sys(S0);
+  const int M = S.M, MX = S.MX, MY = S.MY, MZ = S.MZ, JX = S.JX, JY = S.JY;
+  const int tid = work_tid(S), nt = S.threads;
+  const bool max_g = S.max_g, cyl = S.geometry != 0;
+  gdbl *gp = hbm(S.g) + S.nitm * M;
+  const gdbl *xp = hbm(S.x) + S.nitm * M;
+  const gdbl *E = hbm(S.eps);
+  gdbl *q = hbm(S.q), *P = hbm(S.psi), *X = hbm(S.elA);
+  const int L = S.el_last_fixed;
+  const double C = S.el_C0, Cq = -S.el_C0;
+  auto on_mask = [&](int i) {
+    bool m = false;
+    for (int k = 0; k < S.nseg; k++)
+      if (S.seg_fixpsi[k] && hbm(S.seg_mask)[(size_t)k * M + i] != 0.0) m = true;
+    return m;
+  };
+  // the last plate's overwritten, bounded psi; on 2 gradients each plate's
+  // charge from its own overwrite
+  for (int i = tid; i < M; i += nt) {
+    int x, y, z;
+    site_coords(S, i == 0 ? 1 : i, x, y, z);
+    P[i] = fixed_psi_xyz(S, L, x, y, z);
+    site_coords(S, i, x, y, z);
+    if (!is_interior(S, x, y, z)) continue;
+    gp[i] = xp[i];  // g starts as the potential block of x (DoElectrostatics)
+    if (S.gradients != 2) continue;
+    for (int k = 0; k < S.nseg; k++) {
+      if (!S.seg_fixpsi[k] || hbm(S.seg_mask)[(size_t)k * M + i] != 1.0) continue;
+      const double pm = fixed_psi_xyz(S, k, x - 1, y, 0), pp = fixed_psi_xyz(S, k, x + 1, y, 0);
+      const double ym = fixed_psi_xyz(S, k, x, y - 1, 0), yp = fixed_psi_xyz(S, k, x, y + 1, 0);
+      const double p0 = fixed_psi_xyz(S, k, x, y, 0);
+      double qx = -0.5 * (pm + pp + ym + yp - 4 * p0) * E[i] / Cq;
+      if (cyl) qx -= (pp - pm) / (2.0 * (S.el_r0 + x - 1 + 0.5)) * E[x] / Cq;
+      if (S.grad_eps) qx -= 0.25 * (E[i + JX] - E[i - JX]) * (pp - pm + E[i + 1] - E[i - 1]) * (yp - ym) / Cq;
+      q[i] = qx;
+      gp[i] = -qx;
+    }
+  }
+  __syncthreads();
+  if (S.gradients == 2) {
+    // one Jacobi sweep off the masks, from the last plate's overwrite
+    for (int w = tid; w < MX * MY; w += nt) {
+      const int x = 1 + w / MY, y = 1 + w % MY, i = x * JX + y;
+      if (on_mask(i)) continue;
+      double v = 0.25 * (P[i - JX] + P[i + JX] + P[i - 1] + P[i + 1]) + 0.5 * q[i] * C / E[i];
+      if (cyl) v += (P[i + JX] - P[i - JX]) / (2.0 * (S.el_r0 + x - 1 + 0.5));
+      if (S.grad_eps)
+        v += 0.25 * (E[i + JX] - E[i - JX]) * (P[i + JX] - P[i - JX] + E[i + 1] - E[i - 1]) * (P[i + 1] - P[i - 1]) / E[i];
+      X[i] = v;
+    }
+    __syncthreads();
+    for (int w = tid; w < MX * MY; w += nt) {
+      const int x = 1 + w / MY, y = 1 + w % MY, i = x * JX + y;
+      if (!on_mask(i)) {
+        P[i] = X[i];
+        gp[i] = xp[i] - X[i];
+      }
+      const double gv = gp[i];
+      acc = max_g ? fmax(acc, fabs(gv)) : acc + gv * gv;
+    }
+    return acc;
+  }
+  // 3 gradients: UpdateQ in place on the last plate's mask (the earlier
+  // plates' psi is overwritten by the later ones), g = -q on every plate's
+  // mask inside its loop range
+  for (int w = tid; w < MX * MY * MZ; w += nt) {
+    const int x = 1 + w / (MY * MZ), r = w % (MY * MZ), y = 1 + r / MZ, z = 1 + r % MZ;
+    if (x > MX - 1 || y > MY - 1 || z > MZ - 1) continue;
+    const int i = x * JX + y * JY + z;
+    for (int k = 0; k < S.nseg; k++)
+      if (S.seg_fixpsi[k] && hbm(S.seg_mask)[(size_t)k * M + i] == 1.0) gp[i] = -q[i];
+  }
+  for (int kk = 0; kk <= MX + MY + MZ - 6; kk++) {
+    const int k = 3 + kk;
+    for (int w = tid; w < (MX - 1) * (MY - 1); w += nt) {
+      const int x = 1 + w / (MY - 1), y = 1 + w % (MY - 1), z = k - x - y;
+      if (z < 1 || z > MZ - 1) continue;
+      const int i = x * JX + y * JY + z;
+      if (hbm(S.seg_mask)[(size_t)L * M + i] != 1.0) continue;
+      const double epsZmin = E[i] + E[i - 1], epsZplus = E[i] + E[i + 1];
+      const double epsYmin = E[i] + E[i - JY], epsYplus = E[i] + E[i + JY];
+      const double epsXmin = E[i] + E[i - JX], epsXplus = E[i] + E[i + JX];
+      P[i] = (epsXmin * P[i - JX] + epsXplus * P[i + JX] + epsYmin * P[i - JY] + epsYplus * P[i + JY] +
+              epsZmin * P[i - 1] + epsZplus * P[i + 1] -
+              (epsXmin + epsXplus + epsYmin + epsYplus + epsZmin + epsZplus) * P[i]) /
+             Cq;
+    }
+    __syncthreads();
+  }
+  // set_M_bounds again: the faces take the (changed) interior values
+  for (int i = tid; i < M; i += nt) {
+    int x, y, z;
+    site_coords(S, i, x, y, z);
+    const int nghost = (x < 1 || x > MX) + (y < 1 || y > MY) + (z < 1 || z > MZ);
+    if (nghost != 1) continue;
+    const int xs = x < 1 ? 1 : (x > MX ? MX : x), ys = y < 1 ? 1 : (y > MY ? MY : y),
+              zs = z < 1 ? 1 : (z > MZ ? MZ : z);
+    P[i] = P[xs * JX + ys * JY + zs];
+  }
+  __syncthreads();
+  for (int pass = 0; pass < 2; pass++) {
+    for (int kk = 0; kk <= MX + MY + MZ - 3; kk++) {
+      const int k = pass == 0 ? 3 + kk : MX + MY + MZ - kk;
+      for (int w = tid; w < MX * MY; w += nt) {
+        const int x = 1 + w / MY, y = 1 + w % MY, z = k - x - y;
+        if (z < 1 || z > MZ) continue;
+        const int i = x * JX + y * JY + z;
+        if (on_mask(i)) continue;
+        const double epsZmin = E[i] + E[i - 1], epsZplus = E[i] + E[i + 1];
+        const double epsYmin = E[i] + E[i - JY], epsYplus = E[i] + E[i + JY];
+        const double epsXmin = E[i] + E[i - JX], epsXplus = E[i] + E[i + JX];
+        const double v = (epsXmin * P[i - JX] + epsXplus * P[i + JX] + epsYmin * P[i - JY] + epsYplus * P[i + JY] +
+                          epsZmin * P[i - 1] + epsZplus * P[i + 1] + C * q[i]) /
+                         (epsXmin + epsXplus + epsYmin + epsYplus + epsZmin + epsZplus);
+        P[i] = v;
+        if (pass == 1) gp[i] = xp[i] - v;
+      }
+      __syncthreads();
+    }
+  }
+  for (int w = tid; w < MX * MY * MZ; w += nt) {
+    const int x = 1 + w / (MY * MZ), r = w % (MY * MZ), y = 1 + r / MZ, z = 1 + r % MZ;
+    const double gv = gp[x * JX + y * JY + z];
+    acc = max_g ? fmax(acc, fabs(gv)) : acc + gv * gv;
+  }
+  return acc;
+}
+
 // The potential block of the residual of a charged 1-gradient system
 // (System::Classical_residual src/system.cpp:2161-2169): charge density,
 // dielectric constant, psi and EE at every site, then one sweep of the
@@ -1558,7 +1715,7 @@ __device__ __noinline__ double electro_residual(const DevSys &S0, double acc) {
     }
   }
   __syncthreads();
-  if (S.gradients >= 2) return electro_residual_xyz(S0, acc);
+  if (S.gradients >= 2) return S.fixed_psi0 ? electro_fixed_xyz(S0, acc) : electro_residual_xyz(S0, acc);
   if (S.fixed_psi0) return electro_fixed(S0, acc);
   const bool upwind = S.geometry != 2;
   for (int x = f + tid; x < MX + f; x += nt) {

@@ -1441,10 +1441,6 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       if (s.seg[i].epsilon != s.seg[0].epsilon) s.grad_epsilon = true;
       if (s.seg[i].freedom == SEG_FROZEN && s.seg[i].fixed_psi0) s.fixed_psi0 = true;
     }
-    if (s.fixed_psi0 && l.gradients != 1) {
-      err = "fixed surface potentials (e.psi0/kT) are supported on 1-gradient lattices";
-      return false;
-    }
   }
   compute_lambdas(l, any_markov2);
 

@@ -1643,6 +1643,60 @@ struct Oracle {
     const double C = -kE * kE / (kEps0 * k_BT * bond());
     const double *P = psi.data(), *E = eps.data();
     double a, b, c;
+    const int MY = s.lat.MY, MZ = s.lat.MZ, JX = s.lat.JX, JY = s.lat.JY;
+    if (s.lat.gradients == 3) {
+      // LGrad3::UpdateQ (src/LGrad3.cpp:1314-1347): the loops stop one layer
+      // short of each upper face, the charge on the mask is written into psi
+      // (not q), and g = -q with the q of DoElectrostatics
+      double *Q = psi.data();
+      for (int x = 1; x < MX; x++)
+        for (int y = 1; y < MY; y++) {
+          double epsZplus = E[x * JX + y * JY] + E[x * JX + y * JY + 1];
+          for (int z = 1; z < MZ; z++) {
+            const int i = x * JX + y * JY + z;
+            const double epsZmin = epsZplus;
+            epsZplus = E[i] + E[i + 1];
+            const double epsYmin = E[i] + E[i - JY], epsYplus = E[i] + E[i + JY];
+            const double epsXmin = E[i] + E[i - JX], epsXplus = E[i] + E[i + JX];
+            if (mask[i] == 1) {
+              Q[i] = (epsXmin * Q[i - JX] + epsXplus * Q[i + JX] + epsYmin * Q[i - JY] + epsYplus * Q[i + JY] +
+                      epsZmin * Q[i - 1] + epsZplus * Q[i + 1] -
+                      (epsXmin + epsXplus + epsYmin + epsYplus + epsZmin + epsZplus) * Q[i]) /
+                     C;
+              gp[i] = -q[i];
+            }
+          }
+        }
+      return;
+    }
+    if (s.lat.gradients == 2) {
+      // LGrad2::UpdateQ (src/LGrad2.cpp:1447-1484; its radial term reads eps
+      // at index x) and LG2Planar::UpdateQ (src/LG2Planar.cpp:1211-1236)
+      for (int x = f; x < MX + f; x++)
+        for (int y = f; y < MY + f; y++) {
+          const int i = x * JX + y;
+          if (mask[i] == 1) q[i] = -0.5 * (P[i - JX] + P[i + JX] + P[i - 1] + P[i + 1] - 4 * P[i]) * f * f * E[i] / C;
+        }
+      if (s.lat.geometry != GEOM_PLANAR)
+        for (int x = f; x < MX + f; x++)
+          for (int y = f; y < MY + f; y++) {
+            const int i = x * JX + y;
+            if (mask[i] == 1) q[i] -= (P[i + JX] - P[i - JX]) / (2.0 * (s.lat.offset_first_layer + x - f + 0.5)) * f * E[x] / C;
+          }
+      if (s.grad_epsilon)
+        for (int x = f; x < MX + f; x++)
+          for (int y = f; y < MY + f; y++) {
+            const int i = x * JX + y;
+            if (mask[i] == 1)
+              q[i] -= 0.25 * (E[i + JX] - E[i - JX]) * (P[i + JX] - P[i - JX] + E[i + 1] - E[i - 1]) * (P[i + 1] - P[i - 1]) * f * f / C;
+          }
+      for (int x = f; x < MX + f; x++)
+        for (int y = f; y < MY + f; y++) {
+          const int i = x * JX + y;
+          if (mask[i] == 1) gp[i] = -q[i];
+        }
+      return;
+    }
     b = P[f - 1];
     c = P[f];
     for (int x = f; x < MX + f; x++) {
@@ -1692,6 +1746,75 @@ struct Oracle {
     double *P = psi.data();
     const double *E = eps.data();
     double a, b, c;
+    const int MY = s.lat.MY, MZ = s.lat.MZ, JX = s.lat.JX, JY = s.lat.JY;
+    if (s.lat.gradients == 3) {
+      // LGrad3::UpdatePsi with fixed potentials (src/LGrad3.cpp:1239-1282):
+      // the symmetric Gauss-Seidel sweep off the masks
+      auto site = [&](int i, double epsZmin, double epsZplus) {
+        const double epsYmin = E[i] + E[i - JY], epsYplus = E[i] + E[i + JY];
+        const double epsXmin = E[i] + E[i - JX], epsXplus = E[i] + E[i + JX];
+        P[i] = (epsXmin * P[i - JX] + epsXplus * P[i + JX] + epsYmin * P[i - JY] + epsYplus * P[i + JY] +
+                epsZmin * P[i - 1] + epsZplus * P[i + 1] + C * q[i]) /
+               (epsXmin + epsXplus + epsYmin + epsYplus + epsZmin + epsZplus);
+      };
+      for (int x = 1; x < MX + 1; x++)
+        for (int y = 1; y < MY + 1; y++) {
+          double epsZplus = E[x * JX + y * JY] + E[x * JX + y * JY + 1];
+          for (int z = 1; z < MZ + 1; z++) {
+            const int i = x * JX + y * JY + z;
+            const double epsZmin = epsZplus;
+            epsZplus = E[i] + E[i + 1];
+            if (pmask[i] == 0) site(i, epsZmin, epsZplus);
+          }
+        }
+      for (int x = MX; x > 0; x--)
+        for (int y = MY; y > 0; y--) {
+          double epsZmin = E[x * JX + y * JY + MZ + 1] + E[x * JX + y * JY + MZ];
+          for (int z = MZ; z > 0; z--) {
+            const int i = x * JX + y * JY + z;
+            const double epsZplus = epsZmin;
+            epsZmin = E[i] + E[i - 1];
+            if (pmask[i] == 0) {
+              site(i, epsZmin, epsZplus);
+              gp[i] -= P[i];
+            }
+          }
+        }
+      return;
+    }
+    if (s.lat.gradients == 2) {
+      // LGrad2 / LG2Planar::UpdatePsi with fixed potentials
+      // (src/LGrad2.cpp:1448-1482, src/LG2Planar.cpp:1182-1206): Jacobi off
+      // the masks, the radial correction on cylindrical lattices
+      double *X = Xel.data();
+      for (int x = f; x < MX + f; x++)
+        for (int y = f; y < MY + f; y++) {
+          const int i = x * JX + y;
+          if (pmask[i] == 0) X[i] = 0.25 * (P[i - JX] + P[i + JX] + P[i - 1] + P[i + 1]) + 0.5 * q[i] * C / E[i];
+        }
+      if (s.lat.geometry != GEOM_PLANAR)
+        for (int x = f; x < MX + f; x++)
+          for (int y = f; y < MY + f; y++) {
+            const int i = x * JX + y;
+            if (pmask[i] == 0) X[i] += (P[i + JX] - P[i - JX]) / (2.0 * (s.lat.offset_first_layer + x - f + 0.5)) * f;
+          }
+      if (s.grad_epsilon)
+        for (int x = f; x < MX + f; x++)
+          for (int y = f; y < MY + f; y++) {
+            const int i = x * JX + y;
+            if (pmask[i] == 0)
+              X[i] += 0.25 * (E[i + JX] - E[i - JX]) * (P[i + JX] - P[i - JX] + E[i + 1] - E[i - 1]) * (P[i + 1] - P[i - 1]) / E[i] * f * f;
+          }
+      for (int x = f; x < MX + f; x++)
+        for (int y = f; y < MY + f; y++) {
+          const int i = x * JX + y;
+          if (pmask[i] == 0) {
+            P[i] = X[i];
+            gp[i] -= P[i];
+          }
+        }
+      return;
+    }
     b = P[f - 1];
     c = P[f];
     for (int x = f; x < MX + f; x++) {

@@ -12,7 +12,8 @@ walls, against one wall with a mirrored far boundary, and on a refined
 hexagonal lattice; a refined planar hexagonal lattice (fjc 2); frozen plates that fix the potential
 (`e.psi0/kT`, planar and cylindrical, the latter with segments of
 different dielectric constant); 2-gradient cylindrical and planar lattices
-and a 3-gradient box with frozen particles.
+and a 3-gradient box with frozen particles, without and with a fixed
+potential on the particle.
 """
 
 WALL = """
@@ -331,6 +332,93 @@ mol : Cl : phibulk : 0.01
 """,
         eval=[(113, 0.2)],
         converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    # fixed potentials on 2 and 3 gradients (evaluations: the reference's
+    # fixed-potential sweeps are the same overwrite as on 1 gradient)
+    "el2d_fixed_planar": dict(
+        text="""
+lat : L : gradients : 2
+lat : L : n_layers_x : 10
+lat : L : n_layers_y : 12
+lat : L : lattice_type : simple_cubic
+lat : L : lowerbound_y : periodic
+lat : L : upperbound_y : periodic
+lat : L : bondlength : 3e-10
+mon : P : freedom : frozen
+mon : P : frozen_range : 4,5;6,7
+mon : P : e.psi0/kT : 1.0
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.3
+mon : A : epsilon : 20
+mon : P : chi_A : -0.5
+mol : pol : composition : (A)12
+mol : pol : freedom : restricted
+mol : pol : theta : 6
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.01
+""",
+        eval=[(114, 0.2)],
+        converged=[],
+    ),
+    "el2d_fixed_cyl": dict(
+        text="""
+lat : cyl : gradients : 2
+lat : cyl : geometry : cylindrical
+lat : cyl : n_layers_x : 12
+lat : cyl : n_layers_y : 10
+lat : cyl : lattice_type : simple_cubic
+lat : cyl : bondlength : 5e-10
+mon : P : freedom : frozen
+mon : P : frozen_range : 1,1;2,10
+mon : P : e.psi0/kT : -1.5
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : 0.3
+mon : A : epsilon : 10
+mon : P : chi_A : -0.5
+mol : pol : composition : (A)15
+mol : pol : freedom : restricted
+mol : pol : theta : 40
+mol : Na : composition : (Na)1
+mol : Na : freedom : free
+mol : Na : phibulk : 0.02
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : neutralizer
+""",
+        eval=[(115, 0.2)],
+        converged=[],
+    ),
+    "el3d_fixed": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 8
+lat : box : n_layers_y : 7
+lat : box : n_layers_z : 9
+lat : box : lattice_type : simple_cubic
+lat : box : bondlength : 3e-10
+mon : P : freedom : frozen
+mon : P : frozen_range : 3,3,4;4,5,5
+mon : P : e.psi0/kT : 1.0
+""" + SALT + """
+mon : A : freedom : free
+mon : A : valence : -0.3
+mon : A : epsilon : 15
+mon : P : chi_A : -1.0
+mol : pol : composition : (A)10
+mol : pol : freedom : restricted
+mol : pol : theta : 12
+mol : Na : composition : (Na)1
+mol : Na : freedom : neutralizer
+mol : Cl : composition : (Cl)1
+mol : Cl : freedom : free
+mol : Cl : phibulk : 0.01
+""",
+        eval=[(116, 0.2)],
+        converged=[],
     ),
     # a frozen plate at the first layer fixing the potential, a charged
     # polymer and salt in front of it
