@@ -666,6 +666,40 @@ template <typename T, typename Sys, typename Ptr>
 __device__ __forceinline__ void m2h_step(const Sys &S, Ptr g, const double *P, bool forward, int x,
                                          T out[kMaxComp]) {
   const int M = S.M, f = S.fjc, F = 2 * S.fjc + 1;
+  if (S.geometry != 0) {
+    // curved, fjc = 1 (LGrad1::propagateF / propagateB src/LGrad1.cpp:298-321,
+    // 374-398): a component that steps off the axis mixes the incoming
+    // component with its reflection, weighted by l_1 / l_11 (towards the
+    // centre) or l1 / l11 (outwards) at x
+    const T l1 = (T)hbm(S.l1)[x], l11 = (T)hbm(S.l11)[x], l_1 = (T)hbm(S.l_1)[x], l_11 = (T)hbm(S.l_11)[x];
+    for (int p = 0; p < F; p++) {
+      T acc = 0;
+      for (int qq = 0; qq < F; qq++) {
+        const int q = forward ? qq : F - 1 - qq;
+        // the step's direction: towards lower x (d < 0) for p < f forward
+        // and for q > f backward
+        const int d = forward ? p - f : f - q;
+        double c = P[p > q ? p - q : q - p];
+        if (q > 0 && q < F - 1) c += P[F - 1 - abs(F - 1 - p - q)];
+        if (c == 0) continue;
+        T h;
+        if (d < 0) {  // reads x - 1
+          if (x < 1) continue;
+          h = l_1 * m2h_val<T>(S, g, q, x - 1);
+          h += l_11 * m2h_val<T>(S, g, F - 1 - q, x);
+        } else if (d > 0) {  // reads x + 1
+          if (x > M - 2) continue;
+          h = l1 * m2h_val<T>(S, g, q, x + 1);
+          h += l11 * m2h_val<T>(S, g, F - 1 - q, x);
+        } else {
+          h = m2h_val<T>(S, g, q, x);
+        }
+        acc += (T)c * h;
+      }
+      out[p] = acc;
+    }
+    return;
+  }
   for (int p = 0; p < F; p++) {
     T acc = 0;
     if (forward) {

@@ -1341,12 +1341,13 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     m.size = 1;
     if (m.markov == 2) {
       const bool sc1 = l.lattice_type == LAT_SIMPLE_CUBIC && l.fjc == 1;
-      const bool hexp = l.lattice_type == LAT_HEXAGONAL && l.geometry == GEOM_PLANAR;
+      // hexagonal: planar with any refinement, curved with fjc = 1
+      const bool hexp = l.lattice_type == LAT_HEXAGONAL && (l.geometry == GEOM_PLANAR || l.fjc == 1);
       const bool ok1 = l.gradients == 1 && (sc1 || hexp);
       const bool ok23 = l.gradients >= 2 && sc1;
       if (!(ok1 || ok23)) {
-        err = "mol '" + m.name + "': Markov 2 is supported on simple-cubic lattices without refinement and on "
-              "planar 1-gradient hexagonal lattices";
+        err = "mol '" + m.name + "': Markov 2 is supported on simple-cubic lattices without refinement, on "
+              "planar 1-gradient hexagonal lattices and on curved ones without refinement";
         return false;
       }
       any_markov2 = true;

@@ -259,6 +259,62 @@ struct Lat {
       if (q > 0 && q < F - 1) c += P[F - 1 - std::abs(F - 1 - p - q)];
       return c;
     };
+    if (l.geometry != GEOM_PLANAR) {
+      // curved hexagonal lattice, fjc = 1 (LGrad1::propagateF / propagateB
+      // src/LGrad1.cpp:298-321,374-398): a step off the axis mixes the
+      // component with its reflection, with the Markov-2 curvature weights
+      vec Hv(M, 0.0);
+      double *H = Hv.data();
+      const double *l1 = l.l1.data(), *l11 = l.l11.data(), *l_1 = l.l_1.data(), *l_11 = l.l_11.data();
+      auto fillH = [&](int q, int a, int b) {
+        if (a > 0) {
+          Times(H + b, l_1 + a, gs_1 + q * M + b, M - a - b);
+          AddTimes(H + b, l_11 + a, gs_1 + (F - 1 - q) * M + a, M - a - b);
+        }
+        if (b > 0) {
+          Times(H + b, l1 + a, gs_1 + q * M + b, M - a - b);
+          AddTimes(H + b, l11 + a, gs_1 + (F - 1 - q) * M + a, M - a - b);
+        }
+      };
+      auto addp = [&](int p, int q, int a, int b) {
+        const double c = cw(p, q);
+        if (a + b > 0) {
+          if (c != 0) YplusisCtimesX(gs + p * M + a, H + b, c, M - a - b);
+        } else {
+          if (c != 0) YplusisCtimesX(gs + p * M, gs_1 + q * M, c, M);
+        }
+      };
+      if (forward) {
+        for (int p = 0; p < F; p++) {
+          int a = p - f, b;
+          if (a < 0) {
+            b = 0;
+            a = -a;
+          } else {
+            b = a;
+            a = 0;
+          }
+          for (int q = 0; q < F; q++) {
+            fillH(q, a, b);
+            addp(p, q, a, b);
+          }
+        }
+      } else {
+        for (int q = F - 1; q > -1; q--) {
+          int a = q - f, b;
+          if (a > 0) {
+            b = 0;
+          } else {
+            b = -a;
+            a = 0;
+          }
+          fillH(q, a, b);
+          for (int p = F - 1; p > -1; p--) addp(p, q, a, b);
+        }
+      }
+      for (int k = 0; k < F; k++) Times(gs + k * M, gs + k * M, G1, M);
+      return;
+    }
     if (forward) {
       for (int p = 0; p < F; p++) {
         int a = p - f, b;

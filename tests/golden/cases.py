@@ -327,6 +327,54 @@ mol : water : freedom : solvent
 """,
         eval=[(7, 0.3)],
     ),
+    # second-order Markov on curved hexagonal lattices (reflection-weighted
+    # steps off the axis), cylindrical and spherical
+    "hex_markov2_cyl": dict(
+        text="""
+lat : L : gradients : 1
+lat : L : n_layers : 30
+lat : L : geometry : cylindrical
+lat : L : offset_first_layer : 2
+lat : L : lattice_type : hexagonal
+lat : L : lowerbound : surface
+mon : S : freedom : frozen
+mon : S : frozen_range : lowerbound
+mon : A : freedom : free
+mon : W : freedom : free
+mon : S : chi_A : -1.0
+mon : A : chi_W : 0.6
+mol : pol : composition : (A)20
+mol : pol : freedom : restricted
+mol : pol : theta : 30
+mol : pol : Markov : 2
+mol : pol : k_stiff : 1.5
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(51, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-9)],
+    ),
+    "hex_markov2_sph": dict(
+        text="""
+lat : L : gradients : 1
+lat : L : n_layers : 25
+lat : L : geometry : spherical
+lat : L : offset_first_layer : 3
+lat : L : lattice_type : hexagonal
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.9
+mol : pol : composition : (A)15
+mol : pol : freedom : restricted
+mol : pol : theta : 300
+mol : pol : Markov : 2
+mol : pol : k_stiff : 1
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(52, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-9, seed=53, amp=0.1)],
+    ),
     # 2-gradient cylindrical polymer brush (small version of configuration 2)
     "brush2d_cyl": dict(
         text="""
