@@ -557,6 +557,9 @@ int scf_run_input_file(const char *path, int *n_starts_out, int *n_converged_out
         std::snprintf(num, sizeof(num), "%.17g", scan_value(vs, subl));
         if (vs.kind == "mol") {
           t = insert_before_start(t, start, "mol : " + vs.name + " : " + vs.prop + " : " + num);
+        } else if (vs.prop == "valence" || vs.prop == "ePsi0/kT") {
+          t = insert_before_start(t, start, "mon : " + vs.name + " : " + (vs.prop == "valence" ? "valence" : "e.psi0/kT") +
+                                                " : " + num);
         } else {
           const std::string other = vs.prop.substr(4);
           t = insert_before_start(t, start, "mon : " + vs.name + " : chi_" + other + " : " + num);

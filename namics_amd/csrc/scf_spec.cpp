@@ -1552,6 +1552,13 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
           found = true;
           vs.start_value = vs.prop == "theta" ? m.theta : (vs.prop == "n" ? m.n : m.phibulk);
         }
+    } else if (vs.kind == "mon" && (vs.prop == "valence" || vs.prop == "ePsi0/kT")) {
+      // Segment::PutVarInfo src/segment.cpp:1367-1369
+      for (size_t i = 0; i < s.seg.size(); i++)
+        if (s.seg[i].name == vs.name) {
+          found = true;
+          vs.start_value = vs.prop == "valence" ? s.seg[i].valence : s.seg[i].psi0;
+        }
     } else if (vs.kind == "mon" && vs.prop.compare(0, 4, "chi_") == 0) {
       const std::string other = vs.prop.substr(4);
       for (size_t i = 0; i < s.seg.size(); i++)
@@ -1563,7 +1570,8 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
             }
     }
     if (!found) {
-      err = "var scan of " + ln.name + " : " + vs.prop + " is not supported (mol theta, n, phibulk; mon chi_X)";
+      err = "var scan of " + ln.name + " : " + vs.prop +
+            " is not supported (mol theta, n, phibulk; mon chi_X, valence, ePsi0/kT)";
       return false;
     }
     std::string sv;

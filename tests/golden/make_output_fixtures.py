@@ -20,6 +20,7 @@ REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, REPO)
 
 from tests.golden.cases import CASES  # noqa: E402
+from tests.golden.charged_cases import CHARGED  # noqa: E402
 from tests.golden.make_fixtures import REF  # noqa: E402
 
 DIIS = "newton : isaac : method : DIIS\nnewton : isaac : tolerance : 1e-9\nnewton : isaac : iterationlimit : 3000\n"
@@ -129,6 +130,22 @@ var : mon-A : scan : chi_W
 var : mon-A : step : 0.1
 var : mon-A : end_value : 0.7
 sys : NN : final_guess : file
+""" + DIIS + "start\n",
+    # the polyelectrolyte's valence scanned linearly (a charged system;
+    # the potential profile and its drop written too)
+    "scan_valence": CHARGED["el1d_poly"]["text"] + """output : kal : append : false
+kal : sys : NN : free_energy
+kal : sys : NN : grand_potential
+kal : sys : NN : Dpsi
+kal : mol : pol : theta
+kal : mol : Na : phibulk
+kal : mon : A : valence
+output : pro : append : false
+pro : sys : NN : psi
+pro : sys : NN : q
+var : mon-A : scan : valence
+var : mon-A : step : -0.2
+var : mon-A : end_value : -0.6
 """ + DIIS + "start\n",
     "box3d": CASES["diblock3d_periodic"]["text"] + """output : kal : append : false
 kal : sys : NN : free_energy
