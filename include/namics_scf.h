@@ -90,7 +90,10 @@ typedef struct {
  * chains (one workgroup per system), fixed surface potentials on frozen
  * segments (`e.psi0/kT`) included, as the reference computes them; charged
  * branched molecules are refused.  2-gradient surface boundaries run on the
- * same one-workgroup kernels.
+ * same one-workgroup kernels.  A linear molecule may hold one tagged
+ * segment (`mon : <name> : freedom : tagged`, `tagged_range`): its weight
+ * and density are the mask, and the molecule holds one chain per tagged
+ * site.
  * Replaces: Input + Lattice/Segment/Molecule/System/Solve_scf::CheckInput and
  * AllocateMemory (src/namics.cpp:153-417, src/lattice.cpp:267-601), the
  * composition parser Molecule::Decomposition / GenerateTree / Interpret

@@ -352,6 +352,7 @@ int build(scf_system *S, const Placement *pl) {
     h.seg_block[k] = s.seg[k].field_block;
     h.seg_frozen[k] = s.seg[k].freedom == SEG_FROZEN;
     h.seg_pinned[k] = s.seg[k].freedom == SEG_PINNED;
+    h.seg_tagged[k] = s.seg[k].freedom == SEG_TAGGED;
     for (int j = 0; j < nseg; j++) h.chi[k * kMaxSeg + j] = s.seg[k].chi[j];
   }
   for (int i = 0; i < nit; i++) h.itmon[i] = s.itmon[i];
@@ -364,6 +365,8 @@ int build(scf_system *S, const Placement *pl) {
     h.mol_N[i] = m.chainlength;
     h.mol_freedom[i] = m.freedom;
     h.mol_pinned[i] = m.pinned;
+    h.mol_tagged[i] = m.tagged;
+    h.mol_NF[i] = m.chainlength - (m.tagged ? 1 : 0);
     h.mol_markov[i] = m.markov;
     h.mol_size[i] = m.size;
     h.mol_block_begin[i] = nb;
@@ -382,15 +385,24 @@ int build(scf_system *S, const Placement *pl) {
     }
     for (int sg : m.molmon) {
       h.mm_seg[nmm++] = sg;
-      h.sysmon[sg] = 1;
+      h.sysmon[sg] = s.seg[sg].freedom != SEG_TAGGED;  // SysMonList leaves tagged segments out
     }
     for (int k = 0; k < kMaxComp && k < (int)m.P.size(); k++) h.mol_P[i][k] = m.P[k];
     h.mol_phibulk_in[i] = m.phibulk;
     h.mol_n_in[i] = m.n;
     h.mol_charge[i] = s.mol_charge[i];
+    // fractions of a molecule with a tagged segment are scaled by N / (N - 1)
+    // (System::GetFreeEnergy src/system.cpp:2988-3012)
     double cf = 0;
     for (int a : m.molmon)
-      for (int b : m.molmon) cf -= fraction(m, a) * fraction(m, b) * (s.seg[a].chi[b] / 2);
+      for (int b : m.molmon) {
+        double fA = fraction(m, a), fB = fraction(m, b);
+        if (m.tagged) {
+          fA = m.chainlength > 1 ? fA * m.chainlength / (m.chainlength - 1) : 0.0;
+          fB = m.chainlength > 1 ? fB * m.chainlength / (m.chainlength - 1) : 0.0;
+        }
+        cf -= fA * fB * (s.seg[a].chi[b] / 2);
+      }
     h.mol_constF[i] = cf;
     for (int k = 0; k < nseg; k++) h.frac[i][k] = fraction(m, k);
     maxR = std::max(maxR, m.size);

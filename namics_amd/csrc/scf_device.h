@@ -81,6 +81,7 @@ struct DevSys {
   int nseg, nit;
   int seg_block[kMaxSeg];   // field block of x, -1 for frozen segments
   int seg_frozen[kMaxSeg], seg_pinned[kMaxSeg];
+  int seg_tagged[kMaxSeg];  // tagged: weight and density are the mask, no field
   int itmon[kMaxSeg];
   int sysmon[kMaxSeg];      // 1 if the segment is used by a molecule
   double chi[kMaxSeg * kMaxSeg];
@@ -89,6 +90,7 @@ struct DevSys {
   // ---- molecules (linear chains: a sequence of blocks)
   int nmol, solvent, nblocks, nmm;
   int mol_N[kMaxMol], mol_freedom[kMaxMol], mol_pinned[kMaxMol], mol_markov[kMaxMol], mol_size[kMaxMol];
+  int mol_tagged[kMaxMol], mol_NF[kMaxMol];  // holds a tagged segment; N less the tagged segment
   int mol_block_begin[kMaxMol + 1];
   int mol_mm_begin[kMaxMol + 1];
   int block_seg[kMaxBlocks], block_n[kMaxBlocks], block_mm[kMaxBlocks];
@@ -1109,7 +1111,12 @@ __device__ __forceinline__ void site_prepare(const Sys &S, int i, int x, int y, 
     }
     double w = exp(-u);
     if (S.seg_pinned[k]) w = w * hbm(S.seg_mask)[k * M + i];
-    w = w * hbm(S.ksam)[i];
+    // a tagged segment's weight is its mask, without KSAM
+    // (Segment::PrepareForCalculations src/segment.cpp:859-879)
+    if (S.seg_tagged[k])
+      w = hbm(S.seg_mask)[k * M + i];
+    else
+      w = w * hbm(S.ksam)[i];
     hbm(S.G1)[k * M + i] = w;
   }
 }
@@ -1146,6 +1153,8 @@ __device__ __forceinline__ void site_densities(const Sys &S, int i) {
     double p = S.seg_frozen[k] ? hbm(S.seg_mask)[k * M + i] : 0.0;
     for (int mm = 0; mm < S.nmm; mm++)
       if (S.mm_seg[mm] == k) p += hbm(S.rho)[mm * M + i];
+    // a tagged segment's density is its mask (src/system.cpp:2596-2601)
+    if (S.seg_tagged[k]) p = hbm(S.seg_mask)[k * M + i];
     hbm(S.phi)[k * M + i] = p;
   }
   hbm(S.phitot)[i] = phitot;
@@ -1225,12 +1234,13 @@ __device__ __forceinline__ void site_thermo(const Sys &S, int i, double logc[kMa
     double t = g1 > 0 ? -log(g1) : 0.0;
     f += -(hbm(S.phi)[j * M + i] * t);
   }
+  // frozen and tagged partners count in full, their own pairs not at all
   for (int j = 0; j < S.nseg; j++) {
-    if (S.seg_frozen[j]) continue;
+    if (S.seg_frozen[j] || S.seg_tagged[j]) continue;
     const double pj = hbm(S.phi)[j * M + i];
     for (int k = 0; k < S.nseg; k++) {
       double chi = S.chi[j * kMaxSeg + k];
-      if (!S.seg_frozen[k]) chi = chi / 2;
+      if (!(S.seg_frozen[k] || S.seg_tagged[k])) chi = chi / 2;
       if (chi != 0) f += pj * hbm(S.phi_side)[k * M + i] * chi;
     }
   }
@@ -1247,13 +1257,13 @@ __device__ __forceinline__ void site_thermo(const Sys &S, int i, double logc[kMa
   }
   double gp = 0;
   for (int mo = 0; mo < S.nmol; mo++)
-    gp += (hbm(S.mol_phitot)[mo * M + i] - sc.phibulk[mo]) * (1.0 / S.mol_N[mo]);
+    gp += (hbm(S.mol_phitot)[mo * M + i] - sc.phibulk[mo]) * (1.0 / S.mol_NF[mo]);
   gp += hbm(S.alpha)[i];
   for (int j = 0; j < S.nseg; j++) {
-    if (S.seg_frozen[j]) continue;
+    if (S.seg_frozen[j] || S.seg_tagged[j]) continue;
     const double pj = hbm(S.phi)[j * M + i], pbA = sc.seg_phibulk[j];
     for (int k = 0; k < S.nseg; k++) {
-      if (S.seg_frozen[k]) continue;
+      if (S.seg_frozen[k] || S.seg_tagged[k]) continue;
       const double chi = S.chi[j * kMaxSeg + k] / 2;
       if (chi != 0) gp += (pj * hbm(S.phi_side)[k * M + i] - pbA * sc.seg_phibulk[k]) * chi;
     }
@@ -1320,7 +1330,7 @@ __device__ inline void scalars_norms(const DevSys &S) {
         return;
       }
     }
-    if (S.mol_pinned[i]) {
+    if (S.mol_pinned[i] || S.mol_tagged[i]) {
       if (sc.GN[i] > 0) {
         if (sc.n[i] == 0) sc.n[i] = 1;
         nrm = sc.n[i] / sc.GN[i];
@@ -1360,20 +1370,26 @@ __device__ inline void scalars_norms(const DevSys &S) {
     }
 }
 
-// Chemical potentials (System::CreateMu src/system.cpp:3246-3428), single thread.
+// Chemical potentials (System::CreateMu src/system.cpp:3246-3428), single
+// thread.  A molecule with a tagged segment counts N - 1 segments; the tagged
+// segment has no fraction and the others are scaled by N / (N - 1).
 __device__ inline void scalars_mu(const DevSys &S) {
   DevScalars &sc = *S.sc;
   for (int i = 0; i < S.nmol; i++) {
-    const double NA = S.mol_N[i];
+    const double NA = S.mol_NF[i];
     double mu = log(NA * sc.n[i] / sc.GN[i]) + 1;
     double constant = 0;
-    for (int k = 0; k < S.nmol; k++) constant += sc.phibulk[k] / S.mol_N[k];
+    for (int k = 0; k < S.nmol; k++) constant += sc.phibulk[k] / S.mol_NF[k];
     mu = mu - NA * constant;
     for (int j = 0; j < S.nseg; j++) {
-      const double pbA = sc.seg_phibulk[j], FA = S.frac[i][j];
+      const double pbA = sc.seg_phibulk[j];
+      double FA = S.seg_tagged[j] ? 0.0 : S.frac[i][j];
+      if (S.mol_tagged[i]) FA *= (NA + 1) / (NA);
       for (int k = 0; k < S.nseg; k++) {
         const double chi = S.chi[j * kMaxSeg + k] / 2;
-        mu = mu - NA * chi * (pbA - FA) * (sc.seg_phibulk[k] - S.frac[i][k]);
+        double FB = S.seg_tagged[k] ? 0.0 : S.frac[i][k];
+        if (S.mol_tagged[i]) FB *= (NA + 1) / (NA);
+        mu = mu - NA * chi * (pbA - FA) * (sc.seg_phibulk[k] - FB);
       }
     }
     sc.Mu[i] = mu;

@@ -1872,7 +1872,7 @@ __device__ __noinline__ void fused_thermo(const DevSys &S0) {
   for (int mo = 0; mo < kMaxMol; mo++) {
     logc[mo] = 0.0;
     if (mo < nmol) {
-      const double N = S.mol_N[mo];
+      const double N = S.mol_NF[mo];  // a tagged segment does not count
       logc[mo] = log(N * sc.n[mo] / sc.GN[mo]) / N;
     }
   }

@@ -579,7 +579,7 @@ __global__ void __launch_bounds__(kGridThreads) k_thermo_partial(const DevSys *_
   for (int mo = 0; mo < kMaxMol; mo++) {
     logc[mo] = 0.0;
     if (mo < S.nmol) {
-      const double N = S.mol_N[mo];
+      const double N = S.mol_NF[mo];  // a tagged segment does not count
       logc[mo] = log(N * sc.n[mo] / sc.GN[mo]) / N;
     }
   }

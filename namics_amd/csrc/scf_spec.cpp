@@ -1145,9 +1145,18 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
       sg.freedom = SEG_FROZEN;
     else if (v == "pinned")
       sg.freedom = SEG_PINNED;
+    else if (v == "tagged")
+      sg.freedom = SEG_TAGGED;
     else {
-      err = "mon '" + sg.name + "': freedom '" + v + "' is not supported (free, frozen, pinned)";
+      err = "mon '" + sg.name + "': freedom '" + v + "' is not supported (free, frozen, pinned, tagged)";
       return false;
+    }
+    if (sg.freedom == SEG_TAGGED) {
+      if (!get(p, "mon", sg.name, "tagged_range", v)) {
+        err = "mon '" + sg.name + "': tagged segments need 'tagged_range'";
+        return false;
+      }
+      if (!parse_range(l, v, sg.mask, err)) return false;
     }
     if (sg.freedom == SEG_FROZEN) {
       if (!get(p, "mon", sg.name, "frozen_range", v)) {
@@ -1258,6 +1267,17 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
         return false;
       }
       if (s.seg[b.seg].freedom == SEG_PINNED) m.pinned = true;
+      if (s.seg[b.seg].freedom == SEG_TAGGED) {
+        if (m.tagged) {
+          err = "mol '" + m.name + "': one tagged segment per molecule is supported";
+          return false;
+        }
+        m.tagged = true;
+        if (b.n != 1) {
+          err = "mol '" + m.name + "': a tagged block must be a single segment";
+          return false;
+        }
+      }
       auto it = std::find(m.molmon.begin(), m.molmon.end(), b.seg);
       if (it == m.molmon.end()) {
         m.block_molmon.push_back((int)m.molmon.size());
@@ -1266,57 +1286,79 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
         m.block_molmon.push_back((int)(it - m.molmon.begin()));
       }
     }
-    if (!get(p, "mol", m.name, "freedom", v)) {
-      err = "mol '" + m.name + "' has no freedom";
-      return false;
-    }
-    s.mol_freedom_name.push_back(v);
-    if (v == "free")
-      m.freedom = MOL_FREE;
-    else if (v == "restricted")
-      m.freedom = MOL_RESTRICTED;
-    else if (v == "solvent")
-      m.freedom = MOL_SOLVENT;
-    else if (v == "neutralizer")
-      m.freedom = MOL_NEUTRALIZER;
-    else {
-      err = "mol '" + m.name + "': freedom '" + v + "' is not supported (free, restricted, solvent, neutralizer)";
-      return false;
-    }
-    if (m.pinned && m.freedom != MOL_RESTRICTED) {
-      err = "mol '" + m.name + "' is pinned and must be restricted";
-      return false;
-    }
-    if (m.freedom == MOL_FREE) {
-      if (!get(p, "mol", m.name, "phibulk", v) || !to_real(v, m.phibulk) || m.phibulk < 0 || m.phibulk > 1) {
-        err = "mol '" + m.name + "': free molecules need phibulk in 0..1";
+    if (m.tagged) {
+      // n is the number of tagged sites; freedom, theta, n, phibulk are
+      // ignored (src/system.cpp:606-611, src/molecule.cpp:578-580)
+      if (m.pinned || m.scheduled()) {
+        err = "mol '" + m.name + "': tagged molecules must be linear and unpinned";
         return false;
       }
-    }
-    if (m.freedom == MOL_RESTRICTED) {
-      bool hn = get(p, "mol", m.name, "n", v);
-      std::string vt;
-      bool ht = get(p, "mol", m.name, "theta", vt);
-      if (hn == ht) {
-        err = "mol '" + m.name + "': restricted molecules need exactly one of 'n' or 'theta'";
+      if (m.chainlength < 2) {  // F and mu divide by N - 1
+        err = "mol '" + m.name + "': a tagged molecule needs segments besides the tagged one";
         return false;
       }
-      if (hn) {
-        if (!to_real(v, m.n)) {
-          err = "mol '" + m.name + "': bad n";
+      int npos = 0;
+      for (const BlockSpec &b : m.blocks)
+        if (s.seg[b.seg].freedom == SEG_TAGGED)
+          for (int z = 0; z < l.M; z++) npos += s.seg[b.seg].mask[z];
+      m.freedom = MOL_TAGGED;
+      m.n = npos;
+      m.theta = m.n * m.chainlength;
+      s.mol_freedom_name.push_back("tagged");
+    }
+    if (!m.tagged) {
+      if (!get(p, "mol", m.name, "freedom", v)) {
+        err = "mol '" + m.name + "' has no freedom";
+        return false;
+      }
+      s.mol_freedom_name.push_back(v);
+      if (v == "free")
+        m.freedom = MOL_FREE;
+      else if (v == "restricted")
+        m.freedom = MOL_RESTRICTED;
+      else if (v == "solvent")
+        m.freedom = MOL_SOLVENT;
+      else if (v == "neutralizer")
+        m.freedom = MOL_NEUTRALIZER;
+      else {
+        err = "mol '" + m.name + "': freedom '" + v + "' is not supported (free, restricted, solvent, neutralizer)";
+        return false;
+      }
+      if (m.pinned && m.freedom != MOL_RESTRICTED) {
+        err = "mol '" + m.name + "' is pinned and must be restricted";
+        return false;
+      }
+      if (m.freedom == MOL_FREE) {
+        if (!get(p, "mol", m.name, "phibulk", v) || !to_real(v, m.phibulk) || m.phibulk < 0 || m.phibulk > 1) {
+          err = "mol '" + m.name + "': free molecules need phibulk in 0..1";
           return false;
         }
-        m.theta = m.n * m.chainlength;
-      } else {
-        if (!to_real(vt, m.theta)) {
-          err = "mol '" + m.name + "': bad theta";
+      }
+      if (m.freedom == MOL_RESTRICTED) {
+        bool hn = get(p, "mol", m.name, "n", v);
+        std::string vt;
+        bool ht = get(p, "mol", m.name, "theta", vt);
+        if (hn == ht) {
+          err = "mol '" + m.name + "': restricted molecules need exactly one of 'n' or 'theta'";
           return false;
         }
-        m.n = m.theta / m.chainlength;
-      }
-      if (m.theta < 0 || m.theta > l.volume) {
-        err = "mol '" + m.name + "': theta out of range 0..volume";
-        return false;
+        if (hn) {
+          if (!to_real(v, m.n)) {
+            err = "mol '" + m.name + "': bad n";
+            return false;
+          }
+          m.theta = m.n * m.chainlength;
+        } else {
+          if (!to_real(vt, m.theta)) {
+            err = "mol '" + m.name + "': bad theta";
+            return false;
+          }
+          m.n = m.theta / m.chainlength;
+        }
+        if (m.theta < 0 || m.theta > l.volume) {
+          err = "mol '" + m.name + "': theta out of range 0..volume";
+          return false;
+        }
       }
     }
     m.markov = 1;
@@ -1448,7 +1490,7 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
   // KSAM (System::generate_mask src/system.cpp:204-273)
   s.ksam.assign(l.M, 1);
   for (int i = 0; i < nseg; i++)
-    if (s.seg[i].freedom == SEG_FROZEN)
+    if (s.seg[i].freedom == SEG_FROZEN || s.seg[i].freedom == SEG_TAGGED)
       for (int z = 0; z < l.M; z++)
         if (s.seg[i].mask[z]) s.ksam[z] = 0;
 
@@ -1457,6 +1499,7 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
   std::vector<int> sysmon;
   for (auto &m : s.mol)
     for (int sg : m.molmon) {
+      if (s.seg[sg].freedom == SEG_TAGGED) continue;  // no field (SysTagList, src/system.cpp:466-474)
       if (std::find(sysmon.begin(), sysmon.end(), sg) != sysmon.end()) continue;
       sysmon.push_back(sg);
       int copy = -1;

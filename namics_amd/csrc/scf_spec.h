@@ -29,10 +29,15 @@ constexpr double kPIE = 3.14159265;
 enum Geometry : int { GEOM_PLANAR = 0, GEOM_CYLINDRICAL = 1, GEOM_SPHERICAL = 2 };
 enum BoundaryKind : int { BC_MIRROR = 0, BC_PERIODIC = 1, BC_SURFACE = 2 };
 enum LatticeType : int { LAT_SIMPLE_CUBIC = 0, LAT_HEXAGONAL = 1 };
-enum SegFreedom : int { SEG_FREE = 0, SEG_FROZEN = 1, SEG_PINNED = 2 };
+// SEG_TAGGED: the segment sits on its mask only, with weight 1 there and no
+// field (Segment::PrepareForCalculations src/segment.cpp:859-879).
+enum SegFreedom : int { SEG_FREE = 0, SEG_FROZEN = 1, SEG_PINNED = 2, SEG_TAGGED = 3 };
 // MOL_NEUTRALIZER: its bulk volume fraction follows from electroneutrality
 // of the bulk at every evaluation (System::ComputePhis src/system.cpp:2487-2516).
-enum MolFreedom : int { MOL_FREE = 0, MOL_RESTRICTED = 1, MOL_SOLVENT = 2, MOL_NEUTRALIZER = 3 };
+// MOL_TAGGED: a molecule with a tagged segment; n is the number of tagged
+// sites and its freedom key is ignored (Molecule::CheckInput
+// src/molecule.cpp:455-459,578-580, System::CheckInput src/system.cpp:606-611).
+enum MolFreedom : int { MOL_FREE = 0, MOL_RESTRICTED = 1, MOL_SOLVENT = 2, MOL_NEUTRALIZER = 3, MOL_TAGGED = 4 };
 enum SolverMethod : int {
   SOLVER_DIIS = 0,
   SOLVER_PSEUDOHESSIAN = 1,
@@ -102,6 +107,7 @@ struct MoleculeSpec {
   int chainlength = 0;
   int freedom = MOL_FREE;
   bool pinned = false;
+  bool tagged = false;         // contains a tagged segment (one per molecule)
   bool has_n = false, has_theta = false;
   double phibulk = 0.0, n = 0.0, theta = 0.0;
   int markov = 1;

@@ -2056,6 +2056,10 @@ struct Oracle {
       } else {
         Zero(phi[i].data(), M);
       }
+      if (sg.freedom == SEG_TAGGED) {  // no field; the weight is the mask, without KSAM
+        for (int z = 0; z < M; z++) G1[i][z] = sg.mask[z];
+        continue;
+      }
       lat.set_bounds(u[i].data());
       for (int z = 0; z < M; z++) G1[i][z] = std::exp(-u[i][z]);
       if (sg.freedom == SEG_PINNED)
@@ -2093,7 +2097,7 @@ struct Oracle {
           return;
         }
       }
-      if (m.pinned) {
+      if (m.pinned || m.tagged) {
         if (GN[i] > 0) {
           if (n[i] == 0) n[i] = 1;
           nrm = n[i] / GN[i];
@@ -2150,6 +2154,11 @@ struct Oracle {
       }
       if (m.freedom != MOL_RESTRICTED && m.freedom != MOL_SOLVENT) theta[i] = n[i] * m.chainlength;
     }
+    // a tagged segment's density is its mask (System::ComputePhis
+    // src/system.cpp:2596-2601); phitot keeps the molecule's contribution
+    for (int i = 0; i < nseg; i++)
+      if (s.seg[i].freedom == SEG_TAGGED)
+        for (int z = 0; z < M; z++) phi[i][z] = s.seg[i].mask[z];
     for (int i = 0; i < nseg; i++) lat.set_bounds(phi[i].data());
     for (int i = 0; i < nseg; i++) lat.side(phi_side[i].data(), phi[i].data());
     // residual (src/system.cpp:2099-2157)
@@ -2224,17 +2233,20 @@ struct Oracle {
     double *Fd = F_dens.data();
     Zero(Fd, M);
     for (int i = 0; i < nmol; i++) {
-      const int N = s.mol[i].chainlength;
+      // the tagged segment does not count (src/system.cpp:2861-2863)
+      const int N = s.mol[i].chainlength - (s.mol[i].tagged ? 1 : 0);
       double constant = std::log(N * n[i] / GN[i]) / N;
       Cp(TEMP.data(), mol_phitot[i].data(), M);
       Norm(TEMP.data(), constant, M);
       Add(Fd, TEMP.data(), M);
     }
-    // SysMonList: segments used by molecules, in order
+    // SysMonList: segments used by molecules, in order, tagged ones left
+    // out (src/system.cpp:455-459)
     std::vector<int> sysmon;
     for (auto &m : s.mol)
       for (int sg : m.molmon)
-        if (std::find(sysmon.begin(), sysmon.end(), sg) == sysmon.end()) sysmon.push_back(sg);
+        if (s.seg[sg].freedom != SEG_TAGGED && std::find(sysmon.begin(), sysmon.end(), sg) == sysmon.end())
+          sysmon.push_back(sg);
     for (int j : sysmon) {
       for (int z = 0; z < M; z++) TEMP[z] = G1[j][z] > 0 ? -std::log(G1[j][z]) : 0;
       Times(TEMP.data(), phi[j].data(), TEMP.data(), M);
@@ -2243,8 +2255,9 @@ struct Oracle {
     }
     for (int j = 0; j < nseg; j++) {
       for (int k = 0; k < nseg; k++) {
-        double chi = s.seg[k].freedom == SEG_FROZEN ? s.seg[j].chi[k] : s.seg[j].chi[k] / 2;
-        if (!(s.seg[j].freedom == SEG_FROZEN || chi == 0)) {
+        const bool fixk = s.seg[k].freedom == SEG_FROZEN || s.seg[k].freedom == SEG_TAGGED;
+        double chi = fixk ? s.seg[j].chi[k] : s.seg[j].chi[k] / 2;
+        if (!(s.seg[j].freedom == SEG_FROZEN || s.seg[j].freedom == SEG_TAGGED || chi == 0)) {
           Times(TEMP.data(), phi[j].data(), phi_side[k].data(), M);
           Norm(TEMP.data(), chi, M);
           Add(Fd, TEMP.data(), M);
@@ -2257,6 +2270,16 @@ struct Oracle {
       for (int a : m.molmon)
         for (int b : m.molmon) {
           double fA = fraction(i, a), fB = fraction(i, b);
+          if (m.tagged) {  // src/system.cpp:2994-3007
+            const int N = m.chainlength;
+            if (N > 1) {
+              fA = fA * N / (N - 1);
+              fB = fB * N / (N - 1);
+            } else {
+              fA = 0;
+              fB = 0;
+            }
+          }
           double chi = s.seg[a].chi[b] / 2;
           constant -= fA * fB * chi;
         }
@@ -2272,7 +2295,8 @@ struct Oracle {
     double *GPd = GP_dens.data();
     Zero(GPd, M);
     for (int i = 0; i < nmol; i++) {
-      const int N = s.mol[i].chainlength;
+      // tagged: N - 1 and no bulk (src/system.cpp:3053-3057)
+      const int N = s.mol[i].chainlength - (s.mol[i].tagged ? 1 : 0);
       Cp(TEMP.data(), mol_phitot[i].data(), M);
       for (int z = 0; z < M; z++) TEMP[z] = TEMP[z] + (-phibulk[i]);
       Norm(TEMP.data(), 1.0 / N, M);
@@ -2281,10 +2305,10 @@ struct Oracle {
     Add(GPd, alpha.data(), M);
     for (int i = 0; i < nseg; i++) lat.remove_bounds(phi_side[i].data());
     for (int j = 0; j < nseg; j++) {
-      if (s.seg[j].freedom == SEG_FROZEN) continue;
+      if (s.seg[j].freedom == SEG_FROZEN || s.seg[j].freedom == SEG_TAGGED) continue;
       double pbA = seg_phibulk[j];
       for (int k = 0; k < nseg; k++) {
-        if (s.seg[k].freedom == SEG_FROZEN) continue;
+        if (s.seg[k].freedom == SEG_FROZEN || s.seg[k].freedom == SEG_TAGGED) continue;
         double chi = s.seg[j].chi[k] / 2;
         double pbB = seg_phibulk[k];
         if (chi != 0) {
@@ -2314,17 +2338,26 @@ struct Oracle {
     GP = lat.weighted_sum(GPd);
 
     for (int i = 0; i < nmol; i++) {
-      const double NA = s.mol[i].chainlength;
+      // a tagged molecule counts N - 1 segments; its tagged segment has no
+      // fraction and the others are scaled to N / (N - 1)
+      // (src/system.cpp:3262-3340)
+      const bool tg = s.mol[i].tagged;
+      const double NA = s.mol[i].chainlength - (tg ? 1 : 0);
       double mu = std::log(NA * n[i] / GN[i]) + 1;
       double constant = 0;
-      for (int k = 0; k < nmol; k++) constant += phibulk[k] / s.mol[k].chainlength;
+      for (int k = 0; k < nmol; k++) {
+        const double NB = s.mol[k].chainlength - (s.mol[k].tagged ? 1 : 0);
+        constant += phibulk[k] / NB;
+      }
       mu = mu - NA * constant;
       for (int j = 0; j < nseg; j++) {
         double pbA = seg_phibulk[j];
-        double FA = fraction(i, j);
+        double FA = s.seg[j].freedom == SEG_TAGGED ? 0 : fraction(i, j);
+        if (tg) FA *= (NA + 1) / (NA);
         for (int k = 0; k < nseg; k++) {
           double pbB = seg_phibulk[k];
-          double FB = fraction(i, k);
+          double FB = s.seg[k].freedom == SEG_TAGGED ? 0 : fraction(i, k);
+          if (tg) FB *= (NA + 1) / (NA);
           double chi = s.seg[j].chi[k] / 2;
           mu = mu - NA * chi * (pbA - FA) * (pbB - FB);
         }
