@@ -232,6 +232,10 @@ int scf_write_pro(scf_system *sys, const char *path);
  * writes no 1-gradient vtk file).  Replaces: LGrad2::vtk / LGrad3::vtk
  * (src/LGrad2.cpp:159-183, src/LGrad3.cpp:48-79). */
 int scf_write_vtk(scf_system *sys, const char *path);
+/* Append the requested `vec` profiles to `path`, one line each: the name,
+ * then every site's value (ghosts included) as %e.  Replaces:
+ * Output::WriteOutput for vec (src/output.cpp:452-473). */
+int scf_write_vec(scf_system *sys, const char *path);
 /* Write the system's fields as a guess file.  Replaces: Lattice::StoreGuess
  * (src/lattice.cpp:971-1019). */
 int scf_store_guess(scf_system *sys, const char *path);

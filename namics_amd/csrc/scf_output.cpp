@@ -442,6 +442,48 @@ int scf_write_pro(scf_system *S, const char *path) {
   return SCF_OK;
 }
 
+int scf_write_vec(scf_system *S, const char *path) {
+  if (!S || !path) return fail(SCF_E_ARG, "null argument");
+  const SystemSpec &s = S->spec;
+  const LatticeSpec &l = s.lat;
+  const OutputRequest &req = s.out.vec;
+  State st;
+  int rc = load_state(S, st);
+  if (rc != SCF_OK) return rc;
+  FILE *fp = std::fopen(path, "a");
+  if (!fp) return fail(SCF_E_ARG, std::string("cannot open ") + path);
+  // one line per requested profile, every site ghosts included: segment
+  // densities with their bounds set (a frozen segment's is its mask),
+  // molecule densities with their bounds set, system profiles as they are
+  // (Output::WriteOutput src/output.cpp:452-473, Segment::GetPointer
+  // src/segment.cpp:1870-1882, Molecule::GetPointer src/molecule.cpp:2103-2113)
+  for (const OutputItem &it : req.items) {
+    const double *src = nullptr;
+    bool bounds = true;
+    if (it.prop == "phi" && it.key == "mon")
+      for (size_t k = 0; k < s.seg.size(); k++)
+        if (s.seg[k].name == it.name) {
+          src = st.seg_phi.data() + k * (size_t)l.M;
+          bounds = s.seg[k].freedom != SEG_FROZEN;
+        }
+    if (it.prop == "phi" && it.key == "mol")
+      for (size_t i = 0; i < s.mol.size(); i++)
+        if (s.mol[i].name == it.name) src = st.mol_phi.data() + i * (size_t)l.M;
+    if (!src) {
+      src = sys_profile(s, st, it);
+      bounds = false;
+    }
+    if (!src) continue;  // not a profile this system knows: left out, as the reference does
+    std::vector<double> X(src, src + l.M);
+    if (bounds) set_ghosts(l, X);
+    std::fprintf(fp, "%s\t", header_name(it, req.sep).c_str());
+    for (double v : X) std::fprintf(fp, "%e\t", v);
+    std::fprintf(fp, "\n");
+  }
+  std::fclose(fp);
+  return SCF_OK;
+}
+
 int scf_write_vtk(scf_system *S, const char *path) {
   if (!S || !path) return fail(SCF_E_ARG, "null argument");
   const SystemSpec &s = S->spec;
@@ -605,6 +647,8 @@ int scf_run_input_file(const char *path, int *n_starts_out, int *n_converged_out
         rc = scf_write_pro(S, (s.out.path + stem + tag + ".pro").c_str());
       if (rc == SCF_OK && s.out.vtk.present && s.out.vtk.write)
         rc = scf_write_vtk(S, (s.out.path + stem + tag + ".vtk").c_str());
+      if (rc == SCF_OK && s.out.vec.present && s.out.vec.write)
+        rc = scf_write_vec(S, (s.out.path + stem + ".vec").c_str());
       if (rc == SCF_OK && s.final_guess == "file") {
         std::string name;
         const std::string gtag = "_" + std::to_string(start) + (subl > 0 ? "_" + std::to_string(subl) : std::string());

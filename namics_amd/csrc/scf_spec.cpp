@@ -1723,14 +1723,15 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     }
   }
   // output requests (Output::CheckInput / Load, src/output.cpp:32-178):
-  // `output : kal | pro : option : value` and `kal | pro : key : name : prop`
+  // `output : kal | pro | vtk | vec : option : value` and
+  // `kal | pro | vtk | vec : key : name : prop`
   // item lines in input order; name `*` expands to every molecule / segment
   OutputSpec &out = s.out;
   for (auto &ln : p.lines) {
     if (ln.key == "out_info" && ln.name == "folder" && ln.prop == "path" && !ln.value.empty())
       out.path = ln.value.back() == '/' ? ln.value : ln.value + "/";
-    if (ln.key != "output" || (ln.name != "kal" && ln.name != "pro" && ln.name != "vtk")) continue;
-    OutputRequest &r = ln.name == "kal" ? out.kal : (ln.name == "pro" ? out.pro : out.vtk);
+    if (ln.key != "output" || (ln.name != "kal" && ln.name != "pro" && ln.name != "vtk" && ln.name != "vec")) continue;
+    OutputRequest &r = ln.name == "kal" ? out.kal : (ln.name == "pro" ? out.pro : (ln.name == "vtk" ? out.vtk : out.vec));
     r.present = true;
     bool b;
     if (ln.prop == "append" && to_bool(ln.value, b)) {
@@ -1745,9 +1746,10 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
   }
   out.pro.append = false;  // a profile file is always rewritten
   out.vtk.append = false;
+  out.vec.append = true;  // a vec file is always appended to (src/output.cpp:115-124, 452-455)
   for (auto &ln : p.lines) {
-    if (ln.key != "kal" && ln.key != "pro" && ln.key != "vtk") continue;
-    OutputRequest &r = ln.key == "kal" ? out.kal : (ln.key == "pro" ? out.pro : out.vtk);
+    if (ln.key != "kal" && ln.key != "pro" && ln.key != "vtk" && ln.key != "vec") continue;
+    OutputRequest &r = ln.key == "kal" ? out.kal : (ln.key == "pro" ? out.pro : (ln.key == "vtk" ? out.vtk : out.vec));
     if (!r.present) continue;
     const std::string &k = ln.name;
     if (ln.prop == "*" && (k == "mol" || k == "mon")) {

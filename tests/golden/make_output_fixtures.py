@@ -1,7 +1,7 @@
 """Generate the output-format fixtures from the reference Namics program.
 
-Each input below is a case of tests/golden/cases.py with `kal` / `pro`
-requests, DIIS to 1e-9 and, for some, a second start block and a final guess
+Each input below is a case of tests/golden/cases.py (or charged_cases.py)
+with `kal` / `pro` / `vtk` / `vec` requests, DIIS to 1e-9 and, for some, a second start block and a final guess
 file.  The reference (oracle/_ref/namics, built from its own sources) runs it
 in a scratch directory; every file it writes to output/ is recorded as text.
 Run here (where the reference binary is built); writes
@@ -146,6 +146,19 @@ pro : sys : NN : q
 var : mon-A : scan : valence
 var : mon-A : step : -0.2
 var : mon-A : end_value : -0.6
+""" + DIIS + "start\n",
+    # vec files: every requested profile on one line per solve, appended
+    # (two start blocks), a frozen segment's mask without its bounds set
+    "vec1d": CASES["cfg1_planar_adsorption"]["text"] + """output : vec : append : false
+vec : mon : S : phi
+vec : mon : A : phi
+vec : mol : * : phi
+vec : mon : A : nonsense
+""" + DIIS + "start\nmon : A : chi_W : 0.6\nstart\n",
+    "vec_charged": CHARGED["el1d_poly"]["text"] + """output : vec : append : true
+vec : sys : NN : psi
+vec : sys : NN : q
+vec : mon : A : phi
 """ + DIIS + "start\n",
     "box3d": CASES["diblock3d_periodic"]["text"] + """output : kal : append : false
 kal : sys : NN : free_energy

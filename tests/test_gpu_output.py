@@ -3,7 +3,7 @@
 Each fixture input (tests/golden/make_output_fixtures.py) was run by the
 reference program; here the same input file is run by scf_run_input_file in
 a scratch directory and every file the reference wrote is compared with the
-one written here (kal, pro, vtk and guess files): the same file names, the
+one written here (kal, pro, vtk, vec and guess files): the same file names, the
 same header lines and row counts,
 the same coordinate columns (text), the same strings, integers (iteration
 counts within the drift DIIS shows between builds) and "NiN" cells, and
@@ -114,6 +114,14 @@ def test_output_files_match_the_reference(name, tmp_path, monkeypatch):
             vr = np.array([float(v) for v in lr[10:] if v])
             assert vo.shape == vr.shape
             assert np.all(np.abs(vo - vr) <= 2e-6 * np.abs(vr) + 1.5e-6)
+        elif fn.endswith(".vec"):  # one line per profile and solve: name, then %e values
+            lo, lr = ours.strip("\n").split("\n"), ref.strip("\n").split("\n")
+            assert len(lo) == len(lr)
+            for a, b in zip(lo, lr):
+                ca, cb = a.rstrip("\t").split("\t"), b.rstrip("\t").split("\t")
+                assert ca[0] == cb[0] and len(ca) == len(cb), (ca[0], cb[0])
+                vo, vr = np.array([float(v) for v in ca[1:]]), np.array([float(v) for v in cb[1:]])
+                assert np.all(np.abs(vo - vr) <= 2e-6 * np.abs(vr) + 1e-7 * np.max(np.abs(vr))), cb[0]
         else:  # guess file: header lines, then %e values
             lo, lr = ours.split("\n"), ref.split("\n")
             k = 4 + int(lr[3]) + 1
