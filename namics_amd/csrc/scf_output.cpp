@@ -649,6 +649,15 @@ int scf_run_input_file(const char *path, int *n_starts_out, int *n_converged_out
         rc = scf_write_vtk(S, (s.out.path + stem + tag + ".vtk").c_str());
       if (rc == SCF_OK && s.out.vec.present && s.out.vec.write)
         rc = scf_write_vec(S, (s.out.path + stem + ".vec").c_str());
+      if (rc == SCF_OK && s.out.pos.present && s.out.pos.write) {
+        // pos prints integer arrays, and no object offers one
+        // (Segment/Molecule/System::GetPointerInt return none,
+        // src/segment.cpp:1922-1930): the file is opened for appending and
+        // stays as it was (src/output.cpp:429-450)
+        FILE *fp = std::fopen((s.out.path + stem + ".pos").c_str(), "a");
+        if (!fp) rc = fail(SCF_E_ARG, "cannot open " + s.out.path + stem + ".pos");
+        else std::fclose(fp);
+      }
       if (rc == SCF_OK && s.final_guess == "file") {
         std::string name;
         const std::string gtag = "_" + std::to_string(start) + (subl > 0 ? "_" + std::to_string(subl) : std::string());

@@ -1723,15 +1723,21 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     }
   }
   // output requests (Output::CheckInput / Load, src/output.cpp:32-178):
-  // `output : kal | pro | vtk | vec : option : value` and
+  // `output : kal | pro | vtk | vec | pos : option : value` and
   // `kal | pro | vtk | vec : key : name : prop`
   // item lines in input order; name `*` expands to every molecule / segment
   OutputSpec &out = s.out;
   for (auto &ln : p.lines) {
     if (ln.key == "out_info" && ln.name == "folder" && ln.prop == "path" && !ln.value.empty())
       out.path = ln.value.back() == '/' ? ln.value : ln.value + "/";
-    if (ln.key != "output" || (ln.name != "kal" && ln.name != "pro" && ln.name != "vtk" && ln.name != "vec")) continue;
-    OutputRequest &r = ln.name == "kal" ? out.kal : (ln.name == "pro" ? out.pro : (ln.name == "vtk" ? out.vtk : out.vec));
+    if (ln.key != "output" || (ln.name != "kal" && ln.name != "pro" && ln.name != "vtk" && ln.name != "vec" &&
+                               ln.name != "pos"))
+      continue;
+    OutputRequest &r = ln.name == "kal"   ? out.kal
+                       : ln.name == "pro" ? out.pro
+                       : ln.name == "vtk" ? out.vtk
+                       : ln.name == "vec" ? out.vec
+                                          : out.pos;
     r.present = true;
     bool b;
     if (ln.prop == "append" && to_bool(ln.value, b)) {

@@ -209,7 +209,7 @@ struct OutputRequest {
 };
 
 struct OutputSpec {
-  OutputRequest kal, pro, vtk, vec;
+  OutputRequest kal, pro, vtk, vec, pos;
   bool use_output_folder = true;     // strip directories from the file stem
   std::string path = "./output/";    // `out_info : folder : path : ...`
 };

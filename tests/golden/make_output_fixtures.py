@@ -148,12 +148,15 @@ var : mon-A : step : -0.2
 var : mon-A : end_value : -0.6
 """ + DIIS + "start\n",
     # vec files: every requested profile on one line per solve, appended
-    # (two start blocks), a frozen segment's mask without its bounds set
+    # (two start blocks), a frozen segment's mask without its bounds set;
+    # pos files: no object offers an integer array, so the file stays empty
     "vec1d": CASES["cfg1_planar_adsorption"]["text"] + """output : vec : append : false
 vec : mon : S : phi
 vec : mon : A : phi
 vec : mol : * : phi
 vec : mon : A : nonsense
+output : pos : append : true
+pos : mon : S : phi
 """ + DIIS + "start\nmon : A : chi_W : 0.6\nstart\n",
     "vec_charged": CHARGED["el1d_poly"]["text"] + """output : vec : append : true
 vec : sys : NN : psi
