@@ -93,7 +93,8 @@ typedef struct {
  * same one-workgroup kernels.  A linear molecule may hold one tagged
  * segment (`mon : <name> : freedom : tagged`, `tagged_range`): its weight
  * and density are the mask, and the molecule holds one chain per tagged
- * site.
+ * site.  Linear rings (`mol : <name> : ring : true`, first and last blocks
+ * one segment of the same type) run on 1-gradient lattices with fjc 1.
  * Replaces: Input + Lattice/Segment/Molecule/System/Solve_scf::CheckInput and
  * AllocateMemory (src/namics.cpp:153-417, src/lattice.cpp:267-601), the
  * composition parser Molecule::Decomposition / GenerateTree / Interpret

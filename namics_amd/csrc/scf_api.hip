@@ -156,6 +156,10 @@ int check_layout(const scf_system *S, int mode, int strip_sx) {
   if ((S->spec.charged || S->literal()) && mode != SCF_MODE_FUSED)
     return fail(SCF_E_UNSUPPORTED, "charged systems and 2- or 3-gradient surface boundaries run on the fused "
                                    "(one workgroup per system) path");
+  if (S->has_ring() && (mode != SCF_MODE_FUSED || S->spec.lat.M > 1024 ||
+                        (size_t)2 * S->spec.lat.M * sizeof(double) > lds_limit()))
+    return fail(SCF_E_UNSUPPORTED, "ring molecules run on the fused (one workgroup per system) path of 1-gradient "
+                                   "lattices up to 1022 layers");
   bool markov1 = true;
   for (auto &m : S->spec.mol) markov1 = markov1 && m.markov == 1;
   const bool strip = mode == SCF_MODE_FUSED && strip_sx > 0;
@@ -175,8 +179,10 @@ int check_layout(const scf_system *S, int mode, int strip_sx) {
 // share of segment seg in molecule m (Molecule::fraction src/molecule.cpp:2686-2697,
 // with the blocks' degeneracy for dendrimers, mol_dend::fraction src/mol_dendrimer.cpp:12-22)
 double fraction(const MoleculeSpec &m, int seg) {
+  // a ring's first segment is merged with its last and not counted
+  // (Molecule::fraction src/molecule.cpp:2686-2697)
   int n = 0;
-  for (size_t b = 0; b < m.blocks.size(); b++)
+  for (size_t b = m.ring ? 1 : 0; b < m.blocks.size(); b++)
     if (m.blocks[b].seg == seg) n += m.blocks[b].n * (b < m.block_deg.size() ? m.block_deg[b] : 1);
   return 1.0 * n / m.chainlength;
 }
@@ -366,6 +372,7 @@ int build(scf_system *S, const Placement *pl) {
     h.mol_freedom[i] = m.freedom;
     h.mol_pinned[i] = m.pinned;
     h.mol_tagged[i] = m.tagged;
+    h.mol_ring[i] = m.ring;
     h.mol_NF[i] = m.chainlength - (m.tagged ? 1 : 0);
     h.mol_markov[i] = m.markov;
     h.mol_size[i] = m.size;

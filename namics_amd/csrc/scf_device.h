@@ -91,6 +91,7 @@ struct DevSys {
   int nmol, solvent, nblocks, nmm;
   int mol_N[kMaxMol], mol_freedom[kMaxMol], mol_pinned[kMaxMol], mol_markov[kMaxMol], mol_size[kMaxMol];
   int mol_tagged[kMaxMol], mol_NF[kMaxMol];  // holds a tagged segment; N less the tagged segment
+  int mol_ring[kMaxMol];                      // ring: first and last one-segment blocks merged
   int mol_block_begin[kMaxMol + 1];
   int mol_mm_begin[kMaxMol + 1];
   int block_seg[kMaxBlocks], block_n[kMaxBlocks], block_mm[kMaxBlocks];

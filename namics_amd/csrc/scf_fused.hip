@@ -228,6 +228,116 @@ __device__ __noinline__ void chain1d_m1(const DevSys &S0, int mo) {
   __syncthreads();
 }
 
+// Ring of a first-order chain on a 1-gradient lattice (fjc = 1), one site
+// per thread (mol_linear::ComputePhi src/mol_linear.cpp:35-111).  The first
+// and last blocks are one segment each of the same type, merged: for every
+// interior site x where that segment may sit, a chain starts there
+// (row 0 = delta_x G1), runs through the middle blocks (rows 1 .. N-1, kept
+// in HBM) and ends on x (row N, needed on x only); GN sums the end weights
+// over x.  The merged segment's density on x is G1 times that end weight;
+// the middle blocks' densities come from the backward pass started at
+// delta_x G1, with the same stencil, ghost reads and composition order as
+// chain1d_m1.  Densities and GN collect over the starts in registers.
+__device__ __noinline__ void chain1d_ring(const DevSys &S0, int mo) {
+  CSys &S = uniform_sys(S0);
+  const int M = S.M;
+  const int i = threadIdx.x;
+  const bool active = i < M;
+  const int ii = active ? i : 0;
+  const int i0 = map_x(S, ii);
+  const int im = ii >= 1 ? map_x(S, ii - 1) : i0;
+  const int ip = ii <= M - 2 ? map_x(S, ii + 1) : i0;
+  const bool planar = S.geometry == 0;
+  double wm, w0, wp;
+  if (planar) {
+    wm = S.lambda;
+    w0 = 1.0 - 2.0 * S.lambda;
+    wp = S.lambda;
+  } else {
+    wm = hbm(S.lam_m)[ii];
+    w0 = hbm(S.lam_0)[ii];
+    wp = hbm(S.lam_p)[ii];
+  }
+  if (ii < 1) wm = 0.0;
+  if (ii > M - 2) wp = 0.0;
+  auto step = [&](const ldbl *row) {
+    double t;
+    if (planar) {
+      t = wm * row[im];
+      t += w0 * row[i0];
+    } else {
+      t = w0 * row[i0];
+      t += wm * row[im];
+    }
+    t += wp * row[ip];
+    return t;
+  };
+  const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
+  const int N = S.mol_N[mo];
+  gdbl *Gf = hbm(S.Gf);
+  const gdbl *G1 = hbm(S.G1);
+  ldbl *ra = lds_row(0), *rb = lds_row(S.row_stride);
+  const int sa = S.block_seg[b0];
+  const double g1a = active ? G1[(size_t)sa * M + ii] : 0.0;
+  const double wsite = active && ii >= 1 && ii <= S.MX ? (planar ? 1.0 : hbm(S.L)[ii]) : 0.0;
+  double gnacc = 0.0, racc_a = 0.0;
+  for (int x = 1; x <= S.MX; x++) {
+    if (!(G1[(size_t)sa * M + x] > 0)) continue;  // the same on every thread
+    if (active) ra[ii] = ii == x ? g1a : 0.0;
+    lds_barrier();
+    ldbl *prev = ra, *next = rb;
+    int s = 1;
+    for (int b = b0 + 1; b < b1 - 1; b++) {
+      const double g1 = active ? G1[(size_t)S.block_seg[b] * M + ii] : 0.0;
+      for (int j = 0; j < S.block_n[b]; j++, s++) {
+        if (active) {
+          const double v = step(prev) * g1;
+          Gf[(size_t)s * M + ii] = v;
+          next[ii] = v;
+        }
+        lds_barrier();
+        ldbl *tmp = prev;
+        prev = next;
+        next = tmp;
+      }
+    }
+    // the merged end segment on x (the row masked to x)
+    double last = 0.0;
+    if (active && ii == x) last = step(prev) * g1a;
+    gnacc += wsite * last;
+    racc_a += g1a * last;
+    lds_barrier();  // every thread has read prev before it is overwritten
+    // backward from delta_x G1 through the middle blocks
+    ldbl *ga = prev, *gb = next;
+    if (active) ga[ii] = ii == x ? g1a : 0.0;
+    double gf = 0.0;
+    if (active && N - 1 >= 1) gf = Gf[(size_t)(N - 1) * M + i0];
+    lds_barrier();
+    s = N - 1;
+    for (int b = b1 - 2; b >= b0 + 1; b--) {
+      const double g1 = active ? G1[(size_t)S.block_seg[b] * M + ii] : 0.0;
+      double racc = 0.0;
+      for (int j = 0; j < S.block_n[b]; j++, s--) {
+        if (active) {
+          const double v = step(ga) * g1;
+          racc += gf * v;
+          gb[ii] = v;
+          if (s - 1 >= 1) gf = Gf[(size_t)(s - 1) * M + i0];
+        }
+        lds_barrier();
+        ldbl *tmp = ga;
+        ga = gb;
+        gb = tmp;
+      }
+      if (active) hbm(S.rho)[(size_t)S.block_mm[b] * M + ii] += racc;
+    }
+  }
+  if (active) hbm(S.rho)[(size_t)S.block_mm[b0] * M + ii] += racc_a;
+  const double gn = wg_sum(gnacc);
+  if (threadIdx.x == 0) S0.sc->GN[mo] = gn;
+  __syncthreads();
+}
+
 // Second-order (Markov 2) chain on a 1-gradient simple-cubic lattice, one
 // site per thread (M <= threads), rows in LDS: three direction components per
 // site (+x, lateral, -x), the stencil of m2_step, composition weights
@@ -1804,6 +1914,8 @@ __device__ double fused_residual(const DevSys &S0) {
         else
           chain_markov2<false, M2_D3>(S0, mo);
       }
+    } else if (S.mol_ring[mo]) {
+      chain1d_ring(S0, mo);
     } else if (S.all_sites) {
       chain_literal(S0, mo);
     } else if (S.strip_sx > 0) {

@@ -54,6 +54,11 @@ struct scf_system {
   // potential reads densities on ghosts) and surface boundaries (a surface
   // ghost keeps its own value).  They run one workgroup per system with the
   // chains over every site (chain_literal).
+  bool has_ring() const {
+    for (auto &m : spec.mol)
+      if (m.ring) return true;
+    return false;
+  }
   bool literal() const {
     if (spec.lat.gradients < 2) return false;
     if (spec.charged) return true;
@@ -67,7 +72,7 @@ struct scf_system {
     // branched molecules and dendrimers run their row schedules on the grid path
     if (branched()) return SCF_MODE_GRID;
     // charged systems solve the potential in the one-workgroup kernels
-    if (spec.charged || literal()) return SCF_MODE_FUSED;
+    if (spec.charged || literal() || has_ring()) return SCF_MODE_FUSED;
     // checkpointed end-point storage runs on the strip chains or the grid path
     if (h.ckpt > 1 && h.strip_sx == 0) return SCF_MODE_GRID;
     if (tuned_mode != SCF_MODE_AUTO) return tuned_mode;

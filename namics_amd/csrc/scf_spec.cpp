@@ -1286,6 +1286,26 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
         m.block_molmon.push_back((int)(it - m.molmon.begin()));
       }
     }
+    // ring : true (Molecule::CheckInput src/molecule.cpp:584-625): the first
+    // and last blocks of a linear chain, one segment each of the same type,
+    // are merged and the chain is one segment shorter; a chain that does not
+    // qualify stays linear, as in the reference
+    if (get(p, "mol", m.name, "ring", v)) {
+      bool r = false;
+      if (!to_bool(v, r)) {
+        err = "mol '" + m.name + "': ring is either 'true' or 'false'";
+        return false;
+      }
+      if (r && m.scheduled()) {
+        err = "mol '" + m.name + "': rings of branched molecules, dendrimers and combs are not supported";
+        return false;
+      }
+      const size_t nb = m.blocks.size();
+      if (r && nb >= 2 && m.blocks[0].seg == m.blocks[nb - 1].seg && m.blocks[0].n == 1 && m.blocks[nb - 1].n == 1) {
+        m.ring = true;
+        m.chainlength--;
+      }
+    }
     if (m.tagged) {
       // n is the number of tagged sites; freedom, theta, n, phibulk are
       // ignored (src/system.cpp:606-611, src/molecule.cpp:578-580)
@@ -1342,18 +1362,22 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
           err = "mol '" + m.name + "': restricted molecules need exactly one of 'n' or 'theta'";
           return false;
         }
+        // a ring's n and theta convert with the composition's length: the
+        // reference reads them before it shortens the chain
+        // (Molecule::CheckInput src/molecule.cpp:545-546 before :609)
+        const int nin = m.chainlength + (m.ring ? 1 : 0);
         if (hn) {
           if (!to_real(v, m.n)) {
             err = "mol '" + m.name + "': bad n";
             return false;
           }
-          m.theta = m.n * m.chainlength;
+          m.theta = m.n * nin;
         } else {
           if (!to_real(vt, m.theta)) {
             err = "mol '" + m.name + "': bad theta";
             return false;
           }
-          m.n = m.theta / m.chainlength;
+          m.n = m.theta / nin;
         }
         if (m.theta < 0 || m.theta > l.volume) {
           err = "mol '" + m.name + "': theta out of range 0..volume";
@@ -1373,12 +1397,11 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     }
     m.save_memory = false;
     if (get(p, "mol", m.name, "save_memory", v)) to_bool(v, m.save_memory);
-    if (get(p, "mol", m.name, "ring", v)) {
-      bool r = false;
-      if (to_bool(v, r) && r) {
-        err = "mol '" + m.name + "': rings are not supported";
-        return false;
-      }
+    if (m.ring && (m.markov != 1 || m.save_memory || m.pinned || m.tagged || l.gradients != 1 || l.fjc != 1)) {
+      err = "mol '" + m.name +
+            "': rings are supported for first-order chains on 1-gradient lattices with fjc 1, without save_memory, "
+            "pinned or tagged segments";
+      return false;
     }
     m.size = 1;
     if (m.markov == 2) {
@@ -1439,6 +1462,10 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     double q = 0;
     for (const BlockSpec &b : m.blocks) q += s.seg[b.seg].valence * b.n;
     s.mol_charge[i] = q / m.chainlength;
+    if (q != 0 && m.ring) {
+      err = "mol '" + m.name + "': charged rings are not supported";
+      return false;
+    }
     if (q != 0) {
       if (m.dendrimer || m.comb) {
         err = "mol '" + m.name + "': charged dendrimers and combs are not supported";
@@ -1800,7 +1827,8 @@ std::string describe_json(const SystemSpec &s) {
     o << "{\"name\":";
     str(m.name);
     o << ",\"chainlength\":" << m.chainlength << ",\"freedom\":" << m.freedom << ",\"markov\":" << m.markov
-      << ",\"pinned\":" << (m.pinned ? "true" : "false") << ",\"save_memory\":" << (m.save_memory ? "true" : "false")
+      << ",\"pinned\":" << (m.pinned ? "true" : "false") << ",\"ring\":" << (m.ring ? "true" : "false")
+      << ",\"save_memory\":" << (m.save_memory ? "true" : "false")
       << ",\"size\":" << m.size << ",\"blocks\":" << m.blocks.size() << ",\"molmon\":" << m.molmon.size();
     if (m.scheduled()) {
       // the branched molecule's or dendrimer's blocks, generations and row schedule

@@ -108,6 +108,7 @@ struct MoleculeSpec {
   int freedom = MOL_FREE;
   bool pinned = false;
   bool tagged = false;         // contains a tagged segment (one per molecule)
+  bool ring = false;           // linear ring: first and last one-segment blocks merged
   bool has_n = false, has_theta = false;
   double phibulk = 0.0, n = 0.0, theta = 0.0;
   int markov = 1;

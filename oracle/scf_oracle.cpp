@@ -1357,8 +1357,8 @@ struct Oracle {
   // their degeneracy (mol_dend::fraction src/mol_dendrimer.cpp:12-22)
   double fraction(int mol, int seg) const {
     const MoleculeSpec &m = s.mol[mol];
-    int Nseg = 0;
-    for (size_t b = 0; b < m.blocks.size(); b++)
+    int Nseg = 0;  // a ring's first segment is not counted (src/molecule.cpp:2686-2697)
+    for (size_t b = m.ring ? 1 : 0; b < m.blocks.size(); b++)
       if (m.blocks[b].seg == seg) Nseg += m.blocks[b].n * (b < m.block_deg.size() ? m.block_deg[b] : 1);
     return 1.0 * Nseg / m.chainlength;
   }
@@ -1409,6 +1409,53 @@ struct Oracle {
   }
 
   // Molecule::ComputePhi for a linear chain; accumulates Gf*Gb into mol_phi
+  // mol_linear::ComputePhi for a ring (src/mol_linear.cpp:35-111), first
+  // order, one gradient: for every interior site x where the first segment
+  // may sit, a chain starts there (G0 = delta_x G1), runs through the middle
+  // blocks and ends on x with the last segment (the row masked to x); GN is
+  // the sum over x, the merged end segment's density is G0 times that row,
+  // and the middle segments' densities come from the backward pass started
+  // at G0.  Rows: 0 .. N (N = chainlength, one less than the composition).
+  void compute_phi_ring(int mi) {
+    const MoleculeSpec &m = s.mol[mi];
+    Lat lat(s.lat);
+    const int N = m.chainlength;
+    vec &rho = mol_phi[mi];
+    std::fill(rho.begin(), rho.end(), 0.0);
+    std::vector<int> seg_of, mm_of;
+    for (size_t b = 0; b < m.blocks.size(); b++)
+      for (int j = 0; j < m.blocks[b].n; j++) {
+        seg_of.push_back(m.blocks[b].seg);
+        mm_of.push_back(m.block_molmon[b]);
+      }
+    if ((int)Gf.size() < N + 1) Gf.resize(N + 1);
+    for (int k = 0; k <= N; k++) Gf[k].assign(M, 0.0);
+    Gb.assign(2 * M, 0.0);
+    vec G0(M, 0.0);
+    const double *g1a = G1[seg_of[0]].data();
+    double gn = 0;
+    for (int x = 1; x < s.lat.MX + 1; x++) {
+      if (!(g1a[x] > 0)) continue;
+      Zero(G0.data(), M);
+      G0[x] = g1a[x];
+      Cp(Gf[0].data(), G0.data(), M);
+      for (int k = 1; k <= N; k++) lat.propagate(Gf[k].data(), Gf[k - 1].data(), G1[seg_of[k]].data());
+      double *last = Gf[N].data();
+      for (int z = 0; z < M; z++)
+        if (z != x) last[z] = 0.0;
+      gn += lat.weighted_sum(last);
+      double *gb = Gb.data() + (N % 2) * M;
+      Cp(gb, G0.data(), M);
+      YplusisCtimesAtimesB(rho.data() + mm_of[0] * M, gb, last, 1.0, M);
+      for (int k = N - 1; k >= 1; k--) {
+        gb = Gb.data() + (k % 2) * M;
+        lat.propagate(gb, Gb.data() + ((k + 1) % 2) * M, G1[seg_of[k]].data());
+        YplusisCtimesAtimesB(rho.data() + mm_of[k] * M, Gf[k].data(), gb, 1.0, M);
+      }
+    }
+    GN[mi] = gn;
+  }
+
   void compute_phi_mol(int mi) {
     const MoleculeSpec &m = s.mol[mi];
     if (m.scheduled()) {
@@ -1430,6 +1477,10 @@ struct Oracle {
           mm_of[k] = m.block_molmon[b];
           k++;
         }
+    }
+    if (m.ring) {
+      compute_phi_ring(mi);
+      return;
     }
     if ((int)Gf.size() < N) Gf.resize(N);
     for (int k = 0; k < N; k++) Gf[k].assign(W, 0.0);
