@@ -94,7 +94,8 @@ typedef struct {
  * segment (`mon : <name> : freedom : tagged`, `tagged_range`): its weight
  * and density are the mask, and the molecule holds one chain per tagged
  * site.  Linear rings (`mol : <name> : ring : true`, first and last blocks
- * one segment of the same type) run on 1-gradient lattices with fjc 1.
+ * one segment of the same type) run with first-order chains on lattices
+ * with fjc 1 (not with 2- or 3-gradient surfaces or charges).
  * Replaces: Input + Lattice/Segment/Molecule/System/Solve_scf::CheckInput and
  * AllocateMemory (src/namics.cpp:153-417, src/lattice.cpp:267-601), the
  * composition parser Molecule::Decomposition / GenerateTree / Interpret

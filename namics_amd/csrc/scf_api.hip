@@ -156,10 +156,12 @@ int check_layout(const scf_system *S, int mode, int strip_sx) {
   if ((S->spec.charged || S->literal()) && mode != SCF_MODE_FUSED)
     return fail(SCF_E_UNSUPPORTED, "charged systems and 2- or 3-gradient surface boundaries run on the fused "
                                    "(one workgroup per system) path");
-  if (S->has_ring() && (mode != SCF_MODE_FUSED || S->spec.lat.M > 1024 ||
-                        (size_t)2 * S->spec.lat.M * sizeof(double) > lds_limit()))
-    return fail(SCF_E_UNSUPPORTED, "ring molecules run on the fused (one workgroup per system) path of 1-gradient "
-                                   "lattices up to 1022 layers");
+  if (S->has_ring() && (mode != SCF_MODE_FUSED || S->literal() ||
+                        (S->spec.lat.gradients == 1 && (S->spec.lat.M > 1024 ||
+                                                        (size_t)2 * S->spec.lat.M * sizeof(double) > lds_limit()))))
+    return fail(SCF_E_UNSUPPORTED, "ring molecules run on the fused (one workgroup per system) path, on 1-gradient "
+                                   "lattices up to 1022 layers, and without surface boundaries or charges in 2 and 3 "
+                                   "gradients");
   bool markov1 = true;
   for (auto &m : S->spec.mol) markov1 = markov1 && m.markov == 1;
   const bool strip = mode == SCF_MODE_FUSED && strip_sx > 0;
@@ -210,7 +212,7 @@ void plan_layout(scf_system *S, int want_sx) {
   // Markov-1 chains, the 9-point stencil, ghosts that map onto interior
   // sites, and the rows of one strip layout in LDS
   const bool strip_ok = l.gradients == 2 && l.fjc == 1 && maxR == 1 && (l.geometry != 0 || l.stencil_full) &&
-                        !S->literal() &&
+                        !S->literal() && !S->has_ring() &&
                         l.MX + 2 <= 1026 && h.bx1 >= 1 && h.bx1 <= l.MX && h.bxm >= 1 && h.bxm <= l.MX &&
                         h.by1 >= 1 && h.by1 <= l.MY && h.bym >= 1 && h.bym <= l.MY;
   for (int sx : {8, 4}) {

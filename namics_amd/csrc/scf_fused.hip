@@ -611,6 +611,115 @@ __device__ __noinline__ void chain_markov1(const DevSys &S0, int mo) {
   __syncthreads();
 }
 
+// Ring of a first-order chain on a 2- or 3-gradient lattice (fjc = 1), the
+// sites strided over the workgroup as in chain_markov1: the same chain as
+// chain1d_ring (a start on every interior site where the merged end segment
+// may sit, mol_linear::ComputePhi src/mol_linear.cpp:35-111), with the
+// lattice's own stencil.  A site belongs to one thread throughout, so the
+// merged segment's density on the start site is added by its owner.
+template <bool kLds>
+__device__ __noinline__ void chain_ring(const DevSys &S0, int mo) {
+  typedef typename Rows<kLds>::T RowT;
+  CSys &S = uniform_sys(S0);
+  const int M = S.M, W = n_work_sites(S);
+  const int tid = work_tid(S), nt = S.threads;
+  const int b0 = S.mol_block_begin[mo], b1 = S.mol_block_begin[mo + 1];
+  const int N = S.mol_N[mo];
+  gdbl *Gf = hbm(S.Gf);
+  const gdbl *G1 = hbm(S.G1);
+  const gdbl *g1a = G1 + (size_t)S.block_seg[b0] * M;
+  gdbl *rho_a = hbm(S.rho) + (size_t)S.block_mm[b0] * M;
+  RowT *ra, *rb;
+  if constexpr (kLds) {
+    ra = lds_row(0);
+    rb = lds_row(S.row_stride);
+  } else {
+    ra = hbm(S.Gb);
+    rb = hbm(S.Gb) + M;
+  }
+  double gnacc = 0.0;
+  for (int wx = 0; wx < W; wx++) {
+    int sx, sy, sz;
+    const int ix = work_site(S, wx, sx, sy, sz);
+    if (!is_interior(S, sx, sy, sz) || !(g1a[ix] > 0)) continue;  // the same on every thread
+    // row 0 = delta_x G1
+    RowT *prev;
+    if constexpr (kLds)
+      prev = ra;
+    else
+      prev = Gf;
+    for (int w = tid; w < W; w += nt) {
+      int x, y, z;
+      const int i = work_site(S, w, x, y, z);
+      prev[i] = i == ix ? g1a[i] : 0.0;
+    }
+    Rows<kLds>::barrier();
+    RowT *next = rb;
+    int s = 1;
+    for (int b = b0 + 1; b < b1 - 1; b++) {
+      const gdbl *g1 = G1 + (size_t)S.block_seg[b] * M;
+      for (int j = 0; j < S.block_n[b]; j++, s++) {
+        gdbl *gs = Gf + (size_t)s * M;
+        for (int w = tid; w < W; w += nt) {
+          int x, y, z;
+          const int i = work_site(S, w, x, y, z);
+          const double v = stencil_prop<double>(S, prev, i, x, y, z) * g1[i];
+          gs[i] = v;
+          if constexpr (kLds) next[i] = v;
+        }
+        Rows<kLds>::barrier();
+        if constexpr (kLds) {
+          RowT *t = prev;
+          prev = next;
+          next = t;
+        } else {
+          prev = gs;
+        }
+      }
+    }
+    // the merged end segment, on the start site only
+    for (int w = tid; w < W; w += nt) {
+      int x, y, z;
+      const int i = work_site(S, w, x, y, z);
+      if (i != ix) continue;
+      const double last = stencil_prop<double>(S, prev, i, x, y, z) * g1a[i];
+      gnacc += site_weight(S0, i) * last;
+      rho_a[i] += g1a[i] * last;
+    }
+    Rows<kLds>::barrier();
+    // backward from delta_x G1 through the middle blocks
+    RowT *gbA = ra, *gbB = rb;
+    for (int w = tid; w < W; w += nt) {
+      int x, y, z;
+      const int i = work_site(S, w, x, y, z);
+      gbA[i] = i == ix ? g1a[i] : 0.0;
+    }
+    Rows<kLds>::barrier();
+    s = N - 1;
+    for (int b = b1 - 2; b >= b0 + 1; b--) {
+      const gdbl *g1 = G1 + (size_t)S.block_seg[b] * M;
+      gdbl *rho = hbm(S.rho) + (size_t)S.block_mm[b] * M;
+      for (int j = 0; j < S.block_n[b]; j++, s--) {
+        const gdbl *gf = Gf + (size_t)s * M;
+        for (int w = tid; w < W; w += nt) {
+          int x, y, z;
+          const int i = work_site(S, w, x, y, z);
+          const double v = stencil_prop<double>(S, gbA, i, x, y, z) * g1[i];
+          gbB[i] = v;
+          rho[i] += gf[i] * v;
+        }
+        Rows<kLds>::barrier();
+        RowT *t = gbA;
+        gbA = gbB;
+        gbB = t;
+      }
+    }
+  }
+  const double gn = wg_sum(gnacc);
+  if (threadIdx.x == 0) S0.sc->GN[mo] = gn;
+  __syncthreads();
+}
+
 // Chains of a charged 2- or 3-gradient system over every site, ghosts
 // included, in the reference's array form (Markov 1, fjc = 1): before each
 // step the previous row's ghosts take their bounded values in place
@@ -1915,7 +2024,12 @@ __device__ double fused_residual(const DevSys &S0) {
           chain_markov2<false, M2_D3>(S0, mo);
       }
     } else if (S.mol_ring[mo]) {
-      chain1d_ring(S0, mo);
+      if (S.gradients == 1)
+        chain1d_ring(S0, mo);
+      else if (S.use_lds)
+        chain_ring<true>(S0, mo);
+      else
+        chain_ring<false>(S0, mo);
     } else if (S.all_sites) {
       chain_literal(S0, mo);
     } else if (S.strip_sx > 0) {

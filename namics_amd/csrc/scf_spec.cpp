@@ -1397,9 +1397,9 @@ bool parse_input(const std::string &text, int start, SystemSpec &s, std::string 
     }
     m.save_memory = false;
     if (get(p, "mol", m.name, "save_memory", v)) to_bool(v, m.save_memory);
-    if (m.ring && (m.markov != 1 || m.save_memory || m.pinned || m.tagged || l.gradients != 1 || l.fjc != 1)) {
+    if (m.ring && (m.markov != 1 || m.save_memory || m.pinned || m.tagged || l.fjc != 1)) {
       err = "mol '" + m.name +
-            "': rings are supported for first-order chains on 1-gradient lattices with fjc 1, without save_memory, "
+            "': rings are supported for first-order chains on lattices with fjc 1, without save_memory, "
             "pinned or tagged segments";
       return false;
     }

@@ -1410,7 +1410,7 @@ struct Oracle {
 
   // Molecule::ComputePhi for a linear chain; accumulates Gf*Gb into mol_phi
   // mol_linear::ComputePhi for a ring (src/mol_linear.cpp:35-111), first
-  // order, one gradient: for every interior site x where the first segment
+  // order: for every interior site x where the first segment
   // may sit, a chain starts there (G0 = delta_x G1), runs through the middle
   // blocks and ends on x with the last segment (the row masked to x); GN is
   // the sum over x, the merged end segment's density is G0 times that row,
@@ -1434,7 +1434,18 @@ struct Oracle {
     vec G0(M, 0.0);
     const double *g1a = G1[seg_of[0]].data();
     double gn = 0;
-    for (int x = 1; x < s.lat.MX + 1; x++) {
+    // interior sites in the reference's order: z outer, x inner
+    std::vector<int> starts;
+    const LatticeSpec &L = s.lat;
+    const int nz = L.gradients == 3 ? L.MZ : 1, ny = L.gradients >= 2 ? L.MY : 1;
+    for (int z = 0; z < nz; z++)
+      for (int y = 0; y < ny; y++)
+        for (int x = 1; x < L.MX + 1; x++) {
+          if (L.gradients == 1) starts.push_back(x);
+          else if (L.gradients == 2) starts.push_back(x * L.JX + (y + 1) * L.JY);
+          else starts.push_back(x * L.JX + (y + 1) * L.JY + (z + 1));
+        }
+    for (int x : starts) {
       if (!(g1a[x] > 0)) continue;
       Zero(G0.data(), M);
       G0[x] = g1a[x];
@@ -1462,6 +1473,10 @@ struct Oracle {
       compute_phi_branched(mi);
       return;
     }
+    if (m.ring) {  // its blocks hold one segment more than the chain
+      compute_phi_ring(mi);
+      return;
+    }
     Lat lat(s.lat);
     const int N = m.chainlength, size = m.size;
     const int W = size * M;
@@ -1477,10 +1492,6 @@ struct Oracle {
           mm_of[k] = m.block_molmon[b];
           k++;
         }
-    }
-    if (m.ring) {
-      compute_phi_ring(mi);
-      return;
     }
     if ((int)Gf.size() < N) Gf.resize(N);
     for (int k = 0; k < N; k++) Gf[k].assign(W, 0.0);

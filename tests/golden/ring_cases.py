@@ -2,7 +2,8 @@
 parity tests.  The first and last blocks of a linear chain, one segment each
 of the same type, are merged: a chain of the composition's length less one
 that starts and ends on the same site.  Cases: rings adsorbing on a planar
-wall next to free linear chains, and rings around a spherical particle.
+wall next to free linear chains, rings around a spherical particle, rings
+in a 2-gradient cylinder and in a periodic 3-gradient box.
 """
 
 RING = {
@@ -54,6 +55,54 @@ mol : water : composition : (W)1
 mol : water : freedom : solvent
 """,
         eval=[(402, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    "ring2d_cyl": dict(
+        text="""
+lat : cyl : gradients : 2
+lat : cyl : geometry : cylindrical
+lat : cyl : n_layers_x : 10
+lat : cyl : n_layers_y : 12
+lat : cyl : lattice_type : simple_cubic
+mon : A : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.5
+mol : ring : composition : (A)1(A)10(A)1
+mol : ring : ring : true
+mol : ring : freedom : restricted
+mol : ring : theta : 6
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(403, 0.3)],
+        converged=[dict(method="DIIS", tol=1e-8)],
+    ),
+    "ring3d_box": dict(
+        text="""
+lat : box : gradients : 3
+lat : box : n_layers_x : 6
+lat : box : n_layers_y : 5
+lat : box : n_layers_z : 7
+lat : box : lattice_type : simple_cubic
+lat : box : lowerbound_x : periodic
+lat : box : upperbound_x : periodic
+lat : box : lowerbound_y : periodic
+lat : box : upperbound_y : periodic
+lat : box : lowerbound_z : periodic
+lat : box : upperbound_z : periodic
+mon : A : freedom : free
+mon : B : freedom : free
+mon : W : freedom : free
+mon : A : chi_W : 0.8
+mon : B : chi_W : 0.3
+mol : ring : composition : (A)1(B)4(A)2(B)3(A)1
+mol : ring : ring : true
+mol : ring : freedom : restricted
+mol : ring : theta : 2
+mol : water : composition : (W)1
+mol : water : freedom : solvent
+""",
+        eval=[(404, 0.3)],
         converged=[dict(method="DIIS", tol=1e-8)],
     ),
 }

@@ -97,11 +97,8 @@ def test_ring_refusals_and_fallback():
     assert not m["ring"] and m["chainlength"] == 21
     assert [d for d in Oracle(base).desc["molecules"] if d["name"] == "ring"][0]["chainlength"] == 19
     for bad, msg in ((base + "mol : ring : Markov : 2\n", "first-order"),
-                     ("lat : L : gradients : 2\nlat : L : n_layers_x : 10\nlat : L : n_layers_y : 10\n"
-                      "lat : L : lattice_type : simple_cubic\nmon : A : freedom : free\n"
-                      "mon : W : freedom : free\nmol : r : composition : (A)1(A)8(A)1\n"
-                      "mol : r : ring : true\nmol : r : freedom : restricted\nmol : r : theta : 5\n"
-                      "mol : w : composition : (W)1\nmol : w : freedom : solvent\n", "1-gradient"),
+                     (base.replace("lattice_type : simple_cubic",
+                                   "lattice_type : hexagonal\nlat : flat : FJC_choices : 5"), "fjc 1"),
                      (base + "lat : flat : bondlength : 5e-10\nmon : B : valence : -1\nmol : Na : composition : (Na)1\n"
                       "mol : Na : freedom : neutralizer\nmon : Na : valence : 1\nmon : Na : freedom : free\n",
                       "charged rings")):
